@@ -1,0 +1,119 @@
+"""ctypes binding of libcdjhip.so (include/cdj.h).
+
+There is no CPU fallback: if the library is missing or no HIP device is present, calls raise.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcdjhip.so")
+
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_p = ctypes.c_void_p
+
+CDJ_OK = 0
+CDJ_ERR_DIMS = 1
+CDJ_ERR_HALFSPACE = 2
+CDJ_ERR_DEGENERATE = 3
+CDJ_ERR_DEVICE = 4
+CDJ_ERR_NOMEM = 5
+CDJ_ERR_NOT_CONVERGED = 6
+CDJ_ERR_ARG = 7
+CDJ_ERR_SINGULAR = 8
+
+CDJ_PREC_FP64 = 0
+CDJ_PREC_MIXED32 = 1
+
+# every symbol declared in include/cdj.h
+EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_td_influence", "cdj_td_field",
+           "cdj_traction_influence", "cdj_slip_calculator", "cdj_solve", "cdj_assemble_dev",
+           "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev"]
+
+
+class CdjError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("precision", _i32), ("tile", _i32), ("tol", _d), ("max_iter", _i32), ("near_factor", _d)]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [("path", _i32), ("iterations", _i32), ("residual", _d), ("assembly_ms", _d), ("solve_ms", _d),
+                ("near_entries", _i64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_LIB = None
+_CTX = None
+_LOCK = threading.Lock()
+
+
+def load(path=LIB_PATH):
+    """Load the shared library and declare every entry point's signature."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise CdjError(CDJ_ERR_DEVICE, "libcdjhip.so is not built (%s); run cutanddisplacejulia_amd.build" % path)
+    L = ctypes.CDLL(path)
+    L.cdj_create.argtypes = [ctypes.c_int, ctypes.POINTER(_p)]
+    L.cdj_destroy.argtypes = [_p]
+    L.cdj_last_error.argtypes = [_p]
+    L.cdj_last_error.restype = ctypes.c_char_p
+    td_args = [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _d, _d, _i32, _i32, _i32, _p, _p]
+    L.cdj_td_influence.argtypes = td_args
+    L.cdj_td_field.argtypes = td_args
+    L.cdj_td_influence_dev.argtypes = td_args + [_p]
+    L.cdj_traction_influence.argtypes = [_p, _p, _p, _p, _p, _p, _i64, _d, _d, _d, _i32, _p, _p, _p]
+    L.cdj_slip_calculator.argtypes = [_p, _p, _p, _p, _i64, _d, _d, _d, _p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p]
+    L.cdj_solve.argtypes = [_p, _p, _p, _i64, _i64, _p, _p, _p]
+    L.cdj_assemble_dev.argtypes = [_p, _p, _p, _p, _p, _p, _i64, _d, _d, _d, _i32, _p, _i64, _p, _p]
+    L.cdj_lu_factor_dev.argtypes = [_p, _p, _i64, _i64, _p, _p]
+    L.cdj_lu_solve_dev.argtypes = [_p, _p, _i64, _i64, _p, _p, _i64, _i64, _p]
+    for name in EXPORTS:
+        f = getattr(L, name)
+        if name != "cdj_last_error":
+            f.restype = ctypes.c_int
+    _LIB = L
+    return L
+
+
+def ctx():
+    """Process-wide context on device 0 (created on first use)."""
+    global _CTX
+    with _LOCK:
+        if _CTX is None:
+            L = load()
+            h = _p()
+            rc = L.cdj_create(0, ctypes.byref(h))
+            if rc != CDJ_OK:
+                raise CdjError(rc, "cdj_create failed: " + L.cdj_last_error(None).decode())
+            _CTX = h
+        return _CTX
+
+
+def check(rc):
+    if rc != CDJ_OK:
+        raise CdjError(rc, load().cdj_last_error(ctx()).decode())
+
+
+def f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(_p)
+
+
+def ptr_array(arrs):
+    return (_p * len(arrs))(*[a.ctypes.data for a in arrs])

@@ -1,0 +1,387 @@
+"""Python mirror of CutAndDisplaceJulia's TD / influence / slip-solve interface, running on the HIP engine.
+
+Names, argument order and meaning follow the reference so that callers (and tests) read like the reference:
+  TD                              src/TD.jl:77 (allocating), :135 (preallocated StrainInf/DispInf), :219 (summed)
+  CalculateInfluenceMatrices3D    src/CalculateInfluenceMatrices3D.jl:2
+  SlipCalculator3D                src/SlipCalculator3D.jl:1 (base method)
+  SetupTractionVector             src/SetupTractionVector.jl:2-41
+  CreateFaceNormalAndMidPoint     src/CreateFaceNormalAndMidPoint.jl (P1, P2, P3 form)
+  ElasticConstantsCheck           src/ElasticConstantsCheck.jl:37-49 (G, nu form)
+Data types mirror src/CutAndDisplaceJulia.jl:46-91.  Arrays are numpy; N x 3 arrays hold one triangle per row
+(P1[:, 0] are x coordinates), as in the reference.  Every TD / assembly / solve call runs in libcdjhip.so; the
+O(N) host-side helpers here (normals, incentres, remote-stress tractions) prepare inputs as the reference's
+callers do.
+"""
+import ctypes
+from dataclasses import dataclass, field, fields
+from typing import Any
+
+import numpy as np
+
+from . import _lib
+from ._lib import CdjError, Opts, Report, check, ctx, f64, load, ptr, ptr_array
+
+# ---- data types (CutAndDisplaceJulia.jl:46-91) -----------------------------------------------------------
+
+
+@dataclass
+class Stresses:
+    σxx: Any = None
+    σyy: Any = None
+    σzz: Any = None
+    σxy: Any = None
+    σxz: Any = None
+    σyz: Any = None
+
+
+@dataclass
+class Strains:
+    εxx: Any = field(default_factory=list)
+    εyy: Any = field(default_factory=list)
+    εzz: Any = field(default_factory=list)
+    εxy: Any = field(default_factory=list)
+    εxz: Any = field(default_factory=list)
+    εyz: Any = field(default_factory=list)
+
+
+@dataclass
+class Disps:
+    Ux: Any = field(default_factory=list)
+    Uy: Any = field(default_factory=list)
+    Uz: Any = field(default_factory=list)
+
+
+@dataclass
+class Tractions:
+    Tn: Any = None
+    Tss: Any = None
+    Tds: Any = None
+
+
+@dataclass
+class MixedBoundaryConditions:
+    Stresses: Any = None
+    Tractions: Any = None
+
+
+@dataclass
+class TractionInf:
+    DnTn: Any = None
+    DnTss: Any = None
+    DnTds: Any = None
+    DssTn: Any = None
+    DssTss: Any = None
+    DssTds: Any = None
+    DdsTn: Any = None
+    DdsTss: Any = None
+    DdsTds: Any = None
+
+
+@dataclass
+class StrainInf:
+    εxxDn: Any = field(default_factory=list)
+    εyyDn: Any = field(default_factory=list)
+    εzzDn: Any = field(default_factory=list)
+    εxyDn: Any = field(default_factory=list)
+    εxzDn: Any = field(default_factory=list)
+    εyzDn: Any = field(default_factory=list)
+    εxxDss: Any = field(default_factory=list)
+    εyyDss: Any = field(default_factory=list)
+    εzzDss: Any = field(default_factory=list)
+    εxyDss: Any = field(default_factory=list)
+    εxzDss: Any = field(default_factory=list)
+    εyzDss: Any = field(default_factory=list)
+    εxxDds: Any = field(default_factory=list)
+    εyyDds: Any = field(default_factory=list)
+    εzzDds: Any = field(default_factory=list)
+    εxyDds: Any = field(default_factory=list)
+    εxzDds: Any = field(default_factory=list)
+    εyzDds: Any = field(default_factory=list)
+
+
+@dataclass
+class DispInf:
+    DnUx: Any = field(default_factory=list)
+    DnUy: Any = field(default_factory=list)
+    DnUz: Any = field(default_factory=list)
+    DssUx: Any = field(default_factory=list)
+    DssUy: Any = field(default_factory=list)
+    DssUz: Any = field(default_factory=list)
+    DdsUx: Any = field(default_factory=list)
+    DdsUy: Any = field(default_factory=list)
+    DdsUz: Any = field(default_factory=list)
+
+
+STRAIN_FIELDS = [f.name for f in fields(StrainInf)]
+DISP_FIELDS = [f.name for f in fields(DispInf)]
+TRACTION_FIELDS = [f.name for f in fields(TractionInf)]
+
+# Module-level solver settings (the reference's signatures have no room for them).
+_SETTINGS = {"precision": _lib.CDJ_PREC_FP64, "tile": 0, "tol": 1e-10, "max_iter": 0, "near_factor": 2.0}
+LAST_REPORT = {}
+
+
+def set_precision(mode):
+    """'fp64' (default, the reference's arithmetic) or 'mixed32' (FP32 far-field + FP64 near-field assembly,
+    FP32 LU, FP64 GMRES iterative refinement)."""
+    _SETTINGS["precision"] = {"fp64": _lib.CDJ_PREC_FP64, "mixed32": _lib.CDJ_PREC_MIXED32}[mode]
+
+
+def _opts():
+    return Opts(_SETTINGS["precision"], _SETTINGS["tile"], _SETTINGS["tol"], _SETTINGS["max_iter"],
+                _SETTINGS["near_factor"])
+
+
+def _tri(P):
+    P = np.asarray(P, dtype=np.float64)
+    if P.ndim == 1:  # a single triangle given as a 3-vector (TD.jl:80)
+        P = P.reshape(1, 3)
+    return f64(P)
+
+
+def _vec(v, n=None):
+    v = np.asarray(v, dtype=np.float64).ravel(order="F")
+    if n is not None and v.size == 1 and n != 1:
+        v = np.full(n, float(v[0]))
+    return f64(v)
+
+
+# ---- TD ------------------------------------------------------------------------------------------------
+
+
+def TD(X, Y, Z, P1List, P2List, P3List, Dss, Dds, Dn, ν, G, DispFlag, StrainFlag, HSflag,
+       StrainInfMat=None, DispInfMat=None):
+    """Triangular dislocation strains/displacements (Nikkhoo & Walter 2015) on the GPU.
+
+    Forms (by the type of the last two arguments, as the reference's methods):
+      * none                       -> (StrainInf, DispInf) influence matrices, len(X) x ntri each (TD.jl:77)
+      * StrainInf, DispInf         -> fills the given matrices in place and returns them (TD.jl:135)
+      * Strains, Disps             -> summed fields over all triangles, shaped like X (TD.jl:219)
+    Raises CdjError with the reference's messages for mismatched sizes or points above a half-space.
+    """
+    shape = np.shape(X)
+    x, y, z = _vec(X), _vec(Y), _vec(Z)
+    P1, P2, P3 = _tri(P1List), _tri(P2List), _tri(P3List)
+    n = P1.shape[0]
+    dss, dds, dn = _vec(Dss), _vec(Dds), _vec(Dn)
+    if not (P2.shape[0] == n and P3.shape[0] == n and dss.size == n and dds.size == n and dn.size == n
+            and y.size == x.size and z.size == x.size):
+        raise CdjError(_lib.CDJ_ERR_DIMS, "Element size inputs must be the same dimensions")
+    m = x.size
+    L = load()
+    if isinstance(StrainInfMat, Strains) or isinstance(DispInfMat, Disps):
+        s6 = [np.zeros(m) for _ in range(6)] if StrainFlag == 1 else None
+        d3 = [np.zeros(m) for _ in range(3)] if DispFlag == 1 else None
+        check(L.cdj_td_field(ctx(), ptr(x), ptr(y), ptr(z), m, ptr(P1), ptr(P2), ptr(P3), n, ptr(dss), ptr(dds),
+                             ptr(dn), float(ν), float(G), int(DispFlag), int(StrainFlag), int(HSflag),
+                             ptr_array(s6) if s6 else None, ptr_array(d3) if d3 else None))
+        rs = (lambda a: a.reshape(shape, order="F")) if len(shape) > 1 and shape[1] != 1 else (lambda a: a)
+        S = Strains(*[rs(a) for a in s6]) if s6 else Strains([], [], [], [], [], [])
+        D = Disps(*[rs(a) for a in d3]) if d3 else Disps([], [], [])
+        return S, D
+    strain = None
+    disp = None
+    if StrainFlag == 1:
+        if isinstance(StrainInfMat, StrainInf) and not isinstance(StrainInfMat.εxxDn, list):
+            strain = [getattr(StrainInfMat, k) for k in STRAIN_FIELDS]
+            for a in strain:
+                if a.shape != (m, n) or not a.flags.f_contiguous or a.dtype != np.float64:
+                    raise CdjError(_lib.CDJ_ERR_ARG, "StrainInf matrices must be Float64 %d x %d column-major" % (m, n))
+        else:
+            strain = [np.zeros((m, n), order="F") for _ in range(18)]
+    if DispFlag == 1:
+        if isinstance(DispInfMat, DispInf) and not isinstance(DispInfMat.DnUx, list):
+            disp = [getattr(DispInfMat, k) for k in DISP_FIELDS]
+            for a in disp:
+                if a.shape != (m, n) or not a.flags.f_contiguous or a.dtype != np.float64:
+                    raise CdjError(_lib.CDJ_ERR_ARG, "DispInf matrices must be Float64 %d x %d column-major" % (m, n))
+        else:
+            disp = [np.zeros((m, n), order="F") for _ in range(9)]
+    check(L.cdj_td_influence(ctx(), ptr(x), ptr(y), ptr(z), m, ptr(P1), ptr(P2), ptr(P3), n, ptr(dss), ptr(dds),
+                             ptr(dn), float(ν), float(G), int(DispFlag), int(StrainFlag), int(HSflag),
+                             ptr_array(strain) if strain else None, ptr_array(disp) if disp else None))
+    S = StrainInf(*strain) if strain else StrainInf()
+    D = DispInf(*disp) if disp else DispInf()
+    return S, D
+
+
+# ---- influence matrices and slip ------------------------------------------------------------------------
+
+
+def CalculateInfluenceMatrices3D(FaceNormalVector, MidPoint, P1, P2, P3, ν, G, λ, FixedEls, HSFlag, n):
+    """Traction influence matrices at the collocation points (CalculateInfluenceMatrices3D.jl:2-48).
+
+    With fixed elements (any FixedEls == 1) the fixed receivers' rows hold displacement influences and the fixed
+    sources' columns are dropped (CalculateInfluenceMatrices3D.jl:10-33, 145-198).
+    """
+    nrm, mid = f64(FaceNormalVector), f64(MidPoint)
+    P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
+    fixed = np.asarray(FixedEls).ravel() == 1 if FixedEls is not None else np.zeros(n, bool)
+    T = [np.zeros((n, n), order="F") for _ in range(9)]
+    check(load().cdj_traction_influence(ctx(), ptr(nrm), ptr(mid), ptr(P1), ptr(P2), ptr(P3), int(n), float(ν),
+                                        float(G), float(λ), int(HSFlag), None, ptr_array(T), ctypes.byref(_opts())))
+    if not fixed.any():
+        return TractionInf(*T)
+    # fixed elements: displacement at the fixed collocation points, placed into the fixed rows
+    x = mid[:, 0] - nrm[:, 0] * 1e-12
+    y = mid[:, 1] - nrm[:, 1] * 1e-12
+    z = mid[:, 2] - nrm[:, 2] * 1e-12
+    ones = np.ones(n)
+    _, D = TD(x[fixed], y[fixed], z[fixed], P1, P2, P3, ones, ones, ones, ν, G, 1, 0, HSFlag)
+    free = ~fixed
+    out = []
+    for k, name in enumerate(TRACTION_FIELDS):
+        full = np.zeros((n, n), order="F")
+        full[free, :] = T[k][free, :]
+        full[fixed, :] = getattr(D, DISP_FIELDS[k])  # DnTn<-DnUx, DnTss<-DnUy, ... (ConcatInfMats field order)
+        out.append(np.asfortranarray(full[:, free]))
+    return TractionInf(*out)
+
+
+def SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, BoundaryConditions, FixedEls):
+    """Slip on the elements that satisfies the traction boundary conditions (SlipCalculator3D.jl:1-33).
+
+    Returns (Dn, Dss, Dds, A, b) like the reference.  b may also be given directly as a 3N x nrhs array in place
+    of BoundaryConditions, in which case Dn/Dss/Dds are N x nrhs.
+    """
+    nrm = f64(FaceNormalVector)
+    n = nrm.shape[0]
+    fixed = np.asarray(FixedEls).ravel() == 1 if FixedEls is not None else np.zeros(n, bool)
+    if isinstance(BoundaryConditions, np.ndarray):
+        b = f64(BoundaryConditions.reshape(3 * n, -1, order="F"))
+    else:
+        Tn, Tds, Tss = SetupTractionVector(BoundaryConditions, nrm, n, G, λ)
+        Tn, Tds, Tss = KnockOutFixedRows(FixedEls, Tn, Tds, Tss)
+        b = f64(np.concatenate([Tn, Tss, Tds]).reshape(3 * n, 1))
+    if fixed.any():
+        Ti = CalculateInfluenceMatrices3D(nrm, MidPoint, P1, P2, P3, ν, G, λ, FixedEls, HSFlag, n)
+        A = -np.block([[Ti.DnTn, Ti.DssTn, Ti.DdsTn], [Ti.DnTss, Ti.DssTss, Ti.DdsTss], [Ti.DnTds, Ti.DssTds, Ti.DdsTds]])
+        # Non-square system: Julia's `\` returns the least-squares solution; solved here through the normal
+        # equations on the GPU LU.
+        AtA = f64(A.T @ A)
+        Atb = f64(A.T @ b)
+        D = solve(AtA, Atb)
+        nf = int((~fixed).sum())
+    else:
+        P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
+        mid = f64(MidPoint)
+        A = np.zeros((3 * n, 3 * n), order="F")
+        D = np.zeros(b.shape, order="F")
+        rep = Report()
+        check(load().cdj_slip_calculator(ctx(), ptr(P1), ptr(P2), ptr(P3), n, float(ν), float(G), float(λ),
+                                         ptr(mid), ptr(nrm), int(HSFlag), ptr(b), b.shape[1], None, ptr(D), ptr(A),
+                                         ctypes.byref(_opts()), ctypes.byref(rep)))
+        LAST_REPORT.clear()
+        LAST_REPORT.update(rep.as_dict())
+        nf = n
+    squeeze = not isinstance(BoundaryConditions, np.ndarray)
+    Dn, Dss, Dds = D[:nf], D[nf:2 * nf], D[2 * nf:3 * nf]
+    if squeeze:
+        Dn, Dss, Dds, b = Dn[:, 0], Dss[:, 0], Dds[:, 0], b[:, 0]
+    return Dn, Dss, Dds, A, b
+
+
+def solve(A, b):
+    """x = A \\ b for square A (blocked LU with partial pivoting on the GPU)."""
+    A = f64(A)
+    n = A.shape[0]
+    b2 = f64(np.asarray(b).reshape(n, -1, order="F"))
+    x = np.zeros_like(b2, order="F")
+    rep = Report()
+    check(load().cdj_solve(ctx(), ptr(A), ptr(b2), n, b2.shape[1], ptr(x), ctypes.byref(_opts()), ctypes.byref(rep)))
+    LAST_REPORT.clear()
+    LAST_REPORT.update(rep.as_dict())
+    return x if np.ndim(b) > 1 else x[:, 0]
+
+
+# ---- host-side input preparation (the reference's callers) ----------------------------------------------
+
+
+def ElasticConstantsCheck(G, ν):
+    """(K, E, λ, ν, G) from shear modulus and Poisson's ratio (ElasticConstantsCheck.jl:37-49)."""
+    K = ((2 * G) * (1 + ν)) / (3 * (1 - (2 * ν)))
+    E = (2 * G) * (1 + ν)
+    λ = (2 * G * ν) / (1 - (2 * ν))
+    return K, E, λ, ν, G
+
+
+def CreateFaceNormalAndMidPoint(P1, P2, P3):
+    """Unit normals (CreateTriangleNormal!.jl:14-30) and incentre midpoints (CreateMidPoint!.jl:3-16)."""
+    P1, P2, P3 = (np.asarray(P, dtype=np.float64) for P in (P1, P2, P3))
+    U = P2 - P1
+    V = P3 - P1
+    N = np.stack([U[:, 1] * V[:, 2] - U[:, 2] * V[:, 1], U[:, 2] * V[:, 0] - U[:, 0] * V[:, 2],
+                  U[:, 0] * V[:, 1] - U[:, 1] * V[:, 0]], axis=1)
+    N = N / np.sqrt(N[:, 0] * N[:, 0] + N[:, 1] * N[:, 1] + N[:, 2] * N[:, 2])[:, None]
+    c = np.sqrt(((P1 - P2) ** 2).sum(1))
+    a = np.sqrt(((P2 - P3) ** 2).sum(1))
+    bb = np.sqrt(((P1 - P3) ** 2).sum(1))
+    if np.any(a == 0) or np.any(bb == 0) or np.any(c == 0):
+        raise CdjError(_lib.CDJ_ERR_DEGENERATE, "Slither triangles or triangles with no area exist on your surface")
+    M = (P1 * a[:, None] + P2 * bb[:, None] + P3 * c[:, None]) / (a + bb + c)[:, None]
+    return f64(N), f64(M)
+
+
+def CalculateSSandDSDirs(CosAx, CosAy, CosAz):
+    """Strike-slip and dip-slip direction cosines of planes with normal (CosAx, CosAy, CosAz)
+    (CalculateSSandDSDirs.jl:37-98)."""
+    CosAx, CosAy, CosAz = (np.atleast_1d(np.asarray(v, dtype=np.float64)) for v in (CosAx, CosAy, CosAz))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        h = np.sqrt(CosAx ** 2 + CosAy ** 2)
+        ss = np.stack([-(CosAy / h), CosAx / h, np.zeros_like(CosAx)], axis=1)
+    DSCosAz = np.cos(np.arccos(CosAz) - np.pi / 2)
+    L = np.sin(np.arccos(DSCosAz))
+    L = np.where(CosAz < 0, -L, L)
+    a = np.arctan2(CosAy, CosAx)
+    ds = np.stack([-(np.cos(a) * L), -(np.sin(a) * L), DSCosAz], axis=1)
+    flat = (CosAx == 0) & (CosAy == 0)
+    ds[flat] = [-1.0, 0.0, 0.0]
+    ss[flat & (CosAz < 1.0)] = [0.0, -1.0, 0.0]
+    ss[flat & ~(CosAz < 1.0)] = [0.0, 1.0, 0.0]
+    return ss, ds
+
+
+def CalculateNormalAndShearTractions3D(σxx, σyy, σzz, σxy, σxz, σyz, FaceNormalVector):
+    """(Tn, Tds, Tss) on planes of the given normals (CalculateNormalAndShearTractions3D.jl:37-58)."""
+    nrm = np.atleast_2d(np.asarray(FaceNormalVector, dtype=np.float64))
+    nx, ny, nz = nrm[:, 0], nrm[:, 1], nrm[:, 2]
+    ss, ds = CalculateSSandDSDirs(nx, ny, nz)
+    Sxx, Syy, Szz, Sxy, Sxz, Syz = (np.broadcast_to(np.asarray(v, dtype=np.float64), nx.shape)
+                                    for v in (σxx, σyy, σzz, σxy, σxz, σyz))
+    Tn = Sxx * nx ** 2 + Syy * ny ** 2 + Szz * nz ** 2 + 2 * Sxy * nx * ny + 2 * Syz * ny * nz + 2 * Sxz * nz * nx
+    Tx = Sxx * nx + Sxy * ny + Sxz * nz
+    Ty = Sxy * nx + Syy * ny + Syz * nz
+    Tz = Sxz * nx + Syz * ny + Szz * nz
+
+    def along(d):
+        A = (Tx * (1 - nx ** 2) - Ty * (nx * ny) - Tz * (nz * nx)) * d[:, 0]
+        B = (Tx * -(nx * ny) + Ty * (1 - ny ** 2) - Tz * (ny * nz)) * d[:, 1]
+        C = (Tx * -(nz * nx) - Ty * (ny * nz) + Tz * (1 - nz ** 2)) * d[:, 2]
+        return A + B + C
+
+    return Tn, along(ds), along(ss)
+
+
+def SetupTractionVector(BC, FaceNormalVector, n, G, λ):
+    """Boundary-condition tractions (Tn, Tds, Tss) on every element (SetupTractionVector.jl:2-41)."""
+    if isinstance(BC, Stresses):
+        return CalculateNormalAndShearTractions3D(BC.σxx, BC.σyy, BC.σzz, BC.σxy, BC.σxz, BC.σyz, FaceNormalVector)
+    if isinstance(BC, Tractions):
+        return (np.asarray(BC.Tn, dtype=np.float64) * np.ones(n), np.asarray(BC.Tds, dtype=np.float64) * np.ones(n),
+                np.asarray(BC.Tss, dtype=np.float64) * np.ones(n))
+    if isinstance(BC, MixedBoundaryConditions):
+        sTn, sTds, sTss = SetupTractionVector(BC.Stresses, FaceNormalVector, n, G, λ)
+        Tn, Tds, Tss = SetupTractionVector(BC.Tractions, FaceNormalVector, n, G, λ)
+        return Tn + sTn, Tds + sTds, Tss + sTss
+    raise TypeError("unsupported boundary condition type %r" % type(BC))
+
+
+def KnockOutFixedRows(FixedEls, Tn, Tds, Tss):
+    """Zero the tractions of fixed elements (KnockOutFixedRows.jl:1-17)."""
+    Tn, Tds, Tss = (np.array(v, dtype=np.float64) for v in (Tn, Tds, Tss))
+    if FixedEls is not None:
+        f = np.asarray(FixedEls).ravel() == 1
+        Tn[f] = 0.0
+        Tds[f] = 0.0
+        Tss[f] = 0.0
+    return Tn, Tds, Tss

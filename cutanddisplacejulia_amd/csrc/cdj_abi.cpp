@@ -1,0 +1,551 @@
+// C ABI of the engine (include/cdj.h): contexts, device buffers, host<->device staging and dispatch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cdj.h"
+#include "linalg.hpp"
+#include "td_kernels.hpp"
+
+using namespace cdj;
+
+struct cdj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(cdj_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(cdj_ctx* c, hipError_t e, const char* where) {
+  return fail(c, e == hipErrorOutOfMemory ? CDJ_ERR_NOMEM : CDJ_ERR_DEVICE,
+              std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(expr)                                   \
+  do {                                                 \
+    hipError_t _e = (expr);                            \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+  } while (0)
+
+// Scoped device allocation.
+struct DBuf {
+  void* p = nullptr;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    return hipMalloc(&p, bytes);
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+int default_tile(int hs, int64_t nobs) {
+  (void)nobs;
+  return hs ? 4 : 16;
+}
+
+int check_flags(cdj_ctx* ctx, int32_t disp_flag, int32_t strain_flag, double* const* strain, double* const* disp) {
+  if (strain_flag == 1 && !strain) return fail(ctx, CDJ_ERR_ARG, "strain outputs missing");
+  if (disp_flag == 1 && !disp) return fail(ctx, CDJ_ERR_ARG, "displacement outputs missing");
+  if (strain_flag == 1)
+    for (int k = 0; k < 18; ++k)
+      if (!strain[k]) return fail(ctx, CDJ_ERR_ARG, "strain output pointer is NULL");
+  if (disp_flag == 1)
+    for (int k = 0; k < 9; ++k)
+      if (!disp[k]) return fail(ctx, CDJ_ERR_ARG, "displacement output pointer is NULL");
+  return CDJ_OK;
+}
+
+// Half-space validity check on device arrays (TD.jl:606-619); synchronises the stream.
+int check_hs_dev(cdj_ctx* ctx, const double* Z, int64_t nobs, const double* P1, const double* P2, const double* P3,
+                 int64_t ntri, hipStream_t st) {
+  DBuf flag;
+  HIPCHK(flag.alloc(sizeof(int)));
+  HIPCHK(hipMemsetAsync(flag.p, 0, sizeof(int), st));
+  launch_check_hs(Z, nobs, P1, P2, P3, ntri, flag.as<int>(), st);
+  HIPCHK(hipGetLastError());
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (h) return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+  return CDJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdj_create(int device, cdj_ctx** out) {
+  if (!out) return CDJ_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    g_create_err = std::string("no HIP device: ") + hipGetErrorString(e);
+    return CDJ_ERR_DEVICE;
+  }
+  if (device < 0 || device >= count) {
+    g_create_err = "device index out of range";
+    return CDJ_ERR_ARG;
+  }
+  e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    g_create_err = hipGetErrorString(e);
+    return CDJ_ERR_DEVICE;
+  }
+  cdj_ctx* c = new cdj_ctx();
+  c->device = device;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_create_err = hipGetErrorString(e);
+    delete c;
+    return CDJ_ERR_DEVICE;
+  }
+  *out = c;
+  return CDJ_OK;
+}
+
+int cdj_destroy(cdj_ctx* ctx) {
+  if (!ctx) return CDJ_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CDJ_OK;
+}
+
+const char* cdj_last_error(const cdj_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
+                         const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
+                         const double* Dds, const double* Dn, double nu, double G, int32_t disp_flag,
+                         int32_t strain_flag, int32_t hs_flag, double* const* strain, double* const* disp,
+                         void* stream) {
+  (void)G;
+  if (!ctx) return CDJ_ERR_ARG;
+  if (nobs < 0 || ntri < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  int rc = check_flags(ctx, disp_flag, strain_flag, strain, disp);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  HIPCHK(hipSetDevice(ctx->device));
+  const int hs = hs_flag == 1;
+  if (hs) {
+    rc = check_hs_dev(ctx, Z, nobs, P1, P2, P3, ntri, st);
+    if (rc) return rc;
+  }
+  if (nobs == 0 || ntri == 0 || (disp_flag != 1 && strain_flag != 1)) return CDJ_OK;
+  const int RS = hs ? REC_HS : REC_FS;
+  DBuf rec;
+  HIPCHK(rec.alloc(sizeof(double) * RS * ntri));
+  launch_setup(P1, P2, P3, ntri, hs, rec.as<double>(), st);
+  OutPtrs<double> out{};
+  out.ld = nobs;
+  out.sign = 1.0;
+  out.strain_on = strain_flag == 1;
+  out.disp_on = disp_flag == 1;
+  if (out.strain_on)
+    for (int k = 0; k < 18; ++k) out.p[k] = strain[k];
+  if (out.disp_on)
+    for (int k = 0; k < 9; ++k) out.p[18 + k] = disp[k];
+  launch_influence<double>(X, Y, Z, nobs, rec.as<double>(), ntri, hs, default_tile(hs, nobs), Dn, Dss, Dds, nu, out,
+                           st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_td_influence(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
+                     const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
+                     const double* Dds, const double* Dn, double nu, double G, int32_t disp_flag,
+                     int32_t strain_flag, int32_t hs_flag, double* const* strain, double* const* disp) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (nobs < 0 || ntri < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  int rc = check_flags(ctx, disp_flag, strain_flag, strain, disp);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t vo = sizeof(double) * nobs, vt = sizeof(double) * ntri;
+  DBuf dX, dY, dZ, dP1, dP2, dP3, dDn, dDss, dDds;
+  HIPCHK(dX.alloc(vo));
+  HIPCHK(dY.alloc(vo));
+  HIPCHK(dZ.alloc(vo));
+  HIPCHK(dP1.alloc(3 * vt));
+  HIPCHK(dP2.alloc(3 * vt));
+  HIPCHK(dP3.alloc(3 * vt));
+  HIPCHK(dDn.alloc(vt));
+  HIPCHK(dDss.alloc(vt));
+  HIPCHK(dDds.alloc(vt));
+  HIPCHK(hipMemcpyAsync(dX.p, X, vo, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dY.p, Y, vo, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dZ.p, Z, vo, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dP1.p, P1, 3 * vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dP2.p, P2, 3 * vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dP3.p, P3, 3 * vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dDn.p, Dn, vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dDss.p, Dss, vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dDds.p, Dds, vt, hipMemcpyHostToDevice, st));
+  const size_t vm = sizeof(double) * nobs * ntri;
+  const int nout = (strain_flag == 1 ? 18 : 0) + (disp_flag == 1 ? 9 : 0);
+  DBuf dout;
+  HIPCHK(dout.alloc(vm * nout));
+  double* ds[18] = {};
+  double* dd[9] = {};
+  int q = 0;
+  if (strain_flag == 1)
+    for (int k = 0; k < 18; ++k) ds[k] = dout.as<double>() + (q++) * nobs * ntri;
+  if (disp_flag == 1)
+    for (int k = 0; k < 9; ++k) dd[k] = dout.as<double>() + (q++) * nobs * ntri;
+  rc = cdj_td_influence_dev(ctx, dX.as<double>(), dY.as<double>(), dZ.as<double>(), nobs, dP1.as<double>(),
+                            dP2.as<double>(), dP3.as<double>(), ntri, dDss.as<double>(), dDds.as<double>(),
+                            dDn.as<double>(), nu, G, disp_flag, strain_flag, hs_flag, strain_flag == 1 ? ds : nullptr,
+                            disp_flag == 1 ? dd : nullptr, st);
+  if (rc) return rc;
+  if (strain_flag == 1)
+    for (int k = 0; k < 18; ++k) HIPCHK(hipMemcpyAsync(strain[k], ds[k], vm, hipMemcpyDeviceToHost, st));
+  if (disp_flag == 1)
+    for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(disp[k], dd[k], vm, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_td_field(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs, const double* P1,
+                 const double* P2, const double* P3, int64_t ntri, const double* Dss, const double* Dds,
+                 const double* Dn, double nu, double G, int32_t disp_flag, int32_t strain_flag, int32_t hs_flag,
+                 double* const* strain6, double* const* disp3) {
+  (void)G;
+  if (!ctx) return CDJ_ERR_ARG;
+  if (nobs < 0 || ntri < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  if (strain_flag == 1 && !strain6) return fail(ctx, CDJ_ERR_ARG, "strain outputs missing");
+  if (disp_flag == 1 && !disp3) return fail(ctx, CDJ_ERR_ARG, "displacement outputs missing");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int hs = hs_flag == 1;
+  const size_t vo = sizeof(double) * nobs, vt = sizeof(double) * ntri;
+  DBuf dX, dY, dZ, dP1, dP2, dP3, dDn, dDss, dDds;
+  HIPCHK(dX.alloc(vo));
+  HIPCHK(dY.alloc(vo));
+  HIPCHK(dZ.alloc(vo));
+  HIPCHK(dP1.alloc(3 * vt));
+  HIPCHK(dP2.alloc(3 * vt));
+  HIPCHK(dP3.alloc(3 * vt));
+  HIPCHK(dDn.alloc(vt));
+  HIPCHK(dDss.alloc(vt));
+  HIPCHK(dDds.alloc(vt));
+  HIPCHK(hipMemcpyAsync(dX.p, X, vo, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dY.p, Y, vo, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dZ.p, Z, vo, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dP1.p, P1, 3 * vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dP2.p, P2, 3 * vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dP3.p, P3, 3 * vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dDn.p, Dn, vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dDss.p, Dss, vt, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dDds.p, Dds, vt, hipMemcpyHostToDevice, st));
+  if (hs) {
+    int rc = check_hs_dev(ctx, dZ.as<double>(), nobs, dP1.as<double>(), dP2.as<double>(), dP3.as<double>(), ntri, st);
+    if (rc) return rc;
+  }
+  const int strain_on = strain_flag == 1, disp_on = disp_flag == 1;
+  const int ncomp = (strain_on ? 6 : 0) + (disp_on ? 3 : 0);
+  if (nobs == 0 || ncomp == 0) return CDJ_OK;
+  const int tpb = hs ? 8 : 32;
+  const int64_t nchunk = ntri > 0 ? (ntri + tpb - 1) / tpb : 0;
+  const int RS = hs ? REC_HS : REC_FS;
+  DBuf rec, part, dout;
+  HIPCHK(rec.alloc(sizeof(double) * RS * std::max<int64_t>(ntri, 1)));
+  HIPCHK(part.alloc(sizeof(double) * std::max<int64_t>(nchunk, 1) * ncomp * nobs));
+  HIPCHK(dout.alloc(sizeof(double) * ncomp * nobs));
+  launch_setup(dP1.as<double>(), dP2.as<double>(), dP3.as<double>(), ntri, hs, rec.as<double>(), st);
+  launch_field<double>(dX.as<double>(), dY.as<double>(), dZ.as<double>(), nobs, rec.as<double>(), ntri, hs, tpb,
+                       dDn.as<double>(), dDss.as<double>(), dDds.as<double>(), nu, strain_on, disp_on,
+                       part.as<double>(), dout.as<double>(), st);
+  HIPCHK(hipGetLastError());
+  int q = 0;
+  if (strain_on)
+    for (int k = 0; k < 6; ++k)
+      HIPCHK(hipMemcpyAsync(strain6[k], dout.as<double>() + (q++) * nobs, vo, hipMemcpyDeviceToHost, st));
+  if (disp_on)
+    for (int k = 0; k < 3; ++k)
+      HIPCHK(hipMemcpyAsync(disp3[k], dout.as<double>() + (q++) * nobs, vo, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
+                     const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                     int32_t hs_flag, double* A, int64_t lda, const cdj_opts* opts, void* stream) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0 || lda < 3 * n) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (n == 0) return CDJ_OK;
+  const int hs = hs_flag == 1;
+  DBuf recv, rec;
+  HIPCHK(recv.alloc(sizeof(double) * RECV * n));
+  const int RS = hs ? REC_HS : REC_FS;
+  HIPCHK(rec.alloc(sizeof(double) * RS * n));
+  launch_receivers(normals, midpoints, n, recv.as<double>(), st);
+  if (hs) {
+    // collocation z-coordinates live in recv (stride RECV); check vertices and receivers on the host copy
+    std::vector<double> rv(RECV * n);
+    HIPCHK(hipMemcpyAsync(rv.data(), recv.p, sizeof(double) * RECV * n, hipMemcpyDeviceToHost, st));
+    std::vector<double> z1(n), z2(n), z3(n);
+    HIPCHK(hipMemcpyAsync(z1.data(), P1 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(z2.data(), P2 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(z3.data(), P3 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < n; ++i)
+      if (rv[RECV * i + 2] > 0 || z1[i] > 0 || z2[i] > 0 || z3[i] > 0)
+        return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+  }
+  launch_setup(P1, P2, P3, n, hs, rec.as<double>(), st);
+  OutPtrs<double> out{};
+  out.ld = lda;
+  out.sign = -1.0;
+  for (int m = 0; m < 3; ++m)
+    for (int q = 0; q < 3; ++q) out.p[3 * m + q] = A + (int64_t)q * n + (int64_t)m * n * lda;
+  const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, n);
+  launch_traction<double>(recv.as<double>(), n, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
+                           const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                           int32_t hs_flag, const int32_t* fixed, double* const* T, const cdj_opts* opts) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  if (fixed)
+    for (int64_t i = 0; i < n; ++i)
+      if (fixed[i] == 1) return fail(ctx, CDJ_ERR_ARG, "fixed elements are not supported by this entry point");
+  for (int k = 0; k < 9; ++k)
+    if (!T || !T[k]) return fail(ctx, CDJ_ERR_ARG, "traction output pointer is NULL");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (n == 0) return CDJ_OK;
+  const int hs = hs_flag == 1;
+  const size_t v3 = sizeof(double) * 3 * n;
+  DBuf dn, dm, d1, d2, d3, recv, rec, dT;
+  HIPCHK(dn.alloc(v3));
+  HIPCHK(dm.alloc(v3));
+  HIPCHK(d1.alloc(v3));
+  HIPCHK(d2.alloc(v3));
+  HIPCHK(d3.alloc(v3));
+  HIPCHK(hipMemcpyAsync(dn.p, normals, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dm.p, midpoints, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(recv.alloc(sizeof(double) * RECV * n));
+  const int RS = hs ? REC_HS : REC_FS;
+  HIPCHK(rec.alloc(sizeof(double) * RS * n));
+  const size_t vm = sizeof(double) * n * n;
+  HIPCHK(dT.alloc(9 * vm));
+  launch_receivers(dn.as<double>(), dm.as<double>(), n, recv.as<double>(), st);
+  if (hs) {
+    std::vector<double> rv(RECV * n);
+    HIPCHK(hipMemcpyAsync(rv.data(), recv.p, sizeof(double) * RECV * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < n; ++i)
+      if (rv[RECV * i + 2] > 0 || P1[2 * n + i] > 0 || P2[2 * n + i] > 0 || P3[2 * n + i] > 0)
+        return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+  }
+  launch_setup(d1.as<double>(), d2.as<double>(), d3.as<double>(), n, hs, rec.as<double>(), st);
+  OutPtrs<double> out{};
+  out.ld = n;
+  out.sign = 1.0;
+  for (int k = 0; k < 9; ++k) out.p[k] = dT.as<double>() + (int64_t)k * n * n;
+  const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, n);
+  launch_traction<double>(recv.as<double>(), n, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(T[k], out.p[k], vm, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+int cdj_lu_factor_dev(cdj_ctx* ctx, double* A, int64_t n, int64_t lda, int32_t* ipiv, void* stream) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0 || lda < n) return fail(ctx, CDJ_ERR_DIMS, "invalid matrix dimensions");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (n == 0) return CDJ_OK;
+  DBuf ws, info;
+  HIPCHK(ws.alloc(lu_workspace_bytes(n)));
+  HIPCHK(info.alloc(sizeof(int)));
+  HIPCHK(lu_factor<double>(A, n, lda, ipiv, info.as<int>(), ws.p, st));
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (h) return fail(ctx, CDJ_ERR_SINGULAR, "matrix is exactly singular (zero pivot " + std::to_string(h) + ")");
+  return CDJ_OK;
+}
+
+int cdj_lu_solve_dev(cdj_ctx* ctx, const double* LU, int64_t n, int64_t lda, const int32_t* ipiv, double* B,
+                     int64_t nrhs, int64_t ldb, void* stream) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0 || lda < n || ldb < n || nrhs < 0) return fail(ctx, CDJ_ERR_DIMS, "invalid matrix dimensions");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (n == 0 || nrhs == 0) return CDJ_OK;
+  HIPCHK(lu_solve<double>(LU, n, lda, ipiv, B, nrhs, ldb, st));
+  return CDJ_OK;
+}
+
+namespace {
+
+// Factor the augmented device matrix [A | B] (n x (n + nrhs), leading dimension lda) in FP64, back-substitute
+// and leave X in the B columns.  Aorig (device, may be NULL) is used for the FP64 residual.
+int solve_augmented_fp64(cdj_ctx* ctx, double* AB, int64_t n, int64_t lda, int64_t nrhs, const double* Aorig,
+                         const double* Borig, cdj_report* report, hipStream_t st) {
+  DBuf ws, info, ipiv;
+  HIPCHK(ws.alloc(lu_workspace_bytes(n)));
+  HIPCHK(info.alloc(sizeof(int)));
+  HIPCHK(ipiv.alloc(sizeof(int32_t) * n));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  HIPCHK(lu_factor_cols<double>(AB, n, n + nrhs, lda, ipiv.as<int32_t>(), info.as<int>(), ws.p, st));
+  HIPCHK(tri_solve_upper<double>(AB, n, lda, AB + n * lda, nrhs, lda, st));
+  HIPCHK(hipEventRecord(e1, st));
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (h) return fail(ctx, CDJ_ERR_SINGULAR, "matrix is exactly singular (zero pivot " + std::to_string(h) + ")");
+  if (report) {
+    report->path = CDJ_PREC_FP64;
+    report->iterations = 0;
+    report->solve_ms = ms;
+    report->residual = -1.0;
+    if (Aorig && Borig) {
+      DBuf r;
+      HIPCHK(r.alloc(sizeof(double) * n));
+      double worst = 0.0;
+      for (int64_t q = 0; q < nrhs; ++q) {
+        HIPCHK(hipMemcpyAsync(r.p, Borig + q * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        gemv<double>(Aorig, n, n, n, AB + (n + q) * lda, r.as<double>(), -1.0, 1.0, st);
+        std::vector<double> hr(n), hb(n);
+        HIPCHK(hipMemcpyAsync(hr.data(), r.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hb.data(), Borig + q * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        double nr = 0, nb = 0;
+        for (int64_t i = 0; i < n; ++i) {
+          nr += hr[i] * hr[i];
+          nb += hb[i] * hb[i];
+        }
+        const double rel = nb > 0 ? std::sqrt(nr / nb) : std::sqrt(nr);
+        worst = std::max(worst, rel);
+      }
+      report->residual = worst;
+    }
+  }
+  return CDJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdj_solve(cdj_ctx* ctx, const double* A, const double* b, int64_t n, int64_t nrhs, double* x,
+              const cdj_opts* opts, cdj_report* report) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0 || nrhs < 0) return fail(ctx, CDJ_ERR_DIMS, "invalid matrix dimensions");
+  if (!A || !b || !x) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  (void)opts;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (n == 0 || nrhs == 0) return CDJ_OK;
+  const int64_t lda = n;
+  DBuf AB, Ao, Bo;
+  HIPCHK(AB.alloc(sizeof(double) * lda * (n + nrhs)));
+  HIPCHK(hipMemcpyAsync(AB.p, A, sizeof(double) * n * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(AB.as<double>() + n * lda, b, sizeof(double) * n * nrhs, hipMemcpyHostToDevice, st));
+  if (report) {
+    HIPCHK(Ao.alloc(sizeof(double) * n * n));
+    HIPCHK(Bo.alloc(sizeof(double) * n * nrhs));
+    HIPCHK(hipMemcpyAsync(Ao.p, AB.p, sizeof(double) * n * n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(Bo.p, AB.as<double>() + n * lda, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st));
+  }
+  int rc = solve_augmented_fp64(ctx, AB.as<double>(), n, lda, nrhs, Ao.as<double>(), Bo.as<double>(), report, st);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(x, AB.as<double>() + n * lda, sizeof(double) * n * nrhs, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,
+                        double G, double lambda, const double* midpoints, const double* normals, int32_t hs_flag,
+                        const double* b, int64_t nrhs, const int32_t* fixed, double* x, double* A_out,
+                        const cdj_opts* opts, cdj_report* report) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0 || nrhs < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  if (!P1 || !P2 || !P3 || !midpoints || !normals || !b || !x) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  if (fixed)
+    for (int64_t i = 0; i < n; ++i)
+      if (fixed[i] == 1) return fail(ctx, CDJ_ERR_ARG, "fixed elements are not supported by this entry point");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (n == 0 || nrhs == 0) return CDJ_OK;
+  const int64_t N3 = 3 * n, lda = N3;
+  const size_t v3 = sizeof(double) * 3 * n;
+  DBuf dn, dm, d1, d2, d3, AB, Ao, Bo;
+  HIPCHK(dn.alloc(v3));
+  HIPCHK(dm.alloc(v3));
+  HIPCHK(d1.alloc(v3));
+  HIPCHK(d2.alloc(v3));
+  HIPCHK(d3.alloc(v3));
+  HIPCHK(hipMemcpyAsync(dn.p, normals, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dm.p, midpoints, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(AB.alloc(sizeof(double) * lda * (N3 + nrhs)));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  int rc = cdj_assemble_dev(ctx, dn.as<double>(), dm.as<double>(), d1.as<double>(), d2.as<double>(), d3.as<double>(),
+                            n, nu, G, lambda, hs_flag, AB.as<double>(), lda, opts, st);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipMemcpyAsync(AB.as<double>() + N3 * lda, b, sizeof(double) * N3 * nrhs, hipMemcpyHostToDevice, st));
+  if (A_out) HIPCHK(hipMemcpyAsync(A_out, AB.p, sizeof(double) * N3 * N3, hipMemcpyDeviceToHost, st));
+  if (report) {
+    HIPCHK(Ao.alloc(sizeof(double) * N3 * N3));
+    HIPCHK(Bo.alloc(sizeof(double) * N3 * nrhs));
+    HIPCHK(hipMemcpyAsync(Ao.p, AB.p, sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(Bo.p, AB.as<double>() + N3 * lda, sizeof(double) * N3 * nrhs, hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  float ams = 0.f;
+  (void)hipEventElapsedTime(&ams, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  rc = solve_augmented_fp64(ctx, AB.as<double>(), N3, lda, nrhs, Ao.as<double>(), Bo.as<double>(), report, st);
+  if (rc) return rc;
+  if (report) report->assembly_ms = ams;
+  HIPCHK(hipMemcpyAsync(x, AB.as<double>() + N3 * lda, sizeof(double) * N3 * nrhs, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+}  // extern "C"

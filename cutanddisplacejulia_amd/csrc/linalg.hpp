@@ -1,0 +1,51 @@
+// Dense linear algebra on gfx950: blocked LU with partial pivoting and triangular solves.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cdj {
+
+static constexpr int LU_NB = 64;  // panel width
+
+// Workspace sizes (bytes) for lu_factor.
+size_t lu_workspace_bytes(int64_t n);
+
+// In-place LU of the n x n column-major matrix A (leading dimension lda) with partial pivoting.
+// ipiv (device, length n) receives 0-based pivot rows; info (device int) receives 0 or 1 + the first zero
+// pivot's index.  ws must hold lu_workspace_bytes(n) bytes.  T = double or float.
+template <class T>
+hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st);
+
+// LU of an n x ncols matrix (ncols >= n): the extra columns are treated as right-hand sides, so on return they
+// hold L^-1 P B.
+template <class T>
+hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
+                          hipStream_t st);
+
+// Row permutation implied by the pivots (row i of P A is row perm[i] of A); perm_dev is a device array.
+// Synchronises the stream.
+hipError_t lu_permutation(const int32_t* ipiv, int64_t n, int32_t* perm_dev, hipStream_t st);
+
+// Blocked triangular solves on the factors (B is n x nrhs, leading dimension ldb).
+template <class T>
+hipError_t tri_solve_lower_unit(const T* LU, int64_t n, int64_t lda, T* B, int64_t nrhs, int64_t ldb, hipStream_t st);
+template <class T>
+hipError_t tri_solve_upper(const T* LU, int64_t n, int64_t lda, T* B, int64_t nrhs, int64_t ldb, hipStream_t st);
+
+// Solve A X = B with the factors of lu_factor (B overwritten by X, n x nrhs, leading dimension ldb).
+template <class T>
+hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int32_t* ipiv, T* B, int64_t nrhs, int64_t ldb,
+                    hipStream_t st);
+
+// y = alpha * A x + beta * y (A: m x n column-major, FP64 accumulate).  Used for residuals.
+template <class TA>
+void gemv(const TA* A, int64_t m, int64_t n, int64_t lda, const double* x, double* y, double alpha, double beta,
+          hipStream_t st);
+
+// Element-wise conversion between precisions.
+void convert_d2f(const double* a, float* b, int64_t n, hipStream_t st);
+void convert_f2d(const float* a, double* b, int64_t n, hipStream_t st);
+void convert_matrix_d2f(const double* A, int64_t lda, float* B, int64_t ldb, int64_t m, int64_t n, hipStream_t st);
+
+}  // namespace cdj

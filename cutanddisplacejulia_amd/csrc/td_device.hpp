@@ -1,0 +1,741 @@
+// Device-side triangular-dislocation (TD) math for gfx950.
+//
+// Layout and work split
+//   * Every source triangle is reduced once, by tdk_setup, to a fixed-size record (FP64 or FP32) holding its
+//     TDCS frame, vertex coordinates, angular-dislocation (AD) trig constants and, for the half-space, the
+//     image triangle and the three free-surface-correction (FSC) side records.  A wavefront always works on
+//     one triangle at a time, so the record address is wave-uniform and the record is read through the
+//     scalar cache into SGPRs (a free broadcast to all 64 lanes).
+//   * Each lane owns one observation point and accumulates the 3 slip modes x 6 strain components (or
+//     3 x 3 displacements) of that pair in registers; the artefact-free configuration (TD.jl:1010-1079) and
+//     the FSC configuration I/II (TD.jl:1943-1974) are per-lane *data* selects, not branches, so all lanes
+//     run the same instruction stream.
+//   * All influences are computed for unit slip and scaled by the caller's slip afterwards; a mode whose
+//     slip is zero contributes exactly zero, as the reference's `if Dn != 0` guards do.
+//   * FSC strains are obtained by forward-mode differentiation (3 directional derivatives carried with every
+//     value) of the FSC displacement closed form: the strain closed form of the reference (AngDisStrainFSC,
+//     TD.jl:2084-2472) is the symmetric gradient of its displacement closed form (AngDisDispFSC,
+//     TD.jl:1412-1496) divided by 4*pi*(1-nu).
+//
+// Citations are to the reference's files; `TD.jl` is src/TD.jl.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "td_kernels.hpp"
+
+namespace cdj {
+
+static constexpr double kPi = 3.141592653589793;
+static constexpr double kEps = 2.220446049250313e-16;
+
+// ---- record layout (element offsets) --------------------------------------------------------------------
+// TD part (one per real or image triangle), 32 elements
+enum : int {
+  R_VN = 0,     // normal (3)
+  R_VS = 3,     // strike (3)
+  R_VD = 6,     // dip (3)
+  R_P2 = 9,     // EFCS P2, TDCS origin (3)
+  R_P1Y = 12,   // TDCS p1 (y, z)
+  R_P1Z = 13,
+  R_P3Y = 14,   // TDCS p3 (y, z)
+  R_P3Z = 15,
+  R_E13Y = 16,  // unit side vectors in TDCS (y, z components only are used)
+  R_E13Z = 17,
+  R_E12Y = 18,
+  R_E12Z = 19,
+  R_E23Y = 20,
+  R_E23Z = 21,
+  R_CA = 22,    // cos/sin(-pi + A), (-pi + B), (-pi + C)
+  R_SA = 23,
+  R_CB = 24,
+  R_SB = 25,
+  R_CC = 26,
+  R_SC = 27,
+  R_FLAT = 28,  // 1 if all three vertices have z == 0
+  R_BASE = 29,  // barycentric denominator of trimodefinder
+  R_TD_SIZE = 32
+};
+// FSC side part, 32 elements
+enum : int {
+  S_SKIP = 0,   // 1 when the side is (near) vertical or |cot(beta)| > 5e3 (TD.jl:1270, 1878)
+  S_PA = 1,     // EFCS start vertex (3)
+  S_EY1 = 4,    // ADCS first axis (x, y); ey2 = (ey1y, -ey1x, 0), ey3 = (0, 0, -1)
+  S_YB = 6,     // side vector in ADCS (3)
+  S_AA = 9,     // -PA.z, -PB.z
+  S_AB = 10,
+  S_B = 11,     // unit-slip Burgers vectors in ADCS, b[mode][k] (9)
+  S_COSB = 20,  // beta
+  S_SINB = 21,
+  S_COTB = 22,
+  S_COTB2 = 23,
+  S_COSPS = 24,  // pi - beta (strain configuration I)
+  S_SINPS = 25,
+  S_COTPS = 26,
+  S_COTPS2 = 27,
+  S_COSPD = 28,  // -pi + beta (displacement configuration I)
+  S_SINPD = 29,
+  S_COTPD = 30,
+  S_COTPD2 = 31,
+  R_SIDE_SIZE = 32
+};
+static_assert(REC_FS == R_TD_SIZE, "full-space record size");
+static_assert(REC_HS == 2 * R_TD_SIZE + 3 * R_SIDE_SIZE, "half-space record size");
+static constexpr int OFF_IMG = R_TD_SIZE;
+static constexpr int OFF_SIDE = 2 * R_TD_SIZE;
+
+// ---- forward-mode dual number: value + d/dy1, d/dy2, d/dy3 ---------------------------------------------
+template <class T> struct D3 {
+  T v, a, b, c;
+};
+template <class T> __device__ __forceinline__ D3<T> mk(T v) { return {v, T(0), T(0), T(0)}; }
+template <class T> __device__ __forceinline__ D3<T> operator+(D3<T> x, D3<T> y) { return {x.v + y.v, x.a + y.a, x.b + y.b, x.c + y.c}; }
+template <class T> __device__ __forceinline__ D3<T> operator-(D3<T> x, D3<T> y) { return {x.v - y.v, x.a - y.a, x.b - y.b, x.c - y.c}; }
+template <class T> __device__ __forceinline__ D3<T> operator-(D3<T> x) { return {-x.v, -x.a, -x.b, -x.c}; }
+template <class T> __device__ __forceinline__ D3<T> operator*(D3<T> x, D3<T> y) {
+  return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b, x.c * y.v + x.v * y.c};
+}
+template <class T> __device__ __forceinline__ D3<T> operator+(D3<T> x, T s) { return {x.v + s, x.a, x.b, x.c}; }
+template <class T> __device__ __forceinline__ D3<T> operator+(T s, D3<T> x) { return {x.v + s, x.a, x.b, x.c}; }
+template <class T> __device__ __forceinline__ D3<T> operator-(D3<T> x, T s) { return {x.v - s, x.a, x.b, x.c}; }
+template <class T> __device__ __forceinline__ D3<T> operator-(T s, D3<T> x) { return {s - x.v, -x.a, -x.b, -x.c}; }
+template <class T> __device__ __forceinline__ D3<T> operator*(D3<T> x, T s) { return {x.v * s, x.a * s, x.b * s, x.c * s}; }
+template <class T> __device__ __forceinline__ D3<T> operator*(T s, D3<T> x) { return {x.v * s, x.a * s, x.b * s, x.c * s}; }
+template <class T> __device__ __forceinline__ D3<T> rcp(D3<T> x) {
+  const T r = T(1) / x.v;
+  const T m = -r * r;
+  return {r, x.a * m, x.b * m, x.c * m};
+}
+template <class T> __device__ __forceinline__ T rcp(T x) { return T(1) / x; }
+template <class T> __device__ __forceinline__ D3<T> dsqrt(D3<T> x) {
+  const T s = sqrt(x.v);
+  const T h = T(0.5) / s;
+  return {s, x.a * h, x.b * h, x.c * h};
+}
+template <class T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
+template <class T> __device__ __forceinline__ D3<T> dlog(D3<T> x) {
+  const T r = T(1) / x.v;
+  return {log(x.v), x.a * r, x.b * r, x.c * r};
+}
+template <class T> __device__ __forceinline__ T dlog(T x) { return log(x); }
+template <class T> __device__ __forceinline__ D3<T> datan(D3<T> x) {
+  const T r = T(1) / (T(1) + x.v * x.v);
+  return {atan(x.v), x.a * r, x.b * r, x.c * r};
+}
+template <class T> __device__ __forceinline__ T datan(T x) { return atan(x); }
+// 2*atan(-y2/D), the FSC Burgers' function (TD.jl:1430), with the derivative written as
+// 2*(y2 dD - D dy2)/(D^2 + y2^2) so it stays finite where D crosses zero.
+template <class T> __device__ __forceinline__ T burgers2(T y2, T D) { return T(2) * atan(-y2 / D); }
+template <class T> __device__ __forceinline__ D3<T> burgers2(D3<T> y2, D3<T> D) {
+  const T f = T(2) / (D.v * D.v + y2.v * y2.v);
+  return {T(2) * atan(-y2.v / D.v), (y2.v * D.a - D.v * y2.a) * f, (y2.v * D.b - D.v * y2.b) * f,
+          (y2.v * D.c - D.v * y2.c) * f};
+}
+
+// ---- small linear algebra -------------------------------------------------------------------------------
+// T' = A T A^T with A given by its three columns c0, c1, c2 (TensorTransformation3D!.jl:35-53 with A in
+// column-major order).  t = {xx, yy, zz, xy, xz, yz}.
+template <class T>
+__device__ __forceinline__ void tensor_rotate(T* t, const T* c0, const T* c1, const T* c2) {
+  // A(i,a) = c_a[i]
+  const T Txx = t[0], Tyy = t[1], Tzz = t[2], Txy = t[3], Txz = t[4], Tyz = t[5];
+  T o[6];
+  const int I[6] = {0, 1, 2, 0, 0, 1};
+  const int J[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int i = I[q], j = J[q];
+    const T a0 = c0[i], a1 = c1[i], a2 = c2[i];
+    const T b0 = c0[j], b1 = c1[j], b2 = c2[j];
+    o[q] = a0 * b0 * Txx + a1 * b1 * Tyy + a2 * b2 * Tzz + (a0 * b1 + a1 * b0) * Txy + (a0 * b2 + a2 * b0) * Txz +
+           (a1 * b2 + a2 * b1) * Tyz;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) t[q] = o[q];
+}
+
+// ADCS -> TDCS for TDSetupS: B = [1 0 0; 0 c s; 0 -s c] (TD.jl:1661-1662), T' = B T B^T.
+template <class T> __device__ __forceinline__ void rotate_b(T* t, T c, T s) {
+  const T Txx = t[0], Tyy = t[1], Tzz = t[2], Txy = t[3], Txz = t[4], Tyz = t[5];
+  const T cc = c * c, ss = s * s, cs = c * s;
+  t[0] = Txx;
+  t[1] = cc * Tyy + T(2) * cs * Tyz + ss * Tzz;
+  t[2] = ss * Tyy - T(2) * cs * Tyz + cc * Tzz;
+  t[3] = c * Txy + s * Txz;
+  t[4] = -s * Txy + c * Txz;
+  t[5] = -cs * Tyy + (cc - ss) * Tyz + cs * Tzz;
+}
+
+// ---- angular dislocation, full space, strain: AngDisStrain TD.jl:1753-1828 ------------------------------
+// o = {xxBx,xxBy,xxBz, yyBx,yyBy,yyBz, zzBx,zzBy,zzBz, xyBx,xyBy,xyBz, xzBx,xzBy,xzBz, yzBx,yzBy,yzBz,
+//      rFi_rx, rFi_ry, rFi_rz}; divisions are replaced by products with 1/r, 1/(r-z), 1/W.
+template <class T> __device__ __forceinline__ void ad_strain(T x, T y, T z, T cosA, T sinA, T nu, T* o) {
+  const T E2 = T(2) * nu + T(1);
+  const T eta = y * cosA - z * sinA;
+  const T zeta = y * sinA + z * cosA;
+  const T x2 = x * x, y2 = y * y, z2 = z * z;
+  const T r2 = x2 + y2 + z2;
+  const T r = sqrt(r2);
+  const T ir = T(1) / r;
+  const T ir2 = ir * ir;
+  const T ir3 = ir2 * ir;
+  const T rmz = r - z;
+  const T irz = T(1) / rmz;
+  const T W = zeta - r;
+  const T iW = T(1) / W;
+  const T iWr = iW * ir;         // 1/(W r)
+  const T iW2 = iW * iW;         // 1/W^2
+  const T iW2r = iW2 * ir;       // 1/(W^2 r)
+  const T iW2r2 = iW2 * ir2;     // 1/(W^2 r^2)
+  const T iWr3 = iW * ir3;       // 1/(W r^3)
+  const T irz1 = ir * irz;       // 1/(r (r-z))
+  const T irz2 = ir2 * irz * irz;  // 1/(r^2 (r-z)^2)
+  const T irz3 = ir3 * irz;      // 1/(r^3 (r-z))
+  const T C = (r * cosA - z) * iWr;
+  const T S = (r * sinA - y) * iWr;
+  const T i4pi = T(1.0 / (4.0 * kPi));
+  // 1/(r - zeta) = -1/W
+  o[18] = (-eta * ir * iW - y * irz1) * i4pi;
+  o[19] = (x * irz1 + cosA * x * ir * iW) * i4pi;
+  o[20] = (-sinA * x * ir * iW) * i4pi;
+  const T Fx = x2 * iW2r2 - x2 * iWr3;
+  o[0] = eta * iWr + eta * Fx + y * irz1 - x2 * y * irz2 - x2 * y * irz3;
+  o[1] = (E2 * iWr + Fx) * cosA + E2 * irz1 - x2 * irz2 - x2 * irz3;
+  o[2] = E2 * iWr + Fx;
+  const T Gy = iWr + S * S - y2 * iWr3;
+  o[3] = Gy * eta + E2 * y * irz1 - y2 * y * irz2 - y2 * y * irz3 - T(2) * nu * cosA * S;
+  o[4] = irz1 - y2 * irz2 - y2 * irz3 + Gy * cosA;
+  o[5] = Gy;
+  const T Gz = iWr + C * C - z2 * iWr3;
+  o[6] = eta * iWr + eta * C * C - eta * z2 * iWr3 + y * z * ir3 + T(2) * nu * sinA * C;
+  o[7] = Gz * cosA + z * ir3;
+  o[8] = Gz;
+  o[9] = x * y2 * irz2 - nu * x * irz1 + x * y2 * irz3 - nu * x * cosA * iWr + eta * x * S * iWr + eta * x * y * iWr3;
+  o[10] = x2 * y * irz2 - nu * y * irz1 + x2 * y * irz3 + nu * cosA * S + x2 * y * cosA * iWr3 + x2 * cosA * S * iWr;
+  o[11] = nu * S + x2 * S * iWr + x2 * y * iWr3;
+  o[12] = -x * y * ir3 + nu * x * sinA * iWr + eta * x * C * iWr + eta * x * z * iWr3;
+  o[13] = -x2 * ir3 + nu * ir + nu * cosA * C + x2 * z * cosA * iWr3 + x2 * cosA * C * iWr;
+  o[14] = nu * C + x2 * C * iWr + x2 * z * iWr3;
+  const T yczs = y * cosA + z * sinA;
+  o[15] = y2 * ir3 - nu * ir - nu * cosA * C + nu * sinA * S + eta * sinA * cosA * iW2 - eta * yczs * iW2r +
+          eta * y * z * iW2r2 - eta * y * z * iWr3;
+  o[16] = y * ir3 + sinA * cosA * cosA * iW2 - cosA * yczs * iW2r + y * z * cosA * iW2r2 - y * z * cosA * iWr3;
+  o[17] = y * z * iWr3 - sinA * cosA * iW2 + yczs * iW2r - y * z * iW2r2;
+}
+
+// ---- angular dislocation, full space, displacement: AngDisDisp TD.jl:1177-1220 ---------------------------
+// o = {ux,uy,uz, vx,vy,vz, wx,wy,wz}
+template <class T> __device__ __forceinline__ void ad_disp(T x, T y, T z, T cosA, T sinA, T nu, T* o) {
+  const T E2 = T(1) - T(2) * nu;
+  const T eta = y * cosA - z * sinA;
+  T zeta = y * sinA + z * cosA;
+  const T r = sqrt(x * x + y * y + z * z);
+  zeta = zeta > r ? r : zeta;  // TD.jl:1190-1195
+  z = z > r ? r : z;
+  const T rMzeta = r - zeta;
+  const T rMz = r - z;
+  const T ir = T(1) / r;
+  const T iz = T(1) / rMzeta;
+  const T iz2 = T(1) / rMz;
+  const T lz = log(rMzeta);
+  const T lz2 = log(rMz);
+  o[0] = x * y * ir * iz2 - x * eta * ir * iz;
+  o[3] = eta * sinA * iz - y * eta * ir * iz + y * y * ir * iz2 + E2 * (cosA * lz - lz2);
+  o[6] = eta * cosA * iz - y * ir - eta * z * ir * iz - E2 * sinA * lz;
+  o[1] = x * x * cosA * ir * iz - x * x * ir * iz2 - E2 * (cosA * lz - lz2);
+  o[4] = y * cosA * ir * iz - sinA * cosA * iz - y * ir * iz2;
+  o[7] = z * cosA * ir * iz - cosA * cosA * iz + ir;
+  o[2] = E2 * lz - x * x * ir * iz;
+  o[5] = sinA * iz - y * ir * iz;
+  o[8] = cosA * iz - z * ir * iz;
+}
+
+// ---- angular dislocation, half-space harmonic part, displacement: AngDisDispFSC TD.jl:1412-1496 ---------
+// S is T (displacement) or D3<T> (displacement plus its gradient).  o = {ux,uy,uz, vx,vy,vz, wx,wy,wz}
+// (u,v,w = component; x,y,z suffix = Burgers component); the 1/(4 pi (1-nu)) factor is applied by callers.
+template <class T, class S>
+__device__ __forceinline__ void fsc_disp(S y1, S y2, S y3, T cosB, T sinB, T cotB, T cotB2, T icosB, T nu, T a, S* o) {
+  const T c1 = T(1) - T(2) * nu;
+  const T c2 = T(1) - nu;
+  const T nu2 = T(2) * nu;
+  const T cot2 = cotB * cotB;
+  const S y3b = y3 + T(2) * a;
+  const S z1b = y1 * cosB + y3b * sinB;
+  const S z3b = y3b * cosB - y1 * sinB;
+  const S rb = dsqrt(y1 * y1 + y2 * y2 + y3b * y3b);
+  const S irb = rcp(rb);
+  const S irb2 = irb * irb;
+  const S irb3 = irb2 * irb;
+  const S W6 = rb + y3b;
+  const S W7 = rb + z3b;
+  const S iW6 = rcp(W6);
+  const S iW7 = rcp(W7);
+  const S lW6 = dlog(W6);
+  const S lW7 = dlog(W7);
+  const S Fib = burgers2(y2, y1 - W6 * cotB2);
+  const S aR = a * irb;
+  const S y3ma = y3b - a;
+  const S rcy = rb * cosB + y3b;   // rb cos(B) + y3b
+  const S rsy = rb * sinB - y1;    // rb sin(B) - y1
+  const S cpa = cosB + aR;          // cos(B) + a/rb
+  const S y22 = y2 * y2;
+  const S y11 = y1 * y1;
+  const S q6 = y3ma * iW6;
+  const S q7 = y3ma * iW7;
+  const S p1 = T(1) + aR * icosB;   // 1 + a/(rb cos(B))
+
+  const S ux = -T(2) * c2 * c1 * cot2 * Fib +
+               c1 * y2 * iW6 * ((c1 - aR) * cotB - y1 * iW6 * (nu + aR)) +
+               c1 * cosB * cotB * y2 * iW7 * cpa +
+               a * cotB * y2 * y3ma * irb3 +
+               y2 * y3ma * irb * iW6 * (-c1 * cotB + y1 * iW6 * (nu2 + aR) + a * y1 * irb2) +
+               y2 * y3ma * irb * iW7 * (cosB * iW7 * (rcy * (c1 * cosB - aR) * cotB + T(2) * c2 * cosB * rsy) -
+                                        a * cosB * cotB * y3b * irb2);
+  const S vx = c1 * ((T(2) * c2 * cot2 - nu) * lW6 - (T(2) * c2 * cot2 + c1) * cosB * lW7) -
+               c1 * iW6 * (y1 * cotB * (c1 - aR) + nu * y3b - a + y22 * iW6 * (nu + aR)) -
+               c1 * cotB * z1b * iW7 * cpa -
+               a * cotB * y1 * y3ma * irb3 +
+               q6 * (-nu2 + irb * (c1 * cotB * y1 - a) + y22 * irb * iW6 * (nu2 + aR) + a * y22 * irb3) +
+               q7 * (cosB * cosB - irb * (c1 * cotB * z1b + a * cosB) + a * cotB * y3b * z1b * irb3 -
+                     irb * iW7 * (cosB * cosB * y22 - a * cotB * z1b * irb * rcy));
+  const S wx = T(2) * c2 * (c1 * cotB * Fib + y2 * iW6 * (nu2 + aR) - cosB * y2 * iW7 * cpa) +
+               y2 * y3ma * irb * (nu2 * iW6 + a * irb2) +
+               cosB * y2 * y3ma * irb * iW7 * (c1 - rcy * iW7 * cpa - a * y3b * irb2);
+  const S uy = c1 * ((T(2) * c2 * cot2 + nu) * lW6 - (T(2) * c2 * cot2 + T(1)) * cosB * lW7) +
+               c1 * iW6 * (-c1 * cotB * y1 + nu * y3b - a + a * cotB * y1 * irb + y11 * iW6 * (nu + aR)) -
+               c1 * cotB * iW7 * (cosB * z1b - a * icosB * rsy * irb) -
+               a * cotB * y1 * y3ma * irb3 +
+               q6 * (nu2 + irb * (c1 * cotB * y1 + a) - y11 * irb * iW6 * (nu2 + aR) - a * y11 * irb3) +
+               cotB * q7 * (-cosB * sinB + a * icosB * y1 * y3b * irb3 + rsy * irb * (T(2) * c2 * cosB - rcy * iW7 * p1));
+  const S vy = T(2) * c2 * c1 * cot2 * Fib +
+               c1 * y2 * iW6 * (-(c1 - aR) * cotB + y1 * iW6 * (nu + aR)) -
+               c1 * cotB * y2 * iW7 * p1 -
+               a * cotB * y2 * y3ma * irb3 +
+               y2 * y3ma * irb * iW6 * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb * (irb + iW6)) +
+               cotB * y2 * y3ma * irb * iW7 * (-T(2) * c2 * cosB + rcy * iW7 * p1 + a * icosB * y3b * irb2);
+  const S wy = -T(2) * c2 * c1 * cotB * (lW6 - cosB * lW7) -
+               T(2) * c2 * y1 * iW6 * (nu2 + aR) +
+               T(2) * c2 * z1b * iW7 * cpa +
+               y3ma * irb * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb2) -
+               q7 * (cosB * sinB + cotB * rcy * irb * (T(2) * c2 * cosB - rcy * iW7) +
+                     aR * (sinB - y3b * z1b * irb2 - z1b * rcy * irb * iW7));
+  const S uz = c1 * (y2 * iW6 * (T(1) + aR) - cosB * y2 * iW7 * cpa) -
+               y2 * y3ma * irb * (a * irb2 + iW6) +
+               cosB * y2 * y3ma * irb * iW7 * (rcy * iW7 * cpa + a * y3b * irb2);
+  const S vz = c1 * (-sinB * lW7 - y1 * iW6 * (T(1) + aR) + z1b * iW7 * cpa) +
+               y1 * y3ma * irb * (a * irb2 + iW6) -
+               q7 * (sinB * (cosB - aR) + z1b * irb * (T(1) + a * y3b * irb2) -
+                     irb * iW7 * (cosB * sinB * y22 - a * z1b * irb * rcy));
+  const S wz = T(2) * c2 * Fib + T(2) * c2 * sinB * y2 * iW7 * cpa +
+               sinB * y2 * y3ma * irb * iW7 * (T(1) + rcy * iW7 * cpa + a * y3b * irb2);
+  o[0] = ux;
+  o[1] = uy;
+  o[2] = uz;
+  o[3] = vx;
+  o[4] = vy;
+  o[5] = vz;
+  o[6] = wx;
+  o[7] = wy;
+  o[8] = wz;
+}
+
+// ---- per-pair drivers -----------------------------------------------------------------------------------
+// TDCS coordinates of an EFCS point (GlobalToTDECoords TD.jl:891) and trimodefinder (TD.jl:1010-1079).
+// Contraction is disabled here so that the configuration decisions round exactly like the oracle's.
+template <class T>
+__device__ __forceinline__ void tdcs_point(const T* __restrict__ r, T X, T Y, T Z, T& x, T& y, T& z, int& mode) {
+#pragma clang fp contract(off)
+  const T dX = X - r[R_P2 + 0], dY = Y - r[R_P2 + 1], dZ = Z - r[R_P2 + 2];
+  x = (r[R_VN + 0] * dX) + (r[R_VN + 1] * dY) + (r[R_VN + 2] * dZ);
+  y = (r[R_VS + 0] * dX) + (r[R_VS + 1] * dY) + (r[R_VS + 2] * dZ);
+  z = (r[R_VD + 0] * dX) + (r[R_VD + 1] * dY) + (r[R_VD + 2] * dZ);
+  // trimodefinder(y, z, x, ...): its (x, y, z) are the TDCS (y, z, x); p2 = (0, 0)
+  const T p1y = r[R_P1Y], p1z = r[R_P1Z], p3y = r[R_P3Y], p3z = r[R_P3Z];
+  const T p2_2_m_p3_2 = T(0) - p3z;
+  const T p3_1_m_p2_1 = p3y - T(0);
+  const T p1_1_m_p3_1 = p1y - p3y;
+  const T p3_2_m_p1_2 = p3z - p1z;
+  const T Base = r[R_BASE];
+  const T a = (p2_2_m_p3_2 * (y - p3y) + p3_1_m_p2_1 * (z - p3z)) / Base;
+  const T b = (p3_2_m_p1_2 * (y - p3y) + p1_1_m_p3_1 * (z - p3z)) / Base;
+  const T c = T(1) - a - b;
+  const bool casen = (a <= T(0) && b > c && c > a) || (b <= T(0) && c > a && a > b) || (c <= T(0) && a > b && b > c);
+  const bool casez = !casen && ((a == T(0) && b >= T(0) && c >= T(0)) || (a >= T(0) && b == T(0) && c >= T(0)) ||
+                                (a >= T(0) && b >= T(0) && c == T(0)));
+  mode = casen ? -1 : ((casez && x == T(0)) ? 0 : 1);
+}
+
+// One angular dislocation of TDSetupS (TD.jl:1623-1750) at unit slip for all three modes, added in TDCS.
+template <class T>
+__device__ __forceinline__ void ad_strain_add(T x, T y, T z, T cosA, T sinA, T ty, T tz, T CosT, T SinT, T nu,
+                                              T acc[3][6]) {
+  const T Y = CosT * (y - ty) - SinT * (z - tz);
+  const T Z = SinT * (y - ty) + CosT * (z - tz);
+  T e[21];
+  ad_strain(x, Y, Z, cosA, sinA, nu, e);
+  const T K = T(1.0 / (8.0 * kPi)) / (T(1) - nu);  // 1/(8 pi (1-nu))
+  const T sK = sinA * K;                            // sinADE1
+  const T rx = e[18], ry = e[19], rz = e[20];
+  {
+    T t[6] = {rx + K * e[0], K * e[3], K * e[6], T(0.5) * ry - K * e[9], T(0.5) * rz - K * e[12], K * e[15]};
+    rotate_b(t, CosT, SinT);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[0][c] += t[c];
+  }
+  // Dss: (by, bz) = (CosT, SinT); Dds: (by, bz) = (-SinT, CosT)   (RotateObject2D TD.jl:1637-1638)
+#pragma unroll
+  for (int m = 1; m < 3; ++m) {
+    const T by = (m == 1) ? CosT : -SinT;
+    const T bz = (m == 1) ? SinT : CosT;
+    const T E4 = by * x * K;
+    const T E5 = by * K;
+    const T bzx = bz * x * sK;
+    const T bzs = bz * sK;
+    T t[6] = {-E4 * e[1] + bzx * e[2],
+              by * ry - E4 * e[4] + bzx * e[5],
+              -E4 * e[7] + bz * rz + bzx * e[8],
+              T(0.5) * by * rx + E5 * e[10] - bzs * e[11],
+              E5 * e[13] + T(0.5) * bz * rx - bzs * e[14],
+              T(0.5) * by * rz - E4 * e[16] + T(0.5) * bz * ry - bzx * e[17]};
+    rotate_b(t, CosT, SinT);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[m][c] += t[c];
+  }
+}
+
+// One angular dislocation of TDSetupD (TD.jl:1081-1173) at unit slip, added in TDCS.
+template <class T>
+__device__ __forceinline__ void ad_disp_add(T x, T y, T z, T cosA, T sinA, T ty, T tz, T CosT, T SinT, T nu,
+                                            T acc[3][3]) {
+  const T Y = CosT * (y - ty) - SinT * (z - tz);
+  const T Z = SinT * (y - ty) + CosT * (z - tz);
+  T u[9];
+  ad_disp(x, Y, Z, cosA, sinA, nu, u);
+  const T K = T(1.0 / (8.0 * kPi)) / (T(1) - nu);
+  const T sK = sinA * K;
+  {
+    const T a = K * u[3], b = K * u[6];
+    acc[0][0] += K * u[0];
+    acc[0][1] += CosT * a + SinT * b;
+    acc[0][2] += -SinT * a + CosT * b;
+  }
+#pragma unroll
+  for (int m = 1; m < 3; ++m) {
+    const T by = (m == 1) ? CosT : -SinT;
+    const T bz = (m == 1) ? SinT : CosT;
+    acc[m][0] += by * K * u[1] + bz * sK * u[2];
+    const T a = by * x * K * u[4] + bz * x * sK * u[5];
+    const T b = by * x * K * u[7] + bz * x * sK * u[8];
+    acc[m][1] += CosT * a + SinT * b;
+    acc[m][2] += -SinT * a + CosT * b;
+  }
+}
+
+// Full-space TD strain of one triangle record at one point, TDCS accumulation (TDstrainFS TD.jl:1528-1595).
+// Returns the configuration (casep 1, casen -1, casez 0).
+template <class T>
+__device__ __forceinline__ int td_strain_tdcs(const T* __restrict__ r, T X, T Y, T Z, T nu, T acc[3][6]) {
+  T x, y, z;
+  int mode;
+  tdcs_point(r, X, Y, Z, x, y, z, mode);
+  const T sg = mode < 0 ? T(-1) : T(1);  // side-vector signs of configuration I vs II
+  // A at p1 with -e13 (I) / e13 (II); B at p2 with e12 / -e12; C at p3 with e23 / -e23
+  ad_strain_add(x, y, z, r[R_CA], r[R_SA], r[R_P1Y], r[R_P1Z], -sg * r[R_E13Z], -sg * r[R_E13Y], nu, acc);
+  ad_strain_add(x, y, z, r[R_CB], r[R_SB], T(0), T(0), sg * r[R_E12Z], sg * r[R_E12Y], nu, acc);
+  ad_strain_add(x, y, z, r[R_CC], r[R_SC], r[R_P3Y], r[R_P3Z], sg * r[R_E23Z], sg * r[R_E23Y], nu, acc);
+  return mode;
+}
+
+template <class T>
+__device__ __forceinline__ int td_disp_tdcs(const T* __restrict__ r, T X, T Y, T Z, T nu, T acc[3][3], T& Fi) {
+  T x, y, z;
+  int mode;
+  tdcs_point(r, X, Y, Z, x, y, z, mode);
+  const T sg = mode < 0 ? T(-1) : T(1);
+  ad_disp_add(x, y, z, r[R_CA], r[R_SA], r[R_P1Y], r[R_P1Z], -sg * r[R_E13Z], -sg * r[R_E13Y], nu, acc);
+  ad_disp_add(x, y, z, r[R_CB], r[R_SB], T(0), T(0), sg * r[R_E12Z], sg * r[R_E12Y], nu, acc);
+  ad_disp_add(x, y, z, r[R_CC], r[R_SC], r[R_P3Y], r[R_P3Z], sg * r[R_E23Z], sg * r[R_E23Y], nu, acc);
+  // Burgers' function (TD.jl:820-835)
+  const T a1 = -x, a2 = r[R_P1Y] - y, a3 = r[R_P1Z] - z;
+  const T b2 = -y, b3 = -z;
+  const T c2 = r[R_P3Y] - y, c3 = r[R_P3Z] - z;
+  const T a12 = a1 * a1;
+  const T na = sqrt(a12 + a2 * a2 + a3 * a3);
+  const T nb = sqrt(a12 + b2 * b2 + b3 * b3);
+  const T nc = sqrt(a12 + c2 * c2 + c3 * c3);
+  const T one = a1 * (b2 * c3 - b3 * c2) - a2 * (a1 * c3 - b3 * a1) + a3 * (a1 * c2 - b2 * a1);
+  const T ab = a1 * a1 + a2 * b2 + a3 * b3;
+  const T ac = a1 * a1 + a2 * c2 + a3 * c3;
+  const T bc = a1 * a1 + b2 * c2 + b3 * c3;
+  const T two = na * nb * nc + ab * nc + ac * nb + bc * na;
+  Fi = T(-2) * atan2(one, two) * T(1.0 / (4.0 * kPi));
+  return mode;
+}
+
+template <class T> __device__ __forceinline__ T qnan() { return __builtin_nan(""); }
+template <> __device__ __forceinline__ float qnan<float>() { return __builtin_nanf(""); }
+
+// Free-surface correction of one side at one point, strain, added in EFCS (AngSetupStrainFSC TD.jl:1935-2075).
+template <class T>
+__device__ __forceinline__ void fsc_strain_side(const T* __restrict__ s, T X, T Y, T Z, T nu, T eps[3][6]) {
+  if (s[S_SKIP] != T(0)) return;  // triangle-uniform
+  T x, y, z;
+  {
+#pragma clang fp contract(off)
+    const T dX = X - s[S_PA + 0], dY = Y - s[S_PA + 1], dZ = Z - s[S_PA + 2];
+    const T p = s[S_EY1 + 0], q = s[S_EY1 + 1];
+    x = (p * dX) + (q * dY) + (T(-0.0) * dZ);
+    y = (q * dX) + ((-p) * dY) + (T(-0.0) * dZ);
+    z = (T(0) * dX) + (T(0) * dY) + (T(-1) * dZ);
+  }
+  const bool cfg1 = x >= T(0);  // beta * x >= 0 with 0 < beta < pi
+  const T sg = cfg1 ? T(-1) : T(1);
+  const T cB = cfg1 ? s[S_COSPS] : s[S_COSB];
+  const T sB = cfg1 ? s[S_SINPS] : s[S_SINB];
+  const T ctB = cfg1 ? s[S_COTPS] : s[S_COTB];
+  const T ctB2 = cfg1 ? s[S_COTPS2] : s[S_COTB2];
+  const T icB = T(1) / cB;
+  const T k4 = T(1.0 / (4.0 * kPi)) / (T(1) - nu);
+  // v = k4 * {d1u1, d2u2, d3u3, d2u1+d1u2, d3u1+d1u3, d3u2+d2u3} of the FSC displacement (the strain-FSC
+  // closed form); w[k][c] = vB - vA per Burgers component k.  Configuration I negates v13, v23 and the first
+  // two Burgers components (TD.jl:1951-1962).
+  T w[3][6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) w[k][c] = T(0);
+#pragma unroll 1
+  for (int ad = 0; ad < 2; ++ad) {
+    const T px = ad == 0 ? x : x - s[S_YB + 0];
+    const T py = ad == 0 ? y : y - s[S_YB + 1];
+    const T pz = ad == 0 ? z : z - s[S_YB + 2];
+    const T aa = ad == 0 ? s[S_AA] : s[S_AB];
+    const T sgn = ad == 0 ? T(-1) : T(1);
+    D3<T> o[9];
+    fsc_disp<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
+                       D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, o);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const D3<T> u1 = o[k], u2 = o[3 + k], u3 = o[6 + k];
+      w[k][0] += sgn * u1.a;
+      w[k][1] += sgn * u2.b;
+      w[k][2] += sgn * u3.c;
+      w[k][3] += sgn * (u1.b + u2.a);
+      w[k][4] += sgn * sg * (u1.c + u3.a);
+      w[k][5] += sgn * sg * (u2.c + u3.b);
+    }
+  }
+  const T c0[3] = {s[S_EY1 + 0], s[S_EY1 + 1], T(0)};
+  const T c1[3] = {s[S_EY1 + 1], -s[S_EY1 + 0], T(0)};
+  const T c2[3] = {T(-0.0), T(-0.0), T(-1)};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const T b1 = sg * s[S_B + 3 * m + 0] * k4, b2 = sg * s[S_B + 3 * m + 1] * k4, b3 = s[S_B + 3 * m + 2] * k4;
+    T e[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const T h = c < 3 ? T(1) : T(0.5);
+      e[c] = h * (b1 * w[0][c] + b2 * w[1][c] + b3 * w[2][c]);
+    }
+    tensor_rotate(e, c0, c1, c2);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) eps[m][c] += e[c];
+  }
+}
+
+// Free-surface correction of one side at one point, displacement, added in EFCS (AngSetupDispFSC
+// TD.jl:1324-1405).
+template <class T>
+__device__ __forceinline__ void fsc_disp_side(const T* __restrict__ s, T X, T Y, T Z, T nu, T u[3][3]) {
+  if (s[S_SKIP] != T(0)) return;
+  T x, y, z;
+  {
+#pragma clang fp contract(off)
+    const T dX = X - s[S_PA + 0], dY = Y - s[S_PA + 1], dZ = Z - s[S_PA + 2];
+    const T p = s[S_EY1 + 0], q = s[S_EY1 + 1];
+    x = (p * dX) + (q * dY) + (T(-0.0) * dZ);
+    y = (q * dX) + ((-p) * dY) + (T(-0.0) * dZ);
+    z = (T(0) * dX) + (T(0) * dY) + (T(-1) * dZ);
+  }
+  const bool cfg1 = x >= T(0);
+  const T cB = cfg1 ? s[S_COSPD] : s[S_COSB];
+  const T sB = cfg1 ? s[S_SINPD] : s[S_SINB];
+  const T ctB = cfg1 ? s[S_COTPD] : s[S_COTB];
+  const T ctB2 = cfg1 ? s[S_COTPD2] : s[S_COTB2];
+  const T icB = T(1) / cB;
+  const T k4 = T(1.0 / (4.0 * kPi)) / (T(1) - nu);
+  T uA[9], uB[9];
+  fsc_disp<T, T>(x, y, z, cB, sB, ctB, ctB2, icB, nu, s[S_AA], uA);
+  fsc_disp<T, T>(x - s[S_YB + 0], y - s[S_YB + 1], z - s[S_YB + 2], cB, sB, ctB, ctB2, icB, nu, s[S_AB], uB);
+  const T p = s[S_EY1 + 0], q = s[S_EY1 + 1];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const T b1 = s[S_B + 3 * m + 0] * k4, b2 = s[S_B + 3 * m + 1] * k4, b3 = s[S_B + 3 * m + 2] * k4;
+    T w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      w[c] = b1 * (uB[3 * c] - uA[3 * c]) + b2 * (uB[3 * c + 1] - uA[3 * c + 1]) + b3 * (uB[3 * c + 2] - uA[3 * c + 2]);
+    // ADCS -> EFCS with [ey1 ey2 ey3] (symmetric)
+    u[m][0] += p * w[0] + q * w[1] + T(-0.0) * w[2];
+    u[m][1] += q * w[0] - p * w[1] + T(-0.0) * w[2];
+    u[m][2] += T(0) * w[0] + T(0) * w[1] - w[2];
+  }
+}
+
+// All of one triangle's strain influence at one EFCS point, unit slip, EFCS result
+// (ComputeStrainInfluences TD.jl:670-708).
+template <class T, bool HS>
+__device__ __forceinline__ void strain_pair(const T* __restrict__ rec, T X, T Y, T Z, T nu, T eps[3][6]) {
+  T acc[3][6];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[m][c] = T(0);
+  const int mode = td_strain_tdcs(rec, X, Y, Z, nu, acc);
+  const T bad = mode == 0 ? qnan<T>() : T(0);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[m][c] += bad;
+    tensor_rotate(acc[m], rec + R_VN, rec + R_VS, rec + R_VD);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) eps[m][c] = acc[m][c];
+  }
+  if constexpr (HS) {
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) fsc_strain_side(rec + OFF_SIDE + k * R_SIDE_SIZE, X, Y, Z, nu, eps);
+    const T* ri = rec + OFF_IMG;
+    const bool flat = ri[R_FLAT] != T(0);
+    if (!flat) {
+      // image contribution rotated from the image TDCS and added (TD.jl:1509-1516, 1597-1602)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[m][c] = T(0);
+      const int mi = td_strain_tdcs(ri, X, Y, Z, nu, acc);
+      const T badi = mi == 0 ? qnan<T>() : T(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[m][c] += badi;
+        tensor_rotate(acc[m], ri + R_VN, ri + R_VS, ri + R_VD);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) eps[m][c] += acc[m][c];
+      }
+    } else {
+      // Triangle lying in z = 0: the reference rotates the accumulated tensor into the image TDCS in place,
+      // then its xz/yz sign flips rebind local copies (TD.jl:1518-1525, 1610-1617), so the caller keeps the
+      // image-TDCS xz/yz and receives xx..xy of the rotated-back flipped copy.
+      const T ex[3] = {ri[R_VN + 0], ri[R_VS + 0], ri[R_VD + 0]};
+      const T ey[3] = {ri[R_VN + 1], ri[R_VS + 1], ri[R_VD + 1]};
+      const T ez[3] = {ri[R_VN + 2], ri[R_VS + 2], ri[R_VD + 2]};
+      T keep[3][2];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[m][c] = eps[m][c];
+        tensor_rotate(acc[m], ex, ey, ez);
+        keep[m][0] = acc[m][4];
+        keep[m][1] = acc[m][5];
+        acc[m][4] = -acc[m][4];
+        acc[m][5] = -acc[m][5];
+      }
+      const int mi = td_strain_tdcs(ri, X, Y, Z, nu, acc);
+      const T badi = mi == 0 ? qnan<T>() : T(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[m][c] += badi;
+        tensor_rotate(acc[m], ri + R_VN, ri + R_VS, ri + R_VD);
+        eps[m][0] = acc[m][0];
+        eps[m][1] = acc[m][1];
+        eps[m][2] = acc[m][2];
+        eps[m][3] = acc[m][3];
+        eps[m][4] = keep[m][0];
+        eps[m][5] = keep[m][1];
+      }
+    }
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void rot_to_efcs(const T* __restrict__ r, const T* v, T* o) {
+  o[0] = r[R_VN + 0] * v[0] + r[R_VS + 0] * v[1] + r[R_VD + 0] * v[2];
+  o[1] = r[R_VN + 1] * v[0] + r[R_VS + 1] * v[1] + r[R_VD + 1] * v[2];
+  o[2] = r[R_VN + 2] * v[0] + r[R_VS + 2] * v[1] + r[R_VD + 2] * v[2];
+}
+
+// All of one triangle's displacement influence at one EFCS point (ComputeDispInfluences TD.jl:631-668).
+template <class T, bool HS>
+__device__ __forceinline__ void disp_pair(const T* __restrict__ rec, T X, T Y, T Z, T nu, T u[3][3]) {
+  T acc[3][3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[m][c] = T(0);
+  T Fi;
+  const int mode = td_disp_tdcs(rec, X, Y, Z, nu, acc, Fi);
+  const T bad = mode == 0 ? qnan<T>() : T(0);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[m][c] += bad;
+    acc[m][m] += Fi;  // Dn -> x, Dss -> y, Dds -> z in TDCS
+    rot_to_efcs(rec, acc[m], u[m]);
+  }
+  if constexpr (HS) {
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) fsc_disp_side(rec + OFF_SIDE + k * R_SIDE_SIZE, X, Y, Z, nu, u);
+    const T* ri = rec + OFF_IMG;
+    const bool flat = ri[R_FLAT] != T(0);
+    if (!flat) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[m][c] = T(0);
+      T Fii;
+      const int mi = td_disp_tdcs(ri, X, Y, Z, nu, acc, Fii);
+      const T badi = mi == 0 ? qnan<T>() : T(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[m][c] += badi;
+        acc[m][m] += Fii;
+        T o[3];
+        rot_to_efcs(ri, acc[m], o);
+        u[m][0] += o[0];
+        u[m][1] += o[1];
+        u[m][2] += o[2];
+      }
+    } else {
+      // flat image triangle: the Uz sign flip rebinds a local copy (TD.jl:759-763, 854-864)
+      T keep[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const T v0 = u[m][0], v1 = u[m][1], v2 = u[m][2];
+        acc[m][0] = ri[R_VN + 0] * v0 + ri[R_VN + 1] * v1 + ri[R_VN + 2] * v2;
+        acc[m][1] = ri[R_VS + 0] * v0 + ri[R_VS + 1] * v1 + ri[R_VS + 2] * v2;
+        acc[m][2] = ri[R_VD + 0] * v0 + ri[R_VD + 1] * v1 + ri[R_VD + 2] * v2;
+        keep[m] = acc[m][2];
+        acc[m][2] = -acc[m][2];
+      }
+      T Fii;
+      const int mi = td_disp_tdcs(ri, X, Y, Z, nu, acc, Fii);
+      const T badi = mi == 0 ? qnan<T>() : T(0);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[m][c] += badi;
+        acc[m][m] += Fii;
+        T o[3];
+        rot_to_efcs(ri, acc[m], o);
+        u[m][0] = o[0];
+        u[m][1] = o[1];
+        u[m][2] = keep[m];
+      }
+    }
+  }
+}
+
+}  // namespace cdj

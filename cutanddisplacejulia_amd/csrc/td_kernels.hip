@@ -1,0 +1,458 @@
+// TD kernels for gfx950: triangle/receiver set-up, influence matrices, summed fields and the fused
+// strain -> traction assembly of the collocation matrix.
+#include <hip/hip_runtime.h>
+
+#include "td_device.hpp"
+#include "td_kernels.hpp"
+
+namespace cdj {
+
+// ---- per-triangle records (CalculateLocalTriCoords! TD.jl:712-732, GlobalToTDECoords TD.jl:870-908,
+//      CalcTDVectsAndAngles TD.jl:962-1008, CalcSideVec TD.jl:935-960, AngSetup*FSC set-up) -------------
+__device__ static void build_td(const double* P1, const double* P2, const double* P3, double* r) {
+#pragma clang fp contract(off)
+  double a[3], b[3], Vn[3], Vs[3], Vd[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = P2[k] - P1[k];
+    b[k] = P3[k] - P1[k];
+  }
+  Vn[0] = a[1] * b[2] - a[2] * b[1];
+  Vn[1] = a[2] * b[0] - a[0] * b[2];
+  Vn[2] = a[0] * b[1] - a[1] * b[0];
+  double nrm = sqrt(Vn[0] * Vn[0] + Vn[1] * Vn[1] + Vn[2] * Vn[2]);
+  for (int k = 0; k < 3; ++k) Vn[k] = Vn[k] / nrm;
+  // Vstrike = eZ x Vnorm, or eY * Vnorm_z for a horizontal triangle
+  Vs[0] = 0.0 * Vn[2] - 1.0 * Vn[1];
+  Vs[1] = 1.0 * Vn[0] - 0.0 * Vn[2];
+  Vs[2] = 0.0 * Vn[1] - 0.0 * Vn[0];
+  if (Vs[0] == 0.0 && Vs[1] == 0.0 && Vs[2] == 0.0) {
+    Vs[0] = 0.0 * Vn[2];
+    Vs[1] = 1.0 * Vn[2];
+    Vs[2] = 0.0 * Vn[2];
+  }
+  nrm = sqrt(Vs[0] * Vs[0] + Vs[1] * Vs[1] + Vs[2] * Vs[2]);
+  for (int k = 0; k < 3; ++k) Vs[k] = Vs[k] / nrm;
+  Vd[0] = Vn[1] * Vs[2] - Vn[2] * Vs[1];
+  Vd[1] = Vn[2] * Vs[0] - Vn[0] * Vs[2];
+  Vd[2] = Vn[0] * Vs[1] - Vn[1] * Vs[0];
+  double p1[3], p3[3];
+  {
+    const double d1[3] = {P1[0] - P2[0], P1[1] - P2[1], P1[2] - P2[2]};
+    const double d3[3] = {P3[0] - P2[0], P3[1] - P2[1], P3[2] - P2[2]};
+    p1[0] = (Vn[0] * d1[0]) + (Vn[1] * d1[1]) + (Vn[2] * d1[2]);
+    p1[1] = (Vs[0] * d1[0]) + (Vs[1] * d1[1]) + (Vs[2] * d1[2]);
+    p1[2] = (Vd[0] * d1[0]) + (Vd[1] * d1[1]) + (Vd[2] * d1[2]);
+    p3[0] = (Vn[0] * d3[0]) + (Vn[1] * d3[1]) + (Vn[2] * d3[2]);
+    p3[1] = (Vs[0] * d3[0]) + (Vs[1] * d3[1]) + (Vs[2] * d3[2]);
+    p3[2] = (Vd[0] * d3[0]) + (Vd[1] * d3[1]) + (Vd[2] * d3[2]);
+  }
+  double e12[3], e13[3], e23[3];
+  for (int k = 0; k < 3; ++k) {
+    e12[k] = 0.0 - p1[k];
+    e13[k] = p3[k] - p1[k];
+    e23[k] = p3[k] - 0.0;
+  }
+  nrm = sqrt(e12[0] * e12[0] + e12[1] * e12[1] + e12[2] * e12[2]);
+  for (int k = 0; k < 3; ++k) e12[k] = e12[k] / nrm;
+  nrm = sqrt(e13[0] * e13[0] + e13[1] * e13[1] + e13[2] * e13[2]);
+  for (int k = 0; k < 3; ++k) e13[k] = e13[k] / nrm;
+  nrm = sqrt(e23[0] * e23[0] + e23[1] * e23[1] + e23[2] * e23[2]);
+  for (int k = 0; k < 3; ++k) e23[k] = e23[k] / nrm;
+  const double A = acos((e12[0] * e13[0]) + (e12[1] * e13[1]) + (e12[2] * e13[2]));
+  const double B = acos((-e12[0] * e23[0]) + (-e12[1] * e23[1]) + (-e12[2] * e23[2]));
+  const double C = acos((e23[0] * e13[0]) + (e23[1] * e13[1]) + (e23[2] * e13[2]));
+  for (int k = 0; k < 3; ++k) {
+    r[R_VN + k] = Vn[k];
+    r[R_VS + k] = Vs[k];
+    r[R_VD + k] = Vd[k];
+    r[R_P2 + k] = P2[k];
+  }
+  r[R_P1Y] = p1[1];
+  r[R_P1Z] = p1[2];
+  r[R_P3Y] = p3[1];
+  r[R_P3Z] = p3[2];
+  r[R_E13Y] = e13[1];
+  r[R_E13Z] = e13[2];
+  r[R_E12Y] = e12[1];
+  r[R_E12Z] = e12[2];
+  r[R_E23Y] = e23[1];
+  r[R_E23Z] = e23[2];
+  r[R_CA] = cos(-kPi + A);
+  r[R_SA] = sin(-kPi + A);
+  r[R_CB] = cos(-kPi + B);
+  r[R_SB] = sin(-kPi + B);
+  r[R_CC] = cos(-kPi + C);
+  r[R_SC] = sin(-kPi + C);
+  r[R_FLAT] = (P1[2] == 0.0 && P2[2] == 0.0 && P3[2] == 0.0) ? 1.0 : 0.0;
+  // trimodefinder denominator with p2 = (0, 0): (p2z - p3z)(p1y - p3y) + (p3y - p2y)(p1z - p3z)
+  r[R_BASE] = ((0.0 - p3[2]) * (p1[1] - p3[1]) + (p3[1] - 0.0) * (p1[2] - p3[2]));
+  r[30] = 0.0;
+  r[31] = 0.0;
+}
+
+__device__ static double dcot(double x) { return 1.0 / tan(x); }
+
+__device__ static void build_side(const double* PA, const double* PB, const double* rtd, double* s) {
+#pragma clang fp contract(off)
+  const double sv[3] = {PB[0] - PA[0], PB[1] - PA[1], PB[2] - PA[2]};
+  const double Sv = sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]);
+  const double beta = acos((((-sv[0]) * 0.0) - sv[1] * 0.0 - sv[2] * 1.0) / Sv);
+  for (int k = 0; k < 32; ++k) s[k] = 0.0;
+  const bool skip = fabs(beta) < kEps || fabs(kPi - beta) < kEps || fabs(dcot(beta)) > 5e3;
+  s[S_SKIP] = skip ? 1.0 : 0.0;
+  if (skip) return;
+  double p = sv[0], q = sv[1];
+  const double n1 = sqrt(p * p + q * q + 0.0 * 0.0);
+  p = p / n1;
+  q = q / n1;
+  for (int k = 0; k < 3; ++k) s[S_PA + k] = PA[k];
+  s[S_EY1 + 0] = p;
+  s[S_EY1 + 1] = q;
+  // side vector in ADCS: rot3 with (ey1, ey2, ey3), ey2 = (q, -p, 0), ey3 = (-0, -0, -1)
+  s[S_YB + 0] = (p * sv[0]) + (q * sv[1]) + (-0.0 * sv[2]);
+  s[S_YB + 1] = (q * sv[0]) + ((-p) * sv[1]) + (-0.0 * sv[2]);
+  s[S_YB + 2] = (0.0 * sv[0]) + (0.0 * sv[1]) + (-1.0 * sv[2]);
+  s[S_AA] = -PA[2];
+  s[S_AB] = -PB[2];
+  // unit slip of mode m in EFCS is Vn / Vs / Vd; then EFCS -> ADCS
+  const double* V[3] = {rtd + R_VN, rtd + R_VS, rtd + R_VD};
+  for (int m = 0; m < 3; ++m) {
+    const double e0 = V[m][0], e1 = V[m][1], e2 = V[m][2];
+    s[S_B + 3 * m + 0] = (p * e0) + (q * e1) + (-0.0 * e2);
+    s[S_B + 3 * m + 1] = (q * e0) + ((-p) * e1) + (-0.0 * e2);
+    s[S_B + 3 * m + 2] = (0.0 * e0) + (0.0 * e1) + (-1.0 * e2);
+  }
+  s[S_COSB] = cos(beta);
+  s[S_SINB] = sin(beta);
+  s[S_COTB] = dcot(beta);
+  s[S_COTB2] = dcot(beta / 2);
+  const double ps = kPi - beta;
+  s[S_COSPS] = cos(ps);
+  s[S_SINPS] = sin(ps);
+  s[S_COTPS] = dcot(ps);
+  s[S_COTPS2] = dcot(ps / 2);
+  const double pd = -kPi + beta;
+  s[S_COSPD] = cos(pd);
+  s[S_SINPD] = sin(pd);
+  s[S_COTPD] = dcot(pd);
+  s[S_COTPD2] = dcot(pd / 2);
+}
+
+__global__ void tdk_setup(const double* __restrict__ P1, const double* __restrict__ P2,
+                          const double* __restrict__ P3, int64_t ntri, int hs, double* __restrict__ rec) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= ntri) return;
+  const double a[3] = {P1[j], P1[ntri + j], P1[2 * ntri + j]};
+  const double b[3] = {P2[j], P2[ntri + j], P2[2 * ntri + j]};
+  const double c[3] = {P3[j], P3[ntri + j], P3[2 * ntri + j]};
+  const int RS = hs ? REC_HS : REC_FS;
+  double* r = rec + j * RS;
+  build_td(a, b, c, r);
+  if (hs) {
+    const double ai[3] = {a[0], a[1], -a[2]};
+    const double bi[3] = {b[0], b[1], -b[2]};
+    const double ci[3] = {c[0], c[1], -c[2]};
+    build_td(ai, bi, ci, r + OFF_IMG);
+    build_side(a, b, r, r + OFF_SIDE);
+    build_side(b, c, r, r + OFF_SIDE + R_SIDE_SIZE);
+    build_side(c, a, r, r + OFF_SIDE + 2 * R_SIDE_SIZE);
+  }
+}
+
+__global__ void tdk_rec_to_f32(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// ---- receivers: collocation points and plane directions --------------------------------------------------
+// SetupCollationPoints (CalculateInfluenceMatrices3D.jl:52-71) and CalculateSSandDSDirs.jl:37-98.
+// recv[i*RECV + ...] = {x, y, z, nx, ny, nz, sx, sy, sz, dx, dy, dz}
+__global__ void tdk_receivers(const double* __restrict__ nrm, const double* __restrict__ mid, int64_t n,
+                              double* __restrict__ recv) {
+#pragma clang fp contract(off)
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double nx = nrm[i], ny = nrm[n + i], nz = nrm[2 * n + i];
+  double* o = recv + i * RECV;
+  o[0] = mid[i] - (nx * 1e-12);
+  o[1] = mid[n + i] - (ny * 1e-12);
+  o[2] = mid[2 * n + i] - (nz * 1e-12);
+  o[3] = nx;
+  o[4] = ny;
+  o[5] = nz;
+  const double h = sqrt(nx * nx + ny * ny + 0.0 * 0.0);
+  double sx = -(ny / h), sy = nx / h, sz = 0.0;
+  const double DSCosAz = cos(acos(nz) - (kPi / 2));
+  double L = sin(acos(DSCosAz));
+  if (nz < 0) L = -L;
+  const double a = atan2(ny, nx);
+  double dx = -(cos(a) * L), dy = -(sin(a) * L), dz = DSCosAz;
+  const double vs0 = 0.0 * nz - 1.0 * ny, vs1 = 1.0 * nx - 0.0 * nz, vs2 = 0.0 * ny - 0.0 * nx;
+  if (vs0 == 0.0 && vs1 == 0.0 && vs2 == 0.0) {
+    dx = -1.0;
+    dy = 0.0;
+    dz = 0.0;
+    sx = 0.0;
+    sy = nz < 1.0 ? -1.0 : 1.0;
+    sz = 0.0;
+  }
+  o[6] = sx;
+  o[7] = sy;
+  o[8] = sz;
+  o[9] = dx;
+  o[10] = dy;
+  o[11] = dz;
+}
+
+// ---- HS validity check (ViewInLoop TD.jl:606-619) --------------------------------------------------------
+__global__ void tdk_check_hs(const double* __restrict__ Z, int64_t nobs, const double* __restrict__ P1,
+                             const double* __restrict__ P2, const double* __restrict__ P3, int64_t ntri,
+                             int* __restrict__ flag) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < nobs) bad = Z[i] > 0;
+  if (i < ntri) bad = bad || P1[2 * ntri + i] > 0 || P2[2 * ntri + i] > 0 || P3[2 * ntri + i] > 0;
+  if (bad) atomicOr(flag, 1);
+}
+
+// ---- influence matrices ----------------------------------------------------------------------------------
+// grid.x: 256-point tiles of observation points, grid.y: chunks of `tpb` triangles.  Each lane loops over the
+// chunk; the triangle index is wave-uniform so its record comes through scalar loads.
+template <class T, bool HS>
+__global__ void __launch_bounds__(256) tdk_influence(const T* __restrict__ X, const T* __restrict__ Y,
+                                                     const T* __restrict__ Z, int64_t nobs,
+                                                     const T* __restrict__ rec, int64_t ntri, int tpb,
+                                                     const double* __restrict__ Dn, const double* __restrict__ Dss,
+                                                     const double* __restrict__ Dds, T nu, OutPtrs<T> out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nobs) return;
+  const T Xi = X[i], Yi = Y[i], Zi = Z[i];
+  const int64_t j0 = (int64_t)blockIdx.y * tpb;
+  const int64_t j1 = j0 + tpb < ntri ? j0 + tpb : ntri;
+  constexpr int RS = HS ? REC_HS : REC_FS;
+  for (int64_t j = j0; j < j1; ++j) {
+    const T* r = rec + j * RS;
+    const T sl[3] = {(T)Dn[j], (T)Dss[j], (T)Dds[j]};
+    const int64_t o = i + j * out.ld;
+    if (out.strain_on) {
+      T e[3][6];
+      strain_pair<T, HS>(r, Xi, Yi, Zi, nu, e);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) out.p[6 * m + c][o] = sl[m] == T(0) ? T(0) : sl[m] * e[m][c];
+    }
+    if (out.disp_on) {
+      T u[3][3];
+      disp_pair<T, HS>(r, Xi, Yi, Zi, nu, u);
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out.p[18 + 3 * m + c][o] = sl[m] == T(0) ? T(0) : sl[m] * u[m][c];
+    }
+  }
+}
+
+// ---- summed field: per-chunk partial sums, then an in-order reduction ------------------------------------
+// part[(chunk * ncomp + comp) * nobs + i]; comps: 6 strain (if on) then 3 displacement (if on).
+template <class T, bool HS>
+__global__ void __launch_bounds__(256) tdk_field_part(const T* __restrict__ X, const T* __restrict__ Y,
+                                                      const T* __restrict__ Z, int64_t nobs,
+                                                      const T* __restrict__ rec, int64_t ntri, int tpb,
+                                                      const double* __restrict__ Dn, const double* __restrict__ Dss,
+                                                      const double* __restrict__ Dds, T nu, int strain_on,
+                                                      int disp_on, T* __restrict__ part) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nobs) return;
+  const T Xi = X[i], Yi = Y[i], Zi = Z[i];
+  const int64_t j0 = (int64_t)blockIdx.y * tpb;
+  const int64_t j1 = j0 + tpb < ntri ? j0 + tpb : ntri;
+  constexpr int RS = HS ? REC_HS : REC_FS;
+  T se[6] = {0, 0, 0, 0, 0, 0}, su[3] = {0, 0, 0};
+  for (int64_t j = j0; j < j1; ++j) {
+    const T* r = rec + j * RS;
+    const T sl[3] = {(T)Dn[j], (T)Dss[j], (T)Dds[j]};
+    if (strain_on) {
+      T e[3][6];
+      strain_pair<T, HS>(r, Xi, Yi, Zi, nu, e);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const T a = sl[0] == T(0) ? T(0) : sl[0] * e[0][c];
+        const T b = sl[1] == T(0) ? T(0) : sl[1] * e[1][c];
+        const T d = sl[2] == T(0) ? T(0) : sl[2] * e[2][c];
+        se[c] += (a + b) + d;
+      }
+    }
+    if (disp_on) {
+      T u[3][3];
+      disp_pair<T, HS>(r, Xi, Yi, Zi, nu, u);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T a = sl[0] == T(0) ? T(0) : sl[0] * u[0][c];
+        const T b = sl[1] == T(0) ? T(0) : sl[1] * u[1][c];
+        const T d = sl[2] == T(0) ? T(0) : sl[2] * u[2][c];
+        su[c] += (a + b) + d;
+      }
+    }
+  }
+  const int ncomp = (strain_on ? 6 : 0) + (disp_on ? 3 : 0);
+  T* p = part + (int64_t)blockIdx.y * ncomp * nobs + i;
+  int q = 0;
+  if (strain_on)
+    for (int c = 0; c < 6; ++c) p[(q++) * nobs] = se[c];
+  if (disp_on)
+    for (int c = 0; c < 3; ++c) p[(q++) * nobs] = su[c];
+}
+
+template <class T>
+__global__ void tdk_field_reduce(const T* __restrict__ part, int64_t nobs, int ncomp, int nchunk,
+                                 double* __restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nobs * ncomp) return;
+  double s = 0.0;
+  for (int k = 0; k < nchunk; ++k) s += (double)part[(int64_t)k * ncomp * nobs + t];
+  out[t] = s;
+}
+
+// ---- fused strain -> stress -> traction assembly (ConvertStrainInfMatsToTraction,
+//      CalculateInfluenceMatrices3D.jl:73-142) -------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void traction3(const T* e, T lam, T G, const T* n, const T* s, const T* d, T& Tn, T& Tss,
+                                          T& Tds) {
+  const T G2 = T(2) * G;
+  const T cons = lam * (e[0] + e[1] + e[2]);
+  const T Sxx = G2 * e[0] + cons, Syy = G2 * e[1] + cons, Szz = G2 * e[2] + cons;
+  const T Sxy = G2 * e[3], Sxz = G2 * e[4], Syz = G2 * e[5];
+  const T nx = n[0], ny = n[1], nz = n[2];
+  Tn = Sxx * nx * nx + Syy * ny * ny + Szz * nz * nz + T(2) * Sxy * nx * ny + T(2) * Syz * ny * nz +
+       T(2) * Sxz * nz * nx;
+  const T tx = Sxx * nx + Sxy * ny + Sxz * nz;
+  const T ty = Sxy * nx + Syy * ny + Syz * nz;
+  const T tz = Sxz * nx + Syz * ny + Szz * nz;
+  // ((I - n n^T) t) . dir
+  const T px = tx * (T(1) - nx * nx) - ty * (nx * ny) - tz * (nz * nx);
+  const T py = -tx * (nx * ny) + ty * (T(1) - ny * ny) - tz * (ny * nz);
+  const T pz = -tx * (nz * nx) - ty * (ny * nz) + tz * (T(1) - nz * nz);
+  Tss = px * s[0] + py * s[1] + pz * s[2];
+  Tds = px * d[0] + py * d[1] + pz * d[2];
+}
+
+// out.p[3*m + q] is the block of mode m (Dn, Dss, Dds) and traction q (Tn, Tss, Tds); out.sign = -1 writes
+// A = -[...] (SlipCalculator3D.jl:13-16).  Block pointers are already offset, leading dimension out.ld.
+template <class T, bool HS>
+__global__ void __launch_bounds__(256) tdk_traction(const double* __restrict__ recv, int64_t nrecv,
+                                                    const T* __restrict__ rec, int64_t ntri, int tpb, T nu, T lam,
+                                                    T G, OutPtrs<T> out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nrecv) return;
+  const double* rv = recv + i * RECV;
+  const T Xi = (T)rv[0], Yi = (T)rv[1], Zi = (T)rv[2];
+  const T n[3] = {(T)rv[3], (T)rv[4], (T)rv[5]};
+  const T s[3] = {(T)rv[6], (T)rv[7], (T)rv[8]};
+  const T d[3] = {(T)rv[9], (T)rv[10], (T)rv[11]};
+  const int64_t j0 = (int64_t)blockIdx.y * tpb;
+  const int64_t j1 = j0 + tpb < ntri ? j0 + tpb : ntri;
+  constexpr int RS = HS ? REC_HS : REC_FS;
+  const T sg = out.sign;
+  for (int64_t j = j0; j < j1; ++j) {
+    const T* r = rec + j * RS;
+    T e[3][6];
+    strain_pair<T, HS>(r, Xi, Yi, Zi, nu, e);
+    const int64_t o = i + j * out.ld;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      T tn, tss, tds;
+      traction3(e[m], lam, G, n, s, d, tn, tss, tds);
+      out.p[3 * m + 0][o] = sg * tn;
+      out.p[3 * m + 1][o] = sg * tss;
+      out.p[3 * m + 2][o] = sg * tds;
+    }
+  }
+}
+
+// ---- launch helpers --------------------------------------------------------------------------------------
+void launch_setup(const double* P1, const double* P2, const double* P3, int64_t ntri, int hs, double* rec,
+                  hipStream_t st) {
+  if (ntri <= 0) return;
+  hipLaunchKernelGGL(tdk_setup, dim3((unsigned)((ntri + 127) / 128)), dim3(128), 0, st, P1, P2, P3, ntri, hs, rec);
+}
+
+void launch_rec_to_f32(const double* a, float* b, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tdk_rec_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, n);
+}
+
+void launch_receivers(const double* nrm, const double* mid, int64_t n, double* recv, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tdk_receivers, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, nrm, mid, n, recv);
+}
+
+void launch_check_hs(const double* Z, int64_t nobs, const double* P1, const double* P2, const double* P3,
+                     int64_t ntri, int* flag, hipStream_t st) {
+  const int64_t n = nobs > ntri ? nobs : ntri;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tdk_check_hs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Z, nobs, P1, P2, P3, ntri,
+                     flag);
+}
+
+template <class T>
+void launch_influence(const T* X, const T* Y, const T* Z, int64_t nobs, const T* rec, int64_t ntri, int hs, int tpb,
+                      const double* Dn, const double* Dss, const double* Dds, T nu, const OutPtrs<T>& out,
+                      hipStream_t st) {
+  if (nobs <= 0 || ntri <= 0) return;
+  dim3 grid((unsigned)((nobs + 255) / 256), (unsigned)((ntri + tpb - 1) / tpb));
+  if (hs)
+    hipLaunchKernelGGL((tdk_influence<T, true>), grid, dim3(256), 0, st, X, Y, Z, nobs, rec, ntri, tpb, Dn, Dss, Dds,
+                       nu, out);
+  else
+    hipLaunchKernelGGL((tdk_influence<T, false>), grid, dim3(256), 0, st, X, Y, Z, nobs, rec, ntri, tpb, Dn, Dss, Dds,
+                       nu, out);
+}
+
+template <class T>
+void launch_field(const T* X, const T* Y, const T* Z, int64_t nobs, const T* rec, int64_t ntri, int hs, int tpb,
+                  const double* Dn, const double* Dss, const double* Dds, T nu, int strain_on, int disp_on, T* part,
+                  double* out, hipStream_t st) {
+  if (nobs <= 0) return;
+  const int ncomp = (strain_on ? 6 : 0) + (disp_on ? 3 : 0);
+  const int nchunk = ntri > 0 ? (int)((ntri + tpb - 1) / tpb) : 0;
+  if (nchunk > 0) {
+    dim3 grid((unsigned)((nobs + 255) / 256), (unsigned)nchunk);
+    if (hs)
+      hipLaunchKernelGGL((tdk_field_part<T, true>), grid, dim3(256), 0, st, X, Y, Z, nobs, rec, ntri, tpb, Dn, Dss,
+                         Dds, nu, strain_on, disp_on, part);
+    else
+      hipLaunchKernelGGL((tdk_field_part<T, false>), grid, dim3(256), 0, st, X, Y, Z, nobs, rec, ntri, tpb, Dn, Dss,
+                         Dds, nu, strain_on, disp_on, part);
+  }
+  const int64_t tot = nobs * ncomp;
+  hipLaunchKernelGGL((tdk_field_reduce<T>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part, nobs, ncomp,
+                     nchunk, out);
+}
+
+template <class T>
+void launch_traction(const double* recv, int64_t nrecv, const T* rec, int64_t ntri, int hs, int tpb, T nu, T lam, T G,
+                     const OutPtrs<T>& out, hipStream_t st) {
+  if (nrecv <= 0 || ntri <= 0) return;
+  dim3 grid((unsigned)((nrecv + 255) / 256), (unsigned)((ntri + tpb - 1) / tpb));
+  if (hs)
+    hipLaunchKernelGGL((tdk_traction<T, true>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu, lam, G, out);
+  else
+    hipLaunchKernelGGL((tdk_traction<T, false>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu, lam, G, out);
+}
+
+template void launch_influence<double>(const double*, const double*, const double*, int64_t, const double*, int64_t,
+                                       int, int, const double*, const double*, const double*, double,
+                                       const OutPtrs<double>&, hipStream_t);
+template void launch_influence<float>(const float*, const float*, const float*, int64_t, const float*, int64_t, int,
+                                      int, const double*, const double*, const double*, float, const OutPtrs<float>&,
+                                      hipStream_t);
+template void launch_field<double>(const double*, const double*, const double*, int64_t, const double*, int64_t, int,
+                                   int, const double*, const double*, const double*, double, int, int, double*,
+                                   double*, hipStream_t);
+template void launch_traction<double>(const double*, int64_t, const double*, int64_t, int, int, double, double, double,
+                                      const OutPtrs<double>&, hipStream_t);
+template void launch_traction<float>(const double*, int64_t, const float*, int64_t, int, int, float, float, float,
+                                     const OutPtrs<float>&, hipStream_t);
+
+}  // namespace cdj

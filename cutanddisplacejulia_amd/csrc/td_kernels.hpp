@@ -1,0 +1,43 @@
+// Host-side launch interface of the TD kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cdj {
+
+static constexpr int RECV = 12;  // receiver record: point, normal, strike dir, dip dir
+static constexpr int REC_FS = 32;   // triangle record, full space (elements)
+static constexpr int REC_HS = 160;  // triangle record, half space: real + image TD + 3 free-surface sides
+
+// Output descriptor passed by value to the kernels.
+//   influence: p[0..17] strain (StrainInf order), p[18..26] displacement (DispInf order); ld = nobs
+//   traction:  p[3*m + q] block of mode m (Dn, Dss, Dds) x traction q (Tn, Tss, Tds), leading dimension ld,
+//              every value multiplied by `sign`
+template <class T> struct OutPtrs {
+  T* p[27];
+  int64_t ld;
+  int strain_on;
+  int disp_on;
+  T sign;
+};
+
+void launch_setup(const double* P1, const double* P2, const double* P3, int64_t ntri, int hs, double* rec,
+                  hipStream_t st);
+void launch_rec_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
+void launch_receivers(const double* nrm, const double* mid, int64_t n, double* recv, hipStream_t st);
+void launch_check_hs(const double* Z, int64_t nobs, const double* P1, const double* P2, const double* P3,
+                     int64_t ntri, int* flag, hipStream_t st);
+template <class T>
+void launch_influence(const T* X, const T* Y, const T* Z, int64_t nobs, const T* rec, int64_t ntri, int hs, int tpb,
+                      const double* Dn, const double* Dss, const double* Dds, T nu, const OutPtrs<T>& out,
+                      hipStream_t st);
+template <class T>
+void launch_field(const T* X, const T* Y, const T* Z, int64_t nobs, const T* rec, int64_t ntri, int hs, int tpb,
+                  const double* Dn, const double* Dss, const double* Dds, T nu, int strain_on, int disp_on, T* part,
+                  double* out, hipStream_t st);
+template <class T>
+void launch_traction(const double* recv, int64_t nrecv, const T* rec, int64_t ntri, int hs, int tpb, T nu, T lam, T G,
+                     const OutPtrs<T>& out, hipStream_t st);
+
+}  // namespace cdj

@@ -1,0 +1,125 @@
+/* cdj.h - C ABI of the MI355X (gfx950) triangular-dislocation engine.
+ *
+ * Drop-in replacement for the compute path of CutAndDisplaceJulia: TD (src/TD.jl), CalculateInfluenceMatrices3D
+ * (src/CalculateInfluenceMatrices3D.jl) and SlipCalculator3D (src/SlipCalculator3D.jl).  Host Julia binds these
+ * symbols with ccall; the Python mirror in cutanddisplacejulia_amd/api.py binds them with ctypes.
+ *
+ * Conventions (those of the reference's Julia arrays):
+ *   - all host pointers are caller-owned; the library never keeps them after a call returns;
+ *   - P1, P2, P3, normals and midpoints are N x 3, column-major (coordinate k of triangle j at [k*N + j]);
+ *   - influence matrices are nobs x ntri column-major (receiver i, source j at [i + j*nobs]);
+ *   - the collocation matrix A is 3N x 3N column-major with row blocks (Tn, Tss, Tds) and column blocks
+ *     (Dn, Dss, Dds), A = -[DnTn DssTn DdsTn; DnTss DssTss DdsTss; DnTds DssTds DdsTds];
+ *   - every function returns a status code (CDJ_OK == 0); cdj_last_error gives the message, using the
+ *     reference's own error text where the reference raises one;
+ *   - calls on one context are synchronous and must not overlap.
+ *   - `_dev` entry points take device pointers (HBM-resident data) and a hipStream_t passed as void*.
+ */
+#ifndef CDJ_H
+#define CDJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CDJ_OK 0
+#define CDJ_ERR_DIMS 1           /* "Element size inputs must be the same dimensions" (TD.jl:424) */
+#define CDJ_ERR_HALFSPACE 2      /* "Half-space solution: Z coordinates must be negative!" (TD.jl:617) */
+#define CDJ_ERR_DEGENERATE 3     /* zero-length triangle edge (CreateMidPoint!.jl:41-43) */
+#define CDJ_ERR_DEVICE 4         /* HIP runtime error */
+#define CDJ_ERR_NOMEM 5          /* device allocation failed */
+#define CDJ_ERR_NOT_CONVERGED 6  /* iterative refinement did not reach the tolerance; fallback result given */
+#define CDJ_ERR_ARG 7            /* invalid argument (null pointer, bad option) */
+#define CDJ_ERR_SINGULAR 8       /* exactly singular matrix (zero pivot) */
+
+/* precision of the traction-matrix assembly and of the factorisation */
+#define CDJ_PREC_FP64 0       /* FP64 assembly, FP64 LU (the reference's arithmetic) */
+#define CDJ_PREC_MIXED32 1    /* FP32 far-field + FP64 near-field assembly, FP32 LU, FP64 GMRES-IR */
+
+typedef struct cdj_ctx cdj_ctx;
+
+typedef struct cdj_opts {
+  int32_t precision;   /* CDJ_PREC_* */
+  int32_t tile;        /* triangles per workgroup column in the TD kernels; 0 = library default */
+  double tol;          /* target FP64 relative residual ||b - A x|| / ||b|| for refined solves; 0 = 1e-10 */
+  int32_t max_iter;    /* iteration cap of the refinement; 0 = default */
+  double near_factor;  /* near-field radius in triangle edge lengths (FP64 recompute); 0 = 2.0 */
+} cdj_opts;
+
+typedef struct cdj_report {
+  int32_t path;            /* precision path that produced x (CDJ_PREC_*) */
+  int32_t iterations;      /* refinement iterations (0 for a direct FP64 solve) */
+  double residual;         /* final FP64 relative residual ||b - A x|| / ||b|| (max over right-hand sides) */
+  double assembly_ms;      /* device time of the assembly */
+  double solve_ms;         /* device time of the factorisation and solve */
+  int64_t near_entries;    /* entries recomputed in FP64 by the near-field pass */
+} cdj_report;
+
+/* Context: one per process and device. */
+int cdj_create(int device, cdj_ctx** ctx);
+int cdj_destroy(cdj_ctx* ctx);
+const char* cdj_last_error(const cdj_ctx* ctx);
+
+/* TD with influence matrices: replaces TD(X,Y,Z,P1,P2,P3,Dss,Dds,Dn,nu,G,DispFlag,StrainFlag,HSflag[,StrainInf,
+ * DispInf]) (TD.jl:77-133, 135-216).  strain: 18 nobs x ntri matrices in StrainInf field order (exxDn, eyyDn,
+ * ezzDn, exyDn, exzDn, eyzDn, then Dss, then Dds), NULL when strain_flag != 1; disp: 9 matrices in DispInf order
+ * (DnUx, DnUy, DnUz, DssUx, ..., DdsUz), NULL when disp_flag != 1. */
+int cdj_td_influence(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
+                     const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
+                     const double* Dds, const double* Dn, double nu, double G, int32_t disp_flag,
+                     int32_t strain_flag, int32_t hs_flag, double* const* strain, double* const* disp);
+
+/* TD summed field: replaces TD(..., ::Strains, ::Disps) (TD.jl:219-410).  strain6 = {exx, eyy, ezz, exy, exz,
+ * eyz}, disp3 = {Ux, Uy, Uz}, each of length nobs.  The sum over triangles is in a fixed order, so results are
+ * reproducible run to run (the reference's threaded accumulation at TD.jl:322-324, 375-380 is not). */
+int cdj_td_field(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs, const double* P1,
+                 const double* P2, const double* P3, int64_t ntri, const double* Dss, const double* Dds,
+                 const double* Dn, double nu, double G, int32_t disp_flag, int32_t strain_flag, int32_t hs_flag,
+                 double* const* strain6, double* const* disp3);
+
+/* Traction influence matrices: replaces CalculateInfluenceMatrices3D(FaceNormalVector,MidPoint,P1,P2,P3,nu,G,
+ * lambda,FixedEls,HSFlag,n) (CalculateInfluenceMatrices3D.jl:2-48) for FixedEls all zero (fixed must be NULL or
+ * all zero).  T: 9 N x N matrices in TractionInf field order (DnTn, DnTss, DnTds, DssTn, DssTss, DssTds, DdsTn,
+ * DdsTss, DdsTds). */
+int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
+                           const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                           int32_t hs_flag, const int32_t* fixed, double* const* T, const cdj_opts* opts);
+
+/* Slip solve: replaces the base method of SlipCalculator3D (SlipCalculator3D.jl:1-33).  b: 3N x nrhs
+ * column-major right-hand sides [Tn; Tss; Tds] as built by SetupTractionVector; x: 3N x nrhs solution
+ * [Dn; Dss; Dds]; A_out: 3N x 3N (the assembled A) or NULL. */
+int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,
+                        double G, double lambda, const double* midpoints, const double* normals, int32_t hs_flag,
+                        const double* b, int64_t nrhs, const int32_t* fixed, double* x, double* A_out,
+                        const cdj_opts* opts, cdj_report* report);
+
+/* Dense solve of A x = b (the `A\b` of SlipCalculator3D.jl:22): blocked LU with partial pivoting on the GPU. */
+int cdj_solve(cdj_ctx* ctx, const double* A, const double* b, int64_t n, int64_t nrhs, double* x,
+              const cdj_opts* opts, cdj_report* report);
+
+/* ---- device-pointer entry points (inputs already in HBM) ---- */
+
+/* Assemble A (3N x 3N, leading dimension lda) on `stream` from device arrays. */
+int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
+                     const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                     int32_t hs_flag, double* A, int64_t lda, const cdj_opts* opts, void* stream);
+
+/* TD influence on device arrays; same layout as cdj_td_influence. */
+int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
+                         const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
+                         const double* Dds, const double* Dn, double nu, double G, int32_t disp_flag,
+                         int32_t strain_flag, int32_t hs_flag, double* const* strain, double* const* disp,
+                         void* stream);
+
+/* LU factorisation in place (FP64, partial pivoting; ipiv 0-based, length n) and solve with the factors. */
+int cdj_lu_factor_dev(cdj_ctx* ctx, double* A, int64_t n, int64_t lda, int32_t* ipiv, void* stream);
+int cdj_lu_solve_dev(cdj_ctx* ctx, const double* LU, int64_t n, int64_t lda, const int32_t* ipiv, double* B,
+                     int64_t nrhs, int64_t ldb, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDJ_H */

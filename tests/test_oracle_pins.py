@@ -1,0 +1,167 @@
+"""Pins of the CPU FP64 oracle against closed forms and invariants (no GPU).
+
+* Okada (1985) surface solution, half space, displacement and strain, the reference's own test geometry
+  (test/test_TDvsOkada.jl) - the reference asserts 1e-13; |.| is used here (the reference's signed max is
+  one-sided).
+* strain == sym(grad u) by central differences, full and half space.
+* AngDisStrainFSC == sym(grad AngDisDispFSC) / (4 pi (1 - nu)) by central differences.
+* Displacement jump across the TD interior equals the Burgers vector (full space).
+* Far field of the TD decays like a point dislocation (strain ~ r^-3).
+"""
+import numpy as np
+import pytest
+
+from oracle import analytic as A
+from oracle import oracle as O
+
+
+def test_okada_surface_halfspace():
+    P1, P2, P3, md = A.okada_test_fault()
+    xx = np.linspace(-10, 10, 50)
+    X, Y = np.meshgrid(xx, xx)
+    x, y = X.ravel(order="F"), Y.ravel(order="F")
+    z = np.zeros_like(x)
+    Dds, Dn, Dss = 2.0, 0.6, 0.5
+    s, d = O.td_field(x, y, z, P1, P2, P3, [Dss] * 2, [Dds] * 2, [Dn] * 2, 0.25, 1.0, 1, 1, 1)
+    tot = np.sqrt(Dds ** 2 + Dss ** 2)
+    rake = 90 - np.degrees(np.arctan(Dss / Dds))
+    ok = A.okada85_surface(x, y, md, 60, 45, 5, 5, rake, tot, Dn, 0.25)
+    for mine, ref in zip(d + s, ok):
+        assert np.max(np.abs(mine - ref)) < 1e-13
+
+
+def test_okada_surface_refined_mesh():
+    """The same rectangle split into 2 k^2 triangles still matches Okada (uniform slip)."""
+    P1, P2, P3, md = A.okada_test_fault()
+    # corners of the rectangle from the two reference triangles
+    c = {tuple(np.round(p, 12)): p for p in np.vstack([P1, P2, P3])}
+    pts = np.array(list(c.values()))
+    top = pts[np.argsort(pts[:, 2])[-2:]]
+    bot = pts[np.argsort(pts[:, 2])[:2]]
+    # order so that top[0]-bot[0] is a dip edge
+    if np.linalg.norm(top[0] - bot[0]) > np.linalg.norm(top[0] - bot[1]):
+        bot = bot[::-1]
+    k = 4
+    grid = np.zeros((k + 1, k + 1, 3))
+    for i in range(k + 1):
+        for j in range(k + 1):
+            a, b = i / k, j / k
+            grid[i, j] = (1 - a) * ((1 - b) * top[0] + b * top[1]) + a * ((1 - b) * bot[0] + b * bot[1])
+    Q1, Q2, Q3 = [], [], []
+    for i in range(k):
+        for j in range(k):
+            p, q, r, s_ = grid[i, j], grid[i + 1, j], grid[i + 1, j + 1], grid[i, j + 1]
+            Q1 += [p, p]
+            Q2 += [q, r]
+            Q3 += [r, s_]
+    Q1, Q2, Q3 = map(np.array, (Q1, Q2, Q3))
+    xx = np.linspace(-10, 10, 15)
+    X, Y = np.meshgrid(xx, xx)
+    x, y = X.ravel(), Y.ravel()
+    z = np.zeros_like(x)
+    n = Q1.shape[0]
+    nv = np.cross(Q2 - Q1, Q3 - Q1)
+    # uniform Dss along the strike; orientation of each triangle's frame must agree with the 2-triangle mesh
+    nv0 = np.cross(P2[0] - P1[0], P3[0] - P1[0])
+    sgn = np.sign(nv @ nv0)
+    assert np.all(sgn > 0)
+    s, d = O.td_field(x, y, z, Q1, Q2, Q3, [0.5] * n, [2.0] * n, [0.6] * n, 0.25, 1.0, 1, 1, 1)
+    s0, d0 = O.td_field(x, y, z, P1, P2, P3, [0.5] * 2, [2.0] * 2, [0.6] * 2, 0.25, 1.0, 1, 1, 1)
+    for a, b in zip(d + s, d0 + s0):
+        assert np.max(np.abs(a - b)) < 1e-12 * max(1.0, np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("hs", [0, 1])
+def test_strain_is_symmetric_gradient_of_displacement(hs):
+    rng = np.random.default_rng(1)
+    P1 = np.array([[-1.0, -0.5, -2.0]])
+    P2 = np.array([[1.0, -0.3, -2.7]])
+    P3 = np.array([[0.2, 1.1, -1.5]])
+    pts = rng.uniform(-3, 3, (12, 3))
+    pts[:, 2] = -np.abs(pts[:, 2]) - 0.1
+    h = 1e-5
+    one = [1.0]
+    for p in pts:
+        s, _ = O.td(p[0:1], p[1:2], p[2:3], P1, P2, P3, one, one, one, 0.25, 1.0, 0, 1, hs)
+        g = np.zeros((3, 3, 3))
+        for k in range(3):
+            e = np.zeros(3)
+            e[k] = h
+            _, dp = O.td([p[0] + e[0]], [p[1] + e[1]], [p[2] + e[2]], P1, P2, P3, one, one, one, 0.25, 1.0, 1, 0, hs)
+            _, dm = O.td([p[0] - e[0]], [p[1] - e[1]], [p[2] - e[2]], P1, P2, P3, one, one, one, 0.25, 1.0, 1, 0, hs)
+            for m, md in enumerate(["Dn", "Dss", "Dds"]):
+                for c, cc in enumerate("xyz"):
+                    g[m, c, k] = (dp[md + "U" + cc][0, 0] - dm[md + "U" + cc][0, 0]) / (2 * h)
+        for m, md in enumerate(["Dn", "Dss", "Dds"]):
+            G = g[m]
+            eps = 0.5 * (G + G.T)
+            mine = np.array([s[k + md][0, 0] for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz")])
+            fd = np.array([eps[0, 0], eps[1, 1], eps[2, 2], eps[0, 1], eps[0, 2], eps[1, 2]])
+            assert np.max(np.abs(mine - fd)) <= 1e-7 * max(1e-3, np.max(np.abs(fd)))
+
+
+def test_fsc_strain_is_gradient_of_fsc_displacement():
+    rng = np.random.default_rng(3)
+    nu = 0.25
+    f = 1 / (4 * np.pi * (1 - nu))
+    for _ in range(60):
+        y = rng.uniform(-3, 3, 3)
+        beta = rng.uniform(0.2, 2.9)
+        a = rng.uniform(0.2, 3)
+        v = O.ang_dis_strain_fsc(y[0], y[1], y[2], beta, nu, a)
+        h = 1e-6
+        g = np.zeros((3, 3, 3))
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = h
+            du = (O.ang_dis_disp_fsc(*(y + e), beta, nu, a) - O.ang_dis_disp_fsc(*(y - e), beta, nu, a)) / (2 * h)
+            for c in range(3):
+                for k in range(3):
+                    g[k, c, d] = du[3 * c + k]
+        for k in range(3):
+            G = g[k] * f
+            fd = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[0, 2] + G[2, 0], G[1, 2] + G[2, 1]])
+            cf = v[6 * k:6 * k + 6]
+            assert np.max(np.abs(cf - fd)) <= 1e-6 * max(1e-6, np.max(np.abs(cf)))
+
+
+def test_displacement_jump_equals_burgers_vector():
+    """Full space: u(just below) - u(just above) across the TD interior equals the slip vector."""
+    P1 = np.array([[-1.0, -1.0, 0.0]])
+    P2 = np.array([[1.0, -1.0, 0.0]])
+    P3 = np.array([[0.0, 1.0, 0.0]])
+    Dss, Dds, Dn = 0.3, -0.7, 0.5
+    eps = 1e-9
+    up = O.td([0.1], [-0.2], [eps], P1, P2, P3, [Dss], [Dds], [Dn], 0.25, 1.0, 1, 0, 0)[1]
+    dn = O.td([0.1], [-0.2], [-eps], P1, P2, P3, [Dss], [Dds], [Dn], 0.25, 1.0, 1, 0, 0)[1]
+    jump = np.array([sum(up[m + "U" + c][0, 0] - dn[m + "U" + c][0, 0] for m in ("Dn", "Dss", "Dds")) for c in "xyz"])
+    # flat triangle with upward normal: Vnorm = +z, Vstrike = +y, Vdip = -x (TD.jl:719-730)
+    expected = Dn * np.array([0, 0, 1.0]) + Dss * np.array([0, 1.0, 0]) + Dds * np.array([-1.0, 0, 0])
+    assert np.allclose(np.abs(jump), np.abs(expected), atol=1e-6)
+    assert np.allclose(np.abs(jump).sum(), np.abs(expected).sum(), atol=1e-6)
+
+
+def test_far_field_decay():
+    P1 = np.array([[-0.1, -0.1, -5.0]])
+    P2 = np.array([[0.1, -0.1, -5.0]])
+    P3 = np.array([[0.0, 0.1, -5.1]])
+    d = np.array([1.0, 0.3, -0.4])
+    d /= np.linalg.norm(d)
+    vals = []
+    for r in (20.0, 40.0, 80.0):
+        p = np.array([0, 0, -5.0]) + r * d
+        s, _ = O.td([p[0]], [p[1]], [p[2]], P1, P2, P3, [1.0], [0.0], [0.0], 0.25, 1.0, 0, 1, 0)
+        vals.append(np.sqrt(sum(s[k][0, 0] ** 2 for k in s)))
+    assert abs(np.log2(vals[0] / vals[1]) - 3) < 0.05
+    assert abs(np.log2(vals[1] / vals[2]) - 3) < 0.03
+
+
+def test_halfspace_rejects_points_above_surface():
+    with pytest.raises(O.OracleError, match="Z coordinates must be negative"):
+        O.td([0.0], [0.0], [0.5], [[0, 0, -1.0]], [[1, 0, -1.0]], [[0, 1, -1.0]], [1.0], [1.0], [1.0], 0.25, 1.0, 0, 1, 1)
+
+
+def test_size_mismatch_error():
+    with pytest.raises(O.OracleError, match="same dimensions"):
+        O.td([0.0], [0.0], [-0.5], [[0, 0, -1.0]], [[1, 0, -1.0]], [[0, 1, -1.0]], [1.0, 2.0], [1.0], [1.0], 0.25, 1.0,
+             0, 1, 1)
