@@ -1,0 +1,175 @@
+#!/usr/bin/env python
+"""Benchmark of the triangular-dislocation hot path on one MI355X.
+
+Workload (the half-space configuration the headline metric is quoted on): a 45-degree dipping rectangular fault
+(strike 60, 5 x 5, top at depth 0.5) meshed into 50 x 50 quads = 5000 triangles; one step assembles the full
+15000 x 15000 FP64 collocation matrix A (CalculateInfluenceMatrices3D + the block concatenation of
+SlipCalculator3D) from HBM-resident geometry.  value = A entries per second over the timed steps.
+
+Also reported: the FP64 LU solve of the same A (`A\\b`), the dominant kernel's roofline position (algorithmic
+FP64 flops from the oracle's operation count per receiver/source pair) and the CPU baseline (the FP64 C++
+restatement of TD.jl, OpenMP over source triangles like TD.jl:146) on a bounded sample of receiver rows.
+
+python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TD influence entries/s; slip-solve time at 1e-10 resid, mixed/learned vs FP64"
+# Algorithmic work of one (receiver, source) pair of the half-space traction assembly: +,-,*,/ of the FP64
+# oracle instantiated with an operation-counting scalar, averaged over 4000 random pairs of this workload
+# (tests/test_flops.py recomputes it).  Transcendentals (12 sqrt, 30 trig in set-up) are not included.
+FLOPS_PER_PAIR_HS = 43940.0
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def _log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads):
+    """The oracle (FP64 C++ port of TD.jl) on `rows` receiver rows against all sources; entries/s."""
+    from oracle import oracle as O
+    n = P1.shape[0]
+    O.traction_influence_rows(nrm, mid, P1, P2, P3, nu, G, lam, 1, 0, 8, nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    O.traction_influence_rows(nrm, mid, P1, P2, P3, nu, G, lam, 1, 0, rows, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return 9.0 * rows * n / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=50, help="quads per side of the fault mesh (50 -> 5000 triangles)")
+    ap.add_argument("--cpu-rows", type=int, default=256, help="receiver rows timed for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every GPU box here has one MI355X and this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from cutanddisplacejulia_amd import _lib, api, meshes
+
+    c = meshes.config2(args.n)
+    P1, P2, P3 = c["P1"], c["P2"], c["P3"]
+    n = P1.shape[0]
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    K, E, lam, nu, G = api.ElasticConstantsCheck(1.0, 0.25)
+    N3 = 3 * n
+    dev = torch.device("cuda:0")
+
+    def dcol(a):  # N x 3 column-major on the device
+        return torch.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).to(dev)
+
+    dP1, dP2, dP3, dN, dM = (dcol(a) for a in (P1, P2, P3, nrm, mid))
+    A = torch.empty((N3, N3), dtype=torch.float64, device=dev)  # column-major view: A[col, row] storage
+    L = _lib.load()
+    ctx = _lib.ctx()
+    stream = torch.cuda.current_stream(dev)
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    opts = _lib.Opts(0, 0, 1e-10, 0, 2.0)
+
+    def step():
+        rc = L.cdj_assemble_dev(ctx, ctypes.c_void_p(dN.data_ptr()), ctypes.c_void_p(dM.data_ptr()),
+                                ctypes.c_void_p(dP1.data_ptr()), ctypes.c_void_p(dP2.data_ptr()),
+                                ctypes.c_void_p(dP3.data_ptr()), n, nu, G, lam, 1, ctypes.c_void_p(A.data_ptr()), N3,
+                                ctypes.byref(opts), sp)
+        _lib.check(rc)
+        return L.cdj_last_kernel_ms(ctx)
+
+    _log("workload: %d triangles, A %d x %d (half space)" % (n, N3, N3))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kms.append(step())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    entries = float(N3) * N3
+    value = entries * args.steps / elapsed
+    kavg = float(np.mean(kms))
+    pairs = float(n) * n
+    achieved_tf = FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12
+    _log("assembly: %.2f ms/step, kernel %.2f ms, %.3e entries/s, %.1f TFLOP/s" % (ms_per_step, kavg, value, achieved_tf))
+
+    solve = None
+    if not args.no_solve:
+        # FP64 LU of the assembled A with one right-hand side (remote sigma_zz = 1)
+        Tn, Tds, Tss = api.SetupTractionVector(api.Stresses(0, 0, 1.0, 0, 0, 0), nrm, n, G, lam)
+        b = np.concatenate([Tn, Tss, Tds])
+        Ab = torch.empty((N3 + 1, N3), dtype=torch.float64, device=dev)
+        Ab[:N3].copy_(A)
+        Ab[N3].copy_(torch.from_numpy(b).to(dev))
+        ipiv = torch.empty(N3, dtype=torch.int32, device=dev)
+        A0 = A.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.cdj_lu_factor_dev(ctx, ctypes.c_void_p(Ab.data_ptr()), N3, N3, ctypes.c_void_p(ipiv.data_ptr()), sp))
+        x = Ab[N3].clone()
+        _lib.check(L.cdj_lu_solve_dev(ctx, ctypes.c_void_p(Ab.data_ptr()), N3, N3, ctypes.c_void_p(ipiv.data_ptr()),
+                                      ctypes.c_void_p(x.data_ptr()), 1, N3, sp))
+        torch.cuda.synchronize()
+        t_lu = time.perf_counter() - t0
+        # FP64 residual ||b - A x|| / ||b||  (A0 storage is column-major: A0[col, row])
+        r = torch.from_numpy(b).to(dev) - torch.mv(A0.t(), x)
+        res = float(torch.linalg.norm(r) / torch.linalg.norm(torch.from_numpy(b).to(dev)))
+        lu_tf = (2.0 / 3.0) * N3 ** 3 / t_lu / 1e12
+        _log("FP64 LU+solve: %.1f ms (%.1f TFLOP/s), residual %.2e" % (1e3 * t_lu, lu_tf, res))
+        solve = {"fp64_lu_ms": 1e3 * t_lu, "fp64_lu_tflops": lu_tf, "fp64_residual": res, "n": N3}
+        del Ab, A0
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        rows = args.cpu_rows
+        cv, cdt = cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads)
+        _log("CPU baseline: %.3e entries/s (%d threads, %d rows, %.1f s)" % (cv, threads, rows, cdt))
+        cpu = {"value": cv, "unit": "entries/s", "cores": threads, "kind": "port",
+               "sample": "%d receiver rows x %d sources of the same assembly (%.1f s)" % (rows, n, cdt)}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "entries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "half-space collocation assembly: 45-degree dipping fault, %d triangles, "
+                               "A %d x %d FP64" % (n, N3, N3), "n_tri": n, "hs": 1},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tf / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "tdk_traction<double, true>", "kernel_ms": kavg,
+                     "flops_per_pair": FLOPS_PER_PAIR_HS, "pairs_per_launch": pairs},
+        "cpu_baseline": cpu,
+        "solve": solve,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -30,7 +30,7 @@ CDJ_PREC_FP64 = 0
 CDJ_PREC_MIXED32 = 1
 
 # every symbol declared in include/cdj.h
-EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_td_influence", "cdj_td_field",
+EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", "cdj_td_influence", "cdj_td_field",
            "cdj_traction_influence", "cdj_slip_calculator", "cdj_solve", "cdj_assemble_dev",
            "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev"]
 
@@ -70,6 +70,7 @@ def load(path=LIB_PATH):
     L.cdj_destroy.argtypes = [_p]
     L.cdj_last_error.argtypes = [_p]
     L.cdj_last_error.restype = ctypes.c_char_p
+    L.cdj_last_kernel_ms.argtypes = [_p]
     td_args = [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _d, _d, _i32, _i32, _i32, _p, _p]
     L.cdj_td_influence.argtypes = td_args
     L.cdj_td_field.argtypes = td_args
@@ -82,7 +83,9 @@ def load(path=LIB_PATH):
     L.cdj_lu_solve_dev.argtypes = [_p, _p, _i64, _i64, _p, _p, _i64, _i64, _p]
     for name in EXPORTS:
         f = getattr(L, name)
-        if name != "cdj_last_error":
+        if name == "cdj_last_kernel_ms":
+            f.restype = ctypes.c_double
+        elif name != "cdj_last_error":
             f.restype = ctypes.c_int
     _LIB = L
     return L

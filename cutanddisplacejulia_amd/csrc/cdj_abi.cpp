@@ -10,6 +10,7 @@
 
 #include "../../include/cdj.h"
 #include "linalg.hpp"
+#include "mixed.hpp"
 #include "td_kernels.hpp"
 
 using namespace cdj;
@@ -18,6 +19,12 @@ struct cdj_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // grow-only device scratch for per-call set-up records (triangle and receiver records)
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // HIP events around the last assembly / TD kernel launch, on the stream it ran on
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
 };
 
 namespace {
@@ -52,6 +59,20 @@ struct DBuf {
   }
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
+
+// Device scratch of at least `bytes` owned by the context (not valid across overlapping calls).
+hipError_t ctx_scratch(cdj_ctx* c, size_t bytes, void** out) {
+  if (bytes > c->scratch_bytes) {
+    if (c->scratch) (void)hipFree(c->scratch);
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+    hipError_t e = hipMalloc(&c->scratch, bytes);
+    if (e != hipSuccess) return e;
+    c->scratch_bytes = bytes;
+  }
+  *out = c->scratch;
+  return hipSuccess;
+}
 
 int default_tile(int hs, int64_t nobs) {
   (void)nobs;
@@ -109,6 +130,11 @@ int cdj_create(int device, cdj_ctx** out) {
   }
   cdj_ctx* c = new cdj_ctx();
   c->device = device;
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    g_create_err = "hipEventCreate failed";
+    delete c;
+    return CDJ_ERR_DEVICE;
+  }
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_create_err = hipGetErrorString(e);
@@ -123,11 +149,21 @@ int cdj_destroy(cdj_ctx* ctx) {
   if (!ctx) return CDJ_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   delete ctx;
   return CDJ_OK;
 }
 
 const char* cdj_last_error(const cdj_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+double cdj_last_kernel_ms(cdj_ctx* ctx) {
+  if (!ctx || !ctx->timed) return -1.0;
+  float ms = -1.f;
+  if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) return -1.0;
+  return (double)ms;
+}
 
 int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
                          const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
@@ -292,15 +328,16 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   if (n == 0) return CDJ_OK;
   const int hs = hs_flag == 1;
-  DBuf recv, rec;
-  HIPCHK(recv.alloc(sizeof(double) * RECV * n));
   const int RS = hs ? REC_HS : REC_FS;
-  HIPCHK(rec.alloc(sizeof(double) * RS * n));
-  launch_receivers(normals, midpoints, n, recv.as<double>(), st);
+  void* scr = nullptr;
+  HIPCHK(ctx_scratch(ctx, sizeof(double) * (RECV + RS) * n, &scr));
+  double* recv_p = static_cast<double*>(scr);
+  double* rec_p = recv_p + RECV * n;
+  launch_receivers(normals, midpoints, n, recv_p, st);
   if (hs) {
     // collocation z-coordinates live in recv (stride RECV); check vertices and receivers on the host copy
     std::vector<double> rv(RECV * n);
-    HIPCHK(hipMemcpyAsync(rv.data(), recv.p, sizeof(double) * RECV * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(rv.data(), recv_p, sizeof(double) * RECV * n, hipMemcpyDeviceToHost, st));
     std::vector<double> z1(n), z2(n), z3(n);
     HIPCHK(hipMemcpyAsync(z1.data(), P1 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(z2.data(), P2 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
@@ -310,14 +347,17 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
       if (rv[RECV * i + 2] > 0 || z1[i] > 0 || z2[i] > 0 || z3[i] > 0)
         return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
   }
-  launch_setup(P1, P2, P3, n, hs, rec.as<double>(), st);
+  launch_setup(P1, P2, P3, n, hs, rec_p, st);
   OutPtrs<double> out{};
   out.ld = lda;
   out.sign = -1.0;
   for (int m = 0; m < 3; ++m)
     for (int q = 0; q < 3; ++q) out.p[3 * m + q] = A + (int64_t)q * n + (int64_t)m * n * lda;
   const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, n);
-  launch_traction<double>(recv.as<double>(), n, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  launch_traction<double>(recv_p, n, rec_p, n, hs, tpb, nu, lambda, G, out, st);
+  HIPCHK(hipEventRecord(ctx->ev1, st));
+  ctx->timed = true;
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
   return CDJ_OK;
@@ -461,6 +501,73 @@ int solve_augmented_fp64(cdj_ctx* ctx, double* AB, int64_t n, int64_t lda, int64
   return CDJ_OK;
 }
 
+// Mixed precision: FP32 copy of A factored on MFMA, FP64 FGMRES-IR against the FP64 A (device, n x n, lda).
+// B (n x nrhs, ld n) right-hand sides, X (n x nrhs) solutions.  Falls back to the FP64 LU when the
+// refinement does not reach the tolerance; report->path says which one produced X.
+int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const double* B, double* X, int64_t nrhs,
+                  const cdj_opts* opts, cdj_report* report, hipStream_t st) {
+  GmresOpts go;
+  if (opts && opts->tol > 0) go.tol = opts->tol;
+  if (opts && opts->max_iter > 0) go.max_iter = opts->max_iter;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  DBuf A32, ipiv, perm, ws, info;
+  HIPCHK(A32.alloc(sizeof(float) * n * n));
+  HIPCHK(ipiv.alloc(sizeof(int32_t) * n));
+  HIPCHK(perm.alloc(sizeof(int32_t) * n));
+  HIPCHK(ws.alloc(lu_workspace_bytes(n)));
+  HIPCHK(info.alloc(sizeof(int)));
+  convert_matrix_d2f(A, lda, A32.as<float>(), n, n, n, st);
+  HIPCHK(lu_factor<float>(A32.as<float>(), n, n, ipiv.as<int32_t>(), info.as<int>(), ws.p, st));
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(lu_permutation(ipiv.as<int32_t>(), n, perm.as<int32_t>(), st));
+  bool ok = h == 0;
+  double worst = 0.0;
+  int iters = 0;
+  for (int64_t q = 0; ok && q < nrhs; ++q) {
+    GmresResult gr;
+    HIPCHK(fgmres_ir<float>(A, n, lda, A32.as<float>(), n, perm.as<int32_t>(), B + q * n, X + q * n, go, gr, st));
+    ok = gr.converged && !gr.trsv_timeout;
+    worst = std::max(worst, gr.residual);
+    iters += gr.iterations;
+  }
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ok) {
+    if (report) {
+      report->path = CDJ_PREC_MIXED32;
+      report->iterations = iters;
+      report->residual = worst;
+      report->solve_ms = ms;
+    }
+    return CDJ_OK;
+  }
+  // fallback: FP64 LU on a copy of A
+  DBuf AB;
+  HIPCHK(AB.alloc(sizeof(double) * n * (n + nrhs)));
+  HIPCHK(hipMemcpy2DAsync(AB.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,
+                          hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipMemcpyAsync(AB.as<double>() + n * n, B, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st));
+  int rc = solve_augmented_fp64(ctx, AB.as<double>(), n, n, nrhs, nullptr, nullptr, nullptr, st);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(X, AB.as<double>() + n * n, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (report) {
+    report->path = CDJ_PREC_FP64;
+    report->iterations = iters;
+    report->residual = -1.0;
+    report->solve_ms = ms;
+  }
+  return CDJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -475,6 +582,19 @@ int cdj_solve(cdj_ctx* ctx, const double* A, const double* b, int64_t n, int64_t
   hipStream_t st = ctx->stream;
   if (n == 0 || nrhs == 0) return CDJ_OK;
   const int64_t lda = n;
+  if (opts && opts->precision == CDJ_PREC_MIXED32) {
+    DBuf dA, dB, dX;
+    HIPCHK(dA.alloc(sizeof(double) * n * n));
+    HIPCHK(dB.alloc(sizeof(double) * n * nrhs));
+    HIPCHK(dX.alloc(sizeof(double) * n * nrhs));
+    HIPCHK(hipMemcpyAsync(dA.p, A, sizeof(double) * n * n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dB.p, b, sizeof(double) * n * nrhs, hipMemcpyHostToDevice, st));
+    int rc = solve_mixed32(ctx, dA.as<double>(), n, n, dB.as<double>(), dX.as<double>(), nrhs, opts, report, st);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(x, dX.p, sizeof(double) * n * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return CDJ_OK;
+  }
   DBuf AB, Ao, Bo;
   HIPCHK(AB.alloc(sizeof(double) * lda * (n + nrhs)));
   HIPCHK(hipMemcpyAsync(AB.p, A, sizeof(double) * n * n, hipMemcpyHostToDevice, st));
@@ -529,7 +649,8 @@ int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const 
   HIPCHK(hipEventRecord(e1, st));
   HIPCHK(hipMemcpyAsync(AB.as<double>() + N3 * lda, b, sizeof(double) * N3 * nrhs, hipMemcpyHostToDevice, st));
   if (A_out) HIPCHK(hipMemcpyAsync(A_out, AB.p, sizeof(double) * N3 * N3, hipMemcpyDeviceToHost, st));
-  if (report) {
+  const bool mixed = opts && opts->precision == CDJ_PREC_MIXED32;
+  if (report && !mixed) {
     HIPCHK(Ao.alloc(sizeof(double) * N3 * N3));
     HIPCHK(Bo.alloc(sizeof(double) * N3 * nrhs));
     HIPCHK(hipMemcpyAsync(Ao.p, AB.p, sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
@@ -540,6 +661,17 @@ int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const 
   (void)hipEventElapsedTime(&ams, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (mixed) {
+    DBuf dX;
+    HIPCHK(dX.alloc(sizeof(double) * N3 * nrhs));
+    rc = solve_mixed32(ctx, AB.as<double>(), N3, lda, AB.as<double>() + N3 * lda, dX.as<double>(), nrhs, opts, report,
+                       st);
+    if (rc) return rc;
+    if (report) report->assembly_ms = ams;
+    HIPCHK(hipMemcpyAsync(x, dX.p, sizeof(double) * N3 * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return CDJ_OK;
+  }
   rc = solve_augmented_fp64(ctx, AB.as<double>(), N3, lda, nrhs, Ao.as<double>(), Bo.as<double>(), report, st);
   if (rc) return rc;
   if (report) report->assembly_ms = ams;

@@ -1,0 +1,445 @@
+// Mixed-precision dense solve for gfx950: low-precision LU (FP32 MFMA) as a preconditioner of restarted
+// FGMRES whose matrix-vector products, residuals and updates are FP64, iterated until the FP64 relative
+// residual ||b - A x|| / ||b|| reaches the requested tolerance (GMRES-based iterative refinement).
+//
+//   gemv        : split-K FP64-accumulated matrix-vector product (the HBM-bound residual), deterministic
+//                 two-pass reduction over column chunks.
+//   trsv        : persistent triangular solves.  One workgroup per 64-row block; block b consumes the solved
+//                 blocks c < b (c > b for the upper solve) as their ready flags appear, so the off-diagonal
+//                 products overlap the sequential chain.  Flags carry a per-solve epoch, waits are bounded
+//                 and a timed-out wait sets an error word and exits, so every wave always terminates.
+//   precond     : z = U^-1 L^-1 P r in the factor precision.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "linalg.hpp"
+#include "mixed.hpp"
+
+namespace cdj {
+
+namespace {
+
+constexpr int TB = 64;  // rows per triangular block
+
+unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---- split-K GEMV ---------------------------------------------------------------------------------------
+template <class TA>
+__global__ void __launch_bounds__(256) gemv_part_kernel(const TA* __restrict__ A, int64_t m, int64_t n, int64_t lda,
+                                                        const double* __restrict__ x, int chunk,
+                                                        double* __restrict__ part) {
+  __shared__ double xs[1024];
+  const int64_t c0 = (int64_t)blockIdx.y * chunk;
+  const int64_t c1 = std::min<int64_t>(n, c0 + chunk);
+  for (int64_t c = c0 + threadIdx.x; c < c1; c += 256) xs[c - c0] = x[c];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  double s0 = 0.0, s1 = 0.0;
+  int64_t c = c0;
+  for (; c + 1 < c1; c += 2) {
+    s0 += (double)A[i + c * lda] * xs[c - c0];
+    s1 += (double)A[i + (c + 1) * lda] * xs[c + 1 - c0];
+  }
+  if (c < c1) s0 += (double)A[i + c * lda] * xs[c - c0];
+  part[(int64_t)blockIdx.y * m + i] = s0 + s1;
+}
+
+__global__ void gemv_reduce_kernel(const double* __restrict__ part, int64_t m, int nchunk, double alpha,
+                                   const double* __restrict__ yin, double beta, double* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int k = 0; k < nchunk; ++k) s += part[(int64_t)k * m + i];
+  y[i] = alpha * s + (beta == 0.0 ? 0.0 : beta * yin[i]);
+}
+
+// ---- persistent triangular solves ----------------------------------------------------------------------
+__device__ __forceinline__ bool wait_flag(const int* flag, int epoch, long long spin_limit, int* err) {
+  long long it = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+    if (++it > spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      atomicOr(err, 1);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+// LOWER: unit lower (forward) solve; otherwise upper (backward) solve with the diagonal.
+// x (length n, precision T) holds the right-hand side on entry and the solution on exit.
+template <class T, bool LOWER>
+__global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restrict__ LU, int64_t n, int64_t lda,
+                                                              T* x, int* flags, int epoch, long long spin_limit,
+                                                              int* err) {
+  __shared__ double accd[4][TB];
+  __shared__ T xs[TB];
+  __shared__ T rhs[TB];
+  __shared__ int s_ok;
+  const int nb = (int)((n + TB - 1) / TB);
+  const int b = blockIdx.x;
+  const int64_t r0 = (int64_t)b * TB;
+  const int nr = (int)std::min<int64_t>(TB, n - r0);
+  const int tid = threadIdx.x, row = tid % TB, grp = tid / TB;  // 4 groups of 64 threads
+  if (tid < nr) rhs[tid] = x[r0 + tid];
+  double acc = 0.0;  // per (row, grp) partial of sum_c L[r, c] x[c]
+  // off-diagonal blocks in dependency order: c = 0..b-1 (lower), c = nb-1..b+1 (upper)
+  const int ndep = LOWER ? b : nb - 1 - b;
+  for (int k = 0; k < ndep; ++k) {
+    const int c = LOWER ? k : nb - 1 - k;
+    if (tid == 0) s_ok = wait_flag(&flags[c], epoch, spin_limit, err) ? 1 : 0;
+    __syncthreads();
+    if (!s_ok) return;
+    const int64_t q0 = (int64_t)c * TB;
+    const int nq = (int)std::min<int64_t>(TB, n - q0);
+    if (tid < nq) xs[tid] = __hip_atomic_load(&x[q0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (row < nr)
+      for (int q = grp; q < nq; q += 4) acc += (double)LU[(r0 + row) + (q0 + q) * lda] * (double)xs[q];
+    __syncthreads();
+  }
+  accd[grp][row] = acc;
+  __syncthreads();
+  if (tid < nr) rhs[tid] = (T)((double)rhs[tid] - (accd[0][tid] + accd[1][tid] + accd[2][tid] + accd[3][tid]));
+  __syncthreads();
+  // diagonal block: sequential over its rows, parallel over the updates
+  if (LOWER) {
+    for (int r = 0; r < nr; ++r) {
+      const T xr = rhs[r];
+      __syncthreads();
+      if (tid > r && tid < nr) rhs[tid] -= LU[(r0 + tid) + (r0 + r) * lda] * xr;
+      __syncthreads();
+    }
+  } else {
+    for (int r = nr - 1; r >= 0; --r) {
+      if (tid == 0) rhs[r] = rhs[r] / LU[(r0 + r) + (r0 + r) * lda];
+      __syncthreads();
+      const T xr = rhs[r];
+      if (tid < r) rhs[tid] -= LU[(r0 + tid) + (r0 + r) * lda] * xr;
+      __syncthreads();
+    }
+  }
+  if (tid < nr) __hip_atomic_store(&x[r0 + tid], rhs[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- vector kernels ------------------------------------------------------------------------------------
+template <class TI, class TO>
+__global__ void permute_cast_kernel(const int32_t* __restrict__ perm, const TI* __restrict__ in, TO* __restrict__ out,
+                                    int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (TO)in[perm[i]];
+}
+
+template <class TI, class TO>
+__global__ void cast_kernel(const TI* __restrict__ in, TO* __restrict__ out, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (TO)in[i];
+}
+
+// partial dot products: out[block] = sum over the block's slice of a.b (one value per block)
+__global__ void __launch_bounds__(256) dot_part_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                                       int64_t n, double* __restrict__ out) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) v += a[i] * b[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+// multi-dot: out[j * nblk + block] = partial of V[:, j] . w for j < k
+__global__ void __launch_bounds__(256) mdot_part_kernel(const double* __restrict__ V, int64_t ldv, int k,
+                                                        const double* __restrict__ w, int64_t n,
+                                                        double* __restrict__ out) {
+  __shared__ double s[256];
+  for (int j = 0; j < k; ++j) {
+    double v = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+      v += V[i + j * ldv] * w[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(int64_t)j * gridDim.x + blockIdx.x] = s[0];
+    __syncthreads();
+  }
+}
+
+// w -= V[:, 0:k] h   (h on device)
+__global__ void mgs_update_kernel(const double* __restrict__ V, int64_t ldv, int k, const double* __restrict__ h,
+                                  double* __restrict__ w, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = w[i];
+  for (int j = 0; j < k; ++j) s -= V[i + j * ldv] * h[j];
+  w[i] = s;
+}
+
+__global__ void axpby_kernel(double a, const double* __restrict__ x, double b, double* __restrict__ y, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// x += Z[:, 0:k] yk
+__global__ void combine_kernel(const double* __restrict__ Z, int64_t ldz, int k, const double* __restrict__ yk,
+                               double* __restrict__ x, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = x[i];
+  for (int j = 0; j < k; ++j) s += Z[i + j * ldz] * yk[j];
+  x[i] = s;
+}
+
+constexpr int DOT_BLOCKS = 240;
+
+}  // namespace
+
+// ---- host-side building blocks -----------------------------------------------------------------------------
+size_t gemv_part_bytes(int64_t m, int64_t n) {
+  const int chunk = 1024;
+  return sizeof(double) * (size_t)m * ((n + chunk - 1) / chunk);
+}
+
+template <class TA>
+void gemv_splitk(const TA* A, int64_t m, int64_t n, int64_t lda, const double* x, double* y, double alpha,
+                 const double* yin, double beta, double* part, hipStream_t st) {
+  if (m <= 0) return;
+  const int chunk = 1024;
+  const int nchunk = (int)((n + chunk - 1) / chunk);
+  if (nchunk > 0)
+    hipLaunchKernelGGL(gemv_part_kernel<TA>, dim3(nblk(m, 256), (unsigned)nchunk), dim3(256), 0, st, A, m, n, lda, x,
+                       chunk, part);
+  hipLaunchKernelGGL(gemv_reduce_kernel, dim3(nblk(m, 256)), dim3(256), 0, st, (const double*)part, m, nchunk, alpha,
+                     yin, beta, y);
+}
+
+template <class T>
+hipError_t trsv_persistent(const T* LU, int64_t n, int64_t lda, T* x, bool lower, TrsvState& s, hipStream_t st) {
+  const int nb = (int)((n + TB - 1) / TB);
+  s.epoch += 1;
+  int epoch = s.epoch;
+  long long lim = s.spin_limit;
+  void* args[] = {(void*)&LU, &n, &lda, &x, &s.flags, &epoch, &lim, &s.err};
+  if (lower)
+    return hipLaunchCooperativeKernel((const void*)trsv_persistent_kernel<T, true>, dim3(nb), dim3(256), args, 0, st);
+  return hipLaunchCooperativeKernel((const void*)trsv_persistent_kernel<T, false>, dim3(nb), dim3(256), args, 0, st);
+}
+
+template <class T>
+hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* r, double* z,
+                            T* work, TrsvState& s, hipStream_t st) {
+  hipLaunchKernelGGL((permute_cast_kernel<double, T>), dim3(nblk(n, 256)), dim3(256), 0, st, perm, r, work, n);
+  hipError_t e = trsv_persistent<T>(LU, n, lda, work, true, s, st);
+  if (e != hipSuccess) return e;
+  e = trsv_persistent<T>(LU, n, lda, work, false, s, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((cast_kernel<T, double>), dim3(nblk(n, 256)), dim3(256), 0, st, (const T*)work, z, n);
+  return hipGetLastError();
+}
+
+// ---- FGMRES-IR driver ---------------------------------------------------------------------------------------
+namespace {
+
+struct Dev {
+  std::vector<void*> bufs;
+  ~Dev() {
+    for (void* p : bufs) (void)hipFree(p);
+  }
+  template <class T> T* get(size_t count, hipError_t& e) {
+    void* p = nullptr;
+    e = hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1));
+    if (e == hipSuccess) bufs.push_back(p);
+    return static_cast<T*>(p);
+  }
+};
+
+hipError_t host_dot(const double* a, const double* b, int64_t n, double* part, std::vector<double>& hp, double& out,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(dot_part_kernel, dim3(DOT_BLOCKS), dim3(256), 0, st, a, b, n, part);
+  hipError_t e = hipMemcpyAsync(hp.data(), part, sizeof(double) * DOT_BLOCKS, hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(st);
+  double s = 0.0;
+  for (int k = 0; k < DOT_BLOCKS; ++k) s += hp[k];
+  out = s;
+  return e;
+}
+
+}  // namespace
+
+template <class TF>
+hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int64_t ldlu, const int32_t* perm,
+                     const double* b, double* x, const GmresOpts& o, GmresResult& res, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  Dev d;
+  const int m = o.restart;
+  double* V = d.get<double>((size_t)n * (m + 1), e);
+  if (e) return e;
+  double* Z = d.get<double>((size_t)n * m, e);
+  if (e) return e;
+  double* w = d.get<double>(n, e);
+  if (e) return e;
+  double* r = d.get<double>(n, e);
+  if (e) return e;
+  TF* work = d.get<TF>(n, e);
+  if (e) return e;
+  double* part = d.get<double>(std::max<size_t>(gemv_part_bytes(n, n) / sizeof(double), DOT_BLOCKS * (size_t)(m + 1)), e);
+  if (e) return e;
+  double* hdev = d.get<double>(m + 1, e);
+  if (e) return e;
+  int* flags = d.get<int>((n + TB - 1) / TB + 1, e);
+  if (e) return e;
+  int* err = d.get<int>(1, e);
+  if (e) return e;
+  e = hipMemsetAsync(flags, 0, sizeof(int) * ((n + TB - 1) / TB + 1), st);
+  if (e) return e;
+  e = hipMemsetAsync(err, 0, sizeof(int), st);
+  if (e) return e;
+  TrsvState ts{flags, err, 0, o.spin_limit};
+  std::vector<double> hp(DOT_BLOCKS * (size_t)(m + 1));
+  const unsigned g = nblk(n, 256);
+
+  double bnorm2 = 0.0;
+  e = host_dot(b, b, n, part, hp, bnorm2, st);
+  if (e) return e;
+  const double bnorm = std::sqrt(bnorm2) > 0 ? std::sqrt(bnorm2) : 1.0;
+  res.iterations = 0;
+  res.converged = false;
+  res.history.clear();
+  // initial guess: x = M^-1 b
+  e = apply_lu_precond<TF>(LU, n, ldlu, perm, b, x, work, ts, st);
+  if (e) return e;
+  for (int outer = 0; outer < o.max_restarts; ++outer) {
+    // r = b - A x  (FP64)
+    gemv_splitk<double>(A, n, n, lda, x, r, -1.0, b, 1.0, part, st);
+    double rr = 0.0;
+    e = host_dot(r, r, n, part, hp, rr, st);
+    if (e) return e;
+    double beta = std::sqrt(rr);
+    res.residual = beta / bnorm;
+    res.history.push_back(res.residual);
+    if (res.residual <= o.tol) {
+      res.converged = true;
+      break;
+    }
+    if (res.iterations >= o.max_iter) break;
+    // V[:,0] = r / beta
+    hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, st, 1.0 / beta, (const double*)r, 0.0, V, n);
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gvec(m + 1, 0.0);
+    gvec[0] = beta;
+    int k = 0;
+    for (; k < m && res.iterations < o.max_iter; ++k) {
+      res.iterations++;
+      // z_k = M^-1 v_k ; w = A z_k
+      e = apply_lu_precond<TF>(LU, n, ldlu, perm, V + (int64_t)k * n, Z + (int64_t)k * n, work, ts, st);
+      if (e) return e;
+      gemv_splitk<double>(A, n, n, lda, Z + (int64_t)k * n, w, 1.0, nullptr, 0.0, part, st);
+      // classical Gram-Schmidt, twice (CGS2) for FP64 orthogonality
+      for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(mdot_part_kernel, dim3(DOT_BLOCKS), dim3(256), 0, st, (const double*)V, (int64_t)n, k + 1,
+                           (const double*)w, n, part);
+        e = hipMemcpyAsync(hp.data(), part, sizeof(double) * DOT_BLOCKS * (k + 1), hipMemcpyDeviceToHost, st);
+        if (e) return e;
+        e = hipStreamSynchronize(st);
+        if (e) return e;
+        std::vector<double> h(k + 1, 0.0);
+        for (int j = 0; j <= k; ++j) {
+          double s = 0.0;
+          for (int q = 0; q < DOT_BLOCKS; ++q) s += hp[(size_t)j * DOT_BLOCKS + q];
+          h[j] = s;
+          H[(size_t)j * m + k] += s;
+        }
+        e = hipMemcpyAsync(hdev, h.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, st);
+        if (e) return e;
+        hipLaunchKernelGGL(mgs_update_kernel, dim3(g), dim3(256), 0, st, (const double*)V, (int64_t)n, k + 1,
+                           (const double*)hdev, w, n);
+      }
+      double hh = 0.0;
+      e = host_dot(w, w, n, part, hp, hh, st);
+      if (e) return e;
+      hh = std::sqrt(hh);
+      H[(size_t)(k + 1) * m + k] = hh;
+      if (hh > 0)
+        hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, st, 1.0 / hh, (const double*)w, 0.0,
+                           V + (int64_t)(k + 1) * n, n);
+      // Givens rotations on column k
+      for (int j = 0; j < k; ++j) {
+        const double t = cs[j] * H[(size_t)j * m + k] + sn[j] * H[(size_t)(j + 1) * m + k];
+        H[(size_t)(j + 1) * m + k] = -sn[j] * H[(size_t)j * m + k] + cs[j] * H[(size_t)(j + 1) * m + k];
+        H[(size_t)j * m + k] = t;
+      }
+      const double a = H[(size_t)k * m + k], bb = H[(size_t)(k + 1) * m + k];
+      const double rho = std::hypot(a, bb);
+      cs[k] = rho > 0 ? a / rho : 1.0;
+      sn[k] = rho > 0 ? bb / rho : 0.0;
+      H[(size_t)k * m + k] = rho;
+      H[(size_t)(k + 1) * m + k] = 0.0;
+      gvec[k + 1] = -sn[k] * gvec[k];
+      gvec[k] = cs[k] * gvec[k];
+      const double est = std::fabs(gvec[k + 1]) / bnorm;
+      if (est <= o.tol * 0.5 || hh == 0.0) {
+        ++k;
+        break;
+      }
+    }
+    // solve the k x k triangular system and update x += Z y
+    std::vector<double> yk(k, 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double s = gvec[i];
+      for (int j = i + 1; j < k; ++j) s -= H[(size_t)i * m + j] * yk[j];
+      yk[i] = s / H[(size_t)i * m + i];
+    }
+    if (k > 0) {
+      e = hipMemcpyAsync(hdev, yk.data(), sizeof(double) * k, hipMemcpyHostToDevice, st);
+      if (e) return e;
+      hipLaunchKernelGGL(combine_kernel, dim3(g), dim3(256), 0, st, (const double*)Z, (int64_t)n, k,
+                         (const double*)hdev, x, n);
+    }
+  }
+  if (!res.converged) {
+    gemv_splitk<double>(A, n, n, lda, x, r, -1.0, b, 1.0, part, st);
+    double rr = 0.0;
+    e = host_dot(r, r, n, part, hp, rr, st);
+    if (e) return e;
+    res.residual = std::sqrt(rr) / bnorm;
+    res.history.push_back(res.residual);
+    res.converged = res.residual <= o.tol;
+  }
+  int herr = 0;
+  e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st);
+  if (e) return e;
+  e = hipStreamSynchronize(st);
+  if (e) return e;
+  res.trsv_timeout = herr != 0;
+  return hipGetLastError();
+}
+
+template void gemv_splitk<double>(const double*, int64_t, int64_t, int64_t, const double*, double*, double,
+                                  const double*, double, double*, hipStream_t);
+template void gemv_splitk<float>(const float*, int64_t, int64_t, int64_t, const double*, double*, double,
+                                 const double*, double, double*, hipStream_t);
+template hipError_t trsv_persistent<float>(const float*, int64_t, int64_t, float*, bool, TrsvState&, hipStream_t);
+template hipError_t trsv_persistent<double>(const double*, int64_t, int64_t, double*, bool, TrsvState&, hipStream_t);
+template hipError_t apply_lu_precond<float>(const float*, int64_t, int64_t, const int32_t*, const double*, double*,
+                                            float*, TrsvState&, hipStream_t);
+template hipError_t apply_lu_precond<double>(const double*, int64_t, int64_t, const int32_t*, const double*, double*,
+                                             double*, TrsvState&, hipStream_t);
+template hipError_t fgmres_ir<float>(const double*, int64_t, int64_t, const float*, int64_t, const int32_t*,
+                                     const double*, double*, const GmresOpts&, GmresResult&, hipStream_t);
+template hipError_t fgmres_ir<double>(const double*, int64_t, int64_t, const double*, int64_t, const int32_t*,
+                                      const double*, double*, const GmresOpts&, GmresResult&, hipStream_t);
+
+}  // namespace cdj

@@ -1,0 +1,54 @@
+// Mixed-precision solve: FP64 residual GMRES-based iterative refinement preconditioned by a low-precision LU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace cdj {
+
+struct TrsvState {
+  int* flags;             // device, one per 64-row block, zero-initialised once
+  int* err;               // device error word (set on a timed-out wait)
+  int epoch;              // increments per solve
+  long long spin_limit;   // bound on every wait loop
+};
+
+struct GmresOpts {
+  double tol = 1e-10;     // FP64 relative residual target
+  int restart = 30;
+  int max_iter = 200;
+  int max_restarts = 20;
+  long long spin_limit = 1LL << 26;
+};
+
+struct GmresResult {
+  int iterations = 0;
+  bool converged = false;
+  bool trsv_timeout = false;
+  double residual = 0.0;
+  std::vector<double> history;
+};
+
+size_t gemv_part_bytes(int64_t m, int64_t n);
+
+// y = alpha * A x + beta * yin (FP64 accumulate; A of precision TA); part holds gemv_part_bytes(m, n).
+template <class TA>
+void gemv_splitk(const TA* A, int64_t m, int64_t n, int64_t lda, const double* x, double* y, double alpha,
+                 const double* yin, double beta, double* part, hipStream_t st);
+
+template <class T>
+hipError_t trsv_persistent(const T* LU, int64_t n, int64_t lda, T* x, bool lower, TrsvState& s, hipStream_t st);
+
+// z = U^-1 L^-1 P r with factors of precision T (work: n elements of T).
+template <class T>
+hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* r, double* z,
+                            T* work, TrsvState& s, hipStream_t st);
+
+// Solve A x = b (A FP64, n x n) by right-preconditioned restarted FGMRES with M = P^T L U in precision TF,
+// until ||b - A x|| / ||b|| <= o.tol.  x receives the solution.
+template <class TF>
+hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int64_t ldlu, const int32_t* perm,
+                     const double* b, double* x, const GmresOpts& o, GmresResult& res, hipStream_t st);
+
+}  // namespace cdj

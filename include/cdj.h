@@ -62,6 +62,10 @@ int cdj_create(int device, cdj_ctx** ctx);
 int cdj_destroy(cdj_ctx* ctx);
 const char* cdj_last_error(const cdj_ctx* ctx);
 
+/* Device time (ms, HIP events on the launch stream) of the last assembly kernel run on this context; -1 if
+ * none.  Used by benchmarks to measure the kernel alone. */
+double cdj_last_kernel_ms(cdj_ctx* ctx);
+
 /* TD with influence matrices: replaces TD(X,Y,Z,P1,P2,P3,Dss,Dds,Dn,nu,G,DispFlag,StrainFlag,HSflag[,StrainInf,
  * DispInf]) (TD.jl:77-133, 135-216).  strain: 18 nobs x ntri matrices in StrainInf field order (exxDn, eyyDn,
  * ezzDn, exyDn, exzDn, eyzDn, then Dss, then Dds), NULL when strain_flag != 1; disp: 9 matrices in DispInf order

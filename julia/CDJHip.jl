@@ -1,0 +1,134 @@
+# CDJHip.jl - ccall binding of libcdjhip.so (include/cdj.h) for CutAndDisplaceJulia.
+#
+# Drop-in replacements for TD (src/TD.jl:77, :135, :219), CalculateInfluenceMatrices3D
+# (src/CalculateInfluenceMatrices3D.jl:2) and SlipCalculator3D (src/SlipCalculator3D.jl:1) with the same
+# signatures.  Include this file after CutAndDisplaceJulia and call CDJHip.install!() to route the package's
+# calls to the GPU, or call CDJHip.TD(...) etc. directly.  The precision mode lives in a module-level setting
+# (CDJHip.PRECISION[]), so the public signatures stay unchanged.
+module CDJHip
+
+using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, SetupTractionVector, KnockOutFixedRows
+
+const LIB = get(ENV, "CDJ_LIB", joinpath(@__DIR__, "..", "cutanddisplacejulia_amd", "libcdjhip.so"))
+const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+const PRECISION = Ref{Int32}(0)        # 0 = FP64 (reference arithmetic), 1 = FP32 LU + FP64 GMRES-IR
+const LOCK = ReentrantLock()
+
+struct Opts
+    precision::Int32
+    tile::Int32
+    tol::Float64
+    max_iter::Int32
+    near_factor::Float64
+end
+
+mutable struct Report
+    path::Int32
+    iterations::Int32
+    residual::Float64
+    assembly_ms::Float64
+    solve_ms::Float64
+    near_entries::Int64
+end
+Report() = Report(0, 0, 0.0, 0.0, 0.0, 0)
+
+function ctx()
+    if CTX[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:cdj_create, LIB), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, h)
+        rc == 0 || error("cdj_create failed: " * unsafe_string(ccall((:cdj_last_error, LIB), Cstring, (Ptr{Cvoid},), C_NULL)))
+        CTX[] = h[]
+    end
+    return CTX[]
+end
+
+check(rc) = rc == 0 || error(unsafe_string(ccall((:cdj_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx())))
+
+_f64(a) = convert(Array{Float64}, a)
+
+function TD(X, Y, Z, P1List, P2List, P3List, Dss, Dds, Dn, ν, G, DispFlag, StrainFlag, HSflag)
+    n = length(P1List) == 3 ? 1 : size(P1List, 1)
+    m = length(X)
+    S = StrainFlag == 1 ? StrainInf((zeros(m, n) for _ in 1:18)...) : StrainInf(([] for _ in 1:18)...)
+    D = DispFlag == 1 ? DispInf((zeros(m, n) for _ in 1:9)...) : DispInf(([] for _ in 1:9)...)
+    return TD(X, Y, Z, P1List, P2List, P3List, Dss, Dds, Dn, ν, G, DispFlag, StrainFlag, HSflag, S, D)
+end
+
+function TD(X, Y, Z, P1List, P2List, P3List, Dss, Dds, Dn, ν, G, DispFlag, StrainFlag, HSflag,
+            S::StrainInf, D::DispInf)
+    P1 = _f64(reshape(P1List, :, 3)); P2 = _f64(reshape(P2List, :, 3)); P3 = _f64(reshape(P3List, :, 3))
+    x = _f64(vec(X)); y = _f64(vec(Y)); z = _f64(vec(Z))
+    sp = StrainFlag == 1 ? [pointer(getfield(S, k)) for k in fieldnames(StrainInf)] : Ptr{Float64}[]
+    dp = DispFlag == 1 ? [pointer(getfield(D, k)) for k in fieldnames(DispInf)] : Ptr{Float64}[]
+    lock(LOCK) do
+        GC.@preserve S D P1 P2 P3 x y z sp dp begin
+            check(ccall((:cdj_td_influence, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Int32, Int32, Int32,
+                 Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}),
+                ctx(), x, y, z, length(x), P1, P2, P3, size(P1, 1), _f64(vec(Dss)), _f64(vec(Dds)), _f64(vec(Dn)),
+                ν, G, DispFlag, StrainFlag, HSflag,
+                isempty(sp) ? C_NULL : sp, isempty(dp) ? C_NULL : dp))
+        end
+    end
+    return (S, D)
+end
+
+function TD(X, Y, Z, P1List, P2List, P3List, Dss, Dds, Dn, ν, G, DispFlag, StrainFlag, HSflag,
+            ::Strains, ::Disps)
+    P1 = _f64(reshape(P1List, :, 3)); P2 = _f64(reshape(P2List, :, 3)); P3 = _f64(reshape(P3List, :, 3))
+    x = _f64(vec(X)); y = _f64(vec(Y)); z = _f64(vec(Z)); m = length(x)
+    s6 = [zeros(m) for _ in 1:6]; d3 = [zeros(m) for _ in 1:3]
+    sp = [pointer(a) for a in s6]; dp = [pointer(a) for a in d3]
+    lock(LOCK) do
+        GC.@preserve s6 d3 sp dp P1 P2 P3 x y z begin
+            check(ccall((:cdj_td_field, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Int32, Int32, Int32,
+                 Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}),
+                ctx(), x, y, z, m, P1, P2, P3, size(P1, 1), _f64(vec(Dss)), _f64(vec(Dds)), _f64(vec(Dn)),
+                ν, G, DispFlag, StrainFlag, HSflag, StrainFlag == 1 ? sp : C_NULL, DispFlag == 1 ? dp : C_NULL))
+        end
+    end
+    rs(a) = size(X, 2) != 1 ? reshape(a, size(X)) : a
+    S = StrainFlag == 1 ? Strains(rs.(s6)...) : Strains([], [], [], [], [], [])
+    D = DispFlag == 1 ? Disps(rs.(d3)...) : Disps([], [], [])
+    return (S, D)
+end
+
+function CalculateInfluenceMatrices3D(FaceNormalVector, MidPoint, P1, P2, P3, ν, G, λ, FixedEls, HSFlag, n)
+    any(FixedEls .== 1) && error("fixed elements: use the reference implementation")
+    T = [zeros(n, n) for _ in 1:9]
+    tp = [pointer(a) for a in T]
+    opts = Ref(Opts(PRECISION[], 0, 1e-10, 0, 2.0))
+    nrm = _f64(FaceNormalVector); mid = _f64(MidPoint)
+    lock(LOCK) do
+        GC.@preserve T tp nrm mid begin
+            check(ccall((:cdj_traction_influence, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64,
+                 Float64, Float64, Int32, Ptr{Int32}, Ptr{Ptr{Float64}}, Ref{Opts}),
+                ctx(), nrm, mid, _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, HSFlag, C_NULL, tp, opts))
+        end
+    end
+    return TractionInf(T...)
+end
+
+function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, BoundaryConditions, FixedEls)
+    n = size(FaceNormalVector, 1)
+    (Tn, Tds, Tss) = SetupTractionVector(BoundaryConditions, FaceNormalVector, n, G, λ)
+    (Tn, Tds, Tss) = KnockOutFixedRows(FixedEls, Tn, Tds, Tss)
+    any(FixedEls .== 1) && error("fixed elements: use the reference implementation")
+    b = [Tn; Tss; Tds]
+    x = zeros(3n); A = zeros(3n, 3n)
+    opts = Ref(Opts(PRECISION[], 0, 1e-10, 0, 2.0)); rep = Ref(Report())
+    lock(LOCK) do
+        check(ccall((:cdj_slip_calculator, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Float64, Ptr{Float64},
+             Ptr{Float64}, Int32, Ptr{Float64}, Int64, Ptr{Int32}, Ptr{Float64}, Ptr{Float64}, Ref{Opts}, Ref{Report}),
+            ctx(), _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, _f64(MidPoint), _f64(FaceNormalVector), HSFlag,
+            _f64(b), 1, C_NULL, x, A, opts, rep))
+    end
+    return x[1:n], x[n+1:2n], x[2n+1:3n], A, b
+end
+
+end # module

@@ -191,3 +191,38 @@ def ang_dis_disp(x, y, z, alpha, nu):
     o = np.zeros(9)
     lib().orc_ang_dis_disp(x, y, z, alpha, nu, _ptr(o))
     return o
+
+
+def count_traction_flops(normals, mid, P1, P2, P3, nu, G, lam, hs_flag, recv_idx, src_idx, setup_too=False):
+    """Operation counts of the traction assembly over the given (receiver, source) pairs.
+
+    Returns dict(add, mul, div, sqrt, log, atan, trig, pairs); +, -, *, / count one each.
+    """
+    L = lib()
+    f = L.orc_count_traction_flops
+    f.restype = None
+    f.argtypes = [_p, _p, _p, _p, _p, _i64, _d, _d, _d, _i, _p, _p, _i64, _i, _p]
+    normals, mid = _f64(normals), _f64(mid)
+    P1, P2, P3, n = _tri_arrays(P1, P2, P3)
+    ri = np.ascontiguousarray(recv_idx, dtype=np.int64)
+    si = np.ascontiguousarray(src_idx, dtype=np.int64)
+    out = np.zeros(8)
+    f(_ptr(normals), _ptr(mid), _ptr(P1), _ptr(P2), _ptr(P3), n, float(nu), float(G), float(lam), int(hs_flag),
+      _ptr(ri), _ptr(si), ri.size, int(bool(setup_too)), _ptr(out))
+    return dict(zip(["add", "mul", "div", "sqrt", "log", "atan", "trig", "pairs"], out.tolist()))
+
+
+def traction_influence_rows(normals, mid, P1, P2, P3, nu, G, lam, hs_flag, r0, r1, nthreads=0):
+    """Traction influence of receiver rows [r0, r1) against all sources (bounded CPU-baseline sample)."""
+    L = lib()
+    f = L.orc_traction_influence_rows
+    f.restype = _i
+    f.argtypes = [_p, _p, _p, _p, _p, _i64, _d, _d, _d, _i, _i64, _i64, _p, _i]
+    normals, mid = _f64(normals), _f64(mid)
+    P1, P2, P3, n = _tri_arrays(P1, P2, P3)
+    T = [np.zeros((r1 - r0, n), order="F") for _ in range(9)]
+    rc = f(_ptr(normals), _ptr(mid), _ptr(P1), _ptr(P2), _ptr(P3), n, float(nu), float(G), float(lam), int(hs_flag),
+           int(r0), int(r1), _ptr_array(T), int(nthreads))
+    if rc:
+        raise OracleError(lib().orc_last_error().decode())
+    return dict(zip(TRACTION_FIELDS, T))

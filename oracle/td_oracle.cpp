@@ -173,35 +173,40 @@ void orc_ss_ds_dirs(const double* nrm, int64_t n, double* ss, double* ds) {
 // Strain (6) -> traction (Tn, Tss, Tds) on a receiver plane: HookesLaw3DStrain2Stress!.jl:51-62,
 // CalculateNormalTraction3D!.jl:37-56, TractionVectorCartesianComponents3D!.jl:43-50,
 // CalculateTractionInChosenDirection3D!.jl:48-66.
-static inline void strain_to_traction(const double* e, double lam, double G, const double* nv, const double* s,
-                                      const double* d, double* Tn, double* Tss, double* Tds) {
-  const double G2 = 2.0 * G;
-  const double cons = lam * (e[0] + e[1] + e[2]);
-  const double Sxx = G2 * e[0] + cons, Syy = G2 * e[1] + cons, Szz = G2 * e[2] + cons;
-  const double Sxy = G2 * e[3], Sxz = G2 * e[4], Syz = G2 * e[5];
-  const double nx = nv[0], ny = nv[1], nz = nv[2];
+}  // extern "C"
+
+template <class T>
+static inline void strain_to_traction(const T* e, T lam, T G, const T* nv, const T* s, const T* d, T* Tn, T* Tss,
+                                      T* Tds) {
+  const T G2 = 2.0 * G;
+  const T cons = lam * (e[0] + e[1] + e[2]);
+  const T Sxx = G2 * e[0] + cons, Syy = G2 * e[1] + cons, Szz = G2 * e[2] + cons;
+  const T Sxy = G2 * e[3], Sxz = G2 * e[4], Syz = G2 * e[5];
+  const T nx = nv[0], ny = nv[1], nz = nv[2];
   {
-    const double CosAx2 = nx * nx, CosAy2 = ny * ny, CosAz2 = nz * nz;
-    const double CosAxy = nx * ny, CosAyz = ny * nz, CosAzx = nz * nx;
-    const double AA = Sxx * CosAx2, BB = Syy * CosAy2, CC = Szz * CosAz2;
-    const double DD = 2 * Sxy * CosAxy, EE = 2 * Syz * CosAyz, FF = 2 * Sxz * CosAzx;
+    const T CosAx2 = nx * nx, CosAy2 = ny * ny, CosAz2 = nz * nz;
+    const T CosAxy = nx * ny, CosAyz = ny * nz, CosAzx = nz * nx;
+    const T AA = Sxx * CosAx2, BB = Syy * CosAy2, CC = Szz * CosAz2;
+    const T DD = 2 * Sxy * CosAxy, EE = 2 * Syz * CosAyz, FF = 2 * Sxz * CosAzx;
     *Tn = AA + BB + CC + DD + EE + FF;
   }
-  const double Tx = Sxx * nx + Sxy * ny + Sxz * nz;
-  const double Ty = Sxy * nx + Syy * ny + Syz * nz;
-  const double Tz = Sxz * nx + Syz * ny + Szz * nz;
-  const double CosAx2 = 1 - (nx * nx), CosAy2 = 1 - (ny * ny), CosAz2 = 1 - (nz * nz);
-  const double CosAxy = nx * ny, CosAyz = ny * nz, CosAzx = nz * nx;
-  const double* dirs[2] = {s, d};
-  double* outs[2] = {Tss, Tds};
+  const T Tx = Sxx * nx + Sxy * ny + Sxz * nz;
+  const T Ty = Sxy * nx + Syy * ny + Syz * nz;
+  const T Tz = Sxz * nx + Syz * ny + Szz * nz;
+  const T CosAx2 = 1 - (nx * nx), CosAy2 = 1 - (ny * ny), CosAz2 = 1 - (nz * nz);
+  const T CosAxy = nx * ny, CosAyz = ny * nz, CosAzx = nz * nx;
+  const T* dirs[2] = {s, d};
+  T* outs[2] = {Tss, Tds};
   for (int q = 0; q < 2; ++q) {
-    const double* c = dirs[q];
-    const double A = (Tx * CosAx2 - Ty * CosAxy - Tz * CosAzx) * c[0];
-    const double B = (Tx * -CosAxy + Ty * CosAy2 - Tz * CosAyz) * c[1];
-    const double C = (Tx * -CosAzx - Ty * CosAyz + Tz * CosAz2) * c[2];
+    const T* c = dirs[q];
+    const T A = (Tx * CosAx2 - Ty * CosAxy - Tz * CosAzx) * c[0];
+    const T B = (Tx * -CosAxy + Ty * CosAy2 - Tz * CosAyz) * c[1];
+    const T C = (Tx * -CosAzx - Ty * CosAyz + Tz * CosAz2) * c[2];
     *outs[q] = A + B + C;
   }
 }
+
+extern "C" {
 
 // CalculateInfluenceMatrices3D without fixed elements (CalculateInfluenceMatrices3D.jl:35-43):
 // collocation points (SetupCollationPoints :52-71, unit slip) -> TD strain -> traction
@@ -236,7 +241,7 @@ int orc_traction_influence(const double* normals, const double* mid, const doubl
       const double dv[3] = {ds[i], ds[n + i], ds[2 * n + i]};
       const int64_t o = i + j * n;
       // mode order in e: Dn, Dss, Dds; T9 order: Dn*, Dss*, Dds* each (Tn, Tss, Tds)
-      for (int m = 0; m < 3; ++m) strain_to_traction(e[m], lam, G, nv, sv, dv, &T9[3 * m][o], &T9[3 * m + 1][o], &T9[3 * m + 2][o]);
+      for (int m = 0; m < 3; ++m) strain_to_traction<double>(e[m], lam, G, nv, sv, dv, &T9[3 * m][o], &T9[3 * m + 1][o], &T9[3 * m + 2][o]);
     }
   }
   return 0;
@@ -254,7 +259,7 @@ void orc_normal_shear_tractions(const double* sxx, const double* syy, const doub
     const double nv[3] = {normals[i], normals[n + i], normals[2 * n + i]};
     const double sv[3] = {ss[i], ss[n + i], ss[2 * n + i]};
     const double dv[3] = {ds[i], ds[n + i], ds[2 * n + i]};
-    strain_to_traction(e, 0.0, 0.5, nv, sv, dv, &Tn[i], &Tss[i], &Tds[i]);
+    strain_to_traction<double>(e, 0.0, 0.5, nv, sv, dv, &Tn[i], &Tss[i], &Tds[i]);
   }
 }
 

@@ -1,0 +1,20 @@
+"""The algorithmic flop count per (receiver, source) pair used by bench.py's roofline is what the op-counting
+oracle measures on the benchmark workload."""
+import numpy as np
+
+import bench
+from cutanddisplacejulia_amd import api, meshes
+from oracle import oracle as O
+
+
+def test_flops_per_pair_half_space():
+    c = meshes.config2()
+    P1, P2, P3 = c["P1"], c["P2"], c["P3"]
+    n = P1.shape[0]
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    rng = np.random.default_rng(0)
+    ri = rng.integers(0, n, 4000)
+    si = rng.integers(0, n, 4000)
+    cnt = O.count_traction_flops(nrm, mid, P1, P2, P3, 0.25, 1.0, 0.4, 1, ri, si)
+    per_pair = (cnt["add"] + cnt["mul"] + cnt["div"]) / cnt["pairs"]
+    assert abs(per_pair - bench.FLOPS_PER_PAIR_HS) / bench.FLOPS_PER_PAIR_HS < 0.01
