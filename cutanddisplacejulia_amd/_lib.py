@@ -32,7 +32,9 @@ CDJ_PREC_MIXED32 = 1
 # every symbol declared in include/cdj.h
 EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", "cdj_td_influence", "cdj_td_field",
            "cdj_traction_influence", "cdj_slip_calculator", "cdj_solve", "cdj_assemble_dev",
-           "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev"]
+           "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev", "cdj_mlp_create", "cdj_mlp_destroy",
+           "cdj_mlp_nparams", "cdj_mlp_get_params", "cdj_mlp_set_params", "cdj_mlp_train_dev", "cdj_mlp_loss_grad",
+           "cdj_precond_solve_dev"]
 
 
 class CdjError(RuntimeError):
@@ -85,6 +87,8 @@ def load(path=LIB_PATH):
         f = getattr(L, name)
         if name == "cdj_last_kernel_ms":
             f.restype = ctypes.c_double
+        elif name == "cdj_mlp_nparams":
+            f.restype = ctypes.c_int64
         elif name != "cdj_last_error":
             f.restype = ctypes.c_int
     _LIB = L

@@ -9,11 +9,16 @@
 #include <vector>
 
 #include "../../include/cdj.h"
+#include "learned.hpp"
 #include "linalg.hpp"
 #include "mixed.hpp"
 #include "td_kernels.hpp"
 
 using namespace cdj;
+
+struct cdj_mlp {
+  cdj::Mlp m;
+};
 
 struct cdj_ctx {
   int device = 0;
@@ -677,6 +682,238 @@ int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const 
   if (report) report->assembly_ms = ams;
   HIPCHK(hipMemcpyAsync(x, AB.as<double>() + N3 * lda, sizeof(double) * N3 * nrhs, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+// ---- learned preconditioner -----------------------------------------------------------------------------
+extern "C" {
+
+int cdj_mlp_create(cdj_ctx* ctx, int32_t k, int32_t hidden, uint64_t seed, cdj_mlp** out) {
+  if (!ctx || !out) return CDJ_ERR_ARG;
+  if (k < 1 || k > KNN_MAX || hidden < 1) return fail(ctx, CDJ_ERR_ARG, "invalid MLP shape");
+  HIPCHK(hipSetDevice(ctx->device));
+  cdj_mlp* p = new cdj_mlp();
+  hipError_t e = mlp_init(p->m, k, hidden, seed);
+  if (e != hipSuccess) {
+    mlp_free(p->m);
+    delete p;
+    return hip_fail(ctx, e, "mlp_init");
+  }
+  *out = p;
+  return CDJ_OK;
+}
+
+int cdj_mlp_destroy(cdj_mlp* mlp) {
+  if (mlp) {
+    mlp_free(mlp->m);
+    delete mlp;
+  }
+  return CDJ_OK;
+}
+
+int64_t cdj_mlp_nparams(const cdj_mlp* mlp) { return mlp ? mlp->m.nparams() : 0; }
+
+int cdj_mlp_get_params(cdj_ctx* ctx, const cdj_mlp* mlp, float* params) {
+  if (!ctx || !mlp || !params) return CDJ_ERR_ARG;
+  HIPCHK(hipMemcpy(params, mlp->m.params, sizeof(float) * mlp->m.nparams(), hipMemcpyDeviceToHost));
+  return CDJ_OK;
+}
+
+int cdj_mlp_set_params(cdj_ctx* ctx, cdj_mlp* mlp, const float* params) {
+  if (!ctx || !mlp || !params) return CDJ_ERR_ARG;
+  HIPCHK(hipMemcpy(mlp->m.params, params, sizeof(float) * mlp->m.nparams(), hipMemcpyHostToDevice));
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct MeshGuard {
+  MeshFeat f;
+  ~MeshGuard() { mesh_free(f); }
+};
+
+// loss/gradient of one batch of samples on prepared mesh features; leaves Y (acts) and grads on the device
+int mlp_step(cdj_ctx* ctx, cdj_mlp* mlp, const MeshFeat& f, const double* jac, const double* B, const double* X,
+             int S, double invG, DBuf& acts, DBuf& dY, DBuf& work, DBuf& E, DBuf& lossp, double* loss,
+             hipStream_t st) {
+  const Mlp& m = mlp->m;
+  const int64_t n = f.n;
+  HIPCHK(mlp_forward(m, f.feat, n, acts.as<float>(), st));
+  const float* Y = acts.as<float>() + 2 * n * m.H;
+  HIPCHK(spai_loss_grad(jac, Y, f, invG, B, X, S, E.as<double>(), dY.as<float>(), lossp.as<double>(), st));
+  HIPCHK(mlp_backward(m, f.feat, n, acts.as<float>(), dY.as<float>(), work.as<float>(), st));
+  if (loss) {
+    DBuf tot;
+    HIPCHK(tot.alloc(sizeof(double)));
+    HIPCHK(device_sum(lossp.as<double>(), n, tot.as<double>(), st));
+    HIPCHK(hipMemcpyAsync(loss, tot.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return CDJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdj_mlp_train_dev(cdj_ctx* ctx, cdj_mlp* mlp, const double* P1, const double* P2, const double* P3, int64_t n,
+                      double G, const double* A, int64_t lda, int32_t nsamples, int32_t iters, float lr,
+                      uint64_t seed, double* loss, void* stream) {
+  if (!ctx || !mlp || !A) return CDJ_ERR_ARG;
+  if (n <= 0 || lda < 3 * n || nsamples < 1 || iters < 1 || G <= 0) return fail(ctx, CDJ_ERR_ARG, "invalid training arguments");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  const Mlp& m = mlp->m;
+  const int S = nsamples;
+  const int64_t N3 = 3 * n;
+  MeshGuard g;
+  HIPCHK(mesh_prepare(P1, P2, P3, n, m.k, g.f, st));
+  DBuf jac, X, B, part, acts, dY, work, E, lossp;
+  HIPCHK(jac.alloc(sizeof(double) * 9 * n));
+  HIPCHK(X.alloc(sizeof(double) * N3 * S));
+  HIPCHK(B.alloc(sizeof(double) * N3 * S));
+  HIPCHK(part.alloc(gemv_part_bytes(N3, N3)));
+  HIPCHK(acts.alloc(sizeof(float) * n * (2 * m.H + m.O)));
+  HIPCHK(dY.alloc(sizeof(float) * n * m.O));
+  HIPCHK(work.alloc(sizeof(float) * 2 * n * m.H));
+  HIPCHK(E.alloc(sizeof(double) * n * S * 3));
+  HIPCHK(lossp.alloc(sizeof(double) * n));
+  HIPCHK(jacobi_blocks(A, n, lda, jac.as<double>(), st));
+  HIPCHK(randn(X.as<double>(), N3 * S, seed, st));
+  for (int s = 0; s < S; ++s)
+    gemv_splitk<double>(A, N3, N3, lda, X.as<double>() + s * N3, B.as<double>() + s * N3, 1.0, nullptr, 0.0,
+                        part.as<double>(), st);
+  for (int it = 0; it < iters; ++it) {
+    int rc = mlp_step(ctx, mlp, g.f, jac.as<double>(), B.as<double>(), X.as<double>(), S, 1.0 / G, acts, dY, work, E,
+                      lossp, it == iters - 1 ? loss : nullptr, st);
+    if (rc) return rc;
+    HIPCHK(mlp_adam(mlp->m, lr, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_mlp_loss_grad(cdj_ctx* ctx, cdj_mlp* mlp, const double* P1, const double* P2, const double* P3, int64_t n,
+                      double G, const double* jac, const double* B, const double* X, int32_t nsamples, double* loss,
+                      float* grad, float* feat_out, int32_t* nbr_out, float* geo_out) {
+  if (!ctx || !mlp || !P1 || !P2 || !P3 || !jac || !B || !X || !loss || !grad) return CDJ_ERR_ARG;
+  if (n <= 0 || nsamples < 1 || G <= 0) return fail(ctx, CDJ_ERR_ARG, "invalid arguments");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const Mlp& m = mlp->m;
+  const int S = nsamples;
+  const int64_t N3 = 3 * n;
+  DBuf d1, d2, d3, dj, dB, dX, acts, dY, work, E, lossp;
+  HIPCHK(d1.alloc(sizeof(double) * 3 * n));
+  HIPCHK(d2.alloc(sizeof(double) * 3 * n));
+  HIPCHK(d3.alloc(sizeof(double) * 3 * n));
+  HIPCHK(dj.alloc(sizeof(double) * 9 * n));
+  HIPCHK(dB.alloc(sizeof(double) * N3 * S));
+  HIPCHK(dX.alloc(sizeof(double) * N3 * S));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dj.p, jac, sizeof(double) * 9 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dB.p, B, sizeof(double) * N3 * S, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dX.p, X, sizeof(double) * N3 * S, hipMemcpyHostToDevice, st));
+  MeshGuard g;
+  HIPCHK(mesh_prepare(d1.as<double>(), d2.as<double>(), d3.as<double>(), n, m.k, g.f, st));
+  HIPCHK(acts.alloc(sizeof(float) * n * (2 * m.H + m.O)));
+  HIPCHK(dY.alloc(sizeof(float) * n * m.O));
+  HIPCHK(work.alloc(sizeof(float) * 2 * n * m.H));
+  HIPCHK(E.alloc(sizeof(double) * n * S * 3));
+  HIPCHK(lossp.alloc(sizeof(double) * n));
+  int rc = mlp_step(ctx, mlp, g.f, dj.as<double>(), dB.as<double>(), dX.as<double>(), S, 1.0 / G, acts, dY, work, E,
+                    lossp, loss, st);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(grad, m.grad, sizeof(float) * m.nparams(), hipMemcpyDeviceToHost, st));
+  if (feat_out) HIPCHK(hipMemcpyAsync(feat_out, g.f.feat, sizeof(float) * n * m.F, hipMemcpyDeviceToHost, st));
+  if (nbr_out) HIPCHK(hipMemcpyAsync(nbr_out, g.f.nbr, sizeof(int32_t) * n * m.k, hipMemcpyDeviceToHost, st));
+  if (geo_out) HIPCHK(hipMemcpyAsync(geo_out, g.f.geo, sizeof(float) * n * GEO, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const double* P1, const double* P2,
+                          const double* P3, int64_t n, double G, const double* A, int64_t lda, const double* b,
+                          double* x, const cdj_opts* opts, cdj_report* report, void* stream) {
+  if (!ctx || !A || !b || !x) return CDJ_ERR_ARG;
+  if (pc == CDJ_PC_LEARNED && !mlp) return fail(ctx, CDJ_ERR_ARG, "learned preconditioner needs a model");
+  if (n <= 0 || lda < 3 * n) return fail(ctx, CDJ_ERR_DIMS, "invalid matrix dimensions");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  const int64_t N3 = 3 * n;
+  if (pc == CDJ_PC_LU32) {
+    cdj_opts o = opts ? *opts : cdj_opts{CDJ_PREC_MIXED32, 0, 1e-10, 0, 2.0};
+    return solve_mixed32(ctx, A, N3, lda, b, x, 1, &o, report, st);
+  }
+  GmresOpts go;
+  go.restart = 60;
+  go.max_iter = 3000;
+  go.max_restarts = 200;
+  if (opts && opts->tol > 0) go.tol = opts->tol;
+  if (opts && opts->max_iter > 0) go.max_iter = opts->max_iter;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  MeshGuard g;
+  DBuf jac, acts;
+  const float* Y = nullptr;
+  if (pc == CDJ_PC_JACOBI || pc == CDJ_PC_LEARNED) {
+    HIPCHK(jac.alloc(sizeof(double) * 9 * n));
+    HIPCHK(jacobi_blocks(A, n, lda, jac.as<double>(), st));
+  }
+  if (pc == CDJ_PC_LEARNED) {
+    const Mlp& m = mlp->m;
+    HIPCHK(mesh_prepare(P1, P2, P3, n, m.k, g.f, st));
+    HIPCHK(acts.alloc(sizeof(float) * n * (2 * m.H + m.O)));
+    HIPCHK(mlp_forward(m, g.f.feat, n, acts.as<float>(), st));
+    Y = acts.as<float>() + 2 * n * m.H;
+  }
+  const double invG = 1.0 / G;
+  Precond M = [&](const double* r, double* z) -> hipError_t {
+    if (pc == CDJ_PC_NONE) return hipMemcpyAsync(z, r, sizeof(double) * N3, hipMemcpyDeviceToDevice, st);
+    if (pc == CDJ_PC_JACOBI) {
+      MeshFeat f0;
+      f0.n = n;
+      f0.k = 0;
+      return spai_apply(jac.as<double>(), nullptr, f0, invG, r, z, st);
+    }
+    return spai_apply(jac.as<double>(), Y, g.f, invG, r, z, st);
+  };
+  GmresResult gr;
+  HIPCHK(fgmres(A, N3, lda, M, b, x, go, gr, st));
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (report) {
+    report->path = CDJ_PREC_LEARNED;
+    report->iterations = gr.iterations;
+    report->residual = gr.residual;
+    report->solve_ms = ms;
+  }
+  if (!gr.converged) {
+    // fallback: FP32-LU preconditioned refinement (itself falling back to the FP64 LU)
+    cdj_opts o = opts ? *opts : cdj_opts{CDJ_PREC_MIXED32, 0, 1e-10, 0, 2.0};
+    o.max_iter = 0;
+    cdj_report r2{};
+    int rc = solve_mixed32(ctx, A, N3, lda, b, x, 1, &o, &r2, st);
+    if (rc) return rc;
+    if (report) {
+      report->path = r2.path;
+      report->iterations += r2.iterations;
+      report->residual = r2.residual;
+      report->solve_ms += r2.solve_ms;
+    }
+  }
   return CDJ_OK;
 }
 

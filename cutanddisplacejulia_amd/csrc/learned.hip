@@ -1,0 +1,519 @@
+// Learned sparse-approximate-inverse (SPAI) preconditioner for the collocation system A x = b.
+//
+// For every triangle i a small MLP reads the local neighbourhood geometry - the k nearest triangles'
+// centroids, normals, strike directions and areas, expressed in triangle i's own (normal, strike, dip)
+// frame and scaled by its size h_i - and emits the 3x3 coupling blocks of row block i of M ~ A^-1 on those k
+// neighbours.  M = (exact inverse of A's 3x3 self blocks) + (h_i / G) * MLP(features_i), so the network
+// learns the near-field correction on top of block Jacobi.  The rows of A are ordered (Tn, Tss, Tds) x
+// receivers and the columns (Dn, Dss, Dds) x sources (SlipCalculator3D.jl:13-16), so DOF (i, a) is a*N + i.
+//
+// Training uses (matrix, solution) pairs from the method's own assembly: random slips x, tractions b = A x
+// (FP64 on the GPU), loss = mean ||M b - x||^2, which is a stochastic estimate of ||M A - I||_F^2.  Forward,
+// backward (hand-written GEMMs with fused bias/tanh) and Adam run on the GPU in FP32.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "learned.hpp"
+#include "mixed.hpp"
+
+namespace cdj {
+
+namespace {
+
+unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---- geometry -----------------------------------------------------------------------------------------------
+// geo[i*GEO + ...] = {cx, cy, cz, nx, ny, nz, sx, sy, sz, dx, dy, dz, h, area, 0, 0}
+__global__ void geom_kernel(const double* __restrict__ P1, const double* __restrict__ P2,
+                            const double* __restrict__ P3, int64_t n, float* __restrict__ geo) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double a[3], b[3], c[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = P1[k * n + i];
+    b[k] = P2[k * n + i];
+    c[k] = P3[k * n + i];
+  }
+  const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  double nn[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+  const double an = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+  for (int k = 0; k < 3; ++k) nn[k] /= an;
+  // strike = ez x n normalised (north for flat, as TD.jl:726-728), dip = n x strike
+  double s[3] = {-nn[1], nn[0], 0.0};
+  double sn = sqrt(s[0] * s[0] + s[1] * s[1]);
+  if (sn == 0.0) {
+    s[0] = 0.0;
+    s[1] = nn[2];
+    s[2] = 0.0;
+    sn = fabs(nn[2]);
+  }
+  for (int k = 0; k < 3; ++k) s[k] /= sn;
+  const double d[3] = {nn[1] * s[2] - nn[2] * s[1], nn[2] * s[0] - nn[0] * s[2], nn[0] * s[1] - nn[1] * s[0]};
+  const double e1 = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const double e2 = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const double w[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+  const double e3 = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  float* g = geo + i * GEO;
+  for (int k = 0; k < 3; ++k) {
+    g[k] = (float)((a[k] + b[k] + c[k]) / 3.0);
+    g[3 + k] = (float)nn[k];
+    g[6 + k] = (float)s[k];
+    g[9 + k] = (float)d[k];
+  }
+  g[12] = (float)((e1 + e2 + e3) / 3.0);
+  g[13] = (float)(0.5 * an);
+  g[14] = 0.f;
+  g[15] = 0.f;
+}
+
+// k nearest centroids (itself first), ties broken by index; brute force, one thread per triangle.
+__global__ void knn_kernel(const float* __restrict__ geo, int64_t n, int k, int32_t* __restrict__ nbr) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float bd[KNN_MAX];
+  int bi[KNN_MAX];
+  for (int q = 0; q < k; ++q) {
+    bd[q] = 3.0e38f;
+    bi[q] = -1;
+  }
+  const float cx = geo[i * GEO], cy = geo[i * GEO + 1], cz = geo[i * GEO + 2];
+  for (int64_t j = 0; j < n; ++j) {
+    const float dx = geo[j * GEO] - cx, dy = geo[j * GEO + 1] - cy, dz = geo[j * GEO + 2] - cz;
+    const float dd = j == i ? -1.0f : dx * dx + dy * dy + dz * dz;
+    if (dd < bd[k - 1]) {
+      int q = k - 1;
+      while (q > 0 && bd[q - 1] > dd) {
+        bd[q] = bd[q - 1];
+        bi[q] = bi[q - 1];
+        --q;
+      }
+      bd[q] = dd;
+      bi[q] = (int)j;
+    }
+  }
+  for (int q = 0; q < k; ++q) nbr[i * k + q] = bi[q] >= 0 ? bi[q] : (int)i;
+}
+
+// features[i][q*NF + f]: neighbour q of i in i's frame: relative centroid / h_i (3), normal (3),
+// strike direction (3), area / h_i^2 (1).
+__global__ void features_kernel(const float* __restrict__ geo, const int32_t* __restrict__ nbr, int64_t n, int k,
+                                float* __restrict__ feat) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* gi = geo + i * GEO;
+  const float h = gi[12];
+  const float ih = 1.0f / h;
+  float* f = feat + i * (int64_t)k * NF;
+  for (int q = 0; q < k; ++q) {
+    const float* gj = geo + (int64_t)nbr[i * k + q] * GEO;
+    const float rc[3] = {(gj[0] - gi[0]) * ih, (gj[1] - gi[1]) * ih, (gj[2] - gi[2]) * ih};
+    const float* axes[3] = {gi + 3, gi + 6, gi + 9};
+    for (int a = 0; a < 3; ++a) {
+      f[q * NF + a] = rc[0] * axes[a][0] + rc[1] * axes[a][1] + rc[2] * axes[a][2];
+      f[q * NF + 3 + a] = gj[3] * axes[a][0] + gj[4] * axes[a][1] + gj[5] * axes[a][2];
+      f[q * NF + 6 + a] = gj[6] * axes[a][0] + gj[7] * axes[a][1] + gj[8] * axes[a][2];
+    }
+    f[q * NF + 9] = gj[13] * ih * ih;
+  }
+}
+
+// ---- FP32 GEMM with fused epilogue: C = act(op(A) op(B) + bias) (+ C if accumulate) ------------------------
+// Row-major.  op(A) is M x K, op(B) is K x N.  64 x 64 tiles, 256 threads, 4 x 4 outputs per thread.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) gemm_f32_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                       const float* __restrict__ B, int ldb, float* __restrict__ C,
+                                                       int ldc, const float* __restrict__ bias, int act,
+                                                       int accumulate) {
+  __shared__ float As[16][64 + 1];
+  __shared__ float Bs[16][64 + 1];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int idx = threadIdx.x; idx < 16 * 64; idx += 256) {
+      const int kk = idx / 64, mm = idx % 64;
+      const int gm = m0 + mm, gk = k0 + kk;
+      float va = 0.f;
+      if (gm < M && gk < K) va = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
+      As[kk][mm] = va;
+      const int gn = n0 + mm;
+      float vb = 0.f;
+      if (gn < N && gk < K) vb = TB ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
+      Bs[kk][mm] = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) a[r] = As[kk][ty * 4 + r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) b[c] = Bs[kk][tx * 4 + c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] += a[r] * b[c];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int gm = m0 + ty * 4 + r, gn = n0 + tx * 4 + c;
+      if (gm < M && gn < N) {
+        float v = acc[r][c];
+        if (bias) v += bias[gn];
+        if (act == 1) v = tanhf(v);
+        float* dst = C + (int64_t)gm * ldc + gn;
+        *dst = accumulate ? *dst + v : v;
+      }
+    }
+}
+
+// g = g * (1 - h^2)  (tanh derivative)
+__global__ void tanh_back_kernel(const float* __restrict__ h, float* __restrict__ g, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) g[i] *= (1.f - h[i] * h[i]);
+}
+
+// column sums of an M x N row-major matrix: out[c] = sum_r G[r][c] (one block per 64 columns, fixed order)
+__global__ void colsum_kernel(const float* __restrict__ G, int M, int N, float* __restrict__ out) {
+  __shared__ float s[4][64];
+  const int c = blockIdx.x * 64 + threadIdx.x % 64;
+  const int part = threadIdx.x / 64;
+  float v = 0.f;
+  if (c < N)
+    for (int r = part; r < M; r += 4) v += G[(int64_t)r * N + c];
+  s[part][threadIdx.x % 64] = v;
+  __syncthreads();
+  if (part == 0 && c < N) out[c] = (s[0][threadIdx.x] + s[1][threadIdx.x]) + (s[2][threadIdx.x] + s[3][threadIdx.x]);
+}
+
+// ---- SPAI blocks, loss and gradient -----------------------------------------------------------------------
+// Block-Jacobi: inverse of A's 3x3 self block of every triangle (FP64).  jac[i*9 + a*3 + b].
+__global__ void jacobi_kernel(const double* __restrict__ A, int64_t n, int64_t lda, double* __restrict__ jac) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m[3][3];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) m[a][b] = A[(a * n + i) + (b * n + i) * lda];
+  const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+  const double id = det != 0.0 ? 1.0 / det : 0.0;
+  double inv[3][3];
+  inv[0][0] = c00 * id;
+  inv[1][0] = c01 * id;
+  inv[2][0] = c02 * id;
+  inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+  inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+  inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+  inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+  inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+  inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) jac[i * 9 + a * 3 + b] = inv[a][b];
+}
+
+// z = M r for one vector: z[a*n + i] = sum_b jac_i[a][b] r[b*n + i] + s_i sum_q sum_b Y_i[q][a][b] r[b*n + nbr_q]
+// with s_i = h_i / G (Y == nullptr: block Jacobi only; jac == nullptr: no Jacobi term).
+__global__ void spai_apply_kernel(const double* __restrict__ jac, const float* __restrict__ Y,
+                                  const int32_t* __restrict__ nbr, const float* __restrict__ geo, int64_t n, int k,
+                                  double invG, const double* __restrict__ r, double* __restrict__ z) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double o[3] = {0, 0, 0};
+  if (jac)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) o[a] += jac[i * 9 + a * 3 + b] * r[b * n + i];
+  if (Y) {
+    const double s = (double)geo[i * GEO + 12] * invG;
+    double l[3] = {0, 0, 0};
+    for (int q = 0; q < k; ++q) {
+      const int64_t j = nbr[i * k + q];
+      const float* y = Y + i * (int64_t)k * 9 + q * 9;
+      const double r0 = r[j], r1 = r[n + j], r2 = r[2 * n + j];
+      for (int a = 0; a < 3; ++a) l[a] += (double)y[a * 3] * r0 + (double)y[a * 3 + 1] * r1 + (double)y[a * 3 + 2] * r2;
+    }
+    for (int a = 0; a < 3; ++a) o[a] += s * l[a];
+  }
+  for (int a = 0; a < 3; ++a) z[a * n + i] = o[a];
+}
+
+// Residuals of S samples (B, X: 3n x S column-major, FP64): E[(i*S + s)*3 + a] = (M b_s - x_s)[a*n + i] and
+// lossp[i] = sum_s |E_{i,s}|^2 / (S n).
+__global__ void spai_resid_kernel(const double* __restrict__ jac, const float* __restrict__ Y,
+                                  const int32_t* __restrict__ nbr, const float* __restrict__ geo, int64_t n, int k,
+                                  double invG, const double* __restrict__ B, const double* __restrict__ X, int S,
+                                  double* __restrict__ E, double* __restrict__ lossp) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double sc = (double)geo[i * GEO + 12] * invG;
+  double loss = 0.0;
+  for (int sidx = 0; sidx < S; ++sidx) {
+    const double* b = B + (int64_t)sidx * 3 * n;
+    const double* x = X + (int64_t)sidx * 3 * n;
+    double o[3] = {0, 0, 0};
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) o[a] += jac[i * 9 + a * 3 + c] * b[c * n + i];
+    double l[3] = {0, 0, 0};
+    for (int q = 0; q < k; ++q) {
+      const int64_t j = nbr[i * k + q];
+      const float* y = Y + i * (int64_t)k * 9 + q * 9;
+      const double b0 = b[j], b1 = b[n + j], b2 = b[2 * n + j];
+      for (int a = 0; a < 3; ++a) l[a] += (double)y[a * 3] * b0 + (double)y[a * 3 + 1] * b1 + (double)y[a * 3 + 2] * b2;
+    }
+    for (int a = 0; a < 3; ++a) {
+      const double e = o[a] + sc * l[a] - x[a * n + i];
+      E[(i * S + sidx) * 3 + a] = e;
+      loss += e * e;
+    }
+  }
+  lossp[i] = loss / ((double)S * (double)n);
+}
+
+// dY[i][q*9 + a*3 + c] = (2 / (S n)) (h_i / G) sum_s E_{i,s,a} B_s[c*n + nbr_{i,q}], one thread per entry.
+__global__ void spai_grad_kernel(const double* __restrict__ E, const int32_t* __restrict__ nbr,
+                                 const float* __restrict__ geo, int64_t n, int k, double invG,
+                                 const double* __restrict__ B, int S, float* __restrict__ dY) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)k * 9;
+  if (t >= n * per) return;
+  const int64_t i = t / per;
+  const int r = (int)(t % per), q = r / 9, a = (r % 9) / 3, c = r % 3;
+  const int64_t j = nbr[i * k + q];
+  double g = 0.0;
+  for (int sidx = 0; sidx < S; ++sidx) g += E[(i * S + sidx) * 3 + a] * B[(int64_t)sidx * 3 * n + c * n + j];
+  const double sc = (double)geo[i * GEO + 12] * invG;
+  dY[t] = (float)(2.0 * sc * g / ((double)S * (double)n));
+}
+
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// counter-based normal random numbers (splitmix64 + Box-Muller)
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ void randn_kernel(double* __restrict__ x, int64_t n, uint64_t seed) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = mix64(seed ^ mix64((uint64_t)i * 2 + 1));
+  const uint64_t b = mix64(seed ^ mix64((uint64_t)i * 2 + 2));
+  const double u1 = ((double)(a >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = ((double)(b >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  x[i] = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__global__ void sum_kernel(const double* __restrict__ a, int64_t n, double* __restrict__ out) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) v += a[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+}  // namespace
+
+void gemm_f32(bool ta, bool tb, int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C,
+              int ldc, const float* bias, int act, int accumulate, hipStream_t st) {
+  dim3 grid(nblk(N, 64), nblk(M, 64));
+  if (!ta && !tb)
+    hipLaunchKernelGGL((gemm_f32_kernel<false, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias,
+                       act, accumulate);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((gemm_f32_kernel<true, false>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias,
+                       act, accumulate);
+  else if (!ta && tb)
+    hipLaunchKernelGGL((gemm_f32_kernel<false, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias,
+                       act, accumulate);
+  else
+    hipLaunchKernelGGL((gemm_f32_kernel<true, true>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc, bias,
+                       act, accumulate);
+}
+
+// ---- model --------------------------------------------------------------------------------------------------
+int64_t Mlp::nparams() const { return (int64_t)F * H + H + (int64_t)H * H + H + (int64_t)H * O + O; }
+
+void Mlp::offsets(int64_t off[6]) const {
+  off[0] = 0;                       // W1 F x H
+  off[1] = off[0] + (int64_t)F * H;  // b1
+  off[2] = off[1] + H;              // W2 H x H
+  off[3] = off[2] + (int64_t)H * H;  // b2
+  off[4] = off[3] + H;              // W3 H x O
+  off[5] = off[4] + (int64_t)H * O;  // b3
+}
+
+hipError_t mlp_init(Mlp& m, int k, int hidden, uint64_t seed) {
+  m.k = k;
+  m.F = k * NF;
+  m.H = hidden;
+  m.O = k * 9;
+  m.t = 0;
+  const int64_t np = m.nparams();
+  std::vector<float> h(np, 0.f);
+  int64_t off[6];
+  m.offsets(off);
+  // Glorot-uniform weights from a fixed-seed generator, zero biases; the output layer starts small so the
+  // untrained preconditioner is block Jacobi.
+  uint64_t st = seed ? seed : 1;
+  auto uni = [&]() {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+  };
+  auto fill = [&](int64_t o, int fin, int fout, double gain) {
+    const double lim = gain * std::sqrt(6.0 / (fin + fout));
+    for (int64_t q = 0; q < (int64_t)fin * fout; ++q) h[o + q] = (float)((2 * uni() - 1) * lim);
+  };
+  fill(off[0], m.F, m.H, 1.0);
+  fill(off[2], m.H, m.H, 1.0);
+  fill(off[4], m.H, m.O, 0.01);
+  hipError_t e;
+  for (float** p : {&m.params, &m.grad, &m.adam_m, &m.adam_v}) {
+    e = hipMalloc(p, sizeof(float) * np);
+    if (e) return e;
+    e = hipMemset(*p, 0, sizeof(float) * np);
+    if (e) return e;
+  }
+  return hipMemcpy(m.params, h.data(), sizeof(float) * np, hipMemcpyHostToDevice);
+}
+
+void mlp_free(Mlp& m) {
+  for (float* p : {m.params, m.grad, m.adam_m, m.adam_v})
+    if (p) (void)hipFree(p);
+  m.params = m.grad = m.adam_m = m.adam_v = nullptr;
+}
+
+hipError_t mesh_prepare(const double* P1, const double* P2, const double* P3, int64_t n, int k, MeshFeat& f,
+                        hipStream_t st) {
+  hipError_t e;
+  f.n = n;
+  f.k = k;
+  if ((e = hipMalloc(&f.geo, sizeof(float) * GEO * n))) return e;
+  if ((e = hipMalloc(&f.nbr, sizeof(int32_t) * k * n))) return e;
+  if ((e = hipMalloc(&f.feat, sizeof(float) * (int64_t)k * NF * n))) return e;
+  hipLaunchKernelGGL(geom_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, P1, P2, P3, n, f.geo);
+  hipLaunchKernelGGL(knn_kernel, dim3(nblk(n, 64)), dim3(64), 0, st, (const float*)f.geo, n, k, f.nbr);
+  hipLaunchKernelGGL(features_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, (const float*)f.geo,
+                     (const int32_t*)f.nbr, n, k, f.feat);
+  return hipGetLastError();
+}
+
+void mesh_free(MeshFeat& f) {
+  for (void* p : {(void*)f.geo, (void*)f.nbr, (void*)f.feat})
+    if (p) (void)hipFree(p);
+  f.geo = nullptr;
+  f.nbr = nullptr;
+  f.feat = nullptr;
+}
+
+// Forward pass for n triangles: acts = [h1 (n x H) | h2 (n x H) | Y (n x O)].
+hipError_t mlp_forward(const Mlp& m, const float* feat, int64_t n, float* acts, hipStream_t st) {
+  int64_t off[6];
+  m.offsets(off);
+  float* h1 = acts;
+  float* h2 = acts + n * m.H;
+  float* Y = acts + 2 * n * m.H;
+  const int N = (int)n;
+  gemm_f32(false, false, N, m.H, m.F, feat, m.F, m.params + off[0], m.H, h1, m.H, m.params + off[1], 1, 0, st);
+  gemm_f32(false, false, N, m.H, m.H, h1, m.H, m.params + off[2], m.H, h2, m.H, m.params + off[3], 1, 0, st);
+  gemm_f32(false, false, N, m.O, m.H, h2, m.H, m.params + off[4], m.O, Y, m.O, m.params + off[5], 0, 0, st);
+  return hipGetLastError();
+}
+
+// Backward pass: dY (n x O) -> m.grad (overwritten).  work: 2 n H floats.
+hipError_t mlp_backward(const Mlp& m, const float* feat, int64_t n, const float* acts, const float* dY, float* work,
+                        hipStream_t st) {
+  int64_t off[6];
+  m.offsets(off);
+  const float* h1 = acts;
+  const float* h2 = acts + n * m.H;
+  float* dz2 = work;
+  float* dz1 = work + n * m.H;
+  const int N = (int)n;
+  float* g = m.grad;
+  // layer 3: dW3 = h2^T dY, db3 = colsum(dY)
+  gemm_f32(true, false, m.H, m.O, N, h2, m.H, dY, m.O, g + off[4], m.O, nullptr, 0, 0, st);
+  hipLaunchKernelGGL(colsum_kernel, dim3(nblk(m.O, 64)), dim3(256), 0, st, dY, N, m.O, g + off[5]);
+  // dz2 = (dY W3^T) * (1 - h2^2); dW2 = h1^T dz2, db2 = colsum(dz2)
+  gemm_f32(false, true, N, m.H, m.O, dY, m.O, m.params + off[4], m.O, dz2, m.H, nullptr, 0, 0, st);
+  hipLaunchKernelGGL(tanh_back_kernel, dim3(nblk(n * m.H, 256)), dim3(256), 0, st, h2, dz2, n * m.H);
+  gemm_f32(true, false, m.H, m.H, N, h1, m.H, dz2, m.H, g + off[2], m.H, nullptr, 0, 0, st);
+  hipLaunchKernelGGL(colsum_kernel, dim3(nblk(m.H, 64)), dim3(256), 0, st, (const float*)dz2, N, m.H, g + off[3]);
+  // dz1 = (dz2 W2^T) * (1 - h1^2); dW1 = feat^T dz1, db1 = colsum(dz1)
+  gemm_f32(false, true, N, m.H, m.H, dz2, m.H, m.params + off[2], m.H, dz1, m.H, nullptr, 0, 0, st);
+  hipLaunchKernelGGL(tanh_back_kernel, dim3(nblk(n * m.H, 256)), dim3(256), 0, st, h1, dz1, n * m.H);
+  gemm_f32(true, false, m.F, m.H, N, feat, m.F, dz1, m.H, g + off[0], m.H, nullptr, 0, 0, st);
+  hipLaunchKernelGGL(colsum_kernel, dim3(nblk(m.H, 64)), dim3(256), 0, st, (const float*)dz1, N, m.H, g + off[1]);
+  return hipGetLastError();
+}
+
+hipError_t mlp_adam(Mlp& m, float lr, hipStream_t st) {
+  m.t += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)m.t), c2 = 1.f - std::pow(b2, (float)m.t);
+  const int64_t np = m.nparams();
+  hipLaunchKernelGGL(adam_kernel, dim3(nblk(np, 256)), dim3(256), 0, st, m.params, (const float*)m.grad, m.adam_m,
+                     m.adam_v, np, lr, b1, b2, 1e-8f, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t jacobi_blocks(const double* A, int64_t n, int64_t lda, double* jac, hipStream_t st) {
+  hipLaunchKernelGGL(jacobi_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, A, n, lda, jac);
+  return hipGetLastError();
+}
+
+hipError_t spai_apply(const double* jac, const float* Y, const MeshFeat& f, double invG, const double* r, double* z,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(spai_apply_kernel, dim3(nblk(f.n, 128)), dim3(128), 0, st, jac, Y, (const int32_t*)f.nbr,
+                     (const float*)f.geo, f.n, f.k, invG, r, z);
+  return hipGetLastError();
+}
+
+hipError_t spai_loss_grad(const double* jac, const float* Y, const MeshFeat& f, double invG, const double* B,
+                          const double* X, int S, double* E, float* dY, double* lossp, hipStream_t st) {
+  hipLaunchKernelGGL(spai_resid_kernel, dim3(nblk(f.n, 64)), dim3(64), 0, st, jac, Y, (const int32_t*)f.nbr,
+                     (const float*)f.geo, f.n, f.k, invG, B, X, S, E, lossp);
+  hipLaunchKernelGGL(spai_grad_kernel, dim3(nblk(f.n * f.k * 9, 256)), dim3(256), 0, st, (const double*)E,
+                     (const int32_t*)f.nbr, (const float*)f.geo, f.n, f.k, invG, B, S, dY);
+  return hipGetLastError();
+}
+
+hipError_t randn(double* x, int64_t n, uint64_t seed, hipStream_t st) {
+  hipLaunchKernelGGL(randn_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, x, n, seed);
+  return hipGetLastError();
+}
+
+hipError_t device_sum(const double* a, int64_t n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, a, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace cdj

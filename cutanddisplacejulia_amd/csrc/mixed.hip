@@ -280,9 +280,8 @@ hipError_t host_dot(const double* a, const double* b, int64_t n, double* part, s
 
 }  // namespace
 
-template <class TF>
-hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int64_t ldlu, const int32_t* perm,
-                     const double* b, double* x, const GmresOpts& o, GmresResult& res, hipStream_t st) {
+hipError_t fgmres(const double* A, int64_t n, int64_t lda, const Precond& M, const double* b, double* x,
+                  const GmresOpts& o, GmresResult& res, hipStream_t st) {
   hipError_t e = hipSuccess;
   Dev d;
   const int m = o.restart;
@@ -294,21 +293,10 @@ hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int6
   if (e) return e;
   double* r = d.get<double>(n, e);
   if (e) return e;
-  TF* work = d.get<TF>(n, e);
-  if (e) return e;
   double* part = d.get<double>(std::max<size_t>(gemv_part_bytes(n, n) / sizeof(double), DOT_BLOCKS * (size_t)(m + 1)), e);
   if (e) return e;
   double* hdev = d.get<double>(m + 1, e);
   if (e) return e;
-  int* flags = d.get<int>((n + TB - 1) / TB + 1, e);
-  if (e) return e;
-  int* err = d.get<int>(1, e);
-  if (e) return e;
-  e = hipMemsetAsync(flags, 0, sizeof(int) * ((n + TB - 1) / TB + 1), st);
-  if (e) return e;
-  e = hipMemsetAsync(err, 0, sizeof(int), st);
-  if (e) return e;
-  TrsvState ts{flags, err, 0, o.spin_limit};
   std::vector<double> hp(DOT_BLOCKS * (size_t)(m + 1));
   const unsigned g = nblk(n, 256);
 
@@ -320,7 +308,11 @@ hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int6
   res.converged = false;
   res.history.clear();
   // initial guess: x = M^-1 b
-  e = apply_lu_precond<TF>(LU, n, ldlu, perm, b, x, work, ts, st);
+  if (o.zero_guess) {
+    e = hipMemsetAsync(x, 0, sizeof(double) * n, st);
+  } else {
+    e = M(b, x);
+  }
   if (e) return e;
   for (int outer = 0; outer < o.max_restarts; ++outer) {
     // r = b - A x  (FP64)
@@ -344,7 +336,7 @@ hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int6
     for (; k < m && res.iterations < o.max_iter; ++k) {
       res.iterations++;
       // z_k = M^-1 v_k ; w = A z_k
-      e = apply_lu_precond<TF>(LU, n, ldlu, perm, V + (int64_t)k * n, Z + (int64_t)k * n, work, ts, st);
+      e = M(V + (int64_t)k * n, Z + (int64_t)k * n);
       if (e) return e;
       gemv_splitk<double>(A, n, n, lda, Z + (int64_t)k * n, w, 1.0, nullptr, 0.0, part, st);
       // classical Gram-Schmidt, twice (CGS2) for FP64 orthogonality
@@ -418,13 +410,38 @@ hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int6
     res.history.push_back(res.residual);
     res.converged = res.residual <= o.tol;
   }
+  e = hipStreamSynchronize(st);
+  if (e) return e;
+  return hipGetLastError();
+}
+
+template <class TF>
+hipError_t fgmres_ir(const double* A, int64_t n, int64_t lda, const TF* LU, int64_t ldlu, const int32_t* perm,
+                     const double* b, double* x, const GmresOpts& o, GmresResult& res, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  Dev d;
+  TF* work = d.get<TF>(n, e);
+  if (e) return e;
+  int* flags = d.get<int>((n + TB - 1) / TB + 1, e);
+  if (e) return e;
+  int* err = d.get<int>(1, e);
+  if (e) return e;
+  e = hipMemsetAsync(flags, 0, sizeof(int) * ((n + TB - 1) / TB + 1), st);
+  if (e) return e;
+  e = hipMemsetAsync(err, 0, sizeof(int), st);
+  if (e) return e;
+  TrsvState ts{flags, err, 0, o.spin_limit};
+  Precond M = [&](const double* r, double* z) { return apply_lu_precond<TF>(LU, n, ldlu, perm, r, z, work, ts, st); };
+  e = fgmres(A, n, lda, M, b, x, o, res, st);
+  if (e) return e;
   int herr = 0;
   e = hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st);
   if (e) return e;
   e = hipStreamSynchronize(st);
   if (e) return e;
   res.trsv_timeout = herr != 0;
-  return hipGetLastError();
+  if (res.trsv_timeout) res.converged = false;
+  return hipSuccess;
 }
 
 template void gemv_splitk<double>(const double*, int64_t, int64_t, int64_t, const double*, double*, double,

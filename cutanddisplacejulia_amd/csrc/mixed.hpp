@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace cdj {
@@ -20,7 +21,11 @@ struct GmresOpts {
   int max_iter = 200;
   int max_restarts = 20;
   long long spin_limit = 1LL << 26;
+  bool zero_guess = false;  // start from x = 0 instead of x = M^-1 b
 };
+
+// Preconditioner application z = M^-1 r on device vectors of length n (enqueued on the solver's stream).
+using Precond = std::function<hipError_t(const double* r, double* z)>;
 
 struct GmresResult {
   int iterations = 0;
@@ -44,6 +49,11 @@ hipError_t trsv_persistent(const T* LU, int64_t n, int64_t lda, T* x, bool lower
 template <class T>
 hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* r, double* z,
                             T* work, TrsvState& s, hipStream_t st);
+
+// Right-preconditioned restarted FGMRES on A x = b (A FP64, n x n) with FP64 products and residuals, until
+// ||b - A x|| / ||b|| <= o.tol or o.max_iter iterations.
+hipError_t fgmres(const double* A, int64_t n, int64_t lda, const Precond& M, const double* b, double* x,
+                  const GmresOpts& o, GmresResult& res, hipStream_t st);
 
 // Solve A x = b (A FP64, n x n) by right-preconditioned restarted FGMRES with M = P^T L U in precision TF,
 // until ||b - A x|| / ||b|| <= o.tol.  x receives the solution.

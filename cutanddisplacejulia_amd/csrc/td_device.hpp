@@ -252,92 +252,116 @@ template <class T> __device__ __forceinline__ void ad_disp(T x, T y, T z, T cosA
 }
 
 // ---- angular dislocation, half-space harmonic part, displacement: AngDisDispFSC TD.jl:1412-1496 ---------
-// S is T (displacement) or D3<T> (displacement plus its gradient).  o = {ux,uy,uz, vx,vy,vz, wx,wy,wz}
-// (u,v,w = component; x,y,z suffix = Burgers component); the 1/(4 pi (1-nu)) factor is applied by callers.
+// S is T (displacement) or D3<T> (displacement plus its gradient).  Outputs o = 0..8 are
+// {ux,uy,uz, vx,vy,vz, wx,wy,wz} (u,v,w = component; x,y,z suffix = Burgers component); the 1/(4 pi (1-nu))
+// factor is applied by callers.  The quantities shared by all nine outputs are formed once (FscBase), each
+// output separately (fsc_out), so a caller can evaluate the outputs one at a time and keep register use low.
+template <class T, class S> struct FscBase {
+  T cosB, sinB, cotB, icosB, nu, a, c1, c2, nu2, cot2;
+  S y1, y2, y3b, rb, irb, iW6, iW7, lW6, lW7, Fib;
+};
+
 template <class T, class S>
-__device__ __forceinline__ void fsc_disp(S y1, S y2, S y3, T cosB, T sinB, T cotB, T cotB2, T icosB, T nu, T a, S* o) {
-  const T c1 = T(1) - T(2) * nu;
-  const T c2 = T(1) - nu;
-  const T nu2 = T(2) * nu;
-  const T cot2 = cotB * cotB;
-  const S y3b = y3 + T(2) * a;
+__device__ __forceinline__ void fsc_base(S y1, S y2, S y3, T cosB, T sinB, T cotB, T cotB2, T icosB, T nu, T a,
+                                         FscBase<T, S>& f) {
+  f.cosB = cosB;
+  f.sinB = sinB;
+  f.cotB = cotB;
+  f.icosB = icosB;
+  f.nu = nu;
+  f.a = a;
+  f.c1 = T(1) - T(2) * nu;
+  f.c2 = T(1) - nu;
+  f.nu2 = T(2) * nu;
+  f.cot2 = cotB * cotB;
+  f.y1 = y1;
+  f.y2 = y2;
+  f.y3b = y3 + T(2) * a;
+  const S z3b = f.y3b * cosB - y1 * sinB;
+  f.rb = dsqrt(y1 * y1 + y2 * y2 + f.y3b * f.y3b);
+  f.irb = rcp(f.rb);
+  const S W6 = f.rb + f.y3b;
+  const S W7 = f.rb + z3b;
+  f.iW6 = rcp(W6);
+  f.iW7 = rcp(W7);
+  f.lW6 = dlog(W6);
+  f.lW7 = dlog(W7);
+  f.Fib = burgers2(y2, y1 - W6 * cotB2);
+}
+
+template <class T, class S> __device__ __forceinline__ S fsc_out(const FscBase<T, S>& f, int o) {
+  const T cosB = f.cosB, sinB = f.sinB, cotB = f.cotB, icosB = f.icosB, nu = f.nu, a = f.a, c1 = f.c1, c2 = f.c2,
+          nu2 = f.nu2, cot2 = f.cot2;
+  const S &y1 = f.y1, &y2 = f.y2, &y3b = f.y3b, &rb = f.rb, &irb = f.irb, &iW6 = f.iW6, &iW7 = f.iW7,
+          &lW6 = f.lW6, &lW7 = f.lW7, &Fib = f.Fib;
+  // cheap derived quantities are recomputed per output to keep the live set small
   const S z1b = y1 * cosB + y3b * sinB;
-  const S z3b = y3b * cosB - y1 * sinB;
-  const S rb = dsqrt(y1 * y1 + y2 * y2 + y3b * y3b);
-  const S irb = rcp(rb);
   const S irb2 = irb * irb;
   const S irb3 = irb2 * irb;
-  const S W6 = rb + y3b;
-  const S W7 = rb + z3b;
-  const S iW6 = rcp(W6);
-  const S iW7 = rcp(W7);
-  const S lW6 = dlog(W6);
-  const S lW7 = dlog(W7);
-  const S Fib = burgers2(y2, y1 - W6 * cotB2);
   const S aR = a * irb;
   const S y3ma = y3b - a;
-  const S rcy = rb * cosB + y3b;   // rb cos(B) + y3b
-  const S rsy = rb * sinB - y1;    // rb sin(B) - y1
-  const S cpa = cosB + aR;          // cos(B) + a/rb
+  const S rcy = rb * cosB + y3b;
+  const S rsy = rb * sinB - y1;
+  const S cpa = cosB + aR;
   const S y22 = y2 * y2;
   const S y11 = y1 * y1;
   const S q6 = y3ma * iW6;
   const S q7 = y3ma * iW7;
-  const S p1 = T(1) + aR * icosB;   // 1 + a/(rb cos(B))
+  const S p1 = T(1) + aR * icosB;
+  switch (o) {
+    case 0:  // ux
+      return -T(2) * c2 * c1 * cot2 * Fib + c1 * y2 * iW6 * ((c1 - aR) * cotB - y1 * iW6 * (nu + aR)) +
+             c1 * cosB * cotB * y2 * iW7 * cpa + a * cotB * y2 * y3ma * irb3 +
+             y2 * y3ma * irb * iW6 * (-c1 * cotB + y1 * iW6 * (nu2 + aR) + a * y1 * irb2) +
+             y2 * y3ma * irb * iW7 *
+                 (cosB * iW7 * (rcy * (c1 * cosB - aR) * cotB + T(2) * c2 * cosB * rsy) - a * cosB * cotB * y3b * irb2);
+    case 1:  // uy
+      return c1 * ((T(2) * c2 * cot2 + nu) * lW6 - (T(2) * c2 * cot2 + T(1)) * cosB * lW7) +
+             c1 * iW6 * (-c1 * cotB * y1 + nu * y3b - a + a * cotB * y1 * irb + y11 * iW6 * (nu + aR)) -
+             c1 * cotB * iW7 * (cosB * z1b - a * icosB * rsy * irb) - a * cotB * y1 * y3ma * irb3 +
+             q6 * (nu2 + irb * (c1 * cotB * y1 + a) - y11 * irb * iW6 * (nu2 + aR) - a * y11 * irb3) +
+             cotB * q7 * (-cosB * sinB + a * icosB * y1 * y3b * irb3 + rsy * irb * (T(2) * c2 * cosB - rcy * iW7 * p1));
+    case 2:  // uz
+      return c1 * (y2 * iW6 * (T(1) + aR) - cosB * y2 * iW7 * cpa) - y2 * y3ma * irb * (a * irb2 + iW6) +
+             cosB * y2 * y3ma * irb * iW7 * (rcy * iW7 * cpa + a * y3b * irb2);
+    case 3:  // vx
+      return c1 * ((T(2) * c2 * cot2 - nu) * lW6 - (T(2) * c2 * cot2 + c1) * cosB * lW7) -
+             c1 * iW6 * (y1 * cotB * (c1 - aR) + nu * y3b - a + y22 * iW6 * (nu + aR)) -
+             c1 * cotB * z1b * iW7 * cpa - a * cotB * y1 * y3ma * irb3 +
+             q6 * (-nu2 + irb * (c1 * cotB * y1 - a) + y22 * irb * iW6 * (nu2 + aR) + a * y22 * irb3) +
+             q7 * (cosB * cosB - irb * (c1 * cotB * z1b + a * cosB) + a * cotB * y3b * z1b * irb3 -
+                   irb * iW7 * (cosB * cosB * y22 - a * cotB * z1b * irb * rcy));
+    case 4:  // vy
+      return T(2) * c2 * c1 * cot2 * Fib + c1 * y2 * iW6 * (-(c1 - aR) * cotB + y1 * iW6 * (nu + aR)) -
+             c1 * cotB * y2 * iW7 * p1 - a * cotB * y2 * y3ma * irb3 +
+             y2 * y3ma * irb * iW6 * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb * (irb + iW6)) +
+             cotB * y2 * y3ma * irb * iW7 * (-T(2) * c2 * cosB + rcy * iW7 * p1 + a * icosB * y3b * irb2);
+    case 5:  // vz
+      return c1 * (-sinB * lW7 - y1 * iW6 * (T(1) + aR) + z1b * iW7 * cpa) + y1 * y3ma * irb * (a * irb2 + iW6) -
+             q7 * (sinB * (cosB - aR) + z1b * irb * (T(1) + a * y3b * irb2) -
+                   irb * iW7 * (cosB * sinB * y22 - a * z1b * irb * rcy));
+    case 6:  // wx
+      return T(2) * c2 * (c1 * cotB * Fib + y2 * iW6 * (nu2 + aR) - cosB * y2 * iW7 * cpa) +
+             y2 * y3ma * irb * (nu2 * iW6 + a * irb2) + cosB * y2 * y3ma * irb * iW7 * (c1 - rcy * iW7 * cpa - a * y3b * irb2);
+    case 7:  // wy
+      return -T(2) * c2 * c1 * cotB * (lW6 - cosB * lW7) - T(2) * c2 * y1 * iW6 * (nu2 + aR) +
+             T(2) * c2 * z1b * iW7 * cpa + y3ma * irb * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb2) -
+             q7 * (cosB * sinB + cotB * rcy * irb * (T(2) * c2 * cosB - rcy * iW7) +
+                   aR * (sinB - y3b * z1b * irb2 - z1b * rcy * irb * iW7));
+    default:  // wz
+      return T(2) * c2 * Fib + T(2) * c2 * sinB * y2 * iW7 * cpa +
+             sinB * y2 * y3ma * irb * iW7 * (T(1) + rcy * iW7 * cpa + a * y3b * irb2);
+  }
+}
 
-  const S ux = -T(2) * c2 * c1 * cot2 * Fib +
-               c1 * y2 * iW6 * ((c1 - aR) * cotB - y1 * iW6 * (nu + aR)) +
-               c1 * cosB * cotB * y2 * iW7 * cpa +
-               a * cotB * y2 * y3ma * irb3 +
-               y2 * y3ma * irb * iW6 * (-c1 * cotB + y1 * iW6 * (nu2 + aR) + a * y1 * irb2) +
-               y2 * y3ma * irb * iW7 * (cosB * iW7 * (rcy * (c1 * cosB - aR) * cotB + T(2) * c2 * cosB * rsy) -
-                                        a * cosB * cotB * y3b * irb2);
-  const S vx = c1 * ((T(2) * c2 * cot2 - nu) * lW6 - (T(2) * c2 * cot2 + c1) * cosB * lW7) -
-               c1 * iW6 * (y1 * cotB * (c1 - aR) + nu * y3b - a + y22 * iW6 * (nu + aR)) -
-               c1 * cotB * z1b * iW7 * cpa -
-               a * cotB * y1 * y3ma * irb3 +
-               q6 * (-nu2 + irb * (c1 * cotB * y1 - a) + y22 * irb * iW6 * (nu2 + aR) + a * y22 * irb3) +
-               q7 * (cosB * cosB - irb * (c1 * cotB * z1b + a * cosB) + a * cotB * y3b * z1b * irb3 -
-                     irb * iW7 * (cosB * cosB * y22 - a * cotB * z1b * irb * rcy));
-  const S wx = T(2) * c2 * (c1 * cotB * Fib + y2 * iW6 * (nu2 + aR) - cosB * y2 * iW7 * cpa) +
-               y2 * y3ma * irb * (nu2 * iW6 + a * irb2) +
-               cosB * y2 * y3ma * irb * iW7 * (c1 - rcy * iW7 * cpa - a * y3b * irb2);
-  const S uy = c1 * ((T(2) * c2 * cot2 + nu) * lW6 - (T(2) * c2 * cot2 + T(1)) * cosB * lW7) +
-               c1 * iW6 * (-c1 * cotB * y1 + nu * y3b - a + a * cotB * y1 * irb + y11 * iW6 * (nu + aR)) -
-               c1 * cotB * iW7 * (cosB * z1b - a * icosB * rsy * irb) -
-               a * cotB * y1 * y3ma * irb3 +
-               q6 * (nu2 + irb * (c1 * cotB * y1 + a) - y11 * irb * iW6 * (nu2 + aR) - a * y11 * irb3) +
-               cotB * q7 * (-cosB * sinB + a * icosB * y1 * y3b * irb3 + rsy * irb * (T(2) * c2 * cosB - rcy * iW7 * p1));
-  const S vy = T(2) * c2 * c1 * cot2 * Fib +
-               c1 * y2 * iW6 * (-(c1 - aR) * cotB + y1 * iW6 * (nu + aR)) -
-               c1 * cotB * y2 * iW7 * p1 -
-               a * cotB * y2 * y3ma * irb3 +
-               y2 * y3ma * irb * iW6 * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb * (irb + iW6)) +
-               cotB * y2 * y3ma * irb * iW7 * (-T(2) * c2 * cosB + rcy * iW7 * p1 + a * icosB * y3b * irb2);
-  const S wy = -T(2) * c2 * c1 * cotB * (lW6 - cosB * lW7) -
-               T(2) * c2 * y1 * iW6 * (nu2 + aR) +
-               T(2) * c2 * z1b * iW7 * cpa +
-               y3ma * irb * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb2) -
-               q7 * (cosB * sinB + cotB * rcy * irb * (T(2) * c2 * cosB - rcy * iW7) +
-                     aR * (sinB - y3b * z1b * irb2 - z1b * rcy * irb * iW7));
-  const S uz = c1 * (y2 * iW6 * (T(1) + aR) - cosB * y2 * iW7 * cpa) -
-               y2 * y3ma * irb * (a * irb2 + iW6) +
-               cosB * y2 * y3ma * irb * iW7 * (rcy * iW7 * cpa + a * y3b * irb2);
-  const S vz = c1 * (-sinB * lW7 - y1 * iW6 * (T(1) + aR) + z1b * iW7 * cpa) +
-               y1 * y3ma * irb * (a * irb2 + iW6) -
-               q7 * (sinB * (cosB - aR) + z1b * irb * (T(1) + a * y3b * irb2) -
-                     irb * iW7 * (cosB * sinB * y22 - a * z1b * irb * rcy));
-  const S wz = T(2) * c2 * Fib + T(2) * c2 * sinB * y2 * iW7 * cpa +
-               sinB * y2 * y3ma * irb * iW7 * (T(1) + rcy * iW7 * cpa + a * y3b * irb2);
-  o[0] = ux;
-  o[1] = uy;
-  o[2] = uz;
-  o[3] = vx;
-  o[4] = vy;
-  o[5] = vz;
-  o[6] = wx;
-  o[7] = wy;
-  o[8] = wz;
+// All nine outputs (order {ux,uy,uz, vx,vy,vz, wx,wy,wz}).
+template <class T, class S>
+__device__ __forceinline__ void fsc_disp(S y1, S y2, S y3, T cosB, T sinB, T cotB, T cotB2, T icosB, T nu, T a, S* o) {
+  FscBase<T, S> f;
+  fsc_base<T, S>(y1, y2, y3, cosB, sinB, cotB, cotB2, icosB, nu, a, f);
+  const int map[9] = {0, 1, 2, 3, 4, 5, 6, 7, 8};
+#pragma unroll
+  for (int q = 0; q < 9; ++q) o[q] = fsc_out<T, S>(f, map[q]);
 }
 
 // ---- per-pair drivers -----------------------------------------------------------------------------------
@@ -512,18 +536,33 @@ __device__ __forceinline__ void fsc_strain_side(const T* __restrict__ s, T X, T 
     const T pz = ad == 0 ? z : z - s[S_YB + 2];
     const T aa = ad == 0 ? s[S_AA] : s[S_AB];
     const T sgn = ad == 0 ? T(-1) : T(1);
-    D3<T> o[9];
-    fsc_disp<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
-                       D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, o);
+    FscBase<T, D3<T>> fb;
+    fsc_base<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
+                       D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
+    // one output at a time: output q = 3 c + k is component c of the displacement for Burgers component k;
+    // its gradient feeds v_cc (diagonal), and half of the two shear sums it appears in
+#pragma unroll 1
+    for (int q = 0; q < 9; ++q) {
+      const D3<T> u = fsc_out<T, D3<T>>(fb, q);
+      const int c = q / 3, k = q % 3;
+      const T g0 = sgn * u.a, g1 = sgn * u.b, g2 = sgn * u.c;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const D3<T> u1 = o[k], u2 = o[3 + k], u3 = o[6 + k];
-      w[k][0] += sgn * u1.a;
-      w[k][1] += sgn * u2.b;
-      w[k][2] += sgn * u3.c;
-      w[k][3] += sgn * (u1.b + u2.a);
-      w[k][4] += sgn * sg * (u1.c + u3.a);
-      w[k][5] += sgn * sg * (u2.c + u3.b);
+      for (int kk = 0; kk < 3; ++kk) {
+        if (kk != k) continue;
+        if (c == 0) {
+          w[kk][0] += g0;
+          w[kk][3] += g1;
+          w[kk][4] += sg * g2;
+        } else if (c == 1) {
+          w[kk][3] += g0;
+          w[kk][1] += g1;
+          w[kk][5] += sg * g2;
+        } else {
+          w[kk][4] += sg * g0;
+          w[kk][5] += sg * g1;
+          w[kk][2] += g2;
+        }
+      }
     }
   }
   const T c0[3] = {s[S_EY1 + 0], s[S_EY1 + 1], T(0)};

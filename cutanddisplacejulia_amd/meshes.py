@@ -148,3 +148,21 @@ def config2(n=50):
     P1, P2, P3 = rect_fault(n, n, 60.0, 45.0, 5.0, 5.0, 0.5)
     X, Y, Z = grid_obs(100, 10.0, 0.0)
     return dict(P1=P1, P2=P2, P3=P3, X=X, Y=Y, Z=Z, nu=0.25, G=1.0, hs=1)
+
+
+def growth_sequence(steps=40, n_start=500, n_end=6400, radius=1.0, dip=45.0, jitter=0.03, seed=BASE_SEED + 4):
+    """Penny crack of the given radius remeshed at `steps` growth steps whose triangle counts follow a geometric
+    schedule from n_start to n_end (ring meshes: 6 r^2 triangles for r rings), interior vertices jittered by
+    U(+-jitter) of the ring spacing, the disc dipping `dip` degrees (strike along y).  Each step also gets a
+    remote-load gradient g ~ U(0, 0.5) for sigma_zz = 1 + g z.  Returns a list of dicts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    d = np.deg2rad(dip)
+    normal = (np.sin(d), 0.0, np.cos(d))
+    for s in range(steps):
+        target = n_start * (n_end / n_start) ** (s / max(steps - 1, 1))
+        rings = max(2, int(round(np.sqrt(target / 6.0))))
+        P1, P2, P3 = penny(rings, radius, seed=seed + 1000 + s, jitter=jitter, center=(0.0, 0.0, -2.0),
+                           normal=normal)
+        out.append(dict(step=s + 1, P1=P1, P2=P2, P3=P3, g=float(rng.uniform(0.0, 0.5))))
+    return out

@@ -36,9 +36,17 @@ extern "C" {
 
 /* precision of the traction-matrix assembly and of the factorisation */
 #define CDJ_PREC_FP64 0       /* FP64 assembly, FP64 LU (the reference's arithmetic) */
-#define CDJ_PREC_MIXED32 1    /* FP32 far-field + FP64 near-field assembly, FP32 LU, FP64 GMRES-IR */
+#define CDJ_PREC_MIXED32 1    /* FP32 LU on MFMA as preconditioner, FP64 GMRES-IR against the FP64 A */
+#define CDJ_PREC_LEARNED 2    /* learned SPAI preconditioner, FP64 FGMRES; FP32-LU path as fallback */
+
+/* preconditioners of cdj_precond_solve_dev */
+#define CDJ_PC_NONE 0
+#define CDJ_PC_JACOBI 1       /* exact inverse of each element's 3x3 self block */
+#define CDJ_PC_LEARNED 2      /* block Jacobi + MLP-predicted neighbour blocks */
+#define CDJ_PC_LU32 3         /* FP32 LU */
 
 typedef struct cdj_ctx cdj_ctx;
+typedef struct cdj_mlp cdj_mlp;
 
 typedef struct cdj_opts {
   int32_t precision;   /* CDJ_PREC_* */
@@ -121,6 +129,35 @@ int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const d
 int cdj_lu_factor_dev(cdj_ctx* ctx, double* A, int64_t n, int64_t lda, int32_t* ipiv, void* stream);
 int cdj_lu_solve_dev(cdj_ctx* ctx, const double* LU, int64_t n, int64_t lda, const int32_t* ipiv, double* B,
                      int64_t nrhs, int64_t ldb, void* stream);
+
+/* ---- learned preconditioner (no counterpart in the reference: the build's learned solve path) ----
+ * A per-triangle MLP over the k nearest triangles' geometry (relative centroids, normals, strike directions,
+ * areas in the triangle's own frame) predicts the 3x3 blocks of a sparse approximate inverse row block;
+ * M = block Jacobi + (h_i / G) MLP(features_i).  Trained on the GPU on (x, b = A x) pairs of assembled
+ * matrices, loss mean ||M b - x||^2. */
+int cdj_mlp_create(cdj_ctx* ctx, int32_t k, int32_t hidden, uint64_t seed, cdj_mlp** out);
+int cdj_mlp_destroy(cdj_mlp* mlp);
+int64_t cdj_mlp_nparams(const cdj_mlp* mlp);
+int cdj_mlp_get_params(cdj_ctx* ctx, const cdj_mlp* mlp, float* params);
+int cdj_mlp_set_params(cdj_ctx* ctx, cdj_mlp* mlp, const float* params);
+
+/* `iters` Adam steps on one mesh (device geometry, device A of 3N x 3N, leading dimension lda) with
+ * `nsamples` random slip vectors; *loss receives the last step's mean loss. */
+int cdj_mlp_train_dev(cdj_ctx* ctx, cdj_mlp* mlp, const double* P1, const double* P2, const double* P3, int64_t n,
+                      double G, const double* A, int64_t lda, int32_t nsamples, int32_t iters, float lr,
+                      uint64_t seed, double* loss, void* stream);
+
+/* Loss and parameter gradient for host inputs (gradient checks): geometry P1..P3 (n x 3), Jacobi blocks jac
+ * (n x 9, row-major 3x3 per element), samples B, X (3n x S).  Also returns the features (n x k*10), neighbour
+ * lists (n x k) and per-element geometry records (n x 16) the GPU used. */
+int cdj_mlp_loss_grad(cdj_ctx* ctx, cdj_mlp* mlp, const double* P1, const double* P2, const double* P3, int64_t n,
+                      double G, const double* jac, const double* B, const double* X, int32_t nsamples, double* loss,
+                      float* grad, float* feat_out, int32_t* nbr_out, float* geo_out);
+
+/* FP64 FGMRES on device A with the chosen preconditioner (CDJ_PC_*) to opts->tol; b, x: 3N vectors. */
+int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const double* P1, const double* P2,
+                          const double* P3, int64_t n, double G, const double* A, int64_t lda, const double* b,
+                          double* x, const cdj_opts* opts, cdj_report* report, void* stream);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_binding_declares_signatures_without_gpu():
     L = _lib.load()
+    from cutanddisplacejulia_amd import learned
+    learned._bind()
     for name in _lib.EXPORTS:
         assert getattr(L, name).argtypes is not None or name == "cdj_last_error"
     assert L.cdj_last_kernel_ms(None) == -1.0
