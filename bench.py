@@ -83,7 +83,7 @@ def main():
     ctx = _lib.ctx()
     stream = torch.cuda.current_stream(dev)
     sp = ctypes.c_void_p(stream.cuda_stream)
-    opts = _lib.Opts(0, 0, 1e-10, 0, 2.0)
+    opts = _lib.Opts(tol=1e-10, near_factor=2.0)
 
     def step():
         rc = L.cdj_assemble_dev(ctx, ctypes.c_void_p(dN.data_ptr()), ctypes.c_void_p(dM.data_ptr()),

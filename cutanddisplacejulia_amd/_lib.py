@@ -28,13 +28,16 @@ CDJ_ERR_SINGULAR = 8
 
 CDJ_PREC_FP64 = 0
 CDJ_PREC_MIXED32 = 1
+CDJ_PREC_LEARNED = 2
+CDJ_ASM_FP64 = 0
+CDJ_ASM_MIXED = 1
 
 # every symbol declared in include/cdj.h
 EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", "cdj_td_influence", "cdj_td_field",
            "cdj_traction_influence", "cdj_slip_calculator", "cdj_solve", "cdj_assemble_dev",
            "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev", "cdj_mlp_create", "cdj_mlp_destroy",
            "cdj_mlp_nparams", "cdj_mlp_get_params", "cdj_mlp_set_params", "cdj_mlp_train_dev", "cdj_mlp_loss_grad",
-           "cdj_precond_solve_dev"]
+           "cdj_precond_solve_dev", "cdj_last_near_entries"]
 
 
 class CdjError(RuntimeError):
@@ -44,7 +47,8 @@ class CdjError(RuntimeError):
 
 
 class Opts(ctypes.Structure):
-    _fields_ = [("precision", _i32), ("tile", _i32), ("tol", _d), ("max_iter", _i32), ("near_factor", _d)]
+    _fields_ = [("precision", _i32), ("tile", _i32), ("tol", _d), ("max_iter", _i32), ("assembly", _i32),
+                ("near_factor", _d)]
 
 
 class Report(ctypes.Structure):
@@ -73,6 +77,7 @@ def load(path=LIB_PATH):
     L.cdj_last_error.argtypes = [_p]
     L.cdj_last_error.restype = ctypes.c_char_p
     L.cdj_last_kernel_ms.argtypes = [_p]
+    L.cdj_last_near_entries.argtypes = [_p]
     td_args = [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _d, _d, _i32, _i32, _i32, _p, _p]
     L.cdj_td_influence.argtypes = td_args
     L.cdj_td_field.argtypes = td_args
@@ -87,7 +92,7 @@ def load(path=LIB_PATH):
         f = getattr(L, name)
         if name == "cdj_last_kernel_ms":
             f.restype = ctypes.c_double
-        elif name == "cdj_mlp_nparams":
+        elif name in ("cdj_mlp_nparams", "cdj_last_near_entries"):
             f.restype = ctypes.c_int64
         elif name != "cdj_last_error":
             f.restype = ctypes.c_int

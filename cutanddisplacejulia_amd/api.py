@@ -117,19 +117,31 @@ DISP_FIELDS = [f.name for f in fields(DispInf)]
 TRACTION_FIELDS = [f.name for f in fields(TractionInf)]
 
 # Module-level solver settings (the reference's signatures have no room for them).
-_SETTINGS = {"precision": _lib.CDJ_PREC_FP64, "tile": 0, "tol": 1e-10, "max_iter": 0, "near_factor": 2.0}
+_SETTINGS = {"precision": _lib.CDJ_PREC_FP64, "assembly": _lib.CDJ_ASM_FP64, "tile": 0, "tol": 1e-10,
+             "max_iter": 0, "near_factor": 2.0}
 LAST_REPORT = {}
 
+_MODES = {
+    # the reference's arithmetic: FP64 assembly, FP64 LU
+    "fp64": (_lib.CDJ_PREC_FP64, _lib.CDJ_ASM_FP64),
+    # FP64 assembly; FP32 LU on MFMA as preconditioner of FP64 GMRES-IR against the FP64 matrix
+    "mixed32": (_lib.CDJ_PREC_MIXED32, _lib.CDJ_ASM_FP64),
+    # FP32 far field / FP64 near field assembly, then as "mixed32"; the residual is against that matrix
+    "mixed": (_lib.CDJ_PREC_MIXED32, _lib.CDJ_ASM_MIXED),
+}
 
-def set_precision(mode):
-    """'fp64' (default, the reference's arithmetic) or 'mixed32' (FP32 far-field + FP64 near-field assembly,
-    FP32 LU, FP64 GMRES iterative refinement)."""
-    _SETTINGS["precision"] = {"fp64": _lib.CDJ_PREC_FP64, "mixed32": _lib.CDJ_PREC_MIXED32}[mode]
+
+def set_precision(mode, near_factor=None):
+    """Select the arithmetic of CalculateInfluenceMatrices3D / SlipCalculator3D: 'fp64' (default), 'mixed32' or
+    'mixed' (see _MODES).  near_factor: FP64 radius in longest triangle edges about each source centroid."""
+    _SETTINGS["precision"], _SETTINGS["assembly"] = _MODES[mode]
+    if near_factor is not None:
+        _SETTINGS["near_factor"] = float(near_factor)
 
 
 def _opts():
-    return Opts(_SETTINGS["precision"], _SETTINGS["tile"], _SETTINGS["tol"], _SETTINGS["max_iter"],
-                _SETTINGS["near_factor"])
+    return Opts(precision=_SETTINGS["precision"], tile=_SETTINGS["tile"], tol=_SETTINGS["tol"],
+                max_iter=_SETTINGS["max_iter"], assembly=_SETTINGS["assembly"], near_factor=_SETTINGS["near_factor"])
 
 
 def _tri(P):

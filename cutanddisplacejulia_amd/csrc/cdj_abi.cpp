@@ -30,6 +30,8 @@ struct cdj_ctx {
   // HIP events around the last assembly / TD kernel launch, on the stream it ran on
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  // entries recomputed in FP64 by the last mixed assembly
+  int64_t near_entries = 0;
 };
 
 namespace {
@@ -111,6 +113,31 @@ int check_hs_dev(cdj_ctx* ctx, const double* Z, int64_t nobs, const double* P1, 
   return CDJ_OK;
 }
 
+// FP32-far / FP64-near pass pair into `out` (the assembly's mixed mode).  rec: FP64 records (built), rec32
+// (RS * n floats), near (4 n doubles) and cnt (one counter) are device scratch of the caller.
+hipError_t traction_mixed(const double* recv, const double* rec, float* rec32, double* near,
+                          unsigned long long* cnt, const double* P1, const double* P2, const double* P3, int64_t n,
+                          int hs, int tpb, double nu, double lambda, double G, double near_factor,
+                          const OutPtrs<double>& out, hipEvent_t ev0, hipEvent_t ev1, hipStream_t st) {
+  const int RS = hs ? REC_HS : REC_FS;
+  launch_rec_to_f32(rec, rec32, (int64_t)RS * n, st);
+  launch_near_info(P1, P2, P3, n, near_factor, near, st);
+  hipError_t e = hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st);
+  if (e != hipSuccess) return e;
+  if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
+  launch_traction_f32_to_f64(recv, n, rec32, n, hs, hs ? 8 : 32, (float)nu, (float)lambda, (float)G, out, st);
+  launch_nearfield(recv, n, rec, near, n, hs, tpb, nu, lambda, G, out, cnt, st);
+  if (ev1 && (e = hipEventRecord(ev1, st)) != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+int assembly_mode(cdj_ctx* ctx, const cdj_opts* opts, bool* mixed, double* near_factor) {
+  *mixed = opts && opts->assembly == CDJ_ASM_MIXED;
+  if (opts && opts->assembly != CDJ_ASM_FP64 && !*mixed) return fail(ctx, CDJ_ERR_ARG, "unknown assembly mode");
+  *near_factor = (opts && opts->near_factor > 0) ? opts->near_factor : 2.0;
+  return CDJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -169,6 +196,8 @@ double cdj_last_kernel_ms(cdj_ctx* ctx) {
   if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) return -1.0;
   return (double)ms;
 }
+
+int64_t cdj_last_near_entries(const cdj_ctx* ctx) { return ctx ? ctx->near_entries : -1; }
 
 int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
                          const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
@@ -334,10 +363,20 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
   if (n == 0) return CDJ_OK;
   const int hs = hs_flag == 1;
   const int RS = hs ? REC_HS : REC_FS;
+  bool mixed = false;
+  double near_factor = 2.0;
+  if (int rc = assembly_mode(ctx, opts, &mixed, &near_factor)) return rc;
+  // scratch: receivers | FP64 records | (mixed) FP32 records, near-field table, counter
+  const size_t base = sizeof(double) * (RECV + RS) * n;
+  const size_t extra = mixed ? sizeof(float) * RS * n + sizeof(double) * 4 * n + 64 : 0;
   void* scr = nullptr;
-  HIPCHK(ctx_scratch(ctx, sizeof(double) * (RECV + RS) * n, &scr));
+  HIPCHK(ctx_scratch(ctx, base + extra + 256, &scr));
   double* recv_p = static_cast<double*>(scr);
   double* rec_p = recv_p + RECV * n;
+  float* rec32 = reinterpret_cast<float*>(rec_p + RS * n);
+  double* near_p = reinterpret_cast<double*>(reinterpret_cast<char*>(scr) +
+                                             ((base + sizeof(float) * RS * n + 255) / 256) * 256);
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(near_p + 4 * n);
   launch_receivers(normals, midpoints, n, recv_p, st);
   if (hs) {
     // collocation z-coordinates live in recv (stride RECV); check vertices and receivers on the host copy
@@ -359,12 +398,24 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
   for (int m = 0; m < 3; ++m)
     for (int q = 0; q < 3; ++q) out.p[3 * m + q] = A + (int64_t)q * n + (int64_t)m * n * lda;
   const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, n);
-  HIPCHK(hipEventRecord(ctx->ev0, st));
-  launch_traction<double>(recv_p, n, rec_p, n, hs, tpb, nu, lambda, G, out, st);
-  HIPCHK(hipEventRecord(ctx->ev1, st));
+  if (!mixed) {
+    HIPCHK(hipEventRecord(ctx->ev0, st));
+    launch_traction<double>(recv_p, n, rec_p, n, hs, tpb, nu, lambda, G, out, st);
+    HIPCHK(hipEventRecord(ctx->ev1, st));
+    ctx->timed = true;
+    ctx->near_entries = 0;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return CDJ_OK;
+  }
+  // the timed region covers both passes (the mixed assembly's cost)
+  HIPCHK(traction_mixed(recv_p, rec_p, rec32, near_p, cnt, P1, P2, P3, n, hs, tpb, nu, lambda, G, near_factor, out,
+                        ctx->ev0, ctx->ev1, st));
   ctx->timed = true;
-  HIPCHK(hipGetLastError());
+  unsigned long long h = 0;
+  HIPCHK(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  ctx->near_entries = (int64_t)h;
   return CDJ_OK;
 }
 
@@ -378,6 +429,9 @@ int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* mi
       if (fixed[i] == 1) return fail(ctx, CDJ_ERR_ARG, "fixed elements are not supported by this entry point");
   for (int k = 0; k < 9; ++k)
     if (!T || !T[k]) return fail(ctx, CDJ_ERR_ARG, "traction output pointer is NULL");
+  bool mixed = false;
+  double near_factor = 2.0;
+  if (int rc = assembly_mode(ctx, opts, &mixed, &near_factor)) return rc;
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   if (n == 0) return CDJ_OK;
@@ -414,7 +468,22 @@ int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* mi
   out.sign = 1.0;
   for (int k = 0; k < 9; ++k) out.p[k] = dT.as<double>() + (int64_t)k * n * n;
   const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, n);
-  launch_traction<double>(recv.as<double>(), n, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+  if (mixed) {
+    DBuf r32, nr, cnt;
+    HIPCHK(r32.alloc(sizeof(float) * RS * n));
+    HIPCHK(nr.alloc(sizeof(double) * 4 * n));
+    HIPCHK(cnt.alloc(sizeof(unsigned long long)));
+    HIPCHK(traction_mixed(recv.as<double>(), rec.as<double>(), r32.as<float>(), nr.as<double>(),
+                          cnt.as<unsigned long long>(), d1.as<double>(), d2.as<double>(), d3.as<double>(), n, hs, tpb,
+                          nu, lambda, G, near_factor, out, nullptr, nullptr, st));
+    unsigned long long h = 0;
+    HIPCHK(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    ctx->near_entries = (int64_t)h;
+  } else {
+    launch_traction<double>(recv.as<double>(), n, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+    ctx->near_entries = 0;
+  }
   HIPCHK(hipGetLastError());
   for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(T[k], out.p[k], vm, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -672,14 +741,20 @@ int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const 
     rc = solve_mixed32(ctx, AB.as<double>(), N3, lda, AB.as<double>() + N3 * lda, dX.as<double>(), nrhs, opts, report,
                        st);
     if (rc) return rc;
-    if (report) report->assembly_ms = ams;
+    if (report) {
+      report->assembly_ms = ams;
+      report->near_entries = ctx->near_entries;
+    }
     HIPCHK(hipMemcpyAsync(x, dX.p, sizeof(double) * N3 * nrhs, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     return CDJ_OK;
   }
   rc = solve_augmented_fp64(ctx, AB.as<double>(), N3, lda, nrhs, Ao.as<double>(), Bo.as<double>(), report, st);
   if (rc) return rc;
-  if (report) report->assembly_ms = ams;
+  if (report) {
+    report->assembly_ms = ams;
+    report->near_entries = ctx->near_entries;
+  }
   HIPCHK(hipMemcpyAsync(x, AB.as<double>() + N3 * lda, sizeof(double) * N3 * nrhs, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return CDJ_OK;
@@ -848,7 +923,7 @@ int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const do
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   const int64_t N3 = 3 * n;
   if (pc == CDJ_PC_LU32) {
-    cdj_opts o = opts ? *opts : cdj_opts{CDJ_PREC_MIXED32, 0, 1e-10, 0, 2.0};
+    cdj_opts o = opts ? *opts : cdj_opts{CDJ_PREC_MIXED32, 0, 1e-10, 0, CDJ_ASM_FP64, 2.0};
     return solve_mixed32(ctx, A, N3, lda, b, x, 1, &o, report, st);
   }
   GmresOpts go;
@@ -902,7 +977,7 @@ int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const do
   }
   if (!gr.converged) {
     // fallback: FP32-LU preconditioned refinement (itself falling back to the FP64 LU)
-    cdj_opts o = opts ? *opts : cdj_opts{CDJ_PREC_MIXED32, 0, 1e-10, 0, 2.0};
+    cdj_opts o = opts ? *opts : cdj_opts{CDJ_PREC_MIXED32, 0, 1e-10, 0, CDJ_ASM_FP64, 2.0};
     o.max_iter = 0;
     cdj_report r2{};
     int rc = solve_mixed32(ctx, A, N3, lda, b, x, 1, &o, &r2, st);

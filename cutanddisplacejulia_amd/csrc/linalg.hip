@@ -33,14 +33,19 @@ constexpr int MAX_RPB = 128;   // max panel rows per workgroup (LDS: 128 x 64 x 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA: A operand lane l = A[l % 16][l / 16], B operand lane l = B[l / 16][l % 16].  The accumulator
+// layouts differ: f32 item v of lane l is D[4 (l / 16) + v][l % 16]; f64 item v of lane l is D[4 v + l / 16][l % 16].
+// row(g, v) gives the D row of item v for lane group g = l / 16.
 template <class T> struct Mfma;
 template <> struct Mfma<double> {
   typedef d4 acc;
   __device__ static acc step(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int g, int v) { return 4 * v + g; }
 };
 template <> struct Mfma<float> {
   typedef f4 acc;
   __device__ static acc step(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int g, int v) { return 4 * g + v; }
 };
 
 template <class T>
@@ -232,7 +237,7 @@ __global__ void __launch_bounds__(256) gemm_update_kernel(const T* __restrict__ 
     acc[1][0] = Mfma<T>::step(u1, l0, acc[1][0]);
     acc[1][1] = Mfma<T>::step(u1, l1, acc[1][1]);
   }
-  // acc[tj][ti][v] = (L U)[row = wr + 16 ti + li, col = wc + 16 tj + 4 lk + v]
+  // acc[tj][ti][v] = (L U)[row = wr + 16 ti + li, col = wc + 16 tj + Mfma::row(lk, v)]
 #pragma unroll
   for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
@@ -240,7 +245,7 @@ __global__ void __launch_bounds__(256) gemm_update_kernel(const T* __restrict__ 
       const int64_t row = i0 + wr + 16 * ti + li;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t col = j0 + wc + 16 * tj + 4 * lk + v;
+        const int64_t col = j0 + wc + 16 * tj + Mfma<T>::row(lk, v);
         if (row < M && col < N) C[row + col * lda] -= acc[tj][ti][v];
       }
     }

@@ -339,10 +339,10 @@ __device__ __forceinline__ void traction3(const T* e, T lam, T G, const T* n, co
 
 // out.p[3*m + q] is the block of mode m (Dn, Dss, Dds) and traction q (Tn, Tss, Tds); out.sign = -1 writes
 // A = -[...] (SlipCalculator3D.jl:13-16).  Block pointers are already offset, leading dimension out.ld.
-template <class T, bool HS>
+template <class T, class TO, bool HS>
 __global__ void __launch_bounds__(256) tdk_traction(const double* __restrict__ recv, int64_t nrecv,
                                                     const T* __restrict__ rec, int64_t ntri, int tpb, T nu, T lam,
-                                                    T G, OutPtrs<T> out) {
+                                                    T G, OutPtrs<TO> out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nrecv) return;
   const double* rv = recv + i * RECV;
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(256) tdk_traction(const double* __restrict__ r
   const int64_t j0 = (int64_t)blockIdx.y * tpb;
   const int64_t j1 = j0 + tpb < ntri ? j0 + tpb : ntri;
   constexpr int RS = HS ? REC_HS : REC_FS;
-  const T sg = out.sign;
+  const T sg = (T)out.sign;
   for (int64_t j = j0; j < j1; ++j) {
     const T* r = rec + j * RS;
     T e[3][6];
@@ -363,11 +363,76 @@ __global__ void __launch_bounds__(256) tdk_traction(const double* __restrict__ r
     for (int m = 0; m < 3; ++m) {
       T tn, tss, tds;
       traction3(e[m], lam, G, n, s, d, tn, tss, tds);
-      out.p[3 * m + 0][o] = sg * tn;
-      out.p[3 * m + 1][o] = sg * tss;
-      out.p[3 * m + 2][o] = sg * tds;
+      out.p[3 * m + 0][o] = (TO)(sg * tn);
+      out.p[3 * m + 1][o] = (TO)(sg * tss);
+      out.p[3 * m + 2][o] = (TO)(sg * tds);
     }
   }
+}
+
+// Near-field pass of the mixed assembly: pairs whose collocation point lies within the source triangle's
+// near radius (near[j] = {centroid, radius^2}) are recomputed in FP64 and overwrite the FP32 values.  A wave
+// with no near lane for a source skips it, so the FP64 work is proportional to the near pairs' waves.
+template <bool HS>
+__global__ void __launch_bounds__(256) tdk_nearfield(const double* __restrict__ recv, int64_t nrecv,
+                                                     const double* __restrict__ rec, const double* __restrict__ near,
+                                                     int64_t ntri, int tpb, double nu, double lam, double G,
+                                                     OutPtrs<double> out, unsigned long long* __restrict__ count) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool valid = i < nrecv;
+  const double* rv = recv + (valid ? i : 0) * RECV;
+  const double Xi = rv[0], Yi = rv[1], Zi = rv[2];
+  const double n[3] = {rv[3], rv[4], rv[5]};
+  const double s[3] = {rv[6], rv[7], rv[8]};
+  const double d[3] = {rv[9], rv[10], rv[11]};
+  const int64_t j0 = (int64_t)blockIdx.y * tpb;
+  const int64_t j1 = j0 + tpb < ntri ? j0 + tpb : ntri;
+  constexpr int RS = HS ? REC_HS : REC_FS;
+  unsigned long long cnt = 0;
+  for (int64_t j = j0; j < j1; ++j) {
+    const double* nj = near + j * 4;
+    const double dx = Xi - nj[0], dy = Yi - nj[1], dz = Zi - nj[2];
+    const bool nr = valid && (dx * dx + dy * dy + dz * dz) < nj[3];
+    if (!__any(nr)) continue;
+    double e[3][6];
+    strain_pair<double, HS>(rec + j * RS, Xi, Yi, Zi, nu, e);
+    if (nr) {
+      const int64_t o = i + j * out.ld;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        double tn, tss, tds;
+        traction3(e[m], lam, G, n, s, d, tn, tss, tds);
+        out.p[3 * m + 0][o] = out.sign * tn;
+        out.p[3 * m + 1][o] = out.sign * tss;
+        out.p[3 * m + 2][o] = out.sign * tds;
+      }
+      ++cnt;
+    }
+  }
+  if (cnt) atomicAdd(count, cnt * 9ull);
+}
+
+// near[j] = {centroid x, y, z, (near_factor * longest edge)^2}
+__global__ void tdk_near_info(const double* __restrict__ P1, const double* __restrict__ P2,
+                              const double* __restrict__ P3, int64_t ntri, double factor, double* __restrict__ near) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= ntri) return;
+  double a[3], b[3], c[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = P1[k * ntri + j];
+    b[k] = P2[k * ntri + j];
+    c[k] = P3[k * ntri + j];
+  }
+  double l = 0.0;
+  const double* pa[3] = {a, b, c};
+  const double* pb[3] = {b, c, a};
+  for (int e = 0; e < 3; ++e) {
+    double s2 = 0.0;
+    for (int k = 0; k < 3; ++k) s2 += (pb[e][k] - pa[e][k]) * (pb[e][k] - pa[e][k]);
+    l = s2 > l ? s2 : l;
+  }
+  for (int k = 0; k < 3; ++k) near[j * 4 + k] = (a[k] + b[k] + c[k]) / 3.0;
+  near[j * 4 + 3] = factor * factor * l;
 }
 
 // ---- launch helpers --------------------------------------------------------------------------------------
@@ -436,9 +501,41 @@ void launch_traction(const double* recv, int64_t nrecv, const T* rec, int64_t nt
   if (nrecv <= 0 || ntri <= 0) return;
   dim3 grid((unsigned)((nrecv + 255) / 256), (unsigned)((ntri + tpb - 1) / tpb));
   if (hs)
-    hipLaunchKernelGGL((tdk_traction<T, true>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu, lam, G, out);
+    hipLaunchKernelGGL((tdk_traction<T, T, true>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu, lam, G, out);
   else
-    hipLaunchKernelGGL((tdk_traction<T, false>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu, lam, G, out);
+    hipLaunchKernelGGL((tdk_traction<T, T, false>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu, lam, G, out);
+}
+
+void launch_traction_f32_to_f64(const double* recv, int64_t nrecv, const float* rec, int64_t ntri, int hs, int tpb,
+                                float nu, float lam, float G, const OutPtrs<double>& out, hipStream_t st) {
+  if (nrecv <= 0 || ntri <= 0) return;
+  dim3 grid((unsigned)((nrecv + 255) / 256), (unsigned)((ntri + tpb - 1) / tpb));
+  if (hs)
+    hipLaunchKernelGGL((tdk_traction<float, double, true>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu,
+                       lam, G, out);
+  else
+    hipLaunchKernelGGL((tdk_traction<float, double, false>), grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu,
+                       lam, G, out);
+}
+
+void launch_near_info(const double* P1, const double* P2, const double* P3, int64_t ntri, double factor, double* near,
+                      hipStream_t st) {
+  if (ntri <= 0) return;
+  hipLaunchKernelGGL(tdk_near_info, dim3((unsigned)((ntri + 127) / 128)), dim3(128), 0, st, P1, P2, P3, ntri, factor,
+                     near);
+}
+
+void launch_nearfield(const double* recv, int64_t nrecv, const double* rec, const double* near, int64_t ntri, int hs,
+                      int tpb, double nu, double lam, double G, const OutPtrs<double>& out, unsigned long long* count,
+                      hipStream_t st) {
+  if (nrecv <= 0 || ntri <= 0) return;
+  dim3 grid((unsigned)((nrecv + 255) / 256), (unsigned)((ntri + tpb - 1) / tpb));
+  if (hs)
+    hipLaunchKernelGGL(tdk_nearfield<true>, grid, dim3(256), 0, st, recv, nrecv, rec, near, ntri, tpb, nu, lam, G, out,
+                       count);
+  else
+    hipLaunchKernelGGL(tdk_nearfield<false>, grid, dim3(256), 0, st, recv, nrecv, rec, near, ntri, tpb, nu, lam, G,
+                       out, count);
 }
 
 template void launch_influence<double>(const double*, const double*, const double*, int64_t, const double*, int64_t,

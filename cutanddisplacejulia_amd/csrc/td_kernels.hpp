@@ -39,5 +39,15 @@ void launch_field(const T* X, const T* Y, const T* Z, int64_t nobs, const T* rec
 template <class T>
 void launch_traction(const double* recv, int64_t nrecv, const T* rec, int64_t ntri, int hs, int tpb, T nu, T lam, T G,
                      const OutPtrs<T>& out, hipStream_t st);
+// FP32 arithmetic, FP64 storage (far field of the mixed assembly)
+void launch_traction_f32_to_f64(const double* recv, int64_t nrecv, const float* rec, int64_t ntri, int hs, int tpb,
+                                float nu, float lam, float G, const OutPtrs<double>& out, hipStream_t st);
+// near[j*4] = {centroid, (factor * longest edge)^2}
+void launch_near_info(const double* P1, const double* P2, const double* P3, int64_t ntri, double factor, double* near,
+                      hipStream_t st);
+// FP64 recompute of near pairs over the FP32 values; *count accumulates the number of entries replaced
+void launch_nearfield(const double* recv, int64_t nrecv, const double* rec, const double* near, int64_t ntri, int hs,
+                      int tpb, double nu, double lam, double G, const OutPtrs<double>& out, unsigned long long* count,
+                      hipStream_t st);
 
 }  // namespace cdj

@@ -119,16 +119,17 @@ class LearnedPreconditioner:
 def precond_solve_dev(dA, lda, db, dx, dP1, dP2, dP3, n, G, pc="learned", model=None, tol=1e-10, max_iter=0):
     """FP64 FGMRES on device A (3n x 3n) with the named preconditioner; returns the report dict."""
     rep = Report()
-    opts = Opts(_lib.CDJ_PREC_LEARNED, 0, float(tol), int(max_iter), 2.0)
+    opts = Opts(precision=_lib.CDJ_PREC_LEARNED, tol=float(tol), max_iter=int(max_iter), near_factor=2.0)
     check(_bind().cdj_precond_solve_dev(ctx(), model.h if model is not None else None, PC[pc], _dptr(dP1),
                                         _dptr(dP2), _dptr(dP3), int(n), float(G), _dptr(dA), int(lda), _dptr(db),
                                         _dptr(dx), ctypes.byref(opts), ctypes.byref(rep), _stream()))
     return rep.as_dict()
 
 
-def assemble_dev(dN, dM, dP1, dP2, dP3, n, nu, G, lam, hs, dA, lda, tile=0):
+def assemble_dev(dN, dM, dP1, dP2, dP3, n, nu, G, lam, hs, dA, lda, tile=0, mixed=False, near_factor=2.0):
     """Assemble A (3n x 3n, column-major, leading dimension lda) into the device tensor dA."""
-    opts = Opts(0, int(tile), 1e-10, 0, 2.0)
+    opts = Opts(tile=int(tile), tol=1e-10, assembly=_lib.CDJ_ASM_MIXED if mixed else _lib.CDJ_ASM_FP64,
+                near_factor=float(near_factor))
     check(load().cdj_assemble_dev(ctx(), _dptr(dN), _dptr(dM), _dptr(dP1), _dptr(dP2), _dptr(dP3), int(n),
                                   float(nu), float(G), float(lam), int(hs), _dptr(dA), int(lda), ctypes.byref(opts),
                                   _stream()))

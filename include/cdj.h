@@ -39,6 +39,11 @@ extern "C" {
 #define CDJ_PREC_MIXED32 1    /* FP32 LU on MFMA as preconditioner, FP64 GMRES-IR against the FP64 A */
 #define CDJ_PREC_LEARNED 2    /* learned SPAI preconditioner, FP64 FGMRES; FP32-LU path as fallback */
 
+/* arithmetic of the assembly (cdj_opts.assembly) */
+#define CDJ_ASM_FP64 0        /* every entry in FP64 (the reference's arithmetic) */
+#define CDJ_ASM_MIXED 1       /* FP32 far field, FP64 near field (collocation point within near_factor x the
+                                 source's longest edge of its centroid, which includes every self entry) */
+
 /* preconditioners of cdj_precond_solve_dev */
 #define CDJ_PC_NONE 0
 #define CDJ_PC_JACOBI 1       /* exact inverse of each element's 3x3 self block */
@@ -53,7 +58,8 @@ typedef struct cdj_opts {
   int32_t tile;        /* triangles per workgroup column in the TD kernels; 0 = library default */
   double tol;          /* target FP64 relative residual ||b - A x|| / ||b|| for refined solves; 0 = 1e-10 */
   int32_t max_iter;    /* iteration cap of the refinement; 0 = default */
-  double near_factor;  /* near-field radius in triangle edge lengths (FP64 recompute); 0 = 2.0 */
+  int32_t assembly;    /* CDJ_ASM_*: arithmetic of the traction-matrix assembly */
+  double near_factor;  /* near-field radius in longest triangle edges about the centroid (FP64 recompute); 0 = 2.0 */
 } cdj_opts;
 
 typedef struct cdj_report {
@@ -73,6 +79,10 @@ const char* cdj_last_error(const cdj_ctx* ctx);
 /* Device time (ms, HIP events on the launch stream) of the last assembly kernel run on this context; -1 if
  * none.  Used by benchmarks to measure the kernel alone. */
 double cdj_last_kernel_ms(cdj_ctx* ctx);
+
+/* Entries of the last traction assembly on this context that the mixed mode (CDJ_ASM_MIXED) recomputed in FP64
+ * (0 after an FP64 assembly).  No counterpart in the reference (its assembly is FP64 throughout). */
+int64_t cdj_last_near_entries(const cdj_ctx* ctx);
 
 /* TD with influence matrices: replaces TD(X,Y,Z,P1,P2,P3,Dss,Dds,Dn,nu,G,DispFlag,StrainFlag,HSflag[,StrainInf,
  * DispInf]) (TD.jl:77-133, 135-216).  strain: 18 nobs x ntri matrices in StrainInf field order (exxDn, eyyDn,

@@ -29,6 +29,22 @@ def _cmp(g, o, tol=TOL):
     return err / max(scale, 1e-300)
 
 
+def _cmp_group(gs, os_, tol=TOL):
+    """Normwise over a group of fields (the six strain components, or the three displacement components): on the
+    free surface the traction-free components exz, eyz vanish to rounding, so their own maximum is no scale."""
+    scale = max(np.max(np.abs(np.asarray(o)[~np.isnan(o)]), initial=0.0) for o in os_)
+    worst = 0.0
+    for g, o in zip(gs, os_):
+        g, o = np.asarray(g), np.asarray(o)
+        assert g.shape == o.shape
+        assert np.array_equal(np.isnan(g), np.isnan(o))
+        m = ~np.isnan(o)
+        err = np.max(np.abs(g[m] - o[m])) if m.any() else 0.0
+        assert err <= tol * max(scale, 1e-300), (err, scale, err / max(scale, 1e-300))
+        worst = max(worst, err / max(scale, 1e-300))
+    return worst
+
+
 def _small_fault(n=6):
     P1, P2, P3 = meshes.rect_fault(n, n, 60.0, 45.0, 5.0, 5.0, 0.5)
     return P1, P2, P3
@@ -96,8 +112,8 @@ def test_td_field_okada():
         assert np.max(np.abs(a - b)) < 1e-13
     so, do = O.td_field(X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F"), P1, P2, P3, [Dss] * 2,
                         [Dds] * 2, [Dn] * 2, 0.25, 1.0, 1, 1, 1)
-    for a, b in zip(mine, do + so):
-        _cmp(a.ravel(order="F"), b)
+    _cmp_group([a.ravel(order="F") for a in mine[:3]], do)
+    _cmp_group([a.ravel(order="F") for a in mine[3:]], so)
 
 
 def test_td_field_deterministic():
@@ -110,10 +126,8 @@ def test_td_field_deterministic():
     for a, b in zip([r1[0].εxx, r1[0].εyz, r1[1].Uz], [r2[0].εxx, r2[0].εyz, r2[1].Uz]):
         assert np.array_equal(a, b)
     so, do = O.td_field(X, Y, Z, P1, P2, P3, ones, ones, ones, 0.25, 1.0, 1, 1, 1)
-    for a, b in zip([r1[1].Ux, r1[1].Uy, r1[1].Uz], do):
-        _cmp(a, b)
-    for a, b in zip([r1[0].εxx, r1[0].εyy, r1[0].εzz, r1[0].εxy, r1[0].εxz, r1[0].εyz], so):
-        _cmp(a, b)
+    _cmp_group([r1[1].Ux, r1[1].Uy, r1[1].Uz], do)
+    _cmp_group([r1[0].εxx, r1[0].εyy, r1[0].εzz, r1[0].εxy, r1[0].εxz, r1[0].εyz], so)
 
 
 @pytest.mark.parametrize("hs", [0, 1])
