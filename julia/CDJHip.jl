@@ -2,9 +2,10 @@
 #
 # Drop-in replacements for TD (src/TD.jl:77, :135, :219), CalculateInfluenceMatrices3D
 # (src/CalculateInfluenceMatrices3D.jl:2) and SlipCalculator3D (src/SlipCalculator3D.jl:1) with the same
-# signatures.  Include this file after CutAndDisplaceJulia and call CDJHip.install!() to route the package's
-# calls to the GPU, or call CDJHip.TD(...) etc. directly.  The precision mode lives in a module-level setting
-# (CDJHip.PRECISION[]), so the public signatures stay unchanged.
+# signatures.  Include this file after CutAndDisplaceJulia and call CDJHip.TD(...),
+# CDJHip.CalculateInfluenceMatrices3D(...) and CDJHip.SlipCalculator3D(...) where the package's functions were
+# called.  The precision mode lives in module-level settings (CDJHip.PRECISION[], CDJHip.ASSEMBLY[]), so the
+# public signatures stay unchanged.
 module CDJHip
 
 using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, SetupTractionVector, KnockOutFixedRows
@@ -12,6 +13,8 @@ using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, Se
 const LIB = get(ENV, "CDJ_LIB", joinpath(@__DIR__, "..", "cutanddisplacejulia_amd", "libcdjhip.so"))
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)
 const PRECISION = Ref{Int32}(0)        # 0 = FP64 (reference arithmetic), 1 = FP32 LU + FP64 GMRES-IR
+const ASSEMBLY = Ref{Int32}(0)         # 0 = FP64 assembly, 1 = FP32 far field / FP64 near field
+const NEAR_FACTOR = Ref{Float64}(2.0)  # FP64 radius in longest triangle edges about each source centroid
 const LOCK = ReentrantLock()
 
 struct Opts
@@ -19,8 +22,10 @@ struct Opts
     tile::Int32
     tol::Float64
     max_iter::Int32
+    assembly::Int32
     near_factor::Float64
 end
+Opts() = Opts(PRECISION[], 0, 1e-10, 0, ASSEMBLY[], NEAR_FACTOR[])
 
 mutable struct Report
     path::Int32
@@ -100,7 +105,7 @@ function CalculateInfluenceMatrices3D(FaceNormalVector, MidPoint, P1, P2, P3, ν
     any(FixedEls .== 1) && error("fixed elements: use the reference implementation")
     T = [zeros(n, n) for _ in 1:9]
     tp = [pointer(a) for a in T]
-    opts = Ref(Opts(PRECISION[], 0, 1e-10, 0, 2.0))
+    opts = Ref(Opts())
     nrm = _f64(FaceNormalVector); mid = _f64(MidPoint)
     lock(LOCK) do
         GC.@preserve T tp nrm mid begin
@@ -120,7 +125,7 @@ function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSF
     any(FixedEls .== 1) && error("fixed elements: use the reference implementation")
     b = [Tn; Tss; Tds]
     x = zeros(3n); A = zeros(3n, 3n)
-    opts = Ref(Opts(PRECISION[], 0, 1e-10, 0, 2.0)); rep = Ref(Report())
+    opts = Ref(Opts()); rep = Ref(Report())
     lock(LOCK) do
         check(ccall((:cdj_slip_calculator, LIB), Cint,
             (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Float64, Ptr{Float64},
