@@ -48,13 +48,43 @@ def cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads):
     return 9.0 * rows * n / dt, dt
 
 
+def slip_solve_comparison(step_no):
+    """Slip solve of one penny-crack growth step (full space, remote sigma_zz = 1 + g z) to an FP64 relative
+    residual of 1e-10: the plain FP64 path (assembly + LU) against mixed assembly + learned-preconditioned FGMRES
+    (trained network from cutanddisplacejulia_amd/data; trained here when absent) and the FP32-LU refinement."""
+    from cutanddisplacejulia_amd import meshes, spai
+    seq = meshes.growth_sequence(40, 500, 6400)
+    if os.path.exists(spai.DEFAULT_PARAMS):
+        model, trained = spai.load_model(), "loaded"
+    else:
+        model, _ = spai.train(seq[:30], epochs=2, log=_log)
+        trained = "trained in bench (2 epochs)"
+    step = seq[step_no - 1]
+    spai.evaluate_step(seq[0], model)  # warm-up (allocations, code objects)
+    r = spai.evaluate_step(step, model)
+    plain = r["fp64"]["total_ms"]
+    best = min((k for k in r if k.startswith("mixed_") and isinstance(r[k], dict) and r[k]["residual"] <= 1e-10),
+               key=lambda k: r[k]["total_ms"], default=None)
+    _log("slip solve step %d (%d triangles): FP64 assembly+LU %.1f ms; mixed+learned %.1f ms (%d it); "
+         "mixed+LU32 %.1f ms" % (step_no, r["n_tri"], plain, r.get("mixed_learned", {}).get("total_ms", float("nan")),
+                                 r.get("mixed_learned", {}).get("iterations", -1),
+                                 r.get("mixed_lu32", {}).get("total_ms", float("nan"))))
+    r["model"] = trained
+    r["speedup_mixed_learned_vs_fp64"] = plain / r["mixed_learned"]["total_ms"] if "mixed_learned" in r else None
+    r["fastest_mixed"] = best
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=50, help="quads per side of the fault mesh (50 -> 5000 triangles)")
-    ap.add_argument("--cpu-rows", type=int, default=256, help="receiver rows timed for the CPU baseline")
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="receiver rows timed for the CPU baseline (0: the whole matrix, about 6 s on 16 cores)")
+    ap.add_argument("--solve-step", type=int, default=40,
+                    help="penny-crack growth step of the slip-solve comparison (40: 6534 triangles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     args = ap.parse_args()
@@ -111,7 +141,7 @@ def main():
     achieved_tf = FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12
     _log("assembly: %.2f ms/step, kernel %.2f ms, %.3e entries/s, %.1f TFLOP/s" % (ms_per_step, kavg, value, achieved_tf))
 
-    solve = None
+    lu15 = None
     if not args.no_solve:
         # FP64 LU of the assembled A with one right-hand side (remote sigma_zz = 1)
         Tn, Tds, Tss = api.SetupTractionVector(api.Stresses(0, 0, 1.0, 0, 0, 0), nrm, n, G, lam)
@@ -134,13 +164,17 @@ def main():
         res = float(torch.linalg.norm(r) / torch.linalg.norm(torch.from_numpy(b).to(dev)))
         lu_tf = (2.0 / 3.0) * N3 ** 3 / t_lu / 1e12
         _log("FP64 LU+solve: %.1f ms (%.1f TFLOP/s), residual %.2e" % (1e3 * t_lu, lu_tf, res))
-        solve = {"fp64_lu_ms": 1e3 * t_lu, "fp64_lu_tflops": lu_tf, "fp64_residual": res, "n": N3}
+        lu15 = {"fp64_lu_ms": 1e3 * t_lu, "fp64_lu_tflops": lu_tf, "fp64_residual": res, "n": N3}
         del Ab, A0
+
+    solve = None
+    if not args.no_solve:
+        solve = slip_solve_comparison(args.solve_step)
 
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        rows = args.cpu_rows
+        rows = args.cpu_rows if args.cpu_rows > 0 else n
         cv, cdt = cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads)
         _log("CPU baseline: %.3e entries/s (%d threads, %d rows, %.1f s)" % (cv, threads, rows, cdt))
         cpu = {"value": cv, "unit": "entries/s", "cores": threads, "kind": "port",
@@ -163,9 +197,10 @@ def main():
                                "A %d x %d FP64" % (n, N3, N3), "n_tri": n, "hs": 1},
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tf / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "tdk_traction<double, true>", "kernel_ms": kavg,
+                     "kernel": "tdk_traction<double, double, true>", "kernel_ms": kavg,
                      "flops_per_pair": FLOPS_PER_PAIR_HS, "pairs_per_launch": pairs},
         "cpu_baseline": cpu,
+        "lu_15000": lu15,
         "solve": solve,
     }
     print(json.dumps(out), flush=True)

@@ -28,7 +28,8 @@ namespace {
 
 constexpr int PT = 256;        // panel threads
 constexpr int MAX_P = 1024;    // max panel workgroups
-constexpr int MAX_RPB = 128;   // max panel rows per workgroup (LDS: 128 x 64 x 8 = 64 KiB)
+constexpr int MAX_RPB = PT;    // panel rows per workgroup, one per thread (LDS: 256 x 64 x 8 = 128 KiB)
+constexpr int LU_OUTER = 256;  // columns per outer block (rank of the trailing update)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -53,107 +54,113 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
                                                       int32_t* ipiv, int* info, double* cval, int* crow, T* crows,
                                                       T* rowj) {
   extern __shared__ unsigned char smem_raw[];
-  T* sh = reinterpret_cast<T*>(smem_raw);  // rpb x LU_NB, row-major
-  __shared__ T u[LU_NB];
-  __shared__ double redv[PT];
-  __shared__ int redr[PT];
-  __shared__ int s_w, s_prow;
+  T* sh = reinterpret_cast<T*>(smem_raw);  // column-major panel rows of this workgroup: sh[c * rpb + r]
+  __shared__ T u[LU_NB];                   // pivot row of the current column
+  __shared__ double wv[PT / 64];
+  __shared__ int wr[PT / 64];
+  __shared__ int s_prow;
   cg::grid_group grid = cg::this_grid();
-  const int P = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
+  const int P = gridDim.x, b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = (int64_t)b * rpb;
   const int64_t r1 = r0 + rpb < m ? r0 + rpb : m;
   const int nr = r1 > r0 ? (int)(r1 - r0) : 0;
-  for (int idx = tid; idx < nr * nbw; idx += PT) {
-    const int r = idx % nr, c = idx / nr;
-    sh[r * LU_NB + c] = A[(k + r0 + r) + (k + c) * lda];
-  }
-  __syncthreads();
+  // thread tid owns panel row r0 + tid (rpb <= PT)
+  const bool mine = tid < nr;
+  const int64_t grow = r0 + tid;
+  if (mine)
+    for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = A[(k + grow) + (k + c) * lda];
   for (int j = 0; j < nbw; ++j) {
     const int par = j & 1;
-    double bv = -1.0;
-    int br = -1;
-    for (int r = tid; r < nr; r += PT) {
-      const int gr = (int)(r0 + r);
-      if (gr < j) continue;
-      const double v = fabs((double)sh[r * LU_NB + j]);
-      if (v > bv) {
-        bv = v;
-        br = gr;
+    // local pivot candidate: largest |a|, ties to the lower row (the first maximum, as idamax)
+    double v = -1.0;
+    int r = -1;
+    if (mine && grow >= j) {
+      v = fabs((double)sh[j * rpb + tid]);
+      r = (int)grow;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double v2 = __shfl_xor(v, off);
+      const int r2 = __shfl_xor(r, off);
+      if (r2 >= 0 && (r < 0 || v2 > v || (v2 == v && r2 < r))) {
+        v = v2;
+        r = r2;
       }
     }
-    redv[tid] = bv;
-    redr[tid] = br;
+    if (lane == 0) {
+      wv[wave] = v;
+      wr[wave] = r;
+    }
     __syncthreads();
-    for (int s = PT / 2; s > 0; s >>= 1) {
-      if (tid < s) {
-        const double v2 = redv[tid + s];
-        const int r2 = redr[tid + s];
-        if (r2 >= 0 && (redr[tid] < 0 || v2 > redv[tid] || (v2 == redv[tid] && r2 < redr[tid]))) {
-          redv[tid] = v2;
-          redr[tid] = r2;
-        }
+    double bv = wv[0];
+    int br = wr[0];
+    for (int q = 1; q < PT / 64; ++q) {
+      const double v2 = wv[q];
+      const int r2 = wr[q];
+      if (r2 >= 0 && (br < 0 || v2 > bv || (v2 == bv && r2 < br))) {
+        bv = v2;
+        br = r2;
       }
-      __syncthreads();
     }
-    const int lbest = redr[0];
+    // publish the candidate (value, row, row contents) and, from the owner of row j, row j itself
     if (tid == 0) {
-      cval[par * P + b] = redv[0];
-      crow[par * P + b] = lbest;
+      cval[par * P + b] = bv;
+      crow[par * P + b] = br;
     }
-    if (lbest >= 0)
-      for (int c = tid; c < nbw; c += PT) crows[(int64_t)(par * P + b) * LU_NB + c] = sh[(lbest - r0) * LU_NB + c];
-    if (j >= r0 && j < r1)
-      for (int c = tid; c < nbw; c += PT) rowj[par * LU_NB + c] = sh[(j - r0) * LU_NB + c];
+    if (br >= 0 && tid < nbw) crows[(int64_t)(par * P + b) * LU_NB + tid] = sh[tid * rpb + (br - r0)];
+    if (j >= r0 && j < r1 && tid < nbw) rowj[par * LU_NB + tid] = sh[tid * rpb + (j - r0)];
     __threadfence();
     grid.sync();
-    if (tid == 0) {
-      double best = -1.0;
-      int w = -1, prow = -1;
-      for (int q = 0; q < P; ++q) {
+    // global pivot: wave 0 reduces the P candidates and stages the pivot row
+    if (wave == 0) {
+      double gv = -1.0;
+      int gr = -1, gw = -1;
+      for (int q = lane; q < P; q += 64) {
         const int rq = crow[par * P + q];
-        if (rq < 0) continue;
-        const double v = cval[par * P + q];
-        if (w < 0 || v > best || (v == best && rq < prow)) {
-          best = v;
-          w = q;
-          prow = rq;
+        const double vq = cval[par * P + q];
+        if (rq >= 0 && (gr < 0 || vq > gv || (vq == gv && rq < gr))) {
+          gv = vq;
+          gr = rq;
+          gw = q;
         }
       }
-      s_w = w;
-      s_prow = prow;
-      if (b == 0) {
-        ipiv[k + j] = (int32_t)(k + prow);
-        if (!(best > 0.0) && *info == 0) *info = (int)(k + j + 1);
+      for (int off = 32; off > 0; off >>= 1) {
+        const double v2 = __shfl_xor(gv, off);
+        const int r2 = __shfl_xor(gr, off);
+        const int w2 = __shfl_xor(gw, off);
+        if (r2 >= 0 && (gr < 0 || v2 > gv || (v2 == gv && r2 < gr))) {
+          gv = v2;
+          gr = r2;
+          gw = w2;
+        }
+      }
+      if (lane < nbw) u[lane] = gw >= 0 ? crows[(int64_t)(par * P + gw) * LU_NB + lane] : T(0);
+      if (lane == 0) {
+        s_prow = gr >= 0 ? gr : j;
+        if (b == 0) {
+          ipiv[k + j] = (int32_t)(k + (gr >= 0 ? gr : j));
+          if (!(gv > 0.0) && *info == 0) *info = (int)(k + j + 1);
+        }
       }
     }
     __syncthreads();
-    const int w = s_w, prow = s_prow;
-    for (int c = tid; c < nbw; c += PT) u[c] = crows[(int64_t)(par * P + w) * LU_NB + c];
-    __syncthreads();
-    if (prow != j) {
-      if (prow >= r0 && prow < r1)
-        for (int c = tid; c < nbw; c += PT) sh[(prow - r0) * LU_NB + c] = rowj[par * LU_NB + c];
-      if (j >= r0 && j < r1)
-        for (int c = tid; c < nbw; c += PT) sh[(j - r0) * LU_NB + c] = u[c];
-    }
-    __syncthreads();
-    const T piv = u[j];
-    if (piv != T(0))
-      for (int r = tid; r < nr; r += PT)
-        if (r0 + r > j) sh[r * LU_NB + j] = sh[r * LU_NB + j] / piv;
-    __syncthreads();
-    const int ncol = nbw - j - 1;
-    if (ncol > 0)
-      for (int idx = tid; idx < nr * ncol; idx += PT) {
-        const int r = idx / ncol, c = j + 1 + idx % ncol;
-        if (r0 + r > j) sh[r * LU_NB + c] -= sh[r * LU_NB + j] * u[c];
+    const int prow = s_prow;
+    if (mine) {
+      if (prow != j) {
+        if (grow == prow)
+          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = rowj[par * LU_NB + c];
+        else if (grow == j)
+          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = u[c];
       }
-    __syncthreads();
+      const T piv = u[j];
+      if (grow > j && piv != T(0)) {
+        const T l = sh[j * rpb + tid] / piv;
+        sh[j * rpb + tid] = l;
+        for (int c = j + 1; c < nbw; ++c) sh[c * rpb + tid] -= l * u[c];
+      }
+    }
   }
-  for (int idx = tid; idx < nr * nbw; idx += PT) {
-    const int r = idx % nr, c = idx / nr;
-    A[(k + r0 + r) + (k + c) * lda] = sh[r * LU_NB + c];
-  }
+  if (mine)
+    for (int c = 0; c < nbw; ++c) A[(k + grow) + (k + c) * lda] = sh[c * rpb + tid];
 }
 
 // Row interchanges of one panel applied to columns [0, k) and [k + nbw, ncols).
@@ -203,82 +210,111 @@ __global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k,
   }
 }
 
-// C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N], all column-major with leading dimension lda, K <= 64.
+// C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N], column-major with leading dimensions ldl, ldu, ldc, any K.
+// 128 x 128 tile per workgroup (4 waves in 2 x 2, 64 x 64 each = 4 x 4 MFMA tiles), K streamed in chunks of
+// GB_K through LDS with the next chunk prefetched into registers, C read and written once at the end.  The
+// product is formed as (U^T L^T)^T so each lane's accumulator items run down a column segment of C.
+constexpr int GB_M = 128, GB_N = 128, GB_K = 16;
+
 template <class T>
-__global__ void __launch_bounds__(256) gemm_update_kernel(const T* __restrict__ L, const T* __restrict__ U,
-                                                          T* __restrict__ C, int64_t lda, int64_t M, int64_t N, int K) {
-  __shared__ T Ls[LU_NB][64 + 1];  // Ls[k][i] = L[i0 + i, k]
-  __shared__ T Us[LU_NB][64 + 1];  // Us[k][j] = U[k, j0 + j]
+__global__ void __launch_bounds__(256) gemm_sub_kernel(const T* __restrict__ L, int64_t ldl, const T* __restrict__ U,
+                                                       int64_t ldu, T* __restrict__ C, int64_t ldc, int64_t M,
+                                                       int64_t N, int64_t K) {
+  __shared__ T Ls[GB_K][GB_M + 4];  // Ls[k][i] = L[i0 + i, k0 + k]
+  __shared__ T Us[GB_K][GB_N + 4];  // Us[k][j] = U[k0 + k, j0 + j]
   const int tid = threadIdx.x;
-  const int64_t i0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
-  for (int idx = tid; idx < 64 * LU_NB; idx += 256) {
-    const int i = idx % 64, kk = idx / 64;
-    Ls[kk][i] = (kk < K && i0 + i < M) ? L[(i0 + i) + (int64_t)kk * lda] : T(0);
-  }
-  for (int idx = tid; idx < 64 * LU_NB; idx += 256) {
-    const int kk = idx % LU_NB, j = idx / LU_NB;
-    Us[kk][j] = (kk < K && j0 + j < N) ? U[kk + (j0 + j) * lda] : T(0);
-  }
-  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * GB_M, j0 = (int64_t)blockIdx.y * GB_N;
   const int wave = tid / 64, lane = tid % 64;
-  const int wr = (wave % 2) * 32, wc = (wave / 2) * 32;
+  const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int li = lane % 16, lk = lane / 16;
-  typename Mfma<T>::acc acc[2][2];
+  constexpr int PER = GB_M * GB_K / 256;  // elements of each operand staged per thread (8)
+  T lr[PER], ur[PER];
+  auto fetch = [&](int64_t k0) {
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + 256 * e;
+      const int i = idx % GB_M, kk = idx / GB_M;  // L: rows contiguous
+      lr[e] = (i0 + i < M && k0 + kk < K) ? L[(i0 + i) + (k0 + kk) * ldl] : T(0);
+      const int kq = idx % GB_K, j = idx / GB_K;  // U: k contiguous within a column
+      ur[e] = (k0 + kq < K && j0 + j < N) ? U[(k0 + kq) + (j0 + j) * ldu] : T(0);
+    }
+  };
+  typename Mfma<T>::acc acc[4][4];
 #pragma unroll
-    for (int b2 = 0; b2 < 2; ++b2) acc[a][b2] = typename Mfma<T>::acc{0, 0, 0, 0};
-  const int Kr = (K + 3) & ~3;
-  for (int kk = 0; kk < Kr; kk += 4) {
-    const T l0 = Ls[kk + lk][wr + li], l1 = Ls[kk + lk][wr + 16 + li];
-    const T u0 = Us[kk + lk][wc + li], u1 = Us[kk + lk][wc + 16 + li];
-    acc[0][0] = Mfma<T>::step(u0, l0, acc[0][0]);
-    acc[0][1] = Mfma<T>::step(u0, l1, acc[0][1]);
-    acc[1][0] = Mfma<T>::step(u1, l0, acc[1][0]);
-    acc[1][1] = Mfma<T>::step(u1, l1, acc[1][1]);
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b2 = 0; b2 < 4; ++b2) acc[a][b2] = typename Mfma<T>::acc{0, 0, 0, 0};
+  fetch(0);
+  for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + 256 * e;
+      Ls[idx / GB_M][idx % GB_M] = lr[e];
+      Us[idx % GB_K][idx / GB_K] = ur[e];
+    }
+    __syncthreads();
+    if (k0 + GB_K < K) fetch(k0 + GB_K);
+#pragma unroll
+    for (int kk = 0; kk < GB_K; kk += 4) {
+      T lv[4], uv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        lv[t] = Ls[kk + lk][wr + 16 * t + li];
+        uv[t] = Us[kk + lk][wc + 16 * t + li];
+      }
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti) acc[tj][ti] = Mfma<T>::step(uv[tj], lv[ti], acc[tj][ti]);
+    }
+    __syncthreads();
   }
   // acc[tj][ti][v] = (L U)[row = wr + 16 ti + li, col = wc + 16 tj + Mfma::row(lk, v)]
 #pragma unroll
-  for (int tj = 0; tj < 2; ++tj)
+  for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
-    for (int ti = 0; ti < 2; ++ti) {
+    for (int ti = 0; ti < 4; ++ti) {
       const int64_t row = i0 + wr + 16 * ti + li;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t col = j0 + wc + 16 * tj + Mfma<T>::row(lk, v);
-        if (row < M && col < N) C[row + col * lda] -= acc[tj][ti][v];
+        if (row < M && col < N) C[row + col * ldc] -= acc[tj][ti][v];
       }
     }
 }
 
 // ---- triangular solves with nrhs <= 64 right-hand sides -------------------------------------------------
-// Diagonal-block solve of rows [r0, r0+nb) for all right-hand sides: lower (unit) or upper.
+// Diagonal-block solve of rows [r0, r0+nb) for all right-hand sides: lower (unit) or upper.  One wave per
+// right-hand side: lane i holds x[i]; step r broadcasts x[r] and the lanes below (lower) or above (upper) update.
 template <class T, bool LOWER>
 __global__ void __launch_bounds__(256) trsv_diag_kernel(const T* __restrict__ A, int64_t lda, int64_t r0, int nb,
                                                         T* B, int64_t ldb, int64_t nrhs) {
   __shared__ T D[LU_NB][LU_NB + 1];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int idx = tid; idx < nb * nb; idx += 256) {
     const int r = idx % nb, c = idx / nb;
     D[r][c] = A[(r0 + r) + (r0 + c) * lda];
   }
   __syncthreads();
-  for (int64_t q = tid; q < nrhs; q += 256) {
-    T* x = B + q * ldb + r0;
-    if (LOWER) {
-      for (int r = 1; r < nb; ++r) {
-        T s = x[r];
-        for (int c = 0; c < r; ++c) s -= D[r][c] * x[c];
-        x[r] = s;
-      }
-    } else {
-      for (int r = nb - 1; r >= 0; --r) {
-        T s = x[r];
-        for (int c = r + 1; c < nb; ++c) s -= D[r][c] * x[c];
-        x[r] = s / D[r][r];
-      }
+  const int64_t q = (int64_t)blockIdx.x * 4 + wave;
+  if (q >= nrhs) return;
+  T* x = B + q * ldb + r0;
+  T xi = lane < nb ? x[lane] : T(0);
+  if (LOWER) {
+#pragma unroll 8
+    for (int r = 0; r < nb - 1; ++r) {
+      const T xr = __shfl(xi, r);
+      if (lane > r && lane < nb) xi -= D[lane][r] * xr;
+    }
+  } else {
+#pragma unroll 8
+    for (int r = nb - 1; r >= 0; --r) {
+      if (lane == r) xi = xi / D[r][r];
+      const T xr = __shfl(xi, r);
+      if (lane < r) xi -= D[lane][r] * xr;
     }
   }
+  if (lane < nb) x[lane] = xi;
 }
 
 // B[rows] -= A[rows, c0:c0+nb] * B[c0:c0+nb] for rows in [rb, re).
@@ -331,6 +367,27 @@ __global__ void d2f_matrix_kernel(const double* A, int64_t lda, float* B, int64_
 
 unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+template <class T>
+void gemm_sub(const T* L, int64_t ldl, const T* U, int64_t ldu, T* C, int64_t ldc, int64_t M, int64_t N, int64_t K,
+              hipStream_t st) {
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  hipLaunchKernelGGL(gemm_sub_kernel<T>, dim3(nblk(M, GB_M), nblk(N, GB_N)), dim3(256), 0, st, L, ldl, U, ldu, C, ldc,
+                     M, N, K);
+}
+
+// Rows [r0, r1) of columns [c0, c0 + nc): X = L^-1 X with L the unit-lower block of the sub-panels in [r0, r1)
+// (64 rows each): per sub-panel a diagonal-block solve, then the update of the rows below it inside [r0, r1).
+template <class T>
+void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t nc, hipStream_t st) {
+  if (nc <= 0) return;
+  for (int64_t s = r0; s < r1; s += LU_NB) {
+    const int nb = (int)std::min<int64_t>(LU_NB, r1 - s);
+    hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nc, 64)), dim3(256), 0, st, A, lda, s, nb, c0, nc);
+    gemm_sub<T>(A + (s + nb) + s * lda, lda, A + s + c0 * lda, lda, A + (s + nb) + c0 * lda, lda, r1 - s - nb, nc, nb,
+                st);
+  }
+}
+
 }  // namespace
 
 size_t lu_workspace_bytes(int64_t n) {
@@ -338,6 +395,11 @@ size_t lu_workspace_bytes(int64_t n) {
   return sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) + sizeof(int) * 2 * MAX_P + 1024;
 }
 
+// Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
+// carried along).  Outer blocks of LU_OUTER columns are factored as LU_NB-wide panels, each first brought up to
+// date with the earlier panels of its block (left-looking inside the block); the rest of the matrix then gets a
+// single rank-LU_OUTER update (right-looking across blocks), so the trailing matrix is read and written once per
+// LU_OUTER columns.
 template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
                           hipStream_t st) {
@@ -351,33 +413,37 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   int* crow = reinterpret_cast<int*>(w);
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
   if (e != hipSuccess) return e;
-  for (int64_t k = 0; k < n; k += LU_NB) {
-    const int nbw = (int)std::min<int64_t>(LU_NB, n - k);
-    const int64_t m = n - k;
-    int P = (int)std::min<int64_t>(256, (m + 63) / 64);
-    int rpb = (int)((m + P - 1) / P);
-    if (rpb > MAX_RPB) {
-      P = (int)((m + MAX_RPB - 1) / MAX_RPB);
-      rpb = (int)((m + P - 1) / P);
-    }
-    if (P > MAX_P) return hipErrorInvalidValue;
-    size_t shm = sizeof(T) * (size_t)rpb * LU_NB;
-    void* args[] = {&A, (void*)&lda, &k, (void*)&m, (void*)&nbw, &rpb, &ipiv, &info, &cval, &crow, &crows, &rowj};
-    e = hipLaunchCooperativeKernel((const void*)lu_panel_kernel<T>, dim3(P), dim3(PT), args, shm, st);
+  static bool lds_set = false;  // raise the dynamic LDS limit once per process (128 KiB panels)
+  if (!lds_set) {
+    e = hipFuncSetAttribute((const void*)lu_panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(sizeof(T) * MAX_RPB * LU_NB));
     if (e != hipSuccess) return e;
-    if (ncols > nbw) {
-      hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 256)), dim3(256), 0, st, A, lda, ncols, k, nbw, ipiv);
+    lds_set = true;
+  }
+  for (int64_t K0 = 0; K0 < n; K0 += LU_OUTER) {
+    const int64_t K1 = std::min<int64_t>(n, K0 + LU_OUTER);
+    for (int64_t k = K0; k < K1; k += LU_NB) {
+      const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
+      if (k > K0) {
+        // this panel's columns: U rows from the block's earlier panels, then their update of rows [k, n)
+        block_trsm<T>(A, lda, K0, k, k, nbw, st);
+        gemm_sub<T>(A + k + K0 * lda, lda, A + K0 + k * lda, lda, A + k + k * lda, lda, n - k, nbw, k - K0, st);
+      }
+      const int64_t m = n - k;
+      const int P = (int)((m + MAX_RPB - 1) / MAX_RPB);
+      int rpb = (int)((m + P - 1) / P);
+      if (P > MAX_P) return hipErrorInvalidValue;
+      size_t shm = sizeof(T) * (size_t)rpb * LU_NB;
+      void* args[] = {&A, (void*)&lda, &k, (void*)&m, (void*)&nbw, &rpb, &ipiv, &info, &cval, &crow, &crows, &rowj};
+      e = hipLaunchCooperativeKernel((const void*)lu_panel_kernel<T>, dim3(P), dim3(PT), args, shm, st);
+      if (e != hipSuccess) return e;
+      if (ncols > nbw)
+        hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 256)), dim3(256), 0, st, A, lda, ncols, k, nbw, ipiv);
     }
-    const int64_t c0 = k + nbw;
-    const int64_t nrest = ncols - c0;
-    if (nrest > 0) {
-      hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nrest, 64)), dim3(256), 0, st, A, lda, k, nbw, c0, nrest);
-      const int64_t M = n - c0;
-      if (M > 0)
-        hipLaunchKernelGGL(gemm_update_kernel<T>, dim3(nblk(M, 64), nblk(nrest, 64)), dim3(256), 0, st,
-                           (const T*)(A + c0 + k * lda), (const T*)(A + k + c0 * lda), A + c0 + c0 * lda, lda, M,
-                           nrest, nbw);
-    }
+    // U12 of the block, then the rank-(K1 - K0) update of everything right of and below it
+    block_trsm<T>(A, lda, K0, K1, K1, ncols - K1, st);
+    gemm_sub<T>(A + K1 + K0 * lda, lda, A + K0 + K1 * lda, lda, A + K1 + K1 * lda, lda, n - K1, ncols - K1, K1 - K0,
+                st);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -414,7 +480,8 @@ template <class T>
 hipError_t tri_solve_lower_unit(const T* LU, int64_t n, int64_t lda, T* B, int64_t nrhs, int64_t ldb, hipStream_t st) {
   for (int64_t r0 = 0; r0 < n; r0 += LU_NB) {
     const int nb = (int)std::min<int64_t>(LU_NB, n - r0);
-    hipLaunchKernelGGL((trsv_diag_kernel<T, true>), dim3(1), dim3(256), 0, st, LU, lda, r0, nb, B, ldb, nrhs);
+    hipLaunchKernelGGL((trsv_diag_kernel<T, true>), dim3(nblk(nrhs, 4)), dim3(256), 0, st, LU, lda, r0, nb, B, ldb,
+                       nrhs);
     const int64_t rb = r0 + nb;
     if (rb < n)
       hipLaunchKernelGGL(trsv_update_kernel<T>, dim3(nblk(n - rb, 256)), dim3(256), 0, st, LU, lda, r0, nb, rb, n, B,
@@ -429,7 +496,8 @@ hipError_t tri_solve_upper(const T* LU, int64_t n, int64_t lda, T* B, int64_t nr
   for (int64_t bi = nblocks - 1; bi >= 0; --bi) {
     const int64_t r0 = bi * LU_NB;
     const int nb = (int)std::min<int64_t>(LU_NB, n - r0);
-    hipLaunchKernelGGL((trsv_diag_kernel<T, false>), dim3(1), dim3(256), 0, st, LU, lda, r0, nb, B, ldb, nrhs);
+    hipLaunchKernelGGL((trsv_diag_kernel<T, false>), dim3(nblk(nrhs, 4)), dim3(256), 0, st, LU, lda, r0, nb, B, ldb,
+                       nrhs);
     if (r0 > 0)
       hipLaunchKernelGGL(trsv_update_kernel<T>, dim3(nblk(r0, 256)), dim3(256), 0, st, LU, lda, r0, nb, (int64_t)0, r0,
                          B, ldb, nrhs);

@@ -79,6 +79,7 @@ __global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restric
   __shared__ double accd[4][TB];
   __shared__ T xs[TB];
   __shared__ T rhs[TB];
+  __shared__ T D[TB][TB + 1];  // this block's diagonal block of LU
   __shared__ int s_ok;
   const int nb = (int)((n + TB - 1) / TB);
   const int b = blockIdx.x;
@@ -86,6 +87,10 @@ __global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restric
   const int nr = (int)std::min<int64_t>(TB, n - r0);
   const int tid = threadIdx.x, row = tid % TB, grp = tid / TB;  // 4 groups of 64 threads
   if (tid < nr) rhs[tid] = x[r0 + tid];
+  for (int idx = tid; idx < nr * nr; idx += 256) {
+    const int r = idx % nr, c = idx / nr;
+    D[r][c] = LU[(r0 + r) + (r0 + c) * lda];
+  }
   double acc = 0.0;  // per (row, grp) partial of sum_c L[r, c] x[c]
   // off-diagonal blocks in dependency order: c = 0..b-1 (lower), c = nb-1..b+1 (upper)
   const int ndep = LOWER ? b : nb - 1 - b;
@@ -106,24 +111,25 @@ __global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restric
   __syncthreads();
   if (tid < nr) rhs[tid] = (T)((double)rhs[tid] - (accd[0][tid] + accd[1][tid] + accd[2][tid] + accd[3][tid]));
   __syncthreads();
-  // diagonal block: sequential over its rows, parallel over the updates
-  if (LOWER) {
-    for (int r = 0; r < nr; ++r) {
-      const T xr = rhs[r];
-      __syncthreads();
-      if (tid > r && tid < nr) rhs[tid] -= LU[(r0 + tid) + (r0 + r) * lda] * xr;
-      __syncthreads();
+  // diagonal block: wave 0, lane i holds x[r0 + i]; step r broadcasts x[r0 + r] (the block is staged in LDS)
+  if (tid < 64) {
+    T xi = tid < nr ? rhs[tid] : T(0);
+    if (LOWER) {
+#pragma unroll 8
+      for (int r = 0; r < nr - 1; ++r) {
+        const T xr = __shfl(xi, r);
+        if (tid > r && tid < nr) xi -= D[tid][r] * xr;
+      }
+    } else {
+#pragma unroll 8
+      for (int r = nr - 1; r >= 0; --r) {
+        if (tid == r) xi = xi / D[r][r];
+        const T xr = __shfl(xi, r);
+        if (tid < r) xi -= D[tid][r] * xr;
+      }
     }
-  } else {
-    for (int r = nr - 1; r >= 0; --r) {
-      if (tid == 0) rhs[r] = rhs[r] / LU[(r0 + r) + (r0 + r) * lda];
-      __syncthreads();
-      const T xr = rhs[r];
-      if (tid < r) rhs[tid] -= LU[(r0 + tid) + (r0 + r) * lda] * xr;
-      __syncthreads();
-    }
+    if (tid < nr) __hip_atomic_store(&x[r0 + tid], xi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  if (tid < nr) __hip_atomic_store(&x[r0 + tid], rhs[tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __threadfence();
   __syncthreads();
   if (tid == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
