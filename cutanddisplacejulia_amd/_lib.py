@@ -37,7 +37,7 @@ EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", 
            "cdj_traction_influence", "cdj_slip_calculator", "cdj_solve", "cdj_assemble_dev",
            "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev", "cdj_mlp_create", "cdj_mlp_destroy",
            "cdj_mlp_nparams", "cdj_mlp_get_params", "cdj_mlp_set_params", "cdj_mlp_train_dev", "cdj_mlp_loss_grad",
-           "cdj_precond_solve_dev", "cdj_last_near_entries"]
+           "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning"]
 
 
 class CdjError(RuntimeError):
@@ -71,6 +71,13 @@ def load(path=LIB_PATH):
         return _LIB
     if not os.path.exists(path):
         raise CdjError(CDJ_ERR_DEVICE, "libcdjhip.so is not built (%s); run cutanddisplacejulia_amd.build" % path)
+    # torch ships its own HIP runtime (same soname, libamdhip64.so.7, but opened by the unversioned name).  Loading
+    # it first makes libcdjhip.so bind to that copy, so the process holds one HIP runtime; loading ours first would
+    # make torch open a second one, which then sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(path)
     L.cdj_create.argtypes = [ctypes.c_int, ctypes.POINTER(_p)]
     L.cdj_destroy.argtypes = [_p]
@@ -78,6 +85,8 @@ def load(path=LIB_PATH):
     L.cdj_last_error.restype = ctypes.c_char_p
     L.cdj_last_kernel_ms.argtypes = [_p]
     L.cdj_last_near_entries.argtypes = [_p]
+    L.cdj_batched_slip.argtypes = [_p, _p, _i64, _d, _d, _d, _i32, _p, _p, _p]
+    L.cdj_batch_tuning.argtypes = [_p, _p, _p]
     td_args = [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _d, _d, _i32, _i32, _i32, _p, _p]
     L.cdj_td_influence.argtypes = td_args
     L.cdj_td_field.argtypes = td_args

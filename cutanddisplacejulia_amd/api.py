@@ -397,3 +397,53 @@ def KnockOutFixedRows(FixedEls, Tn, Tds, Tss):
         Tds[f] = 0.0
         Tss[f] = 0.0
     return Tn, Tds, Tss
+
+
+# ---- batched slip solves (many meshes, several remote loads each) -------------------------------------------
+
+
+class _Mesh(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("P1", ctypes.c_void_p), ("P2", ctypes.c_void_p), ("P3", ctypes.c_void_p),
+                ("normals", ctypes.c_void_p), ("midpoints", ctypes.c_void_p), ("nrhs", ctypes.c_int64),
+                ("b", ctypes.c_void_p), ("x", ctypes.c_void_p)]
+
+
+def batched_slip(meshes, nu, G, lam, hs_flag=0):
+    """Slip of every mesh under each of its remote loads.
+
+    meshes: list of dicts with P1, P2, P3 (n x 3) and `stresses` (k x 6 remote tensors sxx, syy, szz, sxy, sxz,
+    syz), or `b` (3n x k right-hand sides [Tn; Tss; Tds]).  Returns a list of 3n x k arrays [Dn; Dss; Dds] and the
+    total report (assembly time of the batched graph, summed solve time, near entries).  The precision is the
+    module setting (set_precision)."""
+    L = load()
+    keep = []
+    arr = (_Mesh * len(meshes))()
+    outs = []
+    for i, m in enumerate(meshes):
+        P1, P2, P3 = _tri(m["P1"]), _tri(m["P2"]), _tri(m["P3"])
+        n = P1.shape[0]
+        nrm, mid = CreateFaceNormalAndMidPoint(P1, P2, P3)
+        if "b" in m:
+            b = f64(np.asarray(m["b"]).reshape(3 * n, -1, order="F"))
+        else:
+            cols = []
+            for sxx, syy, szz, sxy, sxz, syz in np.atleast_2d(m["stresses"]):
+                Tn, Tds, Tss = SetupTractionVector(Stresses(sxx, syy, szz, sxy, sxz, syz), nrm, n, G, lam)
+                cols.append(np.concatenate([Tn, Tss, Tds]))
+            b = f64(np.stack(cols, axis=1))
+        x = np.zeros_like(b, order="F")
+        nrm, mid = f64(nrm), f64(mid)
+        keep += [P1, P2, P3, nrm, mid, b, x]
+        arr[i] = _Mesh(n, P1.ctypes.data, P2.ctypes.data, P3.ctypes.data, nrm.ctypes.data, mid.ctypes.data,
+                       b.shape[1], b.ctypes.data, x.ctypes.data)
+        outs.append(x)
+    rep = Report()
+    check(L.cdj_batched_slip(ctx(), ctypes.byref(arr), len(meshes), float(nu), float(G), float(lam), int(hs_flag),
+                             ctypes.byref(_opts()), None, ctypes.byref(rep)))
+    tpb, wgs = ctypes.c_int32(), ctypes.c_int32()
+    check(L.cdj_batch_tuning(ctx(), ctypes.byref(tpb), ctypes.byref(wgs)))
+    r = rep.as_dict()
+    r["tile"] = {"sources_per_item": tpb.value, "workgroups": wgs.value}
+    LAST_REPORT.clear()
+    LAST_REPORT.update(r)
+    return outs, r

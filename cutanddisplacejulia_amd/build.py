@@ -12,11 +12,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcdjhip.so")
-SOURCES = ["td_kernels.hip", "linalg.hip", "mixed.hip", "learned.hip", "cdj_abi.cpp"]
+SOURCES = ["td_kernels.hip", "linalg.hip", "mixed.hip", "learned.hip", "batch.hip", "cdj_abi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CDJ_OFFLOAD_ARCH", "gfx950")
-# Packed FP32 instructions are disabled in all device code (see the project notes on gfx950 packed-FP32 forms).
+# Packed FP32 instructions are disabled in all device code: some of their forms have given silently wrong
+# results on gfx950, so the compiler is told not to emit them.
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-result"]

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -9,6 +10,7 @@
 #include <vector>
 
 #include "../../include/cdj.h"
+#include "batch.hpp"
 #include "learned.hpp"
 #include "linalg.hpp"
 #include "mixed.hpp"
@@ -19,6 +21,9 @@ using namespace cdj;
 struct cdj_mlp {
   cdj::Mlp m;
 };
+
+struct BatchPlan;
+void batch_plan_free(BatchPlan* p);
 
 struct cdj_ctx {
   int device = 0;
@@ -32,6 +37,8 @@ struct cdj_ctx {
   bool timed = false;
   // entries recomputed in FP64 by the last mixed assembly
   int64_t near_entries = 0;
+  // device buffers, tuned tiles and captured graph of the last batch shape (cdj_batched_slip)
+  BatchPlan* batch = nullptr;
 };
 
 namespace {
@@ -184,6 +191,7 @@ int cdj_destroy(cdj_ctx* ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  batch_plan_free(ctx->batch);
   delete ctx;
   return CDJ_OK;
 }
@@ -505,6 +513,7 @@ int cdj_lu_factor_dev(cdj_ctx* ctx, double* A, int64_t n, int64_t lda, int32_t* 
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (h < 0) return fail(ctx, CDJ_ERR_DEVICE, "LU panel barrier wait timed out");
   if (h) return fail(ctx, CDJ_ERR_SINGULAR, "matrix is exactly singular (zero pivot " + std::to_string(h) + ")");
   return CDJ_OK;
 }
@@ -544,6 +553,7 @@ int solve_augmented_fp64(cdj_ctx* ctx, double* AB, int64_t n, int64_t lda, int64
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (h < 0) return fail(ctx, CDJ_ERR_DEVICE, "LU panel barrier wait timed out");
   if (h) return fail(ctx, CDJ_ERR_SINGULAR, "matrix is exactly singular (zero pivot " + std::to_string(h) + ")");
   if (report) {
     report->path = CDJ_PREC_FP64;
@@ -989,6 +999,314 @@ int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const do
       report->solve_ms += r2.solve_ms;
     }
   }
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+// ---- batched assembly and solve ----------------------------------------------------------------------------
+//
+// A plan holds the device buffers of one batch shape (triangle counts, right-hand-side counts, half-space flag,
+// assembly mode, elastic constants), the tile shape chosen by the autotuner and the captured graph of the
+// assembly sequence.  A call with the same shape reuses all three; another shape replaces the plan.
+
+namespace {
+
+struct TuneKey {
+  int hs, mixed, bucket;
+  bool operator<(const TuneKey& o) const {
+    return hs != o.hs ? hs < o.hs : (mixed != o.mixed ? mixed < o.mixed : bucket < o.bucket);
+  }
+};
+std::map<TuneKey, std::pair<int, int>>& tune_cache() {
+  static std::map<TuneKey, std::pair<int, int>> m;
+  return m;
+}
+
+}  // namespace
+
+struct BatchPlan {
+  std::vector<int64_t> n, nrhs;
+  int hs = 0;
+  bool mixed = false;
+  double near_factor = 2.0, nu = 0, lam = 0, G = 0;
+  int64_t ntot = 0, a_total = 0, b_total = 0;
+  std::vector<BatchMeshDev> mh;      // host copy of the mesh table
+  std::vector<int64_t> a_off, b_off; // per mesh: A (3n x (3n + nrhs)) and B/X (3n x nrhs) offsets, in doubles
+  DBuf geo, recv, rec, rec32, near, meshes, items, counters, A, B, X;
+  int tpb = 0, wgs = 0;
+  int64_t nitems = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~BatchPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  bool same(const std::vector<int64_t>& n2, const std::vector<int64_t>& r2, int hs2, bool mixed2, double f2,
+            double nu2, double lam2, double G2) const {
+    return n == n2 && nrhs == r2 && hs == hs2 && mixed == mixed2 && near_factor == f2 && nu == nu2 && lam == lam2 &&
+           G == G2;
+  }
+};
+
+void batch_plan_free(BatchPlan* p) { delete p; }
+
+namespace {
+
+// item prefix sums for a source-chunk width tpb
+std::vector<int64_t> batch_items(const std::vector<int64_t>& n, int tpb) {
+  std::vector<int64_t> st(n.size() + 1, 0);
+  for (size_t m = 0; m < n.size(); ++m) st[m + 1] = st[m] + ((n[m] + 255) / 256) * ((n[m] + tpb - 1) / tpb);
+  return st;
+}
+
+// The assembly sequence of a plan: counters cleared, receivers and records built, far (and near) passes.
+hipError_t batch_enqueue(BatchPlan& p, hipStream_t st) {
+  const int64_t N = p.ntot;
+  const double* P1 = p.geo.as<double>();
+  const double* P2 = P1 + 3 * N;
+  const double* P3 = P2 + 3 * N;
+  const double* nrm = P3 + 3 * N;
+  const double* mid = nrm + 3 * N;
+  unsigned long long* cnt = p.counters.as<unsigned long long>();
+  launch_batch_clear(cnt, 4, st);
+  launch_receivers(nrm, mid, N, p.recv.as<double>(), st);
+  launch_setup(P1, P2, P3, N, p.hs, p.rec.as<double>(), st);
+  const int nmesh = (int)p.n.size();
+  if (p.mixed) {
+    const int RS = p.hs ? REC_HS : REC_FS;
+    launch_rec_to_f32(p.rec.as<double>(), p.rec32.as<float>(), (int64_t)RS * N, st);
+    launch_near_info(P1, P2, P3, N, p.near_factor, p.near.as<double>(), st);
+    launch_batch_far(p.recv.as<double>(), p.rec32.p, true, p.hs, p.meshes.as<BatchMeshDev>(), p.items.as<int64_t>(),
+                     nmesh, p.tpb, p.wgs, p.nu, p.lam, p.G, p.A.as<double>(), cnt, st);
+    launch_batch_near(p.recv.as<double>(), p.rec.as<double>(), p.near.as<double>(), p.hs, p.meshes.as<BatchMeshDev>(),
+                      p.items.as<int64_t>(), nmesh, p.tpb, p.wgs, p.nu, p.lam, p.G, p.A.as<double>(), cnt + 1,
+                      cnt + 2, st);
+  } else {
+    launch_batch_far(p.recv.as<double>(), p.rec.p, false, p.hs, p.meshes.as<BatchMeshDev>(), p.items.as<int64_t>(),
+                     nmesh, p.tpb, p.wgs, p.nu, p.lam, p.G, p.A.as<double>(), cnt, st);
+  }
+  return hipGetLastError();
+}
+
+// Autotuner: times the assembly for each (source chunk, persistent workgroups) candidate on this batch and keeps
+// the fastest; results are cached per (half-space, mode, size bucket) for later plans.
+int batch_tune(cdj_ctx* ctx, BatchPlan& p, hipStream_t st) {
+  int64_t pairs = 0;
+  for (int64_t v : p.n) pairs += v * v;
+  int bucket = 0;
+  while ((1ll << bucket) < pairs) ++bucket;
+  const TuneKey key{p.hs, p.mixed ? 1 : 0, bucket};
+  auto it = tune_cache().find(key);
+  int cus = 256;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess && prop.multiProcessorCount > 0)
+      cus = prop.multiProcessorCount;
+  }
+  std::vector<std::pair<int, int>> cands;
+  if (it != tune_cache().end()) {
+    cands.push_back(it->second);
+  } else {
+    const int tpbs_fs[] = {8, 16, 32, 64}, tpbs_hs[] = {2, 4, 8, 16};
+    for (int t : (p.hs ? tpbs_hs : tpbs_fs))
+      for (int w : {cus, 2 * cus, 4 * cus}) cands.push_back({t, w});
+  }
+  float best = 1e30f;
+  std::pair<int, int> choice = cands[0];
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (auto c : cands) {
+    p.tpb = c.first;
+    p.wgs = c.second;
+    const std::vector<int64_t> items = batch_items(p.n, p.tpb);
+    HIPCHK(hipMemcpyAsync(p.items.p, items.data(), sizeof(int64_t) * items.size(), hipMemcpyHostToDevice, st));
+    if (cands.size() == 1) break;
+    HIPCHK(batch_enqueue(p, st));  // warm-up (first touch of the code object)
+    HIPCHK(hipEventRecord(e0, st));
+    HIPCHK(batch_enqueue(p, st));
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      choice = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p.tpb = choice.first;
+  p.wgs = choice.second;
+  const std::vector<int64_t> items = batch_items(p.n, p.tpb);
+  p.nitems = items.back();
+  HIPCHK(hipMemcpyAsync(p.items.p, items.data(), sizeof(int64_t) * items.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  tune_cache()[key] = choice;
+  return CDJ_OK;
+}
+
+int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bool mixed, double f, double nu,
+                     double lam, double G, hipStream_t st) {
+  std::vector<int64_t> n(nmesh), r(nmesh);
+  for (int64_t m = 0; m < nmesh; ++m) {
+    n[m] = ms[m].n;
+    r[m] = ms[m].nrhs;
+  }
+  if (ctx->batch && ctx->batch->same(n, r, hs, mixed, f, nu, lam, G)) return CDJ_OK;
+  batch_plan_free(ctx->batch);
+  ctx->batch = new BatchPlan();
+  BatchPlan& p = *ctx->batch;
+  p.n = n;
+  p.nrhs = r;
+  p.hs = hs;
+  p.mixed = mixed;
+  p.near_factor = f;
+  p.nu = nu;
+  p.lam = lam;
+  p.G = G;
+  p.mh.resize(nmesh);
+  p.a_off.resize(nmesh);
+  p.b_off.resize(nmesh);
+  for (int64_t m = 0; m < nmesh; ++m) {
+    const int64_t N3 = 3 * n[m];
+    p.mh[m] = BatchMeshDev{p.ntot, n[m], p.a_total};
+    p.a_off[m] = p.a_total;
+    p.b_off[m] = p.b_total;
+    p.ntot += n[m];
+    p.a_total += N3 * (N3 + r[m]);
+    p.b_total += N3 * r[m];
+  }
+  const int RS = hs ? REC_HS : REC_FS;
+  HIPCHK(p.geo.alloc(sizeof(double) * 15 * p.ntot));
+  HIPCHK(p.recv.alloc(sizeof(double) * RECV * p.ntot));
+  HIPCHK(p.rec.alloc(sizeof(double) * RS * p.ntot));
+  if (mixed) {
+    HIPCHK(p.rec32.alloc(sizeof(float) * RS * p.ntot));
+    HIPCHK(p.near.alloc(sizeof(double) * 4 * p.ntot));
+  }
+  HIPCHK(p.meshes.alloc(sizeof(BatchMeshDev) * nmesh));
+  HIPCHK(p.items.alloc(sizeof(int64_t) * (nmesh + 1)));
+  HIPCHK(p.counters.alloc(sizeof(unsigned long long) * 4));
+  HIPCHK(p.A.alloc(sizeof(double) * p.a_total));
+  HIPCHK(p.B.alloc(sizeof(double) * p.b_total));
+  HIPCHK(p.X.alloc(sizeof(double) * p.b_total));
+  HIPCHK(hipMemcpyAsync(p.meshes.p, p.mh.data(), sizeof(BatchMeshDev) * nmesh, hipMemcpyHostToDevice, st));
+  return CDJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double nu, double G, double lambda,
+                     int32_t hs_flag, const cdj_opts* opts, cdj_report* reports, cdj_report* total) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (nmesh < 0 || (nmesh > 0 && !meshes)) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  bool mixed = false;
+  double near_factor = 2.0;
+  if (int rc = assembly_mode(ctx, opts, &mixed, &near_factor)) return rc;
+  for (int64_t m = 0; m < nmesh; ++m) {
+    const cdj_mesh& M = meshes[m];
+    if (M.n <= 0 || M.nrhs < 1) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+    if (!M.P1 || !M.P2 || !M.P3 || !M.normals || !M.midpoints || !M.b || !M.x)
+      return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+    if (hs_flag == 1)
+      for (int64_t i = 0; i < M.n; ++i)
+        if (M.P1[2 * M.n + i] > 0 || M.P2[2 * M.n + i] > 0 || M.P3[2 * M.n + i] > 0 || M.midpoints[2 * M.n + i] > 0)
+          return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (nmesh == 0) return CDJ_OK;
+  const int hs = hs_flag == 1;
+  if (int rc = batch_plan_build(ctx, meshes, nmesh, hs, mixed, near_factor, nu, lambda, G, st)) return rc;
+  BatchPlan& p = *ctx->batch;
+  // geometry: five N x 3 column-major arrays over the concatenated triangles
+  {
+    std::vector<double> h(15 * (size_t)p.ntot);
+    for (int64_t m = 0; m < nmesh; ++m) {
+      const cdj_mesh& M = meshes[m];
+      const double* src[5] = {M.P1, M.P2, M.P3, M.normals, M.midpoints};
+      for (int a = 0; a < 5; ++a)
+        for (int c = 0; c < 3; ++c)
+          std::memcpy(&h[(size_t)a * 3 * p.ntot + (size_t)c * p.ntot + p.mh[m].tri0], src[a] + c * M.n,
+                      sizeof(double) * M.n);
+    }
+    HIPCHK(hipMemcpyAsync(p.geo.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  if (!p.exec) {
+    if (int rc = batch_tune(ctx, p, st)) return rc;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = batch_enqueue(p, st);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(st, &g);
+    if (ce != hipSuccess) return hip_fail(ctx, ce, "batch capture");
+    if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+    p.graph = g;
+    HIPCHK(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  HIPCHK(hipGraphLaunch(p.exec, st));
+  HIPCHK(hipEventRecord(e1, st));
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(cnt, p.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ams = 0.f;
+  (void)hipEventElapsedTime(&ams, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->near_entries = (int64_t)cnt[2];
+  // solves, mesh by mesh, on the same stream
+  double solve_ms = 0.0, worst = 0.0;
+  int iters = 0, path = CDJ_PREC_FP64;
+  for (int64_t m = 0; m < nmesh; ++m) {
+    const cdj_mesh& M = meshes[m];
+    const int64_t N3 = 3 * M.n;
+    double* Am = p.A.as<double>() + p.a_off[m];
+    double* Bm = p.B.as<double>() + p.b_off[m];
+    double* Xm = p.X.as<double>() + p.b_off[m];
+    HIPCHK(hipMemcpyAsync(Bm, M.b, sizeof(double) * N3 * M.nrhs, hipMemcpyHostToDevice, st));
+    cdj_report r{};
+    int rc;
+    if (opts && opts->precision == CDJ_PREC_MIXED32) {
+      rc = solve_mixed32(ctx, Am, N3, N3, Bm, Xm, M.nrhs, opts, &r, st);
+    } else {
+      // [A | B] in place: the B columns follow A in the plan's buffer (leading dimension 3n)
+      HIPCHK(hipMemcpyAsync(Am + N3 * N3, Bm, sizeof(double) * N3 * M.nrhs, hipMemcpyDeviceToDevice, st));
+      rc = solve_augmented_fp64(ctx, Am, N3, N3, M.nrhs, nullptr, nullptr, &r, st);
+      if (!rc) HIPCHK(hipMemcpyAsync(Xm, Am + N3 * N3, sizeof(double) * N3 * M.nrhs, hipMemcpyDeviceToDevice, st));
+    }
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(M.x, Xm, sizeof(double) * N3 * M.nrhs, hipMemcpyDeviceToHost, st));
+    r.assembly_ms = m == 0 ? ams : 0.0;
+    if (reports) reports[m] = r;
+    solve_ms += r.solve_ms;
+    worst = std::max(worst, r.residual);
+    iters += r.iterations;
+    path = std::max(path, (int)r.path);
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  if (total) {
+    total->path = path;
+    total->iterations = iters;
+    total->residual = worst;
+    total->assembly_ms = ams;
+    total->solve_ms = solve_ms;
+    total->near_entries = (int64_t)cnt[2];
+  }
+  return CDJ_OK;
+}
+
+int cdj_batch_tuning(const cdj_ctx* ctx, int32_t* tpb, int32_t* wgs) {
+  if (!ctx || !ctx->batch) return CDJ_ERR_ARG;
+  if (tpb) *tpb = ctx->batch->tpb;
+  if (wgs) *wgs = ctx->batch->wgs;
   return CDJ_OK;
 }
 

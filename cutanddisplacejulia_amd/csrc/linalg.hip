@@ -49,17 +49,37 @@ template <> struct Mfma<float> {
   __device__ static int row(int g, int v) { return 4 * g + v; }
 };
 
+// Grid-wide barrier of the panel kernel (all workgroups co-resident: cooperative launch).  A monotonically
+// increasing arrival counter; the barrier completes when it reaches `target`.  The wait is bounded: on overrun
+// the error flag is set and the kernel carries on, so it always exits (the host then reports the failure).
+__device__ __forceinline__ void panel_barrier(unsigned* count, unsigned target, int* err) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    long long it = 0;
+    while (__hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++it > (1ll << 24)) {
+        atomicOr(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
+
 template <class T>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
                                                       int32_t* ipiv, int* info, double* cval, int* crow, T* crows,
-                                                      T* rowj) {
+                                                      T* rowj, unsigned* bar, unsigned bar_base, int* bar_err) {
   extern __shared__ unsigned char smem_raw[];
   T* sh = reinterpret_cast<T*>(smem_raw);  // column-major panel rows of this workgroup: sh[c * rpb + r]
   __shared__ T u[LU_NB];                   // pivot row of the current column
   __shared__ double wv[PT / 64];
   __shared__ int wr[PT / 64];
   __shared__ int s_prow;
-  cg::grid_group grid = cg::this_grid();
   const int P = gridDim.x, b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = (int64_t)b * rpb;
   const int64_t r1 = r0 + rpb < m ? r0 + rpb : m;
@@ -108,8 +128,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     }
     if (br >= 0 && tid < nbw) crows[(int64_t)(par * P + b) * LU_NB + tid] = sh[tid * rpb + (br - r0)];
     if (j >= r0 && j < r1 && tid < nbw) rowj[par * LU_NB + tid] = sh[tid * rpb + (j - r0)];
-    __threadfence();
-    grid.sync();
+    panel_barrier(bar, bar_base + (unsigned)(j + 1) * (unsigned)P, bar_err);
     // global pivot: wave 0 reduces the P candidates and stages the pivot row
     if (wave == 0) {
       double gv = -1.0;
@@ -367,6 +386,11 @@ __global__ void d2f_matrix_kernel(const double* A, int64_t lda, float* B, int64_
 
 unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// info = -1 when a panel barrier wait overran (the factors are then invalid)
+__global__ void barrier_status_kernel(const int* bar_err, int* info) {
+  if (*bar_err) *info = -1;
+}
+
 template <class T>
 void gemm_sub(const T* L, int64_t ldl, const T* U, int64_t ldu, T* C, int64_t ldc, int64_t M, int64_t N, int64_t K,
               hipStream_t st) {
@@ -411,7 +435,13 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   T* rowj = reinterpret_cast<T*>(w);
   w += sizeof(double) * 2 * LU_NB;
   int* crow = reinterpret_cast<int*>(w);
+  w += sizeof(int) * 2 * MAX_P;
+  unsigned* bar = reinterpret_cast<unsigned*>(w);  // barrier arrival counter, then its error flag
+  int* bar_err = reinterpret_cast<int*>(bar + 1);
+  unsigned bar_base = 0;
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
   if (e != hipSuccess) return e;
   static bool lds_set = false;  // raise the dynamic LDS limit once per process (128 KiB panels)
   if (!lds_set) {
@@ -434,9 +464,11 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       int rpb = (int)((m + P - 1) / P);
       if (P > MAX_P) return hipErrorInvalidValue;
       size_t shm = sizeof(T) * (size_t)rpb * LU_NB;
-      void* args[] = {&A, (void*)&lda, &k, (void*)&m, (void*)&nbw, &rpb, &ipiv, &info, &cval, &crow, &crows, &rowj};
+      void* args[] = {&A,    (void*)&lda, &k,    (void*)&m, (void*)&nbw, &rpb,     &ipiv,  &info,
+                      &cval, &crow,       &crows, &rowj,    &bar,        &bar_base, &bar_err};
       e = hipLaunchCooperativeKernel((const void*)lu_panel_kernel<T>, dim3(P), dim3(PT), args, shm, st);
       if (e != hipSuccess) return e;
+      bar_base += (unsigned)nbw * (unsigned)P;
       if (ncols > nbw)
         hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 256)), dim3(256), 0, st, A, lda, ncols, k, nbw, ipiv);
     }
@@ -447,7 +479,8 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  hipLaunchKernelGGL(barrier_status_kernel, dim3(1), dim3(1), 0, st, (const int*)bar_err, info);
+  return hipGetLastError();
 }
 
 template <class T>

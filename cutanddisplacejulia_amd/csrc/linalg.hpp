@@ -12,8 +12,9 @@ static constexpr int LU_NB = 64;  // panel width
 size_t lu_workspace_bytes(int64_t n);
 
 // In-place LU of the n x n column-major matrix A (leading dimension lda) with partial pivoting.
-// ipiv (device, length n) receives 0-based pivot rows; info (device int) receives 0 or 1 + the first zero
-// pivot's index.  ws must hold lu_workspace_bytes(n) bytes.  T = double or float.
+// ipiv (device, length n) receives 0-based pivot rows; info (device int) receives 0, 1 + the first zero
+// pivot's index, or -1 if a panel's grid barrier wait overran (factors invalid).  ws must hold
+// lu_workspace_bytes(n) bytes.  T = double or float.
 template <class T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st);
 

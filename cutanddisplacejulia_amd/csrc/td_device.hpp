@@ -777,4 +777,28 @@ __device__ __forceinline__ void disp_pair(const T* __restrict__ rec, T X, T Y, T
   }
 }
 
+// strain of one mode -> normal and shear tractions on the receiver plane (n, s, d): Hooke's law
+// (HookesLaw3DStrain2Stress!.jl:1-66) then CalculateNormalTraction3D!.jl, TractionVectorCartesianComponents3D!.jl
+// and CalculateTractionInChosenDirection3D!.jl as used by CalculateInfluenceMatrices3D.jl:92-110.
+template <class T>
+__device__ __forceinline__ void traction3(const T* e, T lam, T G, const T* n, const T* s, const T* d, T& Tn, T& Tss,
+                                          T& Tds) {
+  const T G2 = T(2) * G;
+  const T cons = lam * (e[0] + e[1] + e[2]);
+  const T Sxx = G2 * e[0] + cons, Syy = G2 * e[1] + cons, Szz = G2 * e[2] + cons;
+  const T Sxy = G2 * e[3], Sxz = G2 * e[4], Syz = G2 * e[5];
+  const T nx = n[0], ny = n[1], nz = n[2];
+  Tn = Sxx * nx * nx + Syy * ny * ny + Szz * nz * nz + T(2) * Sxy * nx * ny + T(2) * Syz * ny * nz +
+       T(2) * Sxz * nz * nx;
+  const T tx = Sxx * nx + Sxy * ny + Sxz * nz;
+  const T ty = Sxy * nx + Syy * ny + Syz * nz;
+  const T tz = Sxz * nx + Syz * ny + Szz * nz;
+  // ((I - n n^T) t) . dir
+  const T px = tx * (T(1) - nx * nx) - ty * (nx * ny) - tz * (nz * nx);
+  const T py = -tx * (nx * ny) + ty * (T(1) - ny * ny) - tz * (ny * nz);
+  const T pz = -tx * (nz * nx) - ty * (ny * nz) + tz * (T(1) - nz * nz);
+  Tss = px * s[0] + py * s[1] + pz * s[2];
+  Tds = px * d[0] + py * d[1] + pz * d[2];
+}
+
 }  // namespace cdj

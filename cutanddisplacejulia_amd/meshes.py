@@ -166,3 +166,32 @@ def growth_sequence(steps=40, n_start=500, n_end=6400, radius=1.0, dip=45.0, jit
                            normal=normal)
         out.append(dict(step=s + 1, P1=P1, P2=P2, P3=P3, g=float(rng.uniform(0.0, 0.5))))
     return out
+
+
+def random_rotation(rng):
+    """Uniformly random rotation matrix (QR of a Gaussian matrix with sign fix)."""
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q @ np.diag(np.sign(np.diag(r)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def elliptical_cracks(count=64, n_min=200, n_max=3000, box=10.0, seed=BASE_SEED + 7, rhs_min=1, rhs_max=8):
+    """Batch of planar elliptical cracks: semi-axes (aspect, 1) with aspect ~ U(1, 4), uniformly random orientation,
+    centre ~ U(-box, box)^3, triangle count ~ U(n_min, n_max) (ring meshes: 6 r^2 triangles for r rings), and
+    1-8 remote stress tensors per crack with N(0, 1) entries (symmetric).  Returns a list of dicts with P1, P2,
+    P3 and `stresses` (k x 6: sxx, syy, szz, sxy, sxz, syz)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(count):
+        target = rng.uniform(n_min, n_max)
+        rings = max(2, int(round(np.sqrt(target / 6.0))))
+        aspect = rng.uniform(1.0, 4.0)
+        R = random_rotation(rng)
+        centre = rng.uniform(-box, box, 3)
+        P1, P2, P3 = penny(rings, 1.0, seed=seed + 100 + c, aspect=aspect)
+        P1, P2, P3 = (P @ R.T + centre for P in (P1, P2, P3))
+        k = int(rng.integers(rhs_min, rhs_max + 1))
+        out.append(dict(P1=P1, P2=P2, P3=P3, stresses=rng.normal(size=(k, 6)), aspect=aspect))
+    return out

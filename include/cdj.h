@@ -169,6 +169,33 @@ int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const do
                           const double* P3, int64_t n, double G, const double* A, int64_t lda, const double* b,
                           double* x, const cdj_opts* opts, cdj_report* report, void* stream);
 
+/* ---- batched slip solves (no single counterpart in the reference: one SlipCalculator3D base-method call,
+ * SlipCalculator3D.jl:1-33, per mesh, with several right-hand sides each) ----
+ * All meshes' collocation matrices are assembled in one persistent work-queue launch per pass (FP64, or the
+ * FP32 far / FP64 near passes under opts->assembly = CDJ_ASM_MIXED), captured in a HIP graph together with the
+ * per-triangle set-up; the source-chunk width and the number of persistent workgroups are chosen by timing the
+ * candidates on the first batch of a shape (cached per half-space flag, mode and size).  A later call with the
+ * same shape reuses the buffers and replays the graph.  Each mesh is then solved as cdj_slip_calculator would
+ * (opts->precision).  Meshes must all be full space or all half space (hs_flag). */
+typedef struct cdj_mesh {
+  int64_t n;                   /* triangles */
+  const double* P1;            /* n x 3, column-major */
+  const double* P2;
+  const double* P3;
+  const double* normals;       /* n x 3 (CreateFaceNormalAndMidPoint) */
+  const double* midpoints;     /* n x 3 */
+  int64_t nrhs;                /* right-hand sides, >= 1 */
+  const double* b;             /* 3n x nrhs, [Tn; Tss; Tds] per column */
+  double* x;                   /* 3n x nrhs output, [Dn; Dss; Dds] */
+} cdj_mesh;
+
+int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double nu, double G, double lambda,
+                     int32_t hs_flag, const cdj_opts* opts, cdj_report* reports /* nmesh or NULL */,
+                     cdj_report* total /* or NULL */);
+
+/* Tile chosen by the autotuner for the current batch plan: sources per work item and persistent workgroups. */
+int cdj_batch_tuning(const cdj_ctx* ctx, int32_t* tpb, int32_t* wgs);
+
 #ifdef __cplusplus
 }
 #endif
