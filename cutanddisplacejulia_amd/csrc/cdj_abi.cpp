@@ -971,6 +971,14 @@ int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const do
     }
     return spai_apply(jac.as<double>(), Y, g.f, invG, r, z, st);
   };
+  // CDJ_PREC_MIXED32: Arnoldi products on an FP32 copy of A, FP64 residual at every restart
+  DBuf A32;
+  if (opts && opts->precision == CDJ_PREC_MIXED32) {
+    HIPCHK(A32.alloc(sizeof(float) * N3 * N3));
+    convert_matrix_d2f(A, lda, A32.as<float>(), N3, N3, N3, st);
+    go.A32 = A32.as<float>();
+    go.lda32 = N3;
+  }
   GmresResult gr;
   HIPCHK(fgmres(A, N3, lda, M, b, x, go, gr, st));
   HIPCHK(hipEventRecord(e1, st));

@@ -72,31 +72,57 @@ __global__ void geom_kernel(const double* __restrict__ P1, const double* __restr
 }
 
 // k nearest centroids (itself first), ties broken by index; brute force, one thread per triangle.
-__global__ void knn_kernel(const float* __restrict__ geo, int64_t n, int k, int32_t* __restrict__ nbr) {
+// k nearest centroids of each element (itself first, distance -1), K a compile-time bound so the sorted lists
+// stay in registers: candidates are staged through LDS 256 at a time and inserted by an unrolled compare-shift.
+template <int K>
+__global__ void __launch_bounds__(256) knn_kernel(const float* __restrict__ geo, int64_t n, int k,
+                                                  int32_t* __restrict__ nbr) {
+  __shared__ float cx[256], cy[256], cz[256];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float bd[KNN_MAX];
-  int bi[KNN_MAX];
-  for (int q = 0; q < k; ++q) {
+  const bool valid = i < n;
+  float bd[K];
+  int bi[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
     bd[q] = 3.0e38f;
     bi[q] = -1;
   }
-  const float cx = geo[i * GEO], cy = geo[i * GEO + 1], cz = geo[i * GEO + 2];
-  for (int64_t j = 0; j < n; ++j) {
-    const float dx = geo[j * GEO] - cx, dy = geo[j * GEO + 1] - cy, dz = geo[j * GEO + 2] - cz;
-    const float dd = j == i ? -1.0f : dx * dx + dy * dy + dz * dz;
-    if (dd < bd[k - 1]) {
-      int q = k - 1;
-      while (q > 0 && bd[q - 1] > dd) {
-        bd[q] = bd[q - 1];
-        bi[q] = bi[q - 1];
-        --q;
-      }
-      bd[q] = dd;
-      bi[q] = (int)j;
+  const float px = valid ? geo[i * GEO] : 0.f, py = valid ? geo[i * GEO + 1] : 0.f, pz = valid ? geo[i * GEO + 2] : 0.f;
+  for (int64_t t0 = 0; t0 < n; t0 += 256) {
+    const int64_t jt = t0 + threadIdx.x;
+    if (jt < n) {
+      cx[threadIdx.x] = geo[jt * GEO];
+      cy[threadIdx.x] = geo[jt * GEO + 1];
+      cz[threadIdx.x] = geo[jt * GEO + 2];
     }
+    __syncthreads();
+    const int cnt = (int)(n - t0 < 256 ? n - t0 : 256);
+    if (valid)
+      for (int c = 0; c < cnt; ++c) {
+        const int64_t j = t0 + c;
+        const float dx = cx[c] - px, dy = cy[c] - py, dz = cz[c] - pz;
+        const float dd = j == i ? -1.0f : dx * dx + dy * dy + dz * dz;
+        if (dd < bd[K - 1]) {
+          // insert into the ascending list (ties keep the earlier index first)
+#pragma unroll
+          for (int q = K - 1; q > 0; --q) {
+            const bool shift = bd[q - 1] > dd;
+            const bool here = !shift && bd[q] > dd;
+            bi[q] = shift ? bi[q - 1] : (here ? (int)j : bi[q]);
+            bd[q] = shift ? bd[q - 1] : (here ? dd : bd[q]);
+          }
+          if (bd[0] > dd) {
+            bd[0] = dd;
+            bi[0] = (int)j;
+          }
+        }
+      }
+    __syncthreads();
   }
-  for (int q = 0; q < k; ++q) nbr[i * k + q] = bi[q] >= 0 ? bi[q] : (int)i;
+  if (valid)
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+      if (q < k) nbr[i * k + q] = bi[q] >= 0 ? bi[q] : (int)i;
 }
 
 // features[i][q*NF + f]: neighbour q of i in i's frame: relative centroid / h_i (3), normal (3),
@@ -420,7 +446,10 @@ hipError_t mesh_prepare(const double* P1, const double* P2, const double* P3, in
   if ((e = hipMalloc(&f.nbr, sizeof(int32_t) * k * n))) return e;
   if ((e = hipMalloc(&f.feat, sizeof(float) * (int64_t)k * NF * n))) return e;
   hipLaunchKernelGGL(geom_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, P1, P2, P3, n, f.geo);
-  hipLaunchKernelGGL(knn_kernel, dim3(nblk(n, 64)), dim3(64), 0, st, (const float*)f.geo, n, k, f.nbr);
+  if (k <= 16)
+    hipLaunchKernelGGL(knn_kernel<16>, dim3(nblk(n, 256)), dim3(256), 0, st, (const float*)f.geo, n, k, f.nbr);
+  else
+    hipLaunchKernelGGL(knn_kernel<KNN_MAX>, dim3(nblk(n, 256)), dim3(256), 0, st, (const float*)f.geo, n, k, f.nbr);
   hipLaunchKernelGGL(features_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, (const float*)f.geo,
                      (const int32_t*)f.nbr, n, k, f.feat);
   return hipGetLastError();

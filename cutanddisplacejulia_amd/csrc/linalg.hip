@@ -50,36 +50,40 @@ template <> struct Mfma<float> {
 };
 
 // Grid-wide barrier of the panel kernel (all workgroups co-resident: cooperative launch).  A monotonically
-// increasing arrival counter; the barrier completes when it reaches `target`.  The wait is bounded: on overrun
-// the error flag is set and the kernel carries on, so it always exits (the host then reports the failure).
+// increasing arrival counter; the barrier completes when it reaches `target`.  The workgroup barrier first
+// completes this workgroup's stores; thread 0's agent-scope release (on the counter) then publishes them and its
+// acquire fence after the wait makes the other workgroups' stores visible to the whole workgroup, so only one
+// thread per workgroup pays for the device-wide cache maintenance.  The wait is bounded: on overrun the error flag
+// is set and the kernel carries on, so it always exits (the host then reports the failure).
 __device__ __forceinline__ void panel_barrier(unsigned* count, unsigned target, int* err) {
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     long long it = 0;
-    while (__hip_atomic_load(count, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (++it > (1ll << 24)) {
         atomicOr(err, 1);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __threadfence();
 }
 
 template <class T>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
                                                       int32_t* ipiv, int* info, double* cval, int* crow, T* crows,
-                                                      T* rowj, unsigned* bar, unsigned bar_base, int* bar_err) {
+                                                      T* rowj, unsigned* bar, unsigned bar_base, int* bar_err,
+                                                      int* perm) {
   extern __shared__ unsigned char smem_raw[];
   T* sh = reinterpret_cast<T*>(smem_raw);  // column-major panel rows of this workgroup: sh[c * rpb + r]
   __shared__ T u[LU_NB];                   // pivot row of the current column
   __shared__ double wv[PT / 64];
   __shared__ int wr[PT / 64];
   __shared__ int s_prow;
+  __shared__ int s_piv[LU_NB];  // block 0: this panel's pivot rows (relative to k)
   const int P = gridDim.x, b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = (int64_t)b * rpb;
   const int64_t r1 = r0 + rpb < m ? r0 + rpb : m;
@@ -156,6 +160,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
       if (lane == 0) {
         s_prow = gr >= 0 ? gr : j;
         if (b == 0) {
+          s_piv[j] = gr >= 0 ? gr : j;
           ipiv[k + j] = (int32_t)(k + (gr >= 0 ? gr : j));
           if (!(gv > 0.0) && *info == 0) *info = (int)(k + j + 1);
         }
@@ -180,52 +185,96 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
   }
   if (mine)
     for (int c = 0; c < nbw; ++c) A[(k + grow) + (k + c) * lda] = sh[c * rpb + tid];
-}
-
-// Row interchanges of one panel applied to columns [0, k) and [k + nbw, ncols).
-template <class T>
-__global__ void laswp_kernel(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw, const int32_t* __restrict__ ipiv) {
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (c >= ncols || (c >= k && c < k + nbw)) return;
-  T* col = A + c * lda;
-  for (int j = 0; j < nbw; ++j) {
-    const int64_t p = ipiv[k + j];
-    if (p != k + j) {
-      const T t = col[k + j];
-      col[k + j] = col[p];
-      col[p] = t;
+  // block 0, wave 0: the panel's swaps as one row permutation of the touched rows (rows j < nbw and the pivot
+  // rows beyond the panel): perm[0] = count, then (destination, source) pairs, relative to k
+  if (b == 0 && wave == 0) {
+    __shared__ int touched[2 * LU_NB], cur[2 * LU_NB];
+    __shared__ int s_cnt;
+    if (lane < nbw) {
+      touched[lane] = lane;
+      cur[lane] = lane;
     }
+    if (lane == 0) s_cnt = nbw;
+    __builtin_amdgcn_wave_barrier();
+    for (int j = 0; j < nbw; ++j) {
+      const int p = s_piv[j];
+      int cnt = s_cnt;
+      // index of row p among the touched rows (rows < nbw are at their own index)
+      int at = p < nbw ? p : -1;
+      if (at < 0) {
+        const bool hit = lane + nbw < cnt && touched[lane + nbw] == p;
+        const unsigned long long m = __ballot(hit);
+        if (m) at = nbw + __ffsll((long long)m) - 1;
+      }
+      if (at < 0) {
+        at = cnt;
+        if (lane == 0) {
+          touched[cnt] = p;
+          cur[cnt] = p;
+          s_cnt = cnt + 1;
+        }
+      }
+      if (lane == 0 && at != j) {
+        const int t = cur[j];
+        cur[j] = cur[at];
+        cur[at] = t;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    const int cnt = s_cnt;
+    for (int t = lane; t < cnt; t += 64) {
+      perm[1 + 2 * t] = touched[t];
+      perm[2 + 2 * t] = cur[t];
+    }
+    if (lane == 0) perm[0] = cnt;
   }
 }
 
-// A[k:k+nbw, c0:c0+ncols] = L11^-1 A[k:k+nbw, c0:...] with L11 unit lower (the panel's diagonal block).
+// Row interchanges of one panel applied to columns [0, k) and [k + nbw, ncols), as the permutation of the
+// touched rows emitted by the panel (perm = {count, (dst, src) pairs} relative to k).  Two columns per block,
+// 128 threads each: all reads of a column complete before any write.
+template <class T>
+__global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw,
+                                                    const int* __restrict__ perm) {
+  const int64_t c = (int64_t)blockIdx.x * 2 + threadIdx.x / 128;
+  const int t = threadIdx.x % 128;
+  const int cnt = perm[0];
+  const bool act = c < ncols && !(c >= k && c < k + nbw) && t < cnt;
+  T v = T(0);
+  int dst = 0;
+  if (act) {
+    dst = perm[1 + 2 * t];
+    v = A[(k + perm[2 + 2 * t]) + c * lda];
+  }
+  __syncthreads();
+  if (act) A[(k + dst) + c * lda] = v;
+}
+
+// A[k:k+nbw, c0:c0+ncols] = L11^-1 A[k:k+nbw, c0:...] with L11 unit lower (the panel's diagonal block).  One
+// wave per column at a time (TRSM_CPW columns per wave): lane r holds x[r]; step q broadcasts x[q] and the lanes
+// below update with L11 read from LDS (stored transposed, so a step's reads are consecutive).
+constexpr int TRSM_CPW = 8;
+
 template <class T>
 __global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t ncols) {
-  __shared__ T L[LU_NB][LU_NB + 1];
-  __shared__ T X[LU_NB][64 + 1];
-  const int tid = threadIdx.x;
-  const int64_t cb = c0 + (int64_t)blockIdx.x * 64;
-  const int nc = (int)std::min<int64_t>(64, c0 + ncols - cb);
-  for (int idx = tid; idx < nbw * nbw; idx += 256) {
-    const int r = idx % nbw, c = idx / nbw;
-    L[r][c] = A[(k + r) + (k + c) * lda];
-  }
-  for (int idx = tid; idx < nbw * nc; idx += 256) {
-    const int r = idx % nbw, c = idx / nbw;
-    X[r][c] = A[(k + r) + (cb + c) * lda];
+  __shared__ T LT[LU_NB][LU_NB + 1];  // LT[q][r] = L11[r][q]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx % LU_NB, q = idx / LU_NB;
+    LT[q][r] = (r < nbw && q < nbw) ? A[(k + r) + (k + q) * lda] : T(0);
   }
   __syncthreads();
-  if (tid < nc) {
-    for (int r = 1; r < nbw; ++r) {
-      T s = X[r][tid];
-      for (int q = 0; q < r; ++q) s -= L[r][q] * X[q][tid];
-      X[r][tid] = s;
+  const int64_t cb = c0 + ((int64_t)blockIdx.x * 4 + wave) * TRSM_CPW;
+  for (int i = 0; i < TRSM_CPW; ++i) {
+    const int64_t c = cb + i;
+    if (c >= c0 + ncols) break;
+    T* col = A + k + c * lda;
+    T x = lane < nbw ? col[lane] : T(0);
+    for (int q = 0; q < nbw - 1; ++q) {
+      const T xq = __shfl(x, q);
+      if (lane > q) x -= LT[q][lane] * xq;
     }
-  }
-  __syncthreads();
-  for (int idx = tid; idx < nbw * nc; idx += 256) {
-    const int r = idx % nbw, c = idx / nbw;
-    A[(k + r) + (cb + c) * lda] = X[r][c];
+    if (lane < nbw) col[lane] = x;
   }
 }
 
@@ -320,13 +369,11 @@ __global__ void __launch_bounds__(256) trsv_diag_kernel(const T* __restrict__ A,
   T* x = B + q * ldb + r0;
   T xi = lane < nb ? x[lane] : T(0);
   if (LOWER) {
-#pragma unroll 8
     for (int r = 0; r < nb - 1; ++r) {
       const T xr = __shfl(xi, r);
       if (lane > r && lane < nb) xi -= D[lane][r] * xr;
     }
   } else {
-#pragma unroll 8
     for (int r = nb - 1; r >= 0; --r) {
       if (lane == r) xi = xi / D[r][r];
       const T xr = __shfl(xi, r);
@@ -406,7 +453,7 @@ void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t n
   if (nc <= 0) return;
   for (int64_t s = r0; s < r1; s += LU_NB) {
     const int nb = (int)std::min<int64_t>(LU_NB, r1 - s);
-    hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nc, 64)), dim3(256), 0, st, A, lda, s, nb, c0, nc);
+    hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nc, 4 * TRSM_CPW)), dim3(256), 0, st, A, lda, s, nb, c0, nc);
     gemm_sub<T>(A + (s + nb) + s * lda, lda, A + s + c0 * lda, lda, A + (s + nb) + c0 * lda, lda, r1 - s - nb, nc, nb,
                 st);
   }
@@ -416,7 +463,8 @@ void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t n
 
 size_t lu_workspace_bytes(int64_t n) {
   (void)n;
-  return sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) + sizeof(int) * 2 * MAX_P + 1024;
+  return sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) + sizeof(int) * 2 * MAX_P +
+         sizeof(int) * (2 + 1 + 4 * LU_NB) + 1024;
 }
 
 // Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
@@ -438,6 +486,7 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   w += sizeof(int) * 2 * MAX_P;
   unsigned* bar = reinterpret_cast<unsigned*>(w);  // barrier arrival counter, then its error flag
   int* bar_err = reinterpret_cast<int*>(bar + 1);
+  int* perm = bar_err + 1;                          // 1 + 4 LU_NB ints: the last panel's row permutation
   unsigned bar_base = 0;
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
   if (e != hipSuccess) return e;
@@ -464,13 +513,14 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       int rpb = (int)((m + P - 1) / P);
       if (P > MAX_P) return hipErrorInvalidValue;
       size_t shm = sizeof(T) * (size_t)rpb * LU_NB;
-      void* args[] = {&A,    (void*)&lda, &k,    (void*)&m, (void*)&nbw, &rpb,     &ipiv,  &info,
-                      &cval, &crow,       &crows, &rowj,    &bar,        &bar_base, &bar_err};
+      void* args[] = {&A,    (void*)&lda, &k,    (void*)&m, (void*)&nbw, &rpb,     &ipiv,    &info,
+                      &cval, &crow,       &crows, &rowj,    &bar,        &bar_base, &bar_err, &perm};
       e = hipLaunchCooperativeKernel((const void*)lu_panel_kernel<T>, dim3(P), dim3(PT), args, shm, st);
       if (e != hipSuccess) return e;
       bar_base += (unsigned)nbw * (unsigned)P;
       if (ncols > nbw)
-        hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 256)), dim3(256), 0, st, A, lda, ncols, k, nbw, ipiv);
+        hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 2)), dim3(256), 0, st, A, lda, ncols, k, nbw,
+                           (const int*)perm);
     }
     // U12 of the block, then the rank-(K1 - K0) update of everything right of and below it
     block_trsm<T>(A, lda, K0, K1, K1, ncols - K1, st);

@@ -115,13 +115,11 @@ __global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restric
   if (tid < 64) {
     T xi = tid < nr ? rhs[tid] : T(0);
     if (LOWER) {
-#pragma unroll 8
       for (int r = 0; r < nr - 1; ++r) {
         const T xr = __shfl(xi, r);
         if (tid > r && tid < nr) xi -= D[tid][r] * xr;
       }
     } else {
-#pragma unroll 8
       for (int r = nr - 1; r >= 0; --r) {
         if (tid == r) xi = xi / D[r][r];
         const T xr = __shfl(xi, r);
@@ -344,7 +342,10 @@ hipError_t fgmres(const double* A, int64_t n, int64_t lda, const Precond& M, con
       // z_k = M^-1 v_k ; w = A z_k
       e = M(V + (int64_t)k * n, Z + (int64_t)k * n);
       if (e) return e;
-      gemv_splitk<double>(A, n, n, lda, Z + (int64_t)k * n, w, 1.0, nullptr, 0.0, part, st);
+      if (o.A32)
+        gemv_splitk<float>(o.A32, n, n, o.lda32, Z + (int64_t)k * n, w, 1.0, nullptr, 0.0, part, st);
+      else
+        gemv_splitk<double>(A, n, n, lda, Z + (int64_t)k * n, w, 1.0, nullptr, 0.0, part, st);
       // classical Gram-Schmidt, twice (CGS2) for FP64 orthogonality
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(mdot_part_kernel, dim3(DOT_BLOCKS), dim3(256), 0, st, (const double*)V, (int64_t)n, k + 1,
@@ -388,7 +389,7 @@ hipError_t fgmres(const double* A, int64_t n, int64_t lda, const Precond& M, con
       gvec[k + 1] = -sn[k] * gvec[k];
       gvec[k] = cs[k] * gvec[k];
       const double est = std::fabs(gvec[k + 1]) / bnorm;
-      if (est <= o.tol * 0.5 || hh == 0.0) {
+      if (est <= o.tol * 0.5 || hh == 0.0 || (o.A32 && std::fabs(gvec[k + 1]) <= o.inner_rel * beta)) {
         ++k;
         break;
       }

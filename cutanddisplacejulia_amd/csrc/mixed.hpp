@@ -22,6 +22,12 @@ struct GmresOpts {
   int max_restarts = 20;
   long long spin_limit = 1LL << 26;
   bool zero_guess = false;  // start from x = 0 instead of x = M^-1 b
+  // FP32 copy of A for the Arnoldi products (nullptr: FP64 A).  The residual at every restart stays FP64, so a
+  // cycle on the FP32 operator is one step of iterative refinement; each cycle then stops once its estimated
+  // residual has fallen by inner_rel (the FP32 operator cannot resolve more) or reached the target.
+  const float* A32 = nullptr;
+  int64_t lda32 = 0;
+  double inner_rel = 1e-6;
 };
 
 // Preconditioner application z = M^-1 r on device vectors of length n (enqueued on the solver's stream).

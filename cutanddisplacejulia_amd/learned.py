@@ -116,10 +116,13 @@ class LearnedPreconditioner:
         return loss.value, grad, feat, nbr, geo
 
 
-def precond_solve_dev(dA, lda, db, dx, dP1, dP2, dP3, n, G, pc="learned", model=None, tol=1e-10, max_iter=0):
-    """FP64 FGMRES on device A (3n x 3n) with the named preconditioner; returns the report dict."""
+def precond_solve_dev(dA, lda, db, dx, dP1, dP2, dP3, n, G, pc="learned", model=None, tol=1e-10, max_iter=0,
+                      krylov32=False):
+    """FGMRES on device A (3n x 3n) with the named preconditioner to an FP64 relative residual `tol`; returns the
+    report dict.  krylov32: Arnoldi products on an FP32 copy of A, FP64 residual at every restart."""
     rep = Report()
-    opts = Opts(precision=_lib.CDJ_PREC_LEARNED, tol=float(tol), max_iter=int(max_iter), near_factor=2.0)
+    opts = Opts(precision=_lib.CDJ_PREC_MIXED32 if krylov32 else _lib.CDJ_PREC_LEARNED, tol=float(tol),
+                max_iter=int(max_iter), near_factor=2.0)
     check(_bind().cdj_precond_solve_dev(ctx(), model.h if model is not None else None, PC[pc], _dptr(dP1),
                                         _dptr(dP2), _dptr(dP3), int(n), float(G), _dptr(dA), int(lda), _dptr(db),
                                         _dptr(dx), ctypes.byref(opts), ctypes.byref(rep), _stream()))

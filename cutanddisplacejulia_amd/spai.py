@@ -85,7 +85,10 @@ def _timed(fn):
     return out, 1e3 * (time.perf_counter() - t0)
 
 
-def evaluate_step(step, model, hs=0, pcs=("none", "jacobi", "learned", "lu32"), tol=1e-10):
+PCS = ("none", "jacobi", "learned", "learned_k32", "lu32")
+
+
+def evaluate_step(step, model, hs=0, pcs=PCS, tol=1e-10):
     """Solve one step with the plain FP64 path (assembly + LU) and the mixed / preconditioned paths.
 
     Returns a dict of wall times (ms, device work synchronised), iterations, FP64 residuals (against the matrix
@@ -125,15 +128,16 @@ def evaluate_step(step, model, hs=0, pcs=("none", "jacobi", "learned", "lu32"), 
             out["near_entries"] = int(L.cdj_last_near_entries(ctx))
             out["mixed_assembly_ms"] = t_asm
             out["mixed_matrix_diff"] = None
-        for pc in pcs:
+        for name in pcs:
+            pc, k32 = (name[:-4], True) if name.endswith("_k32") else (name, False)
             if pc == "learned" and model is None:
                 continue
             x = torch.zeros_like(b)
             rep, t = _timed(lambda: learned.precond_solve_dev(A0, N3, b, x, d["P1"], d["P2"], d["P3"], n, G, pc=pc,
-                                                              model=model, tol=tol))
+                                                              model=model, tol=tol, krylov32=k32))
             res = float(torch.linalg.norm(b - torch.mv(A0.t(), x)) / torch.linalg.norm(b))
             fwd = float(torch.linalg.norm(x - xref) / torch.linalg.norm(xref))
-            key = ("mixed_" if mixed else "") + pc
+            key = ("mixed_" if mixed else "") + name
             out[key] = {"solve_ms": t, "iterations": rep["iterations"], "path": rep["path"], "residual": res,
                         "forward_err_vs_fp64": fwd,
                         "total_ms": t + (out["mixed_assembly_ms"] if mixed else out["fp64"]["assembly_ms"])}
