@@ -31,6 +31,10 @@ METRIC = "TD influence entries/s; slip-solve time at 1e-10 resid, mixed/learned 
 FLOPS_PER_PAIR_HS = 43940.0
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the assembly kernel on the 5000-triangle workload: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE in separate runs, each scaled by what the counter reads for a 1 GiB device copy (fetch x2.0,
+# write x1.0): 9.67e7 B read + 2.073e9 B written.
+TRAFFIC_BYTES_HS_5000 = 2.169e9
 
 
 def _log(msg):
@@ -209,7 +213,8 @@ def main():
         "config": {"workload": "half-space collocation assembly: 45-degree dipping fault, %d triangles, "
                                "A %d x %d FP64" % (n, N3, N3), "n_tri": n, "hs": 1},
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tf / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved_tf / PEAK_FP64_TFLOPS,
+                     "traffic": TRAFFIC_BYTES_HS_5000 if args.n == 50 else None,
                      "kernel": "tdk_traction<double, double, true>", "kernel_ms": kavg,
                      "flops_per_pair": FLOPS_PER_PAIR_HS, "pairs_per_launch": pairs},
         "cpu_baseline": cpu,

@@ -29,6 +29,7 @@ CDJ_ERR_SINGULAR = 8
 CDJ_PREC_FP64 = 0
 CDJ_PREC_MIXED32 = 1
 CDJ_PREC_LEARNED = 2
+CDJ_PREC_MIXED16 = 3
 CDJ_ASM_FP64 = 0
 CDJ_ASM_MIXED = 1
 

@@ -128,12 +128,14 @@ _MODES = {
     "mixed32": (_lib.CDJ_PREC_MIXED32, _lib.CDJ_ASM_FP64),
     # FP32 far field / FP64 near field assembly, then as "mixed32"; the residual is against that matrix
     "mixed": (_lib.CDJ_PREC_MIXED32, _lib.CDJ_ASM_MIXED),
+    # FP64 assembly; LU with FP16-operand trailing updates as preconditioner of FP64 GMRES-IR
+    "mixed16": (_lib.CDJ_PREC_MIXED16, _lib.CDJ_ASM_FP64),
 }
 
 
 def set_precision(mode, near_factor=None):
-    """Select the arithmetic of CalculateInfluenceMatrices3D / SlipCalculator3D: 'fp64' (default), 'mixed32' or
-    'mixed' (see _MODES).  near_factor: FP64 radius in longest triangle edges about each source centroid."""
+    """Select the arithmetic of CalculateInfluenceMatrices3D / SlipCalculator3D: 'fp64' (default), 'mixed32',
+    'mixed16' or 'mixed' (see _MODES).  near_factor: FP64 radius in longest triangle edges about each source centroid."""
     _SETTINGS["precision"], _SETTINGS["assembly"] = _MODES[mode]
     if near_factor is not None:
         _SETTINGS["near_factor"] = float(near_factor)

@@ -604,7 +604,8 @@ int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
   HIPCHK(ws.alloc(lu_workspace_bytes(n)));
   HIPCHK(info.alloc(sizeof(int)));
   convert_matrix_d2f(A, lda, A32.as<float>(), n, n, n, st);
-  HIPCHK(lu_factor<float>(A32.as<float>(), n, n, ipiv.as<int32_t>(), info.as<int>(), ws.p, st));
+  const bool fp16 = opts && opts->precision == CDJ_PREC_MIXED16;
+  HIPCHK(lu_factor<float>(A32.as<float>(), n, n, ipiv.as<int32_t>(), info.as<int>(), ws.p, st, fp16));
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(lu_permutation(ipiv.as<int32_t>(), n, perm.as<int32_t>(), st));
@@ -626,7 +627,7 @@ int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
   (void)hipEventDestroy(e1);
   if (ok) {
     if (report) {
-      report->path = CDJ_PREC_MIXED32;
+      report->path = fp16 ? CDJ_PREC_MIXED16 : CDJ_PREC_MIXED32;
       report->iterations = iters;
       report->residual = worst;
       report->solve_ms = ms;
@@ -666,7 +667,7 @@ int cdj_solve(cdj_ctx* ctx, const double* A, const double* b, int64_t n, int64_t
   hipStream_t st = ctx->stream;
   if (n == 0 || nrhs == 0) return CDJ_OK;
   const int64_t lda = n;
-  if (opts && opts->precision == CDJ_PREC_MIXED32) {
+  if (opts && (opts->precision == CDJ_PREC_MIXED32 || opts->precision == CDJ_PREC_MIXED16)) {
     DBuf dA, dB, dX;
     HIPCHK(dA.alloc(sizeof(double) * n * n));
     HIPCHK(dB.alloc(sizeof(double) * n * nrhs));
@@ -733,7 +734,7 @@ int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const 
   HIPCHK(hipEventRecord(e1, st));
   HIPCHK(hipMemcpyAsync(AB.as<double>() + N3 * lda, b, sizeof(double) * N3 * nrhs, hipMemcpyHostToDevice, st));
   if (A_out) HIPCHK(hipMemcpyAsync(A_out, AB.p, sizeof(double) * N3 * N3, hipMemcpyDeviceToHost, st));
-  const bool mixed = opts && opts->precision == CDJ_PREC_MIXED32;
+  const bool mixed = opts && (opts->precision == CDJ_PREC_MIXED32 || opts->precision == CDJ_PREC_MIXED16);
   if (report && !mixed) {
     HIPCHK(Ao.alloc(sizeof(double) * N3 * N3));
     HIPCHK(Bo.alloc(sizeof(double) * N3 * nrhs));
@@ -1282,7 +1283,7 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
     HIPCHK(hipMemcpyAsync(Bm, M.b, sizeof(double) * N3 * M.nrhs, hipMemcpyHostToDevice, st));
     cdj_report r{};
     int rc;
-    if (opts && opts->precision == CDJ_PREC_MIXED32) {
+    if (opts && (opts->precision == CDJ_PREC_MIXED32 || opts->precision == CDJ_PREC_MIXED16)) {
       rc = solve_mixed32(ctx, Am, N3, N3, Bm, Xm, M.nrhs, opts, &r, st);
     } else {
       // [A | B] in place: the B columns follow A in the plan's buffer (leading dimension 3n)

@@ -351,6 +351,98 @@ __global__ void __launch_bounds__(256) gemm_sub_kernel(const T* __restrict__ L, 
     }
 }
 
+// FP16 variant of the trailing update for FP32 factors: C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N] with both
+// operands rounded to FP16 and the products accumulated in FP32 on v_mfma_f32_16x16x16f16.  |L| <= 1 under
+// partial pivoting; each column c of U is scaled by a power of two sigma[c] (max |U[:, c]| * sigma in [0.5, 1))
+// before rounding, and the accumulated product divided by it, so FP16's range never clips the update.
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+constexpr int GH_K = 32;
+
+__global__ void __launch_bounds__(256) gemm_sub_f16_kernel(const float* __restrict__ L, int64_t ldl,
+                                                           const float* __restrict__ U, int64_t ldu,
+                                                           float* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
+                                                           int64_t K, const float* __restrict__ sigma) {
+  __shared__ _Float16 Ls[GB_M][GH_K + 4];  // Ls[i][k] = L[i0 + i, k0 + k]
+  __shared__ _Float16 Us[GB_N][GH_K + 4];  // Us[j][k] = sigma[j0 + j] U[k0 + k, j0 + j]
+  const int tid = threadIdx.x;
+  const int64_t i0 = (int64_t)blockIdx.x * GB_M, j0 = (int64_t)blockIdx.y * GB_N;
+  const int wave = tid / 64, lane = tid % 64;
+  const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
+  const int li = lane % 16, lk = lane / 16;
+  constexpr int PER = GB_M * GH_K / 256;  // 16 elements of each operand per thread
+  float lr[PER], ur[PER];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + 256 * e;
+      const int i = idx % GB_M, kk = idx / GB_M;
+      lr[e] = (i0 + i < M && k0 + kk < K) ? L[(i0 + i) + (k0 + kk) * ldl] : 0.f;
+      const int kq = idx % GH_K, j = idx / GH_K;
+      ur[e] = (k0 + kq < K && j0 + j < N) ? U[(k0 + kq) + (j0 + j) * ldu] * sigma[j0 + j] : 0.f;
+    }
+  };
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b2 = 0; b2 < 4; ++b2) acc[a][b2] = f4{0.f, 0.f, 0.f, 0.f};
+  fetch(0);
+  for (int64_t k0 = 0; k0 < K; k0 += GH_K) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int idx = tid + 256 * e;
+      Ls[idx % GB_M][idx / GB_M] = (_Float16)lr[e];
+      Us[idx / GH_K][idx % GH_K] = (_Float16)ur[e];
+    }
+    __syncthreads();
+    if (k0 + GH_K < K) fetch(k0 + GH_K);
+#pragma unroll
+    for (int kk = 0; kk < GH_K; kk += 16) {
+      h4 lv[4], uv[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          lv[t][q] = Ls[wr + 16 * t + li][kk + 4 * lk + q];
+          uv[t][q] = Us[wc + 16 * t + li][kk + 4 * lk + q];
+        }
+      }
+#pragma unroll
+      for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+          acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x16f16(uv[tj], lv[ti], acc[tj][ti], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // acc[tj][ti][v] = (L sigma U)[row = wr + 16 ti + li, col = wc + 16 tj + 4 lk + v]
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t col = j0 + wc + 16 * tj + 4 * lk + v;
+      if (col >= N) continue;
+      const float inv = 1.0f / sigma[col];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti) {
+        const int64_t row = i0 + wr + 16 * ti + li;
+        if (row < M) C[row + col * ldc] -= acc[tj][ti][v] * inv;
+      }
+    }
+}
+
+// sigma[c] = 2^-e with max_k |U[k, c]| in [2^(e-1), 2^e) (1 for a zero column), c < N, k < K.
+__global__ void colscale_kernel(const float* __restrict__ U, int64_t ldu, int64_t N, int64_t K,
+                                float* __restrict__ sigma) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  float m = 0.f;
+  for (int64_t k = 0; k < K; ++k) m = fmaxf(m, fabsf(U[k + c * ldu]));
+  int e = 0;
+  if (m > 0.f) (void)frexpf(m, &e);
+  sigma[c] = m > 0.f ? ldexpf(1.0f, -e) : 1.0f;
+}
+
 // ---- triangular solves with nrhs <= 64 right-hand sides -------------------------------------------------
 // Diagonal-block solve of rows [r0, r0+nb) for all right-hand sides: lower (unit) or upper.  One wave per
 // right-hand side: lane i holds x[i]; step r broadcasts x[r] and the lanes below (lower) or above (upper) update.
@@ -464,7 +556,7 @@ void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t n
 size_t lu_workspace_bytes(int64_t n) {
   (void)n;
   return sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) + sizeof(int) * 2 * MAX_P +
-         sizeof(int) * (2 + 1 + 4 * LU_NB) + 1024;
+         sizeof(int) * (2 + 1 + 4 * LU_NB) + sizeof(float) * (size_t)(n > 0 ? n : 1) + 1024;
 }
 
 // Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
@@ -473,8 +565,30 @@ size_t lu_workspace_bytes(int64_t n) {
 // single rank-LU_OUTER update (right-looking across blocks), so the trailing matrix is read and written once per
 // LU_OUTER columns.
 template <class T>
+void trailing_update(T* L, T* U, T* C, int64_t lda, int64_t M, int64_t N, int64_t K, float* sigma, bool fp16,
+                     hipStream_t st);
+
+template <>
+void trailing_update<double>(double* L, double* U, double* C, int64_t lda, int64_t M, int64_t N, int64_t K, float*,
+                             bool, hipStream_t st) {
+  gemm_sub<double>(L, lda, U, lda, C, lda, M, N, K, st);
+}
+
+template <>
+void trailing_update<float>(float* L, float* U, float* C, int64_t lda, int64_t M, int64_t N, int64_t K,
+                            float* sigma, bool fp16, hipStream_t st) {
+  if (!fp16 || M <= 0 || N <= 0 || K <= 0) {
+    gemm_sub<float>(L, lda, U, lda, C, lda, M, N, K, st);
+    return;
+  }
+  hipLaunchKernelGGL(colscale_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, (const float*)U, lda, N, K, sigma);
+  hipLaunchKernelGGL(gemm_sub_f16_kernel, dim3(nblk(M, GB_M), nblk(N, GB_N)), dim3(256), 0, st, (const float*)L, lda,
+                     (const float*)U, lda, C, lda, M, N, K, (const float*)sigma);
+}
+
+template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
-                          hipStream_t st) {
+                          hipStream_t st, bool fp16_update) {
   char* w = static_cast<char*>(ws);
   double* cval = reinterpret_cast<double*>(w);
   w += sizeof(double) * 2 * MAX_P;
@@ -487,6 +601,8 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   unsigned* bar = reinterpret_cast<unsigned*>(w);  // barrier arrival counter, then its error flag
   int* bar_err = reinterpret_cast<int*>(bar + 1);
   int* perm = bar_err + 1;                          // 1 + 4 LU_NB ints: the last panel's row permutation
+  float* sigma = reinterpret_cast<float*>(perm + 1 + 4 * LU_NB);  // n column scales (FP16 trailing update)
+  if (ncols > n) fp16_update = false;
   unsigned bar_base = 0;
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
   if (e != hipSuccess) return e;
@@ -524,8 +640,8 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
     }
     // U12 of the block, then the rank-(K1 - K0) update of everything right of and below it
     block_trsm<T>(A, lda, K0, K1, K1, ncols - K1, st);
-    gemm_sub<T>(A + K1 + K0 * lda, lda, A + K0 + K1 * lda, lda, A + K1 + K1 * lda, lda, n - K1, ncols - K1, K1 - K0,
-                st);
+    trailing_update<T>(A + K1 + K0 * lda, A + K0 + K1 * lda, A + K1 + K1 * lda, lda, n - K1, ncols - K1, K1 - K0,
+                       sigma, fp16_update, st);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -534,8 +650,9 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
 }
 
 template <class T>
-hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st) {
-  return lu_factor_cols<T>(A, n, n, lda, ipiv, info, ws, st);
+hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st,
+                     bool fp16_update) {
+  return lu_factor_cols<T>(A, n, n, lda, ipiv, info, ws, st, fp16_update);
 }
 
 // Permutation of the rows produced by the pivots: row i of P A is row perm[i] of A.
@@ -627,10 +744,11 @@ void convert_matrix_d2f(const double* A, int64_t lda, float* B, int64_t ldb, int
   if (m > 0 && n > 0) hipLaunchKernelGGL(d2f_matrix_kernel, dim3(nblk(m, 256), (unsigned)n), dim3(256), 0, st, A, lda, B, ldb, m, n);
 }
 
-template hipError_t lu_factor<double>(double*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t);
-template hipError_t lu_factor<float>(float*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t);
-template hipError_t lu_factor_cols<double>(double*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t);
-template hipError_t lu_factor_cols<float>(float*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t);
+template hipError_t lu_factor<double>(double*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
+template hipError_t lu_factor<float>(float*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
+template hipError_t lu_factor_cols<double>(double*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t,
+                                           bool);
+template hipError_t lu_factor_cols<float>(float*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
 template hipError_t lu_solve<double>(const double*, int64_t, int64_t, const int32_t*, double*, int64_t, int64_t, hipStream_t);
 template hipError_t lu_solve<float>(const float*, int64_t, int64_t, const int32_t*, float*, int64_t, int64_t, hipStream_t);
 template hipError_t tri_solve_upper<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);

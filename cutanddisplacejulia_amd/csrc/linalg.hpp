@@ -15,14 +15,17 @@ size_t lu_workspace_bytes(int64_t n);
 // ipiv (device, length n) receives 0-based pivot rows; info (device int) receives 0, 1 + the first zero
 // pivot's index, or -1 if a panel's grid barrier wait overran (factors invalid).  ws must hold
 // lu_workspace_bytes(n) bytes.  T = double or float.
+// fp16_update (T = float only): the rank-256 trailing updates take FP16 operands (U scaled per column by a power
+// of two) accumulated in FP32 on MFMA; panels, catch-up updates and triangular solves stay FP32.
 template <class T>
-hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st);
+hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st,
+                     bool fp16_update = false);
 
 // LU of an n x ncols matrix (ncols >= n): the extra columns are treated as right-hand sides, so on return they
 // hold L^-1 P B.
 template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
-                          hipStream_t st);
+                          hipStream_t st, bool fp16_update = false);
 
 // Row permutation implied by the pivots (row i of P A is row perm[i] of A); perm_dev is a device array.
 // Synchronises the stream.

@@ -10,7 +10,7 @@ import numpy as np
 from . import _lib
 from ._lib import CdjError, Opts, Report, check, ctx, f64, load, ptr
 
-PC = {"none": 0, "jacobi": 1, "learned": 2, "lu32": 3}
+PC = {"none": 0, "jacobi": 1, "learned": 2, "lu32": 3, "lu16": 3}
 
 _p = ctypes.c_void_p
 _d = ctypes.c_double
@@ -121,8 +121,10 @@ def precond_solve_dev(dA, lda, db, dx, dP1, dP2, dP3, n, G, pc="learned", model=
     """FGMRES on device A (3n x 3n) with the named preconditioner to an FP64 relative residual `tol`; returns the
     report dict.  krylov32: Arnoldi products on an FP32 copy of A, FP64 residual at every restart."""
     rep = Report()
-    opts = Opts(precision=_lib.CDJ_PREC_MIXED32 if krylov32 else _lib.CDJ_PREC_LEARNED, tol=float(tol),
-                max_iter=int(max_iter), near_factor=2.0)
+    prec = _lib.CDJ_PREC_MIXED32 if krylov32 else _lib.CDJ_PREC_LEARNED
+    if pc == "lu16":  # LU with FP16-operand trailing updates as the refinement's preconditioner
+        prec = _lib.CDJ_PREC_MIXED16
+    opts = Opts(precision=prec, tol=float(tol), max_iter=int(max_iter), near_factor=2.0)
     check(_bind().cdj_precond_solve_dev(ctx(), model.h if model is not None else None, PC[pc], _dptr(dP1),
                                         _dptr(dP2), _dptr(dP3), int(n), float(G), _dptr(dA), int(lda), _dptr(db),
                                         _dptr(dx), ctypes.byref(opts), ctypes.byref(rep), _stream()))

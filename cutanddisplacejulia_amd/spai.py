@@ -85,7 +85,7 @@ def _timed(fn):
     return out, 1e3 * (time.perf_counter() - t0)
 
 
-PCS = ("none", "jacobi", "learned", "learned_k32", "lu32")
+PCS = ("none", "jacobi", "learned", "learned_k32", "lu32", "lu16")
 
 
 def evaluate_step(step, model, hs=0, pcs=PCS, tol=1e-10):

@@ -38,6 +38,8 @@ extern "C" {
 #define CDJ_PREC_FP64 0       /* FP64 assembly, FP64 LU (the reference's arithmetic) */
 #define CDJ_PREC_MIXED32 1    /* FP32 LU on MFMA as preconditioner, FP64 GMRES-IR against the FP64 A */
 #define CDJ_PREC_LEARNED 2    /* learned SPAI preconditioner, FP64 FGMRES; FP32-LU path as fallback */
+#define CDJ_PREC_MIXED16 3    /* as MIXED32, but the LU's trailing updates take FP16 operands (per-column power-of-two
+                                 scaling of U) accumulated in FP32 on MFMA */
 
 /* arithmetic of the assembly (cdj_opts.assembly) */
 #define CDJ_ASM_FP64 0        /* every entry in FP64 (the reference's arithmetic) */

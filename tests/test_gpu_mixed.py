@@ -67,3 +67,42 @@ def test_mixed_falls_back_to_fp64(mixed):
     xr = np.linalg.solve(A, b)
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-4
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-3
+
+
+@pytest.fixture
+def mixed16():
+    api.set_precision("mixed16")
+    yield
+    api.set_precision("fp64")
+    api._SETTINGS["max_iter"] = 0
+
+
+@pytest.mark.parametrize("n", [300, 1500])
+def test_mixed16_dense_solve(mixed16, n):
+    """LU with FP16-operand trailing updates (rank-256 updates from n > 256 on) as the refinement's
+    preconditioner: the FP64 residual still reaches 1e-10, in more iterations than the FP32 LU needs."""
+    rng = np.random.default_rng(n + 1)
+    A = rng.normal(size=(n, n)) + 2 * n ** 0.5 * np.eye(n)
+    A[:, ::7] *= 1e3  # columns of very different scale exercise the per-column scaling of U
+    b = rng.normal(size=(n, 1))
+    x = api.solve(A, b)
+    rep = dict(api.LAST_REPORT)
+    assert rep["path"] == 3, rep
+    res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert res <= 1e-10, res
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= np.linalg.cond(A) * 1e-10
+
+
+def test_mixed16_slip_calculator_halfspace(mixed16):
+    P1, P2, P3 = meshes.rect_fault(14, 14, 60.0, 45.0, 5.0, 5.0, 0.5)
+    n = P1.shape[0]
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    K, E, lam, nu, G = api.ElasticConstantsCheck(1.0, 0.25)
+    Dn, Dss, Dds, A, b = api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 1,
+                                              api.Stresses(0, 0, 1.0, 0.3, 0, 0), np.zeros(n))
+    rep = dict(api.LAST_REPORT)
+    assert rep["path"] == 3 and rep["residual"] <= 1e-10, rep
+    x = np.concatenate([Dn, Dss, Dds])
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-8
