@@ -38,7 +38,8 @@ EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", 
            "cdj_traction_influence", "cdj_slip_calculator", "cdj_solve", "cdj_assemble_dev",
            "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev", "cdj_mlp_create", "cdj_mlp_destroy",
            "cdj_mlp_nparams", "cdj_mlp_get_params", "cdj_mlp_set_params", "cdj_mlp_train_dev", "cdj_mlp_loss_grad",
-           "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning"]
+           "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning",
+           "cdj_traction_rows", "cdj_lstsq"]
 
 
 class CdjError(RuntimeError):
@@ -88,6 +89,8 @@ def load(path=LIB_PATH):
     L.cdj_last_near_entries.argtypes = [_p]
     L.cdj_batched_slip.argtypes = [_p, _p, _i64, _d, _d, _d, _i32, _p, _p, _p]
     L.cdj_batch_tuning.argtypes = [_p, _p, _p]
+    L.cdj_traction_rows.argtypes = [_p, _p, _p, _i64, _p, _p, _p, _i64, _d, _d, _d, _i32, _p, _p]
+    L.cdj_lstsq.argtypes = [_p, _p, _i64, _i64, _p, _i64, _p, _p]
     td_args = [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _d, _d, _i32, _i32, _i32, _p, _p]
     L.cdj_td_influence.argtypes = td_args
     L.cdj_td_field.argtypes = td_args

@@ -231,24 +231,33 @@ def CalculateInfluenceMatrices3D(FaceNormalVector, MidPoint, P1, P2, P3, ν, G, 
     nrm, mid = f64(FaceNormalVector), f64(MidPoint)
     P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
     fixed = np.asarray(FixedEls).ravel() == 1 if FixedEls is not None else np.zeros(n, bool)
-    T = [np.zeros((n, n), order="F") for _ in range(9)]
-    check(load().cdj_traction_influence(ctx(), ptr(nrm), ptr(mid), ptr(P1), ptr(P2), ptr(P3), int(n), float(ν),
-                                        float(G), float(λ), int(HSFlag), None, ptr_array(T), ctypes.byref(_opts())))
     if not fixed.any():
+        T = [np.zeros((n, n), order="F") for _ in range(9)]
+        check(load().cdj_traction_influence(ctx(), ptr(nrm), ptr(mid), ptr(P1), ptr(P2), ptr(P3), int(n), float(ν),
+                                            float(G), float(λ), int(HSFlag), None, ptr_array(T),
+                                            ctypes.byref(_opts())))
         return TractionInf(*T)
-    # fixed elements: displacement at the fixed collocation points, placed into the fixed rows
-    x = mid[:, 0] - nrm[:, 0] * 1e-12
-    y = mid[:, 1] - nrm[:, 1] * 1e-12
-    z = mid[:, 2] - nrm[:, 2] * 1e-12
-    ones = np.ones(n)
-    _, D = TD(x[fixed], y[fixed], z[fixed], P1, P2, P3, ones, ones, ones, ν, G, 1, 0, HSFlag)
+    # Collocation points (SetupCollationPoints, CalculateInfluenceMatrices3D.jl:52-71)
+    pts = mid - nrm * 1e-12
     free = ~fixed
+    nf = int(free.sum())
+    # Free rows: strains at the free collocation points turned into tractions.  The reference passes the full
+    # normal arrays to ConvertStrainInfMatsToTraction with only the free rows of strain, so its i-th free row
+    # takes the plane of element i (CalculateInfluenceMatrices3D.jl:24-26, 73-110); reproduced here.
+    Tf = [np.zeros((nf, n), order="F") for _ in range(9)]
+    fp = f64(pts[free])
+    fn = f64(nrm[:nf])
+    check(load().cdj_traction_rows(ctx(), ptr(fp), ptr(fn), nf, ptr(P1), ptr(P2), ptr(P3), int(n), float(ν), float(G),
+                                   float(λ), int(HSFlag), ptr_array(Tf), ctypes.byref(_opts())))
+    # Fixed rows: displacements at the fixed collocation points
+    ones = np.ones(n)
+    _, D = TD(pts[fixed, 0], pts[fixed, 1], pts[fixed, 2], P1, P2, P3, ones, ones, ones, ν, G, 1, 0, HSFlag)
     out = []
     for k, name in enumerate(TRACTION_FIELDS):
         full = np.zeros((n, n), order="F")
-        full[free, :] = T[k][free, :]
+        full[free, :] = Tf[k]
         full[fixed, :] = getattr(D, DISP_FIELDS[k])  # DnTn<-DnUx, DnTss<-DnUy, ... (ConcatInfMats field order)
-        out.append(np.asfortranarray(full[:, free]))
+        out.append(np.asfortranarray(full[:, free]))  # fixed sources' columns dropped (PutDispInfsIntoInfMat)
     return TractionInf(*out)
 
 
@@ -269,13 +278,13 @@ def SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, 
         b = f64(np.concatenate([Tn, Tss, Tds]).reshape(3 * n, 1))
     if fixed.any():
         Ti = CalculateInfluenceMatrices3D(nrm, MidPoint, P1, P2, P3, ν, G, λ, FixedEls, HSFlag, n)
-        A = -np.block([[Ti.DnTn, Ti.DssTn, Ti.DdsTn], [Ti.DnTss, Ti.DssTss, Ti.DdsTss], [Ti.DnTds, Ti.DssTds, Ti.DdsTds]])
-        # Non-square system: Julia's `\` returns the least-squares solution; solved here through the normal
-        # equations on the GPU LU.
-        AtA = f64(A.T @ A)
-        Atb = f64(A.T @ b)
-        D = solve(AtA, Atb)
+        A = f64(-np.block([[Ti.DnTn, Ti.DssTn, Ti.DdsTn], [Ti.DnTss, Ti.DssTss, Ti.DdsTss],
+                           [Ti.DnTds, Ti.DssTds, Ti.DdsTds]]))
+        # Non-square system (3n x 3nf): Julia's `\` returns the least-squares solution
         nf = int((~fixed).sum())
+        D = np.zeros((3 * nf, b.shape[1]), order="F")
+        rep = Report()
+        check(load().cdj_lstsq(ctx(), ptr(A), A.shape[0], A.shape[1], ptr(b), b.shape[1], ptr(D), ctypes.byref(rep)))
     else:
         P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
         mid = f64(MidPoint)

@@ -1320,3 +1320,85 @@ int cdj_batch_tuning(const cdj_ctx* ctx, int32_t* tpb, int32_t* wgs) {
 }
 
 }  // extern "C"
+
+// ---- traction rows at given receivers, least squares ----------------------------------------------------------
+extern "C" {
+
+int cdj_traction_rows(cdj_ctx* ctx, const double* points, const double* normals, int64_t m, const double* P1,
+                      const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                      int32_t hs_flag, double* const* T, const cdj_opts* opts) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (m < 0 || n < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  if (!points || !normals || !P1 || !P2 || !P3) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  for (int k = 0; k < 9; ++k)
+    if (!T || !T[k]) return fail(ctx, CDJ_ERR_ARG, "traction output pointer is NULL");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (m == 0 || n == 0) return CDJ_OK;
+  const int hs = hs_flag == 1;
+  if (hs) {
+    for (int64_t i = 0; i < m; ++i)
+      if (points[2 * m + i] > 0) return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+    for (int64_t i = 0; i < n; ++i)
+      if (P1[2 * n + i] > 0 || P2[2 * n + i] > 0 || P3[2 * n + i] > 0)
+        return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+  }
+  const int RS = hs ? REC_HS : REC_FS;
+  DBuf dp, dn, d1, d2, d3, recv, rec, dT;
+  HIPCHK(dp.alloc(sizeof(double) * 3 * m));
+  HIPCHK(dn.alloc(sizeof(double) * 3 * m));
+  HIPCHK(d1.alloc(sizeof(double) * 3 * n));
+  HIPCHK(d2.alloc(sizeof(double) * 3 * n));
+  HIPCHK(d3.alloc(sizeof(double) * 3 * n));
+  HIPCHK(hipMemcpyAsync(dp.p, points, sizeof(double) * 3 * m, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dn.p, normals, sizeof(double) * 3 * m, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(recv.alloc(sizeof(double) * RECV * m));
+  HIPCHK(rec.alloc(sizeof(double) * RS * n));
+  const size_t vm = sizeof(double) * m * n;
+  HIPCHK(dT.alloc(9 * vm));
+  launch_receivers_at(dn.as<double>(), dp.as<double>(), m, recv.as<double>(), st);
+  launch_setup(d1.as<double>(), d2.as<double>(), d3.as<double>(), n, hs, rec.as<double>(), st);
+  OutPtrs<double> out{};
+  out.ld = m;
+  out.sign = 1.0;
+  for (int k = 0; k < 9; ++k) out.p[k] = dT.as<double>() + (int64_t)k * m * n;
+  const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, m);
+  launch_traction<double>(recv.as<double>(), m, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(T[k], out.p[k], vm, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_lstsq(cdj_ctx* ctx, const double* A, int64_t m, int64_t k, const double* b, int64_t nrhs, double* x,
+              cdj_report* report) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (m < 0 || k < 0 || nrhs < 0 || k > m) return fail(ctx, CDJ_ERR_DIMS, "invalid matrix dimensions");
+  if (m + k > 65535) return fail(ctx, CDJ_ERR_DIMS, "least-squares system too large (m + k > 65535)");
+  if (!A || !b || !x) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (k == 0 || nrhs == 0) return CDJ_OK;
+  const int64_t N = m + k;
+  DBuf dA, KB;
+  HIPCHK(dA.alloc(sizeof(double) * m * k));
+  HIPCHK(KB.alloc(sizeof(double) * N * (N + nrhs)));
+  HIPCHK(hipMemcpyAsync(dA.p, A, sizeof(double) * m * k, hipMemcpyHostToDevice, st));
+  build_kkt(dA.as<double>(), m, m, k, KB.as<double>(), N, st);
+  // right-hand sides [b; 0]
+  double* R = KB.as<double>() + N * N;
+  HIPCHK(hipMemsetAsync(R, 0, sizeof(double) * N * nrhs, st));
+  for (int64_t q = 0; q < nrhs; ++q)
+    HIPCHK(hipMemcpyAsync(R + q * N, b + q * m, sizeof(double) * m, hipMemcpyHostToDevice, st));
+  int rc = solve_augmented_fp64(ctx, KB.as<double>(), N, N, nrhs, nullptr, nullptr, report, st);
+  if (rc) return rc;
+  for (int64_t q = 0; q < nrhs; ++q)
+    HIPCHK(hipMemcpyAsync(x + q * k, R + q * N + m, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+}  // extern "C"

@@ -525,6 +525,18 @@ __global__ void d2f_matrix_kernel(const double* A, int64_t lda, float* B, int64_
 
 unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// K = [[I, A], [A^T, 0]] ((m + k) x (m + k), leading dimension ldk) from A (m x k, leading dimension lda).
+__global__ void kkt_kernel(const double* __restrict__ A, int64_t lda, int64_t m, int64_t k, double* __restrict__ K,
+                           int64_t ldk) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = blockIdx.y;
+  if (i >= m + k) return;
+  double v;
+  if (j < m) v = i < m ? (i == j ? 1.0 : 0.0) : A[j + (i - m) * lda];
+  else v = i < m ? A[i + (j - m) * lda] : 0.0;
+  K[i + j * ldk] = v;
+}
+
 // info = -1 when a panel barrier wait overran (the factors are then invalid)
 __global__ void barrier_status_kernel(const int* bar_err, int* info) {
   if (*bar_err) *info = -1;
@@ -732,6 +744,11 @@ void gemv(const TA* A, int64_t m, int64_t n, int64_t lda, const double* x, doubl
           hipStream_t st) {
   if (m <= 0) return;
   hipLaunchKernelGGL(gemv_kernel<TA>, dim3(nblk(m, 256)), dim3(256), 0, st, A, m, n, lda, x, y, alpha, beta);
+}
+
+void build_kkt(const double* A, int64_t lda, int64_t m, int64_t k, double* K, int64_t ldk, hipStream_t st) {
+  if (m + k > 0)
+    hipLaunchKernelGGL(kkt_kernel, dim3(nblk(m + k, 256), (unsigned)(m + k)), dim3(256), 0, st, A, lda, m, k, K, ldk);
 }
 
 void convert_d2f(const double* a, float* b, int64_t n, hipStream_t st) {

@@ -52,4 +52,7 @@ void convert_d2f(const double* a, float* b, int64_t n, hipStream_t st);
 void convert_f2d(const float* a, double* b, int64_t n, hipStream_t st);
 void convert_matrix_d2f(const double* A, int64_t lda, float* B, int64_t ldb, int64_t m, int64_t n, hipStream_t st);
 
+// Least-squares system matrix K = [[I, A], [A^T, 0]] ((m + k) x (m + k)) for min ||A x - b||: K [r; x] = [b; 0].
+void build_kkt(const double* A, int64_t lda, int64_t m, int64_t k, double* K, int64_t ldk, hipStream_t st);
+
 }  // namespace cdj

@@ -167,16 +167,23 @@ __global__ void tdk_rec_to_f32(const double* __restrict__ a, float* __restrict__
 // ---- receivers: collocation points and plane directions --------------------------------------------------
 // SetupCollationPoints (CalculateInfluenceMatrices3D.jl:52-71) and CalculateSSandDSDirs.jl:37-98.
 // recv[i*RECV + ...] = {x, y, z, nx, ny, nz, sx, sy, sz, dx, dy, dz}
-__global__ void tdk_receivers(const double* __restrict__ nrm, const double* __restrict__ mid, int64_t n,
-                              double* __restrict__ recv) {
+// at != nullptr: the receiver points are given (n x 3) instead of derived from the midpoints.
+__global__ void tdk_receivers(const double* __restrict__ nrm, const double* __restrict__ mid,
+                              const double* __restrict__ at, int64_t n, double* __restrict__ recv) {
 #pragma clang fp contract(off)
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double nx = nrm[i], ny = nrm[n + i], nz = nrm[2 * n + i];
   double* o = recv + i * RECV;
-  o[0] = mid[i] - (nx * 1e-12);
-  o[1] = mid[n + i] - (ny * 1e-12);
-  o[2] = mid[2 * n + i] - (nz * 1e-12);
+  if (at) {
+    o[0] = at[i];
+    o[1] = at[n + i];
+    o[2] = at[2 * n + i];
+  } else {
+    o[0] = mid[i] - (nx * 1e-12);
+    o[1] = mid[n + i] - (ny * 1e-12);
+    o[2] = mid[2 * n + i] - (nz * 1e-12);
+  }
   o[3] = nx;
   o[4] = ny;
   o[5] = nz;
@@ -428,7 +435,14 @@ void launch_rec_to_f32(const double* a, float* b, int64_t n, hipStream_t st) {
 
 void launch_receivers(const double* nrm, const double* mid, int64_t n, double* recv, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(tdk_receivers, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, nrm, mid, n, recv);
+  hipLaunchKernelGGL(tdk_receivers, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, nrm, mid,
+                     (const double*)nullptr, n, recv);
+}
+
+void launch_receivers_at(const double* nrm, const double* pts, int64_t n, double* recv, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tdk_receivers, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, st, nrm, (const double*)nullptr,
+                     pts, n, recv);
 }
 
 void launch_check_hs(const double* Z, int64_t nobs, const double* P1, const double* P2, const double* P3,

@@ -26,6 +26,8 @@ void launch_setup(const double* P1, const double* P2, const double* P3, int64_t 
                   hipStream_t st);
 void launch_rec_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
 void launch_receivers(const double* nrm, const double* mid, int64_t n, double* recv, hipStream_t st);
+// receivers at given points (n x 3) with the planes of the given normals
+void launch_receivers_at(const double* nrm, const double* pts, int64_t n, double* recv, hipStream_t st);
 void launch_check_hs(const double* Z, int64_t nobs, const double* P1, const double* P2, const double* P3,
                      int64_t ntri, int* flag, hipStream_t st);
 template <class T>

@@ -195,3 +195,12 @@ def elliptical_cracks(count=64, n_min=200, n_max=3000, box=10.0, seed=BASE_SEED 
         k = int(rng.integers(rhs_min, rhs_max + 1))
         out.append(dict(P1=P1, P2=P2, P3=P3, stresses=rng.normal(size=(k, 6)), aspect=aspect))
     return out
+
+
+def config5(rings=52, a=2.0, b=1.0):
+    """Large full-space case: planar elliptical crack with semi-axes a (x) and b (y) in z = 0, ring-meshed
+    (6 rings^2 triangles: 16224 for 52 rings, A is 48672 x 48672 FP64), 200 x 200 observation grid at
+    z = 0.5 over [-4, 4]^2; nu = 0.25, mu = 1, remote sigma_zz = 1."""
+    P1, P2, P3 = penny(rings, b, aspect=a / b)
+    X, Y, Z = grid_obs(200, 4.0, 0.5)
+    return dict(P1=P1, P2=P2, P3=P3, X=X, Y=Y, Z=Z, nu=0.25, G=1.0, hs=0, a=a, b=b)

@@ -119,6 +119,20 @@ int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const 
                         const double* b, int64_t nrhs, const int32_t* fixed, double* x, double* A_out,
                         const cdj_opts* opts, cdj_report* report);
 
+/* Traction influence rows at m given receivers: points (m x 3) with the planes of `normals` (m x 3), against n
+ * source triangles; T: 9 m x n matrices in TractionInf order.  This is ConvertStrainInfMatsToTraction
+ * (CalculateInfluenceMatrices3D.jl:73-142) applied to TD strains at arbitrary points - the free rows of the
+ * fixed-element form (CalculateInfluenceMatrices3D.jl:10-28), where row i takes element i's plane. */
+int cdj_traction_rows(cdj_ctx* ctx, const double* points, const double* normals, int64_t m, const double* P1,
+                      const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                      int32_t hs_flag, double* const* T, const cdj_opts* opts);
+
+/* Least squares min ||A x - b|| for A m x k (k <= m, column-major), the `A\b` of SlipCalculator3D.jl:22 when
+ * fixed elements make A non-square (Julia: pivoted QR).  Solved as [[I, A], [A^T, 0]] [r; x] = [b; 0] with the
+ * FP64 LU; m + k <= 65535. */
+int cdj_lstsq(cdj_ctx* ctx, const double* A, int64_t m, int64_t k, const double* b, int64_t nrhs, double* x,
+              cdj_report* report);
+
 /* Dense solve of A x = b (the `A\b` of SlipCalculator3D.jl:22): blocked LU with partial pivoting on the GPU. */
 int cdj_solve(cdj_ctx* ctx, const double* A, const double* b, int64_t n, int64_t nrhs, double* x,
               const cdj_opts* opts, cdj_report* report);

@@ -250,3 +250,23 @@ def eshelby_penny_shear_slip(rho, a, tau, nu, mu):
 def eshelby_penny_opening(rho, a, p, nu, mu):
     """Opening of a penny crack of radius a under internal pressure p (Sneddon / Eshelby)."""
     return 4 * (1 - nu) * p * a / (np.pi * mu) * np.sqrt(np.clip(1 - (rho / a) ** 2, 0, None))
+
+
+def eshelby1957_penny_profile(G, nu, Tn, Ts, r, a):
+    """Separation of the walls of a penny crack of radius a under uniform normal (Tn) and shear (Ts) traction:
+    one-wall opening 2 a (1 - nu) Tn / (pi G) sqrt(1 - r^2/a^2) and shearing 4 (1 - nu) a Ts / (pi (2 - nu) G)
+    sqrt(1 - r^2/a^2) (Segall 2010, eqs 4.73-4.74), combined as a vector magnitude and doubled
+    (Eshelby1957_PennyCrackSlipProfile.jl)."""
+    s = np.sqrt(np.clip(1.0 - (np.asarray(r) ** 2) / a ** 2, 0.0, None))
+    ds = (4 * (1 - nu) * a * Ts) / (np.pi * (2 - nu) * G) * s
+    dn = (2 * a * (1 - nu) * Tn) / (np.pi * G) * s
+    return 2.0 * np.sqrt(ds ** 2 + dn ** 2)
+
+
+def mogi1958(D, X, Y, radius, P, nu, G):
+    """Surface displacement above a pressurised spherical cavity of the given radius at depth D (Mogi 1958):
+    u = (radius^3 P (1 - nu) / G) (x, y, D) / R^3, R^2 = D^2 + x^2 + y^2 (Mogi1958_SphericalCavity.jl)."""
+    X, Y = np.asarray(X, dtype=float), np.asarray(Y, dtype=float)
+    R = np.sqrt(D ** 2 + X ** 2 + Y ** 2)
+    c = radius ** 3 * P * (1 - nu) / G
+    return X / R ** 3 * c, Y / R ** 3 * c, D / R ** 3 * c

@@ -1,0 +1,95 @@
+"""The reference's own validation cases, on its own meshes (tests/golden, copied from the reference's test/
+folder), run through this engine with the reference's pass criteria.
+
+* test_TDvsEshelbyPennyCrack.jl: a 581-face penny crack of radius 1 in full space under uniform shear, then
+  normal, traction.  Away from the tip (rho < 0.9) the wall separation must be within 11 % of Eshelby's closed
+  form.
+* test_TDvsMogi.jl: a 46-face sphere at 50 m depth in the half space, pressurised, with a 6-face fixed inner
+  sphere appended (fixed elements: displacement rows, least-squares solve).  The opening spread must stay below
+  0.05 / G, and the surface displacement along x = 0..100 must be within 20 % of Mogi's point source.
+"""
+import numpy as np
+import pytest
+
+from cutanddisplacejulia_amd import api
+from oracle import analytic as AN
+from tests import gocad
+
+pytestmark = pytest.mark.gpu
+
+
+def _pct(analytic, numeric):
+    """BAsPercentOfA.jl: 100 / A * B, infinities made NaN (then ignored)."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        p = (100.0 / np.asarray(analytic)) * np.asarray(numeric)
+    p[np.isinf(p)] = np.nan
+    return p
+
+
+def _bad(p, lim):
+    d = np.abs(p - 100.0)
+    return np.any(d[~np.isnan(d)] > lim)
+
+
+def test_eshelby_penny_crack():
+    P, T = gocad.read_ts("CircleMesh_1a_500Faces.ts")
+    P1, P2, P3 = gocad.p1p2p3(P, T)
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    n = T.shape[0]
+    K, E, lam, nu, G = api.ElasticConstantsCheck(1.0, 0.25)
+    fixed = np.zeros(n)
+    # shearing
+    Dn, Dss, Dds, A, b = api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 0,
+                                              api.Tractions(np.zeros(n), np.ones(n), np.zeros(n)), fixed)
+    total_shear = np.sqrt(Dss ** 2 + Dds ** 2)
+    # opening
+    Dn2, _, _, _, _ = api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 0,
+                                           api.Tractions(np.ones(n), np.zeros(n), np.zeros(n)), fixed)
+    rho = np.hypot(mid[:, 0], mid[:, 1])
+    us = AN.eshelby1957_penny_profile(G, nu, 0.0, 1.0, rho, 1.0)
+    un = AN.eshelby1957_penny_profile(G, nu, 1.0, 0.0, rho, 1.0)
+    good = rho < 0.9
+    pds, pdn = _pct(us, total_shear)[good], _pct(un, Dn2)[good]
+    print("penny: shear within %.2f %%, opening within %.2f %% of Eshelby (rho < 0.9)" %
+          (np.nanmax(np.abs(pds - 100)), np.nanmax(np.abs(pdn - 100))))
+    assert not _bad(pds, 11) and not _bad(pdn, 11)
+    # the reference's volume figures (printed by its test)
+    u, v = P2 - P1, P3 - P1
+    area = 0.5 * np.linalg.norm(np.cross(u, v), axis=1)
+    vol_shear, vol_open = np.sum(area * Dss) / 2, np.sum(area * Dn2) / 2
+    an_shear = 8 * (1 - nu) * 1.0 / (3 * (2 - nu) * G)
+    an_open = ((np.pi * (1 - nu) * 1.0) / (2 * G)) * (2 / np.pi) * (1 / 0.75)
+    print("volumes: shear %.4f (analytic %.4f), opening %.4f (analytic %.4f)" % (vol_shear, an_shear, vol_open,
+                                                                             an_open))
+    assert abs(vol_shear - an_shear) < 0.1 * an_shear and abs(vol_open - an_open) < 0.1 * an_open
+
+
+def test_mogi_with_fixed_elements():
+    P, T = gocad.read_ts("SphereUniformDistributionON_46Faces.ts")
+    PF, TF = gocad.read_ts("SphereFix4.ts")
+    PF[:, 1:4] *= 3
+    P, T, fixed = gocad.append(P, T, PF, TF)
+    depth = 50.0
+    P[:, 3] -= depth
+    P1, P2, P3 = gocad.p1p2p3(P, T)
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    n = T.shape[0]
+    K, E, lam, nu, G = api.ElasticConstantsCheck(1.0, 0.25)
+    Dn, Dss, Dds, A, b = api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 1,
+                                              api.Tractions(np.ones(n), np.zeros(n), np.zeros(n)), fixed)
+    free = fixed != 1
+    assert Dn.shape == (int(free.sum()),) and A.shape == (3 * n, 3 * int(free.sum()))
+    spread = Dn.max() - Dn.min()
+    print("mogi: opening spread %.4g (limit %.4g)" % (spread, 0.05 / G))
+    assert spread <= 0.05 / G
+    # RemoveFixedElements3D, then the surface displacement of the free elements' slip
+    P1f, P2f, P3f = P1[free], P2[free], P3[free]
+    X = np.arange(0.0, 101.0)
+    Y = np.zeros_like(X)
+    Z = np.zeros_like(X)
+    S, D = api.TD(X, Y, Z, P1f, P2f, P3f, Dss, Dds, Dn, nu, G, 1, 0, 1, api.Strains(), api.Disps())
+    ux, uy, uz = AN.mogi1958(depth, X, Y, 1.0, 1.0, nu, G)
+    worst = max(np.nanmax(np.abs(_pct(a, d) - 100)) for a, d in ((ux, D.Ux), (uz, D.Uz)))
+    print("mogi: surface displacement within %.2f %% of Mogi" % worst)
+    for a, d in ((ux, D.Ux), (uy, D.Uy), (uz, D.Uz)):
+        assert not _bad(_pct(a, d), 20)
