@@ -29,6 +29,10 @@ METRIC = "TD influence entries/s; slip-solve time at 1e-10 resid, mixed/learned 
 # oracle instantiated with an operation-counting scalar, averaged over 4000 random pairs of this workload
 # (tests/test_flops.py recomputes it).  Transcendentals (12 sqrt, 30 trig in set-up) are not included.
 FLOPS_PER_PAIR_HS = 43940.0
+# FP64 flops the GPU kernel executes per pair on the same workload, from rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64,
+# _MUL_F64, _FMA_F64 (x64 lanes, FMA twice): shared subexpressions and the forward-mode free-surface strain make
+# it 2.7x less than the formulation's count above.
+EXECUTED_FLOPS_PER_PAIR_HS = 16252.5
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the assembly kernel on the 5000-triangle workload: rocprofv3 --pmc FETCH_SIZE and
@@ -41,15 +45,34 @@ def _log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads):
-    """The oracle (FP64 C++ port of TD.jl) on `rows` receiver rows against all sources; entries/s."""
+def cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads, reps=5):
+    """The oracle (FP64 C++ port of TD.jl, OpenMP over source triangles) on `rows` receiver rows against all
+    sources: median entries/s over `reps` timed runs after one warm-up."""
     from oracle import oracle as O
     n = P1.shape[0]
     O.traction_influence_rows(nrm, mid, P1, P2, P3, nu, G, lam, 1, 0, 8, nthreads=threads)  # warm-up
-    t0 = time.perf_counter()
-    O.traction_influence_rows(nrm, mid, P1, P2, P3, nu, G, lam, 1, 0, rows, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return 9.0 * rows * n / dt, dt
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        O.traction_influence_rows(nrm, mid, P1, P2, P3, nu, G, lam, 1, 0, rows, nthreads=threads)
+        ts.append(time.perf_counter() - t0)
+    dt = float(np.median(ts))
+    return 9.0 * rows * n / dt, dt, float(np.sum(ts))
+
+
+def cpu_lu_baseline(A_dev, b):
+    """scipy (OpenBLAS) getrf + getrs of the same FP64 matrix on the host, all threads; median of 3 (ms)."""
+    import scipy.linalg as sl
+    A = A_dev.cpu().numpy().T.copy(order="F")  # device storage is column-major
+    sl.lu_solve(sl.lu_factor(A[:512, :512]), b[:512])  # warm-up
+    ts = []
+    for _ in range(3):
+        Aw = A.copy(order="F")
+        t0 = time.perf_counter()
+        lu = sl.lu_factor(Aw, overwrite_a=True, check_finite=False)
+        sl.lu_solve(lu, b, check_finite=False)
+        ts.append(time.perf_counter() - t0)
+    return 1e3 * float(np.median(ts))
 
 
 def slip_solve_comparison(step_no):
@@ -87,8 +110,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=50, help="quads per side of the fault mesh (50 -> 5000 triangles)")
-    ap.add_argument("--cpu-rows", type=int, default=0,
-                    help="receiver rows timed for the CPU baseline (0: the whole matrix, about 6 s on 16 cores)")
+    ap.add_argument("--cpu-rows", type=int, default=1000,
+                    help="receiver rows per timed CPU-baseline run (5 runs, median); 1000 rows take ~1.1 s on 16 "
+                         "cores")
+    ap.add_argument("--no-cpu-lu", action="store_true", help="skip the host LU baseline")
     ap.add_argument("--solve-step", type=int, default=40,
                     help="penny-crack growth step of the slip-solve comparison (40: 6534 triangles)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -182,6 +207,11 @@ def main():
         lu_tf = (2.0 / 3.0) * N3 ** 3 / t_lu / 1e12
         _log("FP64 LU+solve: %.1f ms (%.1f TFLOP/s), residual %.2e" % (1e3 * t_lu, lu_tf, res))
         lu15 = {"fp64_lu_ms": 1e3 * t_lu, "fp64_lu_tflops": lu_tf, "fp64_residual": res, "n": N3}
+        if not args.no_cpu and not args.no_cpu_lu:
+            cpu_lu = cpu_lu_baseline(A0, b)
+            _log("CPU LU+solve (scipy/OpenBLAS, %s threads): %.0f ms" % (os.environ.get("OMP_NUM_THREADS", "all"),
+                                                                         cpu_lu))
+            lu15["cpu_lu_ms"] = cpu_lu
         del Ab, A0
 
     solve = None
@@ -191,11 +221,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        rows = args.cpu_rows if args.cpu_rows > 0 else n
-        cv, cdt = cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads)
-        _log("CPU baseline: %.3e entries/s (%d threads, %d rows, %.1f s)" % (cv, threads, rows, cdt))
+        rows = min(args.cpu_rows, n) if args.cpu_rows > 0 else n
+        cv, cdt, ctot = cpu_baseline(P1, P2, P3, nrm, mid, nu, G, lam, rows, threads)
+        _log("CPU baseline: %.3e entries/s (%d threads, %d rows, median %.2f s of 5)" % (cv, threads, rows, cdt))
         cpu = {"value": cv, "unit": "entries/s", "cores": threads, "kind": "port",
-               "sample": "%d receiver rows x %d sources of the same assembly (%.1f s)" % (rows, n, cdt)}
+               "sample": "%d receiver rows x %d sources of the same half-space assembly, median of 5 timed runs "
+                         "(%.1f s of CPU work in total)" % (rows, n, ctot)}
 
     out = {
         "metric": METRIC,
@@ -216,7 +247,10 @@ def main():
                      "frac": achieved_tf / PEAK_FP64_TFLOPS,
                      "traffic": TRAFFIC_BYTES_HS_5000 if args.n == 50 else None,
                      "kernel": "tdk_traction<double, double, true>", "kernel_ms": kavg,
-                     "flops_per_pair": FLOPS_PER_PAIR_HS, "pairs_per_launch": pairs},
+                     "flops_per_pair": FLOPS_PER_PAIR_HS, "pairs_per_launch": pairs,
+                     "executed_flops_per_pair": EXECUTED_FLOPS_PER_PAIR_HS if args.n == 50 else None,
+                     "executed_tflops": (EXECUTED_FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12
+                                         if args.n == 50 else None)},
         "cpu_baseline": cpu,
         "lu_15000": lu15,
         "solve": solve,

@@ -362,23 +362,37 @@ __global__ void __launch_bounds__(256) gemm_sub_f16_kernel(const float* __restri
                                                            const float* __restrict__ U, int64_t ldu,
                                                            float* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
                                                            int64_t K, const float* __restrict__ sigma) {
-  __shared__ _Float16 Ls[GB_M][GH_K + 4];  // Ls[i][k] = L[i0 + i, k0 + k]
-  __shared__ _Float16 Us[GB_N][GH_K + 4];  // Us[j][k] = sigma[j0 + j] U[k0 + k, j0 + j]
+  // operands staged as packed quads of consecutive k: Lq[i][q] = L[i0 + i, k0 + 4q .. 4q + 3] and
+  // Uq[j][q] = sigma[j0 + j] U[k0 + 4q .. 4q + 3, j0 + j]; one 8-byte LDS access per MFMA operand
+  constexpr int QK = GH_K / 4;  // quads per row (8)
+  __shared__ h4 Lq[GB_M][QK + 1];
+  __shared__ h4 Uq[GB_N][QK + 1];
   const int tid = threadIdx.x;
   const int64_t i0 = (int64_t)blockIdx.x * GB_M, j0 = (int64_t)blockIdx.y * GB_N;
   const int wave = tid / 64, lane = tid % 64;
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int li = lane % 16, lk = lane / 16;
-  constexpr int PER = GB_M * GH_K / 256;  // 16 elements of each operand per thread
-  float lr[PER], ur[PER];
+  // L: thread -> row i = tid % 128, quads q = tid / 128 + 2 e (e < 4); U: thread -> quad q = tid % 8,
+  // column j = tid / 8 + 32 e (e < 4), so 8 threads read one column's 32 consecutive k (coalesced)
+  float lr[4][4], ur[4][4];
+  const int li_ = tid % GB_M, lq0 = tid / GB_M;
+  const int uq = tid % QK, uj0 = tid / QK;
   auto fetch = [&](int64_t k0) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-      const int idx = tid + 256 * e;
-      const int i = idx % GB_M, kk = idx / GB_M;
-      lr[e] = (i0 + i < M && k0 + kk < K) ? L[(i0 + i) + (k0 + kk) * ldl] : 0.f;
-      const int kq = idx % GH_K, j = idx / GH_K;
-      ur[e] = (k0 + kq < K && j0 + j < N) ? U[(k0 + kq) + (j0 + j) * ldu] * sigma[j0 + j] : 0.f;
+    for (int e = 0; e < 4; ++e) {
+      const int q = lq0 + 2 * e;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int64_t kk = k0 + 4 * q + t;
+        lr[e][t] = (i0 + li_ < M && kk < K) ? L[(i0 + li_) + kk * ldl] : 0.f;
+      }
+      const int j = uj0 + 32 * e;
+      const float sg = j0 + j < N ? sigma[j0 + j] : 0.f;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int64_t kk = k0 + 4 * uq + t;
+        ur[e][t] = (kk < K && j0 + j < N) ? U[kk + (j0 + j) * ldu] * sg : 0.f;
+      }
     }
   };
   f4 acc[4][4];
@@ -389,23 +403,19 @@ __global__ void __launch_bounds__(256) gemm_sub_f16_kernel(const float* __restri
   fetch(0);
   for (int64_t k0 = 0; k0 < K; k0 += GH_K) {
 #pragma unroll
-    for (int e = 0; e < PER; ++e) {
-      const int idx = tid + 256 * e;
-      Ls[idx % GB_M][idx / GB_M] = (_Float16)lr[e];
-      Us[idx / GH_K][idx % GH_K] = (_Float16)ur[e];
+    for (int e = 0; e < 4; ++e) {
+      Lq[li_][lq0 + 2 * e] = h4{(_Float16)lr[e][0], (_Float16)lr[e][1], (_Float16)lr[e][2], (_Float16)lr[e][3]};
+      Uq[uj0 + 32 * e][uq] = h4{(_Float16)ur[e][0], (_Float16)ur[e][1], (_Float16)ur[e][2], (_Float16)ur[e][3]};
     }
     __syncthreads();
     if (k0 + GH_K < K) fetch(k0 + GH_K);
 #pragma unroll
-    for (int kk = 0; kk < GH_K; kk += 16) {
+    for (int kq = 0; kq < QK; kq += 4) {  // 16 k per MFMA: quads kq .. kq + 3, lane group lk takes quad kq + lk
       h4 lv[4], uv[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          lv[t][q] = Ls[wr + 16 * t + li][kk + 4 * lk + q];
-          uv[t][q] = Us[wc + 16 * t + li][kk + 4 * lk + q];
-        }
+        lv[t] = Lq[wr + 16 * t + li][kq + lk];
+        uv[t] = Uq[wc + 16 * t + li][kq + lk];
       }
 #pragma unroll
       for (int tj = 0; tj < 4; ++tj)

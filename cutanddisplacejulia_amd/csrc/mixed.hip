@@ -96,15 +96,26 @@ __global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restric
   const int ndep = LOWER ? b : nb - 1 - b;
   for (int k = 0; k < ndep; ++k) {
     const int c = LOWER ? k : nb - 1 - k;
+    const int64_t q0 = (int64_t)c * TB;
+    const int nq = (int)std::min<int64_t>(TB, n - q0);
+    // this thread's 16 matrix entries of the dependency block do not depend on x: load them before the wait
+    T a[TB / 4];
+    const bool full = nq == TB && row < nr;
+    if (full) {
+#pragma unroll
+      for (int t = 0; t < TB / 4; ++t) a[t] = LU[(r0 + row) + (q0 + grp + 4 * t) * lda];
+    }
     if (tid == 0) s_ok = wait_flag(&flags[c], epoch, spin_limit, err) ? 1 : 0;
     __syncthreads();
     if (!s_ok) return;
-    const int64_t q0 = (int64_t)c * TB;
-    const int nq = (int)std::min<int64_t>(TB, n - q0);
     if (tid < nq) xs[tid] = __hip_atomic_load(&x[q0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    if (row < nr)
+    if (full) {
+#pragma unroll
+      for (int t = 0; t < TB / 4; ++t) acc += (double)a[t] * (double)xs[grp + 4 * t];
+    } else if (row < nr) {
       for (int q = grp; q < nq; q += 4) acc += (double)LU[(r0 + row) + (q0 + q) * lda] * (double)xs[q];
+    }
     __syncthreads();
   }
   accd[grp][row] = acc;

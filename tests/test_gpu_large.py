@@ -6,8 +6,8 @@ Pinned by:
 * each path's FP64 residual against the assembled matrix (<= 1e-10), and agreement between the paths;
 * the interior opening against the closed form for an elliptical crack under remote tension (Green & Sneddon
   1950): w = 2 sigma b (1 - nu) / (mu E(k)) sqrt(1 - x^2/a^2 - y^2/b^2), k^2 = 1 - b^2/a^2;
-* the summed displacement/strain field of the solved slip on the 200 x 200 grid against the oracle on a sample
-  of grid points.
+* the summed displacement/strain field on the 200 x 200 grid against the oracle on a sample of grid points, to
+  1e-11 of the field's largest value (each value is a sum of 16224 partly cancelling contributions).
 """
 import numpy as np
 import pytest
@@ -103,7 +103,8 @@ def test_config5_grid_field_matches_oracle(large):
                         large["G"], 1, 1, 0)
     g_s = [S.εxx, S.εyy, S.εzz, S.εxy, S.εxz, S.εyz]
     g_d = [D.Ux, D.Uy, D.Uz]
+    # each value sums 16224 triangle contributions that partly cancel: 1e-11 of the group's largest value
     for grp_g, grp_o in ((g_s, so), (g_d, do)):
         scale = max(np.max(np.abs(o)) for o in grp_o)
         for g, o in zip(grp_g, grp_o):
-            assert np.max(np.abs(np.asarray(g).ravel(order="F")[idx] - o)) <= 1e-12 * scale
+            assert np.max(np.abs(np.asarray(g).ravel(order="F")[idx] - o)) <= 1e-11 * scale
