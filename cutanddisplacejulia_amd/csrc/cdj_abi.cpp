@@ -174,7 +174,9 @@ int cdj_create(int device, cdj_ctx** out) {
     delete c;
     return CDJ_ERR_DEVICE;
   }
-  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  // A blocking stream: it is ordered with the legacy null stream, which is where a caller that passes stream 0
+  // (a framework's default stream) has queued the work producing our inputs and consuming our outputs.
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
   if (e != hipSuccess) {
     g_create_err = hipGetErrorString(e);
     delete c;

@@ -72,6 +72,15 @@ __device__ __forceinline__ void panel_barrier(unsigned* count, unsigned target, 
   __syncthreads();
 }
 
+// Cross-workgroup data of the panel kernel (candidates, row j) goes through agent-scope atomic accesses: each XCD
+// has its own L2, so a plain load after the barrier could return a line this XCD cached two columns earlier.
+template <class V> __device__ __forceinline__ void put_agent(V* p, V v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class V> __device__ __forceinline__ V get_agent(const V* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class T>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
                                                       int32_t* ipiv, int* info, double* cval, int* crow, T* crows,
@@ -88,6 +97,10 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
   const int64_t r0 = (int64_t)b * rpb;
   const int64_t r1 = r0 + rpb < m ? r0 + rpb : m;
   const int nr = r1 > r0 ? (int)(r1 - r0) : 0;
+  // This kernel is a cooperative launch: make the previous kernels' writes to A visible here (each XCD has its own
+  // L2, and this launch path is not relied on for the boundary's cache maintenance), and publish this kernel's
+  // writes before it ends.
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   // thread tid owns panel row r0 + tid (rpb <= PT)
   const bool mine = tid < nr;
   const int64_t grow = r0 + tid;
@@ -127,19 +140,19 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     }
     // publish the candidate (value, row, row contents) and, from the owner of row j, row j itself
     if (tid == 0) {
-      cval[par * P + b] = bv;
-      crow[par * P + b] = br;
+      put_agent(&cval[par * P + b], bv);
+      put_agent(&crow[par * P + b], br);
     }
-    if (br >= 0 && tid < nbw) crows[(int64_t)(par * P + b) * LU_NB + tid] = sh[tid * rpb + (br - r0)];
-    if (j >= r0 && j < r1 && tid < nbw) rowj[par * LU_NB + tid] = sh[tid * rpb + (j - r0)];
+    if (br >= 0 && tid < nbw) put_agent(&crows[(int64_t)(par * P + b) * LU_NB + tid], sh[tid * rpb + (br - r0)]);
+    if (j >= r0 && j < r1 && tid < nbw) put_agent(&rowj[par * LU_NB + tid], sh[tid * rpb + (j - r0)]);
     panel_barrier(bar, bar_base + (unsigned)(j + 1) * (unsigned)P, bar_err);
     // global pivot: wave 0 reduces the P candidates and stages the pivot row
     if (wave == 0) {
       double gv = -1.0;
       int gr = -1, gw = -1;
       for (int q = lane; q < P; q += 64) {
-        const int rq = crow[par * P + q];
-        const double vq = cval[par * P + q];
+        const int rq = get_agent(&crow[par * P + q]);
+        const double vq = get_agent(&cval[par * P + q]);
         if (rq >= 0 && (gr < 0 || vq > gv || (vq == gv && rq < gr))) {
           gv = vq;
           gr = rq;
@@ -156,7 +169,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
           gw = w2;
         }
       }
-      if (lane < nbw) u[lane] = gw >= 0 ? crows[(int64_t)(par * P + gw) * LU_NB + lane] : T(0);
+      if (lane < nbw) u[lane] = gw >= 0 ? get_agent(&crows[(int64_t)(par * P + gw) * LU_NB + lane]) : T(0);
       if (lane == 0) {
         s_prow = gr >= 0 ? gr : j;
         if (b == 0) {
@@ -171,7 +184,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     if (mine) {
       if (prow != j) {
         if (grow == prow)
-          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = rowj[par * LU_NB + c];
+          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = get_agent(&rowj[par * LU_NB + c]);
         else if (grow == j)
           for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = u[c];
       }
@@ -228,6 +241,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     }
     if (lane == 0) perm[0] = cnt;
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 // Row interchanges of one panel applied to columns [0, k) and [k + nbw, ncols), as the permutation of the

@@ -86,7 +86,9 @@ __global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restric
   const int64_t r0 = (int64_t)b * TB;
   const int nr = (int)std::min<int64_t>(TB, n - r0);
   const int tid = threadIdx.x, row = tid % TB, grp = tid / TB;  // 4 groups of 64 threads
-  if (tid < nr) rhs[tid] = x[r0 + tid];
+  // cooperative launch: acquire the previous kernels' writes (LU, x) before reading them
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (tid < nr) rhs[tid] = __hip_atomic_load(&x[r0 + tid], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (int idx = tid; idx < nr * nr; idx += 256) {
     const int r = idx % nr, c = idx / nr;
     D[r][c] = LU[(r0 + r) + (r0 + c) * lda];
@@ -220,6 +222,24 @@ __global__ void combine_kernel(const double* __restrict__ Z, int64_t ldz, int k,
 
 constexpr int DOT_BLOCKS = 240;
 
+// out[j] = sum over q of part[j * nparts + q], in q order (the same order as a host summation)
+__global__ void reduce_parts_kernel(const double* __restrict__ part, int nvec, int nparts, double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nvec) return;
+  double s = 0.0;
+  for (int q = 0; q < nparts; ++q) s += part[(int64_t)j * nparts + q];
+  out[j] = s;
+}
+
+// v = w / ||w|| with ||w||^2 read from device memory (zero vector when the norm is zero)
+__global__ void scale_by_norm_kernel(const double* __restrict__ w, const double* __restrict__ nrm2,
+                                     double* __restrict__ v, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hh = sqrt(*nrm2);
+  v[i] = hh > 0.0 ? (1.0 / hh) * w[i] : 0.0;
+}
+
 }  // namespace
 
 // ---- host-side building blocks -----------------------------------------------------------------------------
@@ -300,6 +320,7 @@ hipError_t fgmres(const double* A, int64_t n, int64_t lda, const Precond& M, con
   hipError_t e = hipSuccess;
   Dev d;
   const int m = o.restart;
+  if (m < 1 || m > 1023) return hipErrorInvalidValue;  // one workgroup reduces the m + 1 Hessenberg entries
   double* V = d.get<double>((size_t)n * (m + 1), e);
   if (e) return e;
   double* Z = d.get<double>((size_t)n * m, e);
@@ -310,8 +331,9 @@ hipError_t fgmres(const double* A, int64_t n, int64_t lda, const Precond& M, con
   if (e) return e;
   double* part = d.get<double>(std::max<size_t>(gemv_part_bytes(n, n) / sizeof(double), DOT_BLOCKS * (size_t)(m + 1)), e);
   if (e) return e;
-  double* hdev = d.get<double>(m + 1, e);
+  double* hdev = d.get<double>(2 * (m + 1) + 1, e);  // CGS2 coefficients (two passes) and the new norm^2
   if (e) return e;
+  std::vector<double> hcol(2 * (m + 1) + 1);
   std::vector<double> hp(DOT_BLOCKS * (size_t)(m + 1));
   const unsigned g = nblk(n, 256);
 
@@ -357,34 +379,32 @@ hipError_t fgmres(const double* A, int64_t n, int64_t lda, const Precond& M, con
         gemv_splitk<float>(o.A32, n, n, o.lda32, Z + (int64_t)k * n, w, 1.0, nullptr, 0.0, part, st);
       else
         gemv_splitk<double>(A, n, n, lda, Z + (int64_t)k * n, w, 1.0, nullptr, 0.0, part, st);
-      // classical Gram-Schmidt, twice (CGS2) for FP64 orthogonality
+      // classical Gram-Schmidt, twice (CGS2) for FP64 orthogonality; coefficients and the new norm stay on the
+      // device, and the Hessenberg column comes back in one copy
+      double* h1 = hdev;
+      double* h2 = hdev + (m + 1);
+      double* nrm2 = hdev + 2 * (m + 1);
       for (int pass = 0; pass < 2; ++pass) {
+        double* hp_dev = pass == 0 ? h1 : h2;
         hipLaunchKernelGGL(mdot_part_kernel, dim3(DOT_BLOCKS), dim3(256), 0, st, (const double*)V, (int64_t)n, k + 1,
                            (const double*)w, n, part);
-        e = hipMemcpyAsync(hp.data(), part, sizeof(double) * DOT_BLOCKS * (k + 1), hipMemcpyDeviceToHost, st);
-        if (e) return e;
-        e = hipStreamSynchronize(st);
-        if (e) return e;
-        std::vector<double> h(k + 1, 0.0);
-        for (int j = 0; j <= k; ++j) {
-          double s = 0.0;
-          for (int q = 0; q < DOT_BLOCKS; ++q) s += hp[(size_t)j * DOT_BLOCKS + q];
-          h[j] = s;
-          H[(size_t)j * m + k] += s;
-        }
-        e = hipMemcpyAsync(hdev, h.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, st);
-        if (e) return e;
+        hipLaunchKernelGGL(reduce_parts_kernel, dim3(1), dim3(64 * ((k + 1 + 63) / 64)), 0, st, (const double*)part,
+                           k + 1, DOT_BLOCKS, hp_dev);
         hipLaunchKernelGGL(mgs_update_kernel, dim3(g), dim3(256), 0, st, (const double*)V, (int64_t)n, k + 1,
-                           (const double*)hdev, w, n);
+                           (const double*)hp_dev, w, n);
       }
-      double hh = 0.0;
-      e = host_dot(w, w, n, part, hp, hh, st);
+      hipLaunchKernelGGL(dot_part_kernel, dim3(DOT_BLOCKS), dim3(256), 0, st, (const double*)w, (const double*)w, n,
+                         part);
+      hipLaunchKernelGGL(reduce_parts_kernel, dim3(1), dim3(64), 0, st, (const double*)part, 1, DOT_BLOCKS, nrm2);
+      hipLaunchKernelGGL(scale_by_norm_kernel, dim3(g), dim3(256), 0, st, (const double*)w, (const double*)nrm2,
+                         V + (int64_t)(k + 1) * n, n);
+      e = hipMemcpyAsync(hcol.data(), hdev, sizeof(double) * (2 * (m + 1) + 1), hipMemcpyDeviceToHost, st);
       if (e) return e;
-      hh = std::sqrt(hh);
+      e = hipStreamSynchronize(st);
+      if (e) return e;
+      for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] += hcol[j] + hcol[m + 1 + j];
+      const double hh = std::sqrt(hcol[2 * (m + 1)]);
       H[(size_t)(k + 1) * m + k] = hh;
-      if (hh > 0)
-        hipLaunchKernelGGL(axpby_kernel, dim3(g), dim3(256), 0, st, 1.0 / hh, (const double*)w, 0.0,
-                           V + (int64_t)(k + 1) * n, n);
       // Givens rotations on column k
       for (int j = 0; j < k; ++j) {
         const double t = cs[j] * H[(size_t)j * m + k] + sn[j] * H[(size_t)(j + 1) * m + k];

@@ -137,7 +137,10 @@ int cdj_lstsq(cdj_ctx* ctx, const double* A, int64_t m, int64_t k, const double*
 int cdj_solve(cdj_ctx* ctx, const double* A, const double* b, int64_t n, int64_t nrhs, double* x,
               const cdj_opts* opts, cdj_report* report);
 
-/* ---- device-pointer entry points (inputs already in HBM) ---- */
+/* ---- device-pointer entry points (inputs already in HBM) ----
+ * `stream` NULL (0) selects the context's stream, a blocking stream: it is ordered with work on the legacy null
+ * stream (the default stream of a framework that passes 0), so inputs written there are complete before the
+ * entry point reads them and outputs are complete before later null-stream work reads them. */
 
 /* Assemble A (3N x 3N, leading dimension lda) on `stream` from device arrays. */
 int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,

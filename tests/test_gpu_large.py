@@ -90,6 +90,27 @@ def test_config5_three_solvers_agree_and_match_closed_form(large):
     assert rel <= 0.03, rel
 
 
+def test_config5_lu_repeatable_after_framework_copy(large):
+    """The LU reads a copy that torch queued on its default stream (handle 0 = the context's stream, which must
+    be ordered with it): two factorisations of fresh copies are bitwise identical."""
+    import ctypes
+    A, N3 = large["A"], 3 * large["n"]
+    L = _lib.load()
+    ctx = _lib.ctx()
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = []
+    for _ in range(2):
+        LU = A.clone()
+        ipiv = torch.empty(N3, dtype=torch.int32, device="cuda:0")
+        _lib.check(L.cdj_lu_factor_dev(ctx, ctypes.c_void_p(LU.data_ptr()), N3, N3, ctypes.c_void_p(ipiv.data_ptr()),
+                                       sp))
+        out.append((LU, ipiv))
+    assert torch.equal(out[0][1], out[1][1])
+    diff = float((out[0][0] - out[1][0]).abs().max())
+    print("[gpu-test] repeated LU max |diff| %.3e" % diff, flush=True)
+    assert diff == 0.0
+
+
 def test_config5_grid_field_matches_oracle(large):
     c, n = large["c"], large["n"]
     rng = np.random.default_rng(0)
