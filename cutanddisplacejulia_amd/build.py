@@ -17,9 +17,12 @@ SOURCES = ["td_kernels.hip", "linalg.hip", "mixed.hip", "learned.hip", "batch.hi
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CDJ_OFFLOAD_ARCH", "gfx950")
 # Packed FP32 instructions are disabled in all device code: some of their forms have given silently wrong
-# results on gfx950, so the compiler is told not to emit them.
+# results on gfx950, so the compiler is told not to emit them.  MFMA accumulators are kept in ordinary VGPRs
+# (gfx950's unified register file): with the default AGPR form the compiler copied every loop-carried
+# accumulator to and from AGPRs around each K step, which halved the FP64 MFMA issue rate.
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-mllvm", "-amdgpu-mfma-vgpr-form",
           "-Wno-unused-result"]
 
 

@@ -7,8 +7,8 @@
 //             barrier all workgroups know the pivot, swap locally and apply the rank-1 update in LDS.
 //   laswp   : the panel's row interchanges applied to the columns outside the panel.
 //   trsm    : U12 = L11^-1 A12 with L11 staged in LDS.
-//   update  : A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), 64x64 tiles, K = 64
-//             staged in LDS.  The product is formed as (U^T L^T)^T so each lane's accumulator holds a column
+//   update  : A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), 128x128 tiles, K
+//             streamed through double-buffered LDS.  The product is formed as (U^T L^T)^T so each lane's accumulator holds a column
 //             segment of A22, making the read-modify-write of A22 column-contiguous.
 // Matrices may carry extra right-hand-side columns (ncols > n): they are swapped, solved and updated with the
 // rest, so factoring [A | B] leaves L^-1 P B in place of B.
@@ -72,15 +72,6 @@ __device__ __forceinline__ void panel_barrier(unsigned* count, unsigned target, 
   __syncthreads();
 }
 
-// Cross-workgroup data of the panel kernel (candidates, row j) goes through agent-scope atomic accesses: each XCD
-// has its own L2, so a plain load after the barrier could return a line this XCD cached two columns earlier.
-template <class V> __device__ __forceinline__ void put_agent(V* p, V v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class V> __device__ __forceinline__ V get_agent(const V* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 template <class T>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
                                                       int32_t* ipiv, int* info, double* cval, int* crow, T* crows,
@@ -97,10 +88,6 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
   const int64_t r0 = (int64_t)b * rpb;
   const int64_t r1 = r0 + rpb < m ? r0 + rpb : m;
   const int nr = r1 > r0 ? (int)(r1 - r0) : 0;
-  // This kernel is a cooperative launch: make the previous kernels' writes to A visible here (each XCD has its own
-  // L2, and this launch path is not relied on for the boundary's cache maintenance), and publish this kernel's
-  // writes before it ends.
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   // thread tid owns panel row r0 + tid (rpb <= PT)
   const bool mine = tid < nr;
   const int64_t grow = r0 + tid;
@@ -140,19 +127,19 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     }
     // publish the candidate (value, row, row contents) and, from the owner of row j, row j itself
     if (tid == 0) {
-      put_agent(&cval[par * P + b], bv);
-      put_agent(&crow[par * P + b], br);
+      cval[par * P + b] = bv;
+      crow[par * P + b] = br;
     }
-    if (br >= 0 && tid < nbw) put_agent(&crows[(int64_t)(par * P + b) * LU_NB + tid], sh[tid * rpb + (br - r0)]);
-    if (j >= r0 && j < r1 && tid < nbw) put_agent(&rowj[par * LU_NB + tid], sh[tid * rpb + (j - r0)]);
+    if (br >= 0 && tid < nbw) crows[(int64_t)(par * P + b) * LU_NB + tid] = sh[tid * rpb + (br - r0)];
+    if (j >= r0 && j < r1 && tid < nbw) rowj[par * LU_NB + tid] = sh[tid * rpb + (j - r0)];
     panel_barrier(bar, bar_base + (unsigned)(j + 1) * (unsigned)P, bar_err);
     // global pivot: wave 0 reduces the P candidates and stages the pivot row
     if (wave == 0) {
       double gv = -1.0;
       int gr = -1, gw = -1;
       for (int q = lane; q < P; q += 64) {
-        const int rq = get_agent(&crow[par * P + q]);
-        const double vq = get_agent(&cval[par * P + q]);
+        const int rq = crow[par * P + q];
+        const double vq = cval[par * P + q];
         if (rq >= 0 && (gr < 0 || vq > gv || (vq == gv && rq < gr))) {
           gv = vq;
           gr = rq;
@@ -169,7 +156,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
           gw = w2;
         }
       }
-      if (lane < nbw) u[lane] = gw >= 0 ? get_agent(&crows[(int64_t)(par * P + gw) * LU_NB + lane]) : T(0);
+      if (lane < nbw) u[lane] = gw >= 0 ? crows[(int64_t)(par * P + gw) * LU_NB + lane] : T(0);
       if (lane == 0) {
         s_prow = gr >= 0 ? gr : j;
         if (b == 0) {
@@ -184,7 +171,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     if (mine) {
       if (prow != j) {
         if (grow == prow)
-          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = get_agent(&rowj[par * LU_NB + c]);
+          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = rowj[par * LU_NB + c];
         else if (grow == j)
           for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = u[c];
       }
@@ -241,7 +228,6 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     }
     if (lane == 0) perm[0] = cnt;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 }
 
 // Row interchanges of one panel applied to columns [0, k) and [k + nbw, ncols), as the permutation of the
@@ -293,32 +279,68 @@ __global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k,
 }
 
 // C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N], column-major with leading dimensions ldl, ldu, ldc, any K.
-// 128 x 128 tile per workgroup (4 waves in 2 x 2, 64 x 64 each = 4 x 4 MFMA tiles), K streamed in chunks of
-// GB_K through LDS with the next chunk prefetched into registers, C read and written once at the end.  The
-// product is formed as (U^T L^T)^T so each lane's accumulator items run down a column segment of C.
+// 128 x 128 tile per workgroup (4 waves in 2 x 2, 64 x 64 each = 4 x 4 MFMA tiles).  K is streamed in slices of
+// GB_K through two LDS buffers: while slice s is multiplied, slice s + 1 goes from registers into the other
+// buffer and slice s + 2 is fetched into registers, so each slice costs one workgroup barrier.  Loads are
+// branch-free: row and column indices are clamped into the matrix (a clamped row or column only feeds C entries
+// outside [0, M) x [0, N), which are not written) and k >= K reads as zero.  At most 256 VGPRs per lane, so two
+// workgroups share a CU and one wave's stalls are covered by the other's MFMAs.  The product is formed as
+// (U^T L^T)^T so each lane's accumulator items run down a column segment of C.
 constexpr int GB_M = 128, GB_N = 128, GB_K = 16;
 
+// Tile of workgroup b in a gx x gy tile grid.  Workgroups are dispatched round-robin over the 8 XCDs, each with
+// its own L2: the tiles are renumbered so that every XCD owns one contiguous run of tiles, and the run is walked
+// in groups of 8 row tiles per column tile, so the workgroups resident on one XCD at a time share L and U slices
+// in that XCD's L2.
+__device__ __forceinline__ void xcd_tile(int b, int gx, int gy, int& tx, int& ty) {
+  const int T = gx * gy, q = T / 8, r = T % 8, x = b % 8;
+  const int t = x * q + (x < r ? x : r) + b / 8;
+  constexpr int G = 8;
+  const int grp = t / (G * gy), first = grp * G;
+  const int gs = gx - first < G ? gx - first : G;
+  const int w = t - grp * G * gy;
+  tx = first + w % gs;
+  ty = w / gs;
+}
+
 template <class T>
-__global__ void __launch_bounds__(256) gemm_sub_kernel(const T* __restrict__ L, int64_t ldl, const T* __restrict__ U,
-                                                       int64_t ldu, T* __restrict__ C, int64_t ldc, int64_t M,
-                                                       int64_t N, int64_t K) {
-  __shared__ T Ls[GB_K][GB_M + 4];  // Ls[k][i] = L[i0 + i, k0 + k]
-  __shared__ T Us[GB_K][GB_N + 4];  // Us[k][j] = U[k0 + k, j0 + j]
+__global__ void __launch_bounds__(256, 2) gemm_sub_kernel(const T* __restrict__ L, int64_t ldl,
+                                                          const T* __restrict__ U, int64_t ldu, T* __restrict__ C,
+                                                          int64_t ldc, int64_t M, int64_t N, int64_t K, int gx,
+                                                          int gy) {
+  __shared__ T Ls[2][GB_K][GB_M + 4];  // Ls[buf][k][i] = L[i0 + i, k0 + k]
+  __shared__ T Us[2][GB_K][GB_N + 4];  // Us[buf][k][j] = U[k0 + k, j0 + j]
   const int tid = threadIdx.x;
-  const int64_t i0 = (int64_t)blockIdx.x * GB_M, j0 = (int64_t)blockIdx.y * GB_N;
+  int tx, ty;
+  xcd_tile(blockIdx.x, gx, gy, tx, ty);
+  const int64_t i0 = (int64_t)tx * GB_M, j0 = (int64_t)ty * GB_N;
   const int wave = tid / 64, lane = tid % 64;
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int li = lane % 16, lk = lane / 16;
   constexpr int PER = GB_M * GB_K / 256;  // elements of each operand staged per thread (8)
+  // L: thread -> row il, k = kl + 2 e (rows contiguous across the wave); U: thread -> k = ku, column ju + 16 e
+  // (16 consecutive k of one column per 16 lanes)
+  const int il = tid % GB_M, kl = tid / GB_M;
+  const int ku = tid % GB_K, ju = tid / GB_K;
+  const T* Lrow = L + (i0 + il < M ? i0 + il : M - 1);
   T lr[PER], ur[PER];
   auto fetch = [&](int64_t k0) {
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
-      const int idx = tid + 256 * e;
-      const int i = idx % GB_M, kk = idx / GB_M;  // L: rows contiguous
-      lr[e] = (i0 + i < M && k0 + kk < K) ? L[(i0 + i) + (k0 + kk) * ldl] : T(0);
-      const int kq = idx % GB_K, j = idx / GB_K;  // U: k contiguous within a column
-      ur[e] = (k0 + kq < K && j0 + j < N) ? U[(k0 + kq) + (j0 + j) * ldu] : T(0);
+      const int64_t kL = k0 + kl + 2 * e;
+      const T vl = Lrow[(kL < K ? kL : K - 1) * ldl];
+      lr[e] = kL < K ? vl : T(0);
+      const int64_t kU = k0 + ku;
+      const int64_t jU = j0 + ju + 16 * e;
+      const T vu = U[(kU < K ? kU : K - 1) + (jU < N ? jU : N - 1) * ldu];
+      ur[e] = kU < K ? vu : T(0);
+    }
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      Ls[buf][kl + 2 * e][il] = lr[e];
+      Us[buf][ku][ju + 16 * e] = ur[e];
     }
   };
   typename Mfma<T>::acc acc[4][4];
@@ -326,28 +348,29 @@ __global__ void __launch_bounds__(256) gemm_sub_kernel(const T* __restrict__ L, 
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b2 = 0; b2 < 4; ++b2) acc[a][b2] = typename Mfma<T>::acc{0, 0, 0, 0};
+  const int64_t nk = (K + GB_K - 1) / GB_K;
   fetch(0);
-  for (int64_t k0 = 0; k0 < K; k0 += GB_K) {
-#pragma unroll
-    for (int e = 0; e < PER; ++e) {
-      const int idx = tid + 256 * e;
-      Ls[idx / GB_M][idx % GB_M] = lr[e];
-      Us[idx % GB_K][idx / GB_K] = ur[e];
-    }
-    __syncthreads();
-    if (k0 + GB_K < K) fetch(k0 + GB_K);
+  stash(0);
+  if (nk > 1) fetch(GB_K);
+  __syncthreads();
+  for (int64_t s = 0; s < nk; ++s) {
+    const int buf = (int)(s & 1);
 #pragma unroll
     for (int kk = 0; kk < GB_K; kk += 4) {
       T lv[4], uv[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        lv[t] = Ls[kk + lk][wr + 16 * t + li];
-        uv[t] = Us[kk + lk][wc + 16 * t + li];
+        lv[t] = Ls[buf][kk + lk][wr + 16 * t + li];
+        uv[t] = Us[buf][kk + lk][wc + 16 * t + li];
       }
 #pragma unroll
       for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti) acc[tj][ti] = Mfma<T>::step(uv[tj], lv[ti], acc[tj][ti]);
+    }
+    if (s + 1 < nk) {
+      stash(buf ^ 1);
+      if (s + 2 < nk) fetch((s + 2) * GB_K);
     }
     __syncthreads();
   }
@@ -570,8 +593,8 @@ template <class T>
 void gemm_sub(const T* L, int64_t ldl, const T* U, int64_t ldu, T* C, int64_t ldc, int64_t M, int64_t N, int64_t K,
               hipStream_t st) {
   if (M <= 0 || N <= 0 || K <= 0) return;
-  hipLaunchKernelGGL(gemm_sub_kernel<T>, dim3(nblk(M, GB_M), nblk(N, GB_N)), dim3(256), 0, st, L, ldl, U, ldu, C, ldc,
-                     M, N, K);
+  const int gx = (int)nblk(M, GB_M), gy = (int)nblk(N, GB_N);
+  hipLaunchKernelGGL(gemm_sub_kernel<T>, dim3(gx * gy), dim3(256), 0, st, L, ldl, U, ldu, C, ldc, M, N, K, gx, gy);
 }
 
 // Rows [r0, r1) of columns [c0, c0 + nc): X = L^-1 X with L the unit-lower block of the sub-panels in [r0, r1)
