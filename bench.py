@@ -10,7 +10,10 @@ Also reported: the FP64 LU solve of the same A (`A\\b`), the dominant kernel's r
 FP64 flops from the oracle's operation count per receiver/source pair) and the CPU baseline (the FP64 C++
 restatement of TD.jl, OpenMP over source triangles like TD.jl:146) on a bounded sample of receiver rows.
 
-python bench.py [--steps K] [--warmup W] [--gpus 1]
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--batched] [--large]
+
+--batched adds the 64-crack batched assembly + solve, --large the 16224-triangle elliptical crack (A 48672^2 FP64:
+assembly, three solvers to 1e-10, grid field); both are reported in the same JSON line.
 """
 import argparse
 import ctypes
@@ -104,6 +107,116 @@ def slip_solve_comparison(step_no):
     return r
 
 
+def batched_case():
+    """64 planar elliptical cracks (N_i ~ U(200, 3000), 1-8 remote loads each, uniformly random orientation), all
+    assembled by one persistent work-queue launch in a captured HIP graph with FP32 far / FP64 near entries, then
+    solved mesh by mesh (FP32 LU + FP64 GMRES-IR to 1e-10).  The first call tunes the launch and captures the
+    graph; the second replays it, and its figures are reported."""
+    from cutanddisplacejulia_amd import api, meshes
+    batch = meshes.elliptical_cracks()
+    K, E, lam, nu, G = api.ElasticConstantsCheck(1.0, 0.25)
+    ns = [m["P1"].shape[0] for m in batch]
+    entries = float(sum(9 * n * n for n in ns))
+    api.set_precision("mixed")
+    try:
+        t0 = time.perf_counter()
+        api.batched_slip(batch, nu, G, lam, 0)
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        xs, rep = api.batched_slip(batch, nu, G, lam, 0)
+        wall = time.perf_counter() - t0
+    finally:
+        api.set_precision("fp64")
+    out = {"meshes": len(batch), "triangles": int(sum(ns)), "n_min": int(min(ns)), "n_max": int(max(ns)),
+           "rhs": int(sum(np.atleast_2d(m["stresses"]).shape[0] for m in batch)), "entries": entries,
+           "assembly_ms": rep["assembly_ms"], "assembly_entries_per_s": entries / (rep["assembly_ms"] * 1e-3),
+           "near_entries": rep["near_entries"], "solve_ms": rep["solve_ms"], "residual": rep["residual"],
+           "wall_ms": 1e3 * wall, "first_call_ms": 1e3 * first, "tile": rep["tile"]}
+    _log("batched (64 cracks, %d triangles, %.3g entries): assembly %.1f ms = %.3e entries/s, solves %.0f ms, "
+         "worst residual %.1e, wall %.0f ms (first call %.0f ms)" % (out["triangles"], entries, rep["assembly_ms"],
+                                                                    out["assembly_entries_per_s"], rep["solve_ms"],
+                                                                    rep["residual"], 1e3 * wall, 1e3 * first))
+    return out
+
+
+def large_case():
+    """Planar elliptical crack (a/b = 2) of 16224 triangles in full space, A 48672 x 48672 FP64 (19 GB): FP64
+    assembly, then three solves to an FP64 relative residual of 1e-10 (plain FP64 LU; LU with FP16-operand
+    trailing updates preconditioning FP64 GMRES-IR; learned SPAI FGMRES with FP32 Arnoldi products), then the
+    solved crack's summed strain and displacement on a 200 x 200 grid (4e4 points)."""
+    import torch
+    from cutanddisplacejulia_amd import _lib, api, learned, meshes, spai
+    c = meshes.config5()
+    P1, P2, P3 = c["P1"], c["P2"], c["P3"]
+    n = P1.shape[0]
+    N3 = 3 * n
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    K, E, lam, nu, G = api.ElasticConstantsCheck(c["G"], c["nu"])
+    d = {k: learned.to_device_cols(v) for k, v in dict(P1=P1, P2=P2, P3=P3, N=nrm, M=mid).items()}
+    L = _lib.load()
+    ctx = _lib.ctx()
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    A = torch.empty((N3, N3), dtype=torch.float64, device="cuda:0")
+    learned.assemble_dev(d["N"], d["M"], d["P1"], d["P2"], d["P3"], n, nu, G, lam, 0, A, N3)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    learned.assemble_dev(d["N"], d["M"], d["P1"], d["P2"], d["P3"], n, nu, G, lam, 0, A, N3)
+    torch.cuda.synchronize()
+    asm_ms = 1e3 * (time.perf_counter() - t0)
+    Tn, Tds, Tss = api.SetupTractionVector(api.Stresses(0, 0, 1.0, 0, 0, 0), nrm, n, G, lam)
+    b = torch.from_numpy(np.concatenate([Tn, Tss, Tds])).to("cuda:0")
+    bn = float(torch.linalg.norm(b))
+
+    def res(x):
+        return float(torch.linalg.norm(b - torch.mv(A.t(), x)) / bn)
+
+    out = {"triangles": n, "N": N3, "assembly_ms": asm_ms, "assembly_entries_per_s": float(N3) * N3 / (asm_ms * 1e-3)}
+    _log("large: %d triangles, A %d^2 FP64 assembled in %.1f ms" % (n, N3, asm_ms))
+    # plain FP64 LU + solve
+    LU = A.clone()
+    ipiv = torch.empty(N3, dtype=torch.int32, device="cuda:0")
+    x64 = b.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.cdj_lu_factor_dev(ctx, ctypes.c_void_p(LU.data_ptr()), N3, N3, ctypes.c_void_p(ipiv.data_ptr()), sp))
+    _lib.check(L.cdj_lu_solve_dev(ctx, ctypes.c_void_p(LU.data_ptr()), N3, N3, ctypes.c_void_p(ipiv.data_ptr()),
+                                  ctypes.c_void_p(x64.data_ptr()), 1, N3, sp))
+    torch.cuda.synchronize()
+    t = 1e3 * (time.perf_counter() - t0)
+    del LU
+    out["fp64_lu"] = {"solve_ms": t, "tflops": (2.0 / 3.0) * N3 ** 3 / (t * 1e-3) / 1e12, "residual": res(x64)}
+    _log("large: FP64 LU + solve %.0f ms (%.1f TFLOP/s), residual %.1e" % (t, out["fp64_lu"]["tflops"],
+                                                                           out["fp64_lu"]["residual"]))
+    model = spai.load_model()
+    for name, kw in (("lu16_gmres_ir", dict(pc="lu16")), ("learned_fgmres_k32", dict(pc="learned", model=model,
+                                                                                     krylov32=True))):
+        x = torch.zeros_like(b)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = learned.precond_solve_dev(A, N3, b, x, d["P1"], d["P2"], d["P3"], n, G, **kw)
+        torch.cuda.synchronize()
+        t = 1e3 * (time.perf_counter() - t0)
+        fe = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+        out[name] = {"solve_ms": t, "iterations": rep["iterations"], "residual": res(x), "forward_err_vs_fp64": fe}
+        _log("large: %s %.0f ms, %d iterations, residual %.1e, vs FP64 %.1e" % (name, t, rep["iterations"],
+                                                                               res(x), fe))
+    # summed field of the solved slip on the grid (host arrays in and out)
+    xh = x64.cpu().numpy()
+    Dn, Dss, Dds = xh[:n], xh[n:2 * n], xh[2 * n:]
+    api.TD(c["X"][:10], c["Y"][:10], c["Z"][:10], P1, P2, P3, Dss, Dds, Dn, nu, G, 1, 1, 0, api.Strains(),
+           api.Disps())
+    t0 = time.perf_counter()
+    api.TD(c["X"], c["Y"], c["Z"], P1, P2, P3, Dss, Dds, Dn, nu, G, 1, 1, 0, api.Strains(), api.Disps())
+    t = time.perf_counter() - t0
+    pairs = float(c["X"].size) * n
+    out["grid_field"] = {"points": int(c["X"].size), "pairs": pairs, "wall_ms": 1e3 * t, "pairs_per_s": pairs / t}
+    _log("large: grid field %d points x %d triangles in %.1f ms (%.3e pairs/s, host arrays)" % (
+        c["X"].size, n, 1e3 * t, pairs / t))
+    del A
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +234,11 @@ def main():
                          "to calibrate hardware byte counters against)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--batched", action="store_true",
+                    help="also run the 64-crack batched assembly + solve (graph replay) and report it under 'batched'")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 16224-triangle elliptical crack (A 48672^2 FP64): assembly, three solvers, "
+                         "grid field; reported under 'large'")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every GPU box here has one MI355X and this build has no multi-GPU path "
@@ -218,6 +336,9 @@ def main():
     if not args.no_solve:
         solve = slip_solve_comparison(args.solve_step)
 
+    batched = batched_case() if args.batched else None
+    large = large_case() if args.large else None
+
     cpu = None
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -255,6 +376,10 @@ def main():
         "lu_15000": lu15,
         "solve": solve,
     }
+    if batched is not None:
+        out["batched"] = batched
+    if large is not None:
+        out["large"] = large
     print(json.dumps(out), flush=True)
 
 
