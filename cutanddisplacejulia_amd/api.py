@@ -7,6 +7,8 @@ Names, argument order and meaning follow the reference so that callers (and test
   SetupTractionVector             src/SetupTractionVector.jl:2-41
   CreateFaceNormalAndMidPoint     src/CreateFaceNormalAndMidPoint.jl (P1, P2, P3 form)
   ElasticConstantsCheck           src/ElasticConstantsCheck.jl:37-49 (G, nu form)
+  GetCrackTipElements3D           src/GetCrackTipElements3D.jl:1
+  StressIntensity3D               src/StressIntensity3D.jl:1
 Data types mirror src/CutAndDisplaceJulia.jl:46-91.  Arrays are numpy; N x 3 arrays hold one triangle per row
 (P1[:, 0] are x coordinates), as in the reference.  Every TD / assembly / solve call runs in libcdjhip.so; the
 O(N) host-side helpers here (normals, incentres, remote-stress tractions) prepare inputs as the reference's
@@ -411,6 +413,75 @@ def KnockOutFixedRows(FixedEls, Tn, Tds, Tss):
 
 
 # ---- batched slip solves (many meshes, several remote loads each) -------------------------------------------
+
+
+# ---- crack-tip edges and stress intensities (GetCrackTipElements3D.jl, StressIntensity3D.jl) --------------
+
+_CT_GEOM = 13  # FeLe, FeMd xyz, FeEv xyz, FeM2Ev xyz, FeM2ELe, IntAng, Area
+_CT_K = 4      # K1, K2, K3, StrainEnergy
+
+
+@dataclass
+class TriangleEdges:
+    """TriangleEdges of CutAndDisplaceJulia.jl:57: one edge (P1P2, P1P3 or P2P3) of every triangle; NaN where the
+    edge is not a free (crack-tip) edge, except Area."""
+    FeLe: Any = None
+    FeMd: Any = None
+    FeEv: Any = None
+    FeM2Ev: Any = None
+    FreeFlg: Any = None
+    FeM2ELe: Any = None
+    IntAng: Any = None
+    K1: Any = 0.0
+    K2: Any = 0.0
+    K3: Any = 0.0
+    StrainEnergy: Any = 0.0
+    Area: Any = None
+
+
+def GetCrackTipElements3D(MidPoint, P1, P2, P3, FaceNormalVector):
+    """Free edges of the mesh and their geometry -> (FeP1P2S, FeP1P3S, FeP2P3S) (GetCrackTipElements3D.jl:1)."""
+    P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
+    mid, nrm = _tri(MidPoint), _tri(FaceNormalVector)
+    n = P1.shape[0]
+    if not (P2.shape[0] == P3.shape[0] == mid.shape[0] == nrm.shape[0] == n):
+        raise CdjError(_lib.CDJ_ERR_DIMS, "Element size inputs must be the same dimensions")
+    flags = np.zeros(3 * n, dtype=np.uint8)
+    geom = np.zeros(3 * n * _CT_GEOM)
+    check(load().cdj_crack_tip_elements(ctx(), ptr(mid), ptr(P1), ptr(P2), ptr(P3), ptr(nrm), n,
+                                        flags.ctypes.data, ptr(geom)))
+    out = []
+    for s_ in range(3):
+        g = geom[s_ * n * _CT_GEOM:(s_ + 1) * n * _CT_GEOM].reshape(_CT_GEOM, n).T
+        out.append(TriangleEdges(FeLe=g[:, 0].copy(), FeMd=g[:, 1:4].copy(), FeEv=g[:, 4:7].copy(),
+                                 FeM2Ev=g[:, 7:10].copy(), FreeFlg=flags[s_ * n:(s_ + 1) * n].astype(bool),
+                                 FeM2ELe=g[:, 10].copy(), IntAng=g[:, 11].copy(), Area=g[:, 12].copy()))
+    return tuple(out)
+
+
+def StressIntensity3D(Dn, Dss, Dds, G, ν, FaceNormalVector, FeP1P2S, FeP1P3S, FeP2P3S):
+    """K1, K2, K3 and StrainEnergy on the free edges, from the slip of the tip triangles; returns the three
+    TriangleEdges with those fields filled (StressIntensity3D.jl:1)."""
+    nrm = _tri(FaceNormalVector)
+    n = nrm.shape[0]
+    Dn, Dss, Dds = _vec(Dn, n), _vec(Dss, n), _vec(Dds, n)
+    edges = (FeP1P2S, FeP1P3S, FeP2P3S)
+    flags = np.concatenate([np.asarray(e.FreeFlg, dtype=np.uint8).ravel() for e in edges])
+    geom = np.concatenate([np.column_stack([e.FeLe, e.FeMd, e.FeEv, e.FeM2Ev, e.FeM2ELe, e.IntAng, e.Area])
+                           .ravel(order="F") for e in edges])
+    if flags.size != 3 * n or geom.size != 3 * n * _CT_GEOM:
+        raise CdjError(_lib.CDJ_ERR_DIMS, "Element size inputs must be the same dimensions")
+    geom = f64(geom)
+    K = np.zeros(3 * n * _CT_K)
+    check(load().cdj_stress_intensity(ctx(), ptr(Dn), ptr(Dss), ptr(Dds), n, float(G), float(ν), ptr(nrm),
+                                      flags.ctypes.data, ptr(geom), ptr(K)))
+    out = []
+    for s_, e in enumerate(edges):
+        k = K[s_ * n * _CT_K:(s_ + 1) * n * _CT_K].reshape(_CT_K, n)
+        out.append(TriangleEdges(FeLe=e.FeLe, FeMd=e.FeMd, FeEv=e.FeEv, FeM2Ev=e.FeM2Ev, FreeFlg=e.FreeFlg,
+                                 FeM2ELe=e.FeM2ELe, IntAng=e.IntAng, K1=k[0].copy(), K2=k[1].copy(),
+                                 K3=k[2].copy(), StrainEnergy=k[3].copy(), Area=e.Area))
+    return tuple(out)
 
 
 class _Mesh(ctypes.Structure):

@@ -11,6 +11,7 @@
 
 #include "../../include/cdj.h"
 #include "batch.hpp"
+#include "crack_tip.hpp"
 #include "learned.hpp"
 #include "linalg.hpp"
 #include "mixed.hpp"
@@ -1399,6 +1400,108 @@ int cdj_lstsq(cdj_ctx* ctx, const double* A, int64_t m, int64_t k, const double*
   if (rc) return rc;
   for (int64_t q = 0; q < nrhs; ++q)
     HIPCHK(hipMemcpyAsync(x + q * k, R + q * N + m, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+// ---- crack-tip edges and stress intensities ---------------------------------------------------------------
+namespace {
+
+constexpr int64_t CT_MAX_TRI = 700000000;  // 3n edge indices fit an int
+
+int crack_tip_check(cdj_ctx* ctx, int64_t n) {
+  if (n < 0 || n > CT_MAX_TRI) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  return CDJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdj_crack_tip_elements(cdj_ctx* ctx, const double* midpoints, const double* P1, const double* P2,
+                           const double* P3, const double* normals, int64_t n, uint8_t* free_flags,
+                           double* geometry) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (int rc = crack_tip_check(ctx, n)) return rc;
+  if (!midpoints || !P1 || !P2 || !P3 || !normals || !free_flags || !geometry)
+    return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (n == 0) return CDJ_OK;
+  const size_t v3 = sizeof(double) * 3 * n;
+  DBuf dm, d1, d2, d3, dn, slots, matched, flags, geom;
+  HIPCHK(dm.alloc(v3));
+  HIPCHK(d1.alloc(v3));
+  HIPCHK(d2.alloc(v3));
+  HIPCHK(d3.alloc(v3));
+  HIPCHK(dn.alloc(v3));
+  HIPCHK(slots.alloc(sizeof(int) * crack_tip_slots(n)));
+  HIPCHK(matched.alloc(sizeof(int) * 3 * n));
+  HIPCHK(flags.alloc(3 * n));
+  HIPCHK(geom.alloc(sizeof(double) * 3 * n * CT_GEOM));
+  HIPCHK(hipMemcpyAsync(dm.p, midpoints, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dn.p, normals, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(crack_tip_elements(dm.as<double>(), d1.as<double>(), d2.as<double>(), d3.as<double>(), dn.as<double>(), n,
+                            slots.as<int>(), matched.as<int>(), flags.as<uint8_t>(), geom.as<double>(), st));
+  HIPCHK(hipMemcpyAsync(free_flags, flags.p, 3 * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(geometry, geom.p, sizeof(double) * 3 * n * CT_GEOM, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_stress_intensity(cdj_ctx* ctx, const double* Dn, const double* Dss, const double* Dds, int64_t n, double G,
+                         double nu, const double* normals, const uint8_t* free_flags, const double* geometry,
+                         double* K) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (int rc = crack_tip_check(ctx, n)) return rc;
+  if (!Dn || !Dss || !Dds || !normals || !free_flags || !geometry || !K) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (n == 0) return CDJ_OK;
+  DBuf dD, dn, flags, geom, dK;
+  HIPCHK(dD.alloc(sizeof(double) * 3 * n));
+  HIPCHK(dn.alloc(sizeof(double) * 3 * n));
+  HIPCHK(flags.alloc(3 * n));
+  HIPCHK(geom.alloc(sizeof(double) * 3 * n * CT_GEOM));
+  HIPCHK(dK.alloc(sizeof(double) * 3 * n * CT_K));
+  double* D = dD.as<double>();
+  HIPCHK(hipMemcpyAsync(D, Dn, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(D + n, Dss, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(D + 2 * n, Dds, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dn.p, normals, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(flags.p, free_flags, 3 * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(geom.p, geometry, sizeof(double) * 3 * n * CT_GEOM, hipMemcpyHostToDevice, st));
+  HIPCHK(stress_intensity(D, D + n, D + 2 * n, n, G, nu, dn.as<double>(), flags.as<uint8_t>(), geom.as<double>(),
+                          dK.as<double>(), st));
+  HIPCHK(hipMemcpyAsync(K, dK.p, sizeof(double) * 3 * n * CT_K, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+int cdj_stress_intensity_dev(cdj_ctx* ctx, const double* midpoints, const double* P1, const double* P2,
+                             const double* P3, const double* normals, int64_t n, const double* Dn, const double* Dss,
+                             const double* Dds, double G, double nu, uint8_t* free_flags, double* geometry, double* K,
+                             void* stream) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (int rc = crack_tip_check(ctx, n)) return rc;
+  if (!midpoints || !P1 || !P2 || !P3 || !normals || !Dn || !Dss || !Dds || !free_flags || !geometry || !K)
+    return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (n == 0) return CDJ_OK;
+  DBuf slots, matched;
+  HIPCHK(slots.alloc(sizeof(int) * crack_tip_slots(n)));
+  HIPCHK(matched.alloc(sizeof(int) * 3 * n));
+  HIPCHK(hipEventRecord(ctx->ev0, st));
+  HIPCHK(crack_tip_elements(midpoints, P1, P2, P3, normals, n, slots.as<int>(), matched.as<int>(), free_flags,
+                            geometry, st));
+  HIPCHK(stress_intensity(Dn, Dss, Dds, n, G, nu, normals, free_flags, geometry, K, st));
+  HIPCHK(hipEventRecord(ctx->ev1, st));
   HIPCHK(hipStreamSynchronize(st));
   return CDJ_OK;
 }

@@ -215,6 +215,28 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
 /* Tile chosen by the autotuner for the current batch plan: sources per work item and persistent workgroups. */
 int cdj_batch_tuning(const cdj_ctx* ctx, int32_t* tpb, int32_t* wgs);
 
+/* ---- crack-tip edges and stress intensities (GetCrackTipElements3D.jl:1, StressIntensity3D.jl:1) ----
+ * Edge blocks, in this order: P1P2, P1P3, P2P3 (the reference's FeP1P2S, FeP1P3S, FeP2P3S).
+ * free_flags: n x 3 bytes (block-major: flags of block s at [s n, (s + 1) n)), 1 = free (crack-tip) edge.
+ * geometry:   3 blocks of n x 13 doubles, column-major: FeLe, FeMd x y z, FeEv x y z, FeM2Ev x y z, FeM2ELe,
+ *             IntAng (degrees), Area; NaN on rows whose edge is not free, except Area (every row).
+ * K:          3 blocks of n x 4 doubles, column-major: K1, K2, K3, StrainEnergy (the value StressIntensity3D.jl
+ *             stores under that name: sqrt(K1^2 + K2^2 + K3^2 / (1 - nu))); NaN off the free edges.
+ * Free edges are edges of the mesh with no other edge of identical end points (EdgeConstraints.jl); the GPU finds
+ * them with a hash table in O(n) instead of the reference's O(n^2) pairwise search.  n <= 7e8. */
+int cdj_crack_tip_elements(cdj_ctx* ctx, const double* midpoints, const double* P1, const double* P2,
+                           const double* P3, const double* normals, int64_t n, uint8_t* free_flags,
+                           double* geometry);
+int cdj_stress_intensity(cdj_ctx* ctx, const double* Dn, const double* Dss, const double* Dds, int64_t n, double G,
+                         double nu, const double* normals, const uint8_t* free_flags, const double* geometry,
+                         double* K);
+/* Both steps on device arrays (HBM-resident inputs and outputs, same layouts), on `stream`; the device time of
+ * the two steps is then available from cdj_last_kernel_ms. */
+int cdj_stress_intensity_dev(cdj_ctx* ctx, const double* midpoints, const double* P1, const double* P2,
+                             const double* P3, const double* normals, int64_t n, const double* Dn, const double* Dss,
+                             const double* Dds, double G, double nu, uint8_t* free_flags, double* geometry, double* K,
+                             void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -270,3 +270,37 @@ def mogi1958(D, X, Y, radius, P, nu, G):
     R = np.sqrt(D ** 2 + X ** 2 + Y ** 2)
     c = radius ** 3 * P * (1 - nu) / G
     return X / R ** 3 * c, Y / R ** 3 * c, D / R ** 3 * c
+
+
+def _quadrant(x):
+    """Degrees reduced exactly to rem in [-45, 45] and the quadrant q in 0..3 with x = 90 q + rem (mod 360)."""
+    r = np.fmod(np.asarray(x, dtype=np.float64), 360.0)
+    q = np.round(r / 90.0)
+    rem = r - 90.0 * q
+    return np.deg2rad(rem), q.astype(np.int64) % 4
+
+
+def sind(x):
+    """Sine of an angle in degrees, exactly zero at multiples of 180 (as Julia's sind)."""
+    rad, q = _quadrant(x)
+    return np.choose(q, [np.sin(rad), np.cos(rad), -np.sin(rad), -np.cos(rad)])
+
+
+def cosd(x):
+    """Cosine of an angle in degrees, exactly zero at odd multiples of 90 (as Julia's cosd)."""
+    rad, q = _quadrant(x)
+    return np.choose(q, [np.cos(rad), -np.sin(rad), -np.cos(rad), np.sin(rad)])
+
+
+def tada_inclined_penny_tension(szz, beta_deg, theta_deg, radius, nu):
+    """Stress intensities around a penny-shaped crack of the given radius inclined by beta (degrees, from the
+    loading plane) under remote tension szz (Tada, Paris & Irwin, The Stress Analysis of Cracks Handbook), as
+    Tada_StrIntInclinedPennyTension.jl: K_I uniform, K_II ~ cos(theta), K_III ~ sin(theta).  Angles in degrees
+    with exact zeros of sind/cosd, so K_II vanishes exactly at theta = +-90."""
+    theta = np.asarray(theta_deg, dtype=np.float64)
+    sb, cb = float(sind(beta_deg)), float(cosd(beta_deg))
+    root = np.sqrt(np.pi * radius)
+    K1 = np.ones_like(theta) * (2 / np.pi) * (szz * sb ** 2) * root
+    K2 = (4 / (np.pi * (2 - nu))) * (szz * sb * cb) * cosd(theta) * root
+    K3 = ((4 * (1 - nu)) / (np.pi * (2 - nu))) * (szz * sb * cb) * sind(theta) * root
+    return K1, K2, K3
