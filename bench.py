@@ -217,6 +217,53 @@ def large_case():
     return out
 
 
+def crack_tip_case():
+    """Crack-tip edges and K_I..K_III (GetCrackTipElements3D + StressIntensity3D) from HBM-resident geometry and slip
+    on penny meshes of 2400, 16224 and 998784 triangles (the device entry point, timed with HIP events on its
+    stream), beside the oracle's restatement of the reference algorithm (pairwise edge matching) on the 2400-triangle
+    mesh (host, one thread)."""
+    import torch
+    from cutanddisplacejulia_amd import _lib, api, learned, meshes
+    from oracle import crack_tip as CT
+    L = _lib.load()
+    ctx = _lib.ctx()
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rng = np.random.default_rng(1)
+    out = {}
+    for rings in (20, 52, 408):
+        P1, P2, P3 = meshes.penny(rings, 1.0)
+        n = P1.shape[0]
+        nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+        D = rng.normal(size=(3, n))
+        d = [learned.to_device_cols(v) for v in (mid, P1, P2, P3, nrm)]
+        dD = [torch.from_numpy(v.copy()).to("cuda:0") for v in D]
+        flags = torch.empty(3 * n, dtype=torch.uint8, device="cuda:0")
+        geom = torch.empty(3 * n * 13, dtype=torch.float64, device="cuda:0")
+        K = torch.empty(3 * n * 4, dtype=torch.float64, device="cuda:0")
+        args = ([ctypes.c_void_p(t.data_ptr()) for t in d] + [n] + [ctypes.c_void_p(t.data_ptr()) for t in dD] +
+                [1.0, 0.25, ctypes.c_void_p(flags.data_ptr()), ctypes.c_void_p(geom.data_ptr()),
+                 ctypes.c_void_p(K.data_ptr()), sp])
+        ms = []
+        for rep in range(4):
+            _lib.check(L.cdj_stress_intensity_dev(ctx, *args))
+            if rep:
+                ms.append(L.cdj_last_kernel_ms(ctx))
+        t = float(np.median(ms))
+        nfree = int(flags.sum().item())
+        out[str(n)] = {"triangles": n, "tip_edges": nfree, "device_ms": t, "triangles_per_s": n / (t * 1e-3)}
+        _log("crack tip: %d triangles, %d tip edges, %.3f ms on the device (%.3e triangles/s)" % (n, nfree, t,
+                                                                                               n / (t * 1e-3)))
+        if rings == 20:
+            t0 = time.perf_counter()
+            e = CT.crack_tip_elements(mid, P1, P2, P3, nrm)
+            CT.stress_intensity(D[0], D[1], D[2], 1.0, 0.25, nrm, e)
+            tc = time.perf_counter() - t0
+            out["cpu_oracle_2400"] = {"seconds": tc, "triangles_per_s": n / tc, "cores": 1,
+                                      "kind": "port of the reference's pairwise algorithm (numpy)"}
+            _log("crack tip: oracle (pairwise edge matching as the reference) %.3f s for %d triangles" % (tc, n))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +283,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--batched", action="store_true",
                     help="also run the 64-crack batched assembly + solve (graph replay) and report it under 'batched'")
+    ap.add_argument("--crack-tip", action="store_true",
+                    help="also time the crack-tip edges + stress intensities on 2400 / 16224 / 998784-triangle meshes")
     ap.add_argument("--large", action="store_true",
                     help="also run the 16224-triangle elliptical crack (A 48672^2 FP64): assembly, three solvers, "
                          "grid field; reported under 'large'")
@@ -338,6 +387,7 @@ def main():
 
     batched = batched_case() if args.batched else None
     large = large_case() if args.large else None
+    crack_tip = crack_tip_case() if args.crack_tip else None
 
     cpu = None
     if not args.no_cpu:
@@ -380,6 +430,8 @@ def main():
         out["batched"] = batched
     if large is not None:
         out["large"] = large
+    if crack_tip is not None:
+        out["crack_tip"] = crack_tip
     print(json.dumps(out), flush=True)
 
 

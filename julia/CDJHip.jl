@@ -1,14 +1,16 @@
 # CDJHip.jl - ccall binding of libcdjhip.so (include/cdj.h) for CutAndDisplaceJulia.
 #
 # Drop-in replacements for TD (src/TD.jl:77, :135, :219), CalculateInfluenceMatrices3D
-# (src/CalculateInfluenceMatrices3D.jl:2) and SlipCalculator3D (src/SlipCalculator3D.jl:1) with the same
-# signatures.  Include this file after CutAndDisplaceJulia and call CDJHip.TD(...),
+# (src/CalculateInfluenceMatrices3D.jl:2), SlipCalculator3D (src/SlipCalculator3D.jl:1),
+# GetCrackTipElements3D (src/GetCrackTipElements3D.jl:1) and StressIntensity3D (src/StressIntensity3D.jl:1) with
+# the same signatures.  Include this file after CutAndDisplaceJulia and call CDJHip.TD(...),
 # CDJHip.CalculateInfluenceMatrices3D(...) and CDJHip.SlipCalculator3D(...) where the package's functions were
 # called.  The precision mode lives in module-level settings (CDJHip.PRECISION[], CDJHip.ASSEMBLY[]), so the
 # public signatures stay unchanged.
 module CDJHip
 
-using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, SetupTractionVector, KnockOutFixedRows
+using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, SetupTractionVector, KnockOutFixedRows,
+    TriangleEdges
 
 const LIB = get(ENV, "CDJ_LIB", joinpath(@__DIR__, "..", "cutanddisplacejulia_amd", "libcdjhip.so"))
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)
@@ -134,6 +136,43 @@ function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSF
             _f64(b), 1, C_NULL, x, A, opts, rep))
     end
     return x[1:n], x[n+1:2n], x[2n+1:3n], A, b
+end
+
+# Edge blocks P1P2, P1P3, P2P3; geometry columns FeLe, FeMd (3), FeEv (3), FeM2Ev (3), FeM2ELe, IntAng, Area
+function GetCrackTipElements3D(MidPoint, P1, P2, P3, FaceNormalVector)
+    n = size(P1, 1)
+    flags = zeros(UInt8, 3n); geom = zeros(3n * 13)
+    lock(LOCK) do
+        check(ccall((:cdj_crack_tip_elements, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{UInt8},
+             Ptr{Float64}),
+            ctx(), _f64(MidPoint), _f64(P1), _f64(P2), _f64(P3), _f64(FaceNormalVector), n, flags, geom))
+    end
+    blocks = map(0:2) do s
+        g = reshape(geom[s*n*13+1:(s+1)*n*13], n, 13)
+        TriangleEdges(g[:, 1], g[:, 2:4], g[:, 5:7], g[:, 8:10], flags[s*n+1:(s+1)*n] .== 1, g[:, 11], g[:, 12],
+                      0.0, 0.0, 0.0, 0.0, g[:, 13])
+    end
+    return blocks[1], blocks[2], blocks[3]
+end
+
+function StressIntensity3D(Dn, Dss, Dds, G, ν, FaceNormalVector, FeP1P2S, FeP1P3S, FeP2P3S)
+    n = size(FaceNormalVector, 1)
+    E = (FeP1P2S, FeP1P3S, FeP2P3S)
+    flags = vcat([UInt8.(e.FreeFlg) for e in E]...)
+    geom = vcat([vec(hcat(e.FeLe, e.FeMd, e.FeEv, e.FeM2Ev, e.FeM2ELe, e.IntAng, e.Area)) for e in E]...)
+    K = zeros(3n * 4)
+    lock(LOCK) do
+        check(ccall((:cdj_stress_intensity, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Ptr{Float64}, Ptr{UInt8},
+             Ptr{Float64}, Ptr{Float64}),
+            ctx(), _f64(vec(Dn)), _f64(vec(Dss)), _f64(vec(Dds)), n, G, ν, _f64(FaceNormalVector), flags, geom, K))
+    end
+    for (s, e) in enumerate(E)
+        k = reshape(K[(s-1)*n*4+1:s*n*4], n, 4)
+        e.K1 = k[:, 1]; e.K2 = k[:, 2]; e.K3 = k[:, 3]; e.StrainEnergy = k[:, 4]
+    end
+    return FeP1P2S, FeP1P3S, FeP2P3S
 end
 
 end # module
