@@ -49,37 +49,28 @@ template <> struct Mfma<float> {
   __device__ static int row(int g, int v) { return 4 * g + v; }
 };
 
-// Grid-wide barrier of the panel kernel (all workgroups co-resident: cooperative launch).  A monotonically
-// increasing arrival counter; the barrier completes when it reaches `target`.  The workgroup barrier first
-// completes this workgroup's stores; thread 0's agent-scope release (on the counter) then publishes them and its
-// acquire fence after the wait makes the other workgroups' stores visible to the whole workgroup, so only one
-// thread per workgroup pays for the device-wide cache maintenance.  The wait is bounded: on overrun the error flag
-// is set and the kernel carries on, so it always exits (the host then reports the failure).
-__device__ __forceinline__ void panel_barrier(unsigned* count, unsigned target, int* err) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    long long it = 0;
-    while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (++it > (1ll << 24)) {
-        atomicOr(err, 1);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
+// Per-column exchange of the panel kernel (all workgroups co-resident: cooperative launch).  Each workgroup
+// publishes its pivot candidate for global column c as one 64-bit word in its own slot: the bits of |a| (a
+// non-negative double, so its bit pattern orders like its value) with the low 12 bits replaced by c mod 4096, or
+// the top bit set when it has no candidate.  The word is stored with agent-scope release after the workgroup
+// barrier, so the candidate's row, exact value and row contents (plain stores before it) are published with it.
+// Wave 0 of every workgroup then polls all slots until each carries tag c, takes the agent-scope acquire and reads
+// the winner's data: the poll is the grid barrier and the candidate exchange in one round trip.  A slot is
+// rewritten every second column by the same workgroup, so the previous word's tag (c - 2) never matches; the
+// slots start all ones (tag 4095) before the first column.  The wait is bounded (about a second: co-resident
+// workgroups publish within microseconds): on overrun the error flag is set and the kernel carries on, so it
+// always exits (the host then reports the failure).
+constexpr uint64_t SLOT_TAG = 0xFFFull;
+constexpr uint64_t SLOT_NONE = 1ull << 63;
 
 template <class T>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
                                                       int32_t* ipiv, int* info, double* cval, int* crow, T* crows,
-                                                      T* rowj, unsigned* bar, unsigned bar_base, int* bar_err,
-                                                      int* perm) {
+                                                      T* rowj, unsigned long long* slots, int* bar_err, int* perm) {
   extern __shared__ unsigned char smem_raw[];
   T* sh = reinterpret_cast<T*>(smem_raw);  // column-major panel rows of this workgroup: sh[c * rpb + r]
   __shared__ T u[LU_NB];                   // pivot row of the current column
+  __shared__ T rj[LU_NB];                  // row j, staged for the workgroup that holds the pivot row
   __shared__ double wv[PT / 64];
   __shared__ int wr[PT / 64];
   __shared__ int s_prow;
@@ -93,8 +84,10 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
   const int64_t grow = r0 + tid;
   if (mine)
     for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = A[(k + grow) + (k + c) * lda];
+  __syncthreads();
   for (int j = 0; j < nbw; ++j) {
-    const int par = j & 1;
+    const int par = j & 1;  // k is a multiple of LU_NB, so this is also the global column's parity
+    const uint64_t tag = (uint64_t)(k + j) & SLOT_TAG;
     // local pivot candidate: largest |a|, ties to the lower row (the first maximum, as idamax)
     double v = -1.0;
     int r = -1;
@@ -125,38 +118,101 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
         br = r2;
       }
     }
-    // publish the candidate (value, row, row contents) and, from the owner of row j, row j itself
+    // publish the candidate's exact value, row and contents and, from the holder of row j, row j itself
     if (tid == 0) {
-      cval[par * P + b] = bv;
-      crow[par * P + b] = br;
+      cval[par * MAX_P + b] = bv;
+      crow[par * MAX_P + b] = br;
     }
-    if (br >= 0 && tid < nbw) crows[(int64_t)(par * P + b) * LU_NB + tid] = sh[tid * rpb + (br - r0)];
+    if (br >= 0 && tid < nbw) crows[(int64_t)(par * MAX_P + b) * LU_NB + tid] = sh[tid * rpb + (br - r0)];
     if (j >= r0 && j < r1 && tid < nbw) rowj[par * LU_NB + tid] = sh[tid * rpb + (j - r0)];
-    panel_barrier(bar, bar_base + (unsigned)(j + 1) * (unsigned)P, bar_err);
-    // global pivot: wave 0 reduces the P candidates and stages the pivot row
+    __syncthreads();
+    if (tid == 0) {
+      const uint64_t word = br >= 0 ? (((uint64_t)__double_as_longlong(bv) & ~SLOT_TAG) | tag) : (SLOT_NONE | tag);
+      __hip_atomic_store(&slots[par * MAX_P + b], (unsigned long long)word, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // global pivot: wave 0 waits for every slot, picks the winner and stages the pivot row (and row j)
     if (wave == 0) {
-      double gv = -1.0;
-      int gr = -1, gw = -1;
-      for (int q = lane; q < P; q += 64) {
-        const int rq = crow[par * P + q];
-        const double vq = cval[par * P + q];
-        if (rq >= 0 && (gr < 0 || vq > gv || (vq == gv && rq < gr))) {
-          gv = vq;
-          gr = rq;
-          gw = q;
+      constexpr int QPL = MAX_P / 64;  // slots per lane
+      uint64_t wq[QPL];
+      long long it = 0;
+      for (;;) {
+        bool seen = true;
+#pragma unroll
+        for (int i = 0; i < QPL; ++i) {
+          const int q = lane + 64 * i;
+          wq[i] = q < P ? __hip_atomic_load(&slots[par * MAX_P + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : (SLOT_NONE | tag);
+          seen = seen && (wq[i] & SLOT_TAG) == tag;
+        }
+        if (__all(seen)) break;
+        if (++it > (1ll << 20)) {
+          if (lane == 0) atomicOr(bar_err, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // largest truncated |a| over all slots, ties to the lower workgroup (= lower rows)
+      uint64_t kmax = 0;
+      int qmax = -1;
+#pragma unroll
+      for (int i = 0; i < QPL; ++i) {
+        const int q = lane + 64 * i;
+        if (q < P && !(wq[i] & SLOT_NONE)) {
+          const uint64_t key = wq[i] & ~SLOT_TAG;
+          if (qmax < 0 || key > kmax) {
+            kmax = key;
+            qmax = q;
+          }
         }
       }
       for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(gv, off);
-        const int r2 = __shfl_xor(gr, off);
-        const int w2 = __shfl_xor(gw, off);
-        if (r2 >= 0 && (gr < 0 || v2 > gv || (v2 == gv && r2 < gr))) {
-          gv = v2;
-          gr = r2;
-          gw = w2;
+        const uint64_t k2 = __shfl_xor(kmax, off);
+        const int q2 = __shfl_xor(qmax, off);
+        if (q2 >= 0 && (qmax < 0 || k2 > kmax || (k2 == kmax && q2 < qmax))) {
+          kmax = k2;
+          qmax = q2;
         }
       }
-      if (lane < nbw) u[lane] = gw >= 0 ? crows[(int64_t)(par * P + gw) * LU_NB + lane] : T(0);
+      // the winner's row contents (and row j, for the workgroup holding the pivot row) load together with its
+      // exact value and row
+      int gw = qmax;
+      if (lane < nbw) {
+        u[lane] = gw >= 0 ? crows[(int64_t)(par * MAX_P + gw) * LU_NB + lane] : T(0);
+        if (gw == b) rj[lane] = rowj[par * LU_NB + lane];
+      }
+      double gv = gw >= 0 ? cval[par * MAX_P + gw] : -1.0;
+      int gr = gw >= 0 ? crow[par * MAX_P + gw] : -1;
+      // several slots with the same truncated key (equal to 40 bits): compare their exact values, largest |a|
+      // and then the lower row (idamax's first maximum), and restage the pivot row if the winner changes
+      int ntie = 0;
+#pragma unroll
+      for (int i = 0; i < QPL; ++i) {
+        const int q = lane + 64 * i;
+        ntie += __popcll(__ballot(q < P && !(wq[i] & SLOT_NONE) && (wq[i] & ~SLOT_TAG) == kmax));
+      }
+      if (gw >= 0 && ntie > 1) {
+#pragma unroll
+        for (int i = 0; i < QPL; ++i) {
+          unsigned long long msk = __ballot(lane + 64 * i < P && !(wq[i] & SLOT_NONE) && (wq[i] & ~SLOT_TAG) == kmax);
+          while (msk) {
+            const int q = 64 * i + __ffsll((long long)msk) - 1;
+            msk &= msk - 1;
+            const double vq = cval[par * MAX_P + q];
+            const int rq = crow[par * MAX_P + q];
+            if (vq > gv || (vq == gv && rq < gr)) {
+              gv = vq;
+              gr = rq;
+              gw = q;
+            }
+          }
+        }
+        if (gw != qmax && lane < nbw) {
+          u[lane] = crows[(int64_t)(par * MAX_P + gw) * LU_NB + lane];
+          if (gw == b) rj[lane] = rowj[par * LU_NB + lane];
+        }
+      }
       if (lane == 0) {
         s_prow = gr >= 0 ? gr : j;
         if (b == 0) {
@@ -168,21 +224,32 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     }
     __syncthreads();
     const int prow = s_prow;
-    if (mine) {
-      if (prow != j) {
-        if (grow == prow)
-          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = rowj[par * LU_NB + c];
-        else if (grow == j)
-          for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = u[c];
-      }
-      const T piv = u[j];
-      if (grow > j && piv != T(0)) {
-        const T l = sh[j * rpb + tid] / piv;
-        sh[j * rpb + tid] = l;
-        for (int c = j + 1; c < nbw; ++c) sh[c * rpb + tid] -= l * u[c];
+    // row interchange, one column per thread: the pivot row takes row j's contents, row j the pivot row's
+    if (prow != j && tid < nbw) {
+      if (prow >= r0 && prow < r1) sh[tid * rpb + (prow - r0)] = rj[tid];
+      if (j >= r0 && j < r1) sh[tid * rpb + (j - r0)] = u[tid];
+    }
+    __syncthreads();
+    // multiplier and rank-1 update of this thread's row, eight columns at a time so the LDS reads overlap
+    const T piv = u[j];
+    if (mine && grow > j && piv != T(0)) {
+      const T l = sh[j * rpb + tid] / piv;
+      sh[j * rpb + tid] = l;
+      for (int c0 = j + 1; c0 < nbw; c0 += 8) {
+        T x[8], y[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int c = c0 + t < nbw ? c0 + t : nbw - 1;
+          x[t] = sh[c * rpb + tid];
+          y[t] = u[c];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+          if (c0 + t < nbw) sh[(c0 + t) * rpb + tid] = x[t] - l * y[t];
       }
     }
   }
+  __syncthreads();
   if (mine)
     for (int c = 0; c < nbw; ++c) A[(k + grow) + (k + c) * lda] = sh[c * rpb + tid];
   // block 0, wave 0: the panel's swaps as one row permutation of the touched rows (rows j < nbw and the pivot
@@ -614,8 +681,8 @@ void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t n
 
 size_t lu_workspace_bytes(int64_t n) {
   (void)n;
-  return sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) + sizeof(int) * 2 * MAX_P +
-         sizeof(int) * (2 + 1 + 4 * LU_NB) + sizeof(float) * (size_t)(n > 0 ? n : 1) + 1024;
+  return sizeof(unsigned long long) * 2 * MAX_P + sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) +
+         sizeof(int) * 2 * MAX_P + sizeof(int) * (2 + 1 + 4 * LU_NB) + sizeof(float) * (size_t)(n > 0 ? n : 1) + 1024;
 }
 
 // Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
@@ -649,6 +716,8 @@ template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
                           hipStream_t st, bool fp16_update) {
   char* w = static_cast<char*>(ws);
+  unsigned long long* slots = reinterpret_cast<unsigned long long*>(w);  // the panels' candidate words
+  w += sizeof(unsigned long long) * 2 * MAX_P;
   double* cval = reinterpret_cast<double*>(w);
   w += sizeof(double) * 2 * MAX_P;
   T* crows = reinterpret_cast<T*>(w);
@@ -657,15 +726,16 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   w += sizeof(double) * 2 * LU_NB;
   int* crow = reinterpret_cast<int*>(w);
   w += sizeof(int) * 2 * MAX_P;
-  unsigned* bar = reinterpret_cast<unsigned*>(w);  // barrier arrival counter, then its error flag
+  unsigned* bar = reinterpret_cast<unsigned*>(w);  // unused word, then the panels' wait-overrun flag
   int* bar_err = reinterpret_cast<int*>(bar + 1);
   int* perm = bar_err + 1;                          // 1 + 4 LU_NB ints: the last panel's row permutation
   float* sigma = reinterpret_cast<float*>(perm + 1 + 4 * LU_NB);  // n column scales (FP16 trailing update)
   if (ncols > n) fp16_update = false;
-  unsigned bar_base = 0;
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
   if (e != hipSuccess) return e;
   e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(slots, 0xFF, sizeof(unsigned long long) * 2 * MAX_P, st);  // tag 4095: matches no column 0, 1
   if (e != hipSuccess) return e;
   static bool lds_set = false;  // raise the dynamic LDS limit once per process (128 KiB panels)
   if (!lds_set) {
@@ -688,11 +758,10 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       int rpb = (int)((m + P - 1) / P);
       if (P > MAX_P) return hipErrorInvalidValue;
       size_t shm = sizeof(T) * (size_t)rpb * LU_NB;
-      void* args[] = {&A,    (void*)&lda, &k,    (void*)&m, (void*)&nbw, &rpb,     &ipiv,    &info,
-                      &cval, &crow,       &crows, &rowj,    &bar,        &bar_base, &bar_err, &perm};
+      void* args[] = {&A,    (void*)&lda, &k,    (void*)&m, (void*)&nbw, &rpb,    &ipiv, &info,
+                      &cval, &crow,       &crows, &rowj,    &slots,      &bar_err, &perm};
       e = hipLaunchCooperativeKernel((const void*)lu_panel_kernel<T>, dim3(P), dim3(PT), args, shm, st);
       if (e != hipSuccess) return e;
-      bar_base += (unsigned)nbw * (unsigned)P;
       if (ncols > nbw)
         hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 2)), dim3(256), 0, st, A, lda, ncols, k, nbw,
                            (const int*)perm);
