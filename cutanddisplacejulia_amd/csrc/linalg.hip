@@ -459,44 +459,109 @@ __global__ void __launch_bounds__(256, 2) gemm_sub_kernel(const T* __restrict__ 
 // operands rounded to FP16 and the products accumulated in FP32 on v_mfma_f32_16x16x16f16.  |L| <= 1 under
 // partial pivoting; each column c of U is scaled by a power of two sigma[c] (max |U[:, c]| * sigma in [0.5, 1))
 // before rounding, and the accumulated product divided by it, so FP16's range never clips the update.
+// The operands are rounded once per update into packed FP16 copies with k contiguous (Lh: row i's K values,
+// Uh: column j's K values, K <= LU_OUTER), so the GEMM streams them with 16-byte loads.
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
-constexpr int GH_K = 32;
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+constexpr int GH_K = 32;        // k per LDS stage
+constexpr int GH_LD = LU_OUTER; // k stride of the packed copies
 
-__global__ void __launch_bounds__(256) gemm_sub_f16_kernel(const float* __restrict__ L, int64_t ldl,
-                                                           const float* __restrict__ U, int64_t ldu,
-                                                           float* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
-                                                           int64_t K, const float* __restrict__ sigma) {
-  // operands staged as packed quads of consecutive k: Lq[i][q] = L[i0 + i, k0 + 4q .. 4q + 3] and
-  // Uq[j][q] = sigma[j0 + j] U[k0 + 4q .. 4q + 3, j0 + j]; one 8-byte LDS access per MFMA operand
-  constexpr int QK = GH_K / 4;  // quads per row (8)
-  __shared__ h4 Lq[GB_M][QK + 1];
-  __shared__ h4 Uq[GB_N][QK + 1];
+// Bytes from the column scales to the packed FP16 copies: n floats, rounded up to 256 bytes
+inline size_t f16_pack_offset(int64_t n) { return ((sizeof(float) * (size_t)n + 255) / 256) * 256; }
+
+// Lh[i * GH_LD + k] = fp16(L[i, k]) (zero for k >= K): 32 rows per workgroup, read down the columns (lanes over
+// rows) and written along the rows (lanes over 8-value groups of k) through an LDS transpose
+__global__ void __launch_bounds__(256) pack_l_f16_kernel(const float* __restrict__ L, int64_t ldl, int64_t M,
+                                                         int64_t K, _Float16* __restrict__ Lh) {
+  __shared__ _Float16 tile[32][GH_LD + 8];
   const int tid = threadIdx.x;
-  const int64_t i0 = (int64_t)blockIdx.x * GB_M, j0 = (int64_t)blockIdx.y * GB_N;
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  for (int idx = tid; idx < 32 * GH_LD; idx += 256) {
+    const int r = idx % 32, k = idx / 32;
+    const int64_t i = i0 + r;
+    tile[r][k] = (_Float16)((i < M && k < K) ? L[i + k * ldl] : 0.f);
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 32 * (GH_LD / 8); idx += 256) {
+    const int r = idx / (GH_LD / 8), kq = idx % (GH_LD / 8);
+    const int64_t i = i0 + r;
+    if (i >= M) continue;
+    h8 v;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = tile[r][8 * kq + t];
+    *reinterpret_cast<h8*>(Lh + i * GH_LD + 8 * kq) = v;
+  }
+}
+
+// sigma[j] = 2^-e with max_k |U[k, j]| in [2^(e-1), 2^e) (1 for a zero column) and Uh[j * GH_LD + k] =
+// fp16(sigma[j] U[k, j]); one wave per column, lane l taking k = l, l + 64, ...
+__global__ void __launch_bounds__(256) pack_u_f16_kernel(const float* __restrict__ U, int64_t ldu, int64_t N,
+                                                         int64_t K, float* __restrict__ sigma,
+                                                         _Float16* __restrict__ Uh) {
+  const int64_t j = blockIdx.x * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  if (j >= N) return;
+  float x[GH_LD / 64];
+  float mx = 0.f;
+#pragma unroll
+  for (int t = 0; t < GH_LD / 64; ++t) {
+    const int64_t k = lane + 64 * t;
+    x[t] = k < K ? U[k + j * ldu] : 0.f;
+    mx = fmaxf(mx, fabsf(x[t]));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  int e = 0;
+  if (mx > 0.f) (void)frexpf(mx, &e);
+  const float sg = mx > 0.f ? ldexpf(1.0f, -e) : 1.0f;
+  if (lane == 0) sigma[j] = sg;
+#pragma unroll
+  for (int t = 0; t < GH_LD / 64; ++t) Uh[j * GH_LD + lane + 64 * t] = (_Float16)(x[t] * sg);
+}
+
+// 128 x 128 tile per workgroup (4 waves in 2 x 2, 64 x 64 each), K in GH_K slices through two LDS buffers (one
+// barrier per slice), operands from the packed copies: 4 consecutive lanes read one row's (column's) 32 k of a
+// slice as two 16-byte loads each.  Row and column indices are clamped (the extra rows and columns only reach C
+// entries that are not written); the packed copies are zero beyond K.
+__global__ void __launch_bounds__(256, 2) gemm_sub_f16_kernel(const _Float16* __restrict__ Lh,
+                                                              const _Float16* __restrict__ Uh,
+                                                              float* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
+                                                              int64_t K, const float* __restrict__ sigma, int gx,
+                                                              int gy) {
+  constexpr int QK = GH_K / 4;  // quads of k per row and slice (8)
+  __shared__ h4 Lq[2][GB_M][QK + 1];
+  __shared__ h4 Uq[2][GB_N][QK + 1];
+  const int tid = threadIdx.x;
+  int tx, ty;
+  xcd_tile(blockIdx.x, gx, gy, tx, ty);
+  const int64_t i0 = (int64_t)tx * GB_M, j0 = (int64_t)ty * GB_N;
   const int wave = tid / 64, lane = tid % 64;
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int li = lane % 16, lk = lane / 16;
-  // L: thread -> row i = tid % 128, quads q = tid / 128 + 2 e (e < 4); U: thread -> quad q = tid % 8,
-  // column j = tid / 8 + 32 e (e < 4), so 8 threads read one column's 32 consecutive k (coalesced)
-  float lr[4][4], ur[4][4];
-  const int li_ = tid % GB_M, lq0 = tid / GB_M;
-  const int uq = tid % QK, uj0 = tid / QK;
+  // staging: thread -> row (column) r = tid / 4 + 64 e, quads 2 (tid % 4) .. + 1
+  const int sp = tid % 4, sr = tid / 4;
+  const _Float16* lsrc[2];
+  const _Float16* usrc[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    const int64_t i = i0 + sr + 64 * e, j = j0 + sr + 64 * e;
+    lsrc[e] = Lh + (i < M ? i : M - 1) * GH_LD + 8 * sp;
+    usrc[e] = Uh + (j < N ? j : N - 1) * GH_LD + 8 * sp;
+  }
+  h8 lr[2], ur[2];
   auto fetch = [&](int64_t k0) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int q = lq0 + 2 * e;
+    for (int e = 0; e < 2; ++e) {
+      lr[e] = *reinterpret_cast<const h8*>(lsrc[e] + k0);
+      ur[e] = *reinterpret_cast<const h8*>(usrc[e] + k0);
+    }
+  };
+  auto stash = [&](int buf) {
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int64_t kk = k0 + 4 * q + t;
-        lr[e][t] = (i0 + li_ < M && kk < K) ? L[(i0 + li_) + kk * ldl] : 0.f;
-      }
-      const int j = uj0 + 32 * e;
-      const float sg = j0 + j < N ? sigma[j0 + j] : 0.f;
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int64_t kk = k0 + 4 * uq + t;
-        ur[e][t] = (kk < K && j0 + j < N) ? U[kk + (j0 + j) * ldu] * sg : 0.f;
-      }
+    for (int e = 0; e < 2; ++e) {
+      Lq[buf][sr + 64 * e][2 * sp] = h4{lr[e][0], lr[e][1], lr[e][2], lr[e][3]};
+      Lq[buf][sr + 64 * e][2 * sp + 1] = h4{lr[e][4], lr[e][5], lr[e][6], lr[e][7]};
+      Uq[buf][sr + 64 * e][2 * sp] = h4{ur[e][0], ur[e][1], ur[e][2], ur[e][3]};
+      Uq[buf][sr + 64 * e][2 * sp + 1] = h4{ur[e][4], ur[e][5], ur[e][6], ur[e][7]};
     }
   };
   f4 acc[4][4];
@@ -504,28 +569,30 @@ __global__ void __launch_bounds__(256) gemm_sub_f16_kernel(const float* __restri
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b2 = 0; b2 < 4; ++b2) acc[a][b2] = f4{0.f, 0.f, 0.f, 0.f};
+  const int64_t nk = (K + GH_K - 1) / GH_K;
   fetch(0);
-  for (int64_t k0 = 0; k0 < K; k0 += GH_K) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      Lq[li_][lq0 + 2 * e] = h4{(_Float16)lr[e][0], (_Float16)lr[e][1], (_Float16)lr[e][2], (_Float16)lr[e][3]};
-      Uq[uj0 + 32 * e][uq] = h4{(_Float16)ur[e][0], (_Float16)ur[e][1], (_Float16)ur[e][2], (_Float16)ur[e][3]};
-    }
-    __syncthreads();
-    if (k0 + GH_K < K) fetch(k0 + GH_K);
+  stash(0);
+  if (nk > 1) fetch(GH_K);
+  __syncthreads();
+  for (int64_t s = 0; s < nk; ++s) {
+    const int buf = (int)(s & 1);
 #pragma unroll
     for (int kq = 0; kq < QK; kq += 4) {  // 16 k per MFMA: quads kq .. kq + 3, lane group lk takes quad kq + lk
       h4 lv[4], uv[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        lv[t] = Lq[wr + 16 * t + li][kq + lk];
-        uv[t] = Uq[wc + 16 * t + li][kq + lk];
+        lv[t] = Lq[buf][wr + 16 * t + li][kq + lk];
+        uv[t] = Uq[buf][wc + 16 * t + li][kq + lk];
       }
 #pragma unroll
       for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
         for (int ti = 0; ti < 4; ++ti)
           acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x16f16(uv[tj], lv[ti], acc[tj][ti], 0, 0, 0);
+    }
+    if (s + 1 < nk) {
+      stash(buf ^ 1);
+      if (s + 2 < nk) fetch((s + 2) * GH_K);
     }
     __syncthreads();
   }
@@ -543,18 +610,6 @@ __global__ void __launch_bounds__(256) gemm_sub_f16_kernel(const float* __restri
         if (row < M) C[row + col * ldc] -= acc[tj][ti][v] * inv;
       }
     }
-}
-
-// sigma[c] = 2^-e with max_k |U[k, c]| in [2^(e-1), 2^e) (1 for a zero column), c < N, k < K.
-__global__ void colscale_kernel(const float* __restrict__ U, int64_t ldu, int64_t N, int64_t K,
-                                float* __restrict__ sigma) {
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  float m = 0.f;
-  for (int64_t k = 0; k < K; ++k) m = fmaxf(m, fabsf(U[k + c * ldu]));
-  int e = 0;
-  if (m > 0.f) (void)frexpf(m, &e);
-  sigma[c] = m > 0.f ? ldexpf(1.0f, -e) : 1.0f;
 }
 
 // ---- triangular solves with nrhs <= 64 right-hand sides -------------------------------------------------
@@ -680,9 +735,12 @@ void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t n
 }  // namespace
 
 size_t lu_workspace_bytes(int64_t n) {
-  (void)n;
+  const int64_t nn = n > 0 ? n : 1;
+  // slots, candidates, pivot rows, flags and the panel permutation; then n column scales and the two packed FP16
+  // operand copies of the FP16 trailing update (n x LU_OUTER halves each)
   return sizeof(unsigned long long) * 2 * MAX_P + sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) +
-         sizeof(int) * 2 * MAX_P + sizeof(int) * (2 + 1 + 4 * LU_NB) + sizeof(float) * (size_t)(n > 0 ? n : 1) + 1024;
+         sizeof(int) * 2 * MAX_P + sizeof(int) * (2 + 1 + 4 * LU_NB) + f16_pack_offset(nn) +
+         2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
 }
 
 // Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
@@ -707,9 +765,16 @@ void trailing_update<float>(float* L, float* U, float* C, int64_t lda, int64_t M
     gemm_sub<float>(L, lda, U, lda, C, lda, M, N, K, st);
     return;
   }
-  hipLaunchKernelGGL(colscale_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, (const float*)U, lda, N, K, sigma);
-  hipLaunchKernelGGL(gemm_sub_f16_kernel, dim3(nblk(M, GB_M), nblk(N, GB_N)), dim3(256), 0, st, (const float*)L, lda,
-                     (const float*)U, lda, C, lda, M, N, K, (const float*)sigma);
+  // packed FP16 copies after the column scales in the same workspace region, 256-byte aligned
+  const uintptr_t pk = (reinterpret_cast<uintptr_t>(sigma) + sizeof(float) * (size_t)(M > N ? M : N) + 255) &
+                       ~(uintptr_t)255;
+  _Float16* Lh = reinterpret_cast<_Float16*>(pk);
+  _Float16* Uh = Lh + M * GH_LD;
+  hipLaunchKernelGGL(pack_l_f16_kernel, dim3(nblk(M, 32)), dim3(256), 0, st, (const float*)L, lda, M, K, Lh);
+  hipLaunchKernelGGL(pack_u_f16_kernel, dim3(nblk(N, 4)), dim3(256), 0, st, (const float*)U, lda, N, K, sigma, Uh);
+  const int gx = (int)nblk(M, GB_M), gy = (int)nblk(N, GB_N);
+  hipLaunchKernelGGL(gemm_sub_f16_kernel, dim3(gx * gy), dim3(256), 0, st, (const _Float16*)Lh, (const _Float16*)Uh, C,
+                     lda, M, N, K, (const float*)sigma, gx, gy);
 }
 
 template <class T>

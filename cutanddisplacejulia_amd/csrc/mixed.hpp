@@ -20,7 +20,7 @@ struct GmresOpts {
   int restart = 30;
   int max_iter = 200;
   int max_restarts = 20;
-  long long spin_limit = 1LL << 26;
+  long long spin_limit = 1LL << 20;  // about a second of polling: the chained blocks publish within microseconds
   bool zero_guess = false;  // start from x = 0 instead of x = M^-1 b
   // FP32 copy of A for the Arnoldi products (nullptr: FP64 A).  The residual at every restart stays FP64, so a
   // cycle on the FP32 operator is one step of iterative refinement; each cycle then stops once its estimated
