@@ -67,6 +67,18 @@ class MixedBoundaryConditions:
 
 
 @dataclass
+class FrictionParameters:
+    µ: Any = 0.0
+    Sf: Any = 0.0
+
+
+@dataclass
+class MixedBoundaryConditionsFriction:
+    MixedBoundaryConditions: Any = None
+    FrictionParameters: Any = None
+
+
+@dataclass
 class TractionInf:
     DnTn: Any = None
     DnTss: Any = None
@@ -267,10 +279,13 @@ def SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, 
     """Slip on the elements that satisfies the traction boundary conditions (SlipCalculator3D.jl:1-33).
 
     Returns (Dn, Dss, Dds, A, b) like the reference.  b may also be given directly as a 3N x nrhs array in place
-    of BoundaryConditions, in which case Dn/Dss/Dds are N x nrhs.
+    of BoundaryConditions, in which case Dn/Dss/Dds are N x nrhs.  With MixedBoundaryConditionsFriction the
+    frictional method runs instead (SlipCalculator3D.jl:36-84) and, like the reference, returns (Dn, Dss, Dds).
     """
     nrm = f64(FaceNormalVector)
     n = nrm.shape[0]
+    if isinstance(BoundaryConditions, MixedBoundaryConditionsFriction):
+        return _slip_friction(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BoundaryConditions, FixedEls)
     fixed = np.asarray(FixedEls).ravel() == 1 if FixedEls is not None else np.zeros(n, bool)
     if isinstance(BoundaryConditions, np.ndarray):
         b = f64(BoundaryConditions.reshape(3 * n, -1, order="F"))
@@ -304,6 +319,26 @@ def SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, 
     if squeeze:
         Dn, Dss, Dds, b = Dn[:, 0], Dss[:, 0], Dds[:, 0], b[:, 0]
     return Dn, Dss, Dds, A, b
+
+
+def _slip_friction(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BC, FixedEls):
+    """SlipCalculator3D(..., ::MixedBoundaryConditionsFriction, FixedEls) (SlipCalculator3D.jl:36-84)."""
+    n = nrm.shape[0]
+    if FixedEls is not None and np.any(np.asarray(FixedEls).ravel() == 1):
+        raise CdjError(_lib.CDJ_ERR_ARG, "fixed elements are not supported by the frictional solve")
+    Tn, Tds, Tss = SetupTractionVector(BC.MixedBoundaryConditions, nrm, n, G, λ)
+    b = f64(np.concatenate([Tn, Tss, Tds]))
+    mu = _vec(BC.FrictionParameters.µ, n)
+    Sf = _vec(BC.FrictionParameters.Sf, n)
+    P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
+    Dn, Dss, Dds = np.zeros(n), np.zeros(n), np.zeros(n)
+    rep = Report()
+    check(load().cdj_slip_friction(ctx(), ptr(P1), ptr(P2), ptr(P3), n, float(ν), float(G), float(λ),
+                                   ptr(f64(MidPoint)), ptr(nrm), int(HSFlag), ptr(b), ptr(mu), ptr(Sf), ptr(Dn),
+                                   ptr(Dss), ptr(Dds), ctypes.byref(rep)))
+    LAST_REPORT.clear()
+    LAST_REPORT.update(rep.as_dict())
+    return Dn, Dss, Dds
 
 
 def solve(A, b):

@@ -12,6 +12,7 @@
 #include "../../include/cdj.h"
 #include "batch.hpp"
 #include "crack_tip.hpp"
+#include "friction.hpp"
 #include "learned.hpp"
 #include "linalg.hpp"
 #include "mixed.hpp"
@@ -65,8 +66,10 @@ int hip_fail(cdj_ctx* c, hipError_t e, const char* where) {
 // Scoped device allocation.
 struct DBuf {
   void* p = nullptr;
-  ~DBuf() {
+  ~DBuf() { reset(); }
+  void reset() {
     if (p) (void)hipFree(p);
+    p = nullptr;
   }
   hipError_t alloc(size_t bytes) {
     if (bytes == 0) bytes = 8;
@@ -1503,6 +1506,125 @@ int cdj_stress_intensity_dev(cdj_ctx* ctx, const double* midpoints, const double
   HIPCHK(stress_intensity(Dn, Dss, Dds, n, G, nu, normals, free_flags, geometry, K, st));
   HIPCHK(hipEventRecord(ctx->ev1, st));
   HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+// ---- frictional slip (SlipCalculator3D.jl:36-233) ----------------------------------------------------------
+extern "C" {
+
+int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,
+                      double G, double lambda, const double* midpoints, const double* normals, int32_t hs_flag,
+                      const double* b, const double* mu, const double* Sf, double* Dn, double* Dss, double* Dds,
+                      cdj_report* report) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  if (!P1 || !P2 || !P3 || !midpoints || !normals || !b || !mu || !Sf || !Dn || !Dss || !Dds)
+    return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  if (report) *report = cdj_report{};
+  if (n == 0) return CDJ_OK;
+  if (5 * n > 65535) return fail(ctx, CDJ_ERR_DIMS, "frictional solve too large (5 n > 65535)");
+  // SlipCalculator3D.jl:52-59: no element can overcome its frictional strength -> no slip
+  bool locked = true;
+  for (int64_t i = 0; i < n && locked; ++i) {
+    const double fr = b[i] * mu[i];
+    locked = (-std::fabs(b[n + i]) > fr) && (-std::fabs(b[2 * n + i]) > fr);
+  }
+  if (locked) {
+    for (int64_t i = 0; i < n; ++i) Dn[i] = Dss[i] = Dds[i] = 0.0;
+    return CDJ_OK;
+  }
+  // scale: mean half perimeter / G (SlipCalculator3D.jl:66-68, AreaOfTriangle3D.jl)
+  double hsum = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double L[3];
+    const double* Q[3] = {P1, P2, P3};
+    for (int s = 0; s < 3; ++s) {
+      const double* A = Q[s];
+      const double* B = Q[(s + 1) % 3];
+      const double dx = A[i] - B[i], dy = A[n + i] - B[n + i], dz = A[2 * n + i] - B[2 * n + i];
+      L[s] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    hsum += (L[0] + L[1] + L[2]) / 2.0;
+  }
+  const double scale = (hsum / (double)n) / G;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int64_t N3 = 3 * n, N5 = 5 * n;
+  const size_t v3 = sizeof(double) * 3 * n;
+  DBuf dn, dm, d1, d2, d3, AI, db, dD, dmu, dSf, a, bb, J, x, vec, red, ipiv, info, ws, D3;
+  HIPCHK(dn.alloc(v3));
+  HIPCHK(dm.alloc(v3));
+  HIPCHK(d1.alloc(v3));
+  HIPCHK(d2.alloc(v3));
+  HIPCHK(d3.alloc(v3));
+  HIPCHK(hipMemcpyAsync(dn.p, normals, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dm.p, midpoints, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(AI.alloc(sizeof(double) * N3 * 2 * N3));  // [A | I], then [LU | A^-1]
+  cdj_opts o{};
+  o.precision = CDJ_PREC_FP64;
+  o.assembly = CDJ_ASM_FP64;
+  int rc = cdj_assemble_dev(ctx, dn.as<double>(), dm.as<double>(), d1.as<double>(), d2.as<double>(), d3.as<double>(),
+                            n, nu, G, lambda, hs_flag, AI.as<double>(), N3, &o, st);
+  if (rc) return rc;
+  scale_matrix(AI.as<double>(), N3 * N3, scale, st);
+  HIPCHK(ipiv.alloc(sizeof(int32_t) * N5));
+  HIPCHK(info.alloc(sizeof(int)));
+  HIPCHK(ws.alloc(lu_workspace_bytes(N5)));
+  HIPCHK(invert(AI.as<double>(), N3, ipiv.as<int32_t>(), info.as<int>(), ws.p, st));
+  int h_info = 0;
+  HIPCHK(hipMemcpyAsync(&h_info, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (h_info < 0) return fail(ctx, CDJ_ERR_DEVICE, "LU panel barrier wait timed out");
+  if (h_info > 0) return fail(ctx, CDJ_ERR_SINGULAR, "matrix is exactly singular (zero pivot " + std::to_string(h_info) + ")");
+  const double* C = AI.as<double>() + N3 * N3;
+  HIPCHK(db.alloc(v3));
+  HIPCHK(dD.alloc(v3));
+  HIPCHK(hipMemcpyAsync(db.p, b, v3, hipMemcpyHostToDevice, st));
+  gemv<double>(C, N3, N3, N3, db.as<double>(), dD.as<double>(), 1.0, 0.0, st);
+  HIPCHK(dmu.alloc(sizeof(double) * n));
+  HIPCHK(dSf.alloc(sizeof(double) * n));
+  HIPCHK(hipMemcpyAsync(dmu.p, mu, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dSf.p, Sf, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(a.alloc(sizeof(double) * N5 * N5));
+  HIPCHK(bb.alloc(sizeof(double) * N5));
+  build_friction_lcp(C, n, dD.as<double>(), dmu.as<double>(), dSf.as<double>(), a.as<double>(), bb.as<double>(), st);
+  HIPCHK(hipGetLastError());
+  AI.reset();  // the inverse is folded into a: free its 18 n^2 doubles before the Newton scratch
+  HIPCHK(J.alloc(sizeof(double) * N5 * N5));
+  HIPCHK(x.alloc(sizeof(double) * N5));
+  HIPCHK(vec.alloc(sizeof(double) * 8 * N5));
+  HIPCHK(red.alloc(sizeof(double) * 8));
+  FnParams fp;
+  FnResult fr;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  hipError_t fe = fischer_newton(a.as<double>(), bb.as<double>(), N5, x.as<double>(), fp, fr, J.as<double>(),
+                                 ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(), st);
+  if (fe != hipSuccess) return hip_fail(ctx, fe, "Fischer-Newton");
+  HIPCHK(D3.alloc(v3));
+  friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(), st);
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipMemcpyAsync(Dn, D3.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(Dss, D3.as<double>() + n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(Dds, D3.as<double>() + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (report) {
+    report->path = CDJ_PREC_FP64;
+    report->iterations = fr.iterations;
+    report->residual = fr.err;
+    report->solve_ms = ms;
+  }
   return CDJ_OK;
 }
 

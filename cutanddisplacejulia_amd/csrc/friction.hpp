@@ -1,0 +1,45 @@
+// Frictional slip (SlipCalculator3D.jl's MixedBoundaryConditionsFriction method): the linear complementarity
+// problem built from the inverse collocation matrix, solved by a Fischer-Newton method on the GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cdj {
+
+// A[0:count] *= s
+void scale_matrix(double* A, int64_t count, double s, hipStream_t st);
+
+// LCP [y] = [a][x] + [b] (a: 5n x 5n, leading dimension 5n) of Kaven et al. (2012) eq. 28 as modified in
+// SlipCalculator3D.jl:90-167, from C = A^-1 (3n x 3n, leading dimension 3n), D = C b (3n), the friction
+// coefficients mu (n) and cohesions Sf (n); D is added to the first 3n entries of bb.
+void build_friction_lcp(const double* C, int64_t n, const double* D, const double* mu, const double* Sf, double* a,
+                        double* bb, hipStream_t st);
+
+struct FnParams {
+  int max_iter = 100;
+  double tol_abs = 1e-26;  // on 0.5 ||phi||^2
+  double alpha = 0.5;      // Armijo step reduction
+  double beta = 1e-3;      // Armijo sufficient-decrease constant
+  double gamma = 1e-28;    // perturbation size and smallest tau^2
+  double rho = 2.220446049250313e-16;  // sufficient-descent constant (machine epsilon)
+};
+
+struct FnResult {
+  int iterations = 0;
+  double err = 0.0;        // final 0.5 ||phi(a x + b, x)||^2
+  int gradient_steps = 0;  // Newton systems that were singular or gave no descent (gradient direction taken)
+};
+
+// x >= 0 with y = a x + b >= 0 and x'y = 0 (a: N x N, leading dimension N; x: N, output).  Newton systems are solved
+// by the GPU LU (ws: lu_workspace_bytes(N); J: N x N scratch; ipiv: N ints; info: one int; vec: 8 N doubles;
+// red: 8 doubles), the Fischer-Burmeister merit function is reduced on the device and read back once per trial.
+hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* x, const FnParams& p, FnResult& r,
+                          double* J, int32_t* ipiv, int* info, void* ws, double* vec, double* red, hipStream_t st);
+
+// Slip from the LCP solution: Dn = y1 s, Dss = (y2 - x4) s, Dds = (y3 - x5) s with y = a x + b and s the scale
+// applied to A (D3: 3n doubles, [Dn; Dss; Dds]).
+void friction_slip(const double* a, const double* bb, const double* x, int64_t n, double scale, double* y, double* D3,
+                   hipStream_t st);
+
+}  // namespace cdj

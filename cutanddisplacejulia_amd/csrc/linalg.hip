@@ -898,6 +898,37 @@ hipError_t tri_solve_upper(const T* LU, int64_t n, int64_t lda, T* B, int64_t nr
   return hipGetLastError();
 }
 
+// U X = B for many right-hand sides: per 64-row block from the bottom, the diagonal-block solve (one wave per
+// right-hand side) and the update of the rows above as an MFMA GEMM.
+template <class T>
+hipError_t tri_solve_upper_many(const T* LU, int64_t n, int64_t lda, T* B, int64_t nrhs, int64_t ldb,
+                                hipStream_t st) {
+  const int64_t nblocks = (n + LU_NB - 1) / LU_NB;
+  for (int64_t bi = nblocks - 1; bi >= 0; --bi) {
+    const int64_t r0 = bi * LU_NB;
+    const int nb = (int)std::min<int64_t>(LU_NB, n - r0);
+    hipLaunchKernelGGL((trsv_diag_kernel<T, false>), dim3(nblk(nrhs, 4)), dim3(256), 0, st, LU, lda, r0, nb, B, ldb,
+                       nrhs);
+    if (r0 > 0) gemm_sub<T>(LU + r0 * lda, lda, B + r0, ldb, B, ldb, r0, nrhs, nb, st);
+  }
+  return hipGetLastError();
+}
+
+template <class T>
+__global__ void set_identity_kernel(T* X, int64_t n, int64_t ldx) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int64_t i = t % n, j = t / n;
+  X[i + j * ldx] = i == j ? T(1) : T(0);
+}
+
+hipError_t invert(double* AI, int64_t n, int32_t* ipiv, int* info, void* ws, hipStream_t st) {
+  hipLaunchKernelGGL(set_identity_kernel<double>, dim3(nblk(n * n, 256)), dim3(256), 0, st, AI + n * n, n, n);
+  hipError_t e = lu_factor_cols<double>(AI, n, 2 * n, n, ipiv, info, ws, st, false);
+  if (e != hipSuccess) return e;
+  return tri_solve_upper_many<double>(AI, n, n, AI + n * n, n, n, st);
+}
+
 template <class T>
 hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int32_t* ipiv, T* B, int64_t nrhs, int64_t ldb,
                     hipStream_t st) {
@@ -950,6 +981,8 @@ template hipError_t lu_factor_cols<float>(float*, int64_t, int64_t, int64_t, int
 template hipError_t lu_solve<double>(const double*, int64_t, int64_t, const int32_t*, double*, int64_t, int64_t, hipStream_t);
 template hipError_t lu_solve<float>(const float*, int64_t, int64_t, const int32_t*, float*, int64_t, int64_t, hipStream_t);
 template hipError_t tri_solve_upper<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
+template hipError_t tri_solve_upper_many<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t,
+                                                 hipStream_t);
 template hipError_t tri_solve_upper<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);
 template hipError_t tri_solve_lower_unit<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);
 template hipError_t tri_solve_lower_unit<float>(const float*, int64_t, int64_t, float*, int64_t, int64_t, hipStream_t);

@@ -37,6 +37,14 @@ hipError_t tri_solve_lower_unit(const T* LU, int64_t n, int64_t lda, T* B, int64
 template <class T>
 hipError_t tri_solve_upper(const T* LU, int64_t n, int64_t lda, T* B, int64_t nrhs, int64_t ldb, hipStream_t st);
 
+// U X = B for many right-hand sides (B n x nrhs): blocked, the updates on MFMA.
+template <class T>
+hipError_t tri_solve_upper_many(const T* LU, int64_t n, int64_t lda, T* B, int64_t nrhs, int64_t ldb, hipStream_t st);
+
+// Inverse of the n x n FP64 matrix in AI[:, 0:n] (an n x 2n buffer, leading dimension n): AI[:, n:2n] receives
+// A^-1 (AI[:, 0:n] the LU factors).  ipiv: n ints; info as lu_factor; ws: lu_workspace_bytes(n).
+hipError_t invert(double* AI, int64_t n, int32_t* ipiv, int* info, void* ws, hipStream_t st);
+
 // Solve A X = B with the factors of lu_factor (B overwritten by X, n x nrhs, leading dimension ldb).
 template <class T>
 hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int32_t* ipiv, T* B, int64_t nrhs, int64_t ldb,

@@ -215,6 +215,20 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
 /* Tile chosen by the autotuner for the current batch plan: sources per work item and persistent workgroups. */
 int cdj_batch_tuning(const cdj_ctx* ctx, int32_t* tpb, int32_t* wgs);
 
+/* Frictional slip: replaces SlipCalculator3D(P1,P2,P3,nu,G,lambda,MidPoint,FaceNormalVector,HSFlag,
+ * MixedBoundaryConditionsFriction,FixedEls) (SlipCalculator3D.jl:36-233) for FixedEls all zero.  b: 3n tractions
+ * [Tn; Tss; Tds] of the remote stresses plus the prescribed tractions; mu, Sf: n friction coefficients and
+ * cohesions.  When no element can overcome its frictional strength the slip is zero (SlipCalculator3D.jl:52-59).
+ * Otherwise the collocation matrix (scaled by mean half perimeter / G) is inverted on the GPU, the 5n x 5n
+ * complementarity problem of SlipCalculator3D.jl:90-167 is built there and solved by a Fischer-Newton method
+ * (the Newton systems factored by the GPU LU).  report: iterations = Newton iterations, residual = final
+ * 0.5 ||phi||^2 of the Fischer-Burmeister function, solve_ms = device time of the complementarity solve.
+ * 5 n <= 65535. */
+int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,
+                      double G, double lambda, const double* midpoints, const double* normals, int32_t hs_flag,
+                      const double* b, const double* mu, const double* Sf, double* Dn, double* Dss, double* Dds,
+                      cdj_report* report);
+
 /* ---- crack-tip edges and stress intensities (GetCrackTipElements3D.jl:1, StressIntensity3D.jl:1) ----
  * Edge blocks, in this order: P1P2, P1P3, P2P3 (the reference's FeP1P2S, FeP1P3S, FeP2P3S).
  * free_flags: n x 3 bytes (block-major: flags of block s at [s n, (s + 1) n)), 1 = free (crack-tip) edge.

@@ -304,3 +304,22 @@ def tada_inclined_penny_tension(szz, beta_deg, theta_deg, radius, nu):
     K2 = (4 / (np.pi * (2 - nu))) * (szz * sb * cb) * cosd(theta) * root
     K3 = ((4 * (1 - nu)) / (np.pi * (2 - nu))) * (szz * sb * cb) * sind(theta) * root
     return K1, K2, K3
+
+
+def pollard_segall1987_slip(G, nu, syy, sxy, x, a):
+    """Slip of a 2-D crack of half length a under remote shear (and normal) stress (Pollard & Segall 1987), as
+    PollardSegall1987_FractureSlipProfile.jl: 2 (1 - nu) / G sqrt(a^2 - x^2) (syy + sxy)."""
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(invalid="ignore"):
+        return (2 * (1 - nu) / G) * np.sqrt(a ** 2 - x ** 2) * (syy + sxy)
+
+
+def burgmann1994_linear_friction_slip(G, nu, Ts, x, a):
+    """Slip of a 2-D crack of half length a under remote shear Ts with a frictional strength rising linearly from
+    zero at the centre to Ts at the tips (Burgmann et al. 1994), as Burgmann1994_FractureLinearFrictionSlipProfile.jl
+    (x = 0 gives NaN, as there)."""
+    x = np.abs(np.asarray(x, dtype=np.float64))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = np.sqrt(a ** 2 - x ** 2)
+        ach = np.arccosh(a / x)
+        return ((1 - nu) / G) * (2 * ((Ts * r) - ((Ts * (1 / np.pi)) * (r + (x ** 2 / a) * ach))))
