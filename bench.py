@@ -264,6 +264,34 @@ def crack_tip_case():
     return out
 
 
+def friction_case():
+    """Frictional slip of the reference's Burgmann case (a 497-triangle vertical blade fault, mu = 0, cohesion
+    rising linearly from the centre): the GPU path (inverse, 2485 x 2485 complementarity problem, Fischer-Newton
+    with GPU-LU Newton steps) against the oracle's restatement of the same steps on the host (numpy/LAPACK, all
+    cores), both from host arrays."""
+    from cutanddisplacejulia_amd import api
+    from oracle import friction as FR
+    from tests.test_oracle_friction import blade
+    P1, P2, P3, nrm, mid, n, G, nu, lam, stress = blade()
+    z = np.zeros(n)
+    Sf = np.abs(mid[:, 0])
+    bc = api.MixedBoundaryConditionsFriction(api.MixedBoundaryConditions(api.Stresses(*stress), api.Tractions(z, z, z)),
+                                             api.FrictionParameters(0.0, Sf))
+    api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 0, bc, z)  # warm-up
+    t0 = time.perf_counter()
+    api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 0, bc, z)
+    gpu = time.perf_counter() - t0
+    rep = dict(api.LAST_REPORT)
+    t0 = time.perf_counter()
+    _, _, _, info = FR.slip_friction(P1, P2, P3, nu, G, lam, mid, nrm, 0, stress, (z, z, z), 0.0, Sf)
+    cpu = time.perf_counter() - t0
+    out = {"triangles": n, "lcp_size": 5 * n, "gpu_wall_ms": 1e3 * gpu, "gpu_lcp_ms": rep["solve_ms"],
+           "newton_iterations": rep["iterations"], "cpu_oracle_s": cpu, "cpu_newton_iterations": info["iterations"]}
+    _log("friction (Burgmann case, %d triangles, LCP %d): GPU %.0f ms wall (complementarity solve %.0f ms, %d Newton "
+         "steps); oracle on the host %.2f s" % (n, 5 * n, 1e3 * gpu, rep["solve_ms"], rep["iterations"], cpu))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +311,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--batched", action="store_true",
                     help="also run the 64-crack batched assembly + solve (graph replay) and report it under 'batched'")
+    ap.add_argument("--friction", action="store_true",
+                    help="also time the frictional slip of the 497-triangle Burgmann case against the host oracle")
     ap.add_argument("--crack-tip", action="store_true",
                     help="also time the crack-tip edges + stress intensities on 2400 / 16224 / 998784-triangle meshes")
     ap.add_argument("--large", action="store_true",
@@ -388,6 +418,7 @@ def main():
     batched = batched_case() if args.batched else None
     large = large_case() if args.large else None
     crack_tip = crack_tip_case() if args.crack_tip else None
+    friction = friction_case() if args.friction else None
 
     cpu = None
     if not args.no_cpu:
@@ -432,6 +463,8 @@ def main():
         out["large"] = large
     if crack_tip is not None:
         out["crack_tip"] = crack_tip
+    if friction is not None:
+        out["friction"] = friction
     print(json.dumps(out), flush=True)
 
 

@@ -10,7 +10,7 @@
 module CDJHip
 
 using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, SetupTractionVector, KnockOutFixedRows,
-    TriangleEdges
+    TriangleEdges, MixedBoundaryConditionsFriction
 
 const LIB = get(ENV, "CDJ_LIB", joinpath(@__DIR__, "..", "cutanddisplacejulia_amd", "libcdjhip.so"))
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)
@@ -136,6 +136,27 @@ function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSF
             _f64(b), 1, C_NULL, x, A, opts, rep))
     end
     return x[1:n], x[n+1:2n], x[2n+1:3n], A, b
+end
+
+function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag,
+                          BoundaryConditions::MixedBoundaryConditionsFriction, FixedEls)
+    n = size(FaceNormalVector, 1)
+    any(FixedEls .== 1) && error("fixed elements: use the reference implementation")
+    (Tn, Tds, Tss) = SetupTractionVector(BoundaryConditions.MixedBoundaryConditions, FaceNormalVector, n, G, λ)
+    b = [Tn; Tss; Tds]
+    µ = zeros(n) .+ BoundaryConditions.FrictionParameters.µ
+    Sf = zeros(n) .+ BoundaryConditions.FrictionParameters.Sf
+    Dn = zeros(n); Dss = zeros(n); Dds = zeros(n)
+    rep = Ref(Report())
+    lock(LOCK) do
+        check(ccall((:cdj_slip_friction, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Float64, Ptr{Float64},
+             Ptr{Float64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+             Ref{Report}),
+            ctx(), _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, _f64(MidPoint), _f64(FaceNormalVector), HSFlag,
+            _f64(b), µ, Sf, Dn, Dss, Dds, rep))
+    end
+    return Dn, Dss, Dds
 end
 
 # Edge blocks P1P2, P1P3, P2P3; geometry columns FeLe, FeMd (3), FeEv (3), FeM2Ev (3), FeM2ELe, IntAng, Area
