@@ -934,7 +934,8 @@ int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const do
                           const double* P3, int64_t n, double G, const double* A, int64_t lda, const double* b,
                           double* x, const cdj_opts* opts, cdj_report* report, void* stream) {
   if (!ctx || !A || !b || !x) return CDJ_ERR_ARG;
-  if (pc == CDJ_PC_LEARNED && !mlp) return fail(ctx, CDJ_ERR_ARG, "learned preconditioner needs a model");
+  const bool learned = pc == CDJ_PC_LEARNED || pc == CDJ_PC_LEARNED_BF16;
+  if (learned && !mlp) return fail(ctx, CDJ_ERR_ARG, "learned preconditioner needs a model");
   if (n <= 0 || lda < 3 * n) return fail(ctx, CDJ_ERR_DIMS, "invalid matrix dimensions");
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
@@ -954,17 +955,22 @@ int cdj_precond_solve_dev(cdj_ctx* ctx, const cdj_mlp* mlp, int32_t pc, const do
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
   MeshGuard g;
-  DBuf jac, acts;
+  DBuf jac, acts, wbf;
   const float* Y = nullptr;
-  if (pc == CDJ_PC_JACOBI || pc == CDJ_PC_LEARNED) {
+  if (pc == CDJ_PC_JACOBI || learned) {
     HIPCHK(jac.alloc(sizeof(double) * 9 * n));
     HIPCHK(jacobi_blocks(A, n, lda, jac.as<double>(), st));
   }
-  if (pc == CDJ_PC_LEARNED) {
+  if (learned) {
     const Mlp& m = mlp->m;
     HIPCHK(mesh_prepare(P1, P2, P3, n, m.k, g.f, st));
     HIPCHK(acts.alloc(sizeof(float) * n * (2 * m.H + m.O)));
-    HIPCHK(mlp_forward(m, g.f.feat, n, acts.as<float>(), st));
+    if (pc == CDJ_PC_LEARNED_BF16) {
+      HIPCHK(wbf.alloc(sizeof(unsigned short) * mlp_bf16_weight_count(m)));
+      HIPCHK(mlp_forward_bf16(m, g.f.feat, n, acts.as<float>(), wbf.as<unsigned short>(), st));
+    } else {
+      HIPCHK(mlp_forward(m, g.f.feat, n, acts.as<float>(), st));
+    }
     Y = acts.as<float>() + 2 * n * m.H;
   }
   const double invG = 1.0 / G;
@@ -1625,6 +1631,40 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
     report->residual = fr.err;
     report->solve_ms = ms;
   }
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+// ---- MLP evaluation (the learned preconditioner's network on one mesh) -------------------------------------
+extern "C" {
+
+int cdj_mlp_forward(cdj_ctx* ctx, const cdj_mlp* mlp, const double* P1, const double* P2, const double* P3, int64_t n,
+                    int32_t bf16, float* Y) {
+  if (!ctx || !mlp || !P1 || !P2 || !P3 || !Y) return CDJ_ERR_ARG;
+  if (n <= 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const Mlp& m = mlp->m;
+  const size_t v3 = sizeof(double) * 3 * n;
+  DBuf d1, d2, d3, acts, wbf;
+  HIPCHK(d1.alloc(v3));
+  HIPCHK(d2.alloc(v3));
+  HIPCHK(d3.alloc(v3));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, v3, hipMemcpyHostToDevice, st));
+  MeshGuard g;
+  HIPCHK(mesh_prepare(d1.as<double>(), d2.as<double>(), d3.as<double>(), n, m.k, g.f, st));
+  HIPCHK(acts.alloc(sizeof(float) * n * (2 * m.H + m.O)));
+  if (bf16) {
+    HIPCHK(wbf.alloc(sizeof(unsigned short) * mlp_bf16_weight_count(m)));
+    HIPCHK(mlp_forward_bf16(m, g.f.feat, n, acts.as<float>(), wbf.as<unsigned short>(), st));
+  } else {
+    HIPCHK(mlp_forward(m, g.f.feat, n, acts.as<float>(), st));
+  }
+  HIPCHK(hipMemcpyAsync(Y, acts.as<float>() + 2 * n * m.H, sizeof(float) * n * m.O, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   return CDJ_OK;
 }
 

@@ -202,6 +202,75 @@ __global__ void __launch_bounds__(256) gemm_f32_kernel(int M, int N, int K, cons
     }
 }
 
+// ---- BF16 inference GEMM on MFMA: C = act(A W + bias), A (M x K, FP32 row-major) rounded to BF16 as it is
+// staged, W (K x N, BF16 row-major), FP32 accumulation on v_mfma_f32_16x16x16bf16.  64 x 64 tile per workgroup,
+// each wave 16 rows x 64 columns (4 MFMA tiles); K and N multiples of 16.
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (NaN stays a quiet NaN)
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ x, int64_t n, unsigned short* __restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                        const unsigned short* __restrict__ W, int ldw,
+                                                        float* __restrict__ C, int ldc, const float* __restrict__ bias,
+                                                        int act) {
+  __shared__ unsigned short As[64][16 + 4];  // As[r][k]
+  __shared__ unsigned short Ws[64][16 + 4];  // Ws[c][k] (transposed)
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int li = lane % 16, lk = lane / 16;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  f4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      const int r = idx / 16, kk = idx % 16;  // 16 lanes read one row's 16 consecutive k
+      const int gm = m0 + r;
+      As[r][kk] = f2bf(gm < M ? A[(int64_t)gm * lda + k0 + kk] : 0.f);
+      const int c = idx % 64, kw = idx / 64;  // lanes read consecutive columns of one k row of W
+      const int gn = n0 + c;
+      Ws[c][kw] = gn < N ? W[(int64_t)(k0 + kw) * ldw + gn] : (unsigned short)0;
+    }
+    __syncthreads();
+    s4 a, bq[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) a[t] = (short)As[16 * wave + li][4 * lk + t];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) bq[tj][t] = (short)Ws[16 * tj + li][4 * lk + t];
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj) acc[tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[tj], acc[tj], 0, 0, 0);
+    __syncthreads();
+  }
+  // item v of lane l: row 4 (l / 16) + v of the wave's 16 rows, column l % 16 of MFMA tile tj
+#pragma unroll
+  for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int gm = m0 + 16 * wave + 4 * lk + v, gn = n0 + 16 * tj + li;
+      if (gm < M && gn < N) {
+        float val = acc[tj][v];
+        if (bias) val += bias[gn];
+        if (act == 1) val = tanhf(val);
+        C[(int64_t)gm * ldc + gn] = val;
+      }
+    }
+}
+
 // g = g * (1 - h^2)  (tanh derivative)
 __global__ void tanh_back_kernel(const float* __restrict__ h, float* __restrict__ g, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -474,6 +543,37 @@ hipError_t mlp_forward(const Mlp& m, const float* feat, int64_t n, float* acts, 
   gemm_f32(false, false, N, m.H, m.F, feat, m.F, m.params + off[0], m.H, h1, m.H, m.params + off[1], 1, 0, st);
   gemm_f32(false, false, N, m.H, m.H, h1, m.H, m.params + off[2], m.H, h2, m.H, m.params + off[3], 1, 0, st);
   gemm_f32(false, false, N, m.O, m.H, h2, m.H, m.params + off[4], m.O, Y, m.O, m.params + off[5], 0, 0, st);
+  return hipGetLastError();
+}
+
+// Forward pass with BF16 weights and activations on MFMA (FP32 accumulation, biases and tanh in FP32); wbuf
+// receives the BF16 copies of W1, W2, W3 (mlp_bf16_weight_count(m) values).
+int64_t mlp_bf16_weight_count(const Mlp& m) { return (int64_t)m.F * m.H + (int64_t)m.H * m.H + (int64_t)m.H * m.O; }
+
+hipError_t mlp_forward_bf16(const Mlp& m, const float* feat, int64_t n, float* acts, unsigned short* wbuf,
+                            hipStream_t st) {
+  if (m.F % 16 || m.H % 16 || m.O % 16) return hipErrorInvalidValue;
+  int64_t off[6];
+  m.offsets(off);
+  unsigned short* W1 = wbuf;
+  unsigned short* W2 = W1 + (int64_t)m.F * m.H;
+  unsigned short* W3 = W2 + (int64_t)m.H * m.H;
+  const int64_t c1 = (int64_t)m.F * m.H, c2 = (int64_t)m.H * m.H, c3 = (int64_t)m.H * m.O;
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(c1, 256)), dim3(256), 0, st, (const float*)(m.params + off[0]), c1, W1);
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(c2, 256)), dim3(256), 0, st, (const float*)(m.params + off[2]), c2, W2);
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(c3, 256)), dim3(256), 0, st, (const float*)(m.params + off[4]), c3, W3);
+  float* h1 = acts;
+  float* h2 = acts + n * m.H;
+  float* Y = acts + 2 * n * m.H;
+  const int N = (int)n;
+  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(nblk(m.H, 64), nblk(N, 64)), dim3(256), 0, st, N, m.H, m.F, feat, m.F,
+                     (const unsigned short*)W1, m.H, h1, m.H, (const float*)(m.params + off[1]), 1);
+  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(nblk(m.H, 64), nblk(N, 64)), dim3(256), 0, st, N, m.H, m.H,
+                     (const float*)h1, m.H, (const unsigned short*)W2, m.H, h2, m.H,
+                     (const float*)(m.params + off[3]), 1);
+  hipLaunchKernelGGL(gemm_bf16_kernel, dim3(nblk(m.O, 64), nblk(N, 64)), dim3(256), 0, st, N, m.O, m.H,
+                     (const float*)h2, m.H, (const unsigned short*)W3, m.O, Y, m.O, (const float*)(m.params + off[5]),
+                     0);
   return hipGetLastError();
 }
 

@@ -30,6 +30,11 @@ struct MeshFeat {
 };
 
 hipError_t mlp_init(Mlp& m, int k, int hidden, uint64_t seed);
+// BF16 inference: weights and activations rounded to BF16, products accumulated in FP32 on MFMA; wbuf holds
+// mlp_bf16_weight_count(m) BF16 values.
+int64_t mlp_bf16_weight_count(const Mlp& m);
+hipError_t mlp_forward_bf16(const Mlp& m, const float* feat, int64_t n, float* acts, unsigned short* wbuf,
+                            hipStream_t st);
 void mlp_free(Mlp& m);
 hipError_t mesh_prepare(const double* P1, const double* P2, const double* P3, int64_t n, int k, MeshFeat& f,
                         hipStream_t st);

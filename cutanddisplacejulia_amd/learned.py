@@ -10,7 +10,7 @@ import numpy as np
 from . import _lib
 from ._lib import CdjError, Opts, Report, check, ctx, f64, load, ptr
 
-PC = {"none": 0, "jacobi": 1, "learned": 2, "lu32": 3, "lu16": 3}
+PC = {"none": 0, "jacobi": 1, "learned": 2, "lu32": 3, "lu16": 3, "learned_bf16": 4}
 
 _p = ctypes.c_void_p
 _d = ctypes.c_double
@@ -97,6 +97,15 @@ class LearnedPreconditioner:
                                         _dptr(dA), int(lda), int(nsamples), int(iters), float(lr), int(seed),
                                         ctypes.byref(loss), _stream()))
         return loss.value
+
+    def forward(self, P1, P2, P3, bf16=False):
+        """The network's n x O outputs for a mesh (host arrays), in FP32 or with BF16 weights and activations."""
+        P1, P2, P3 = f64(P1), f64(P2), f64(P3)
+        n = P1.shape[0]
+        Y = np.zeros((n, 9 * self.k), dtype=np.float32)
+        check(_bind().cdj_mlp_forward(ctx(), self.h, ptr(P1), ptr(P2), ptr(P3), n, int(bool(bf16)),
+                                      Y.ctypes.data_as(_p)))
+        return Y
 
     def loss_grad(self, P1, P2, P3, G, jac, B, X):
         """Loss, gradient and the GPU's features / neighbours / geometry records for host inputs."""

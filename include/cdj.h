@@ -51,6 +51,8 @@ extern "C" {
 #define CDJ_PC_JACOBI 1       /* exact inverse of each element's 3x3 self block */
 #define CDJ_PC_LEARNED 2      /* block Jacobi + MLP-predicted neighbour blocks */
 #define CDJ_PC_LU32 3         /* FP32 LU */
+#define CDJ_PC_LEARNED_BF16 4 /* as CDJ_PC_LEARNED, the network evaluated with BF16 weights and activations on MFMA
+                                 (FP32 accumulation, biases and tanh in FP32) */
 
 typedef struct cdj_ctx cdj_ctx;
 typedef struct cdj_mlp cdj_mlp;
@@ -214,6 +216,12 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
 
 /* Tile chosen by the autotuner for the current batch plan: sources per work item and persistent workgroups. */
 int cdj_batch_tuning(const cdj_ctx* ctx, int32_t* tpb, int32_t* wgs);
+
+/* The learned preconditioner's network on one mesh (host geometry, n x 3 column-major): Y receives the n x O
+ * outputs (row-major, O = 144 for the 3 x 3 blocks of the 16 neighbours), evaluated in FP32 (bf16 == 0) or with BF16
+ * weights and activations on MFMA (bf16 != 0).  No counterpart in the reference. */
+int cdj_mlp_forward(cdj_ctx* ctx, const cdj_mlp* mlp, const double* P1, const double* P2, const double* P3, int64_t n,
+                    int32_t bf16, float* Y);
 
 /* Frictional slip: replaces SlipCalculator3D(P1,P2,P3,nu,G,lambda,MidPoint,FaceNormalVector,HSFlag,
  * MixedBoundaryConditionsFriction,FixedEls) (SlipCalculator3D.jl:36-233) for FixedEls all zero.  b: 3n tractions

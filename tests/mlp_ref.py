@@ -36,6 +36,22 @@ def forward(params, feat, k, H):
     return h1, h2, h2 @ W3 + b3
 
 
+def to_bf16(x):
+    """Round to BF16 (round to nearest even through FP32), returned as float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(params, feat, k, H):
+    """The network with each GEMM's operands (features or activations, and weights) rounded to BF16; products,
+    sums, biases and tanh in FP64 (the GPU accumulates in FP32)."""
+    W1, b1, W2, b2, W3, b3 = unpack(params, k, H)
+    h1 = np.tanh(to_bf16(feat) @ to_bf16(W1) + b1)
+    h2 = np.tanh(to_bf16(h1) @ to_bf16(W2) + b2)
+    return h1, h2, to_bf16(h2) @ to_bf16(W3) + b3
+
+
 def apply_M(Y, jac, nbr, hscale, r, n, k):
     """z = M r for one vector r (length 3n)."""
     R = r.reshape(3, n)

@@ -118,3 +118,38 @@ def test_training_improves_gmres_on_held_out_mesh():
         assert res <= 1e-10, (pc, res)
         its[pc] = rep["iterations"]
     assert its["learned"] < its["jacobi"] <= its["none"], its
+
+
+def test_bf16_network_matches_rounded_reference_and_preconditions():
+    """The BF16 inference path (CDJ_PC_LEARNED_BF16): outputs equal the FP64 restatement with the same BF16
+    roundings (tests/mlp_ref.forward_bf16) up to FP32 accumulation, stay within 5 % of the FP32 network, and the
+    preconditioner reaches the 1e-10 FP64 residual in about as many iterations as the FP32 one."""
+    from cutanddisplacejulia_amd import spai
+    model = spai.load_model()
+    seq = meshes.growth_sequence(40, 500, 6400)
+    step = seq[33]
+    P1, P2, P3 = step["P1"], step["P2"], step["P3"]
+    n = P1.shape[0]
+    y32 = model.forward(P1, P2, P3, bf16=False)
+    y16 = model.forward(P1, P2, P3, bf16=True)
+    rng = np.random.default_rng(0)
+    _, _, feat, nbr, geo = model.loss_grad(P1, P2, P3, 1.0, rng.normal(size=(n, 9)), rng.normal(size=(3 * n, 1)),
+                                           rng.normal(size=(3 * n, 1)))
+    _, _, yref = mlp_ref.forward_bf16(model.get_params(), feat.astype(np.float64), model.k, model.hidden)
+    scale = np.max(np.abs(yref))
+    d = np.abs(y16 - yref)
+    print("[gpu-test] bf16 network: max |gpu - ref| %.2e, mean %.2e, vs FP32 %.2e (scale %.2e)" % (
+        d.max(), d.mean(), np.abs(y16 - y32).max(), scale))
+    assert d.max() <= 5e-3 * scale and d.mean() <= 1e-4 * scale
+    assert np.abs(y16 - y32).max() <= 5e-2 * scale
+    d, A, nrm, mid, lam = spai.assemble(step)
+    bb = torch.from_numpy(spai.rhs(step, nrm, mid, lam)).to("cuda:0")
+    res = {}
+    for pc in ("learned", "learned_bf16"):
+        x = torch.zeros_like(bb)
+        rep = learned.precond_solve_dev(A, 3 * n, bb, x, d["P1"], d["P2"], d["P3"], n, spai.G, pc=pc, model=model)
+        r = float(torch.linalg.norm(bb - torch.mv(A.t(), x)) / torch.linalg.norm(bb))
+        res[pc] = (rep["iterations"], r)
+    print("[gpu-test] iterations: FP32 network %d, BF16 network %d" % (res["learned"][0], res["learned_bf16"][0]))
+    assert res["learned_bf16"][1] <= 1e-10 and res["learned"][1] <= 1e-10
+    assert res["learned_bf16"][0] <= res["learned"][0] + 5
