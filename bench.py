@@ -95,7 +95,8 @@ def slip_solve_comparison(step_no):
     plain = r["fp64"]["total_ms"]
     best = min((k for k in r if k.startswith("mixed_") and isinstance(r[k], dict) and r[k]["residual"] <= 1e-10),
                key=lambda k: r[k]["total_ms"], default=None)
-    ml = min((k for k in ("mixed_learned", "mixed_learned_k32") if k in r and r[k]["residual"] <= 1e-10),
+    ml = min((k for k in ("mixed_learned", "mixed_learned_k32", "mixed_learned_bf16")
+              if k in r and r[k]["residual"] <= 1e-10),
              key=lambda k: r[k]["total_ms"], default=None)
     _log("slip solve step %d (%d triangles): FP64 assembly+LU %.1f ms; %s %.1f ms (%d it); mixed+LU32 %.1f ms" % (
         step_no, r["n_tri"], plain, ml, r[ml]["total_ms"] if ml else float("nan"),

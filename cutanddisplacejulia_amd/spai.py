@@ -85,7 +85,7 @@ def _timed(fn):
     return out, 1e3 * (time.perf_counter() - t0)
 
 
-PCS = ("none", "jacobi", "learned", "learned_k32", "lu32", "lu16")
+PCS = ("none", "jacobi", "learned", "learned_k32", "learned_bf16", "lu32", "lu16")
 
 
 def evaluate_step(step, model, hs=0, pcs=PCS, tol=1e-10):
@@ -130,7 +130,7 @@ def evaluate_step(step, model, hs=0, pcs=PCS, tol=1e-10):
             out["mixed_matrix_diff"] = None
         for name in pcs:
             pc, k32 = (name[:-4], True) if name.endswith("_k32") else (name, False)
-            if pc == "learned" and model is None:
+            if pc.startswith("learned") and model is None:
                 continue
             x = torch.zeros_like(b)
             rep, t = _timed(lambda: learned.precond_solve_dev(A0, N3, b, x, d["P1"], d["P2"], d["P3"], n, G, pc=pc,
