@@ -1,10 +1,9 @@
 // Blocked right-looking LU with partial pivoting (the dgetrf behind `A\b`, SlipCalculator3D.jl:22) and
 // blocked triangular solves, for gfx950.
 //
-//   panel   : one cooperative launch per 64-column panel.  The panel rows are split over the workgroups and held
-//             in LDS; each column needs one grid barrier: every workgroup publishes its local pivot candidate
-//             (value, row, row contents) and the owner of the diagonal row publishes that row, so after the
-//             barrier all workgroups know the pivot, swap locally and apply the rank-1 update in LDS.
+//   panel   : one cooperative launch per 64-column panel.  The panel rows are split over the workgroups, one row
+//             per thread, held in registers; each column needs one grid-wide exchange of pivot candidates (none
+//             when the panel fits one workgroup), after which every thread applies the rank-1 update to its row.
 //   laswp   : the panel's row interchanges applied to the columns outside the panel.
 //   trsm    : U12 = L11^-1 A12 with L11 staged in LDS.
 //   update  : A22 -= L21 U12 on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32), 128x128 tiles, K
@@ -28,7 +27,7 @@ namespace {
 
 constexpr int PT = 256;        // panel threads
 constexpr int MAX_P = 1024;    // max panel workgroups
-constexpr int MAX_RPB = PT;    // panel rows per workgroup, one per thread (LDS: 256 x 64 x 8 = 128 KiB)
+constexpr int MAX_RPB = PT;    // panel rows per workgroup, one per thread (LDS multipliers: 256 x 64 x 8 = 128 KiB)
 constexpr int LU_OUTER = 256;  // columns per outer block (rank of the trailing update)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -49,209 +48,329 @@ template <> struct Mfma<float> {
   __device__ static int row(int g, int v) { return 4 * g + v; }
 };
 
-// Per-column exchange of the panel kernel (all workgroups co-resident: cooperative launch).  Each workgroup
-// publishes its pivot candidate for global column c as one 64-bit word in its own slot: the bits of |a| (a
-// non-negative double, so its bit pattern orders like its value) with the low 12 bits replaced by c mod 4096, or
-// the top bit set when it has no candidate.  The word is stored with agent-scope release after the workgroup
-// barrier, so the candidate's row, exact value and row contents (plain stores before it) are published with it.
-// Wave 0 of every workgroup then polls all slots until each carries tag c, takes the agent-scope acquire and reads
-// the winner's data: the poll is the grid barrier and the candidate exchange in one round trip.  A slot is
-// rewritten every second column by the same workgroup, so the previous word's tag (c - 2) never matches; the
-// slots start all ones (tag 4095) before the first column.  The wait is bounded (about a second: co-resident
-// workgroups publish within microseconds): on overrun the error flag is set and the kernel carries on, so it
-// always exits (the host then reports the failure).
+// Panel factorization: one cooperative launch per panel of nbw <= 64 columns; the panel's m rows are split over P
+// co-resident workgroups, one row per thread.
+//
+// Rows never move.  Each thread keeps the not-yet-eliminated part of its row in registers, shifted so that x[0] is
+// always the current column j (x[i] is column j + i), and tracks the row's current position `pos`.  The row
+// interchange of step j (position j <-> pivot position) is a relabelling: the pivot row takes position j, the row
+// that held position j takes the pivot's old position.  Multipliers go to LDS (Lsh, by thread), pivot rows to a
+// packed LDS copy of U (Ush), and every thread writes its finished row to its final position at the end.  The
+// arithmetic, the pivot rule (largest |a|, ties to the lowest position, as idamax) and hence the result are those
+// of the row-moving algorithm, bit for bit.
+//
+// Per column: each wave finds its candidate with DPP maxima of the 64-bit key bits(|a|) (a non-negative double's
+// bit pattern orders like its value) and stages the candidate row in LDS; after one workgroup barrier every thread
+// knows the workgroup's candidate.  With one workgroup that is the pivot.  With several, wave 0 publishes the
+// candidate and polls the others (below), then a second barrier hands the pivot row to the workgroup.
+//
+// Grid exchange (P > 1).  Each workgroup owns a slot word per column parity: the key with its low 12 bits replaced
+// by c mod 4096 (c = global column), or the top bit set when it has no candidate, and a record of tagged words
+// (32-bit payload, 32-bit tag c): the key, the candidate's position and its row.  All are relaxed agent-scope
+// atomics: a reader takes a word only when its tag is c, so no fences are needed and a record read can never see
+// another column's data.  Wave 0 polls the slots until all carry tag c (this is the grid barrier), picks the
+// largest key (equal truncated keys are settled from the records' exact keys, then the lowest position) and reads
+// the winner's record in one batch.  A slot or record is rewritten two columns later by the same workgroup, which
+// can only happen after every workgroup has finished reading it; slots and records start all ones before a
+// factorization.  Waits are bounded (about a second: co-resident workgroups publish within microseconds); on
+// overrun the error flag is set and the kernel carries on, so it always exits (the host then reports the failure).
 constexpr uint64_t SLOT_TAG = 0xFFFull;
 constexpr uint64_t SLOT_NONE = 1ull << 63;
+constexpr int XREC = 3 + 2 * LU_NB;              // tagged words per record: key (2), position (1), row (2 per entry)
+constexpr int USH = LU_NB * (LU_NB + 1) / 2;    // packed U rows of one panel
+__device__ __forceinline__ int urow(int j) { return j * LU_NB - (j * (j - 1)) / 2; }  // offset of U row j
 
-template <class T>
+template <int CTRL>
+__device__ __forceinline__ long long dpp64(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned long long)v, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)((unsigned long long)v >> 32), CTRL, 0xF, 0xF, false);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ long long readlane64(long long v, int l) {
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)v, l);
+  const int hi = __builtin_amdgcn_readlane((int)((unsigned long long)v >> 32), l);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ long long kmax64(long long a, long long b) { return a > b ? a : b; }
+
+// wave-wide maximum, in every lane: DPP within rows of 16 lanes, then the four row maxima
+__device__ __forceinline__ long long wave_max(long long k) {
+  k = kmax64(k, dpp64<0xB1>(k));   // quad_perm [1,0,3,2]
+  k = kmax64(k, dpp64<0x4E>(k));   // quad_perm [2,3,0,1]
+  k = kmax64(k, dpp64<0x141>(k));  // row_half_mirror
+  k = kmax64(k, dpp64<0x140>(k));  // row_mirror
+  return kmax64(kmax64(readlane64(k, 0), readlane64(k, 16)), kmax64(readlane64(k, 32), readlane64(k, 48)));
+}
+
+__device__ __forceinline__ void put_tagged(unsigned long long* p, uint32_t tag, uint32_t v) {
+  __hip_atomic_store(p, ((unsigned long long)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double join_words(unsigned long long lo, unsigned long long hi) {
+  return __longlong_as_double((long long)((hi << 32) | (lo & 0xFFFFFFFFull)));
+}
+
+// QPL: slots polled per lane (P <= 64 QPL)
+template <class T, int QPL>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
-                                                      int32_t* ipiv, int* info, double* cval, int* crow, T* crows,
-                                                      T* rowj, unsigned long long* slots, int* bar_err, int* perm) {
+                                                      int32_t* ipiv, int* info, unsigned long long* xw,
+                                                      unsigned long long* slots, int* bar_err, int* perm) {
   extern __shared__ unsigned char smem_raw[];
-  T* sh = reinterpret_cast<T*>(smem_raw);  // column-major panel rows of this workgroup: sh[c * rpb + r]
-  __shared__ T u[LU_NB];                   // pivot row of the current column
-  __shared__ T rj[LU_NB];                  // row j, staged for the workgroup that holds the pivot row
-  __shared__ double wv[PT / 64];
-  __shared__ int wr[PT / 64];
-  __shared__ int s_prow;
-  __shared__ int s_piv[LU_NB];  // block 0: this panel's pivot rows (relative to k)
+  T* Lsh = reinterpret_cast<T*>(smem_raw);  // multipliers of this workgroup's rows: Lsh[c * rpb + tid]
+  T* Ush = Lsh + (size_t)rpb * LU_NB;        // U rows of the panel, packed (row j at urow(j), columns j..)
+  __shared__ T stage[2][PT / 64][LU_NB];     // each wave's candidate row (active part), by column parity
+  __shared__ long long wk[2][PT / 64];       // each wave's candidate key (-1: none) and position
+  __shared__ int wp[2][PT / 64];
+  __shared__ int s_prow[2];
+  __shared__ int s_piv[LU_NB];  // pivot positions of this panel (relative to k)
+  __shared__ int s_info;        // first column with a zero (or NaN) pivot, as a 1-based global index; 0 if none
   const int P = gridDim.x, b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = (int64_t)b * rpb;
   const int64_t r1 = r0 + rpb < m ? r0 + rpb : m;
   const int nr = r1 > r0 ? (int)(r1 - r0) : 0;
-  // thread tid owns panel row r0 + tid (rpb <= PT)
   const bool mine = tid < nr;
-  const int64_t grow = r0 + tid;
-  if (mine)
-    for (int c = 0; c < nbw; ++c) sh[c * rpb + tid] = A[(k + grow) + (k + c) * lda];
-  __syncthreads();
+  int pos = (int)(r0 + tid);  // current position of this thread's row
+  bool done = !mine;          // row already taken as a pivot (or no row)
+  bool pivot = false;         // this row is U row `pos`
+  if (tid == 0) s_info = 0;
+  T x[LU_NB];
+  {
+    // unconditional loads from clamped (in-range) addresses; the clamped values are never used
+    const int64_t row = k + (mine ? pos : (m > 0 ? m - 1 : 0));
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c) x[c] = A[row + (k + (c < nbw ? c : nbw - 1)) * lda];
+  }
   for (int j = 0; j < nbw; ++j) {
-    const int par = j & 1;  // k is a multiple of LU_NB, so this is also the global column's parity
+    const int par = j & 1;
     const uint64_t tag = (uint64_t)(k + j) & SLOT_TAG;
-    // local pivot candidate: largest |a|, ties to the lower row (the first maximum, as idamax)
-    double v = -1.0;
-    int r = -1;
-    if (mine && grow >= j) {
-      v = fabs((double)sh[j * rpb + tid]);
-      r = (int)grow;
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double v2 = __shfl_xor(v, off);
-      const int r2 = __shfl_xor(r, off);
-      if (r2 >= 0 && (r < 0 || v2 > v || (v2 == v && r2 < r))) {
-        v = v2;
-        r = r2;
+    const uint32_t tg = (uint32_t)(k + j);
+    const int na = nbw - j;  // active columns
+    // wave candidate: largest key, ties to the lowest position
+    const long long key = done ? -1ll : (long long)__double_as_longlong(fabs((double)x[0]));
+    const long long kw = wave_max(key);
+    if (kw >= 0) {
+      unsigned long long msk = __ballot(key == kw);
+      int wl = __ffsll((long long)msk) - 1;
+      if (__popcll(msk) > 1) {
+        int bp = __builtin_amdgcn_readlane(pos, wl);
+        for (msk &= msk - 1; msk; msk &= msk - 1) {
+          const int l2 = __ffsll((long long)msk) - 1;
+          const int p2 = __builtin_amdgcn_readlane(pos, l2);
+          if (p2 < bp) {
+            bp = p2;
+            wl = l2;
+          }
+        }
       }
-    }
-    if (lane == 0) {
-      wv[wave] = v;
-      wr[wave] = r;
+      if (lane == wl) {
+#pragma unroll
+        for (int c8 = 0; c8 < LU_NB; c8 += 8)
+          if (c8 < na) {
+#pragma unroll
+            for (int i = c8; i < c8 + 8; ++i) stage[par][wave][i] = x[i];
+          }
+        wk[par][wave] = kw;
+        wp[par][wave] = pos;
+      }
+    } else if (lane == 0) {
+      wk[par][wave] = -1;
+      wp[par][wave] = 0;
     }
     __syncthreads();
-    double bv = wv[0];
-    int br = wr[0];
-    for (int q = 1; q < PT / 64; ++q) {
-      const double v2 = wv[q];
-      const int r2 = wr[q];
-      if (r2 >= 0 && (br < 0 || v2 > bv || (v2 == bv && r2 < br))) {
-        bv = v2;
-        br = r2;
+    // workgroup candidate (uniform)
+    long long bk = wk[par][0];
+    int bpos = wp[par][0], bw = 0;
+#pragma unroll
+    for (int w = 1; w < PT / 64; ++w) {
+      const long long k2 = wk[par][w];
+      const int p2 = wp[par][w];
+      if (k2 > bk || (k2 == bk && k2 >= 0 && p2 < bpos)) {
+        bk = k2;
+        bpos = p2;
+        bw = w;
       }
     }
-    // publish the candidate's exact value, row and contents and, from the holder of row j, row j itself
-    if (tid == 0) {
-      cval[par * MAX_P + b] = bv;
-      crow[par * MAX_P + b] = br;
-    }
-    if (br >= 0 && tid < nbw) crows[(int64_t)(par * MAX_P + b) * LU_NB + tid] = sh[tid * rpb + (br - r0)];
-    if (j >= r0 && j < r1 && tid < nbw) rowj[par * LU_NB + tid] = sh[tid * rpb + (j - r0)];
-    __syncthreads();
-    if (tid == 0) {
-      const uint64_t word = br >= 0 ? (((uint64_t)__double_as_longlong(bv) & ~SLOT_TAG) | tag) : (SLOT_NONE | tag);
-      __hip_atomic_store(&slots[par * MAX_P + b], (unsigned long long)word, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    // global pivot: wave 0 waits for every slot, picks the winner and stages the pivot row (and row j)
-    if (wave == 0) {
-      constexpr int QPL = MAX_P / 64;  // slots per lane
-      uint64_t wq[QPL];
-      long long it = 0;
-      for (;;) {
-        bool seen = true;
+    const T* uu;  // pivot row, active part
+    int prow;
+    if (P == 1) {
+      uu = stage[par][bw];
+      prow = bk >= 0 ? bpos : j;
+      if (wave == 0 && lane < na) Ush[urow(j) + lane] = uu[lane];
+      if (tid == 0) {
+        s_piv[j] = prow;
+        const double gv = bk >= 0 ? __longlong_as_double(bk) : -1.0;
+        if (!(gv > 0.0) && s_info == 0) s_info = (int)(k + j + 1);
+      }
+    } else {
+      if (wave == 0) {
+        unsigned long long* rec = xw + (size_t)(par * MAX_P + b) * XREC;
+        if (bk >= 0 && lane < na) {
+          const unsigned long long xv = (unsigned long long)__double_as_longlong((double)stage[par][bw][lane]);
+          put_tagged(rec + 3 + 2 * lane, tg, (uint32_t)xv);
+          put_tagged(rec + 4 + 2 * lane, tg, (uint32_t)(xv >> 32));
+        }
+        if (lane == 0) {
+          put_tagged(rec, tg, (uint32_t)(unsigned long long)bk);
+          put_tagged(rec + 1, tg, (uint32_t)((unsigned long long)bk >> 32));
+          put_tagged(rec + 2, tg, (uint32_t)bpos);
+          const uint64_t word = bk >= 0 ? (((uint64_t)bk & ~SLOT_TAG) | tag) : (SLOT_NONE | tag);
+          __hip_atomic_store(&slots[par * MAX_P + b], (unsigned long long)word, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint64_t wq[QPL];
+        long long it = 0;
+        for (;;) {
+          bool seen = true;
+#pragma unroll
+          for (int i = 0; i < QPL; ++i) {
+            const int q = lane + 64 * i;
+            wq[i] = q < P ? ld_agent(&slots[par * MAX_P + q]) : (SLOT_NONE | tag);
+            seen = seen && (wq[i] & SLOT_TAG) == tag;
+          }
+          if (__all(seen)) break;
+          if (++it > (1ll << 20)) {
+            if (lane == 0) atomicOr(bar_err, 1);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        uint64_t kmax = 0;
+        int qmax = -1;
 #pragma unroll
         for (int i = 0; i < QPL; ++i) {
           const int q = lane + 64 * i;
-          wq[i] = q < P ? __hip_atomic_load(&slots[par * MAX_P + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                        : (SLOT_NONE | tag);
-          seen = seen && (wq[i] & SLOT_TAG) == tag;
-        }
-        if (__all(seen)) break;
-        if (++it > (1ll << 20)) {
-          if (lane == 0) atomicOr(bar_err, 1);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      // largest truncated |a| over all slots, ties to the lower workgroup (= lower rows)
-      uint64_t kmax = 0;
-      int qmax = -1;
-#pragma unroll
-      for (int i = 0; i < QPL; ++i) {
-        const int q = lane + 64 * i;
-        if (q < P && !(wq[i] & SLOT_NONE)) {
-          const uint64_t key = wq[i] & ~SLOT_TAG;
-          if (qmax < 0 || key > kmax) {
-            kmax = key;
-            qmax = q;
-          }
-        }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const uint64_t k2 = __shfl_xor(kmax, off);
-        const int q2 = __shfl_xor(qmax, off);
-        if (q2 >= 0 && (qmax < 0 || k2 > kmax || (k2 == kmax && q2 < qmax))) {
-          kmax = k2;
-          qmax = q2;
-        }
-      }
-      // the winner's row contents (and row j, for the workgroup holding the pivot row) load together with its
-      // exact value and row
-      int gw = qmax;
-      if (lane < nbw) {
-        u[lane] = gw >= 0 ? crows[(int64_t)(par * MAX_P + gw) * LU_NB + lane] : T(0);
-        if (gw == b) rj[lane] = rowj[par * LU_NB + lane];
-      }
-      double gv = gw >= 0 ? cval[par * MAX_P + gw] : -1.0;
-      int gr = gw >= 0 ? crow[par * MAX_P + gw] : -1;
-      // several slots with the same truncated key (equal to 40 bits): compare their exact values, largest |a|
-      // and then the lower row (idamax's first maximum), and restage the pivot row if the winner changes
-      int ntie = 0;
-#pragma unroll
-      for (int i = 0; i < QPL; ++i) {
-        const int q = lane + 64 * i;
-        ntie += __popcll(__ballot(q < P && !(wq[i] & SLOT_NONE) && (wq[i] & ~SLOT_TAG) == kmax));
-      }
-      if (gw >= 0 && ntie > 1) {
-#pragma unroll
-        for (int i = 0; i < QPL; ++i) {
-          unsigned long long msk = __ballot(lane + 64 * i < P && !(wq[i] & SLOT_NONE) && (wq[i] & ~SLOT_TAG) == kmax);
-          while (msk) {
-            const int q = 64 * i + __ffsll((long long)msk) - 1;
-            msk &= msk - 1;
-            const double vq = cval[par * MAX_P + q];
-            const int rq = crow[par * MAX_P + q];
-            if (vq > gv || (vq == gv && rq < gr)) {
-              gv = vq;
-              gr = rq;
-              gw = q;
+          if (q < P && !(wq[i] & SLOT_NONE)) {
+            const uint64_t kk = wq[i] & ~SLOT_TAG;
+            if (qmax < 0 || kk > kmax) {
+              kmax = kk;
+              qmax = q;
             }
           }
         }
-        if (gw != qmax && lane < nbw) {
-          u[lane] = crows[(int64_t)(par * MAX_P + gw) * LU_NB + lane];
-          if (gw == b) rj[lane] = rowj[par * LU_NB + lane];
+        for (int off = 32; off > 0; off >>= 1) {
+          const uint64_t k2 = __shfl_xor(kmax, off);
+          const int q2 = __shfl_xor(qmax, off);
+          if (q2 >= 0 && (qmax < 0 || k2 > kmax || (k2 == kmax && q2 < qmax))) {
+            kmax = k2;
+            qmax = q2;
+          }
         }
-      }
-      if (lane == 0) {
-        s_prow = gr >= 0 ? gr : j;
-        if (b == 0) {
+        int gw = qmax;
+        double gv = -1.0;
+        int gr = -1;
+        int ntie = 0;
+#pragma unroll
+        for (int i = 0; i < QPL; ++i) {
+          const int q = lane + 64 * i;
+          ntie += __popcll(__ballot(q < P && !(wq[i] & SLOT_NONE) && (wq[i] & ~SLOT_TAG) == kmax));
+        }
+        if (gw >= 0 && ntie > 1) {  // equal truncated keys: exact keys, then the lowest position
+          gw = -1;
+#pragma unroll
+          for (int i = 0; i < QPL; ++i) {
+            unsigned long long msk = __ballot(lane + 64 * i < P && !(wq[i] & SLOT_NONE) && (wq[i] & ~SLOT_TAG) == kmax);
+            while (msk) {
+              const int q = 64 * i + __ffsll((long long)msk) - 1;
+              msk &= msk - 1;
+              const unsigned long long* rw = xw + (size_t)(par * MAX_P + q) * XREC;
+              unsigned long long w0, w1, w2;
+              for (long long t = 0;; ++t) {
+                w0 = ld_agent(rw);
+                w1 = ld_agent(rw + 1);
+                w2 = ld_agent(rw + 2);
+                if ((uint32_t)(w0 >> 32) == tg && (uint32_t)(w1 >> 32) == tg && (uint32_t)(w2 >> 32) == tg) break;
+                if (t > (1ll << 20)) {
+                  if (lane == 0) atomicOr(bar_err, 2);
+                  break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+              }
+              const double vq = join_words(w0, w1);
+              const int rq = (int)(uint32_t)w2;
+              if (gw < 0 || vq > gv || (vq == gv && rq < gr)) {
+                gv = vq;
+                gr = rq;
+                gw = q;
+              }
+            }
+          }
+        }
+        // the winner's record in one batch: lane i its row entry i, lane 63 also key and position
+        if (gw >= 0) {
+          const unsigned long long* rw = xw + (size_t)(par * MAX_P + gw) * XREC;
+          const bool need = lane < na;
+          for (long long t = 0;; ++t) {
+            bool ok = true;
+            unsigned long long a0 = 0, a1 = 0;
+            if (need) {
+              a0 = ld_agent(rw + 3 + 2 * lane);
+              a1 = ld_agent(rw + 4 + 2 * lane);
+              ok = (uint32_t)(a0 >> 32) == tg && (uint32_t)(a1 >> 32) == tg;
+            }
+            unsigned long long w0 = 0, w1 = 0, w2 = 0;
+            if (lane == 63) {
+              w0 = ld_agent(rw);
+              w1 = ld_agent(rw + 1);
+              w2 = ld_agent(rw + 2);
+              ok = ok && (uint32_t)(w0 >> 32) == tg && (uint32_t)(w1 >> 32) == tg && (uint32_t)(w2 >> 32) == tg;
+            }
+            if (__all(ok)) {
+              if (need) Ush[urow(j) + lane] = (T)join_words(a0, a1);
+              if (lane == 63) {
+                gv = join_words(w0, w1);
+                gr = (int)(uint32_t)w2;
+              }
+              break;
+            }
+            if (t > (1ll << 20)) {
+              if (lane == 0) atomicOr(bar_err, 2);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          gv = __shfl(gv, 63);
+          gr = __shfl(gr, 63);
+        }
+        if (lane == 0) {
+          s_prow[par] = gr >= 0 ? gr : j;
           s_piv[j] = gr >= 0 ? gr : j;
-          ipiv[k + j] = (int32_t)(k + (gr >= 0 ? gr : j));
-          if (!(gv > 0.0) && *info == 0) *info = (int)(k + j + 1);
+          if (!(gv > 0.0) && s_info == 0) s_info = (int)(k + j + 1);
         }
       }
+      __syncthreads();
+      uu = Ush + urow(j);
+      prow = s_prow[par];
     }
-    __syncthreads();
-    const int prow = s_prow;
-    // row interchange, one column per thread: the pivot row takes row j's contents, row j the pivot row's
-    if (prow != j && tid < nbw) {
-      if (prow >= r0 && prow < r1) sh[tid * rpb + (prow - r0)] = rj[tid];
-      if (j >= r0 && j < r1) sh[tid * rpb + (j - r0)] = u[tid];
-    }
-    __syncthreads();
-    // multiplier and rank-1 update of this thread's row, eight columns at a time so the LDS reads overlap
-    const T piv = u[j];
-    if (mine && grow > j && piv != T(0)) {
-      const T l = sh[j * rpb + tid] / piv;
-      sh[j * rpb + tid] = l;
-      for (int c0 = j + 1; c0 < nbw; c0 += 8) {
-        T x[8], y[8];
+    // the pivot row becomes U row j; every other active row takes its multiplier and the rank-1 update
+    if (!done) {
+      if (pos == prow) {
+        pos = j;
+        done = true;
+        pivot = true;
+      } else {
+        if (pos == j) pos = prow;
+        const T piv = uu[0];
+        T l = x[0];
+        if (piv != T(0)) {
+          l = x[0] / piv;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const int c = c0 + t < nbw ? c0 + t : nbw - 1;
-          x[t] = sh[c * rpb + tid];
-          y[t] = u[c];
+          for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1] - l * uu[i + 1];
+        } else {
+#pragma unroll
+          for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1];
         }
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-          if (c0 + t < nbw) sh[(c0 + t) * rpb + tid] = x[t] - l * y[t];
+        Lsh[j * rpb + tid] = l;
       }
     }
   }
   __syncthreads();
   if (mine)
-    for (int c = 0; c < nbw; ++c) A[(k + grow) + (k + c) * lda] = sh[c * rpb + tid];
+    for (int c = 0; c < nbw; ++c)
+      A[(k + pos) + (k + c) * lda] = (pivot && c >= pos) ? Ush[urow(pos) + (c - pos)] : Lsh[c * rpb + tid];
+  if (b == 0 && tid < nbw) ipiv[k + tid] = (int32_t)(k + s_piv[tid]);
+  if (b == 0 && tid == 0 && s_info != 0 && *info == 0) *info = s_info;
   // block 0, wave 0: the panel's swaps as one row permutation of the touched rows (rows j < nbw and the pivot
   // rows beyond the panel): perm[0] = count, then (destination, source) pairs, relative to k
   if (b == 0 && wave == 0) {
@@ -270,8 +389,8 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
       int at = p < nbw ? p : -1;
       if (at < 0) {
         const bool hit = lane + nbw < cnt && touched[lane + nbw] == p;
-        const unsigned long long m = __ballot(hit);
-        if (m) at = nbw + __ffsll((long long)m) - 1;
+        const unsigned long long mm = __ballot(hit);
+        if (mm) at = nbw + __ffsll((long long)mm) - 1;
       }
       if (at < 0) {
         at = cnt;
@@ -736,11 +855,10 @@ void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t n
 
 size_t lu_workspace_bytes(int64_t n) {
   const int64_t nn = n > 0 ? n : 1;
-  // slots, candidates, pivot rows, flags and the panel permutation; then n column scales and the two packed FP16
+  // the panels' slots and records, flags and the panel permutation; then n column scales and the two packed FP16
   // operand copies of the FP16 trailing update (n x LU_OUTER halves each)
-  return sizeof(unsigned long long) * 2 * MAX_P + sizeof(double) * (2 * MAX_P + 2 * (size_t)MAX_P * LU_NB + 2 * LU_NB) +
-         sizeof(int) * 2 * MAX_P + sizeof(int) * (2 + 1 + 4 * LU_NB) + f16_pack_offset(nn) +
-         2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
+  return sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) + sizeof(int) * (2 + 1 + 4 * LU_NB) +
+         f16_pack_offset(nn) + 2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
 }
 
 // Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
@@ -781,16 +899,10 @@ template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
                           hipStream_t st, bool fp16_update) {
   char* w = static_cast<char*>(ws);
-  unsigned long long* slots = reinterpret_cast<unsigned long long*>(w);  // the panels' candidate words
-  w += sizeof(unsigned long long) * 2 * MAX_P;
-  double* cval = reinterpret_cast<double*>(w);
-  w += sizeof(double) * 2 * MAX_P;
-  T* crows = reinterpret_cast<T*>(w);
-  w += sizeof(double) * 2 * (size_t)MAX_P * LU_NB;
-  T* rowj = reinterpret_cast<T*>(w);
-  w += sizeof(double) * 2 * LU_NB;
-  int* crow = reinterpret_cast<int*>(w);
-  w += sizeof(int) * 2 * MAX_P;
+  unsigned long long* slots = reinterpret_cast<unsigned long long*>(w);  // the panels' slot words, then records
+  unsigned long long* xw = slots + 2 * MAX_P;
+  const size_t xbytes = sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC);
+  w += xbytes;
   unsigned* bar = reinterpret_cast<unsigned*>(w);  // unused word, then the panels' wait-overrun flag
   int* bar_err = reinterpret_cast<int*>(bar + 1);
   int* perm = bar_err + 1;                          // 1 + 4 LU_NB ints: the last panel's row permutation
@@ -800,12 +912,16 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   if (e != hipSuccess) return e;
   e = hipMemsetAsync(bar, 0, 2 * sizeof(unsigned), st);
   if (e != hipSuccess) return e;
-  e = hipMemsetAsync(slots, 0xFF, sizeof(unsigned long long) * 2 * MAX_P, st);  // tag 4095: matches no column 0, 1
+  // all ones: slot tag 4095 and record tag 2^32 - 1 match no column of this factorization
+  e = hipMemsetAsync(slots, 0xFF, xbytes, st);
   if (e != hipSuccess) return e;
-  static bool lds_set = false;  // raise the dynamic LDS limit once per process (128 KiB panels)
+  static bool lds_set = false;  // raise the dynamic LDS limit once per process (up to 144 KiB per panel block)
   if (!lds_set) {
-    e = hipFuncSetAttribute((const void*)lu_panel_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)(sizeof(T) * MAX_RPB * LU_NB));
+    const int lds = (int)(sizeof(T) * ((size_t)MAX_RPB * LU_NB + USH));
+    e = hipFuncSetAttribute((const void*)lu_panel_kernel<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)lu_panel_kernel<T, MAX_P / 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds);
     if (e != hipSuccess) return e;
     lds_set = true;
   }
@@ -822,10 +938,10 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       const int P = (int)((m + MAX_RPB - 1) / MAX_RPB);
       int rpb = (int)((m + P - 1) / P);
       if (P > MAX_P) return hipErrorInvalidValue;
-      size_t shm = sizeof(T) * (size_t)rpb * LU_NB;
-      void* args[] = {&A,    (void*)&lda, &k,    (void*)&m, (void*)&nbw, &rpb,    &ipiv, &info,
-                      &cval, &crow,       &crows, &rowj,    &slots,      &bar_err, &perm};
-      e = hipLaunchCooperativeKernel((const void*)lu_panel_kernel<T>, dim3(P), dim3(PT), args, shm, st);
+      size_t shm = sizeof(T) * ((size_t)rpb * LU_NB + USH);
+      void* args[] = {&A, (void*)&lda, &k, (void*)&m, (void*)&nbw, &rpb, &ipiv, &info, &xw, &slots, &bar_err, &perm};
+      const void* kern = P <= 128 ? (const void*)lu_panel_kernel<T, 2> : (const void*)lu_panel_kernel<T, MAX_P / 64>;
+      e = hipLaunchCooperativeKernel(kern, dim3(P), dim3(PT), args, shm, st);
       if (e != hipSuccess) return e;
       if (ncols > nbw)
         hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 2)), dim3(256), 0, st, A, lda, ncols, k, nbw,
