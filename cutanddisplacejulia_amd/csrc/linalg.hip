@@ -437,12 +437,12 @@ __global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t n
 }
 
 // A[k:k+nbw, c0:c0+ncols] = L11^-1 A[k:k+nbw, c0:...] with L11 unit lower (the panel's diagonal block).  One
-// wave per column at a time (TRSM_CPW columns per wave): lane r holds x[r]; step q broadcasts x[q] and the lanes
-// below update with L11 read from LDS (stored transposed, so a step's reads are consecutive).
-constexpr int TRSM_CPW = 8;
-
+// wave per column at a time (cpw columns per wave): lane r holds x[r]; step q broadcasts x[q] and the lanes
+// below update with L11 read from LDS (stored transposed, so a step's reads are consecutive).  A column's solve is
+// a latency chain of nbw steps, so few columns get one column per wave (more workgroups, each staging L11).
 template <class T>
-__global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t ncols) {
+__global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t ncols,
+                                                   int cpw) {
   __shared__ T LT[LU_NB][LU_NB + 1];  // LT[q][r] = L11[r][q]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
@@ -450,8 +450,8 @@ __global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k,
     LT[q][r] = (r < nbw && q < nbw) ? A[(k + r) + (k + q) * lda] : T(0);
   }
   __syncthreads();
-  const int64_t cb = c0 + ((int64_t)blockIdx.x * 4 + wave) * TRSM_CPW;
-  for (int i = 0; i < TRSM_CPW; ++i) {
+  const int64_t cb = c0 + ((int64_t)blockIdx.x * 4 + wave) * cpw;
+  for (int i = 0; i < cpw; ++i) {
     const int64_t c = cb + i;
     if (c >= c0 + ncols) break;
     T* col = A + k + c * lda;
@@ -843,9 +843,10 @@ void gemm_sub(const T* L, int64_t ldl, const T* U, int64_t ldu, T* C, int64_t ld
 template <class T>
 void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t nc, hipStream_t st) {
   if (nc <= 0) return;
+  const int cpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, nc / 8192));
   for (int64_t s = r0; s < r1; s += LU_NB) {
     const int nb = (int)std::min<int64_t>(LU_NB, r1 - s);
-    hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nc, 4 * TRSM_CPW)), dim3(256), 0, st, A, lda, s, nb, c0, nc);
+    hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nc, 4 * cpw)), dim3(256), 0, st, A, lda, s, nb, c0, nc, cpw);
     gemm_sub<T>(A + (s + nb) + s * lda, lda, A + s + c0 * lda, lda, A + (s + nb) + c0 * lda, lda, r1 - s - nb, nc, nb,
                 st);
   }
