@@ -45,7 +45,7 @@ EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", 
            "cdj_mlp_nparams", "cdj_mlp_get_params", "cdj_mlp_set_params", "cdj_mlp_train_dev", "cdj_mlp_loss_grad",
            "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning",
            "cdj_traction_rows", "cdj_lstsq", "cdj_crack_tip_elements", "cdj_stress_intensity",
-           "cdj_stress_intensity_dev", "cdj_slip_friction", "cdj_mlp_forward"]
+           "cdj_stress_intensity_dev", "cdj_slip_friction", "cdj_slip_fluid_volume", "cdj_mlp_forward"]
 
 
 class CdjError(RuntimeError):
@@ -99,6 +99,8 @@ def load(path=LIB_PATH):
     L.cdj_lstsq.argtypes = [_p, _p, _i64, _i64, _p, _i64, _p, _p]
     L.cdj_mlp_forward.argtypes = [_p, _p, _p, _p, _p, _i64, _i32, _p]
     L.cdj_slip_friction.argtypes = [_p, _p, _p, _p, _i64, _d, _d, _d, _p, _p, _i32, _p, _p, _p, _p, _p, _p, _p]
+    L.cdj_slip_fluid_volume.argtypes = [_p, _p, _p, _p, _i64, _d, _d, _d, _p, _p, _i32, _p, _p, _d, _p, _p, _p, _p,
+                                        _p]
     L.cdj_crack_tip_elements.argtypes = [_p, _p, _p, _p, _p, _p, _i64, _p, _p]
     L.cdj_stress_intensity.argtypes = [_p, _p, _p, _p, _i64, _d, _d, _p, _p, _p, _p]
     L.cdj_stress_intensity_dev.argtypes = [_p, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _d, _d, _p, _p, _p, _p]

@@ -79,6 +79,12 @@ class MixedBoundaryConditionsFriction:
 
 
 @dataclass
+class MixedBoundaryConditionsFluidVolume:
+    MixedBoundaryConditions: Any = None
+    Volumes: Any = None
+
+
+@dataclass
 class TractionInf:
     DnTn: Any = None
     DnTss: Any = None
@@ -280,12 +286,16 @@ def SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, 
 
     Returns (Dn, Dss, Dds, A, b) like the reference.  b may also be given directly as a 3N x nrhs array in place
     of BoundaryConditions, in which case Dn/Dss/Dds are N x nrhs.  With MixedBoundaryConditionsFriction the
-    frictional method runs instead (SlipCalculator3D.jl:36-84) and, like the reference, returns (Dn, Dss, Dds).
+    frictional method runs instead (SlipCalculator3D.jl:36-84) and, like the reference, returns (Dn, Dss, Dds);
+    with MixedBoundaryConditionsFluidVolume the last argument is the FractureFlag vector and the constant-volume
+    method runs (SlipCalculator3D.jl:242-475), also returning (Dn, Dss, Dds).
     """
     nrm = f64(FaceNormalVector)
     n = nrm.shape[0]
     if isinstance(BoundaryConditions, MixedBoundaryConditionsFriction):
         return _slip_friction(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BoundaryConditions, FixedEls)
+    if isinstance(BoundaryConditions, MixedBoundaryConditionsFluidVolume):
+        return _slip_fluid_volume(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BoundaryConditions, FixedEls)
     fixed = np.asarray(FixedEls).ravel() == 1 if FixedEls is not None else np.zeros(n, bool)
     if isinstance(BoundaryConditions, np.ndarray):
         b = f64(BoundaryConditions.reshape(3 * n, -1, order="F"))
@@ -338,6 +348,31 @@ def _slip_friction(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BC, FixedEls):
                                    ptr(Dss), ptr(Dds), ctypes.byref(rep)))
     LAST_REPORT.clear()
     LAST_REPORT.update(rep.as_dict())
+    return Dn, Dss, Dds
+
+
+def _slip_fluid_volume(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BC, FractureFlag):
+    """SlipCalculator3D(..., ::MixedBoundaryConditionsFluidVolume, FractureFlag) (SlipCalculator3D.jl:242-475).
+
+    The pressure found is left in LAST_REPORT["pressure"]."""
+    n = nrm.shape[0]
+    Tn, Tds, Tss = SetupTractionVector(BC.MixedBoundaryConditions, nrm, n, G, λ)
+    b = f64(np.concatenate([Tn, Tss, Tds]))
+    flag = np.ascontiguousarray(np.asarray(FractureFlag).ravel(), dtype=np.int32)
+    if flag.shape[0] != n:
+        raise CdjError(_lib.CDJ_ERR_DIMS, "Element size inputs must be the same dimensions")
+    V = float(np.asarray(BC.Volumes, dtype=np.float64).ravel()[0])
+    P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
+    Dn, Dss, Dds = np.zeros(n), np.zeros(n), np.zeros(n)
+    pressure = ctypes.c_double(0.0)
+    rep = Report()
+    check(load().cdj_slip_fluid_volume(ctx(), ptr(P1), ptr(P2), ptr(P3), n, float(ν), float(G), float(λ),
+                                       ptr(f64(MidPoint)), ptr(nrm), int(HSFlag), ptr(b),
+                                       flag.ctypes.data_as(ctypes.c_void_p), V, ptr(Dn), ptr(Dss), ptr(Dds),
+                                       ctypes.byref(pressure), ctypes.byref(rep)))
+    LAST_REPORT.clear()
+    LAST_REPORT.update(rep.as_dict())
+    LAST_REPORT["pressure"] = pressure.value
     return Dn, Dss, Dds
 
 

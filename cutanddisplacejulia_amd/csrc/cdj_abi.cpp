@@ -17,6 +17,7 @@
 #include "linalg.hpp"
 #include "mixed.hpp"
 #include "td_kernels.hpp"
+#include "univariate.hpp"
 
 using namespace cdj;
 
@@ -1630,6 +1631,258 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
     report->iterations = fr.iterations;
     report->residual = fr.err;
     report->solve_ms = ms;
+  }
+  return CDJ_OK;
+}
+
+}  // extern "C"
+
+// ---- constant fluid volume (SlipCalculator3D.jl:242-475, ComputePressurisedCrackDn.jl) ----------------------
+
+namespace {
+
+// Area and half perimeter of each triangle (AreaOfTriangle3D.jl: Heron's formula on the edge lengths)
+void area_half_perimeter(const double* P1, const double* P2, const double* P3, int64_t n, std::vector<double>& area,
+                         std::vector<double>& hp) {
+  area.assign(n, 0.0);
+  hp.assign(n, 0.0);
+  const double* Q[3] = {P1, P2, P3};
+  for (int64_t i = 0; i < n; ++i) {
+    double L[3];
+    for (int s = 0; s < 3; ++s) {
+      const double* A = Q[s];
+      const double* B = Q[(s + 1) % 3];
+      const double dx = A[i] - B[i], dy = A[n + i] - B[n + i], dz = A[2 * n + i] - B[2 * n + i];
+      L[s] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    const double h = (L[0] + L[1] + L[2]) / 2.0;
+    hp[i] = h;
+    area[i] = std::sqrt(h * (h - L[0]) * (h - L[1]) * (h - L[2]));
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,
+                          double G, double lambda, const double* midpoints, const double* normals, int32_t hs_flag,
+                          const double* b, const int32_t* fracture_flag, double volume, double* Dn, double* Dss,
+                          double* Dds, double* pressure, cdj_report* report) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  if (!P1 || !P2 || !P3 || !midpoints || !normals || !b || !fracture_flag || !Dn || !Dss || !Dds)
+    return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  if (report) *report = cdj_report{};
+  if (pressure) *pressure = 0.0;
+  if (n == 0) return CDJ_OK;
+  if (5 * n > 65535) return fail(ctx, CDJ_ERR_DIMS, "frictional solve too large (5 n > 65535)");
+  // fracture 1 is the crack (SlipCalculator3D.jl:286: more than one fracture ends in the reference's error),
+  // 0 marks free-surface elements (ComputePressurisedCrackDn.jl:43-46)
+  bool any_free = false;
+  for (int64_t i = 0; i < n; ++i) {
+    if (fracture_flag[i] > 1) return fail(ctx, CDJ_ERR_ARG, "Check scl here - what do we use for one crack below");
+    any_free = any_free || fracture_flag[i] == 0;
+  }
+  std::vector<double> area, hp;
+  area_half_perimeter(P1, P2, P3, n, area, hp);
+  double hsum = 0.0, asum = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    hsum += hp[i];
+    asum += area[i];
+  }
+  const double scale = (hsum / (double)n) / G;  // SlipCalculator3D.jl:257-258
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  const int64_t N3 = 3 * n, N5 = 5 * n;
+  const size_t v3 = sizeof(double) * N3;
+  DBuf dn, dm, d1, d2, d3, AI, dB, dD, dmu, dSf, a, bb0, bb, shift, J, x, vec, red, ipiv, info, ws, D3, dz;
+  HIPCHK(dn.alloc(v3));
+  HIPCHK(dm.alloc(v3));
+  HIPCHK(d1.alloc(v3));
+  HIPCHK(d2.alloc(v3));
+  HIPCHK(d3.alloc(v3));
+  HIPCHK(hipMemcpyAsync(dn.p, normals, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dm.p, midpoints, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d1.p, P1, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d2.p, P2, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d3.p, P3, v3, hipMemcpyHostToDevice, st));
+  HIPCHK(AI.alloc(sizeof(double) * N3 * 2 * N3));  // [A | I], then [LU | A^-1]
+  cdj_opts o{};
+  o.precision = CDJ_PREC_FP64;
+  o.assembly = CDJ_ASM_FP64;
+  int rc = cdj_assemble_dev(ctx, dn.as<double>(), dm.as<double>(), d1.as<double>(), d2.as<double>(), d3.as<double>(),
+                            n, nu, G, lambda, hs_flag, AI.as<double>(), N3, &o, st);
+  if (rc) return rc;
+  scale_matrix(AI.as<double>(), N3 * N3, scale, st);
+  // Norm: the largest normal traction of a uniform opening of the target volume (SlipCalculator3D.jl:348-356)
+  std::vector<double> h3(N3, 0.0);
+  const double height = (volume / asum) / scale;
+  for (int64_t i = 0; i < n; ++i) h3[i] = height;
+  HIPCHK(dB.alloc(v3));
+  HIPCHK(dD.alloc(v3));
+  HIPCHK(hipMemcpyAsync(dB.p, h3.data(), v3, hipMemcpyHostToDevice, st));
+  gemv<double>(AI.as<double>(), N3, N3, N3, dB.as<double>(), dD.as<double>(), 1.0, 0.0, st);
+  HIPCHK(hipMemcpyAsync(h3.data(), dD.p, v3, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double norm = h3[0];
+  for (int64_t i = 1; i < n; ++i) norm = std::max(norm, h3[i]);
+  HIPCHK(ipiv.alloc(sizeof(int32_t) * N5));
+  HIPCHK(info.alloc(sizeof(int)));
+  HIPCHK(ws.alloc(lu_workspace_bytes(N5)));
+  HIPCHK(invert(AI.as<double>(), N3, ipiv.as<int32_t>(), info.as<int>(), ws.p, st));
+  int h_info = 0;
+  HIPCHK(hipMemcpyAsync(&h_info, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (h_info < 0) return fail(ctx, CDJ_ERR_DEVICE, "LU panel barrier wait timed out");
+  if (h_info > 0) return fail(ctx, CDJ_ERR_SINGULAR, "matrix is exactly singular (zero pivot " + std::to_string(h_info) + ")");
+  const double* C = AI.as<double>() + N3 * N3;
+  // the frictional problem with the reference's fixed parameters mu = 0.1, Sf = 0 (SlipCalculator3D.jl:275-276)
+  const std::vector<double> mu(n, 0.1), Sf(n, 0.0), zeros(N3, 0.0);
+  HIPCHK(dmu.alloc(sizeof(double) * n));
+  HIPCHK(dSf.alloc(sizeof(double) * n));
+  HIPCHK(dz.alloc(v3));
+  HIPCHK(hipMemcpyAsync(dmu.p, mu.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dSf.p, Sf.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dz.p, zeros.data(), v3, hipMemcpyHostToDevice, st));
+  HIPCHK(a.alloc(sizeof(double) * N5 * N5));
+  HIPCHK(bb0.alloc(sizeof(double) * N5));
+  HIPCHK(bb.alloc(sizeof(double) * N5));
+  build_friction_lcp(C, n, dz.as<double>(), dmu.as<double>(), dSf.as<double>(), a.as<double>(), bb0.as<double>(), st);
+  HIPCHK(hipGetLastError());
+  // free-surface elements: the shift that keeps their opening entries non-negative (ComputePressurisedCrackDn.jl:
+  // 185-193); the value added to every free element's entry, and subtracted from its opening afterwards
+  double arb = 0.0;
+  if (any_free) {
+    std::vector<double> h0(n);
+    HIPCHK(hipMemcpyAsync(h0.data(), bb0.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    double mn = INFINITY;
+    for (int64_t i = 0; i < n; ++i)
+      if (fracture_flag[i] == 0) mn = std::min(mn, h0[i]);
+    arb = std::fabs(mn * 2);
+  }
+  std::vector<double> hshift(N5, 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    if (fracture_flag[i] == 0) hshift[i] = arb;
+  HIPCHK(shift.alloc(sizeof(double) * N5));
+  HIPCHK(hipMemcpyAsync(shift.p, hshift.data(), sizeof(double) * N5, hipMemcpyHostToDevice, st));
+  HIPCHK(J.alloc(sizeof(double) * N5 * N5));
+  HIPCHK(x.alloc(sizeof(double) * N5));
+  HIPCHK(vec.alloc(sizeof(double) * 8 * N5));
+  HIPCHK(red.alloc(sizeof(double) * 8));
+  HIPCHK(D3.alloc(v3));
+  FnParams fp;
+  int evaluations = 0, newton = 0;
+  hipError_t herr = hipSuccess;
+  std::vector<double> hB(N3), hD(N3), s3(N3);
+  // ComputePressurisedCrackDn(x::Tractions, ...): slip at pressure x Norm on the fracture elements, into s3
+  auto slip_at = [&](double xp) -> bool {
+    ++evaluations;
+    for (int64_t i = 0; i < N3; ++i) hB[i] = b[i];
+    const double add = xp * norm;
+    for (int64_t i = 0; i < n; ++i)
+      if (fracture_flag[i] == 1) hB[i] += add;
+    if ((herr = hipMemcpyAsync(dB.p, hB.data(), v3, hipMemcpyHostToDevice, st)) != hipSuccess) return false;
+    gemv<double>(C, N3, N3, N3, dB.as<double>(), dD.as<double>(), 1.0, 0.0, st);
+    if ((herr = hipMemcpyAsync(hD.data(), dD.p, sizeof(double) * n, hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return false;
+    if ((herr = hipStreamSynchronize(st)) != hipSuccess) return false;
+    bool opening = false;
+    for (int64_t i = 0; i < n && !opening; ++i) opening = (!any_free || fracture_flag[i] != 0) && hD[i] > 0;
+    if (!opening) {  // already closed: no slip
+      std::fill(s3.begin(), s3.end(), 0.0);
+      return true;
+    }
+    // (bb0 + D) + shift where the reference forms (bb0 + shift) + D: with Sf = 0 the shift is zero, so the two agree
+    build_friction_lcp_vector(C, n, dD.as<double>(), dSf.as<double>(), bb.as<double>(), st);
+    if (any_free) add_vector(bb.as<double>(), shift.as<double>(), N5, st);
+    FnResult fr;
+    if ((herr = fischer_newton(a.as<double>(), bb.as<double>(), N5, x.as<double>(), fp, fr, J.as<double>(),
+                               ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(), st)) !=
+        hipSuccess)
+      return false;
+    newton += fr.iterations;
+    friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(), st);
+    if ((herr = hipMemcpyAsync(s3.data(), D3.p, v3, hipMemcpyDeviceToHost, st)) != hipSuccess) return false;
+    if ((herr = hipStreamSynchronize(st)) != hipSuccess) return false;
+    if (any_free)
+      for (int64_t i = 0; i < n; ++i) s3[i] -= arb;
+    return true;
+  };
+  auto vol_of = [&]() {
+    double v = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+      if (fracture_flag[i] == 1) v += s3[i] * area[i];
+    return v;
+  };
+  auto volume_at = [&](double xp, double& v) -> bool {
+    if (!slip_at(xp)) return false;
+    v = vol_of();
+    return true;
+  };
+  auto misfit = [&](double xp, double& f) -> bool {
+    double v;
+    if (!volume_at(xp, v)) return false;
+    f = std::fabs(volume - v) * std::fabs(volume - v);
+    return true;
+  };
+  auto dev_fail = [&]() { return hip_fail(ctx, herr == hipSuccess ? hipErrorUnknown : herr, "fluid-volume solve"); };
+  // scale Norm until the volume at x = 10 lies in [V, 2.5 V] (SlipCalculator3D.jl:365-392)
+  constexpr int kNormLoops = 200;
+  double huge = 0.0;
+  if (!volume_at(10.0, huge)) return dev_fail();
+  for (int it = 0; huge < volume; ++it) {
+    if (it >= kNormLoops) return fail(ctx, CDJ_ERR_ARG, "the target volume is not reached at any pressure scale");
+    norm *= 1.05;
+    if (!volume_at(10.0, huge)) return dev_fail();
+  }
+  double reduce = 0.3;
+  for (int it = 0; huge > volume * 2.5; ++it) {
+    if (it >= kNormLoops) return fail(ctx, CDJ_ERR_ARG, "the pressure scale does not settle");
+    norm *= (1 - reduce);
+    if (!volume_at(10.0, huge)) return dev_fail();
+    if (huge < volume) {
+      norm /= (1 - reduce);
+      reduce /= 2;
+      if (!volume_at(10.0, huge)) return dev_fail();
+    }
+  }
+  // Brent on [0, 10] with abs_tol 0.001; golden section on [0, 50] when its volume is negative
+  // (SlipCalculator3D.jl:404-431)
+  Minimum mn = brent_minimize(misfit, 0.0, 10.0, 0.001);
+  if (!mn.ok) return dev_fail();
+  double vol_out = 0.0;
+  if (!volume_at(mn.x, vol_out)) return dev_fail();
+  if ((volume - vol_out) > volume) {
+    mn = golden_section_minimize(misfit, 0.0, 50.0);
+    if (!mn.ok) return dev_fail();
+  }
+  if (!slip_at(mn.x)) return dev_fail();
+  const double vol = vol_of();
+  const bool bad = std::fabs(vol) > std::fabs(volume * 2);  // SlipCalculator3D.jl:460-465: NaN the result
+  for (int64_t i = 0; i < n; ++i) {
+    Dn[i] = bad ? NAN : s3[i];
+    Dss[i] = bad ? NAN : s3[n + i];
+    Dds[i] = bad ? NAN : s3[2 * n + i];
+  }
+  if (pressure) *pressure = mn.x * norm;
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (report) {
+    report->path = CDJ_PREC_FP64;
+    report->iterations = evaluations;
+    report->residual = volume != 0.0 ? std::fabs(volume - vol) / std::fabs(volume) : std::fabs(vol);
+    report->solve_ms = ms;
+    report->near_entries = newton;
   }
   return CDJ_OK;
 }

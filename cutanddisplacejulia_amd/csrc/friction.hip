@@ -170,6 +170,20 @@ void build_friction_lcp(const double* C, int64_t n, const double* D, const doubl
   hipLaunchKernelGGL(lcp_vector_kernel, dim3(nblk(N5, 256)), dim3(256), 0, st, C, n, D, Sf, bb);
 }
 
+__global__ void add_kernel(double* __restrict__ y, const double* __restrict__ x, int64_t N) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < N) y[i] += x[i];
+}
+
+void add_vector(double* y, const double* x, int64_t N, hipStream_t st) {
+  hipLaunchKernelGGL(add_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, y, x, N);
+}
+
+void build_friction_lcp_vector(const double* C, int64_t n, const double* D, const double* Sf, double* bb,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(lcp_vector_kernel, dim3(nblk(5 * n, 256)), dim3(256), 0, st, C, n, D, Sf, bb);
+}
+
 hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* x, const FnParams& P, FnResult& R,
                           double* J, int32_t* ipiv, int* info, void* ws, double* vec, double* red, hipStream_t st) {
   double* y = vec;

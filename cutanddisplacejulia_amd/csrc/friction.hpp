@@ -16,6 +16,13 @@ void scale_matrix(double* A, int64_t count, double s, hipStream_t st);
 void build_friction_lcp(const double* C, int64_t n, const double* D, const double* mu, const double* Sf, double* a,
                         double* bb, hipStream_t st);
 
+// bb alone (as build_friction_lcp): the vector for another D with the same C and Sf
+void build_friction_lcp_vector(const double* C, int64_t n, const double* D, const double* Sf, double* bb,
+                               hipStream_t st);
+
+// y[0:N] += x[0:N]
+void add_vector(double* y, const double* x, int64_t N, hipStream_t st);
+
 struct FnParams {
   int max_iter = 100;
   double tol_abs = 1e-26;  // on 0.5 ||phi||^2

@@ -237,6 +237,24 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
                       const double* b, const double* mu, const double* Sf, double* Dn, double* Dss, double* Dds,
                       cdj_report* report);
 
+/* Constant fluid volume: replaces SlipCalculator3D(P1,P2,P3,nu,G,lambda,MidPoint,FaceNormalVector,HSFlag,
+ * MixedBoundaryConditionsFluidVolume,FractureFlag) (SlipCalculator3D.jl:242-475 with ComputePressurisedCrackDn.jl),
+ * the single-fracture branch.  b: 3n tractions [Tn; Tss; Tds] of the remote stresses plus the prescribed
+ * tractions; fracture_flag: n ints, 1 = crack element, 0 = free-surface element (more than one fracture, a flag
+ * above 1, ends in the reference's error); volume: the crack's target volume V.  A pressure x Norm on the crack's
+ * normal traction is searched for whose frictional slip (mu = 0.1, Sf = 0, as the reference fixes them) gives the
+ * volume V: Norm from a uniform opening of V, rescaled until the volume at x = 10 lies in [V, 2.5 V], then Brent's
+ * method on (V - volume(x))^2 over [0, 10] (abs_tol 0.001) and a golden-section search over [0, 50] when that
+ * minimiser's volume is negative.  Each volume evaluation is one complementarity solve on the GPU with the
+ * inverse and the 5n x 5n matrix built once.  The slip is NaN when its volume exceeds 2 |V| (as the reference).
+ * pressure (may be NULL): the pressure x Norm found.  report: iterations = volume evaluations, near_entries =
+ * Newton steps over all evaluations, residual = |V - volume| / |V|, solve_ms = device time of the whole call.
+ * 5 n <= 65535. */
+int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,
+                          double G, double lambda, const double* midpoints, const double* normals, int32_t hs_flag,
+                          const double* b, const int32_t* fracture_flag, double volume, double* Dn, double* Dss,
+                          double* Dds, double* pressure, cdj_report* report);
+
 /* ---- crack-tip edges and stress intensities (GetCrackTipElements3D.jl:1, StressIntensity3D.jl:1) ----
  * Edge blocks, in this order: P1P2, P1P3, P2P3 (the reference's FeP1P2S, FeP1P3S, FeP2P3S).
  * free_flags: n x 3 bytes (block-major: flags of block s at [s n, (s + 1) n)), 1 = free (crack-tip) edge.

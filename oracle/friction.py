@@ -9,8 +9,8 @@ problem of Kaven et al. (2012, eq. 28, as modified there) and the slip read back
 The reference solves the LCP with FischerNewton 0.1.4 (a package the reference does not vendor).  Restated here is
 the published Fischer-Newton method of Erleben (num4lcp, "Numerical methods for linear complementarity problems in
 physics-based animation", 2013): the Fischer-Burmeister function phi(y, x) = sqrt(y^2 + x^2) - y - x, the
-perturbation strategy for its generalized Jacobian, a sufficient-descent check with the gradient as fallback and a
-projected Armijo backtracking line search.  The Newton system is solved exactly (LU) where num4lcp uses restarted
+perturbation strategy for its generalized Jacobian, a sufficient-descent check with the gradient as fallback (also
+taken when the Newton system is exactly singular) and a projected Armijo backtracking line search.  The Newton system is solved exactly (LU) where num4lcp uses restarted
 GMRES; the parameters are those of num4lcp's defaults except the iteration cap and tolerances named below.  The
 LCP solution (x >= 0, y = a x + b >= 0, x'y = 0) is unique for these matrices, so any convergent method gives it.
 """
@@ -48,11 +48,16 @@ def fischer_newton(A, b, max_iter=FN["max_iter"], tol_abs=FN["tol_abs"], alpha=F
         p = px / nr - 1.0
         q = y / nr - 1.0
         J = np.diag(p) + q[:, None] * A
-        dx = np.linalg.solve(J, -phi)
         nabla = p * phi + A.T @ (q * phi)
-        if np.max(np.abs(dx)) < np.finfo(float).eps or np.linalg.norm(nabla) < tol_abs:
+        try:
+            dx = np.linalg.solve(J, -phi)
+            grad = False
+        except np.linalg.LinAlgError:  # exactly singular Newton system: gradient direction
+            dx = -nabla
+            grad = True
+        if not grad and (np.max(np.abs(dx)) < np.finfo(float).eps or np.linalg.norm(nabla) < tol_abs):
             break
-        if nabla @ dx > -rho * (dx @ dx):
+        if grad or nabla @ dx > -rho * (dx @ dx):
             dx = -nabla
         tau = 1.0
         f0 = err
