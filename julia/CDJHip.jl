@@ -10,7 +10,7 @@
 module CDJHip
 
 using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, SetupTractionVector, KnockOutFixedRows,
-    TriangleEdges, MixedBoundaryConditionsFriction
+    TriangleEdges, MixedBoundaryConditionsFriction, MixedBoundaryConditionsFluidVolume
 
 const LIB = get(ENV, "CDJ_LIB", joinpath(@__DIR__, "..", "cutanddisplacejulia_amd", "libcdjhip.so"))
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)
@@ -155,6 +155,26 @@ function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSF
              Ref{Report}),
             ctx(), _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, _f64(MidPoint), _f64(FaceNormalVector), HSFlag,
             _f64(b), µ, Sf, Dn, Dss, Dds, rep))
+    end
+    return Dn, Dss, Dds
+end
+
+function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag,
+                          BoundaryConditions::MixedBoundaryConditionsFluidVolume, FractureFlag)
+    n = size(FaceNormalVector, 1)
+    (Tn, Tds, Tss) = SetupTractionVector(BoundaryConditions.MixedBoundaryConditions, FaceNormalVector, n, G, λ)
+    b = [Tn; Tss; Tds]
+    flag = Int32.(vec(FractureFlag))
+    V = Float64(BoundaryConditions.Volumes[1])
+    Dn = zeros(n); Dss = zeros(n); Dds = zeros(n); pressure = Ref(0.0)
+    rep = Ref(Report())
+    lock(LOCK) do
+        check(ccall((:cdj_slip_fluid_volume, LIB), Cint,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Float64, Ptr{Float64},
+             Ptr{Float64}, Int32, Ptr{Float64}, Ptr{Int32}, Float64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+             Ref{Float64}, Ref{Report}),
+            ctx(), _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, _f64(MidPoint), _f64(FaceNormalVector), HSFlag,
+            _f64(b), flag, V, Dn, Dss, Dds, pressure, rep))
     end
     return Dn, Dss, Dds
 end
