@@ -916,16 +916,17 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   // all ones: slot tag 4095 and record tag 2^32 - 1 match no column of this factorization
   e = hipMemsetAsync(slots, 0xFF, xbytes, st);
   if (e != hipSuccess) return e;
-  static bool lds_set = false;  // raise the dynamic LDS limit once per process (up to 144 KiB per panel block)
-  if (!lds_set) {
+  // raise the dynamic LDS limit once per process (up to 144 KiB per panel block); thread-safe initialisation
+  static const hipError_t lds_set = [] {
     const int lds = (int)(sizeof(T) * ((size_t)MAX_RPB * LU_NB + USH));
-    e = hipFuncSetAttribute((const void*)lu_panel_kernel<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)lu_panel_kernel<T, MAX_P / 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            lds);
-    if (e != hipSuccess) return e;
-    lds_set = true;
-  }
+    hipError_t r = hipFuncSetAttribute((const void*)lu_panel_kernel<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       lds);
+    if (r == hipSuccess)
+      r = hipFuncSetAttribute((const void*)lu_panel_kernel<T, MAX_P / 64>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    return r;
+  }();
+  if (lds_set != hipSuccess) return lds_set;
   for (int64_t K0 = 0; K0 < n; K0 += LU_OUTER) {
     const int64_t K1 = std::min<int64_t>(n, K0 + LU_OUTER);
     for (int64_t k = K0; k < K1; k += LU_NB) {
