@@ -1560,7 +1560,7 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
   hipStream_t st = ctx->stream;
   const int64_t N3 = 3 * n, N5 = 5 * n;
   const size_t v3 = sizeof(double) * 3 * n;
-  DBuf dn, dm, d1, d2, d3, AI, db, dD, dmu, dSf, a, bb, J, x, vec, red, ipiv, info, ws, D3;
+  DBuf dn, dm, d1, d2, d3, AI, db, dD, dmu, dSf, a, bb, J, x, vec, red, ipiv, info, ws, D3, part;
   HIPCHK(dn.alloc(v3));
   HIPCHK(dm.alloc(v3));
   HIPCHK(d1.alloc(v3));
@@ -1591,8 +1591,9 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
   const double* C = AI.as<double>() + N3 * N3;
   HIPCHK(db.alloc(v3));
   HIPCHK(dD.alloc(v3));
+  HIPCHK(part.alloc(std::max(gemv_part_bytes(N5, N5), gemv_part_bytes(N3, N3))));
   HIPCHK(hipMemcpyAsync(db.p, b, v3, hipMemcpyHostToDevice, st));
-  gemv<double>(C, N3, N3, N3, db.as<double>(), dD.as<double>(), 1.0, 0.0, st);
+  gemv_splitk<double>(C, N3, N3, N3, db.as<double>(), dD.as<double>(), 1.0, nullptr, 0.0, part.as<double>(), st);
   HIPCHK(dmu.alloc(sizeof(double) * n));
   HIPCHK(dSf.alloc(sizeof(double) * n));
   HIPCHK(hipMemcpyAsync(dmu.p, mu, sizeof(double) * n, hipMemcpyHostToDevice, st));
@@ -1613,10 +1614,12 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
   hipError_t fe = fischer_newton(a.as<double>(), bb.as<double>(), N5, x.as<double>(), fp, fr, J.as<double>(),
-                                 ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(), st);
+                                 ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(),
+                                 part.as<double>(), st);
   if (fe != hipSuccess) return hip_fail(ctx, fe, "Fischer-Newton");
   HIPCHK(D3.alloc(v3));
-  friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(), st);
+  friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(),
+                part.as<double>(), st);
   HIPCHK(hipEventRecord(e1, st));
   HIPCHK(hipMemcpyAsync(Dn, D3.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(Dss, D3.as<double>() + n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
@@ -1700,7 +1703,7 @@ int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, cons
   HIPCHK(hipEventRecord(e0, st));
   const int64_t N3 = 3 * n, N5 = 5 * n;
   const size_t v3 = sizeof(double) * N3;
-  DBuf dn, dm, d1, d2, d3, AI, dB, dD, dmu, dSf, a, bb0, bb, shift, J, x, vec, red, ipiv, info, ws, D3, dz;
+  DBuf dn, dm, d1, d2, d3, AI, dB, dD, dmu, dSf, a, bb0, bb, shift, J, x, vec, red, ipiv, info, ws, D3, dz, part;
   HIPCHK(dn.alloc(v3));
   HIPCHK(dm.alloc(v3));
   HIPCHK(d1.alloc(v3));
@@ -1725,8 +1728,10 @@ int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, cons
   for (int64_t i = 0; i < n; ++i) h3[i] = height;
   HIPCHK(dB.alloc(v3));
   HIPCHK(dD.alloc(v3));
+  HIPCHK(part.alloc(std::max(gemv_part_bytes(N5, N5), gemv_part_bytes(N3, N3))));
   HIPCHK(hipMemcpyAsync(dB.p, h3.data(), v3, hipMemcpyHostToDevice, st));
-  gemv<double>(AI.as<double>(), N3, N3, N3, dB.as<double>(), dD.as<double>(), 1.0, 0.0, st);
+  gemv_splitk<double>(AI.as<double>(), N3, N3, N3, dB.as<double>(), dD.as<double>(), 1.0, nullptr, 0.0,
+                      part.as<double>(), st);
   HIPCHK(hipMemcpyAsync(h3.data(), dD.p, v3, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   double norm = h3[0];
@@ -1788,7 +1793,7 @@ int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, cons
     for (int64_t i = 0; i < n; ++i)
       if (fracture_flag[i] == 1) hB[i] += add;
     if ((herr = hipMemcpyAsync(dB.p, hB.data(), v3, hipMemcpyHostToDevice, st)) != hipSuccess) return false;
-    gemv<double>(C, N3, N3, N3, dB.as<double>(), dD.as<double>(), 1.0, 0.0, st);
+    gemv_splitk<double>(C, N3, N3, N3, dB.as<double>(), dD.as<double>(), 1.0, nullptr, 0.0, part.as<double>(), st);
     if ((herr = hipMemcpyAsync(hD.data(), dD.p, sizeof(double) * n, hipMemcpyDeviceToHost, st)) != hipSuccess)
       return false;
     if ((herr = hipStreamSynchronize(st)) != hipSuccess) return false;
@@ -1803,11 +1808,12 @@ int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, cons
     if (any_free) add_vector(bb.as<double>(), shift.as<double>(), N5, st);
     FnResult fr;
     if ((herr = fischer_newton(a.as<double>(), bb.as<double>(), N5, x.as<double>(), fp, fr, J.as<double>(),
-                               ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(), st)) !=
-        hipSuccess)
+                               ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(),
+                               part.as<double>(), st)) != hipSuccess)
       return false;
     newton += fr.iterations;
-    friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(), st);
+    friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(),
+                  part.as<double>(), st);
     if ((herr = hipMemcpyAsync(s3.data(), D3.p, v3, hipMemcpyDeviceToHost, st)) != hipSuccess) return false;
     if ((herr = hipStreamSynchronize(st)) != hipSuccess) return false;
     if (any_free)

@@ -13,6 +13,7 @@
 
 #include "friction.hpp"
 #include "linalg.hpp"
+#include "mixed.hpp"
 
 namespace cdj {
 
@@ -146,10 +147,10 @@ __global__ void slip_kernel(const double* __restrict__ y, const double* __restri
   D3[2 * n + i] = (y[2 * n + i] - x[4 * n + i]) * s;
 }
 
-// y = a x + bb
-void residual(const double* a, const double* bb, const double* x, int64_t N, double* y, hipStream_t st) {
-  (void)hipMemcpyAsync(y, bb, sizeof(double) * N, hipMemcpyDeviceToDevice, st);
-  gemv<double>(a, N, N, N, x, y, 1.0, 1.0, st);
+// y = a x + bb (split-K product: part holds gemv_part_bytes(N, N))
+void residual(const double* a, const double* bb, const double* x, int64_t N, double* y, double* part,
+              hipStream_t st) {
+  gemv_splitk<double>(a, N, N, N, x, y, 1.0, bb, 1.0, part, st);
 }
 
 __global__ void scale_kernel(double* __restrict__ A, int64_t count, double s) {
@@ -185,7 +186,8 @@ void build_friction_lcp_vector(const double* C, int64_t n, const double* D, cons
 }
 
 hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* x, const FnParams& P, FnResult& R,
-                          double* J, int32_t* ipiv, int* info, void* ws, double* vec, double* red, hipStream_t st) {
+                          double* J, int32_t* ipiv, int* info, void* ws, double* vec, double* red, double* part,
+                          hipStream_t st) {
   double* y = vec;
   double* phi = vec + N;
   double* p = vec + 2 * N;
@@ -197,7 +199,7 @@ hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* 
   hipError_t e = hipMemsetAsync(x, 0, sizeof(double) * N, st);
   if (e != hipSuccess) return e;
   auto merit = [&](const double* xx, double& f) -> hipError_t {
-    residual(a, bb, xx, N, y, st);
+    residual(a, bb, xx, N, y, part, st);
     hipLaunchKernelGGL(fb_kernel, dim3(g), dim3(256), 0, st, (const double*)y, xx, N, phi);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, st, (const double*)phi, (const double*)phi, N, 0, red);
     double h = 0.0;
@@ -260,8 +262,8 @@ hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* 
 }
 
 void friction_slip(const double* a, const double* bb, const double* x, int64_t n, double scale, double* y, double* D3,
-                   hipStream_t st) {
-  residual(a, bb, x, 5 * n, y, st);
+                   double* part, hipStream_t st) {
+  residual(a, bb, x, 5 * n, y, part, st);
   hipLaunchKernelGGL(slip_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, (const double*)y, x, n, scale, D3);
 }
 

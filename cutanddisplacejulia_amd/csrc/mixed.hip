@@ -243,16 +243,26 @@ __global__ void scale_by_norm_kernel(const double* __restrict__ w, const double*
 }  // namespace
 
 // ---- host-side building blocks -----------------------------------------------------------------------------
+// Columns per partial sum: enough chunks that the launch has about 1024 workgroups (a small matrix otherwise leaves
+// most CUs idle and the product latency-bound), between 64 and 1024 columns.
+static int gemv_chunk(int64_t m, int64_t n) {
+  const int64_t rb = (m + 255) / 256;
+  const int64_t want = (1024 + rb - 1) / rb;
+  int64_t chunk = (n + want - 1) / want;
+  chunk = (chunk + 63) / 64 * 64;
+  return (int)std::min<int64_t>(1024, std::max<int64_t>(64, chunk));
+}
+
 size_t gemv_part_bytes(int64_t m, int64_t n) {
-  const int chunk = 1024;
-  return sizeof(double) * (size_t)m * ((n + chunk - 1) / chunk);
+  const int chunk = gemv_chunk(m, n);
+  return sizeof(double) * (size_t)std::max<int64_t>(m, 1) * (size_t)std::max<int64_t>((n + chunk - 1) / chunk, 1);
 }
 
 template <class TA>
 void gemv_splitk(const TA* A, int64_t m, int64_t n, int64_t lda, const double* x, double* y, double alpha,
                  const double* yin, double beta, double* part, hipStream_t st) {
   if (m <= 0) return;
-  const int chunk = 1024;
+  const int chunk = gemv_chunk(m, n);
   const int nchunk = (int)((n + chunk - 1) / chunk);
   if (nchunk > 0)
     hipLaunchKernelGGL(gemv_part_kernel<TA>, dim3(nblk(m, 256), (unsigned)nchunk), dim3(256), 0, st, A, m, n, lda, x,
