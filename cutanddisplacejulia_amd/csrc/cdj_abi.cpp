@@ -1560,7 +1560,7 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
   hipStream_t st = ctx->stream;
   const int64_t N3 = 3 * n, N5 = 5 * n;
   const size_t v3 = sizeof(double) * 3 * n;
-  DBuf dn, dm, d1, d2, d3, AI, db, dD, dmu, dSf, a, bb, J, x, vec, red, ipiv, info, ws, D3, part;
+  DBuf dn, dm, d1, d2, d3, AI, db, dD, dmu, dSf, a, bb, J, x, vec, red, ipiv, info, ws, D3, part, iwork;
   HIPCHK(dn.alloc(v3));
   HIPCHK(dm.alloc(v3));
   HIPCHK(d1.alloc(v3));
@@ -1613,9 +1613,10 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
+  HIPCHK(iwork.alloc(sizeof(int) * fn_iwork_ints(N5)));
   hipError_t fe = fischer_newton(a.as<double>(), bb.as<double>(), N5, x.as<double>(), fp, fr, J.as<double>(),
                                  ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(),
-                                 part.as<double>(), st);
+                                 part.as<double>(), iwork.as<int>(), st);
   if (fe != hipSuccess) return hip_fail(ctx, fe, "Fischer-Newton");
   HIPCHK(D3.alloc(v3));
   friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(),
@@ -1703,7 +1704,8 @@ int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, cons
   HIPCHK(hipEventRecord(e0, st));
   const int64_t N3 = 3 * n, N5 = 5 * n;
   const size_t v3 = sizeof(double) * N3;
-  DBuf dn, dm, d1, d2, d3, AI, dB, dD, dmu, dSf, a, bb0, bb, shift, J, x, vec, red, ipiv, info, ws, D3, dz, part;
+  DBuf dn, dm, d1, d2, d3, AI, dB, dD, dmu, dSf, a, bb0, bb, shift, J, x, vec, red, ipiv, info, ws, D3, dz, part,
+      iwork;
   HIPCHK(dn.alloc(v3));
   HIPCHK(dm.alloc(v3));
   HIPCHK(d1.alloc(v3));
@@ -1781,6 +1783,7 @@ int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, cons
   HIPCHK(vec.alloc(sizeof(double) * 8 * N5));
   HIPCHK(red.alloc(sizeof(double) * 8));
   HIPCHK(D3.alloc(v3));
+  HIPCHK(iwork.alloc(sizeof(int) * fn_iwork_ints(N5)));
   FnParams fp;
   int evaluations = 0, newton = 0;
   hipError_t herr = hipSuccess;
@@ -1809,7 +1812,7 @@ int cdj_slip_fluid_volume(cdj_ctx* ctx, const double* P1, const double* P2, cons
     FnResult fr;
     if ((herr = fischer_newton(a.as<double>(), bb.as<double>(), N5, x.as<double>(), fp, fr, J.as<double>(),
                                ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(),
-                               part.as<double>(), st)) != hipSuccess)
+                               part.as<double>(), iwork.as<int>(), st)) != hipSuccess)
       return false;
     newton += fr.iterations;
     friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(),

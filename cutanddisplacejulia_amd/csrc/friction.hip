@@ -187,7 +187,14 @@ void build_friction_lcp_vector(const double* C, int64_t n, const double* D, cons
 
 hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* x, const FnParams& P, FnResult& R,
                           double* J, int32_t* ipiv, int* info, void* ws, double* vec, double* red, double* part,
-                          hipStream_t st) {
+                          int* iwork, hipStream_t st) {
+  // Newton systems: J's LU, then the two triangular solves as block-chained persistent kernels (one launch each)
+  int32_t* perm = iwork;
+  int* flags = iwork + N;
+  int* terr = flags + (N + 63) / 64 + 1;
+  hipError_t e0 = hipMemsetAsync(flags, 0, sizeof(int) * ((N + 63) / 64 + 2), st);
+  if (e0 != hipSuccess) return e0;
+  TrsvState ts{flags, terr, 0, 1ll << 22};
   double* y = vec;
   double* phi = vec + N;
   double* p = vec + 2 * N;
@@ -195,6 +202,7 @@ hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* 
   double* dx = vec + 4 * N;
   double* nabla = vec + 5 * N;
   double* xk = vec + 6 * N;
+  double* work = vec + 7 * N;
   const unsigned g = nblk(N, 256);
   hipError_t e = hipMemsetAsync(x, 0, sizeof(double) * N, st);
   if (e != hipSuccess) return e;
@@ -226,7 +234,10 @@ hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* 
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
     if (h_info < 0) return hipErrorLaunchTimeOut;  // a panel's wait overran: factors invalid
     bool grad = h_info > 0;                          // singular Newton system
-    if (!grad && (e = lu_solve<double>(J, N, N, ipiv, dx, 1, N, st)) != hipSuccess) return e;
+    if (!grad) {
+      if ((e = lu_permutation(ipiv, N, perm, st)) != hipSuccess) return e;
+      if ((e = apply_lu_precond<double>(J, N, N, perm, dx, dx, work, ts, st)) != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, st, (const double*)dx, (const double*)dx, N, 1, red);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, st, (const double*)nabla, (const double*)nabla, N, 0,
                        red + 1);
@@ -234,8 +245,11 @@ hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* 
                        red + 2);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, st, (const double*)dx, (const double*)dx, N, 0, red + 3);
     double s[4];
+    int h_terr = 0;
     if ((e = hipMemcpyAsync(s, red, sizeof(s), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&h_terr, terr, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    if (h_terr) return hipErrorLaunchTimeOut;  // a triangular solve's block wait overran: dx invalid
     const double dxmax = s[0], nn = s[1];
     double ndx = s[2];
     if (!grad && (dxmax < 2.220446049250313e-16 || std::sqrt(nn) < P.tol_abs)) break;

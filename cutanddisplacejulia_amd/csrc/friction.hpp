@@ -40,11 +40,12 @@ struct FnResult {
 
 // x >= 0 with y = a x + b >= 0 and x'y = 0 (a: N x N, leading dimension N; x: N, output).  Newton systems are solved
 // by the GPU LU (ws: lu_workspace_bytes(N); J: N x N scratch; ipiv: N ints; info: one int; vec: 8 N doubles;
-// red: 8 doubles; part: gemv_part_bytes(N, N) for the products a x), the Fischer-Burmeister merit function is
-// reduced on the device and read back once per trial.
+// red: 8 doubles; part: gemv_part_bytes(N, N) for the products a x; iwork: fn_iwork_ints(N) ints), the
+// Fischer-Burmeister merit function is reduced on the device and read back once per trial.
 hipError_t fischer_newton(const double* a, const double* bb, int64_t N, double* x, const FnParams& p, FnResult& r,
                           double* J, int32_t* ipiv, int* info, void* ws, double* vec, double* red, double* part,
-                          hipStream_t st);
+                          int* iwork, hipStream_t st);
+inline int64_t fn_iwork_ints(int64_t N) { return N + (N + 63) / 64 + 2; }
 
 // Slip from the LCP solution: Dn = y1 s, Dss = (y2 - x4) s, Dds = (y3 - x5) s with y = a x + b and s the scale
 // applied to A (D3: 3n doubles, [Dn; Dss; Dds]; y: 5n doubles; part: gemv_part_bytes(5n, 5n)).
