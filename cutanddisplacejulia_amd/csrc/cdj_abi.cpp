@@ -1060,7 +1060,21 @@ struct BatchPlan {
   int64_t nitems = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // solve phase: worker streams (default priority) that factor and refine different meshes concurrently, the
+  // events that order them with the context's stream, and buffers for this batch shape
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> done;
+  hipEvent_t ready = nullptr;
+  std::vector<int64_t> a32_off, v_off, q_off;  // per mesh: FP32 copy of A, pivots / permutation, column norms
+  int64_t a32_total = 0, v_total = 0, q_total = 0, n3_max = 0;
+  DBuf A32, ipiv, perm, info, W, R, nrm, lu_ws, part, jobs;
+  size_t lu_ws_bytes = 0, part_bytes = 0;
+  bool bufs_mixed = false, bufs_fp64 = false;
   ~BatchPlan() {
+    for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
+    for (hipEvent_t e : done) (void)hipEventDestroy(e);
+    if (ready) (void)hipEventDestroy(ready);
+    for (hipStream_t s : streams) (void)hipStreamDestroy(s);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
   }
@@ -1218,6 +1232,334 @@ int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bo
   return CDJ_OK;
 }
 
+// ---- solve phase of a batch ---------------------------------------------------------------------------------
+// The meshes are independent systems of uneven size.  Each one's LU is a chain of latency-bound panels, so the
+// meshes are spread over a few worker streams (largest first, each to the stream with the least factorization work
+// so far) and their factorizations and refinement steps overlap on the device; the host synchronises once per
+// refinement round for all meshes together.
+constexpr int BATCH_SOLVE_STREAMS = 4;   // the runtime's default number of hardware queues per process
+constexpr int BATCH_IR_ROUNDS = 10;      // refinement rounds before a mesh moves to FGMRES-IR
+
+int batch_solve_setup(cdj_ctx* ctx, BatchPlan& p, bool mixed) {
+  const int64_t nmesh = (int64_t)p.n.size();
+  if (p.streams.empty()) {
+    const int S = (int)std::min<int64_t>(BATCH_SOLVE_STREAMS, std::max<int64_t>(nmesh, 1));
+    for (int s = 0; s < S; ++s) {
+      hipStream_t h = nullptr;
+      HIPCHK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+      p.streams.push_back(h);
+      hipEvent_t e = nullptr;
+      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      p.done.push_back(e);
+    }
+    HIPCHK(hipEventCreateWithFlags(&p.ready, hipEventDisableTiming));
+    p.a32_off.assign(nmesh, 0);
+    p.v_off.assign(nmesh, 0);
+    p.q_off.assign(nmesh, 0);
+    for (int64_t m = 0; m < nmesh; ++m) {
+      const int64_t N3 = 3 * p.n[m];
+      p.a32_off[m] = p.a32_total;
+      p.v_off[m] = p.v_total;
+      p.q_off[m] = p.q_total;
+      p.a32_total += N3 * N3;
+      p.v_total += N3;
+      p.q_total += p.nrhs[m];
+      p.n3_max = std::max(p.n3_max, N3);
+    }
+  }
+  const size_t S = p.streams.size();
+  if (!p.bufs_fp64) {
+    size_t ws = 0;
+    for (int64_t m = 0; m < nmesh; ++m) ws = std::max(ws, lu_workspace_bytes(3 * p.n[m]));
+    p.lu_ws_bytes = (ws + 255) / 256 * 256;
+    HIPCHK(p.lu_ws.alloc(p.lu_ws_bytes * S));
+    HIPCHK(p.ipiv.alloc(sizeof(int32_t) * p.v_total));
+    HIPCHK(p.info.alloc(sizeof(int) * nmesh));
+    p.bufs_fp64 = true;
+  }
+  if (mixed && !p.bufs_mixed) {
+    size_t pb = 0;
+    for (int64_t m = 0; m < nmesh; ++m) pb = std::max(pb, gemv_part_bytes(3 * p.n[m], 3 * p.n[m]));
+    p.part_bytes = (pb + 255) / 256 * 256;
+    HIPCHK(p.part.alloc(p.part_bytes));
+    HIPCHK(p.A32.alloc(sizeof(float) * p.a32_total));
+    HIPCHK(p.perm.alloc(sizeof(int32_t) * p.v_total));
+    HIPCHK(p.W.alloc(sizeof(float) * p.b_total));
+    HIPCHK(p.R.alloc(sizeof(double) * p.b_total));
+    HIPCHK(p.nrm.alloc(sizeof(double) * 2 * p.q_total));
+    int64_t nj = 0;
+    for (int64_t m = 0; m < nmesh; ++m) nj += (p.nrhs[m] + APPLY_NR - 1) / APPLY_NR;
+    HIPCHK(p.jobs.alloc(sizeof(LuApplyJob) * nj));
+    p.bufs_mixed = true;
+  }
+  return CDJ_OK;
+}
+
+// Meshes in decreasing size, each with the worker stream it runs on.
+void batch_solve_order(const BatchPlan& p, std::vector<int64_t>& order, std::vector<int>& sid) {
+  const int64_t nmesh = (int64_t)p.n.size();
+  order.resize(nmesh);
+  for (int64_t m = 0; m < nmesh; ++m) order[m] = m;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return p.n[a] > p.n[b]; });
+  std::vector<double> load(p.streams.size(), 0.0);
+  sid.assign(nmesh, 0);
+  for (int64_t m : order) {
+    const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    sid[m] = s;
+    const double n3 = 3.0 * (double)p.n[m];
+    load[s] += n3 * n3 * n3;
+  }
+}
+
+// worker streams wait for the work queued so far on st / st waits for everything queued on the worker streams
+hipError_t batch_fork(BatchPlan& p, hipStream_t st) {
+  hipError_t e = hipEventRecord(p.ready, st);
+  for (size_t s = 0; e == hipSuccess && s < p.streams.size(); ++s) e = hipStreamWaitEvent(p.streams[s], p.ready, 0);
+  return e;
+}
+
+hipError_t batch_join(BatchPlan& p, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  for (size_t s = 0; e == hipSuccess && s < p.streams.size(); ++s) {
+    e = hipEventRecord(p.done[s], p.streams[s]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st, p.done[s], 0);
+  }
+  return e;
+}
+
+int batch_lu_status(cdj_ctx* ctx, const std::vector<int>& info, bool singular_is_error) {
+  for (size_t m = 0; m < info.size(); ++m) {
+    if (info[m] < 0) return fail(ctx, CDJ_ERR_DEVICE, "LU panel barrier wait timed out");
+    if (info[m] > 0 && singular_is_error)
+      return fail(ctx, CDJ_ERR_SINGULAR, "matrix is exactly singular (mesh " + std::to_string(m) + ", zero pivot " +
+                                             std::to_string(info[m]) + ")");
+  }
+  return CDJ_OK;
+}
+
+// FP64: [A | B] of every mesh factored in place (B carried along as extra columns), then U X = L^-1 P B.
+int batch_solve_fp64(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_t nmesh, cdj_report* reports,
+                     double* solve_ms, hipStream_t st) {
+  if (int rc = batch_solve_setup(ctx, p, false)) return rc;
+  std::vector<int64_t> order;
+  std::vector<int> sid;
+  batch_solve_order(p, order, sid);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  HIPCHK(batch_fork(p, st));
+  for (int64_t m : order) {
+    const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
+    hipStream_t hs = p.streams[sid[m]];
+    double* Am = p.A.as<double>() + p.a_off[m];
+    HIPCHK(hipMemcpyAsync(Am + N3 * N3, p.B.as<double>() + p.b_off[m], sizeof(double) * N3 * nr,
+                          hipMemcpyDeviceToDevice, hs));
+    HIPCHK(lu_factor_cols<double>(Am, N3, N3 + nr, N3, p.ipiv.as<int32_t>() + p.v_off[m], p.info.as<int>() + m,
+                                  p.lu_ws.as<char>() + p.lu_ws_bytes * sid[m], hs));
+    HIPCHK(tri_solve_upper<double>(Am, N3, N3, Am + N3 * N3, nr, N3, hs));
+    HIPCHK(hipMemcpyAsync(p.X.as<double>() + p.b_off[m], Am + N3 * N3, sizeof(double) * N3 * nr,
+                          hipMemcpyDeviceToDevice, hs));
+  }
+  HIPCHK(batch_join(p, st));
+  HIPCHK(hipEventRecord(e1, st));
+  std::vector<int> info(nmesh);
+  HIPCHK(hipMemcpyAsync(info.data(), p.info.p, sizeof(int) * nmesh, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (int rc = batch_lu_status(ctx, info, true)) return rc;
+  *solve_ms = ms;
+  if (reports)
+    for (int64_t m = 0; m < nmesh; ++m) {
+      reports[m].path = CDJ_PREC_FP64;
+      reports[m].iterations = 0;
+      reports[m].residual = -1.0;
+      reports[m].solve_ms = ms;
+    }
+  return CDJ_OK;
+}
+
+// Mixed: FP32 copy of each A factored on MFMA, X = M^-1 B for all of a mesh's right-hand sides at once, then
+// rounds of X += M^-1 (B - A X) with the FP64 residual, until every column reaches the tolerance.  A mesh whose
+// residual stops falling (less than halved by a round) or whose FP32 factor has a zero pivot moves to FGMRES-IR
+// per column, and from there to the FP64 LU, as in the single-mesh solve.
+int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_t nmesh, const cdj_opts* opts,
+                      cdj_report* reports, double* solve_ms, double* worst, int* iters, int* path, hipStream_t st) {
+  if (int rc = batch_solve_setup(ctx, p, true)) return rc;
+  GmresOpts go;
+  if (opts && opts->tol > 0) go.tol = opts->tol;
+  if (opts && opts->max_iter > 0) go.max_iter = opts->max_iter;
+  const bool fp16 = opts && opts->precision == CDJ_PREC_MIXED16;
+  std::vector<int64_t> order;
+  std::vector<int> sid;
+  batch_solve_order(p, order, sid);
+  float* A32 = p.A32.as<float>();
+  int32_t* ipiv = p.ipiv.as<int32_t>();
+  int32_t* perm = p.perm.as<int32_t>();
+  double* nrm = p.nrm.as<double>();  // ||R||^2 per column, then ||B||^2 per column
+  auto Am = [&](int64_t m) { return p.A.as<double>() + p.a_off[m]; };
+  auto Bm = [&](int64_t m) { return p.B.as<double>() + p.b_off[m]; };
+  auto Xm = [&](int64_t m) { return p.X.as<double>() + p.b_off[m]; };
+  auto Rm = [&](int64_t m) { return p.R.as<double>() + p.b_off[m]; };
+  auto Wm = [&](int64_t m) { return p.W.as<float>() + p.b_off[m]; };
+  auto part = [&](int s) { return reinterpret_cast<double*>(p.part.as<char>() + p.part_bytes * s); };
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, st));
+  HIPCHK(batch_fork(p, st));
+  for (int64_t m : order) {
+    const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
+    hipStream_t hs = p.streams[sid[m]];
+    convert_matrix_d2f(Am(m), N3, A32 + p.a32_off[m], N3, N3, N3, hs);
+    HIPCHK(lu_factor<float>(A32 + p.a32_off[m], N3, N3, ipiv + p.v_off[m], p.info.as<int>() + m,
+                            p.lu_ws.as<char>() + p.lu_ws_bytes * sid[m], hs, fp16));
+    colnorm2(Bm(m), N3, N3, nr, nrm + p.q_total + p.q_off[m], hs);
+  }
+  HIPCHK(batch_join(p, st));
+  std::vector<int> info(nmesh);
+  std::vector<int32_t> hp(p.v_total);
+  HIPCHK(hipMemcpyAsync(info.data(), p.info.p, sizeof(int) * nmesh, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hp.data(), ipiv, sizeof(int32_t) * p.v_total, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (int rc = batch_lu_status(ctx, info, false)) return rc;
+  // row permutations of the pivot sequences (row i of P A is row perm[i] of A)
+  for (int64_t m = 0; m < nmesh; ++m) {
+    const int64_t N3 = 3 * meshes[m].n;
+    int32_t* v = hp.data() + p.v_off[m];
+    std::vector<int32_t> q(N3);
+    for (int64_t i = 0; i < N3; ++i) q[i] = (int32_t)i;
+    for (int64_t i = 0; i < N3; ++i) std::swap(q[i], q[v[i]]);
+    std::copy(q.begin(), q.end(), v);
+  }
+  HIPCHK(hipMemcpyAsync(perm, hp.data(), sizeof(int32_t) * p.v_total, hipMemcpyHostToDevice, st));
+  // 0: refining, 1: converged, 2: to the fallback
+  std::vector<char> state(nmesh, 0);
+  std::vector<int> rounds(nmesh, 0);
+  std::vector<double> res(nmesh, 0.0), prev(nmesh, HUGE_VAL);
+  for (int64_t m = 0; m < nmesh; ++m)
+    if (info[m] != 0) state[m] = 2;
+  // the factor applications of a round: one workgroup per mesh and group of at most APPLY_NR columns, all meshes
+  // in one launch; then the FP64 residuals and their column norms
+  std::vector<LuApplyJob> hj;
+  auto add_jobs = [&](int64_t m, const double* Rsrc, bool accumulate) {
+    const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
+    for (int64_t q0 = 0; q0 < nr; q0 += APPLY_NR) {
+      LuApplyJob j;
+      j.LU = A32 + p.a32_off[m];
+      j.perm = perm + p.v_off[m];
+      j.R = Rsrc + q0 * N3;
+      j.X = Xm(m) + q0 * N3;
+      j.W = Wm(m) + q0 * N3;
+      j.n = N3;
+      j.lda = N3;
+      j.nrhs = (int32_t)std::min<int64_t>(APPLY_NR, nr - q0);
+      j.accumulate = accumulate ? 1 : 0;
+      hj.push_back(j);
+    }
+  };
+  auto run_round = [&](const std::vector<int64_t>& ms) -> hipError_t {
+    hipError_t e = hipMemcpyAsync(p.jobs.p, hj.data(), sizeof(LuApplyJob) * hj.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = lu_apply_batched(p.jobs.as<LuApplyJob>(), (int)hj.size(), st);
+    for (size_t k = 0; e == hipSuccess && k < ms.size(); ++k) {
+      const int64_t m = ms[k], N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
+      e = residual_multi(Am(m), N3, N3, Bm(m), N3, Xm(m), N3, Rm(m), N3, nr, part(0), nrm + p.q_off[m], st);
+    }
+    return e;
+  };
+  std::vector<int64_t> live;
+  for (int64_t m : order)
+    if (!state[m]) {
+      add_jobs(m, Bm(m), false);
+      live.push_back(m);
+    }
+  HIPCHK(run_round(live));
+  std::vector<double> hn(2 * p.q_total);
+  for (int round = 0;; ++round) {
+    HIPCHK(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * p.q_total, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    hj.clear();
+    live.clear();
+    for (int64_t m : order) {
+      if (state[m]) continue;
+      const int64_t nr = meshes[m].nrhs;
+      double rel = 0.0;
+      for (int64_t q = 0; q < nr; ++q) {
+        const double rr = hn[p.q_off[m] + q], bb = hn[p.q_total + p.q_off[m] + q];
+        const double r = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+        rel = std::isnan(r) || std::isnan(rel) ? NAN : std::max(rel, r);
+      }
+      res[m] = rel;
+      if (rel <= go.tol) {
+        state[m] = 1;
+        continue;
+      }
+      if (round >= BATCH_IR_ROUNDS || !(rel < 0.5 * prev[m])) {
+        state[m] = 2;
+        continue;
+      }
+      prev[m] = rel;
+      rounds[m]++;
+      add_jobs(m, Rm(m), true);
+      live.push_back(m);
+    }
+    if (live.empty()) break;
+    HIPCHK(run_round(live));
+  }
+  // fallbacks, one mesh at a time on the context's stream
+  std::vector<int> mpath(nmesh, fp16 ? CDJ_PREC_MIXED16 : CDJ_PREC_MIXED32);
+  std::vector<int> fit(nmesh, 0);
+  for (int64_t m = 0; m < nmesh; ++m) {
+    if (state[m] != 2) continue;
+    const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
+    bool ok = info[m] == 0;
+    double w = 0.0;
+    for (int64_t q = 0; ok && q < nr; ++q) {
+      GmresResult gr;
+      HIPCHK(fgmres_ir<float>(Am(m), N3, N3, A32 + p.a32_off[m], N3, perm + p.v_off[m], Bm(m) + q * N3,
+                              Xm(m) + q * N3, go, gr, st));
+      ok = gr.converged && !gr.trsv_timeout;
+      w = std::max(w, gr.residual);
+      fit[m] += gr.iterations;
+    }
+    if (ok) {
+      res[m] = w;
+      continue;
+    }
+    DBuf AB;
+    HIPCHK(AB.alloc(sizeof(double) * N3 * (N3 + nr)));
+    HIPCHK(hipMemcpyAsync(AB.p, Am(m), sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(AB.as<double>() + N3 * N3, Bm(m), sizeof(double) * N3 * nr, hipMemcpyDeviceToDevice, st));
+    if (int rc = solve_augmented_fp64(ctx, AB.as<double>(), N3, N3, nr, nullptr, nullptr, nullptr, st)) return rc;
+    HIPCHK(hipMemcpyAsync(Xm(m), AB.as<double>() + N3 * N3, sizeof(double) * N3 * nr, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    mpath[m] = CDJ_PREC_FP64;
+    res[m] = -1.0;
+  }
+  HIPCHK(hipEventRecord(e1, st));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *solve_ms = ms;
+  for (int64_t m = 0; m < nmesh; ++m) {
+    *worst = std::max(*worst, res[m]);
+    *iters += rounds[m] + fit[m];
+    *path = std::max(*path, mpath[m]);
+    if (reports) {
+      reports[m].path = mpath[m];
+      reports[m].iterations = rounds[m] + fit[m];
+      reports[m].residual = res[m];
+      reports[m].solve_ms = ms;
+    }
+  }
+  return CDJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1284,34 +1626,25 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->near_entries = (int64_t)cnt[2];
-  // solves, mesh by mesh, on the same stream
+  // right-hand sides to the device, then every mesh's solve (worker streams, meshes concurrently)
+  for (int64_t m = 0; m < nmesh; ++m)
+    HIPCHK(hipMemcpyAsync(p.B.as<double>() + p.b_off[m], meshes[m].b, sizeof(double) * 3 * meshes[m].n * meshes[m].nrhs,
+                          hipMemcpyHostToDevice, st));
   double solve_ms = 0.0, worst = 0.0;
   int iters = 0, path = CDJ_PREC_FP64;
+  if (opts && (opts->precision == CDJ_PREC_MIXED32 || opts->precision == CDJ_PREC_MIXED16)) {
+    if (int rc = batch_solve_mixed(ctx, p, meshes, nmesh, opts, reports, &solve_ms, &worst, &iters, &path, st))
+      return rc;
+  } else {
+    if (int rc = batch_solve_fp64(ctx, p, meshes, nmesh, reports, &solve_ms, st)) return rc;
+  }
   for (int64_t m = 0; m < nmesh; ++m) {
-    const cdj_mesh& M = meshes[m];
-    const int64_t N3 = 3 * M.n;
-    double* Am = p.A.as<double>() + p.a_off[m];
-    double* Bm = p.B.as<double>() + p.b_off[m];
-    double* Xm = p.X.as<double>() + p.b_off[m];
-    HIPCHK(hipMemcpyAsync(Bm, M.b, sizeof(double) * N3 * M.nrhs, hipMemcpyHostToDevice, st));
-    cdj_report r{};
-    int rc;
-    if (opts && (opts->precision == CDJ_PREC_MIXED32 || opts->precision == CDJ_PREC_MIXED16)) {
-      rc = solve_mixed32(ctx, Am, N3, N3, Bm, Xm, M.nrhs, opts, &r, st);
-    } else {
-      // [A | B] in place: the B columns follow A in the plan's buffer (leading dimension 3n)
-      HIPCHK(hipMemcpyAsync(Am + N3 * N3, Bm, sizeof(double) * N3 * M.nrhs, hipMemcpyDeviceToDevice, st));
-      rc = solve_augmented_fp64(ctx, Am, N3, N3, M.nrhs, nullptr, nullptr, &r, st);
-      if (!rc) HIPCHK(hipMemcpyAsync(Xm, Am + N3 * N3, sizeof(double) * N3 * M.nrhs, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(meshes[m].x, p.X.as<double>() + p.b_off[m], sizeof(double) * 3 * meshes[m].n * meshes[m].nrhs,
+                          hipMemcpyDeviceToHost, st));
+    if (reports) {
+      reports[m].assembly_ms = m == 0 ? ams : 0.0;
+      reports[m].near_entries = 0;
     }
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(M.x, Xm, sizeof(double) * N3 * M.nrhs, hipMemcpyDeviceToHost, st));
-    r.assembly_ms = m == 0 ? ams : 0.0;
-    if (reports) reports[m] = r;
-    solve_ms += r.solve_ms;
-    worst = std::max(worst, r.residual);
-    iters += r.iterations;
-    path = std::max(path, (int)r.path);
   }
   HIPCHK(hipStreamSynchronize(st));
   if (total) {

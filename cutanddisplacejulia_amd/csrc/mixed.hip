@@ -231,6 +231,143 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int nvec, i
   out[j] = s;
 }
 
+// out[i + q ldo] = (TO) in[perm[i] + q ldi] for q < nrhs (perm == nullptr: identity)
+template <class TI, class TO>
+__global__ void permute_cast_multi_kernel(const int32_t* __restrict__ perm, const TI* __restrict__ in, int64_t ldi,
+                                          TO* __restrict__ out, int64_t ldo, int64_t n, int64_t nrhs) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * nrhs) return;
+  const int64_t q = t / n, i = t - q * n;
+  out[i + q * ldo] = (TO)in[(perm ? (int64_t)perm[i] : i) + q * ldi];
+}
+
+// x[i + q ldx] (+)= w[i + q ldw]
+template <class TW>
+__global__ void add_cast_multi_kernel(const TW* __restrict__ w, int64_t ldw, double* __restrict__ x, int64_t ldx,
+                                      int64_t n, int64_t nrhs, int accumulate) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n * nrhs) return;
+  const int64_t q = t / n, i = t - q * n;
+  const double v = (double)w[i + q * ldw];
+  x[i + q * ldx] = accumulate ? x[i + q * ldx] + v : v;
+}
+
+// out[q] = sum_i R[i + q ld]^2, one workgroup per column, fixed summation order (bitwise reproducible)
+__global__ void __launch_bounds__(256) colnorm2_kernel(const double* __restrict__ R, int64_t ld, int64_t n,
+                                                       double* __restrict__ out) {
+  __shared__ double s[256];
+  const double* r = R + (int64_t)blockIdx.x * ld;
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) v += r[i] * r[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+// ---- batched factor application: one workgroup per system -------------------------------------------------
+// X (+)= U^-1 L^-1 P R for up to APPLY_NR columns of one system per workgroup.  The workgroup walks the
+// 64-row blocks itself: the diagonal block is staged in LDS and solved by one wave per column, then all 1024
+// threads update the remaining rows with the block's columns of the factor (coalesced column reads).  No other
+// workgroup is involved, so there are no flags or waits, and many systems of uneven size share one launch.
+constexpr int APPLY_NB = 64;
+constexpr int APPLY_THREADS = 1024;
+
+__global__ void __launch_bounds__(APPLY_THREADS) lu_apply_batched_kernel(const LuApplyJob* __restrict__ jobs) {
+  __shared__ float D[APPLY_NB][APPLY_NB + 1];
+  __shared__ float xs[APPLY_NR][APPLY_NB];
+  const LuApplyJob J = jobs[blockIdx.x];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t n = J.n, lda = J.lda;
+  const int nr = J.nrhs;
+  const float* __restrict__ LU = J.LU;
+  float* W = J.W;
+  for (int64_t t = tid; t < n * nr; t += APPLY_THREADS) {
+    const int64_t q = t / n, i = t - q * n;
+    W[i + q * n] = (float)J.R[(int64_t)J.perm[i] + q * n];
+  }
+  __syncthreads();
+  // forward: L unit lower
+  for (int64_t k0 = 0; k0 < n; k0 += APPLY_NB) {
+    const int nb = (int)(n - k0 < APPLY_NB ? n - k0 : APPLY_NB);
+    for (int idx = tid; idx < nb * nb; idx += APPLY_THREADS) {
+      const int r = idx % nb, c = idx / nb;
+      D[r][c] = LU[(k0 + r) + (k0 + c) * lda];
+    }
+    __syncthreads();
+    if (wave < nr) {
+      float xi = lane < nb ? W[k0 + lane + wave * n] : 0.f;
+      for (int r = 0; r < nb - 1; ++r) {
+        const float xr = __shfl(xi, r);
+        if (lane > r && lane < nb) xi -= D[lane][r] * xr;
+      }
+      if (lane < nb) {
+        xs[wave][lane] = xi;
+        W[k0 + lane + wave * n] = xi;
+      }
+    }
+    __syncthreads();
+    for (int64_t i = k0 + nb + tid; i < n; i += APPLY_THREADS) {
+      float acc[APPLY_NR];
+#pragma unroll
+      for (int q = 0; q < APPLY_NR; ++q) acc[q] = 0.f;
+      const float* col = LU + i + k0 * lda;
+#pragma unroll 16
+      for (int c = 0; c < nb; ++c) {
+        const float l = col[c * lda];
+#pragma unroll
+        for (int q = 0; q < APPLY_NR; ++q) acc[q] += l * xs[q][c];
+      }
+      for (int q = 0; q < nr; ++q) W[i + q * n] -= acc[q];
+    }
+    __syncthreads();
+  }
+  // backward: U with its diagonal
+  for (int64_t k0 = ((n - 1) / APPLY_NB) * APPLY_NB; k0 >= 0; k0 -= APPLY_NB) {
+    const int nb = (int)(n - k0 < APPLY_NB ? n - k0 : APPLY_NB);
+    for (int idx = tid; idx < nb * nb; idx += APPLY_THREADS) {
+      const int r = idx % nb, c = idx / nb;
+      D[r][c] = LU[(k0 + r) + (k0 + c) * lda];
+    }
+    __syncthreads();
+    if (wave < nr) {
+      float xi = lane < nb ? W[k0 + lane + wave * n] : 0.f;
+      for (int r = nb - 1; r >= 0; --r) {
+        if (lane == r) xi = xi / D[r][r];
+        const float xr = __shfl(xi, r);
+        if (lane < r) xi -= D[lane][r] * xr;
+      }
+      if (lane < nb) {
+        xs[wave][lane] = xi;
+        W[k0 + lane + wave * n] = xi;
+      }
+    }
+    __syncthreads();
+    for (int64_t i = tid; i < k0; i += APPLY_THREADS) {
+      float acc[APPLY_NR];
+#pragma unroll
+      for (int q = 0; q < APPLY_NR; ++q) acc[q] = 0.f;
+      const float* col = LU + i + k0 * lda;
+#pragma unroll 16
+      for (int c = 0; c < nb; ++c) {
+        const float u = col[c * lda];
+#pragma unroll
+        for (int q = 0; q < APPLY_NR; ++q) acc[q] += u * xs[q][c];
+      }
+      for (int q = 0; q < nr; ++q) W[i + q * n] -= acc[q];
+    }
+    __syncthreads();
+  }
+  for (int64_t t = tid; t < n * nr; t += APPLY_THREADS) {
+    const int64_t q = t / n, i = t - q * n;
+    const double v = (double)W[i + q * n];
+    J.X[i + q * n] = J.accumulate ? J.X[i + q * n] + v : v;
+  }
+}
+
 // v = w / ||w|| with ||w||^2 read from device memory (zero vector when the norm is zero)
 __global__ void scale_by_norm_kernel(const double* __restrict__ w, const double* __restrict__ nrm2,
                                      double* __restrict__ v, int64_t n) {
@@ -292,6 +429,42 @@ hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* 
   e = trsv_persistent<T>(LU, n, lda, work, false, s, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((cast_kernel<T, double>), dim3(nblk(n, 256)), dim3(256), 0, st, (const T*)work, z, n);
+  return hipGetLastError();
+}
+
+// ---- many right-hand sides ----------------------------------------------------------------------------------
+template <class T>
+hipError_t lu_apply_multi(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
+                          double* X, int64_t ldx, int64_t nrhs, bool accumulate, T* W, hipStream_t st) {
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  const unsigned g = nblk(n * nrhs, 256);
+  hipLaunchKernelGGL((permute_cast_multi_kernel<double, T>), dim3(g), dim3(256), 0, st, perm, R, ldr, W, n, n, nrhs);
+  hipError_t e = tri_solve_lower_unit<T>(LU, n, lda, W, nrhs, n, st);
+  if (e != hipSuccess) return e;
+  e = tri_solve_upper<T>(LU, n, lda, W, nrhs, n, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(add_cast_multi_kernel<T>, dim3(g), dim3(256), 0, st, (const T*)W, n, X, ldx, n, nrhs,
+                     accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t lu_apply_batched(const LuApplyJob* jobs, int njobs, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(lu_apply_batched_kernel, dim3((unsigned)njobs), dim3(APPLY_THREADS), 0, st, jobs);
+  return hipGetLastError();
+}
+
+void colnorm2(const double* R, int64_t ld, int64_t n, int64_t nrhs, double* out, hipStream_t st) {
+  if (nrhs <= 0) return;
+  hipLaunchKernelGGL(colnorm2_kernel, dim3((unsigned)nrhs), dim3(256), 0, st, R, ld, n, out);
+}
+
+hipError_t residual_multi(const double* A, int64_t n, int64_t lda, const double* B, int64_t ldb, const double* X,
+                          int64_t ldx, double* R, int64_t ldr, int64_t nrhs, double* part, double* nrm2,
+                          hipStream_t st) {
+  for (int64_t q = 0; q < nrhs; ++q)
+    gemv_splitk<double>(A, n, n, lda, X + q * ldx, R + q * ldr, -1.0, B + q * ldb, 1.0, part, st);
+  colnorm2(R, ldr, n, nrhs, nrm2, st);
   return hipGetLastError();
 }
 
@@ -502,6 +675,10 @@ template hipError_t apply_lu_precond<float>(const float*, int64_t, int64_t, cons
                                             float*, TrsvState&, hipStream_t);
 template hipError_t apply_lu_precond<double>(const double*, int64_t, int64_t, const int32_t*, const double*, double*,
                                              double*, TrsvState&, hipStream_t);
+template hipError_t lu_apply_multi<float>(const float*, int64_t, int64_t, const int32_t*, const double*, int64_t,
+                                          double*, int64_t, int64_t, bool, float*, hipStream_t);
+template hipError_t lu_apply_multi<double>(const double*, int64_t, int64_t, const int32_t*, const double*, int64_t,
+                                           double*, int64_t, int64_t, bool, double*, hipStream_t);
 template hipError_t fgmres_ir<float>(const double*, int64_t, int64_t, const float*, int64_t, const int32_t*,
                                      const double*, double*, const GmresOpts&, GmresResult&, hipStream_t);
 template hipError_t fgmres_ir<double>(const double*, int64_t, int64_t, const double*, int64_t, const int32_t*,
