@@ -41,7 +41,8 @@ CDJ_ASM_MIXED = 1
 # every symbol declared in include/cdj.h
 EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", "cdj_td_influence", "cdj_td_field",
            "cdj_traction_influence", "cdj_slip_calculator", "cdj_solve", "cdj_assemble_dev",
-           "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_solve_dev", "cdj_mlp_create", "cdj_mlp_destroy",
+           "cdj_td_influence_dev", "cdj_lu_factor_dev", "cdj_lu_factor_batched_dev", "cdj_lu_solve_dev",
+           "cdj_mlp_create", "cdj_mlp_destroy",
            "cdj_mlp_nparams", "cdj_mlp_get_params", "cdj_mlp_set_params", "cdj_mlp_train_dev", "cdj_mlp_loss_grad",
            "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning",
            "cdj_traction_rows", "cdj_lstsq", "cdj_crack_tip_elements", "cdj_stress_intensity",
@@ -113,6 +114,7 @@ def load(path=LIB_PATH):
     L.cdj_solve.argtypes = [_p, _p, _p, _i64, _i64, _p, _p, _p]
     L.cdj_assemble_dev.argtypes = [_p, _p, _p, _p, _p, _p, _i64, _d, _d, _d, _i32, _p, _i64, _p, _p]
     L.cdj_lu_factor_dev.argtypes = [_p, _p, _i64, _i64, _p, _p]
+    L.cdj_lu_factor_batched_dev.argtypes = [_p, _p, _p, _p, _i64, _p, _p, _p]
     L.cdj_lu_solve_dev.argtypes = [_p, _p, _i64, _i64, _p, _p, _i64, _i64, _p]
     for name in EXPORTS:
         f = getattr(L, name)

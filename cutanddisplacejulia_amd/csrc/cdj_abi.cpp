@@ -525,6 +525,45 @@ int cdj_lu_factor_dev(cdj_ctx* ctx, double* A, int64_t n, int64_t lda, int32_t* 
   return CDJ_OK;
 }
 
+int cdj_lu_factor_batched_dev(cdj_ctx* ctx, double* const* A, const int64_t* n, const int64_t* lda, int64_t count,
+                              int32_t* const* ipiv, int32_t* info, void* stream) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (count < 0 || (count > 0 && (!A || !n || !lda || !ipiv || !info))) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+  for (int64_t j = 0; j < count; ++j)
+    if (n[j] < 0 || lda[j] < n[j] || (n[j] > 0 && (!A[j] || !ipiv[j])))
+      return fail(ctx, CDJ_ERR_DIMS, "invalid matrix dimensions");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  std::vector<int64_t> idx;
+  size_t ws_total = 0;
+  std::vector<size_t> off(count, 0);
+  for (int64_t j = 0; j < count; ++j) {
+    info[j] = 0;
+    if (n[j] == 0) continue;
+    idx.push_back(j);
+    off[j] = ws_total;
+    ws_total += (lu_workspace_bytes(n[j]) + 255) / 256 * 256;
+  }
+  if (idx.empty()) return CDJ_OK;
+  DBuf ws, dinfo;
+  HIPCHK(ws.alloc(ws_total));
+  HIPCHK(dinfo.alloc(sizeof(int) * idx.size()));
+  std::vector<LuJob<double>> jobs;
+  for (size_t t = 0; t < idx.size(); ++t) {
+    const int64_t j = idx[t];
+    jobs.push_back(LuJob<double>{A[j], n[j], n[j], lda[j], ipiv[j], dinfo.as<int>() + t, ws.as<char>() + off[j]});
+  }
+  HIPCHK(lu_factor_batched<double>(jobs.data(), (int)jobs.size(), st));
+  std::vector<int> h(idx.size());
+  HIPCHK(hipMemcpyAsync(h.data(), dinfo.p, sizeof(int) * h.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (size_t t = 0; t < idx.size(); ++t) {
+    if (h[t] < 0) return fail(ctx, CDJ_ERR_DEVICE, "LU panel barrier wait timed out");
+    info[idx[t]] = h[t];
+  }
+  return CDJ_OK;
+}
+
 int cdj_lu_solve_dev(cdj_ctx* ctx, const double* LU, int64_t n, int64_t lda, const int32_t* ipiv, double* B,
                      int64_t nrhs, int64_t ldb, void* stream) {
   if (!ctx) return CDJ_ERR_ARG;
@@ -1060,21 +1099,14 @@ struct BatchPlan {
   int64_t nitems = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  // solve phase: worker streams (default priority) that factor and refine different meshes concurrently, the
-  // events that order them with the context's stream, and buffers for this batch shape
-  std::vector<hipStream_t> streams;
-  std::vector<hipEvent_t> done;
-  hipEvent_t ready = nullptr;
+  // solve phase buffers for this batch shape
+  bool solve_layout = false, bufs_mixed = false, bufs_fp64 = false;
   std::vector<int64_t> a32_off, v_off, q_off;  // per mesh: FP32 copy of A, pivots / permutation, column norms
-  int64_t a32_total = 0, v_total = 0, q_total = 0, n3_max = 0;
+  std::vector<size_t> ws_off;                  // per mesh: LU workspace
+  int64_t a32_total = 0, v_total = 0, q_total = 0;
+  size_t ws_total = 0, part_bytes = 0;
   DBuf A32, ipiv, perm, info, W, R, nrm, lu_ws, part, jobs;
-  size_t lu_ws_bytes = 0, part_bytes = 0;
-  bool bufs_mixed = false, bufs_fp64 = false;
   ~BatchPlan() {
-    for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
-    for (hipEvent_t e : done) (void)hipEventDestroy(e);
-    if (ready) (void)hipEventDestroy(ready);
-    for (hipStream_t s : streams) (void)hipStreamDestroy(s);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
   }
@@ -1233,46 +1265,33 @@ int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bo
 }
 
 // ---- solve phase of a batch ---------------------------------------------------------------------------------
-// The meshes are independent systems of uneven size.  Each one's LU is a chain of latency-bound panels, so the
-// meshes are spread over a few worker streams (largest first, each to the stream with the least factorization work
-// so far) and their factorizations and refinement steps overlap on the device; the host synchronises once per
-// refinement round for all meshes together.
-constexpr int BATCH_SOLVE_STREAMS = 4;   // the runtime's default number of hardware queues per process
-constexpr int BATCH_IR_ROUNDS = 10;      // refinement rounds before a mesh moves to FGMRES-IR
+// The meshes are independent systems of uneven size.  Their LUs are stepped together by panel column, so that one
+// cooperative launch factors the panels of many meshes (lu_factor_batched); the refinement applies every mesh's
+// factors in one launch per round, with one host synchronisation per round for all meshes.
+constexpr int BATCH_IR_ROUNDS = 10;  // refinement rounds before a mesh moves to FGMRES-IR
 
 int batch_solve_setup(cdj_ctx* ctx, BatchPlan& p, bool mixed) {
   const int64_t nmesh = (int64_t)p.n.size();
-  if (p.streams.empty()) {
-    const int S = (int)std::min<int64_t>(BATCH_SOLVE_STREAMS, std::max<int64_t>(nmesh, 1));
-    for (int s = 0; s < S; ++s) {
-      hipStream_t h = nullptr;
-      HIPCHK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
-      p.streams.push_back(h);
-      hipEvent_t e = nullptr;
-      HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-      p.done.push_back(e);
-    }
-    HIPCHK(hipEventCreateWithFlags(&p.ready, hipEventDisableTiming));
+  if (!p.solve_layout) {
     p.a32_off.assign(nmesh, 0);
     p.v_off.assign(nmesh, 0);
     p.q_off.assign(nmesh, 0);
+    p.ws_off.assign(nmesh, 0);
     for (int64_t m = 0; m < nmesh; ++m) {
       const int64_t N3 = 3 * p.n[m];
       p.a32_off[m] = p.a32_total;
       p.v_off[m] = p.v_total;
       p.q_off[m] = p.q_total;
+      p.ws_off[m] = p.ws_total;
       p.a32_total += N3 * N3;
       p.v_total += N3;
       p.q_total += p.nrhs[m];
-      p.n3_max = std::max(p.n3_max, N3);
+      p.ws_total += (lu_workspace_bytes(N3) + 255) / 256 * 256;
     }
+    p.solve_layout = true;
   }
-  const size_t S = p.streams.size();
   if (!p.bufs_fp64) {
-    size_t ws = 0;
-    for (int64_t m = 0; m < nmesh; ++m) ws = std::max(ws, lu_workspace_bytes(3 * p.n[m]));
-    p.lu_ws_bytes = (ws + 255) / 256 * 256;
-    HIPCHK(p.lu_ws.alloc(p.lu_ws_bytes * S));
+    HIPCHK(p.lu_ws.alloc(p.ws_total));
     HIPCHK(p.ipiv.alloc(sizeof(int32_t) * p.v_total));
     HIPCHK(p.info.alloc(sizeof(int) * nmesh));
     p.bufs_fp64 = true;
@@ -1280,7 +1299,7 @@ int batch_solve_setup(cdj_ctx* ctx, BatchPlan& p, bool mixed) {
   if (mixed && !p.bufs_mixed) {
     size_t pb = 0;
     for (int64_t m = 0; m < nmesh; ++m) pb = std::max(pb, gemv_part_bytes(3 * p.n[m], 3 * p.n[m]));
-    p.part_bytes = (pb + 255) / 256 * 256;
+    p.part_bytes = pb;
     HIPCHK(p.part.alloc(p.part_bytes));
     HIPCHK(p.A32.alloc(sizeof(float) * p.a32_total));
     HIPCHK(p.perm.alloc(sizeof(int32_t) * p.v_total));
@@ -1295,36 +1314,12 @@ int batch_solve_setup(cdj_ctx* ctx, BatchPlan& p, bool mixed) {
   return CDJ_OK;
 }
 
-// Meshes in decreasing size, each with the worker stream it runs on.
-void batch_solve_order(const BatchPlan& p, std::vector<int64_t>& order, std::vector<int>& sid) {
-  const int64_t nmesh = (int64_t)p.n.size();
-  order.resize(nmesh);
-  for (int64_t m = 0; m < nmesh; ++m) order[m] = m;
+// Meshes in decreasing size (largest panels first in each launch).
+std::vector<int64_t> batch_solve_order(const BatchPlan& p) {
+  std::vector<int64_t> order(p.n.size());
+  for (size_t m = 0; m < order.size(); ++m) order[m] = (int64_t)m;
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return p.n[a] > p.n[b]; });
-  std::vector<double> load(p.streams.size(), 0.0);
-  sid.assign(nmesh, 0);
-  for (int64_t m : order) {
-    const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    sid[m] = s;
-    const double n3 = 3.0 * (double)p.n[m];
-    load[s] += n3 * n3 * n3;
-  }
-}
-
-// worker streams wait for the work queued so far on st / st waits for everything queued on the worker streams
-hipError_t batch_fork(BatchPlan& p, hipStream_t st) {
-  hipError_t e = hipEventRecord(p.ready, st);
-  for (size_t s = 0; e == hipSuccess && s < p.streams.size(); ++s) e = hipStreamWaitEvent(p.streams[s], p.ready, 0);
-  return e;
-}
-
-hipError_t batch_join(BatchPlan& p, hipStream_t st) {
-  hipError_t e = hipSuccess;
-  for (size_t s = 0; e == hipSuccess && s < p.streams.size(); ++s) {
-    e = hipEventRecord(p.done[s], p.streams[s]);
-    if (e == hipSuccess) e = hipStreamWaitEvent(st, p.done[s], 0);
-  }
-  return e;
+  return order;
 }
 
 int batch_lu_status(cdj_ctx* ctx, const std::vector<int>& info, bool singular_is_error) {
@@ -1341,27 +1336,28 @@ int batch_lu_status(cdj_ctx* ctx, const std::vector<int>& info, bool singular_is
 int batch_solve_fp64(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_t nmesh, cdj_report* reports,
                      double* solve_ms, hipStream_t st) {
   if (int rc = batch_solve_setup(ctx, p, false)) return rc;
-  std::vector<int64_t> order;
-  std::vector<int> sid;
-  batch_solve_order(p, order, sid);
+  const std::vector<int64_t> order = batch_solve_order(p);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
-  HIPCHK(batch_fork(p, st));
+  std::vector<LuJob<double>> jobs;
   for (int64_t m : order) {
     const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
-    hipStream_t hs = p.streams[sid[m]];
     double* Am = p.A.as<double>() + p.a_off[m];
     HIPCHK(hipMemcpyAsync(Am + N3 * N3, p.B.as<double>() + p.b_off[m], sizeof(double) * N3 * nr,
-                          hipMemcpyDeviceToDevice, hs));
-    HIPCHK(lu_factor_cols<double>(Am, N3, N3 + nr, N3, p.ipiv.as<int32_t>() + p.v_off[m], p.info.as<int>() + m,
-                                  p.lu_ws.as<char>() + p.lu_ws_bytes * sid[m], hs));
-    HIPCHK(tri_solve_upper<double>(Am, N3, N3, Am + N3 * N3, nr, N3, hs));
-    HIPCHK(hipMemcpyAsync(p.X.as<double>() + p.b_off[m], Am + N3 * N3, sizeof(double) * N3 * nr,
-                          hipMemcpyDeviceToDevice, hs));
+                          hipMemcpyDeviceToDevice, st));
+    jobs.push_back(LuJob<double>{Am, N3, N3 + nr, N3, p.ipiv.as<int32_t>() + p.v_off[m], p.info.as<int>() + m,
+                                 p.lu_ws.as<char>() + p.ws_off[m]});
   }
-  HIPCHK(batch_join(p, st));
+  HIPCHK(lu_factor_batched<double>(jobs.data(), (int)jobs.size(), st));
+  for (int64_t m : order) {
+    const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
+    double* Am = p.A.as<double>() + p.a_off[m];
+    HIPCHK(tri_solve_upper<double>(Am, N3, N3, Am + N3 * N3, nr, N3, st));
+    HIPCHK(hipMemcpyAsync(p.X.as<double>() + p.b_off[m], Am + N3 * N3, sizeof(double) * N3 * nr,
+                          hipMemcpyDeviceToDevice, st));
+  }
   HIPCHK(hipEventRecord(e1, st));
   std::vector<int> info(nmesh);
   HIPCHK(hipMemcpyAsync(info.data(), p.info.p, sizeof(int) * nmesh, hipMemcpyDeviceToHost, st));
@@ -1393,9 +1389,7 @@ int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_
   if (opts && opts->tol > 0) go.tol = opts->tol;
   if (opts && opts->max_iter > 0) go.max_iter = opts->max_iter;
   const bool fp16 = opts && opts->precision == CDJ_PREC_MIXED16;
-  std::vector<int64_t> order;
-  std::vector<int> sid;
-  batch_solve_order(p, order, sid);
+  const std::vector<int64_t> order = batch_solve_order(p);
   float* A32 = p.A32.as<float>();
   int32_t* ipiv = p.ipiv.as<int32_t>();
   int32_t* perm = p.perm.as<int32_t>();
@@ -1405,21 +1399,20 @@ int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_
   auto Xm = [&](int64_t m) { return p.X.as<double>() + p.b_off[m]; };
   auto Rm = [&](int64_t m) { return p.R.as<double>() + p.b_off[m]; };
   auto Wm = [&](int64_t m) { return p.W.as<float>() + p.b_off[m]; };
-  auto part = [&](int s) { return reinterpret_cast<double*>(p.part.as<char>() + p.part_bytes * s); };
+  double* part = p.part.as<double>();
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
-  HIPCHK(batch_fork(p, st));
+  std::vector<LuJob<float>> jobs;
   for (int64_t m : order) {
     const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
-    hipStream_t hs = p.streams[sid[m]];
-    convert_matrix_d2f(Am(m), N3, A32 + p.a32_off[m], N3, N3, N3, hs);
-    HIPCHK(lu_factor<float>(A32 + p.a32_off[m], N3, N3, ipiv + p.v_off[m], p.info.as<int>() + m,
-                            p.lu_ws.as<char>() + p.lu_ws_bytes * sid[m], hs, fp16));
-    colnorm2(Bm(m), N3, N3, nr, nrm + p.q_total + p.q_off[m], hs);
+    convert_matrix_d2f(Am(m), N3, A32 + p.a32_off[m], N3, N3, N3, st);
+    colnorm2(Bm(m), N3, N3, nr, nrm + p.q_total + p.q_off[m], st);
+    jobs.push_back(LuJob<float>{A32 + p.a32_off[m], N3, N3, N3, ipiv + p.v_off[m], p.info.as<int>() + m,
+                                p.lu_ws.as<char>() + p.ws_off[m]});
   }
-  HIPCHK(batch_join(p, st));
+  HIPCHK(lu_factor_batched<float>(jobs.data(), (int)jobs.size(), st, fp16));
   std::vector<int> info(nmesh);
   std::vector<int32_t> hp(p.v_total);
   HIPCHK(hipMemcpyAsync(info.data(), p.info.p, sizeof(int) * nmesh, hipMemcpyDeviceToHost, st));
@@ -1466,7 +1459,7 @@ int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_
     if (e == hipSuccess) e = lu_apply_batched(p.jobs.as<LuApplyJob>(), (int)hj.size(), st);
     for (size_t k = 0; e == hipSuccess && k < ms.size(); ++k) {
       const int64_t m = ms[k], N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
-      e = residual_multi(Am(m), N3, N3, Bm(m), N3, Xm(m), N3, Rm(m), N3, nr, part(0), nrm + p.q_off[m], st);
+      e = residual_multi(Am(m), N3, N3, Bm(m), N3, Xm(m), N3, Rm(m), N3, nr, part, nrm + p.q_off[m], st);
     }
     return e;
   };
@@ -1626,7 +1619,7 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   ctx->near_entries = (int64_t)cnt[2];
-  // right-hand sides to the device, then every mesh's solve (worker streams, meshes concurrently)
+  // right-hand sides to the device, then the solves of all meshes together
   for (int64_t m = 0; m < nmesh; ++m)
     HIPCHK(hipMemcpyAsync(p.B.as<double>() + p.b_off[m], meshes[m].b, sizeof(double) * 3 * meshes[m].n * meshes[m].nrhs,
                           hipMemcpyHostToDevice, st));

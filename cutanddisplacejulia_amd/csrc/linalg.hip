@@ -113,10 +113,11 @@ __device__ __forceinline__ double join_words(unsigned long long lo, unsigned lon
 }
 
 // QPL: slots polled per lane (P <= 64 QPL)
+// The panel of one matrix, factored by workgroup b of the P workgroups assigned to it.
 template <class T, int QPL>
-__global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
-                                                      int32_t* ipiv, int* info, unsigned long long* xw,
-                                                      unsigned long long* slots, int* bar_err, int* perm) {
+__device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb, int32_t* ipiv,
+                                           int* info, unsigned long long* xw, unsigned long long* slots, int* bar_err,
+                                           int* perm, const int P, const int b) {
   extern __shared__ unsigned char smem_raw[];
   T* Lsh = reinterpret_cast<T*>(smem_raw);  // multipliers of this workgroup's rows: Lsh[c * rpb + tid]
   T* Ush = Lsh + (size_t)rpb * LU_NB;        // U rows of the panel, packed (row j at urow(j), columns j..)
@@ -126,7 +127,7 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
   __shared__ int s_prow[2];
   __shared__ int s_piv[LU_NB];  // pivot positions of this panel (relative to k)
   __shared__ int s_info;        // first column with a zero (or NaN) pivot, as a 1-based global index; 0 if none
-  const int P = gridDim.x, b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t r0 = (int64_t)b * rpb;
   const int64_t r1 = r0 + rpb < m ? r0 + rpb : m;
   const int nr = r1 > r0 ? (int)(r1 - r0) : 0;
@@ -414,6 +415,43 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
     }
     if (lane == 0) perm[0] = cnt;
   }
+}
+
+template <class T, int QPL>
+__global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
+                                                      int32_t* ipiv, int* info, unsigned long long* xw,
+                                                      unsigned long long* slots, int* bar_err, int* perm) {
+  panel_body<T, QPL>(A, lda, k, m, nbw, rpb, ipiv, info, xw, slots, bar_err, perm, (int)gridDim.x, (int)blockIdx.x);
+}
+
+// Panels of several matrices at the same column k in one cooperative launch: matrix j owns workgroups
+// [b0, b0 + P) and exchanges its pivot candidates only among them, through its own workspace.
+template <class T>
+struct PanelJob {
+  T* A;
+  int64_t lda, m;
+  int32_t nbw, rpb, P, b0;
+  int32_t* ipiv;
+  int* info;
+  unsigned long long* xw;
+  unsigned long long* slots;
+  int* bar_err;
+  int* perm;
+};
+constexpr int PANEL_JOBS = 24;  // matrices per launch (the job table travels in the kernel arguments)
+template <class T>
+struct PanelBatch {
+  int njobs;
+  PanelJob<T> job[PANEL_JOBS];
+};
+
+template <class T, int QPL>
+__global__ void __launch_bounds__(PT) lu_panel_batched_kernel(const PanelBatch<T> jb, int64_t k) {
+  int j = 0;
+  while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].b0) ++j;
+  const PanelJob<T>& J = jb.job[j];
+  panel_body<T, QPL>(J.A, J.lda, k, J.m, J.nbw, J.rpb, J.ipiv, J.info, J.xw, J.slots, J.bar_err, J.perm, J.P,
+                     (int)blockIdx.x - J.b0);
 }
 
 // Row interchanges of one panel applied to columns [0, k) and [k + nbw, ncols), as the permutation of the
@@ -960,6 +998,141 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   return hipGetLastError();
 }
 
+// Several matrices factored together, each exactly as lu_factor_cols would factor it alone (the same operations
+// in the same order, so the same factors bit for bit), but stepped in lockstep by panel column so that the panels
+// of all matrices at column k share cooperative launches: a panel's cost is the latency chain of its 64 columns,
+// which one launch now pays once for up to PANEL_JOBS matrices.  The catch-up and trailing updates, triangular
+// solves and row interchanges stay per matrix, on the same stream.
+template <class T>
+hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bool fp16_update) {
+  if (njobs <= 0) return hipSuccess;
+  struct WsView {
+    unsigned long long *slots, *xw;
+    unsigned* bar;
+    int *bar_err, *perm;
+    float* sigma;
+    size_t xbytes;
+  };
+  std::vector<WsView> wv(njobs);
+  int64_t nmax = 0;
+  for (int j = 0; j < njobs; ++j) {
+    const LuJob<T>& J = jobs[j];
+    if (J.n < 0 || J.ncols < J.n || J.lda < J.n) return hipErrorInvalidValue;
+    if ((J.n + MAX_RPB - 1) / MAX_RPB > MAX_P) return hipErrorInvalidValue;
+    char* w = static_cast<char*>(J.ws);
+    WsView& v = wv[j];
+    v.slots = reinterpret_cast<unsigned long long*>(w);
+    v.xw = v.slots + 2 * MAX_P;
+    v.xbytes = sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC);
+    w += v.xbytes;
+    v.bar = reinterpret_cast<unsigned*>(w);
+    v.bar_err = reinterpret_cast<int*>(v.bar + 1);
+    v.perm = v.bar_err + 1;
+    v.sigma = reinterpret_cast<float*>(v.perm + 1 + 4 * LU_NB);
+    hipError_t e = hipMemsetAsync(J.info, 0, sizeof(int), st);
+    if (e == hipSuccess) e = hipMemsetAsync(v.bar, 0, 2 * sizeof(unsigned), st);
+    if (e == hipSuccess) e = hipMemsetAsync(v.slots, 0xFF, v.xbytes, st);
+    if (e != hipSuccess) return e;
+    nmax = std::max(nmax, J.n);
+  }
+  const int lds_max = (int)(sizeof(T) * ((size_t)MAX_RPB * LU_NB + USH));
+  // co-resident panel workgroups: what one cooperative launch may hold
+  static const int capacity = [lds_max] {
+    hipError_t r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, 2>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    if (r == hipSuccess)
+      r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    int dev = 0, cus = 0, per_cu = 0;
+    if (r == hipSuccess) r = hipGetDevice(&dev);
+    if (r == hipSuccess) r = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (r == hipSuccess)
+      r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
+                                                       PT, lds_max);
+    return r == hipSuccess ? per_cu * cus : 0;
+  }();
+  if (capacity <= 0) return hipErrorInvalidConfiguration;
+  std::vector<int> act;
+  for (int64_t K0 = 0; K0 < nmax; K0 += LU_OUTER) {
+    for (int64_t k = K0; k < K0 + LU_OUTER && k < nmax; k += LU_NB) {
+      act.clear();
+      for (int j = 0; j < njobs; ++j)
+        if (jobs[j].n > k) act.push_back(j);
+      // this panel's columns of every matrix: U rows from the block's earlier panels, then their update
+      if (k > K0)
+        for (int j : act) {
+          const LuJob<T>& J = jobs[j];
+          const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
+          const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
+          block_trsm<T>(J.A, J.lda, K0, k, k, nbw, st);
+          gemm_sub<T>(J.A + k + K0 * J.lda, J.lda, J.A + K0 + k * J.lda, J.lda, J.A + k + k * J.lda, J.lda, J.n - k, nbw,
+                      k - K0, st);
+        }
+      // the panels, as few launches as co-residency and the job table allow
+      size_t a = 0;
+      while (a < act.size()) {
+        PanelBatch<T> pb;
+        pb.njobs = 0;
+        int blocks = 0, rpb_max = 1, pmax = 1;
+        for (; a < act.size() && pb.njobs < PANEL_JOBS; ++a) {
+          const LuJob<T>& J = jobs[act[a]];
+          const int64_t m = J.n - k;
+          const int P = (int)((m + MAX_RPB - 1) / MAX_RPB);
+          if (blocks > 0 && blocks + P > capacity) break;
+          const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
+          PanelJob<T>& q = pb.job[pb.njobs++];
+          q.A = J.A;
+          q.lda = J.lda;
+          q.m = m;
+          q.nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
+          q.rpb = (int)((m + P - 1) / P);
+          q.P = P;
+          q.b0 = blocks;
+          q.ipiv = J.ipiv;
+          q.info = J.info;
+          q.xw = wv[act[a]].xw;
+          q.slots = wv[act[a]].slots;
+          q.bar_err = wv[act[a]].bar_err;
+          q.perm = wv[act[a]].perm;
+          blocks += P;
+          rpb_max = std::max(rpb_max, q.rpb);
+          pmax = std::max(pmax, P);
+        }
+        if (blocks > capacity) return hipErrorCooperativeLaunchTooLarge;
+        const size_t shm = sizeof(T) * ((size_t)rpb_max * LU_NB + USH);
+        int64_t kk = k;
+        void* args[] = {&pb, &kk};
+        const void* kern = pmax <= 128 ? (const void*)lu_panel_batched_kernel<T, 2>
+                                       : (const void*)lu_panel_batched_kernel<T, MAX_P / 64>;
+        hipError_t e = hipLaunchCooperativeKernel(kern, dim3(blocks), dim3(PT), args, shm, st);
+        if (e != hipSuccess) return e;
+      }
+      for (int j : act) {
+        const LuJob<T>& J = jobs[j];
+        const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
+        const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
+        if (J.ncols > nbw)
+          hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(J.ncols, 2)), dim3(256), 0, st, J.A, J.lda, J.ncols, k, nbw,
+                             (const int*)wv[j].perm);
+      }
+    }
+    // U12 of the block, then the rank-(K1 - K0) update of everything right of and below it
+    for (int j = 0; j < njobs; ++j) {
+      const LuJob<T>& J = jobs[j];
+      if (J.n <= K0) continue;
+      const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
+      block_trsm<T>(J.A, J.lda, K0, K1, K1, J.ncols - K1, st);
+      trailing_update<T>(J.A + K1 + K0 * J.lda, J.A + K0 + K1 * J.lda, J.A + K1 + K1 * J.lda, J.lda, J.n - K1,
+                         J.ncols - K1, K1 - K0, wv[j].sigma, fp16_update && J.ncols == J.n, st);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  for (int j = 0; j < njobs; ++j)
+    hipLaunchKernelGGL(barrier_status_kernel, dim3(1), dim3(1), 0, st, (const int*)wv[j].bar_err, jobs[j].info);
+  return hipGetLastError();
+}
+
 template <class T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st,
                      bool fp16_update) {
@@ -1091,6 +1264,8 @@ void convert_matrix_d2f(const double* A, int64_t lda, float* B, int64_t ldb, int
   if (m > 0 && n > 0) hipLaunchKernelGGL(d2f_matrix_kernel, dim3(nblk(m, 256), (unsigned)n), dim3(256), 0, st, A, lda, B, ldb, m, n);
 }
 
+template hipError_t lu_factor_batched<double>(const LuJob<double>*, int, hipStream_t, bool);
+template hipError_t lu_factor_batched<float>(const LuJob<float>*, int, hipStream_t, bool);
 template hipError_t lu_factor<double>(double*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
 template hipError_t lu_factor<float>(float*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
 template hipError_t lu_factor_cols<double>(double*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t,

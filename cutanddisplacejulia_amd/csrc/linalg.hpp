@@ -27,6 +27,22 @@ template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
                           hipStream_t st, bool fp16_update = false);
 
+// One matrix of a batched factorization: as lu_factor_cols(A, n, ncols, lda, ipiv, info, ws, ...), with ws of
+// lu_workspace_bytes(n) bytes (one per matrix).
+template <class T>
+struct LuJob {
+  T* A;
+  int64_t n, ncols, lda;
+  int32_t* ipiv;
+  int* info;
+  void* ws;
+};
+
+// Factors every job's matrix as lu_factor_cols would, with the panels of all matrices at the same column sharing
+// cooperative launches.  fp16_update applies to the jobs with ncols == n.
+template <class T>
+hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bool fp16_update = false);
+
 // Row permutation implied by the pivots (row i of P A is row perm[i] of A); perm_dev is a device array.
 // Synchronises the stream.
 hipError_t lu_permutation(const int32_t* ipiv, int64_t n, int32_t* perm_dev, hipStream_t st);

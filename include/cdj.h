@@ -158,6 +158,13 @@ int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const d
 
 /* LU factorisation in place (FP64, partial pivoting; ipiv 0-based, length n) and solve with the factors. */
 int cdj_lu_factor_dev(cdj_ctx* ctx, double* A, int64_t n, int64_t lda, int32_t* ipiv, void* stream);
+/* Many independent FP64 LUs at once (no counterpart in the reference, which factors one matrix per `A\b`; this is
+ * the factorization behind cdj_batched_slip): matrix j is A[j] (device, n[j] x n[j], leading dimension lda[j]),
+ * factored in place with pivots ipiv[j] (device) exactly as cdj_lu_factor_dev factors it alone; the matrices'
+ * panels share cooperative launches.  info (host, count entries) receives 0 or 1 + the first zero pivot's index
+ * for each matrix; a singular matrix is not an error here.  Synchronises the stream. */
+int cdj_lu_factor_batched_dev(cdj_ctx* ctx, double* const* A, const int64_t* n, const int64_t* lda, int64_t count,
+                              int32_t* const* ipiv, int32_t* info, void* stream);
 int cdj_lu_solve_dev(cdj_ctx* ctx, const double* LU, int64_t n, int64_t lda, const int32_t* ipiv, double* B,
                      int64_t nrhs, int64_t ldb, void* stream);
 

@@ -91,3 +91,56 @@ def test_batched_graph_replay_with_new_geometry(batch):
     for a, ref in zip(x2, refs):
         assert np.max(np.abs(a - ref)) <= 1e-12 * np.max(np.abs(ref))
     assert any(np.max(np.abs(a - b)) > 1e-6 for a, b in zip(x1, x2))
+
+
+def _lu_single(L, ctx, sp, A, n):
+    import ctypes
+    import torch
+    LU = A.clone()
+    ipiv = torch.empty(max(n, 1), dtype=torch.int32, device="cuda:0")
+    rc = L.cdj_lu_factor_dev(ctx, ctypes.c_void_p(LU.data_ptr()), n, n, ctypes.c_void_p(ipiv.data_ptr()), sp)
+    return rc, LU, ipiv
+
+
+def test_batched_lu_bitwise_equals_single():
+    """cdj_lu_factor_batched_dev factors each matrix exactly as cdj_lu_factor_dev does alone (same operations in
+    the same order), over sizes that need one and several panel workgroups, more matrices than one panel launch
+    holds, integer-valued matrices with exact pivot ties across workgroups, and a singular matrix."""
+    import ctypes
+    import torch
+    from cutanddisplacejulia_amd import _lib
+    L = _lib.load()
+    ctx = _lib.ctx()
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rng = np.random.default_rng(11)
+    sizes = [700, 1, 64, 65, 300, 257, 1100, 513, 0] + [int(s) for s in rng.integers(2, 200, 24)]
+    mats = []
+    for t, n in enumerate(sizes):
+        if t % 3 == 1:  # small integers: many exactly equal candidates, including across workgroups
+            a = rng.integers(-3, 4, size=(n, n)).astype(np.float64) + 7.0 * np.eye(n)
+        else:
+            a = rng.standard_normal((n, n))
+        mats.append(torch.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).to("cuda:0"))
+    sing = rng.standard_normal((40, 40))
+    sing[:, 17] = 0.0  # a zero column stays zero through the elimination: an exact zero pivot at column 17
+    sizes.append(40)
+    mats.append(torch.from_numpy(np.asfortranarray(sing).ravel(order="F").copy()).to("cuda:0"))
+    work = [m.clone() for m in mats]
+    pivs = [torch.empty(max(n, 1), dtype=torch.int32, device="cuda:0") for n in sizes]
+    cnt = len(sizes)
+    Ap = (ctypes.c_void_p * cnt)(*[m.data_ptr() for m in work])
+    Pp = (ctypes.c_void_p * cnt)(*[p.data_ptr() for p in pivs])
+    nn = (ctypes.c_int64 * cnt)(*sizes)
+    info = (ctypes.c_int32 * cnt)()
+    _lib.check(L.cdj_lu_factor_batched_dev(ctx, Ap, nn, nn, cnt, Pp, info, sp))
+    torch.cuda.synchronize()
+    for t, n in enumerate(sizes):
+        rc, LU, ipiv = _lu_single(L, ctx, sp, mats[t], n)
+        torch.cuda.synchronize()
+        if t == cnt - 1:
+            assert rc == _lib.CDJ_ERR_SINGULAR and info[t] == 18
+        else:
+            assert rc == 0 and info[t] == 0, (t, n, rc, info[t])
+        assert torch.equal(work[t], LU), (t, n)
+        if n > 0:
+            assert torch.equal(pivs[t][:n], ipiv[:n]), (t, n)
