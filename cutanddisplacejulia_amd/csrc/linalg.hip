@@ -479,8 +479,8 @@ __global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t n
 // below update with L11 read from LDS (stored transposed, so a step's reads are consecutive).  A column's solve is
 // a latency chain of nbw steps, so few columns get one column per wave (more workgroups, each staging L11).
 template <class T>
-__global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t ncols,
-                                                   int cpw) {
+__device__ __forceinline__ void trsm_body(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t ncols, int cpw,
+                                          int64_t blk) {
   __shared__ T LT[LU_NB][LU_NB + 1];  // LT[q][r] = L11[r][q]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
@@ -488,7 +488,7 @@ __global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k,
     LT[q][r] = (r < nbw && q < nbw) ? A[(k + r) + (k + q) * lda] : T(0);
   }
   __syncthreads();
-  const int64_t cb = c0 + ((int64_t)blockIdx.x * 4 + wave) * cpw;
+  const int64_t cb = c0 + (blk * 4 + wave) * cpw;
   for (int i = 0; i < cpw; ++i) {
     const int64_t c = cb + i;
     if (c >= c0 + ncols) break;
@@ -500,6 +500,12 @@ __global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k,
     }
     if (lane < nbw) col[lane] = x;
   }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t ncols,
+                                                   int cpw) {
+  trsm_body<T>(A, lda, k, nbw, c0, ncols, cpw, (int64_t)blockIdx.x);
 }
 
 // C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N], column-major with leading dimensions ldl, ldu, ldc, any K.
@@ -528,15 +534,12 @@ __device__ __forceinline__ void xcd_tile(int b, int gx, int gy, int& tx, int& ty
 }
 
 template <class T>
-__global__ void __launch_bounds__(256, 2) gemm_sub_kernel(const T* __restrict__ L, int64_t ldl,
-                                                          const T* __restrict__ U, int64_t ldu, T* __restrict__ C,
-                                                          int64_t ldc, int64_t M, int64_t N, int64_t K, int gx,
-                                                          int gy) {
+__device__ __forceinline__ void gemm_sub_tile(const T* __restrict__ L, int64_t ldl, const T* __restrict__ U,
+                                              int64_t ldu, T* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
+                                              int64_t K, int tx, int ty) {
   __shared__ T Ls[2][GB_K][GB_M + 4];  // Ls[buf][k][i] = L[i0 + i, k0 + k]
   __shared__ T Us[2][GB_K][GB_N + 4];  // Us[buf][k][j] = U[k0 + k, j0 + j]
   const int tid = threadIdx.x;
-  int tx, ty;
-  xcd_tile(blockIdx.x, gx, gy, tx, ty);
   const int64_t i0 = (int64_t)tx * GB_M, j0 = (int64_t)ty * GB_N;
   const int wave = tid / 64, lane = tid % 64;
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
@@ -610,6 +613,63 @@ __global__ void __launch_bounds__(256, 2) gemm_sub_kernel(const T* __restrict__ 
         if (row < M && col < N) C[row + col * ldc] -= acc[tj][ti][v];
       }
     }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256, 2) gemm_sub_kernel(const T* __restrict__ L, int64_t ldl,
+                                                          const T* __restrict__ U, int64_t ldu, T* __restrict__ C,
+                                                          int64_t ldc, int64_t M, int64_t N, int64_t K, int gx,
+                                                          int gy) {
+  int tx, ty;
+  xcd_tile(blockIdx.x, gx, gy, tx, ty);
+  gemm_sub_tile<T>(L, ldl, U, ldu, C, ldc, M, N, K, tx, ty);
+}
+
+// Several independent products in one launch (job j owns workgroups [t0, t0 + gx gy)), for the small per-matrix
+// updates of a batched factorization, which alone would occupy a fraction of the CUs.
+template <class T>
+struct GemmJob {
+  const T* L;
+  const T* U;
+  T* C;
+  int64_t ldl, ldu, ldc, M, N, K;
+  int32_t gx, gy, t0, pad;
+};
+constexpr int BATCH_JOBS = 24;
+template <class T>
+struct GemmBatch {
+  int njobs;
+  GemmJob<T> job[BATCH_JOBS];
+};
+
+template <class T>
+__global__ void __launch_bounds__(256, 2) gemm_sub_batched_kernel(const GemmBatch<T> jb) {
+  int j = 0;
+  while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].t0) ++j;
+  const GemmJob<T>& J = jb.job[j];
+  int tx, ty;
+  xcd_tile((int)blockIdx.x - J.t0, J.gx, J.gy, tx, ty);
+  gemm_sub_tile<T>(J.L, J.ldl, J.U, J.ldu, J.C, J.ldc, J.M, J.N, J.K, tx, ty);
+}
+
+template <class T>
+struct TrsmJob {
+  T* A;
+  int64_t lda, k, c0, ncols;
+  int32_t nbw, cpw, b0, pad;
+};
+template <class T>
+struct TrsmBatch {
+  int njobs;
+  TrsmJob<T> job[BATCH_JOBS];
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) trsm_batched_kernel(const TrsmBatch<T> jb) {
+  int j = 0;
+  while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].b0) ++j;
+  const TrsmJob<T>& J = jb.job[j];
+  trsm_body<T>(J.A, J.lda, J.k, J.nbw, J.c0, J.ncols, J.cpw, (int64_t)blockIdx.x - J.b0);
 }
 
 // FP16 variant of the trailing update for FP32 factors: C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N] with both
@@ -890,6 +950,56 @@ void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t n
   }
 }
 
+// Job lists of the batched factorization, launched BATCH_JOBS products (or solves) at a time.
+template <class T>
+struct GemmList {
+  std::vector<GemmJob<T>> v;
+  void add(const T* L, int64_t ldl, const T* U, int64_t ldu, T* C, int64_t ldc, int64_t M, int64_t N, int64_t K) {
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    v.push_back(GemmJob<T>{L, U, C, ldl, ldu, ldc, M, N, K, (int32_t)nblk(M, GB_M), (int32_t)nblk(N, GB_N), 0, 0});
+  }
+  void launch(hipStream_t st) {
+    for (size_t a = 0; a < v.size(); a += BATCH_JOBS) {
+      GemmBatch<T> b;
+      b.njobs = 0;
+      int t = 0;
+      for (size_t i = a; i < v.size() && b.njobs < BATCH_JOBS; ++i) {
+        GemmJob<T> j = v[i];
+        j.t0 = t;
+        t += j.gx * j.gy;
+        b.job[b.njobs++] = j;
+      }
+      hipLaunchKernelGGL(gemm_sub_batched_kernel<T>, dim3(t), dim3(256), 0, st, b);
+    }
+    v.clear();
+  }
+};
+
+template <class T>
+struct TrsmList {
+  std::vector<TrsmJob<T>> v;
+  void add(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t nc) {
+    if (nc <= 0) return;
+    const int cpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, nc / 8192));
+    v.push_back(TrsmJob<T>{A, lda, k, c0, nc, nbw, cpw, 0, 0});
+  }
+  void launch(hipStream_t st) {
+    for (size_t a = 0; a < v.size(); a += BATCH_JOBS) {
+      TrsmBatch<T> b;
+      b.njobs = 0;
+      int t = 0;
+      for (size_t i = a; i < v.size() && b.njobs < BATCH_JOBS; ++i) {
+        TrsmJob<T> j = v[i];
+        j.b0 = t;
+        t += (int)nblk(j.ncols, 4 * j.cpw);
+        b.job[b.njobs++] = j;
+      }
+      hipLaunchKernelGGL(trsm_batched_kernel<T>, dim3(t), dim3(256), 0, st, b);
+    }
+    v.clear();
+  }
+};
+
 }  // namespace
 
 size_t lu_workspace_bytes(int64_t n) {
@@ -1053,21 +1163,37 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
   }();
   if (capacity <= 0) return hipErrorInvalidConfiguration;
   std::vector<int> act;
+  GemmList<T> gl;
+  TrsmList<T> tl;
   for (int64_t K0 = 0; K0 < nmax; K0 += LU_OUTER) {
     for (int64_t k = K0; k < K0 + LU_OUTER && k < nmax; k += LU_NB) {
       act.clear();
       for (int j = 0; j < njobs; ++j)
         if (jobs[j].n > k) act.push_back(j);
-      // this panel's columns of every matrix: U rows from the block's earlier panels, then their update
-      if (k > K0)
+      // this panel's columns of every matrix: U rows from the block's earlier panels (sub-panel by sub-panel, a
+      // triangular solve and the update of the rows below it), then their update of rows [k, n)
+      auto nbw_of = [&](const LuJob<T>& J) {
+        return (int)std::min<int64_t>(LU_NB, std::min<int64_t>(J.n, K0 + LU_OUTER) - k);
+      };
+      if (k > K0) {
+        for (int64_t s = K0; s < k; s += LU_NB) {
+          for (int j : act) tl.add(jobs[j].A, jobs[j].lda, s, LU_NB, k, nbw_of(jobs[j]));
+          tl.launch(st);
+          for (int j : act) {
+            const LuJob<T>& J = jobs[j];
+            const int64_t ld = J.lda;
+            gl.add(J.A + (s + LU_NB) + s * ld, ld, J.A + s + k * ld, ld, J.A + (s + LU_NB) + k * ld, ld, k - s - LU_NB,
+                   nbw_of(J), LU_NB);
+          }
+          gl.launch(st);
+        }
         for (int j : act) {
           const LuJob<T>& J = jobs[j];
-          const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
-          const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
-          block_trsm<T>(J.A, J.lda, K0, k, k, nbw, st);
-          gemm_sub<T>(J.A + k + K0 * J.lda, J.lda, J.A + K0 + k * J.lda, J.lda, J.A + k + k * J.lda, J.lda, J.n - k, nbw,
-                      k - K0, st);
+          const int64_t ld = J.lda;
+          gl.add(J.A + k + K0 * ld, ld, J.A + K0 + k * ld, ld, J.A + k + k * ld, ld, J.n - k, nbw_of(J), k - K0);
         }
+        gl.launch(st);
+      }
       // the panels, as few launches as co-residency and the job table allow
       size_t a = 0;
       while (a < act.size()) {
@@ -1116,15 +1242,35 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
                              (const int*)wv[j].perm);
       }
     }
-    // U12 of the block, then the rank-(K1 - K0) update of everything right of and below it
+    // U12 of the block (sub-panel by sub-panel), then the rank-(K1 - K0) update of everything right of and below it
+    for (int64_t s = K0; s < K0 + LU_OUTER && s < nmax; s += LU_NB) {
+      for (int j = 0; j < njobs; ++j) {
+        const LuJob<T>& J = jobs[j];
+        const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
+        if (J.n > K0 && s < K1) tl.add(J.A, J.lda, s, (int)std::min<int64_t>(LU_NB, K1 - s), K1, J.ncols - K1);
+      }
+      tl.launch(st);
+      for (int j = 0; j < njobs; ++j) {
+        const LuJob<T>& J = jobs[j];
+        const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER), ld = J.lda;
+        if (J.n <= K0 || s >= K1) continue;
+        const int64_t nb = std::min<int64_t>(LU_NB, K1 - s);
+        gl.add(J.A + (s + nb) + s * ld, ld, J.A + s + K1 * ld, ld, J.A + (s + nb) + K1 * ld, ld, K1 - s - nb,
+               J.ncols - K1, nb);
+      }
+      gl.launch(st);
+    }
     for (int j = 0; j < njobs; ++j) {
       const LuJob<T>& J = jobs[j];
       if (J.n <= K0) continue;
-      const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
-      block_trsm<T>(J.A, J.lda, K0, K1, K1, J.ncols - K1, st);
-      trailing_update<T>(J.A + K1 + K0 * J.lda, J.A + K0 + K1 * J.lda, J.A + K1 + K1 * J.lda, J.lda, J.n - K1,
-                         J.ncols - K1, K1 - K0, wv[j].sigma, fp16_update && J.ncols == J.n, st);
+      const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER), ld = J.lda;
+      if (fp16_update && J.ncols == J.n)
+        trailing_update<T>(J.A + K1 + K0 * ld, J.A + K0 + K1 * ld, J.A + K1 + K1 * ld, ld, J.n - K1, J.ncols - K1,
+                           K1 - K0, wv[j].sigma, true, st);
+      else
+        gl.add(J.A + K1 + K0 * ld, ld, J.A + K0 + K1 * ld, ld, J.A + K1 + K1 * ld, ld, J.n - K1, J.ncols - K1, K1 - K0);
     }
+    gl.launch(st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
