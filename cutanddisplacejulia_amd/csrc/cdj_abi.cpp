@@ -1298,7 +1298,7 @@ int batch_solve_setup(cdj_ctx* ctx, BatchPlan& p, bool mixed) {
   }
   if (mixed && !p.bufs_mixed) {
     size_t pb = 0;
-    for (int64_t m = 0; m < nmesh; ++m) pb = std::max(pb, gemv_part_bytes(3 * p.n[m], 3 * p.n[m]));
+    for (int64_t m = 0; m < nmesh; ++m) pb = std::max(pb, residual_multi_bytes(3 * p.n[m], p.nrhs[m]));
     p.part_bytes = pb;
     HIPCHK(p.part.alloc(p.part_bytes));
     HIPCHK(p.A32.alloc(sizeof(float) * p.a32_total));

@@ -458,9 +458,9 @@ __global__ void __launch_bounds__(PT) lu_panel_batched_kernel(const PanelBatch<T
 // touched rows emitted by the panel (perm = {count, (dst, src) pairs} relative to k).  Two columns per block,
 // 128 threads each: all reads of a column complete before any write.
 template <class T>
-__global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw,
-                                                    const int* __restrict__ perm) {
-  const int64_t c = (int64_t)blockIdx.x * 2 + threadIdx.x / 128;
+__device__ __forceinline__ void laswp_body(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw,
+                                           const int* __restrict__ perm, int64_t blk) {
+  const int64_t c = blk * 2 + threadIdx.x / 128;
   const int t = threadIdx.x % 128;
   const int cnt = perm[0];
   const bool act = c < ncols && !(c >= k && c < k + nbw) && t < cnt;
@@ -472,6 +472,35 @@ __global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t n
   }
   __syncthreads();
   if (act) A[(k + dst) + c * lda] = v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw,
+                                                    const int* __restrict__ perm) {
+  laswp_body<T>(A, lda, ncols, k, nbw, perm, (int64_t)blockIdx.x);
+}
+
+// The row interchanges of the panels at column k of several matrices (job j owns workgroups [b0, ...)).
+template <class T>
+struct SwapJob {
+  T* A;
+  int64_t lda, ncols;
+  const int* perm;
+  int32_t nbw, b0;
+};
+constexpr int SWAP_JOBS = 48;
+template <class T>
+struct SwapBatch {
+  int njobs;
+  SwapJob<T> job[SWAP_JOBS];
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) laswp_batched_kernel(const SwapBatch<T> jb, int64_t k) {
+  int j = 0;
+  while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].b0) ++j;
+  const SwapJob<T>& J = jb.job[j];
+  laswp_body<T>(J.A, J.lda, J.ncols, k, J.nbw, J.perm, (int64_t)blockIdx.x - J.b0);
 }
 
 // A[k:k+nbw, c0:c0+ncols] = L11^-1 A[k:k+nbw, c0:...] with L11 unit lower (the panel's diagonal block).  One
@@ -1233,13 +1262,25 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
         hipError_t e = hipLaunchCooperativeKernel(kern, dim3(blocks), dim3(PT), args, shm, st);
         if (e != hipSuccess) return e;
       }
-      for (int j : act) {
-        const LuJob<T>& J = jobs[j];
-        const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER);
-        const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
-        if (J.ncols > nbw)
-          hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(J.ncols, 2)), dim3(256), 0, st, J.A, J.lda, J.ncols, k, nbw,
-                             (const int*)wv[j].perm);
+      // the panels' row interchanges in the other columns, SWAP_JOBS matrices per launch
+      {
+        SwapBatch<T> sb;
+        sb.njobs = 0;
+        int blocks = 0;
+        for (size_t a2 = 0; a2 <= act.size(); ++a2) {
+          if (sb.njobs == SWAP_JOBS || (a2 == act.size() && sb.njobs > 0)) {
+            int64_t kk = k;
+            hipLaunchKernelGGL(laswp_batched_kernel<T>, dim3(blocks), dim3(256), 0, st, sb, kk);
+            sb.njobs = 0;
+            blocks = 0;
+          }
+          if (a2 == act.size()) break;
+          const LuJob<T>& J = jobs[act[a2]];
+          const int nbw = nbw_of(J);
+          if (J.ncols <= nbw) continue;
+          sb.job[sb.njobs++] = SwapJob<T>{J.A, J.lda, J.ncols, (const int*)wv[act[a2]].perm, nbw, blocks};
+          blocks += (int)nblk(J.ncols, 2);
+        }
       }
     }
     // U12 of the block (sub-panel by sub-panel), then the rank-(K1 - K0) update of everything right of and below it

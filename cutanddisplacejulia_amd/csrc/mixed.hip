@@ -231,27 +231,6 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int nvec, i
   out[j] = s;
 }
 
-// out[i + q ldo] = (TO) in[perm[i] + q ldi] for q < nrhs (perm == nullptr: identity)
-template <class TI, class TO>
-__global__ void permute_cast_multi_kernel(const int32_t* __restrict__ perm, const TI* __restrict__ in, int64_t ldi,
-                                          TO* __restrict__ out, int64_t ldo, int64_t n, int64_t nrhs) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n * nrhs) return;
-  const int64_t q = t / n, i = t - q * n;
-  out[i + q * ldo] = (TO)in[(perm ? (int64_t)perm[i] : i) + q * ldi];
-}
-
-// x[i + q ldx] (+)= w[i + q ldw]
-template <class TW>
-__global__ void add_cast_multi_kernel(const TW* __restrict__ w, int64_t ldw, double* __restrict__ x, int64_t ldx,
-                                      int64_t n, int64_t nrhs, int accumulate) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= n * nrhs) return;
-  const int64_t q = t / n, i = t - q * n;
-  const double v = (double)w[i + q * ldw];
-  x[i + q * ldx] = accumulate ? x[i + q * ldx] + v : v;
-}
-
 // out[q] = sum_i R[i + q ld]^2, one workgroup per column, fixed summation order (bitwise reproducible)
 __global__ void __launch_bounds__(256) colnorm2_kernel(const double* __restrict__ R, int64_t ld, int64_t n,
                                                        double* __restrict__ out) {
@@ -268,6 +247,49 @@ __global__ void __launch_bounds__(256) colnorm2_kernel(const double* __restrict_
   if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
+// ---- multi-column residual: R = B - A X for up to APPLY_NR columns, A read once ----------------------------
+constexpr int RES_CHUNK = 256;  // columns of A per partial sum
+constexpr int RES_NR = 8;       // right-hand sides per pass over A
+
+__global__ void __launch_bounds__(256) gemv_nr_part_kernel(const double* __restrict__ A, int64_t m, int64_t n,
+                                                           int64_t lda, const double* __restrict__ X, int64_t ldx,
+                                                           int nr, double* __restrict__ part) {
+  __shared__ double xs[RES_NR][RES_CHUNK];
+  const int64_t c0 = (int64_t)blockIdx.y * RES_CHUNK;
+  const int nc = (int)(n - c0 < RES_CHUNK ? n - c0 : RES_CHUNK);
+  for (int t = threadIdx.x; t < nr * RES_CHUNK; t += 256) {
+    const int q = t / RES_CHUNK, c = t % RES_CHUNK;
+    xs[q][c] = c < nc ? X[c0 + c + q * ldx] : 0.0;
+  }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= m) return;
+  double acc[RES_NR];
+#pragma unroll
+  for (int q = 0; q < RES_NR; ++q) acc[q] = 0.0;
+  const double* a = A + i + c0 * lda;
+#pragma unroll 4
+  for (int c = 0; c < nc; ++c) {
+    const double v = a[c * lda];
+#pragma unroll
+    for (int q = 0; q < RES_NR; ++q)
+      if (q < nr) acc[q] += v * xs[q][c];
+  }
+  for (int q = 0; q < nr; ++q) part[((int64_t)blockIdx.y * nr + q) * m + i] = acc[q];
+}
+
+// R[i, q] = B[i, q] - sum over chunks of the partials, in chunk order
+__global__ void __launch_bounds__(256) gemv_nr_reduce_kernel(const double* __restrict__ part, int64_t m, int nchunk,
+                                                             int nr, const double* __restrict__ B, int64_t ldb,
+                                                             double* __restrict__ R, int64_t ldr) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= m * nr) return;
+  const int64_t q = t / m, i = t - q * m;
+  double s = 0.0;
+  for (int k = 0; k < nchunk; ++k) s += part[((int64_t)k * nr + q) * m + i];
+  R[i + q * ldr] = B[i + q * ldb] - s;
+}
+
 // ---- batched factor application: one workgroup per system -------------------------------------------------
 // X (+)= U^-1 L^-1 P R for up to APPLY_NR columns of one system per workgroup.  The workgroup walks the
 // 64-row blocks itself: the diagonal block is staged in LDS and solved by one wave per column, then all 1024
@@ -276,9 +298,40 @@ __global__ void __launch_bounds__(256) colnorm2_kernel(const double* __restrict_
 constexpr int APPLY_NB = 64;
 constexpr int APPLY_THREADS = 1024;
 
+// W[i, q] -= sum_c F[i, k0 + c] xs[q][c] for one row i: a full 64-column block takes its factor entries in
+// groups of 4 and the solved values as float4 LDS reads (one read serves four columns)
+__device__ __forceinline__ void apply_row(const float* __restrict__ col, int64_t lda, int nb,
+                                          const float (*xs)[APPLY_NB], float* W, int64_t i, int64_t n, int nr) {
+  float acc[APPLY_NR];
+#pragma unroll
+  for (int q = 0; q < APPLY_NR; ++q) acc[q] = 0.f;
+  if (nb == APPLY_NB) {
+#pragma unroll 1
+    for (int c0 = 0; c0 < APPLY_NB; c0 += 16) {
+      float l[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) l[t] = col[(c0 + t) * lda];
+#pragma unroll
+      for (int t = 0; t < 16; t += 4)
+#pragma unroll
+        for (int q = 0; q < APPLY_NR; ++q) {
+          const float4 xv = *reinterpret_cast<const float4*>(&xs[q][c0 + t]);
+          acc[q] += l[t] * xv.x + l[t + 1] * xv.y + l[t + 2] * xv.z + l[t + 3] * xv.w;
+        }
+    }
+  } else {
+    for (int c = 0; c < nb; ++c) {
+      const float l = col[c * lda];
+#pragma unroll
+      for (int q = 0; q < APPLY_NR; ++q) acc[q] += l * xs[q][c];
+    }
+  }
+  for (int q = 0; q < nr; ++q) W[i + q * n] -= acc[q];
+}
+
 __global__ void __launch_bounds__(APPLY_THREADS) lu_apply_batched_kernel(const LuApplyJob* __restrict__ jobs) {
   __shared__ float D[APPLY_NB][APPLY_NB + 1];
-  __shared__ float xs[APPLY_NR][APPLY_NB];
+  __shared__ __align__(16) float xs[APPLY_NR][APPLY_NB];
   const LuApplyJob J = jobs[blockIdx.x];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t n = J.n, lda = J.lda;
@@ -310,19 +363,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) lu_apply_batched_kernel(const L
       }
     }
     __syncthreads();
-    for (int64_t i = k0 + nb + tid; i < n; i += APPLY_THREADS) {
-      float acc[APPLY_NR];
-#pragma unroll
-      for (int q = 0; q < APPLY_NR; ++q) acc[q] = 0.f;
-      const float* col = LU + i + k0 * lda;
-#pragma unroll 16
-      for (int c = 0; c < nb; ++c) {
-        const float l = col[c * lda];
-#pragma unroll
-        for (int q = 0; q < APPLY_NR; ++q) acc[q] += l * xs[q][c];
-      }
-      for (int q = 0; q < nr; ++q) W[i + q * n] -= acc[q];
-    }
+    for (int64_t i = k0 + nb + tid; i < n; i += APPLY_THREADS) apply_row(LU + i + k0 * lda, lda, nb, xs, W, i, n, nr);
     __syncthreads();
   }
   // backward: U with its diagonal
@@ -346,19 +387,7 @@ __global__ void __launch_bounds__(APPLY_THREADS) lu_apply_batched_kernel(const L
       }
     }
     __syncthreads();
-    for (int64_t i = tid; i < k0; i += APPLY_THREADS) {
-      float acc[APPLY_NR];
-#pragma unroll
-      for (int q = 0; q < APPLY_NR; ++q) acc[q] = 0.f;
-      const float* col = LU + i + k0 * lda;
-#pragma unroll 16
-      for (int c = 0; c < nb; ++c) {
-        const float u = col[c * lda];
-#pragma unroll
-        for (int q = 0; q < APPLY_NR; ++q) acc[q] += u * xs[q][c];
-      }
-      for (int q = 0; q < nr; ++q) W[i + q * n] -= acc[q];
-    }
+    for (int64_t i = tid; i < k0; i += APPLY_THREADS) apply_row(LU + i + k0 * lda, lda, nb, xs, W, i, n, nr);
     __syncthreads();
   }
   for (int64_t t = tid; t < n * nr; t += APPLY_THREADS) {
@@ -433,21 +462,6 @@ hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* 
 }
 
 // ---- many right-hand sides ----------------------------------------------------------------------------------
-template <class T>
-hipError_t lu_apply_multi(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
-                          double* X, int64_t ldx, int64_t nrhs, bool accumulate, T* W, hipStream_t st) {
-  if (n <= 0 || nrhs <= 0) return hipSuccess;
-  const unsigned g = nblk(n * nrhs, 256);
-  hipLaunchKernelGGL((permute_cast_multi_kernel<double, T>), dim3(g), dim3(256), 0, st, perm, R, ldr, W, n, n, nrhs);
-  hipError_t e = tri_solve_lower_unit<T>(LU, n, lda, W, nrhs, n, st);
-  if (e != hipSuccess) return e;
-  e = tri_solve_upper<T>(LU, n, lda, W, nrhs, n, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(add_cast_multi_kernel<T>, dim3(g), dim3(256), 0, st, (const T*)W, n, X, ldx, n, nrhs,
-                     accumulate ? 1 : 0);
-  return hipGetLastError();
-}
-
 hipError_t lu_apply_batched(const LuApplyJob* jobs, int njobs, hipStream_t st) {
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(lu_apply_batched_kernel, dim3((unsigned)njobs), dim3(APPLY_THREADS), 0, st, jobs);
@@ -459,11 +473,24 @@ void colnorm2(const double* R, int64_t ld, int64_t n, int64_t nrhs, double* out,
   hipLaunchKernelGGL(colnorm2_kernel, dim3((unsigned)nrhs), dim3(256), 0, st, R, ld, n, out);
 }
 
+size_t residual_multi_bytes(int64_t n, int64_t nrhs) {
+  const int64_t nq = std::min<int64_t>(std::max<int64_t>(nrhs, 1), RES_NR);
+  return sizeof(double) * (size_t)std::max<int64_t>(n, 1) * (size_t)nq *
+         (size_t)std::max<int64_t>((n + RES_CHUNK - 1) / RES_CHUNK, 1);
+}
+
 hipError_t residual_multi(const double* A, int64_t n, int64_t lda, const double* B, int64_t ldb, const double* X,
                           int64_t ldx, double* R, int64_t ldr, int64_t nrhs, double* part, double* nrm2,
                           hipStream_t st) {
-  for (int64_t q = 0; q < nrhs; ++q)
-    gemv_splitk<double>(A, n, n, lda, X + q * ldx, R + q * ldr, -1.0, B + q * ldb, 1.0, part, st);
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  const int nchunk = (int)((n + RES_CHUNK - 1) / RES_CHUNK);
+  for (int64_t q0 = 0; q0 < nrhs; q0 += RES_NR) {
+    const int nr = (int)std::min<int64_t>(RES_NR, nrhs - q0);
+    hipLaunchKernelGGL(gemv_nr_part_kernel, dim3(nblk(n, 256), (unsigned)nchunk), dim3(256), 0, st, A, n, n, lda,
+                       X + q0 * ldx, ldx, nr, part);
+    hipLaunchKernelGGL(gemv_nr_reduce_kernel, dim3(nblk(n * nr, 256)), dim3(256), 0, st, (const double*)part, n,
+                       nchunk, nr, B + q0 * ldb, ldb, R + q0 * ldr, ldr);
+  }
   colnorm2(R, ldr, n, nrhs, nrm2, st);
   return hipGetLastError();
 }
@@ -675,10 +702,6 @@ template hipError_t apply_lu_precond<float>(const float*, int64_t, int64_t, cons
                                             float*, TrsvState&, hipStream_t);
 template hipError_t apply_lu_precond<double>(const double*, int64_t, int64_t, const int32_t*, const double*, double*,
                                              double*, TrsvState&, hipStream_t);
-template hipError_t lu_apply_multi<float>(const float*, int64_t, int64_t, const int32_t*, const double*, int64_t,
-                                          double*, int64_t, int64_t, bool, float*, hipStream_t);
-template hipError_t lu_apply_multi<double>(const double*, int64_t, int64_t, const int32_t*, const double*, int64_t,
-                                           double*, int64_t, int64_t, bool, double*, hipStream_t);
 template hipError_t fgmres_ir<float>(const double*, int64_t, int64_t, const float*, int64_t, const int32_t*,
                                      const double*, double*, const GmresOpts&, GmresResult&, hipStream_t);
 template hipError_t fgmres_ir<double>(const double*, int64_t, int64_t, const double*, int64_t, const int32_t*,

@@ -56,15 +56,10 @@ template <class T>
 hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* r, double* z,
                             T* work, TrsvState& s, hipStream_t st);
 
-// Many right-hand sides at once (n x nrhs, column-major): X (+)= U^-1 L^-1 P R with the factors in precision T,
-// through the blocked triangular solves; W holds n x nrhs elements of T.  perm == nullptr: no row permutation.
-template <class T>
-hipError_t lu_apply_multi(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* R, int64_t ldr,
-                          double* X, int64_t ldx, int64_t nrhs, bool accumulate, T* W, hipStream_t st);
-
 // One system of a batched application of FP32 LU factors: X (+)= U^-1 L^-1 P R for nrhs <= APPLY_NR columns.
-// R, X and W (FP32 work) are n x nrhs, column-major with leading dimension n.
-constexpr int APPLY_NR = 8;
+// R, X and W (FP32 work) are n x nrhs, column-major with leading dimension n.  A system with more right-hand sides
+// is split into several jobs, which then run on different CUs.
+constexpr int APPLY_NR = 2;
 struct LuApplyJob {
   const float* LU;       // unit lower L and U in place, n x n, leading dimension lda
   const int32_t* perm;   // row i of P A is row perm[i] of A
@@ -78,7 +73,9 @@ struct LuApplyJob {
 // All jobs (device array) in one launch, one workgroup per job.
 hipError_t lu_apply_batched(const LuApplyJob* jobs, int njobs, hipStream_t st);
 
-// R = B - A X (FP64, n x nrhs) and nrm2[q] = ||R[:, q]||^2; part holds gemv_part_bytes(n, n).
+// R = B - A X (FP64, n x nrhs) and nrm2[q] = ||R[:, q]||^2, A read once per 8 columns; part holds
+// residual_multi_bytes(n, nrhs).
+size_t residual_multi_bytes(int64_t n, int64_t nrhs);
 hipError_t residual_multi(const double* A, int64_t n, int64_t lda, const double* B, int64_t ldb, const double* X,
                           int64_t ldx, double* R, int64_t ldr, int64_t nrhs, double* part, double* nrm2,
                           hipStream_t st);
