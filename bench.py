@@ -10,10 +10,11 @@ Also reported: the FP64 LU solve of the same A (`A\\b`), the dominant kernel's r
 FP64 flops from the oracle's operation count per receiver/source pair) and the CPU baseline (the FP64 C++
 restatement of TD.jl, OpenMP over source triangles like TD.jl:146) on a bounded sample of receiver rows.
 
-python bench.py [--steps K] [--warmup W] [--gpus 1] [--batched] [--large]
+python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-batched] [--large] [--friction] [--crack-tip]
 
---batched adds the 64-crack batched assembly + solve, --large the 16224-triangle elliptical crack (A 48672^2 FP64:
-assembly, three solvers to 1e-10, grid field); both are reported in the same JSON line.
+The 64-crack batched assembly + solve runs by default (--no-batched skips it); --large adds the 16224-triangle
+elliptical crack (A 48672^2 FP64: assembly, three solvers to 1e-10, grid field), --friction and --crack-tip their
+cases; all are reported in the same JSON line.
 """
 import argparse
 import ctypes
@@ -104,6 +105,8 @@ def slip_solve_comparison(step_no):
     r["model"] = trained
     r["mixed_learned_best"] = ml
     r["speedup_mixed_learned_vs_fp64"] = plain / r[ml]["total_ms"] if ml else None
+    # whole solves (assembly included) per second of one GPU
+    r["solves_per_s"] = {"fp64": 1e3 / plain, "mixed_learned": 1e3 / r[ml]["total_ms"] if ml else None}
     r["fastest_mixed"] = best
     return r
 
@@ -111,8 +114,9 @@ def slip_solve_comparison(step_no):
 def batched_case():
     """64 planar elliptical cracks (N_i ~ U(200, 3000), 1-8 remote loads each, uniformly random orientation), all
     assembled by one persistent work-queue launch in a captured HIP graph with FP32 far / FP64 near entries, then
-    solved mesh by mesh (FP32 LU + FP64 GMRES-IR to 1e-10).  The first call tunes the launch and captures the
-    graph; the second replays it, and its figures are reported."""
+    solved together (the FP32 LUs of all meshes stepped in lockstep with shared panel, update and interchange
+    launches; multi-right-hand-side FP64 refinement rounds to 1e-10).  The first call tunes the launch and captures
+    the graph; the second replays it, and its figures are reported."""
     from cutanddisplacejulia_amd import api, meshes
     batch = meshes.elliptical_cracks()
     K, E, lam, nu, G = api.ElasticConstantsCheck(1.0, 0.25)
@@ -132,6 +136,8 @@ def batched_case():
            "rhs": int(sum(np.atleast_2d(m["stresses"]).shape[0] for m in batch)), "entries": entries,
            "assembly_ms": rep["assembly_ms"], "assembly_entries_per_s": entries / (rep["assembly_ms"] * 1e-3),
            "near_entries": rep["near_entries"], "solve_ms": rep["solve_ms"], "residual": rep["residual"],
+           "refinement_rounds": rep["iterations"],
+           "solves_per_s": sum(np.atleast_2d(m["stresses"]).shape[0] for m in batch) / (rep["solve_ms"] * 1e-3),
            "wall_ms": 1e3 * wall, "first_call_ms": 1e3 * first, "tile": rep["tile"]}
     _log("batched (64 cracks, %d triangles, %.3g entries): assembly %.1f ms = %.3e entries/s, solves %.0f ms, "
          "worst residual %.1e, wall %.0f ms (first call %.0f ms)" % (out["triangles"], entries, rep["assembly_ms"],
@@ -309,9 +315,11 @@ def main():
                     help="after the timed steps, copy this many MiB device-to-device (a kernel of known traffic, "
                          "to calibrate hardware byte counters against)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-solve", action="store_true")
-    ap.add_argument("--batched", action="store_true",
-                    help="also run the 64-crack batched assembly + solve (graph replay) and report it under 'batched'")
+    ap.add_argument("--no-solve", action="store_true", help="skip the FP64 LU of the assembled matrix and the "
+                    "slip-solve comparison")
+    ap.add_argument("--no-compare", action="store_true", help="skip the slip-solve comparison only")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 64-crack batched assembly + solve (graph replay), reported under 'batched'")
     ap.add_argument("--friction", action="store_true",
                     help="also time the frictional slip of the 497-triangle Burgmann case against the host oracle")
     ap.add_argument("--crack-tip", action="store_true",
@@ -413,10 +421,10 @@ def main():
         del Ab, A0
 
     solve = None
-    if not args.no_solve:
+    if not args.no_solve and not args.no_compare:
         solve = slip_solve_comparison(args.solve_step)
 
-    batched = batched_case() if args.batched else None
+    batched = None if args.no_batched else batched_case()
     large = large_case() if args.large else None
     crack_tip = crack_tip_case() if args.crack_tip else None
     friction = friction_case() if args.friction else None

@@ -268,8 +268,19 @@ __global__ void __launch_bounds__(256) gemv_nr_part_kernel(const double* __restr
 #pragma unroll
   for (int q = 0; q < RES_NR; ++q) acc[q] = 0.0;
   const double* a = A + i + c0 * lda;
-#pragma unroll 4
-  for (int c = 0; c < nc; ++c) {
+  int c = 0;
+  // 16 independent column loads in flight per thread, then their products
+  for (; c + 16 <= nc; c += 16) {
+    double v[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) v[t] = a[(c + t) * lda];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+      for (int q = 0; q < RES_NR; ++q)
+        if (q < nr) acc[q] += v[t] * xs[q][c + t];
+  }
+  for (; c < nc; ++c) {
     const double v = a[c * lda];
 #pragma unroll
     for (int q = 0; q < RES_NR; ++q)
