@@ -1099,21 +1099,31 @@ struct BatchPlan {
   int64_t nitems = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  // solve phase buffers for this batch shape
-  bool solve_layout = false, bufs_mixed = false, bufs_fp64 = false;
+  // solve phase: precision (CDJ_PREC_*), buffers for this batch shape, and for the mixed precisions the captured
+  // graph of the solve's fixed part (FP32 LUs, permutations, first solve and one correction of every mesh)
+  int solve_prec = CDJ_PREC_FP64;
+  bool solve_layout = false, bufs_mixed = false, bufs_fp64 = false, device_perm = false;
   std::vector<int64_t> a32_off, v_off, q_off;  // per mesh: FP32 copy of A, pivots / permutation, column norms
   std::vector<size_t> ws_off;                  // per mesh: LU workspace
-  int64_t a32_total = 0, v_total = 0, q_total = 0;
+  int64_t a32_total = 0, v_total = 0, q_total = 0, n3_max = 0, napply = 0;
   size_t ws_total = 0, part_bytes = 0;
-  DBuf A32, ipiv, perm, info, W, R, nrm, lu_ws, part, jobs;
+  // nrm: ||R||^2 per column (latest residual), ||B||^2 per column, ||R||^2 after the first solve
+  // jobs0 / jobs1: factor applications to B (X = M^-1 B) and to R (X += M^-1 R); jobsx: a round's subset
+  DBuf A32, ipiv, perm, info, W, R, nrm, lu_ws, part, jobs0, jobs1, jobsx, pjobs;
+  std::vector<LuApplyJob> j1_host;  // host copy of jobs1
+  std::vector<int64_t> jfirst;      // mesh m's applications are jobs [jfirst[m], jfirst[m + 1])
+  hipGraph_t sgraph = nullptr;
+  hipGraphExec_t sexec = nullptr;
   ~BatchPlan() {
+    if (sexec) (void)hipGraphExecDestroy(sexec);
+    if (sgraph) (void)hipGraphDestroy(sgraph);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
   }
   bool same(const std::vector<int64_t>& n2, const std::vector<int64_t>& r2, int hs2, bool mixed2, double f2,
-            double nu2, double lam2, double G2) const {
+            double nu2, double lam2, double G2, int prec2) const {
     return n == n2 && nrhs == r2 && hs == hs2 && mixed == mixed2 && near_factor == f2 && nu == nu2 && lam == lam2 &&
-           G == G2;
+           G == G2 && solve_prec == prec2;
   }
 };
 
@@ -1216,13 +1226,13 @@ int batch_tune(cdj_ctx* ctx, BatchPlan& p, hipStream_t st) {
 }
 
 int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bool mixed, double f, double nu,
-                     double lam, double G, hipStream_t st) {
+                     double lam, double G, int prec, hipStream_t st) {
   std::vector<int64_t> n(nmesh), r(nmesh);
   for (int64_t m = 0; m < nmesh; ++m) {
     n[m] = ms[m].n;
     r[m] = ms[m].nrhs;
   }
-  if (ctx->batch && ctx->batch->same(n, r, hs, mixed, f, nu, lam, G)) return CDJ_OK;
+  if (ctx->batch && ctx->batch->same(n, r, hs, mixed, f, nu, lam, G, prec)) return CDJ_OK;
   batch_plan_free(ctx->batch);
   ctx->batch = new BatchPlan();
   BatchPlan& p = *ctx->batch;
@@ -1231,6 +1241,7 @@ int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bo
   p.hs = hs;
   p.mixed = mixed;
   p.near_factor = f;
+  p.solve_prec = prec;
   p.nu = nu;
   p.lam = lam;
   p.G = G;
@@ -1305,10 +1316,37 @@ int batch_solve_setup(cdj_ctx* ctx, BatchPlan& p, bool mixed) {
     HIPCHK(p.perm.alloc(sizeof(int32_t) * p.v_total));
     HIPCHK(p.W.alloc(sizeof(float) * p.b_total));
     HIPCHK(p.R.alloc(sizeof(double) * p.b_total));
-    HIPCHK(p.nrm.alloc(sizeof(double) * 2 * p.q_total));
-    int64_t nj = 0;
-    for (int64_t m = 0; m < nmesh; ++m) nj += (p.nrhs[m] + APPLY_NR - 1) / APPLY_NR;
-    HIPCHK(p.jobs.alloc(sizeof(LuApplyJob) * nj));
+    HIPCHK(p.nrm.alloc(sizeof(double) * 3 * p.q_total));
+    // factor-application and permutation job tables, fixed for the plan (uploaded once, outside any capture)
+    std::vector<LuApplyJob> j0, j1;
+    std::vector<PermJob> pj;
+    for (int64_t m = 0; m < nmesh; ++m) {
+      const int64_t N3 = 3 * p.n[m], nr = p.nrhs[m];
+      float* LUm = p.A32.as<float>() + p.a32_off[m];
+      int32_t* pm = p.perm.as<int32_t>() + p.v_off[m];
+      pj.push_back(PermJob{p.ipiv.as<int32_t>() + p.v_off[m], pm, N3});
+      for (int64_t q0 = 0; q0 < nr; q0 += APPLY_NR) {
+        const int64_t o = p.b_off[m] + q0 * N3;
+        const int32_t nq = (int32_t)std::min<int64_t>(APPLY_NR, nr - q0);
+        j0.push_back(LuApplyJob{LUm, pm, p.B.as<double>() + o, p.X.as<double>() + o, p.W.as<float>() + o, N3, N3, nq, 0});
+        j1.push_back(LuApplyJob{LUm, pm, p.R.as<double>() + o, p.X.as<double>() + o, p.W.as<float>() + o, N3, N3, nq, 1});
+      }
+    }
+    p.napply = (int64_t)j0.size();
+    p.j1_host = j1;
+    p.jfirst.assign(nmesh + 1, 0);
+    for (int64_t m = 0; m < nmesh; ++m) p.jfirst[m + 1] = p.jfirst[m] + (p.nrhs[m] + APPLY_NR - 1) / APPLY_NR;
+    HIPCHK(p.jobs0.alloc(sizeof(LuApplyJob) * j0.size()));
+    HIPCHK(p.jobs1.alloc(sizeof(LuApplyJob) * j1.size()));
+    HIPCHK(p.jobsx.alloc(sizeof(LuApplyJob) * j1.size()));
+    HIPCHK(p.pjobs.alloc(sizeof(PermJob) * pj.size()));
+    HIPCHK(hipMemcpy(p.jobs0.p, j0.data(), sizeof(LuApplyJob) * j0.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.jobs1.p, j1.data(), sizeof(LuApplyJob) * j1.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p.pjobs.p, pj.data(), sizeof(PermJob) * pj.size(), hipMemcpyHostToDevice));
+    for (int64_t m = 0; m < nmesh; ++m) p.n3_max = std::max(p.n3_max, 3 * p.n[m]);
+    // the one-time kernel set-ups happen here, never inside a capture
+    if (lu_batched_capacity<float>() <= 0) return fail(ctx, CDJ_ERR_DEVICE, "batched LU set-up failed");
+    p.device_perm = p.n3_max <= pivots_to_perm_max();
     p.bufs_mixed = true;
   }
   return CDJ_OK;
@@ -1382,109 +1420,115 @@ int batch_solve_fp64(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_t
 // rounds of X += M^-1 (B - A X) with the FP64 residual, until every column reaches the tolerance.  A mesh whose
 // residual stops falling (less than halved by a round) or whose FP32 factor has a zero pivot moves to FGMRES-IR
 // per column, and from there to the FP64 LU, as in the single-mesh solve.
-int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_t nmesh, const cdj_opts* opts,
-                      cdj_report* reports, double* solve_ms, double* worst, int* iters, int* path, hipStream_t st) {
-  if (int rc = batch_solve_setup(ctx, p, true)) return rc;
-  GmresOpts go;
-  if (opts && opts->tol > 0) go.tol = opts->tol;
-  if (opts && opts->max_iter > 0) go.max_iter = opts->max_iter;
-  const bool fp16 = opts && opts->precision == CDJ_PREC_MIXED16;
+//
+// The fixed part (FP32 copies and LUs of all meshes, the permutations, the first solve and one correction, each
+// followed by its FP64 residual) is only kernel launches on st: with device_perm it is captured in the plan's solve
+// graph.  Without it (a mesh above PERM_MAX unknowns) the permutations come from the host between the two halves.
+hipError_t batch_solve_lu(BatchPlan& p, hipStream_t st) {
   const std::vector<int64_t> order = batch_solve_order(p);
-  float* A32 = p.A32.as<float>();
-  int32_t* ipiv = p.ipiv.as<int32_t>();
-  int32_t* perm = p.perm.as<int32_t>();
-  double* nrm = p.nrm.as<double>();  // ||R||^2 per column, then ||B||^2 per column
-  auto Am = [&](int64_t m) { return p.A.as<double>() + p.a_off[m]; };
-  auto Bm = [&](int64_t m) { return p.B.as<double>() + p.b_off[m]; };
-  auto Xm = [&](int64_t m) { return p.X.as<double>() + p.b_off[m]; };
-  auto Rm = [&](int64_t m) { return p.R.as<double>() + p.b_off[m]; };
-  auto Wm = [&](int64_t m) { return p.W.as<float>() + p.b_off[m]; };
-  double* part = p.part.as<double>();
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, st));
+  double* nrm = p.nrm.as<double>();
   std::vector<LuJob<float>> jobs;
   for (int64_t m : order) {
-    const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
-    convert_matrix_d2f(Am(m), N3, A32 + p.a32_off[m], N3, N3, N3, st);
-    colnorm2(Bm(m), N3, N3, nr, nrm + p.q_total + p.q_off[m], st);
-    jobs.push_back(LuJob<float>{A32 + p.a32_off[m], N3, N3, N3, ipiv + p.v_off[m], p.info.as<int>() + m,
-                                p.lu_ws.as<char>() + p.ws_off[m]});
+    const int64_t N3 = 3 * p.n[m];
+    convert_matrix_d2f(p.A.as<double>() + p.a_off[m], N3, p.A32.as<float>() + p.a32_off[m], N3, N3, N3, st);
+    colnorm2(p.B.as<double>() + p.b_off[m], N3, N3, p.nrhs[m], nrm + p.q_total + p.q_off[m], st);
+    jobs.push_back(LuJob<float>{p.A32.as<float>() + p.a32_off[m], N3, N3, N3, p.ipiv.as<int32_t>() + p.v_off[m],
+                                p.info.as<int>() + m, p.lu_ws.as<char>() + p.ws_off[m]});
   }
-  HIPCHK(lu_factor_batched<float>(jobs.data(), (int)jobs.size(), st, fp16));
-  std::vector<int> info(nmesh);
+  return lu_factor_batched<float>(jobs.data(), (int)jobs.size(), st, p.solve_prec == CDJ_PREC_MIXED16);
+}
+
+hipError_t batch_residuals(BatchPlan& p, double* nrm_out, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  for (size_t m = 0; e == hipSuccess && m < p.n.size(); ++m) {
+    const int64_t N3 = 3 * p.n[m];
+    const int64_t o = p.b_off[m];
+    e = residual_multi(p.A.as<double>() + p.a_off[m], N3, N3, p.B.as<double>() + o, N3, p.X.as<double>() + o, N3,
+                       p.R.as<double>() + o, N3, p.nrhs[m], p.part.as<double>(), nrm_out + p.q_off[m], st);
+  }
+  return e;
+}
+
+hipError_t batch_solve_apply(BatchPlan& p, hipStream_t st) {
+  double* nrm = p.nrm.as<double>();
+  hipError_t e = lu_apply_batched(p.jobs0.as<LuApplyJob>(), (int)p.napply, st);
+  if (e == hipSuccess) e = batch_residuals(p, nrm + 2 * p.q_total, st);
+  if (e == hipSuccess) e = lu_apply_batched(p.jobs1.as<LuApplyJob>(), (int)p.napply, st);
+  if (e == hipSuccess) e = batch_residuals(p, nrm, st);
+  return e;
+}
+
+// the whole fixed part, for the capture
+hipError_t batch_solve_enqueue(BatchPlan& p, hipStream_t st) {
+  hipError_t e = batch_solve_lu(p, st);
+  if (e == hipSuccess)
+    e = pivots_to_perm(p.pjobs.as<PermJob>(), (int)p.n.size(), p.n3_max, st);
+  if (e == hipSuccess) e = batch_solve_apply(p, st);
+  return e;
+}
+
+// The permutations on the host (for meshes too large for the device kernel): synchronises the stream.
+int batch_host_perms(cdj_ctx* ctx, BatchPlan& p, hipStream_t st) {
   std::vector<int32_t> hp(p.v_total);
-  HIPCHK(hipMemcpyAsync(info.data(), p.info.p, sizeof(int) * nmesh, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(hp.data(), ipiv, sizeof(int32_t) * p.v_total, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hp.data(), p.ipiv.p, sizeof(int32_t) * p.v_total, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (int rc = batch_lu_status(ctx, info, false)) return rc;
-  // row permutations of the pivot sequences (row i of P A is row perm[i] of A)
-  for (int64_t m = 0; m < nmesh; ++m) {
-    const int64_t N3 = 3 * meshes[m].n;
+  for (size_t m = 0; m < p.n.size(); ++m) {
+    const int64_t N3 = 3 * p.n[m];
     int32_t* v = hp.data() + p.v_off[m];
     std::vector<int32_t> q(N3);
     for (int64_t i = 0; i < N3; ++i) q[i] = (int32_t)i;
     for (int64_t i = 0; i < N3; ++i) std::swap(q[i], q[v[i]]);
     std::copy(q.begin(), q.end(), v);
   }
-  HIPCHK(hipMemcpyAsync(perm, hp.data(), sizeof(int32_t) * p.v_total, hipMemcpyHostToDevice, st));
-  // 0: refining, 1: converged, 2: to the fallback
+  HIPCHK(hipMemcpyAsync(p.perm.p, hp.data(), sizeof(int32_t) * p.v_total, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
+// After the fixed part: status, refinement rounds while needed, fallbacks, reports.
+int batch_solve_mixed_finish(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_t nmesh,
+                             const cdj_opts* opts, cdj_report* reports, double* worst, int* iters, int* path,
+                             hipStream_t st) {
+  GmresOpts go;
+  if (opts && opts->tol > 0) go.tol = opts->tol;
+  if (opts && opts->max_iter > 0) go.max_iter = opts->max_iter;
+  const bool fp16 = p.solve_prec == CDJ_PREC_MIXED16;
+  const std::vector<int64_t> order = batch_solve_order(p);
+  double* nrm = p.nrm.as<double>();
+  std::vector<int> info(nmesh);
+  std::vector<double> hn(3 * p.q_total);
+  HIPCHK(hipMemcpyAsync(info.data(), p.info.p, sizeof(int) * nmesh, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 3 * p.q_total, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (int rc = batch_lu_status(ctx, info, false)) return rc;
+  // worst relative residual of a mesh's columns from a norm section (NaN propagates)
+  auto rel_of = [&](int64_t m, int64_t section) {
+    double rel = 0.0;
+    for (int64_t q = 0; q < meshes[m].nrhs; ++q) {
+      const double rr = hn[section * p.q_total + p.q_off[m] + q], bb = hn[p.q_total + p.q_off[m] + q];
+      const double r = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+      rel = std::isnan(r) || std::isnan(rel) ? NAN : std::max(rel, r);
+    }
+    return rel;
+  };
+  // 0: refining, 1: converged, 2: to the fallback.  Every mesh has had its first solve and one correction.
   std::vector<char> state(nmesh, 0);
-  std::vector<int> rounds(nmesh, 0);
+  std::vector<int> rounds(nmesh, 1);
   std::vector<double> res(nmesh, 0.0), prev(nmesh, HUGE_VAL);
-  for (int64_t m = 0; m < nmesh; ++m)
-    if (info[m] != 0) state[m] = 2;
-  // the factor applications of a round: one workgroup per mesh and group of at most APPLY_NR columns, all meshes
-  // in one launch; then the FP64 residuals and their column norms
+  for (int64_t m = 0; m < nmesh; ++m) {
+    if (info[m] != 0) {
+      state[m] = 2;
+      rounds[m] = 0;
+    } else {
+      prev[m] = rel_of(m, 2);
+    }
+  }
   std::vector<LuApplyJob> hj;
-  auto add_jobs = [&](int64_t m, const double* Rsrc, bool accumulate) {
-    const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
-    for (int64_t q0 = 0; q0 < nr; q0 += APPLY_NR) {
-      LuApplyJob j;
-      j.LU = A32 + p.a32_off[m];
-      j.perm = perm + p.v_off[m];
-      j.R = Rsrc + q0 * N3;
-      j.X = Xm(m) + q0 * N3;
-      j.W = Wm(m) + q0 * N3;
-      j.n = N3;
-      j.lda = N3;
-      j.nrhs = (int32_t)std::min<int64_t>(APPLY_NR, nr - q0);
-      j.accumulate = accumulate ? 1 : 0;
-      hj.push_back(j);
-    }
-  };
-  auto run_round = [&](const std::vector<int64_t>& ms) -> hipError_t {
-    hipError_t e = hipMemcpyAsync(p.jobs.p, hj.data(), sizeof(LuApplyJob) * hj.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = lu_apply_batched(p.jobs.as<LuApplyJob>(), (int)hj.size(), st);
-    for (size_t k = 0; e == hipSuccess && k < ms.size(); ++k) {
-      const int64_t m = ms[k], N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
-      e = residual_multi(Am(m), N3, N3, Bm(m), N3, Xm(m), N3, Rm(m), N3, nr, part, nrm + p.q_off[m], st);
-    }
-    return e;
-  };
-  std::vector<int64_t> live;
-  for (int64_t m : order)
-    if (!state[m]) {
-      add_jobs(m, Bm(m), false);
-      live.push_back(m);
-    }
-  HIPCHK(run_round(live));
-  std::vector<double> hn(2 * p.q_total);
-  for (int round = 0;; ++round) {
-    HIPCHK(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * p.q_total, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+  for (int round = 1;; ++round) {
     hj.clear();
-    live.clear();
     for (int64_t m : order) {
       if (state[m]) continue;
-      const int64_t nr = meshes[m].nrhs;
-      double rel = 0.0;
-      for (int64_t q = 0; q < nr; ++q) {
-        const double rr = hn[p.q_off[m] + q], bb = hn[p.q_total + p.q_off[m] + q];
-        const double r = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-        rel = std::isnan(r) || std::isnan(rel) ? NAN : std::max(rel, r);
-      }
+      const double rel = rel_of(m, 0);
       res[m] = rel;
       if (rel <= go.tol) {
         state[m] = 1;
@@ -1496,24 +1540,35 @@ int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_
       }
       prev[m] = rel;
       rounds[m]++;
-      add_jobs(m, Rm(m), true);
-      live.push_back(m);
+      for (int64_t t = p.jfirst[m]; t < p.jfirst[m + 1]; ++t) hj.push_back(p.j1_host[t]);
     }
-    if (live.empty()) break;
-    HIPCHK(run_round(live));
+    if (hj.empty()) break;
+    HIPCHK(hipMemcpyAsync(p.jobsx.p, hj.data(), sizeof(LuApplyJob) * hj.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(lu_apply_batched(p.jobsx.as<LuApplyJob>(), (int)hj.size(), st));
+    for (int64_t m : order) {
+      if (state[m]) continue;
+      const int64_t N3 = 3 * meshes[m].n, o = p.b_off[m];
+      HIPCHK(residual_multi(p.A.as<double>() + p.a_off[m], N3, N3, p.B.as<double>() + o, N3, p.X.as<double>() + o, N3,
+                            p.R.as<double>() + o, N3, meshes[m].nrhs, p.part.as<double>(), nrm + p.q_off[m], st));
+    }
+    HIPCHK(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * p.q_total, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
   }
-  // fallbacks, one mesh at a time on the context's stream
+  // fallbacks, one mesh at a time
   std::vector<int> mpath(nmesh, fp16 ? CDJ_PREC_MIXED16 : CDJ_PREC_MIXED32);
   std::vector<int> fit(nmesh, 0);
   for (int64_t m = 0; m < nmesh; ++m) {
     if (state[m] != 2) continue;
     const int64_t N3 = 3 * meshes[m].n, nr = meshes[m].nrhs;
+    double* Am = p.A.as<double>() + p.a_off[m];
+    double* Bm = p.B.as<double>() + p.b_off[m];
+    double* Xm = p.X.as<double>() + p.b_off[m];
     bool ok = info[m] == 0;
     double w = 0.0;
     for (int64_t q = 0; ok && q < nr; ++q) {
       GmresResult gr;
-      HIPCHK(fgmres_ir<float>(Am(m), N3, N3, A32 + p.a32_off[m], N3, perm + p.v_off[m], Bm(m) + q * N3,
-                              Xm(m) + q * N3, go, gr, st));
+      HIPCHK(fgmres_ir<float>(Am, N3, N3, p.A32.as<float>() + p.a32_off[m], N3, p.perm.as<int32_t>() + p.v_off[m],
+                              Bm + q * N3, Xm + q * N3, go, gr, st));
       ok = gr.converged && !gr.trsv_timeout;
       w = std::max(w, gr.residual);
       fit[m] += gr.iterations;
@@ -1524,21 +1579,14 @@ int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_
     }
     DBuf AB;
     HIPCHK(AB.alloc(sizeof(double) * N3 * (N3 + nr)));
-    HIPCHK(hipMemcpyAsync(AB.p, Am(m), sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(AB.as<double>() + N3 * N3, Bm(m), sizeof(double) * N3 * nr, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(AB.p, Am, sizeof(double) * N3 * N3, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(AB.as<double>() + N3 * N3, Bm, sizeof(double) * N3 * nr, hipMemcpyDeviceToDevice, st));
     if (int rc = solve_augmented_fp64(ctx, AB.as<double>(), N3, N3, nr, nullptr, nullptr, nullptr, st)) return rc;
-    HIPCHK(hipMemcpyAsync(Xm(m), AB.as<double>() + N3 * N3, sizeof(double) * N3 * nr, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(Xm, AB.as<double>() + N3 * N3, sizeof(double) * N3 * nr, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     mpath[m] = CDJ_PREC_FP64;
     res[m] = -1.0;
   }
-  HIPCHK(hipEventRecord(e1, st));
-  HIPCHK(hipEventSynchronize(e1));
-  float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  *solve_ms = ms;
   for (int64_t m = 0; m < nmesh; ++m) {
     *worst = std::max(*worst, res[m]);
     *iters += rounds[m] + fit[m];
@@ -1547,7 +1595,6 @@ int batch_solve_mixed(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes, int64_
       reports[m].path = mpath[m];
       reports[m].iterations = rounds[m] + fit[m];
       reports[m].residual = res[m];
-      reports[m].solve_ms = ms;
     }
   }
   return CDJ_OK;
@@ -1578,9 +1625,12 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
   hipStream_t st = ctx->stream;
   if (nmesh == 0) return CDJ_OK;
   const int hs = hs_flag == 1;
-  if (int rc = batch_plan_build(ctx, meshes, nmesh, hs, mixed, near_factor, nu, lambda, G, st)) return rc;
+  const int prec = opts && (opts->precision == CDJ_PREC_MIXED32 || opts->precision == CDJ_PREC_MIXED16)
+                       ? opts->precision
+                       : CDJ_PREC_FP64;
+  if (int rc = batch_plan_build(ctx, meshes, nmesh, hs, mixed, near_factor, nu, lambda, G, prec, st)) return rc;
   BatchPlan& p = *ctx->batch;
-  // geometry: five N x 3 column-major arrays over the concatenated triangles
+  // geometry: five N x 3 column-major arrays over the concatenated triangles; the right-hand sides
   {
     std::vector<double> h(15 * (size_t)p.ntot);
     for (int64_t m = 0; m < nmesh; ++m) {
@@ -1592,44 +1642,70 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
                       sizeof(double) * M.n);
     }
     HIPCHK(hipMemcpyAsync(p.geo.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, st));
+    for (int64_t m = 0; m < nmesh; ++m)
+      HIPCHK(hipMemcpyAsync(p.B.as<double>() + p.b_off[m], meshes[m].b,
+                            sizeof(double) * 3 * meshes[m].n * meshes[m].nrhs, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
   }
+  // graphs: the assembly (after tuning); for the mixed precisions, the solve's fixed part as a second graph
+  auto capture = [&](auto&& enqueue, hipGraph_t* g, hipGraphExec_t* x, const char* what) -> int {
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = enqueue();
+    hipError_t ee = hipStreamEndCapture(st, g);
+    if (ce != hipSuccess) return hip_fail(ctx, ce, what);
+    if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+    HIPCHK(hipGraphInstantiate(x, *g, nullptr, nullptr, 0));
+    return CDJ_OK;
+  };
   if (!p.exec) {
     if (int rc = batch_tune(ctx, p, st)) return rc;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t ce = batch_enqueue(p, st);
-    hipGraph_t g = nullptr;
-    hipError_t ee = hipStreamEndCapture(st, &g);
-    if (ce != hipSuccess) return hip_fail(ctx, ce, "batch capture");
-    if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
-    p.graph = g;
-    HIPCHK(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+    if (int rc = capture([&] { return batch_enqueue(p, st); }, &p.graph, &p.exec, "batch capture")) return rc;
   }
-  hipEvent_t e0, e1;
+  const bool mixed_solve = prec != CDJ_PREC_FP64;
+  if (mixed_solve) {
+    if (int rc = batch_solve_setup(ctx, p, true)) return rc;
+    if (p.device_perm && !p.sexec)
+      if (int rc = capture([&] { return batch_solve_enqueue(p, st); }, &p.sgraph, &p.sexec, "batch solve capture"))
+        return rc;
+  }
+  hipEvent_t e0, e1, e2;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventCreate(&e2));
   HIPCHK(hipEventRecord(e0, st));
   HIPCHK(hipGraphLaunch(p.exec, st));
   HIPCHK(hipEventRecord(e1, st));
   unsigned long long cnt[4] = {0, 0, 0, 0};
-  HIPCHK(hipMemcpyAsync(cnt, p.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  float ams = 0.f;
-  (void)hipEventElapsedTime(&ams, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  ctx->near_entries = (int64_t)cnt[2];
-  // right-hand sides to the device, then the solves of all meshes together
-  for (int64_t m = 0; m < nmesh; ++m)
-    HIPCHK(hipMemcpyAsync(p.B.as<double>() + p.b_off[m], meshes[m].b, sizeof(double) * 3 * meshes[m].n * meshes[m].nrhs,
-                          hipMemcpyHostToDevice, st));
   double solve_ms = 0.0, worst = 0.0;
   int iters = 0, path = CDJ_PREC_FP64;
-  if (opts && (opts->precision == CDJ_PREC_MIXED32 || opts->precision == CDJ_PREC_MIXED16)) {
-    if (int rc = batch_solve_mixed(ctx, p, meshes, nmesh, opts, reports, &solve_ms, &worst, &iters, &path, st))
-      return rc;
+  int rc = CDJ_OK;
+  if (mixed_solve) {
+    if (p.sexec) {
+      HIPCHK(hipGraphLaunch(p.sexec, st));
+    } else {
+      HIPCHK(batch_solve_lu(p, st));
+      rc = batch_host_perms(ctx, p, st);
+      if (!rc) HIPCHK(batch_solve_apply(p, st));
+    }
+    if (!rc) rc = batch_solve_mixed_finish(ctx, p, meshes, nmesh, opts, reports, &worst, &iters, &path, st);
+    if (!rc) HIPCHK(hipEventRecord(e2, st));
+  }
+  HIPCHK(hipMemcpyAsync(cnt, p.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float ams = 0.f, sms = 0.f;
+  (void)hipEventElapsedTime(&ams, e0, e1);
+  if (mixed_solve && !rc) (void)hipEventElapsedTime(&sms, e1, e2);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  if (rc) return rc;
+  ctx->near_entries = (int64_t)cnt[2];
+  if (mixed_solve) {
+    solve_ms = sms;
+    if (reports)
+      for (int64_t m = 0; m < nmesh; ++m) reports[m].solve_ms = solve_ms;
   } else {
-    if (int rc = batch_solve_fp64(ctx, p, meshes, nmesh, reports, &solve_ms, st)) return rc;
+    if (int rc2 = batch_solve_fp64(ctx, p, meshes, nmesh, reports, &solve_ms, st)) return rc2;
   }
   for (int64_t m = 0; m < nmesh; ++m) {
     HIPCHK(hipMemcpyAsync(meshes[m].x, p.X.as<double>() + p.b_off[m], sizeof(double) * 3 * meshes[m].n * meshes[m].nrhs,

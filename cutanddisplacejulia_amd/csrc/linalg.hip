@@ -953,6 +953,19 @@ __global__ void kkt_kernel(const double* __restrict__ A, int64_t lda, int64_t m,
 }
 
 // info = -1 when a panel barrier wait overran (the factors are then invalid)
+// A matrix's factorization state before its first panel: info and the wait flags cleared, slots and records all
+// ones (tag 4095 / 2^32 - 1 match no column).  A kernel rather than memsets, so the factorization can be captured
+// in a graph.
+__global__ void lu_init_kernel(int* info, unsigned* bar, unsigned long long* slots, int64_t nslots) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t == 0) {
+    *info = 0;
+    bar[0] = 0;
+    bar[1] = 0;
+  }
+  for (int64_t i = t; i < nslots; i += (int64_t)gridDim.x * blockDim.x) slots[i] = ~0ull;
+}
+
 __global__ void barrier_status_kernel(const int* bar_err, int* info) {
   if (*bar_err) *info = -1;
 }
@@ -1137,6 +1150,28 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   return hipGetLastError();
 }
 
+// Co-resident workgroups of the batched panel kernel (what one cooperative launch may hold); the first call also
+// raises the kernel's dynamic LDS limit.  Thread-safe one-time set-up.
+template <class T>
+int lu_batched_capacity() {
+  static const int capacity = [] {
+    const int lds_max = (int)(sizeof(T) * ((size_t)MAX_RPB * LU_NB + USH));
+    hipError_t r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, 2>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    if (r == hipSuccess)
+      r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
+    int dev = 0, cus = 0, per_cu = 0;
+    if (r == hipSuccess) r = hipGetDevice(&dev);
+    if (r == hipSuccess) r = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (r == hipSuccess)
+      r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
+                                                       PT, lds_max);
+    return r == hipSuccess ? per_cu * cus : 0;
+  }();
+  return capacity;
+}
+
 // Several matrices factored together, each exactly as lu_factor_cols would factor it alone (the same operations
 // in the same order, so the same factors bit for bit), but stepped in lockstep by panel column so that the panels
 // of all matrices at column k share cooperative launches: a panel's cost is the latency chain of its 64 columns,
@@ -1168,28 +1203,11 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
     v.bar_err = reinterpret_cast<int*>(v.bar + 1);
     v.perm = v.bar_err + 1;
     v.sigma = reinterpret_cast<float*>(v.perm + 1 + 4 * LU_NB);
-    hipError_t e = hipMemsetAsync(J.info, 0, sizeof(int), st);
-    if (e == hipSuccess) e = hipMemsetAsync(v.bar, 0, 2 * sizeof(unsigned), st);
-    if (e == hipSuccess) e = hipMemsetAsync(v.slots, 0xFF, v.xbytes, st);
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(lu_init_kernel, dim3(64), dim3(256), 0, st, J.info, v.bar, v.slots,
+                       (int64_t)(v.xbytes / sizeof(unsigned long long)));
     nmax = std::max(nmax, J.n);
   }
-  const int lds_max = (int)(sizeof(T) * ((size_t)MAX_RPB * LU_NB + USH));
-  // co-resident panel workgroups: what one cooperative launch may hold
-  static const int capacity = [lds_max] {
-    hipError_t r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, 2>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-    if (r == hipSuccess)
-      r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-    int dev = 0, cus = 0, per_cu = 0;
-    if (r == hipSuccess) r = hipGetDevice(&dev);
-    if (r == hipSuccess) r = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (r == hipSuccess)
-      r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
-                                                       PT, lds_max);
-    return r == hipSuccess ? per_cu * cus : 0;
-  }();
+  const int capacity = lu_batched_capacity<T>();
   if (capacity <= 0) return hipErrorInvalidConfiguration;
   std::vector<int> act;
   GemmList<T> gl;
@@ -1451,6 +1469,8 @@ void convert_matrix_d2f(const double* A, int64_t lda, float* B, int64_t ldb, int
   if (m > 0 && n > 0) hipLaunchKernelGGL(d2f_matrix_kernel, dim3(nblk(m, 256), (unsigned)n), dim3(256), 0, st, A, lda, B, ldb, m, n);
 }
 
+template int lu_batched_capacity<double>();
+template int lu_batched_capacity<float>();
 template hipError_t lu_factor_batched<double>(const LuJob<double>*, int, hipStream_t, bool);
 template hipError_t lu_factor_batched<float>(const LuJob<float>*, int, hipStream_t, bool);
 template hipError_t lu_factor<double>(double*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);

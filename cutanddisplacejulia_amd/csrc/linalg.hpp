@@ -39,9 +39,15 @@ struct LuJob {
 };
 
 // Factors every job's matrix as lu_factor_cols would, with the panels of all matrices at the same column sharing
-// cooperative launches.  fp16_update applies to the jobs with ncols == n.
+// cooperative launches.  fp16_update applies to the jobs with ncols == n.  Only kernel launches are enqueued (no
+// memsets, no host synchronisation), so the sequence can be captured in a graph once lu_batched_capacity<T>() has
+// been called outside the capture.
 template <class T>
 hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bool fp16_update = false);
+
+// Workgroups one batched panel launch may hold (0 on failure); one-time set-up of the panel kernel's LDS limit.
+template <class T>
+int lu_batched_capacity();
 
 // Row permutation implied by the pivots (row i of P A is row perm[i] of A); perm_dev is a device array.
 // Synchronises the stream.

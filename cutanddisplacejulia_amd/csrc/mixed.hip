@@ -301,6 +301,31 @@ __global__ void __launch_bounds__(256) gemv_nr_reduce_kernel(const double* __res
   R[i + q * ldr] = B[i + q * ldb] - s;
 }
 
+// ---- row permutations from pivot sequences, one workgroup per system ---------------------------------------
+// perm[i] = row of A that ends in row i of P A: the pivots' interchanges applied in order to the identity.  The
+// sequence is inherently serial; it runs in LDS (the pivots staged first), one thread, all systems in parallel.
+__global__ void __launch_bounds__(256) pivots_to_perm_kernel(const PermJob* __restrict__ jobs) {
+  extern __shared__ int32_t psh[];
+  const PermJob J = jobs[blockIdx.x];
+  const int n = (int)J.n;
+  int32_t* p = psh;
+  int32_t* v = psh + n;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    p[i] = i;
+    v[i] = J.ipiv[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 0; i < n; ++i) {
+      const int j = v[i];
+      const int32_t t = p[i];
+      p[i] = p[j];
+      p[j] = t;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 256) J.perm[i] = p[i];
+}
+
 // ---- batched factor application: one workgroup per system -------------------------------------------------
 // X (+)= U^-1 L^-1 P R for up to APPLY_NR columns of one system per workgroup.  The workgroup walks the
 // 64-row blocks itself: the diagonal block is staged in LDS and solved by one wave per column, then all 1024
@@ -473,6 +498,23 @@ hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* 
 }
 
 // ---- many right-hand sides ----------------------------------------------------------------------------------
+int64_t pivots_to_perm_max() {
+  static const int64_t nmax = [] {
+    const int64_t n = PERM_MAX;
+    hipError_t r = hipFuncSetAttribute((const void*)pivots_to_perm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(2 * sizeof(int32_t) * n));
+    return r == hipSuccess ? n : (int64_t)0;
+  }();
+  return nmax;
+}
+
+hipError_t pivots_to_perm(const PermJob* jobs, int njobs, int64_t nmax, hipStream_t st) {
+  if (njobs <= 0) return hipSuccess;
+  if (nmax > pivots_to_perm_max()) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(pivots_to_perm_kernel, dim3((unsigned)njobs), dim3(256), 2 * sizeof(int32_t) * nmax, st, jobs);
+  return hipGetLastError();
+}
+
 hipError_t lu_apply_batched(const LuApplyJob* jobs, int njobs, hipStream_t st) {
   if (njobs <= 0) return hipSuccess;
   hipLaunchKernelGGL(lu_apply_batched_kernel, dim3((unsigned)njobs), dim3(APPLY_THREADS), 0, st, jobs);

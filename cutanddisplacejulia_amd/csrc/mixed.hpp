@@ -73,6 +73,19 @@ struct LuApplyJob {
 // All jobs (device array) in one launch, one workgroup per job.
 hipError_t lu_apply_batched(const LuApplyJob* jobs, int njobs, hipStream_t st);
 
+// perm (device, n) from the 0-based pivot rows ipiv of an LU: row i of P A is row perm[i] of A.
+struct PermJob {
+  const int32_t* ipiv;
+  int32_t* perm;
+  int64_t n;
+};
+constexpr int64_t PERM_MAX = 16384;  // largest n (the pivots and the permutation are staged in LDS)
+// One-time set-up of the kernel's LDS limit; returns the largest n supported (0 on failure).  Call it outside any
+// graph capture.
+int64_t pivots_to_perm_max();
+// All jobs (device array) in one launch; nmax >= every job's n.  Kernel launches only (graph-capturable).
+hipError_t pivots_to_perm(const PermJob* jobs, int njobs, int64_t nmax, hipStream_t st);
+
 // R = B - A X (FP64, n x nrhs) and nrm2[q] = ||R[:, q]||^2, A read once per 8 columns; part holds
 // residual_multi_bytes(n, nrhs).
 size_t residual_multi_bytes(int64_t n, int64_t nrhs);

@@ -58,6 +58,35 @@ def test_batched_equals_unbatched(batch, mode):
         assert rep["path"] == (3 if mode == "mixed16" else 1)
 
 
+def test_batched_mixed_graph_replays(batch):
+    """Mixed precision: the assembly and the solve's fixed part (FP32 LUs, permutations, first solve and one
+    correction) are captured in graphs on the first call and replayed after.  Three calls with eager work between
+    them (new geometry, then the first geometry again): each matches the unbatched mixed path, and the third
+    repeats the first bit for bit."""
+    K, E, lam, nu, G_ = api.ElasticConstantsCheck(G, NU)
+    rng = np.random.default_rng(5)
+    moved = []
+    for m in batch:
+        R = meshes.random_rotation(rng)
+        moved.append(dict(m, P1=m["P1"] @ R.T, P2=m["P2"] @ R.T, P3=m["P3"] @ R.T))
+    api.set_precision("mixed")
+    try:
+        x1, r1 = api.batched_slip(batch, nu, G_, lam, 0)
+        x2, r2 = api.batched_slip(moved, nu, G_, lam, 0)
+        x3, r3 = api.batched_slip(batch, nu, G_, lam, 0)
+        ref_b = [_unbatched(m, lam) for m in batch]
+        ref_m = [_unbatched(m, lam) for m in moved]
+    finally:
+        api.set_precision("fp64")
+    for xs, refs in ((x1, ref_b), (x2, ref_m), (x3, ref_b)):
+        for x, ref in zip(xs, refs):
+            assert np.max(np.abs(x - ref)) <= 1e-8 * np.max(np.abs(ref))
+    for a, b in zip(x1, x3):
+        assert np.array_equal(a, b)
+    for r in (r1, r2, r3):
+        assert r["residual"] <= 1e-10 and r["path"] == 1
+
+
 def test_batched_mixed_fallback_chain(batch):
     """A tolerance below FP64 rounding: the refinement rounds stall, FGMRES-IR stops at its iteration cap, and
     every mesh ends on the FP64 LU, whose solution equals the unbatched FP64 one."""
