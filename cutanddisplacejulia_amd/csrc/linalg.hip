@@ -28,7 +28,10 @@ namespace {
 constexpr int PT = 256;        // panel threads
 constexpr int MAX_P = 1024;    // max panel workgroups
 constexpr int MAX_RPB = PT;    // panel rows per workgroup, one per thread (LDS multipliers: 256 x 64 x 8 = 128 KiB)
-constexpr int LU_OUTER = 256;  // columns per outer block (rank of the trailing update)
+constexpr int LU_OUTER = 256;      // columns per outer block (rank of the trailing update)
+constexpr int LU_OUTER_BIG = 512;  // the same for n >= LU_BIG: the trailing matrix is read and written half as often
+constexpr int64_t LU_BIG = 32768;  // (measured: 512 is faster at n = 48672, slower at 2485, 15000 and 19602)
+inline int64_t lu_outer(int64_t n) { return n >= LU_BIG ? LU_OUTER_BIG : LU_OUTER; }
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -706,11 +709,11 @@ __global__ void __launch_bounds__(256) trsm_batched_kernel(const TrsmBatch<T> jb
 // partial pivoting; each column c of U is scaled by a power of two sigma[c] (max |U[:, c]| * sigma in [0.5, 1))
 // before rounding, and the accumulated product divided by it, so FP16's range never clips the update.
 // The operands are rounded once per update into packed FP16 copies with k contiguous (Lh: row i's K values,
-// Uh: column j's K values, K <= LU_OUTER), so the GEMM streams them with 16-byte loads.
+// Uh: column j's K values, K <= LU_OUTER_BIG), so the GEMM streams them with 16-byte loads.
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 constexpr int GH_K = 32;        // k per LDS stage
-constexpr int GH_LD = LU_OUTER; // k stride of the packed copies
+constexpr int GH_LD = LU_OUTER_BIG; // k stride of the packed copies (the largest outer block)
 
 // Bytes from the column scales to the packed FP16 copies: n floats, rounded up to 256 bytes
 inline size_t f16_pack_offset(int64_t n) { return ((sizeof(float) * (size_t)n + 255) / 256) * 256; }
@@ -722,14 +725,15 @@ __global__ void __launch_bounds__(256) pack_l_f16_kernel(const float* __restrict
   __shared__ _Float16 tile[32][GH_LD + 8];
   const int tid = threadIdx.x;
   const int64_t i0 = (int64_t)blockIdx.x * 32;
-  for (int idx = tid; idx < 32 * GH_LD; idx += 256) {
+  const int KP = (int)((K + GH_K - 1) / GH_K) * GH_K;  // the GEMM reads whole slices: zeros up to KP
+  for (int idx = tid; idx < 32 * KP; idx += 256) {
     const int r = idx % 32, k = idx / 32;
     const int64_t i = i0 + r;
     tile[r][k] = (_Float16)((i < M && k < K) ? L[i + k * ldl] : 0.f);
   }
   __syncthreads();
-  for (int idx = tid; idx < 32 * (GH_LD / 8); idx += 256) {
-    const int r = idx / (GH_LD / 8), kq = idx % (GH_LD / 8);
+  for (int idx = tid; idx < 32 * (KP / 8); idx += 256) {
+    const int r = idx / (KP / 8), kq = idx % (KP / 8);
     const int64_t i = i0 + r;
     if (i >= M) continue;
     h8 v;
@@ -760,8 +764,10 @@ __global__ void __launch_bounds__(256) pack_u_f16_kernel(const float* __restrict
   if (mx > 0.f) (void)frexpf(mx, &e);
   const float sg = mx > 0.f ? ldexpf(1.0f, -e) : 1.0f;
   if (lane == 0) sigma[j] = sg;
+  const int64_t KP = (K + GH_K - 1) / GH_K * GH_K;
 #pragma unroll
-  for (int t = 0; t < GH_LD / 64; ++t) Uh[j * GH_LD + lane + 64 * t] = (_Float16)(x[t] * sg);
+  for (int t = 0; t < GH_LD / 64; ++t)
+    if (lane + 64 * t < KP) Uh[j * GH_LD + lane + 64 * t] = (_Float16)(x[t] * sg);
 }
 
 // 128 x 128 tile per workgroup (4 waves in 2 x 2, 64 x 64 each), K in GH_K slices through two LDS buffers (one
@@ -1047,16 +1053,16 @@ struct TrsmList {
 size_t lu_workspace_bytes(int64_t n) {
   const int64_t nn = n > 0 ? n : 1;
   // the panels' slots and records, flags and the panel permutation; then n column scales and the two packed FP16
-  // operand copies of the FP16 trailing update (n x LU_OUTER halves each)
+  // operand copies of the FP16 trailing update (n x GH_LD halves each)
   return sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) + sizeof(int) * (2 + 1 + 4 * LU_NB) +
          f16_pack_offset(nn) + 2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
 }
 
 // Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
-// carried along).  Outer blocks of LU_OUTER columns are factored as LU_NB-wide panels, each first brought up to
+// carried along).  Outer blocks of lu_outer(n) columns are factored as LU_NB-wide panels, each first brought up to
 // date with the earlier panels of its block (left-looking inside the block); the rest of the matrix then gets a
-// single rank-LU_OUTER update (right-looking across blocks), so the trailing matrix is read and written once per
-// LU_OUTER columns.
+// single rank-lu_outer(n) update (right-looking across blocks), so the trailing matrix is read and written once
+// per outer block.
 template <class T>
 void trailing_update(T* L, T* U, T* C, int64_t lda, int64_t M, int64_t N, int64_t K, float* sigma, bool fp16,
                      hipStream_t st);
@@ -1117,8 +1123,9 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
     return r;
   }();
   if (lds_set != hipSuccess) return lds_set;
-  for (int64_t K0 = 0; K0 < n; K0 += LU_OUTER) {
-    const int64_t K1 = std::min<int64_t>(n, K0 + LU_OUTER);
+  const int64_t ob = lu_outer(n);
+  for (int64_t K0 = 0; K0 < n; K0 += ob) {
+    const int64_t K1 = std::min<int64_t>(n, K0 + ob);
     for (int64_t k = K0; k < K1; k += LU_NB) {
       const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
       if (k > K0) {
@@ -1172,8 +1179,9 @@ int lu_batched_capacity() {
   return capacity;
 }
 
-// Several matrices factored together, each exactly as lu_factor_cols would factor it alone (the same operations
-// in the same order, so the same factors bit for bit), but stepped in lockstep by panel column so that the panels
+// Several matrices factored together, each exactly as lu_factor_cols would factor it alone when n < LU_BIG (the
+// same operations in the same order, so the same factors bit for bit; larger matrices keep LU_OUTER-column blocks
+// here), but stepped in lockstep by panel column so that the panels
 // of all matrices at column k share cooperative launches: a panel's cost is the latency chain of its 64 columns,
 // which one launch now pays once for up to PANEL_JOBS matrices.  The catch-up and trailing updates, triangular
 // solves and row interchanges stay per matrix, on the same stream.
