@@ -15,13 +15,13 @@ pytestmark = pytest.mark.gpu
 NU, G = 0.25, 1.0
 
 
-def _unbatched(m, lam):
+def _unbatched(m, lam, hs=0):
     P1, P2, P3 = m["P1"], m["P2"], m["P3"]
     n = P1.shape[0]
     nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
     cols = []
     for sxx, syy, szz, sxy, sxz, syz in m["stresses"]:
-        Dn, Dss, Dds, A, b = api.SlipCalculator3D(P1, P2, P3, NU, G, lam, mid, nrm, 0,
+        Dn, Dss, Dds, A, b = api.SlipCalculator3D(P1, P2, P3, NU, G, lam, mid, nrm, hs,
                                                   api.Stresses(sxx, syy, szz, sxy, sxz, syz), np.zeros(n))
         cols.append(np.concatenate([Dn, Dss, Dds]))
     return np.stack(cols, axis=1)
@@ -85,6 +85,31 @@ def test_batched_mixed_graph_replays(batch):
         assert np.array_equal(a, b)
     for r in (r1, r2, r3):
         assert r["residual"] <= 1e-10 and r["path"] == 1
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_batched_halfspace_many_loads(mode):
+    """Half-space batch (cracks moved below the free surface) with one mesh carrying 11 remote loads: its factor
+    applications split into several column groups and its residual into two passes over A."""
+    K, E, lam, nu, G_ = api.ElasticConstantsCheck(G, NU)
+    rng = np.random.default_rng(23)
+    batch = meshes.elliptical_cracks(count=3, n_min=100, n_max=300, rhs_max=2, seed=29)
+    down = np.array([0.0, 0.0, -16.0])
+    batch = [dict(m, P1=m["P1"] + down, P2=m["P2"] + down, P3=m["P3"] + down) for m in batch]
+    batch[1] = dict(batch[1], stresses=rng.normal(size=(11, 6)))
+    assert all(np.max(m[k][:, 2]) < 0 for m in batch for k in ("P1", "P2", "P3"))
+    api.set_precision(mode)
+    try:
+        xs, rep = api.batched_slip(batch, nu, G_, lam, 1)
+        refs = [_unbatched(m, lam, hs=1) for m in batch]
+    finally:
+        api.set_precision("fp64")
+    tol = 1e-12 if mode == "fp64" else 1e-8
+    for x, ref in zip(xs, refs):
+        assert x.shape == ref.shape
+        assert np.max(np.abs(x - ref)) <= tol * np.max(np.abs(ref))
+    if mode == "mixed":
+        assert rep["residual"] <= 1e-10 and rep["path"] == 1
 
 
 def test_batched_mixed_fallback_chain(batch):
