@@ -112,6 +112,28 @@ def test_batched_halfspace_many_loads(mode):
         assert rep["residual"] <= 1e-10 and rep["path"] == 1
 
 
+def test_batched_mixed_large_mesh_host_permutations():
+    """A mesh above 16384 unknowns (5766 triangles, 17298 unknowns) is too large for the device permutation kernel:
+    the batch then runs its fixed part eagerly with the permutations from the host, and still matches the
+    unbatched mixed solve."""
+    K, E, lam, nu, G_ = api.ElasticConstantsCheck(G, NU)
+    rng = np.random.default_rng(31)
+    small = meshes.elliptical_cracks(count=1, n_min=100, n_max=150, rhs_max=2, seed=37)[0]
+    P1, P2, P3 = meshes.penny(31, 1.0, seed=41, aspect=1.5)
+    big = dict(P1=P1, P2=P2, P3=P3, stresses=rng.normal(size=(2, 6)))
+    assert 3 * P1.shape[0] > 16384
+    batch = [small, big]
+    api.set_precision("mixed")
+    try:
+        xs, rep = api.batched_slip(batch, nu, G_, lam, 0)
+        refs = [_unbatched(m, lam) for m in batch]
+    finally:
+        api.set_precision("fp64")
+    for x, ref in zip(xs, refs):
+        assert np.max(np.abs(x - ref)) <= 1e-8 * np.max(np.abs(ref))
+    assert rep["residual"] <= 1e-10 and rep["path"] == 1
+
+
 def test_batched_mixed_fallback_chain(batch):
     """A tolerance below FP64 rounding: the refinement rounds stall, FGMRES-IR stops at its iteration cap, and
     every mesh ends on the FP64 LU, whose solution equals the unbatched FP64 one."""
