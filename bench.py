@@ -111,6 +111,30 @@ def slip_solve_comparison(step_no):
     return r
 
 
+def halfspace_solve_case(n_quads):
+    """The half-space fault of the headline assembly (5000 triangles, A 15000^2) solved under remote sigma_zz = 1:
+    FP64 assembly + FP64 LU against FP32-far / FP64-near assembly with the FP32 MFMA LU (and the FP16-update LU) as
+    preconditioner of FP64 GMRES-IR, to an FP64 relative residual of 1e-10; also block Jacobi and the learned
+    preconditioner (trained on full-space penny cracks)."""
+    from cutanddisplacejulia_amd import meshes, spai
+    c = meshes.config2(n_quads)
+    step = dict(P1=c["P1"], P2=c["P2"], P3=c["P3"], step=0, g=0.0)
+    model = spai.load_model() if os.path.exists(spai.DEFAULT_PARAMS) else None
+    spai.evaluate_step(dict(step, P1=c["P1"][:200], P2=c["P2"][:200], P3=c["P3"][:200]), model, hs=1,
+                       pcs=("lu32",))  # warm-up
+    r = spai.evaluate_step(step, model, hs=1, pcs=("jacobi", "learned_k32", "lu32", "lu16"))
+    plain = r["fp64"]["total_ms"]
+    ok = [k for k in r if k.startswith("mixed_") and isinstance(r[k], dict) and r[k].get("residual", 1.0) <= 1e-10]
+    best = min(ok, key=lambda k: r[k]["total_ms"], default=None)
+    r["fastest_mixed"] = best
+    r["speedup_fastest_mixed_vs_fp64"] = plain / r[best]["total_ms"] if best else None
+    _log("half-space solve (%d triangles): FP64 assembly+LU %.1f ms; mixed assembly + FP32 LU + GMRES-IR %.1f ms "
+         "(residual %.1e, vs FP64 %.1e); fastest mixed %s %.1f ms" % (
+             r["n_tri"], plain, r["mixed_lu32"]["total_ms"], r["mixed_lu32"]["residual"],
+             r["mixed_lu32"]["forward_err_vs_fp64"], best, r[best]["total_ms"] if best else float("nan")))
+    return r
+
+
 def batched_case():
     """64 planar elliptical cracks (N_i ~ U(200, 3000), 1-8 remote loads each, uniformly random orientation), all
     assembled by one persistent work-queue launch in a captured HIP graph with FP32 far / FP64 near entries, then
@@ -424,6 +448,7 @@ def main():
     if not args.no_solve and not args.no_compare:
         solve = slip_solve_comparison(args.solve_step)
 
+    hs_solve = None if (args.no_solve or args.no_compare) else halfspace_solve_case(args.n)
     batched = None if args.no_batched else batched_case()
     large = large_case() if args.large else None
     crack_tip = crack_tip_case() if args.crack_tip else None
@@ -465,6 +490,7 @@ def main():
         "cpu_baseline": cpu,
         "lu_15000": lu15,
         "solve": solve,
+        "halfspace_solve": hs_solve,
     }
     if batched is not None:
         out["batched"] = batched
