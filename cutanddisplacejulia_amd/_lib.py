@@ -128,6 +128,12 @@ def load(path=LIB_PATH):
     return L
 
 
+def last_error():
+    """The context's last message (an error, or why a mixed-precision solve fell back to FP64)."""
+    msg = load().cdj_last_error(ctx())
+    return msg.decode() if msg else ""
+
+
 def ctx():
     """Process-wide context on device 0 (created on first use)."""
     global _CTX

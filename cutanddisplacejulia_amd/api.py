@@ -386,6 +386,8 @@ def solve(A, b):
     check(load().cdj_solve(ctx(), ptr(A), ptr(b2), n, b2.shape[1], ptr(x), ctypes.byref(_opts()), ctypes.byref(rep)))
     LAST_REPORT.clear()
     LAST_REPORT.update(rep.as_dict())
+    if _SETTINGS["precision"] != _lib.CDJ_PREC_FP64 and rep.path == _lib.CDJ_PREC_FP64:
+        LAST_REPORT["fallback"] = _lib.last_error()
     return x if np.ndim(b) > 1 else x[:, 0]
 
 

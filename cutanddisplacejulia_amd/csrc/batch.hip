@@ -38,8 +38,10 @@ __device__ __forceinline__ int64_t next_item(unsigned long long* counter) {
 
 // A_m = -[T blocks]: entry (receiver il, source j) of block (mode m, traction q) at
 // A_m[(q n + il) + (m n + j) lda], lda = 3 n (SlipCalculator3D.jl:13-16).
+// FP32 instances are held to two waves per SIMD (the half-space one then spills 12 bytes per lane instead of
+// running one wave per SIMD); FP64 keeps the register budget of one wave.
 template <class T, class TO, bool HS>
-__global__ void __launch_bounds__(256) batch_far_kernel(const double* __restrict__ recv, const T* __restrict__ rec,
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) batch_far_kernel(const double* __restrict__ recv, const T* __restrict__ rec,
                                                         const BatchMeshDev* __restrict__ meshes,
                                                         const int64_t* __restrict__ item_start, int nmesh, int tpb,
                                                         T nu, T lam, T G, TO* __restrict__ A,

@@ -658,12 +658,18 @@ int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
   bool ok = h == 0;
   double worst = 0.0;
   int iters = 0;
+  std::string why = h ? "the FP32 factor has a zero pivot (column " + std::to_string(h) + ")" : std::string();
   for (int64_t q = 0; ok && q < nrhs; ++q) {
     GmresResult gr;
     HIPCHK(fgmres_ir<float>(A, n, lda, A32.as<float>(), n, perm.as<int32_t>(), B + q * n, X + q * n, go, gr, st));
     ok = gr.converged && !gr.trsv_timeout;
     worst = std::max(worst, gr.residual);
     iters += gr.iterations;
+    if (!ok)
+      why = gr.trsv_timeout ? "a triangular-solve wait timed out (right-hand side " + std::to_string(q) + ")"
+                            : "FGMRES-IR stopped at residual " + std::to_string(gr.residual) + " after " +
+                                  std::to_string(gr.iterations) + " iterations (right-hand side " +
+                                  std::to_string(q) + ")";
   }
   HIPCHK(hipEventRecord(e1, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -680,7 +686,8 @@ int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
     }
     return CDJ_OK;
   }
-  // fallback: FP64 LU on a copy of A
+  // fallback: FP64 LU on a copy of A; the reason stays readable through cdj_last_error
+  ctx->err = "mixed-precision refinement fell back to the FP64 LU: " + why;
   DBuf AB;
   HIPCHK(AB.alloc(sizeof(double) * n * (n + nrhs)));
   HIPCHK(hipMemcpy2DAsync(AB.p, sizeof(double) * n, A, sizeof(double) * lda, sizeof(double) * n, n,

@@ -207,7 +207,8 @@ __global__ void mgs_update_kernel(const double* __restrict__ V, int64_t ldv, int
 
 __global__ void axpby_kernel(double a, const double* __restrict__ x, double b, double* __restrict__ y, int64_t n) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) y[i] = a * x[i] + b * y[i];
+  // b == 0 never reads y: y may be fresh, uninitialised memory (0 * NaN would poison the result)
+  if (i < n) y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
 }
 
 // x += Z[:, 0:k] yk

@@ -325,8 +325,10 @@ __global__ void tdk_field_reduce(const T* __restrict__ part, int64_t nobs, int n
 //      CalculateInfluenceMatrices3D.jl:73-142) -------------------------------------------------------------
 // out.p[3*m + q] is the block of mode m (Dn, Dss, Dds) and traction q (Tn, Tss, Tds); out.sign = -1 writes
 // A = -[...] (SlipCalculator3D.jl:13-16).  Block pointers are already offset, leading dimension out.ld.
+// FP32 instances are held to two waves per SIMD (the half-space one then spills 12 bytes per lane instead of
+// running one wave per SIMD); FP64 keeps the register budget of one wave.
 template <class T, class TO, bool HS>
-__global__ void __launch_bounds__(256) tdk_traction(const double* __restrict__ recv, int64_t nrecv,
+__global__ void __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) tdk_traction(const double* __restrict__ recv, int64_t nrecv,
                                                     const T* __restrict__ rec, int64_t ntri, int tpb, T nu, T lam,
                                                     T G, OutPtrs<TO> out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;

@@ -106,3 +106,23 @@ def test_mixed16_slip_calculator_halfspace(mixed16):
     x = np.concatenate([Dn, Dss, Dds])
     xr = np.linalg.solve(A, b)
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-8
+
+
+def test_mixed_solve_ignores_stale_device_memory(mixed):
+    """Regression: the Krylov basis was started as y = a x + 0 y on freshly allocated memory, so NaNs left in
+    reused device memory turned the second right-hand side's refinement into NaN (and the solve fell back to
+    FP64).  Device memory is filled with NaN and released first; the mixed path must still converge."""
+    import torch
+    t = torch.full((1 << 28,), float("nan"), dtype=torch.float64, device="cuda:0")  # 2 GiB of NaN
+    torch.cuda.synchronize()
+    del t
+    torch.cuda.empty_cache()
+    n = 777
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n)) + 2 * n ** 0.5 * np.eye(n)
+    b = rng.normal(size=(n, 3))
+    x = api.solve(A, b)
+    rep = dict(api.LAST_REPORT)
+    assert rep["path"] == 1, rep
+    res = np.linalg.norm(b - A @ x, axis=0) / np.linalg.norm(b, axis=0)
+    assert np.all(res <= 1e-10), res
