@@ -116,6 +116,33 @@ def test_td_field_okada():
     _cmp_group([a.ravel(order="F") for a in mine[3:]], so)
 
 
+def test_config2_full_mesh_matches_okada():
+    """The half-space benchmark fault at full size (5000 triangles; the 100 x 100 free-surface grid over
+    [-10, 10]^2): with uniform slip the summed TD field equals Okada's (1985) solution for the same rectangle, which
+    pins the main, image and free-surface terms at the benchmark's scale.  The fault is the reference's Okada test
+    rectangle (test_TDvsOkada.jl) with its top at depth 0.5."""
+    c = meshes.config2(50)
+    P1, P2, P3 = c["P1"], c["P2"], c["P3"]
+    n = P1.shape[0]
+    assert n == 5000
+    X = c["X"].reshape(100, 100, order="F")
+    Y = c["Y"].reshape(100, 100, order="F")
+    Z = c["Z"].reshape(100, 100, order="F")
+    Dds, Dn, Dss = 2.0, 0.6, 0.5
+    S, D = api.TD(X, Y, Z, P1, P2, P3, [Dss] * n, [Dds] * n, [Dn] * n, 0.25, 1.0, 1, 1, 1, api.Strains(),
+                  api.Disps())
+    tot = np.sqrt(Dds ** 2 + Dss ** 2)
+    rake = 90 - np.degrees(np.arctan(Dss / Dds))
+    centroid_depth = 0.5 + np.sin(np.deg2rad(45.0)) * 5.0 / 2
+    ok = A.okada85_surface(X, Y, centroid_depth, 60, 45, 5, 5, rake, tot, Dn, 0.25)
+    mine = [D.Ux, D.Uy, D.Uz, S.εxx, S.εyy, S.εzz, S.εxy, S.εxz, S.εyz]
+    # displacements and strains each against their group's largest value (exz, eyz vanish on the free surface)
+    for grp in (range(0, 3), range(3, 9)):
+        scale = max(np.max(np.abs(ok[k])) for k in grp)
+        for k in grp:
+            assert np.max(np.abs(mine[k] - ok[k])) <= 1e-10 * scale, k
+
+
 def test_td_field_deterministic():
     P1, P2, P3 = _small_fault(8)
     X, Y, Z = _obs(P1, P2, P3, 16)
