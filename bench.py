@@ -353,8 +353,8 @@ def main():
                          "grid field; reported under 'large'")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every GPU box here has one MI355X and this build has no multi-GPU path "
-              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU: the engine has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
