@@ -2,6 +2,7 @@
 
 There is no CPU fallback: if the library is missing or no HIP device is present, calls raise.
 """
+import atexit
 import ctypes
 import os
 import threading
@@ -145,7 +146,18 @@ def ctx():
             if rc != CDJ_OK:
                 raise CdjError(rc, "cdj_create failed: " + L.cdj_last_error(None).decode())
             _CTX = h
+            # release the context (its stream, events, scratch, captured graphs) while the HIP runtime and any
+            # profiler attached to it are still up, rather than leaving it to process teardown
+            atexit.register(_destroy)
         return _CTX
+
+
+def _destroy():
+    global _CTX
+    with _LOCK:
+        if _CTX is not None and _LIB is not None:
+            _LIB.cdj_destroy(_CTX)
+        _CTX = None
 
 
 def check(rc):

@@ -250,35 +250,23 @@ def CalculateInfluenceMatrices3D(FaceNormalVector, MidPoint, P1, P2, P3, ν, G, 
     """
     nrm, mid = f64(FaceNormalVector), f64(MidPoint)
     P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
-    fixed = np.asarray(FixedEls).ravel() == 1 if FixedEls is not None else np.zeros(n, bool)
-    if not fixed.any():
-        T = [np.zeros((n, n), order="F") for _ in range(9)]
-        check(load().cdj_traction_influence(ctx(), ptr(nrm), ptr(mid), ptr(P1), ptr(P2), ptr(P3), int(n), float(ν),
-                                            float(G), float(λ), int(HSFlag), None, ptr_array(T),
-                                            ctypes.byref(_opts())))
-        return TractionInf(*T)
-    # Collocation points (SetupCollationPoints, CalculateInfluenceMatrices3D.jl:52-71)
-    pts = mid - nrm * 1e-12
-    free = ~fixed
-    nf = int(free.sum())
-    # Free rows: strains at the free collocation points turned into tractions.  The reference passes the full
-    # normal arrays to ConvertStrainInfMatsToTraction with only the free rows of strain, so its i-th free row
-    # takes the plane of element i (CalculateInfluenceMatrices3D.jl:24-26, 73-110); reproduced here.
-    Tf = [np.zeros((nf, n), order="F") for _ in range(9)]
-    fp = f64(pts[free])
-    fn = f64(nrm[:nf])
-    check(load().cdj_traction_rows(ctx(), ptr(fp), ptr(fn), nf, ptr(P1), ptr(P2), ptr(P3), int(n), float(ν), float(G),
-                                   float(λ), int(HSFlag), ptr_array(Tf), ctypes.byref(_opts())))
-    # Fixed rows: displacements at the fixed collocation points
-    ones = np.ones(n)
-    _, D = TD(pts[fixed, 0], pts[fixed, 1], pts[fixed, 2], P1, P2, P3, ones, ones, ones, ν, G, 1, 0, HSFlag)
-    out = []
-    for k, name in enumerate(TRACTION_FIELDS):
-        full = np.zeros((n, n), order="F")
-        full[free, :] = Tf[k]
-        full[fixed, :] = getattr(D, DISP_FIELDS[k])  # DnTn<-DnUx, DnTss<-DnUy, ... (ConcatInfMats field order)
-        out.append(np.asfortranarray(full[:, free]))  # fixed sources' columns dropped (PutDispInfsIntoInfMat)
-    return TractionInf(*out)
+    fx = _fixed_flags(FixedEls, n)
+    nf = n - (int(fx.sum()) if fx is not None else 0)
+    T = [np.zeros((n, nf), order="F") for _ in range(9)]
+    check(load().cdj_traction_influence(ctx(), ptr(nrm), ptr(mid), ptr(P1), ptr(P2), ptr(P3), int(n), float(ν),
+                                        float(G), float(λ), int(HSFlag), ptr(fx), ptr_array(T),
+                                        ctypes.byref(_opts())))
+    return TractionInf(*T)
+
+
+def _fixed_flags(FixedEls, n):
+    """FixedEls as the int32 flags of the C ABI (1 = fixed), or None when there are none."""
+    if FixedEls is None:
+        return None
+    fx = np.ascontiguousarray((np.asarray(FixedEls).ravel() == 1).astype(np.int32))
+    if fx.shape[0] != n:
+        raise CdjError(_lib.CDJ_ERR_DIMS, "Element size inputs must be the same dimensions")
+    return fx if fx.any() else None
 
 
 def SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, BoundaryConditions, FixedEls):
@@ -296,34 +284,24 @@ def SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag, 
         return _slip_friction(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BoundaryConditions, FixedEls)
     if isinstance(BoundaryConditions, MixedBoundaryConditionsFluidVolume):
         return _slip_fluid_volume(P1, P2, P3, ν, G, λ, MidPoint, nrm, HSFlag, BoundaryConditions, FixedEls)
-    fixed = np.asarray(FixedEls).ravel() == 1 if FixedEls is not None else np.zeros(n, bool)
     if isinstance(BoundaryConditions, np.ndarray):
         b = f64(BoundaryConditions.reshape(3 * n, -1, order="F"))
     else:
         Tn, Tds, Tss = SetupTractionVector(BoundaryConditions, nrm, n, G, λ)
         Tn, Tds, Tss = KnockOutFixedRows(FixedEls, Tn, Tds, Tss)
         b = f64(np.concatenate([Tn, Tss, Tds]).reshape(3 * n, 1))
-    if fixed.any():
-        Ti = CalculateInfluenceMatrices3D(nrm, MidPoint, P1, P2, P3, ν, G, λ, FixedEls, HSFlag, n)
-        A = f64(-np.block([[Ti.DnTn, Ti.DssTn, Ti.DdsTn], [Ti.DnTss, Ti.DssTss, Ti.DdsTss],
-                           [Ti.DnTds, Ti.DssTds, Ti.DdsTds]]))
-        # Non-square system (3n x 3nf): Julia's `\` returns the least-squares solution
-        nf = int((~fixed).sum())
-        D = np.zeros((3 * nf, b.shape[1]), order="F")
-        rep = Report()
-        check(load().cdj_lstsq(ctx(), ptr(A), A.shape[0], A.shape[1], ptr(b), b.shape[1], ptr(D), ctypes.byref(rep)))
-    else:
-        P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
-        mid = f64(MidPoint)
-        A = np.zeros((3 * n, 3 * n), order="F")
-        D = np.zeros(b.shape, order="F")
-        rep = Report()
-        check(load().cdj_slip_calculator(ctx(), ptr(P1), ptr(P2), ptr(P3), n, float(ν), float(G), float(λ),
-                                         ptr(mid), ptr(nrm), int(HSFlag), ptr(b), b.shape[1], None, ptr(D), ptr(A),
-                                         ctypes.byref(_opts()), ctypes.byref(rep)))
-        LAST_REPORT.clear()
-        LAST_REPORT.update(rep.as_dict())
-        nf = n
+    fx = _fixed_flags(FixedEls, n)
+    nf = n - (int(fx.sum()) if fx is not None else 0)
+    P1, P2, P3 = _tri(P1), _tri(P2), _tri(P3)
+    mid = f64(MidPoint)
+    A = np.zeros((3 * n, 3 * nf), order="F")
+    D = np.zeros((3 * nf, b.shape[1]), order="F")
+    rep = Report()
+    check(load().cdj_slip_calculator(ctx(), ptr(P1), ptr(P2), ptr(P3), n, float(ν), float(G), float(λ), ptr(mid),
+                                     ptr(nrm), int(HSFlag), ptr(b), b.shape[1], ptr(fx), ptr(D), ptr(A),
+                                     ctypes.byref(_opts()), ctypes.byref(rep)))
+    LAST_REPORT.clear()
+    LAST_REPORT.update(rep.as_dict())
     squeeze = not isinstance(BoundaryConditions, np.ndarray)
     Dn, Dss, Dds = D[:nf], D[nf:2 * nf], D[2 * nf:3 * nf]
     if squeeze:
@@ -567,7 +545,8 @@ def batched_slip(meshes, nu, G, lam, hs_flag=0):
 
     meshes: list of dicts with P1, P2, P3 (n x 3) and `stresses` (k x 6 remote tensors sxx, syy, szz, sxy, sxz,
     syz), or `b` (3n x k right-hand sides [Tn; Tss; Tds]).  Returns a list of 3n x k arrays [Dn; Dss; Dds] and the
-    total report (assembly time of the batched graph, summed solve time, near entries).  The precision is the
+    total report (assembly time of the batched graph, summed solve time, near entries; "meshes": the per-mesh
+    reports; "fallbacks": meshes of a mixed-precision batch that fell back to the FP64 LU).  The precision is the
     module setting (set_precision)."""
     L = load()
     keep = []
@@ -592,11 +571,16 @@ def batched_slip(meshes, nu, G, lam, hs_flag=0):
                        b.shape[1], b.ctypes.data, x.ctypes.data)
         outs.append(x)
     rep = Report()
+    per = (Report * len(meshes))()
     check(L.cdj_batched_slip(ctx(), ctypes.byref(arr), len(meshes), float(nu), float(G), float(lam), int(hs_flag),
-                             ctypes.byref(_opts()), None, ctypes.byref(rep)))
+                             ctypes.byref(_opts()), per, ctypes.byref(rep)))
     tpb, wgs = ctypes.c_int32(), ctypes.c_int32()
     check(L.cdj_batch_tuning(ctx(), ctypes.byref(tpb), ctypes.byref(wgs)))
     r = rep.as_dict()
+    # per-mesh path, iterations and residual: a mesh that fell back to the FP64 LU reports path FP64, residual -1
+    r["meshes"] = [per[i].as_dict() for i in range(len(meshes))]
+    r["fallbacks"] = sum(1 for q in r["meshes"] if q["path"] == _lib.CDJ_PREC_FP64) \
+        if _SETTINGS["precision"] != _lib.CDJ_PREC_FP64 else 0
     r["tile"] = {"sources_per_item": tpb.value, "workgroups": wgs.value}
     LAST_REPORT.clear()
     LAST_REPORT.update(r)

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -150,6 +151,103 @@ int assembly_mode(cdj_ctx* ctx, const cdj_opts* opts, bool* mixed, double* near_
   return CDJ_OK;
 }
 
+int64_t count_fixed(const int32_t* fixed, int64_t n) {
+  int64_t c = 0;
+  if (fixed)
+    for (int64_t i = 0; i < n; ++i) c += fixed[i] == 1;
+  return c;
+}
+
+// Fixed-element system from host geometry (CalculateInfluenceMatrices3D.jl:10-33, 145-198), in FP64 whatever the
+// assembly option: block k of `out` (device, n rows x nf columns) receives
+//   * at a free element's row, the traction of the strain at its collocation point.  The reference hands only the
+//     free strain rows to ConvertStrainInfMatsToTraction with the full normal arrays, so the i-th free row takes
+//     the plane (normal, strike and dip directions) of element i (CalculateInfluenceMatrices3D.jl:24-26, 73-110);
+//   * at a fixed element's row, the displacement of its collocation point (DnTn <- DnUx, DnTss <- DnUy, ...).
+// Only free sources are evaluated: the reference drops the fixed sources' columns (PutDispInfsIntoInfMat).
+int assemble_fixed(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1, const double* P2,
+                   const double* P3, int64_t n, const int32_t* fixed, double nu, double G, double lambda, int hs,
+                   const cdj_opts* opts, const OutPtrs<double>& out, hipStream_t st) {
+  std::vector<int64_t> fr, fx;
+  for (int64_t i = 0; i < n; ++i) (fixed[i] == 1 ? fx : fr).push_back(i);
+  const int64_t nf = (int64_t)fr.size(), nfix = (int64_t)fx.size();
+  // collocation points (SetupCollationPoints, CalculateInfluenceMatrices3D.jl:52-71)
+  std::vector<double> pts(3 * n);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) pts[k * n + i] = midpoints[k * n + i] - (normals[k * n + i] * 1e-12);
+  if (hs) {
+    for (int64_t i = 0; i < n; ++i)
+      if (pts[2 * n + i] > 0 || P1[2 * n + i] > 0 || P2[2 * n + i] > 0 || P3[2 * n + i] > 0)
+        return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+  }
+  if (nf == 0) return CDJ_OK;
+  // host gathers: free receivers (points, planes of elements 0..nf-1), fixed receivers, free sources
+  std::vector<double> pf(3 * nf), nr(3 * nf), px(3 * std::max<int64_t>(nfix, 1)), s1(3 * nf), s2(3 * nf),
+      s3(3 * nf), ones(nf, 1.0);
+  std::vector<int32_t> rowmap(n);
+  for (int64_t t = 0; t < nf; ++t) {
+    const int64_t i = fr[t];
+    rowmap[i] = (int32_t)t;
+    for (int k = 0; k < 3; ++k) {
+      pf[k * nf + t] = pts[k * n + i];
+      nr[k * nf + t] = normals[k * n + t];
+      s1[k * nf + t] = P1[k * n + i];
+      s2[k * nf + t] = P2[k * n + i];
+      s3[k * nf + t] = P3[k * n + i];
+    }
+  }
+  for (int64_t t = 0; t < nfix; ++t) {
+    rowmap[fx[t]] = (int32_t)(-1 - t);
+    for (int k = 0; k < 3; ++k) px[k * nfix + t] = pts[k * n + fx[t]];
+  }
+  const int RS = hs ? REC_HS : REC_FS;
+  const size_t v3 = sizeof(double) * 3 * nf;
+  DBuf dpf, dnr, ds1, ds2, ds3, dpx, done, dmap, recv, rec, Tf, Ud;
+  HIPCHK(dpf.alloc(v3));
+  HIPCHK(dnr.alloc(v3));
+  HIPCHK(ds1.alloc(v3));
+  HIPCHK(ds2.alloc(v3));
+  HIPCHK(ds3.alloc(v3));
+  HIPCHK(dpx.alloc(sizeof(double) * px.size()));
+  HIPCHK(done.alloc(sizeof(double) * nf));
+  HIPCHK(dmap.alloc(sizeof(int32_t) * n));
+  HIPCHK(recv.alloc(sizeof(double) * RECV * nf));
+  HIPCHK(rec.alloc(sizeof(double) * RS * nf));
+  HIPCHK(Tf.alloc(sizeof(double) * 9 * nf * nf));
+  HIPCHK(Ud.alloc(sizeof(double) * 9 * std::max<int64_t>(nfix, 1) * nf));
+  HIPCHK(hipMemcpyAsync(dpf.p, pf.data(), v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dnr.p, nr.data(), v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ds1.p, s1.data(), v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ds2.p, s2.data(), v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ds3.p, s3.data(), v3, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dpx.p, px.data(), sizeof(double) * px.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(done.p, ones.data(), sizeof(double) * nf, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dmap.p, rowmap.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+  launch_receivers_at(dnr.as<double>(), dpf.as<double>(), nf, recv.as<double>(), st);
+  launch_setup(ds1.as<double>(), ds2.as<double>(), ds3.as<double>(), nf, hs, rec.as<double>(), st);
+  OutPtrs<double> ot{};
+  ot.ld = nf;
+  ot.sign = 1.0;
+  for (int k = 0; k < 9; ++k) ot.p[k] = Tf.as<double>() + (int64_t)k * nf * nf;
+  const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, nf);
+  launch_traction<double>(recv.as<double>(), nf, rec.as<double>(), nf, hs, tpb, nu, lambda, G, ot, st);
+  if (nfix > 0) {
+    OutPtrs<double> od{};
+    od.ld = nfix;
+    od.sign = 1.0;
+    od.disp_on = 1;
+    for (int k = 0; k < 9; ++k) od.p[18 + k] = Ud.as<double>() + (int64_t)k * nfix * nf;
+    const double* x = dpx.as<double>();
+    launch_influence<double>(x, x + nfix, x + 2 * nfix, nfix, rec.as<double>(), nf, hs, default_tile(hs, nfix),
+                             done.as<double>(), done.as<double>(), done.as<double>(), nu, od, st);
+  }
+  launch_fixed_scatter(Tf.as<double>(), Ud.as<double>(), dmap.as<int32_t>(), n, nf, nfix, nf, out, st);
+  HIPCHK(hipGetLastError());
+  // the scoped buffers are released on return, so finish with them first
+  HIPCHK(hipStreamSynchronize(st));
+  return CDJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -194,11 +292,14 @@ int cdj_create(int device, cdj_ctx** out) {
 int cdj_destroy(cdj_ctx* ctx) {
   if (!ctx) return CDJ_OK;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  // drain the stream, then release what was made on it (graphs and their buffers) before the stream itself
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  batch_plan_free(ctx->batch);
+  ctx->batch = nullptr;
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-  batch_plan_free(ctx->batch);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CDJ_OK;
 }
@@ -439,9 +540,6 @@ int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* mi
                            int32_t hs_flag, const int32_t* fixed, double* const* T, const cdj_opts* opts) {
   if (!ctx) return CDJ_ERR_ARG;
   if (n < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
-  if (fixed)
-    for (int64_t i = 0; i < n; ++i)
-      if (fixed[i] == 1) return fail(ctx, CDJ_ERR_ARG, "fixed elements are not supported by this entry point");
   for (int k = 0; k < 9; ++k)
     if (!T || !T[k]) return fail(ctx, CDJ_ERR_ARG, "traction output pointer is NULL");
   bool mixed = false;
@@ -451,6 +549,24 @@ int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* mi
   hipStream_t st = ctx->stream;
   if (n == 0) return CDJ_OK;
   const int hs = hs_flag == 1;
+  if (const int64_t nfix = count_fixed(fixed, n)) {
+    if (!normals || !midpoints || !P1 || !P2 || !P3) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
+    const int64_t nf = n - nfix;
+    const size_t vm = sizeof(double) * n * nf;
+    DBuf dT;
+    HIPCHK(dT.alloc(9 * vm));
+    OutPtrs<double> out{};
+    out.ld = n;
+    out.sign = 1.0;
+    for (int k = 0; k < 9; ++k) out.p[k] = dT.as<double>() + (int64_t)k * n * nf;
+    int rc = assemble_fixed(ctx, normals, midpoints, P1, P2, P3, n, fixed, nu, G, lambda, hs, opts, out, st);
+    if (rc) return rc;
+    ctx->near_entries = 0;
+    if (nf > 0)
+      for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(T[k], out.p[k], vm, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return CDJ_OK;
+  }
   const size_t v3 = sizeof(double) * 3 * n;
   DBuf dn, dm, d1, d2, d3, recv, rec, dT;
   HIPCHK(dn.alloc(v3));
@@ -655,10 +771,13 @@ int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(lu_permutation(ipiv.as<int32_t>(), n, perm.as<int32_t>(), st));
+  HIPCHK(hipStreamSynchronize(st));  // h is read below
   bool ok = h == 0;
   double worst = 0.0;
   int iters = 0;
-  std::string why = h ? "the FP32 factor has a zero pivot (column " + std::to_string(h) + ")" : std::string();
+  std::string why = h < 0   ? std::string("an FP32 LU panel wait timed out")
+                    : h > 0 ? "the FP32 factor has a zero pivot (column " + std::to_string(h) + ")"
+                            : std::string();
   for (int64_t q = 0; ok && q < nrhs; ++q) {
     GmresResult gr;
     HIPCHK(fgmres_ir<float>(A, n, lda, A32.as<float>(), n, perm.as<int32_t>(), B + q * n, X + q * n, go, gr, st));
@@ -703,6 +822,51 @@ int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
     report->residual = -1.0;
     report->solve_ms = ms;
   }
+  return CDJ_OK;
+}
+
+// Least squares min ||A x - b|| for device A (m x k, k <= m, leading dimension lda): the `A\b` of
+// SlipCalculator3D.jl:22 when fixed elements make A non-square (Julia: pivoted QR).  Solved through the augmented
+// system [[I, A], [A^T, 0]] [r; x] = [b; 0] with the FP64 LU, so no normal equations are formed.  B (device,
+// m x nrhs, ld m) -> X (device, k x nrhs, ld k).  report->residual is the largest ||b - A x|| / ||b|| over the
+// right-hand sides: the least-squares residual, which is not small in general.
+int lstsq_dev(cdj_ctx* ctx, const double* A, int64_t m, int64_t k, int64_t lda, const double* B, int64_t nrhs,
+              double* X, cdj_report* report, hipStream_t st) {
+  if (m + k > 65535) return fail(ctx, CDJ_ERR_DIMS, "least-squares system too large (m + k > 65535)");
+  if (k == 0 || nrhs == 0) return CDJ_OK;
+  const int64_t N = m + k;
+  DBuf KB;
+  HIPCHK(KB.alloc(sizeof(double) * N * (N + nrhs)));
+  build_kkt(A, lda, m, k, KB.as<double>(), N, st);
+  double* R = KB.as<double>() + N * N;
+  HIPCHK(hipMemsetAsync(R, 0, sizeof(double) * N * nrhs, st));
+  HIPCHK(hipMemcpy2DAsync(R, sizeof(double) * N, B, sizeof(double) * m, sizeof(double) * m, nrhs,
+                          hipMemcpyDeviceToDevice, st));
+  int rc = solve_augmented_fp64(ctx, KB.as<double>(), N, N, nrhs, nullptr, nullptr, report, st);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy2DAsync(X, sizeof(double) * k, R + m, sizeof(double) * N, sizeof(double) * k, nrhs,
+                          hipMemcpyDeviceToDevice, st));
+  if (report) {
+    // r = b - A x in the B columns of the (no longer needed) augmented right-hand sides
+    double worst = 0.0;
+    std::vector<double> hr(m), hb(m);
+    for (int64_t q = 0; q < nrhs; ++q) {
+      double* r = R + q * N;
+      HIPCHK(hipMemcpyAsync(r, B + q * m, sizeof(double) * m, hipMemcpyDeviceToDevice, st));
+      gemv<double>(A, m, k, lda, X + q * k, r, -1.0, 1.0, st);
+      HIPCHK(hipMemcpyAsync(hr.data(), r, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(hb.data(), B + q * m, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      double nr = 0, nb = 0;
+      for (int64_t i = 0; i < m; ++i) {
+        nr += hr[i] * hr[i];
+        nb += hb[i] * hb[i];
+      }
+      worst = std::max(worst, nb > 0 ? std::sqrt(nr / nb) : std::sqrt(nr));
+    }
+    report->residual = worst;
+  }
+  HIPCHK(hipStreamSynchronize(st));
   return CDJ_OK;
 }
 
@@ -757,12 +921,48 @@ int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const 
   if (!ctx) return CDJ_ERR_ARG;
   if (n < 0 || nrhs < 0) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
   if (!P1 || !P2 || !P3 || !midpoints || !normals || !b || !x) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
-  if (fixed)
-    for (int64_t i = 0; i < n; ++i)
-      if (fixed[i] == 1) return fail(ctx, CDJ_ERR_ARG, "fixed elements are not supported by this entry point");
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   if (n == 0 || nrhs == 0) return CDJ_OK;
+  if (const int64_t nfix = count_fixed(fixed, n)) {
+    // fixed elements: A is 3n x 3nf (displacement rows at fixed elements, fixed sources' columns dropped) and
+    // x its least-squares solution (SlipCalculator3D.jl:13-29); FP64 whatever opts->precision
+    const int64_t nf = n - nfix, m = 3 * n, k = 3 * nf;
+    if (m + k > 65535) return fail(ctx, CDJ_ERR_DIMS, "least-squares system too large (3n + 3nf > 65535)");
+    DBuf dA, dB, dX;
+    HIPCHK(dA.alloc(sizeof(double) * m * std::max<int64_t>(k, 1)));
+    HIPCHK(dB.alloc(sizeof(double) * m * nrhs));
+    HIPCHK(dX.alloc(sizeof(double) * std::max<int64_t>(k, 1) * nrhs));
+    OutPtrs<double> out{};
+    out.ld = m;
+    out.sign = -1.0;
+    for (int mo = 0; mo < 3; ++mo)
+      for (int q = 0; q < 3; ++q) out.p[3 * mo + q] = dA.as<double>() + (int64_t)q * n + (int64_t)mo * nf * m;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, st));
+    int rc = assemble_fixed(ctx, normals, midpoints, P1, P2, P3, n, fixed, nu, G, lambda, hs_flag == 1, opts, out, st);
+    (void)hipEventRecord(e1, st);
+    (void)hipStreamSynchronize(st);
+    float ams = 0.f;
+    (void)hipEventElapsedTime(&ams, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (A_out && k > 0) HIPCHK(hipMemcpyAsync(A_out, dA.p, sizeof(double) * m * k, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(dB.p, b, sizeof(double) * m * nrhs, hipMemcpyHostToDevice, st));
+    rc = lstsq_dev(ctx, dA.as<double>(), m, k, m, dB.as<double>(), nrhs, dX.as<double>(), report, st);
+    if (rc) return rc;
+    if (report) {
+      report->assembly_ms = ams;
+      report->near_entries = 0;
+    }
+    ctx->near_entries = 0;
+    if (k > 0) HIPCHK(hipMemcpyAsync(x, dX.p, sizeof(double) * k * nrhs, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return CDJ_OK;
+  }
   const int64_t N3 = 3 * n, lda = N3;
   const size_t v3 = sizeof(double) * 3 * n;
   DBuf dn, dm, d1, d2, d3, AB, Ao, Bo;
@@ -1241,8 +1441,11 @@ int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bo
   }
   if (ctx->batch && ctx->batch->same(n, r, hs, mixed, f, nu, lam, G, prec)) return CDJ_OK;
   batch_plan_free(ctx->batch);
-  ctx->batch = new BatchPlan();
-  BatchPlan& p = *ctx->batch;
+  ctx->batch = nullptr;
+  // built aside and installed only once every allocation has succeeded, so a failed build never leaves a plan
+  // that a later call of the same shape would take for complete
+  std::unique_ptr<BatchPlan> plan(new BatchPlan());
+  BatchPlan& p = *plan;
   p.n = n;
   p.nrhs = r;
   p.hs = hs;
@@ -1279,6 +1482,7 @@ int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bo
   HIPCHK(p.B.alloc(sizeof(double) * p.b_total));
   HIPCHK(p.X.alloc(sizeof(double) * p.b_total));
   HIPCHK(hipMemcpyAsync(p.meshes.p, p.mh.data(), sizeof(BatchMeshDev) * nmesh, hipMemcpyHostToDevice, st));
+  ctx->batch = plan.release();
   return CDJ_OK;
 }
 
@@ -1597,7 +1801,8 @@ int batch_solve_mixed_finish(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes,
   for (int64_t m = 0; m < nmesh; ++m) {
     *worst = std::max(*worst, res[m]);
     *iters += rounds[m] + fit[m];
-    *path = std::max(*path, mpath[m]);
+    // the batch's path is FP64 as soon as one mesh fell back (FP64 is the smallest code)
+    *path = m == 0 ? mpath[m] : std::min(*path, mpath[m]);
     if (reports) {
       reports[m].path = mpath[m];
       reports[m].iterations = rounds[m] + fit[m];
@@ -1611,9 +1816,12 @@ int batch_solve_mixed_finish(cdj_ctx* ctx, BatchPlan& p, const cdj_mesh* meshes,
 
 extern "C" {
 
-int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double nu, double G, double lambda,
+}  // extern "C"
+
+namespace {
+
+int batched_slip_run(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double nu, double G, double lambda,
                      int32_t hs_flag, const cdj_opts* opts, cdj_report* reports, cdj_report* total) {
-  if (!ctx) return CDJ_ERR_ARG;
   if (nmesh < 0 || (nmesh > 0 && !meshes)) return fail(ctx, CDJ_ERR_ARG, "NULL argument");
   bool mixed = false;
   double near_factor = 2.0;
@@ -1734,6 +1942,22 @@ int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double
   return CDJ_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int cdj_batched_slip(cdj_ctx* ctx, const cdj_mesh* meshes, int64_t nmesh, double nu, double G, double lambda,
+                     int32_t hs_flag, const cdj_opts* opts, cdj_report* reports, cdj_report* total) {
+  if (!ctx) return CDJ_ERR_ARG;
+  const int rc = batched_slip_run(ctx, meshes, nmesh, nu, G, lambda, hs_flag, opts, reports, total);
+  if (rc != CDJ_OK) {
+    // whatever failed (an allocation, a capture, a solve), the next call starts from a fresh plan
+    batch_plan_free(ctx->batch);
+    ctx->batch = nullptr;
+  }
+  return rc;
+}
+
 int cdj_batch_tuning(const cdj_ctx* ctx, int32_t* tpb, int32_t* wgs) {
   if (!ctx || !ctx->batch) return CDJ_ERR_ARG;
   if (tpb) *tpb = ctx->batch->tpb;
@@ -1804,21 +2028,15 @@ int cdj_lstsq(cdj_ctx* ctx, const double* A, int64_t m, int64_t k, const double*
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   if (k == 0 || nrhs == 0) return CDJ_OK;
-  const int64_t N = m + k;
-  DBuf dA, KB;
+  DBuf dA, dB, dX;
   HIPCHK(dA.alloc(sizeof(double) * m * k));
-  HIPCHK(KB.alloc(sizeof(double) * N * (N + nrhs)));
+  HIPCHK(dB.alloc(sizeof(double) * m * nrhs));
+  HIPCHK(dX.alloc(sizeof(double) * k * nrhs));
   HIPCHK(hipMemcpyAsync(dA.p, A, sizeof(double) * m * k, hipMemcpyHostToDevice, st));
-  build_kkt(dA.as<double>(), m, m, k, KB.as<double>(), N, st);
-  // right-hand sides [b; 0]
-  double* R = KB.as<double>() + N * N;
-  HIPCHK(hipMemsetAsync(R, 0, sizeof(double) * N * nrhs, st));
-  for (int64_t q = 0; q < nrhs; ++q)
-    HIPCHK(hipMemcpyAsync(R + q * N, b + q * m, sizeof(double) * m, hipMemcpyHostToDevice, st));
-  int rc = solve_augmented_fp64(ctx, KB.as<double>(), N, N, nrhs, nullptr, nullptr, report, st);
+  HIPCHK(hipMemcpyAsync(dB.p, b, sizeof(double) * m * nrhs, hipMemcpyHostToDevice, st));
+  int rc = lstsq_dev(ctx, dA.as<double>(), m, k, m, dB.as<double>(), nrhs, dX.as<double>(), report, st);
   if (rc) return rc;
-  for (int64_t q = 0; q < nrhs; ++q)
-    HIPCHK(hipMemcpyAsync(x + q * k, R + q * N + m, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(x, dX.p, sizeof(double) * k * nrhs, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   return CDJ_OK;
 }

@@ -423,7 +423,37 @@ __global__ void tdk_near_info(const double* __restrict__ P1, const double* __res
   near[j * 4 + 3] = factor * factor * l;
 }
 
+// ---- fixed elements: traction rows at free receivers, displacement rows at fixed ones ------------------
+// (PutDispInfsIntoInfMat / ConcatInfMats, CalculateInfluenceMatrices3D.jl:145-198).  Block k = 3 m + q of
+// the output (mode m, traction q) takes rows of Tf block k at free receivers and rows of Ud block k (the
+// DispInf field at the same position: DnTn <- DnUx, DnTss <- DnUy, ...) at fixed ones.  Only free sources
+// were evaluated, so the columns are already those the reference keeps.  rowmap[r] = index among the free
+// receivers, or -1 - index among the fixed ones.  Tf: 9 blocks nf x nsrc (ld nf); Ud: 9 blocks nfix x nsrc.
+__global__ void __launch_bounds__(256) tdk_fixed_scatter(const double* __restrict__ Tf, const double* __restrict__ Ud,
+                                                         const int32_t* __restrict__ rowmap, int64_t n, int64_t nf,
+                                                         int64_t nfix, int64_t nsrc, OutPtrs<double> out) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int32_t map = rowmap[r];
+  for (int64_t c = blockIdx.y; c < nsrc; c += gridDim.y) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const double v = map >= 0 ? Tf[(int64_t)k * nf * nsrc + map + c * nf]
+                                : Ud[(int64_t)k * nfix * nsrc + (-1 - map) + c * nfix];
+      out.p[k][r + c * out.ld] = out.sign * v;
+    }
+  }
+}
+
 // ---- launch helpers --------------------------------------------------------------------------------------
+void launch_fixed_scatter(const double* Tf, const double* Ud, const int32_t* rowmap, int64_t n, int64_t nf,
+                          int64_t nfix, int64_t nsrc, const OutPtrs<double>& out, hipStream_t st) {
+  if (n <= 0 || nsrc <= 0) return;
+  const unsigned gy = (unsigned)(nsrc < 65535 ? nsrc : 65535);
+  hipLaunchKernelGGL(tdk_fixed_scatter, dim3((unsigned)((n + 255) / 256), gy), dim3(256), 0, st, Tf, Ud, rowmap, n,
+                     nf, nfix, nsrc, out);
+}
+
 void launch_setup(const double* P1, const double* P2, const double* P3, int64_t ntri, int hs, double* rec,
                   hipStream_t st) {
   if (ntri <= 0) return;

@@ -44,6 +44,11 @@ void launch_traction(const double* recv, int64_t nrecv, const T* rec, int64_t nt
 // FP32 arithmetic, FP64 storage (far field of the mixed assembly)
 void launch_traction_f32_to_f64(const double* recv, int64_t nrecv, const float* rec, int64_t ntri, int hs, int tpb,
                                 float nu, float lam, float G, const OutPtrs<double>& out, hipStream_t st);
+// Fixed-element rows: out block k (rows 0..n-1, columns 0..nsrc-1, times out.sign) from the compact traction
+// rows Tf (9 blocks nf x nsrc) and displacement rows Ud (9 blocks nfix x nsrc); rowmap[r] >= 0 selects free row
+// rowmap[r], otherwise fixed row -1 - rowmap[r].
+void launch_fixed_scatter(const double* Tf, const double* Ud, const int32_t* rowmap, int64_t n, int64_t nf,
+                          int64_t nfix, int64_t nsrc, const OutPtrs<double>& out, hipStream_t st);
 // near[j*4] = {centroid, (factor * longest edge)^2}
 void launch_near_info(const double* P1, const double* P2, const double* P3, int64_t ntri, double factor, double* near,
                       hipStream_t st);

@@ -106,16 +106,27 @@ int cdj_td_field(cdj_ctx* ctx, const double* X, const double* Y, const double* Z
                  double* const* strain6, double* const* disp3);
 
 /* Traction influence matrices: replaces CalculateInfluenceMatrices3D(FaceNormalVector,MidPoint,P1,P2,P3,nu,G,
- * lambda,FixedEls,HSFlag,n) (CalculateInfluenceMatrices3D.jl:2-48) for FixedEls all zero (fixed must be NULL or
- * all zero).  T: 9 N x N matrices in TractionInf field order (DnTn, DnTss, DnTds, DssTn, DssTss, DssTds, DdsTn,
- * DdsTss, DdsTds). */
+ * lambda,FixedEls,HSFlag,n) (CalculateInfluenceMatrices3D.jl:2-48).  T: 9 matrices in TractionInf field order
+ * (DnTn, DnTss, DnTds, DssTn, DssTss, DssTds, DdsTn, DdsTss, DdsTds).
+ * fixed: NULL or n flags (FixedEls; 1 = fixed).  Without fixed elements each T[k] is N x N.  With nfix fixed
+ * elements each T[k] is N x nf (nf = N - nfix, leading dimension N), as the reference's FixedEls branch returns
+ * (CalculateInfluenceMatrices3D.jl:10-33, 145-198): a fixed element's row holds the displacement influence at its
+ * collocation point (DnTn <- DnUx, DnTss <- DnUy, DnTds <- DnUz, ...), a free element's row the traction, where
+ * the i-th free row takes element i's plane as the reference's ConvertStrainInfMatsToTraction does when handed
+ * the free rows only; the fixed sources' columns are dropped.  The fixed form is assembled in FP64 whatever
+ * opts->assembly. */
 int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
                            const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
                            int32_t hs_flag, const int32_t* fixed, double* const* T, const cdj_opts* opts);
 
 /* Slip solve: replaces the base method of SlipCalculator3D (SlipCalculator3D.jl:1-33).  b: 3N x nrhs
- * column-major right-hand sides [Tn; Tss; Tds] as built by SetupTractionVector; x: 3N x nrhs solution
- * [Dn; Dss; Dds]; A_out: 3N x 3N (the assembled A) or NULL. */
+ * column-major right-hand sides [Tn; Tss; Tds] as built by SetupTractionVector (and KnockOutFixedRows);
+ * x: 3N x nrhs solution [Dn; Dss; Dds]; A_out: 3N x 3N (the assembled A) or NULL.
+ * fixed: NULL or N flags (FixedEls).  With nfix > 0 fixed elements, A is the 3N x 3nf matrix of the reference's
+ * FixedEls branch (nf = N - nfix; see cdj_traction_influence), x (3nf x nrhs) its least-squares solution
+ * [Dn; Dss; Dds] of the free elements (Julia's `\` on a non-square A), A_out 3N x 3nf; FP64 whatever
+ * opts->precision, through the augmented system of cdj_lstsq (3N + 3nf <= 65535); report->residual is then the
+ * least-squares residual ||b - A x|| / ||b||. */
 int cdj_slip_calculator(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,
                         double G, double lambda, const double* midpoints, const double* normals, int32_t hs_flag,
                         const double* b, int64_t nrhs, const int32_t* fixed, double* x, double* A_out,

@@ -103,18 +103,22 @@ function TD(X, Y, Z, P1List, P2List, P3List, Dss, Dds, Dn, ν, G, DispFlag, Stra
     return (S, D)
 end
 
+# FixedEls as the library's Int32 flags (1 = fixed), C_NULL when no element is fixed
+_fixed(FixedEls) = any(FixedEls .== 1) ? Int32.(vec(FixedEls) .== 1) : C_NULL
+
 function CalculateInfluenceMatrices3D(FaceNormalVector, MidPoint, P1, P2, P3, ν, G, λ, FixedEls, HSFlag, n)
-    any(FixedEls .== 1) && error("fixed elements: use the reference implementation")
-    T = [zeros(n, n) for _ in 1:9]
+    fx = _fixed(FixedEls)
+    nf = fx === C_NULL ? n : n - sum(fx)
+    T = [zeros(n, nf) for _ in 1:9]       # N x nf with fixed elements (CalculateInfluenceMatrices3D.jl:10-33)
     tp = [pointer(a) for a in T]
     opts = Ref(Opts())
     nrm = _f64(FaceNormalVector); mid = _f64(MidPoint)
     lock(LOCK) do
-        GC.@preserve T tp nrm mid begin
+        GC.@preserve T tp nrm mid fx begin
             check(ccall((:cdj_traction_influence, LIB), Cint,
                 (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64,
                  Float64, Float64, Int32, Ptr{Int32}, Ptr{Ptr{Float64}}, Ref{Opts}),
-                ctx(), nrm, mid, _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, HSFlag, C_NULL, tp, opts))
+                ctx(), nrm, mid, _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, HSFlag, fx, tp, opts))
         end
     end
     return TractionInf(T...)
@@ -124,18 +128,22 @@ function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSF
     n = size(FaceNormalVector, 1)
     (Tn, Tds, Tss) = SetupTractionVector(BoundaryConditions, FaceNormalVector, n, G, λ)
     (Tn, Tds, Tss) = KnockOutFixedRows(FixedEls, Tn, Tds, Tss)
-    any(FixedEls .== 1) && error("fixed elements: use the reference implementation")
     b = [Tn; Tss; Tds]
-    x = zeros(3n); A = zeros(3n, 3n)
+    fx = _fixed(FixedEls)
+    nf = fx === C_NULL ? n : n - sum(fx)  # with fixed elements A is 3n x 3nf and x its least-squares solution
+    x = zeros(3nf); A = zeros(3n, 3nf)
     opts = Ref(Opts()); rep = Ref(Report())
     lock(LOCK) do
-        check(ccall((:cdj_slip_calculator, LIB), Cint,
-            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Float64, Ptr{Float64},
-             Ptr{Float64}, Int32, Ptr{Float64}, Int64, Ptr{Int32}, Ptr{Float64}, Ptr{Float64}, Ref{Opts}, Ref{Report}),
-            ctx(), _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, _f64(MidPoint), _f64(FaceNormalVector), HSFlag,
-            _f64(b), 1, C_NULL, x, A, opts, rep))
+        GC.@preserve fx begin
+            check(ccall((:cdj_slip_calculator, LIB), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Float64, Float64, Float64, Ptr{Float64},
+                 Ptr{Float64}, Int32, Ptr{Float64}, Int64, Ptr{Int32}, Ptr{Float64}, Ptr{Float64}, Ref{Opts},
+                 Ref{Report}),
+                ctx(), _f64(P1), _f64(P2), _f64(P3), n, ν, G, λ, _f64(MidPoint), _f64(FaceNormalVector), HSFlag,
+                _f64(b), 1, fx, x, A, opts, rep))
+        end
     end
-    return x[1:n], x[n+1:2n], x[2n+1:3n], A, b
+    return x[1:nf], x[nf+1:2nf], x[2nf+1:3nf], A, b
 end
 
 function SlipCalculator3D(P1, P2, P3, ν, G, λ, MidPoint, FaceNormalVector, HSFlag,

@@ -226,3 +226,21 @@ def traction_influence_rows(normals, mid, P1, P2, P3, nu, G, lam, hs_flag, r0, r
     if rc:
         raise OracleError(lib().orc_last_error().decode())
     return dict(zip(TRACTION_FIELDS, T))
+
+
+def traction_influence_fixed(normals, mid, P1, P2, P3, nu, G, lam, hs_flag, fixed, nthreads=0):
+    """CalculateInfluenceMatrices3D with fixed elements -> dict of the 9 TractionInf matrices, each n x nf."""
+    L = lib()
+    f = L.orc_traction_influence_fixed
+    f.restype = _i
+    f.argtypes = [_p, _p, _p, _p, _p, _i64, _d, _d, _d, _i, _p, _p, _i]
+    normals, mid = _f64(normals), _f64(mid)
+    P1, P2, P3, n = _tri_arrays(P1, P2, P3)
+    fx = np.ascontiguousarray((np.asarray(fixed).ravel() == 1).astype(np.int32))
+    nf = n - int(fx.sum())
+    T = [np.zeros((n, nf), order="F") for _ in range(9)]
+    rc = f(_ptr(normals), _ptr(mid), _ptr(P1), _ptr(P2), _ptr(P3), n, float(nu), float(G), float(lam), int(hs_flag),
+           _ptr(fx), _ptr_array(T), int(nthreads))
+    if rc:
+        raise OracleError(lib().orc_last_error().decode())
+    return dict(zip(TRACTION_FIELDS, T))

@@ -247,6 +247,69 @@ int orc_traction_influence(const double* normals, const double* mid, const doubl
   return 0;
 }
 
+// CalculateInfluenceMatrices3D with fixed elements (CalculateInfluenceMatrices3D.jl:10-33, 145-198).
+//   * TD displacement at the fixed collocation points (:13-19) and TD strain at the free ones (:21-28), both for
+//     every source with unit slip;
+//   * the free strain rows go through ConvertStrainInfMatsToTraction with the full CosAx/CosAy/CosAz arrays
+//     (:26), whose loops index the normals and the strike/dip cosines by row (CalculateNormalTraction3D!.jl:37,
+//     CalculateTractionInChosenDirection3D!.jl:47): the t-th free row takes element t's plane;
+//   * PutDispInfsIntoInfMat (:145-176) writes the fixed elements' rows from the DispInf field at the same
+//     position (DnTn <- DnUx, DnTss <- DnUy, DnTds <- DnUz, ...) and the free rows from the traction, then
+//     keeps the columns of the free sources.
+// fixed: n flags (1 = fixed).  T9: TractionInf order, each n x nf column-major (nf = free count).
+int orc_traction_influence_fixed(const double* normals, const double* mid, const double* P1, const double* P2,
+                                 const double* P3, int64_t n, double nu, double G, double lam, int hs_flag,
+                                 const int* fixed, double* const* T9, int nthreads) {
+  std::vector<double> x(n), y(n), z(n);
+  for (int64_t i = 0; i < n; ++i) {
+    x[i] = mid[i] - (normals[i] * 1e-12);
+    y[i] = mid[n + i] - (normals[n + i] * 1e-12);
+    z[i] = mid[2 * n + i] - (normals[2 * n + i] * 1e-12);
+  }
+  int rc = check_inputs(z.data(), n, P1, P2, P3, n, hs_flag);
+  if (rc) return rc;
+  std::vector<double> ss(3 * n), ds(3 * n);
+  orc_ss_ds_dirs(normals, n, ss.data(), ds.data());
+  // row r -> free count t (traction with element t's plane) or fixed; source j -> output column
+  std::vector<int64_t> rowfree(n, -1), col(n, -1);
+  int64_t nf = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (fixed[i] != 1) {
+      rowfree[i] = nf;
+      col[i] = nf;
+      ++nf;
+    }
+  const bool hs = hs_flag == 1;
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int64_t j = 0; j < n; ++j) {
+    if (col[j] < 0) continue;
+    double a[3], b[3], c[3];
+    load_tri(P1, P2, P3, n, j, a, b, c);
+    TriFull<double> f;
+    orc::tri_full_setup(a, b, c, 1.0, 1.0, 1.0, hs, f);
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t o = i + col[j] * n;
+      if (rowfree[i] < 0) {
+        double u[3][3];
+        orc::disp_influence(f, hs, x[i], y[i], z[i], 1.0, 1.0, 1.0, nu, u);
+        for (int m = 0; m < 3; ++m)
+          for (int q = 0; q < 3; ++q) T9[3 * m + q][o] = u[m][q];
+        continue;
+      }
+      const int64_t t = rowfree[i];
+      double e[3][6];
+      orc::strain_influence(f, hs, x[i], y[i], z[i], 1.0, 1.0, 1.0, nu, e);
+      const double nv[3] = {normals[t], normals[n + t], normals[2 * n + t]};
+      const double sv[3] = {ss[t], ss[n + t], ss[2 * n + t]};
+      const double dv[3] = {ds[t], ds[n + t], ds[2 * n + t]};
+      for (int m = 0; m < 3; ++m)
+        strain_to_traction<double>(e[m], lam, G, nv, sv, dv, &T9[3 * m][o], &T9[3 * m + 1][o], &T9[3 * m + 2][o]);
+    }
+  }
+  return 0;
+}
+
 // CalculateNormalAndShearTractions3D.jl:37-58 for a per-element remote stress: returns Tn, Tds, Tss.
 void orc_normal_shear_tractions(const double* sxx, const double* syy, const double* szz, const double* sxy,
                                 const double* sxz, const double* syz, const double* normals, int64_t n, double* Tn,

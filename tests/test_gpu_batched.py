@@ -27,6 +27,13 @@ def _unbatched(m, lam, hs=0):
     return np.stack(cols, axis=1)
 
 
+def _all_on_path(rep, path):
+    """Every mesh's own report names `path` (the batch's path alone would hide a single fallback)."""
+    assert [q["path"] for q in rep["meshes"]] == [path] * len(rep["meshes"])
+    if path != 0:
+        assert rep["fallbacks"] == 0 and all(q["residual"] <= 1e-10 for q in rep["meshes"])
+
+
 @pytest.fixture(scope="module")
 def batch():
     return meshes.elliptical_cracks(count=6, n_min=100, n_max=500, rhs_max=3, seed=17)
@@ -56,6 +63,7 @@ def test_batched_equals_unbatched(batch, mode):
         # every mesh refined to the tolerance on the mixed path itself (no FP64 fallback)
         assert rep["residual"] <= 1e-10 and rep["iterations"] > 0
         assert rep["path"] == (3 if mode == "mixed16" else 1)
+        _all_on_path(rep, 3 if mode == "mixed16" else 1)
 
 
 def test_batched_mixed_graph_replays(batch):
@@ -85,6 +93,7 @@ def test_batched_mixed_graph_replays(batch):
         assert np.array_equal(a, b)
     for r in (r1, r2, r3):
         assert r["residual"] <= 1e-10 and r["path"] == 1
+        _all_on_path(r, 1)
 
 
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
@@ -110,6 +119,7 @@ def test_batched_halfspace_many_loads(mode):
         assert np.max(np.abs(x - ref)) <= tol * np.max(np.abs(ref))
     if mode == "mixed":
         assert rep["residual"] <= 1e-10 and rep["path"] == 1
+        _all_on_path(rep, 1)
 
 
 def test_batched_mixed_large_mesh_host_permutations():
@@ -132,6 +142,7 @@ def test_batched_mixed_large_mesh_host_permutations():
     for x, ref in zip(xs, refs):
         assert np.max(np.abs(x - ref)) <= 1e-8 * np.max(np.abs(ref))
     assert rep["residual"] <= 1e-10 and rep["path"] == 1
+    _all_on_path(rep, 1)
 
 
 def test_batched_mixed_fallback_chain(batch):
@@ -151,6 +162,8 @@ def test_batched_mixed_fallback_chain(batch):
     for x, ref in zip(xs, refs):
         assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
     assert rep["path"] == 0
+    _all_on_path(rep, 0)
+    assert rep["fallbacks"] == len(batch)
 
 
 def test_batched_graph_replay_with_new_geometry(batch):
