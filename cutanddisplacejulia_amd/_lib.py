@@ -47,7 +47,8 @@ EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", 
            "cdj_mlp_nparams", "cdj_mlp_get_params", "cdj_mlp_set_params", "cdj_mlp_train_dev", "cdj_mlp_loss_grad",
            "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning",
            "cdj_traction_rows", "cdj_lstsq", "cdj_crack_tip_elements", "cdj_stress_intensity",
-           "cdj_stress_intensity_dev", "cdj_slip_friction", "cdj_slip_fluid_volume", "cdj_mlp_forward"]
+           "cdj_stress_intensity_dev", "cdj_slip_friction", "cdj_slip_fluid_volume", "cdj_mlp_forward",
+           "cdj_assemble_dev_noalloc"]
 
 
 class CdjError(RuntimeError):
@@ -114,6 +115,7 @@ def load(path=LIB_PATH):
     L.cdj_slip_calculator.argtypes = [_p, _p, _p, _p, _i64, _d, _d, _d, _p, _p, _i32, _p, _i64, _p, _p, _p, _p, _p]
     L.cdj_solve.argtypes = [_p, _p, _p, _i64, _i64, _p, _p, _p]
     L.cdj_assemble_dev.argtypes = [_p, _p, _p, _p, _p, _p, _i64, _d, _d, _d, _i32, _p, _i64, _p, _p]
+    L.cdj_assemble_dev_noalloc.argtypes = L.cdj_assemble_dev.argtypes
     L.cdj_lu_factor_dev.argtypes = [_p, _p, _i64, _i64, _p, _p]
     L.cdj_lu_factor_batched_dev.argtypes = [_p, _p, _p, _p, _i64, _p, _p, _p]
     L.cdj_lu_solve_dev.argtypes = [_p, _p, _i64, _i64, _p, _p, _i64, _i64, _p]

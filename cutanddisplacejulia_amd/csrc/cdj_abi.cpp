@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +30,11 @@ struct cdj_mlp {
 struct BatchPlan;
 void batch_plan_free(BatchPlan* p);
 
+// Scratch of cdj_assemble_dev_noalloc: a fixed device array, one copy per GPU, held until the process ends.  In host
+// code `scratch` is only a stand-in for the symbol; its device address on the current GPU comes from
+// hipGetSymbolAddress (noalloc_scratch_ptr) and only that address is handed to kernels.
+static __device__ __align__(256) unsigned char scratch[CDJ_NOALLOC_SCRATCH_BYTES];
+
 struct cdj_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -43,11 +49,29 @@ struct cdj_ctx {
   int64_t near_entries = 0;
   // device buffers, tuned tiles and captured graph of the last batch shape (cdj_batched_slip)
   BatchPlan* batch = nullptr;
+  // this device's copy of `scratch` (cdj_assemble_dev_noalloc), shared by every context on the device
+  char* noalloc_scratch = nullptr;
 };
 
 namespace {
 
 thread_local std::string g_create_err;
+
+// Device address of `scratch` on `device` (which must be current), looked up once per device.
+hipError_t noalloc_scratch_ptr(int device, char** out) {
+  static std::mutex mu;
+  static std::map<int, char*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it == addr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    it = addr.emplace(device, static_cast<char*>(p)).first;
+  }
+  *out = it->second;
+  return hipSuccess;
+}
 
 int fail(cdj_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -135,8 +159,8 @@ hipError_t traction_mixed(const double* recv, const double* rec, float* rec32, d
   const int RS = hs ? REC_HS : REC_FS;
   launch_rec_to_f32(rec, rec32, (int64_t)RS * n, st);
   launch_near_info(P1, P2, P3, n, near_factor, near, st);
-  hipError_t e = hipMemsetAsync(cnt, 0, sizeof(unsigned long long), st);
-  if (e != hipSuccess) return e;
+  launch_zero_u32(reinterpret_cast<unsigned*>(cnt), 2, st);
+  hipError_t e = hipSuccess;
   if (ev0 && (e = hipEventRecord(ev0, st)) != hipSuccess) return e;
   launch_traction_f32_to_f64(recv, n, rec32, n, hs, hs ? 8 : 32, (float)nu, (float)lambda, (float)G, out, st);
   launch_nearfield(recv, n, rec, near, n, hs, tpb, nu, lambda, G, out, cnt, st);
@@ -272,6 +296,12 @@ int cdj_create(int device, cdj_ctx** out) {
   }
   cdj_ctx* c = new cdj_ctx();
   c->device = device;
+  e = noalloc_scratch_ptr(device, &c->noalloc_scratch);
+  if (e != hipSuccess) {
+    g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    delete c;
+    return CDJ_ERR_DEVICE;
+  }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     g_create_err = "hipEventCreate failed";
     delete c;
@@ -469,43 +499,43 @@ int cdj_td_field(cdj_ctx* ctx, const double* X, const double* Y, const double* Z
   return CDJ_OK;
 }
 
-int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
-                     const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
-                     int32_t hs_flag, double* A, int64_t lda, const cdj_opts* opts, void* stream) {
-  if (!ctx) return CDJ_ERR_ARG;
-  if (n < 0 || lda < 3 * n) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
-  HIPCHK(hipSetDevice(ctx->device));
-  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
-  if (n == 0) return CDJ_OK;
-  const int hs = hs_flag == 1;
+}  // extern "C"
+
+namespace {
+
+// Device scratch of the traction assembly of n triangles: a 256-byte header (the near-entry counter at 0, the
+// half-space check flag at 8), then receiver records, triangle records and, for the mixed mode, FP32 records and
+// the near-field table.
+size_t assemble_scratch_bytes(int64_t n, int hs, bool mixed) {
+  const size_t RS = hs ? REC_HS : REC_FS;
+  size_t b = 256 + sizeof(double) * (RECV + RS) * (size_t)n;
+  if (mixed) b += ((sizeof(float) * RS * (size_t)n + 255) / 256) * 256 + sizeof(double) * 4 * (size_t)n;
+  return b;
+}
+
+// cdj_assemble_dev on the scratch `scr` (assemble_scratch_bytes(n, hs, mixed) bytes of device memory, 256-byte
+// aligned); allocates nothing.
+int assemble_into(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1, const double* P2,
+                  const double* P3, int64_t n, double nu, double G, double lambda, int hs, bool mixed,
+                  double near_factor, double* A, int64_t lda, const cdj_opts* opts, char* scr, hipStream_t st) {
   const int RS = hs ? REC_HS : REC_FS;
-  bool mixed = false;
-  double near_factor = 2.0;
-  if (int rc = assembly_mode(ctx, opts, &mixed, &near_factor)) return rc;
-  // scratch: receivers | FP64 records | (mixed) FP32 records, near-field table, counter
-  const size_t base = sizeof(double) * (RECV + RS) * n;
-  const size_t extra = mixed ? sizeof(float) * RS * n + sizeof(double) * 4 * n + 64 : 0;
-  void* scr = nullptr;
-  HIPCHK(ctx_scratch(ctx, base + extra + 256, &scr));
-  double* recv_p = static_cast<double*>(scr);
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(scr);
+  int* flag = reinterpret_cast<int*>(scr + 8);
+  double* recv_p = reinterpret_cast<double*>(scr + 256);
   double* rec_p = recv_p + RECV * n;
   float* rec32 = reinterpret_cast<float*>(rec_p + RS * n);
-  double* near_p = reinterpret_cast<double*>(reinterpret_cast<char*>(scr) +
-                                             ((base + sizeof(float) * RS * n + 255) / 256) * 256);
-  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(near_p + 4 * n);
+  double* near_p = reinterpret_cast<double*>(reinterpret_cast<char*>(rec32) +
+                                             ((sizeof(float) * RS * (size_t)n + 255) / 256) * 256);
   launch_receivers(normals, midpoints, n, recv_p, st);
   if (hs) {
-    // collocation z-coordinates live in recv (stride RECV); check vertices and receivers on the host copy
-    std::vector<double> rv(RECV * n);
-    HIPCHK(hipMemcpyAsync(rv.data(), recv_p, sizeof(double) * RECV * n, hipMemcpyDeviceToHost, st));
-    std::vector<double> z1(n), z2(n), z3(n);
-    HIPCHK(hipMemcpyAsync(z1.data(), P1 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(z2.data(), P2 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(z3.data(), P3 + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    // collocation points (z in the receiver records) and triangle corners must lie in z <= 0 (TD.jl:606-619)
+    launch_zero_u32(reinterpret_cast<unsigned*>(flag), 1, st);
+    launch_check_hs(recv_p + 2, n, P1, P2, P3, n, flag, st, RECV);
+    HIPCHK(hipGetLastError());
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    for (int64_t i = 0; i < n; ++i)
-      if (rv[RECV * i + 2] > 0 || z1[i] > 0 || z2[i] > 0 || z3[i] > 0)
-        return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
+    if (h) return fail(ctx, CDJ_ERR_HALFSPACE, "Half-space solution: Z coordinates must be negative!");
   }
   launch_setup(P1, P2, P3, n, hs, rec_p, st);
   OutPtrs<double> out{};
@@ -533,6 +563,51 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
   HIPCHK(hipStreamSynchronize(st));
   ctx->near_entries = (int64_t)h;
   return CDJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
+                     const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                     int32_t hs_flag, double* A, int64_t lda, const cdj_opts* opts, void* stream) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0 || lda < 3 * n) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (n == 0) return CDJ_OK;
+  const int hs = hs_flag == 1;
+  bool mixed = false;
+  double near_factor = 2.0;
+  if (int rc = assembly_mode(ctx, opts, &mixed, &near_factor)) return rc;
+  void* scr = nullptr;
+  HIPCHK(ctx_scratch(ctx, assemble_scratch_bytes(n, hs, mixed), &scr));
+  return assemble_into(ctx, normals, midpoints, P1, P2, P3, n, nu, G, lambda, hs, mixed, near_factor, A, lda, opts,
+                       static_cast<char*>(scr), st);
+}
+
+int cdj_assemble_dev_noalloc(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
+                             const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                             int32_t hs_flag, double* A, int64_t lda, const cdj_opts* opts, void* stream) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (n < 0 || lda < 3 * n) return fail(ctx, CDJ_ERR_DIMS, "Element size inputs must be the same dimensions");
+  const int hs = hs_flag == 1;
+  bool mixed = false;
+  double near_factor = 2.0;
+  if (int rc = assembly_mode(ctx, opts, &mixed, &near_factor)) return rc;
+  const size_t need = assemble_scratch_bytes(n, hs, mixed);
+  if (need > CDJ_NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, CDJ_ERR_ARG,
+                "cdj_assemble_dev_noalloc: " + std::to_string(n) + " triangles need " + std::to_string(need) +
+                    " bytes of scratch, above the fixed " + std::to_string((long long)CDJ_NOALLOC_SCRATCH_BYTES) +
+                    " bytes (use cdj_assemble_dev)");
+  if (!ctx->noalloc_scratch) return fail(ctx, CDJ_ERR_DEVICE, "no-allocation scratch address unavailable");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (n == 0) return CDJ_OK;
+  return assemble_into(ctx, normals, midpoints, P1, P2, P3, n, nu, G, lambda, hs, mixed, near_factor, A, lda, opts,
+                       ctx->noalloc_scratch, st);
 }
 
 int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,

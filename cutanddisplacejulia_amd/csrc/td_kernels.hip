@@ -212,12 +212,13 @@ __global__ void tdk_receivers(const double* __restrict__ nrm, const double* __re
 }
 
 // ---- HS validity check (ViewInLoop TD.jl:606-619) --------------------------------------------------------
-__global__ void tdk_check_hs(const double* __restrict__ Z, int64_t nobs, const double* __restrict__ P1,
-                             const double* __restrict__ P2, const double* __restrict__ P3, int64_t ntri,
-                             int* __restrict__ flag) {
+// Z[i * zstride] for i < nobs: the observation points' z coordinates (a plain array, or the receiver records)
+__global__ void tdk_check_hs(const double* __restrict__ Z, int64_t zstride, int64_t nobs,
+                             const double* __restrict__ P1, const double* __restrict__ P2,
+                             const double* __restrict__ P3, int64_t ntri, int* __restrict__ flag) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   bool bad = false;
-  if (i < nobs) bad = Z[i] > 0;
+  if (i < nobs) bad = Z[i * zstride] > 0;
   if (i < ntri) bad = bad || P1[2 * ntri + i] > 0 || P2[2 * ntri + i] > 0 || P3[2 * ntri + i] > 0;
   if (bad) atomicOr(flag, 1);
 }
@@ -478,11 +479,21 @@ void launch_receivers_at(const double* nrm, const double* pts, int64_t n, double
 }
 
 void launch_check_hs(const double* Z, int64_t nobs, const double* P1, const double* P2, const double* P3,
-                     int64_t ntri, int* flag, hipStream_t st) {
+                     int64_t ntri, int* flag, hipStream_t st, int64_t zstride) {
   const int64_t n = nobs > ntri ? nobs : ntri;
   if (n <= 0) return;
-  hipLaunchKernelGGL(tdk_check_hs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Z, nobs, P1, P2, P3, ntri,
-                     flag);
+  hipLaunchKernelGGL(tdk_check_hs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Z, zstride, nobs, P1, P2, P3,
+                     ntri, flag);
+}
+
+__global__ void tdk_zero_u32(unsigned* __restrict__ p, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+void launch_zero_u32(unsigned* p, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(tdk_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
 
 template <class T>

@@ -28,8 +28,11 @@ void launch_rec_to_f32(const double* a, float* b, int64_t n, hipStream_t st);
 void launch_receivers(const double* nrm, const double* mid, int64_t n, double* recv, hipStream_t st);
 // receivers at given points (n x 3) with the planes of the given normals
 void launch_receivers_at(const double* nrm, const double* pts, int64_t n, double* recv, hipStream_t st);
+// *flag |= 1 when a point (Z[i * zstride], i < nobs) or a triangle corner has z > 0 (TD.jl:606-619)
 void launch_check_hs(const double* Z, int64_t nobs, const double* P1, const double* P2, const double* P3,
-                     int64_t ntri, int* flag, hipStream_t st);
+                     int64_t ntri, int* flag, hipStream_t st, int64_t zstride = 1);
+// p[0..n) = 0 (a kernel, not a memset node, so that the clear can sit in a captured graph)
+void launch_zero_u32(unsigned* p, int64_t n, hipStream_t st);
 template <class T>
 void launch_influence(const T* X, const T* Y, const T* Z, int64_t nobs, const T* rec, int64_t ntri, int hs, int tpb,
                       const double* Dn, const double* Dss, const double* Dds, T nu, const OutPtrs<T>& out,

@@ -160,6 +160,19 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
                      const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
                      int32_t hs_flag, double* A, int64_t lda, const cdj_opts* opts, void* stream);
 
+/* cdj_assemble_dev without any device allocation, for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same arguments, layout,
+ * kernels and results as cdj_assemble_dev; its scratch (receiver and triangle records, and for CDJ_ASM_MIXED the
+ * FP32 records, near-field table and counter) is a fixed array of CDJ_NOALLOC_SCRATCH_BYTES in device memory, one per
+ * GPU, held until the process ends.  Needs 256 + 8 (RECV + RS) n bytes, RECV = 12, RS = 32 (full space) or 160
+ * (half space), plus 4 RS n (rounded up to 256) + 32 n for CDJ_ASM_MIXED: at most 95324 triangles in full space and
+ * 24385 in half space (FP64 assembly); larger n is rejected with CDJ_ERR_ARG.  Every call on one GPU uses the same
+ * array, whatever the context: calls that overlap on one GPU (other threads, other streams) race. */
+#define CDJ_NOALLOC_SCRATCH_BYTES (32ll << 20)
+int cdj_assemble_dev_noalloc(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
+                             const double* P2, const double* P3, int64_t n, double nu, double G, double lambda,
+                             int32_t hs_flag, double* A, int64_t lda, const cdj_opts* opts, void* stream);
+
 /* TD influence on device arrays; same layout as cdj_td_influence. */
 int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
                          const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
