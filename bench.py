@@ -35,16 +35,22 @@ METRIC = "TD influence entries/s; slip-solve time at 1e-10 resid, mixed/learned 
 # runs on the FP64 vector ALUs (the bound is VALU issue, 78.6 TFLOP/s on MI355X); "executed_frac" is the fraction
 # of that peak in the FP64 operations the kernel actually issues (below).
 FLOPS_PER_PAIR_HS = 43940.0
-# FP64 flops the GPU kernel executes per pair on the same workload, from rocprofv3 --pmc SQ_INSTS_VALU_ADD_F64,
-# _MUL_F64, _FMA_F64 (x64 lanes, FMA twice): shared subexpressions and the forward-mode free-surface strain make
-# it 2.7x less than the formulation's count above.
-EXECUTED_FLOPS_PER_PAIR_HS = 16252.5
+# Algorithmic work of one (receiver, mesh edge) unit of the dominant kernel, tdk_edge_response: the oracle's
+# operation count of one triangle side's free-surface strain for the three slip modes (AngSetupStrainFSC), averaged
+# over 6000 random (receiver, side) pairs of this workload (tests/test_flops.py recomputes it).  The reference
+# evaluates it once per triangle side and pair; the engine once per mesh edge and receiver (7600 edges for the
+# 15000 sides of this mesh), then each triangle adds +- the responses of its edges (tdk_traction_combine).
+FLOPS_PER_EDGE_HS = 12428.0
+# FP64 flops tdk_edge_response executes per (receiver, edge) on the same workload, from rocprofv3 --pmc
+# SQ_INSTS_VALU_ADD_F64, _MUL_F64, _FMA_F64, _TRANS_F64 (x64 lanes, FMA twice): 1.664e11 per launch over
+# 5000 x 7600 units.
+EXECUTED_FLOPS_PER_EDGE_HS = 4378.5
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the assembly kernel on the 5000-triangle workload: rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE in separate runs, each scaled by what the counter reads for a 1 GiB device copy (fetch x2.0,
-# write x1.0): 9.67e7 B read + 2.073e9 B written.
-TRAFFIC_BYTES_HS_5000 = 2.169e9
+# HBM bytes per launch of tdk_edge_response on the 5000-triangle workload: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE in separate runs, each scaled by what the counter reads for a 1 GiB device copy (fetch x2, write x1):
+# 0.025 GB read + 2.736 GB written (its algorithmic 72 B per unit).
+TRAFFIC_BYTES_EDGE_HS_5000 = 2.761e9
 
 
 def _log(msg):
@@ -440,31 +446,41 @@ def main():
     sp = ctypes.c_void_p(stream.cuda_stream)
     opts = _lib.Opts(tol=1e-10, near_factor=2.0)
 
+    edges = ctypes.c_int64()
+    edge_ms = ctypes.c_double()
+
     def step():
         rc = L.cdj_assemble_dev(ctx, ctypes.c_void_p(dN.data_ptr()), ctypes.c_void_p(dM.data_ptr()),
                                 ctypes.c_void_p(dP1.data_ptr()), ctypes.c_void_p(dP2.data_ptr()),
                                 ctypes.c_void_p(dP3.data_ptr()), n, nu, G, lam, 1, ctypes.c_void_p(A.data_ptr()), N3,
                                 ctypes.byref(opts), sp)
         _lib.check(rc)
-        return L.cdj_last_kernel_ms(ctx)
+        _lib.check(L.cdj_last_edge_stats(ctx, ctypes.byref(edges), ctypes.byref(edge_ms)))
+        return L.cdj_last_kernel_ms(ctx), edge_ms.value
 
     _log("workload: %d triangles, A %d x %d (half space)" % (n, N3, N3))
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    kms = []
+    kms, ems = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        kms.append(step())
+        k, e = step()
+        kms.append(k)
+        ems.append(e)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     entries = float(N3) * N3
     value = entries * args.steps / elapsed
-    kavg = float(np.mean(kms))
+    kavg = float(np.mean(kms))       # all assembly kernels of a step
+    eavg = float(np.mean(ems))       # tdk_edge_response alone
     pairs = float(n) * n
-    achieved_tf = FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12
-    _log("assembly: %.2f ms/step, kernel %.2f ms, %.3e entries/s, %.1f TFLOP/s" % (ms_per_step, kavg, value, achieved_tf))
+    units = float(edges.value) * n   # (receiver, edge) units of one edge-response launch
+    achieved_tf = FLOPS_PER_EDGE_HS * units / (eavg * 1e-3) / 1e12
+    formulation_tf = FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12
+    _log("assembly: %.2f ms/step, kernels %.2f ms (edge responses %.2f ms over %d edges), %.3e entries/s; edge "
+         "kernel %.1f TFLOP/s algorithmic" % (ms_per_step, kavg, eavg, edges.value, value, achieved_tf))
 
     if args.copy_mib > 0:
         src = torch.ones(args.copy_mib * (1 << 20) // 8, dtype=torch.float64, device=dev)
@@ -554,14 +570,17 @@ def main():
                                "A %d x %d FP64" % (n, N3, N3), "n_tri": n, "hs": 1},
         "roofline": {"bound": "valu", "achieved": achieved_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tf / PEAK_FP64_TFLOPS,
-                     "executed_frac": (EXECUTED_FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
-                                       if args.n == 50 else None),
-                     "traffic": TRAFFIC_BYTES_HS_5000 if args.n == 50 else None,
-                     "kernel": "tdk_traction<double, double, true>", "kernel_ms": kavg,
-                     "flops_per_pair": FLOPS_PER_PAIR_HS, "pairs_per_launch": pairs,
-                     "executed_flops_per_pair": EXECUTED_FLOPS_PER_PAIR_HS if args.n == 50 else None,
-                     "executed_tflops": (EXECUTED_FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12
-                                         if args.n == 50 else None)},
+                     "traffic": TRAFFIC_BYTES_EDGE_HS_5000 if args.n == 50 else None,
+                     "kernel": "tdk_edge_response", "kernel_ms": eavg, "flops_per_unit": FLOPS_PER_EDGE_HS,
+                     "unit_of_work": "(receiver, mesh edge)", "units_per_launch": units,
+                     "executed_flops_per_unit": EXECUTED_FLOPS_PER_EDGE_HS if args.n == 50 else None,
+                     "executed_frac": (EXECUTED_FLOPS_PER_EDGE_HS * units / (eavg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
+                                       if args.n == 50 and EXECUTED_FLOPS_PER_EDGE_HS else None),
+                     "assembly_kernels_ms": kavg,
+                     "assembly_formulation_tflops": formulation_tf,
+                     "note": "assembly_formulation_tflops counts the reference formulation's %g flops per pair over "
+                             "the whole assembly; it exceeds the peak because each mesh edge's free-surface part "
+                             "is evaluated once, not once per triangle side" % FLOPS_PER_PAIR_HS},
         "cpu_baseline": cpu,
         "parity_vs_oracle_rows": parity,
         "lu_15000": lu15,

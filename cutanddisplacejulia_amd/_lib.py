@@ -48,7 +48,7 @@ EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", 
            "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning",
            "cdj_traction_rows", "cdj_lstsq", "cdj_crack_tip_elements", "cdj_stress_intensity",
            "cdj_stress_intensity_dev", "cdj_slip_friction", "cdj_slip_fluid_volume", "cdj_mlp_forward",
-           "cdj_assemble_dev_noalloc"]
+           "cdj_assemble_dev_noalloc", "cdj_last_edge_stats"]
 
 
 class CdjError(RuntimeError):
@@ -96,6 +96,7 @@ def load(path=LIB_PATH):
     L.cdj_last_error.restype = ctypes.c_char_p
     L.cdj_last_kernel_ms.argtypes = [_p]
     L.cdj_last_near_entries.argtypes = [_p]
+    L.cdj_last_edge_stats.argtypes = [_p, _p, _p]
     L.cdj_batched_slip.argtypes = [_p, _p, _i64, _d, _d, _d, _i32, _p, _p, _p]
     L.cdj_batch_tuning.argtypes = [_p, _p, _p]
     L.cdj_traction_rows.argtypes = [_p, _p, _p, _i64, _p, _p, _p, _i64, _d, _d, _d, _i32, _p, _p]

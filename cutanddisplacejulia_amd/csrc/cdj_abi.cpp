@@ -42,9 +42,17 @@ struct cdj_ctx {
   // grow-only device scratch for per-call set-up records (triangle and receiver records)
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
-  // HIP events around the last assembly / TD kernel launch, on the stream it ran on
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // grow-only device scratch of the half-space assembly: edge topology and edge records, edge responses
+  void* scratch_topo = nullptr;
+  size_t scratch_topo_bytes = 0;
+  void* scratch_resp = nullptr;
+  size_t scratch_resp_bytes = 0;
+  // HIP events around the last assembly / TD kernel launch, on the stream it ran on; ev2..ev3 around the edge
+  // responses of the last half-space assembly (when it ran them in one launch)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
   bool timed = false;
+  bool edge_timed = false;
+  int64_t last_edges = 0;
   // entries recomputed in FP64 by the last mixed assembly
   int64_t near_entries = 0;
   // device buffers, tuned tiles and captured graph of the last batch shape (cdj_batched_slip)
@@ -104,18 +112,82 @@ struct DBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Grow-only device buffer (`buf`, `have` bytes) of at least `bytes`; the previous contents are not kept.
+hipError_t grow(void** buf, size_t* have, size_t bytes) {
+  if (bytes > *have) {
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    hipError_t e = hipMalloc(buf, bytes);
+    if (e != hipSuccess) return e;
+    *have = bytes;
+  }
+  return hipSuccess;
+}
+
 // Device scratch of at least `bytes` owned by the context (not valid across overlapping calls).
 hipError_t ctx_scratch(cdj_ctx* c, size_t bytes, void** out) {
-  if (bytes > c->scratch_bytes) {
-    if (c->scratch) (void)hipFree(c->scratch);
-    c->scratch = nullptr;
-    c->scratch_bytes = 0;
-    hipError_t e = hipMalloc(&c->scratch, bytes);
-    if (e != hipSuccess) return e;
-    c->scratch_bytes = bytes;
-  }
+  hipError_t e = grow(&c->scratch, &c->scratch_bytes, bytes);
   *out = c->scratch;
-  return hipSuccess;
+  return e;
+}
+
+// Edge responses are kept for at most this many bytes at once; larger problems take the receivers in chunks.
+constexpr size_t kEdgeResponseBytes = size_t(4) << 30;
+
+// FP64 half-space traction entries of nrecv receivers (device records `recv`) against the n source triangles
+// (device corners P1..P3, records `rec` built), written through `out`.  The free-surface part is evaluated once per
+// mesh edge and shared by the edge's triangles (tdk_edge_response, tdk_traction_combine); when a triangle lies in
+// z = 0 the fused per-pair kernel runs instead, since its image pass needs the accumulated strain.  With `timed`,
+// ev0..ev1 bracket the assembly kernels (and ev2..ev3 the edge responses when they run in one launch).
+int traction_hs_fp64(cdj_ctx* ctx, const double* recv, int64_t nrecv, const double* P1, const double* P2,
+                     const double* P3, const double* rec, int64_t n, int tpb, double nu, double lam, double G,
+                     const OutPtrs<double>& out, bool timed, hipStream_t st) {
+  ctx->edge_timed = false;
+  ctx->last_edges = 0;
+  if (nrecv <= 0 || n <= 0) return CDJ_OK;
+  const uint64_t S = edge_table_slots(n);
+  const size_t ints = 2 * S + 4 * 3 * (size_t)n + 2;
+  const size_t topo_bytes = ((sizeof(int) * ints + 255) / 256) * 256;
+  HIPCHK(grow(&ctx->scratch_topo, &ctx->scratch_topo_bytes, topo_bytes + sizeof(double) * REC_EDGE * 3 * n));
+  int* ip = static_cast<int*>(ctx->scratch_topo);
+  EdgeTopo topo{ip, ip + S, ip + 2 * S, ip + 2 * S + 3 * n, ip + 2 * S + 6 * n, ip + 2 * S + 9 * n,
+                ip + 2 * S + 12 * n};
+  double* erec = reinterpret_cast<double*>(static_cast<char*>(ctx->scratch_topo) + topo_bytes);
+  launch_edge_topology(P1, P2, P3, n, topo, st);
+  int cnt[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(cnt, topo.count, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  const int64_t E = cnt[0];
+  if (cnt[1] || E <= 0) {
+    if (timed) HIPCHK(hipEventRecord(ctx->ev0, st));
+    launch_traction<double>(recv, nrecv, rec, n, 1, tpb, nu, lam, G, out, st);
+    if (timed) HIPCHK(hipEventRecord(ctx->ev1, st));
+    HIPCHK(hipGetLastError());
+    return CDJ_OK;
+  }
+  ctx->last_edges = E;
+  int64_t chunk = (int64_t)(kEdgeResponseBytes / (sizeof(double) * 9 * (size_t)E));
+  chunk = std::max<int64_t>(256, chunk / 256 * 256);
+  chunk = std::min<int64_t>(chunk, nrecv);
+  HIPCHK(grow(&ctx->scratch_resp, &ctx->scratch_resp_bytes, sizeof(double) * 9 * (size_t)E * chunk));
+  double* Tb = static_cast<double*>(ctx->scratch_resp);
+  if (timed) HIPCHK(hipEventRecord(ctx->ev0, st));
+  launch_edge_setup(P1, P2, P3, n, rec, topo, erec, st);
+  const bool one = chunk == nrecv;
+  for (int64_t r0 = 0; r0 < nrecv; r0 += chunk) {
+    const int64_t m = std::min<int64_t>(chunk, nrecv - r0);
+    if (timed && one) HIPCHK(hipEventRecord(ctx->ev2, st));
+    launch_edge_response(recv + r0 * RECV, m, erec, E, 4, nu, lam, G, Tb, m, st);
+    if (timed && one) HIPCHK(hipEventRecord(ctx->ev3, st));
+    OutPtrs<double> o = out;
+    for (int k = 0; k < 9; ++k) o.p[k] = out.p[k] + r0;
+    launch_traction_combine(recv + r0 * RECV, m, rec, n, tpb, nu, lam, G, topo.side_edge, Tb, m, o, st);
+  }
+  if (timed) HIPCHK(hipEventRecord(ctx->ev1, st));
+  ctx->edge_timed = timed && one;
+  HIPCHK(hipGetLastError());
+  return CDJ_OK;
 }
 
 int default_tile(int hs, int64_t nobs) {
@@ -302,7 +374,8 @@ int cdj_create(int device, cdj_ctx** out) {
     delete c;
     return CDJ_ERR_DEVICE;
   }
-  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
     g_create_err = "hipEventCreate failed";
     delete c;
     return CDJ_ERR_DEVICE;
@@ -327,8 +400,12 @@ int cdj_destroy(cdj_ctx* ctx) {
   batch_plan_free(ctx->batch);
   ctx->batch = nullptr;
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->scratch_topo) (void)hipFree(ctx->scratch_topo);
+  if (ctx->scratch_resp) (void)hipFree(ctx->scratch_resp);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
+  if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CDJ_OK;
@@ -344,6 +421,17 @@ double cdj_last_kernel_ms(cdj_ctx* ctx) {
 }
 
 int64_t cdj_last_near_entries(const cdj_ctx* ctx) { return ctx ? ctx->near_entries : -1; }
+
+int cdj_last_edge_stats(cdj_ctx* ctx, int64_t* edges, double* ms) {
+  if (!ctx) return CDJ_ERR_ARG;
+  if (edges) *edges = ctx->last_edges;
+  if (ms) {
+    float t = -1.f;
+    if (!ctx->edge_timed || hipEventElapsedTime(&t, ctx->ev2, ctx->ev3) != hipSuccess) t = -1.f;
+    *ms = (double)t;
+  }
+  return CDJ_OK;
+}
 
 int cdj_td_influence_dev(cdj_ctx* ctx, const double* X, const double* Y, const double* Z, int64_t nobs,
                          const double* P1, const double* P2, const double* P3, int64_t ntri, const double* Dss,
@@ -514,10 +602,13 @@ size_t assemble_scratch_bytes(int64_t n, int hs, bool mixed) {
 }
 
 // cdj_assemble_dev on the scratch `scr` (assemble_scratch_bytes(n, hs, mixed) bytes of device memory, 256-byte
-// aligned); allocates nothing.
+// aligned).  With share_edges the FP64 half-space assembly evaluates each mesh edge's free-surface part once
+// (traction_hs_fp64, which keeps its own context buffers); without it nothing is allocated and every pair
+// evaluates its triangle's three sides.
 int assemble_into(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1, const double* P2,
                   const double* P3, int64_t n, double nu, double G, double lambda, int hs, bool mixed,
-                  double near_factor, double* A, int64_t lda, const cdj_opts* opts, char* scr, hipStream_t st) {
+                  double near_factor, double* A, int64_t lda, const cdj_opts* opts, char* scr, bool share_edges,
+                  hipStream_t st) {
   const int RS = hs ? REC_HS : REC_FS;
   unsigned long long* cnt = reinterpret_cast<unsigned long long*>(scr);
   int* flag = reinterpret_cast<int*>(scr + 8);
@@ -544,10 +635,16 @@ int assemble_into(cdj_ctx* ctx, const double* normals, const double* midpoints, 
   for (int m = 0; m < 3; ++m)
     for (int q = 0; q < 3; ++q) out.p[3 * m + q] = A + (int64_t)q * n + (int64_t)m * n * lda;
   const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, n);
+  ctx->edge_timed = false;
+  ctx->last_edges = 0;
   if (!mixed) {
-    HIPCHK(hipEventRecord(ctx->ev0, st));
-    launch_traction<double>(recv_p, n, rec_p, n, hs, tpb, nu, lambda, G, out, st);
-    HIPCHK(hipEventRecord(ctx->ev1, st));
+    if (hs && share_edges) {
+      if (int rc = traction_hs_fp64(ctx, recv_p, n, P1, P2, P3, rec_p, n, tpb, nu, lambda, G, out, true, st)) return rc;
+    } else {
+      HIPCHK(hipEventRecord(ctx->ev0, st));
+      launch_traction<double>(recv_p, n, rec_p, n, hs, tpb, nu, lambda, G, out, st);
+      HIPCHK(hipEventRecord(ctx->ev1, st));
+    }
     ctx->timed = true;
     ctx->near_entries = 0;
     HIPCHK(hipGetLastError());
@@ -584,7 +681,7 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
   void* scr = nullptr;
   HIPCHK(ctx_scratch(ctx, assemble_scratch_bytes(n, hs, mixed), &scr));
   return assemble_into(ctx, normals, midpoints, P1, P2, P3, n, nu, G, lambda, hs, mixed, near_factor, A, lda, opts,
-                       static_cast<char*>(scr), st);
+                       static_cast<char*>(scr), true, st);
 }
 
 int cdj_assemble_dev_noalloc(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
@@ -607,7 +704,7 @@ int cdj_assemble_dev_noalloc(cdj_ctx* ctx, const double* normals, const double* 
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   if (n == 0) return CDJ_OK;
   return assemble_into(ctx, normals, midpoints, P1, P2, P3, n, nu, G, lambda, hs, mixed, near_factor, A, lda, opts,
-                       ctx->noalloc_scratch, st);
+                       ctx->noalloc_scratch, false, st);
 }
 
 int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* midpoints, const double* P1,
@@ -687,7 +784,13 @@ int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* mi
     HIPCHK(hipStreamSynchronize(st));
     ctx->near_entries = (int64_t)h;
   } else {
-    launch_traction<double>(recv.as<double>(), n, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+    if (hs) {
+      if (int rc = traction_hs_fp64(ctx, recv.as<double>(), n, d1.as<double>(), d2.as<double>(), d3.as<double>(),
+                                    rec.as<double>(), n, tpb, nu, lambda, G, out, false, st))
+        return rc;
+    } else {
+      launch_traction<double>(recv.as<double>(), n, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+    }
     ctx->near_entries = 0;
   }
   HIPCHK(hipGetLastError());
@@ -2087,7 +2190,13 @@ int cdj_traction_rows(cdj_ctx* ctx, const double* points, const double* normals,
   out.sign = 1.0;
   for (int k = 0; k < 9; ++k) out.p[k] = dT.as<double>() + (int64_t)k * m * n;
   const int tpb = (opts && opts->tile > 0) ? opts->tile : default_tile(hs, m);
-  launch_traction<double>(recv.as<double>(), m, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+  if (hs) {
+    if (int rc = traction_hs_fp64(ctx, recv.as<double>(), m, d1.as<double>(), d2.as<double>(), d3.as<double>(),
+                                  rec.as<double>(), n, tpb, nu, lambda, G, out, false, st))
+      return rc;
+  } else {
+    launch_traction<double>(recv.as<double>(), m, rec.as<double>(), n, hs, tpb, nu, lambda, G, out, st);
+  }
   HIPCHK(hipGetLastError());
   for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(T[k], out.p[k], vm, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));

@@ -82,6 +82,7 @@ enum : int {
 };
 static_assert(REC_FS == R_TD_SIZE, "full-space record size");
 static_assert(REC_HS == 2 * R_TD_SIZE + 3 * R_SIDE_SIZE, "half-space record size");
+static_assert(REC_EDGE == R_SIDE_SIZE, "edge record size");
 static constexpr int OFF_IMG = R_TD_SIZE;
 static constexpr int OFF_SIDE = 2 * R_TD_SIZE;
 
@@ -500,7 +501,75 @@ __device__ __forceinline__ int td_disp_tdcs(const T* __restrict__ r, T X, T Y, T
 template <class T> __device__ __forceinline__ T qnan() { return __builtin_nan(""); }
 template <> __device__ __forceinline__ float qnan<float>() { return __builtin_nanf(""); }
 
+// Free-surface correction of one side at one point, strain (AngSetupStrainFSC TD.jl:1935-2075), before the
+// Burgers vector enters: w[k][c] = sum over the side's two angular dislocations (B minus A) of
+// {d1u1, d2u2, d3u3, d2u1+d1u2, d3u1+d1u3, d3u2+d2u3} of the FSC displacement for ADCS Burgers component k (the
+// strain-FSC closed form without its 1/(4 pi (1-nu))); sg = -1 in configuration I (beta x >= 0), whose v13, v23
+// and first two Burgers components are negated (TD.jl:1951-1962), +1 in configuration II.  The side must not be
+// a skipped one.
+template <class T>
+__device__ __forceinline__ void fsc_side_w(const T* __restrict__ s, T X, T Y, T Z, T nu, T w[3][6], T& sg) {
+  T x, y, z;
+  {
+#pragma clang fp contract(off)
+    const T dX = X - s[S_PA + 0], dY = Y - s[S_PA + 1], dZ = Z - s[S_PA + 2];
+    const T p = s[S_EY1 + 0], q = s[S_EY1 + 1];
+    x = (p * dX) + (q * dY) + (T(-0.0) * dZ);
+    y = (q * dX) + ((-p) * dY) + (T(-0.0) * dZ);
+    z = (T(0) * dX) + (T(0) * dY) + (T(-1) * dZ);
+  }
+  const bool cfg1 = x >= T(0);  // beta * x >= 0 with 0 < beta < pi
+  sg = cfg1 ? T(-1) : T(1);
+  const T cB = cfg1 ? s[S_COSPS] : s[S_COSB];
+  const T sB = cfg1 ? s[S_SINPS] : s[S_SINB];
+  const T ctB = cfg1 ? s[S_COTPS] : s[S_COTB];
+  const T ctB2 = cfg1 ? s[S_COTPS2] : s[S_COTB2];
+  const T icB = T(1) / cB;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) w[k][c] = T(0);
+#pragma unroll 1
+  for (int ad = 0; ad < 2; ++ad) {
+    const T px = ad == 0 ? x : x - s[S_YB + 0];
+    const T py = ad == 0 ? y : y - s[S_YB + 1];
+    const T pz = ad == 0 ? z : z - s[S_YB + 2];
+    const T aa = ad == 0 ? s[S_AA] : s[S_AB];
+    const T sgn = ad == 0 ? T(-1) : T(1);
+    FscBase<T, D3<T>> fb;
+    fsc_base<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
+                       D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
+    // one output at a time: output q = 3 c + k is component c of the displacement for Burgers component k;
+    // its gradient feeds v_cc (diagonal), and half of the two shear sums it appears in
+#pragma unroll 1
+    for (int q = 0; q < 9; ++q) {
+      const D3<T> u = fsc_out<T, D3<T>>(fb, q);
+      const int c = q / 3, k = q % 3;
+      const T g0 = sgn * u.a, g1 = sgn * u.b, g2 = sgn * u.c;
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        if (kk != k) continue;
+        if (c == 0) {
+          w[kk][0] += g0;
+          w[kk][3] += g1;
+          w[kk][4] += sg * g2;
+        } else if (c == 1) {
+          w[kk][3] += g0;
+          w[kk][1] += g1;
+          w[kk][5] += sg * g2;
+        } else {
+          w[kk][4] += sg * g0;
+          w[kk][5] += sg * g1;
+          w[kk][2] += g2;
+        }
+      }
+    }
+  }
+}
+
 // Free-surface correction of one side at one point, strain, added in EFCS (AngSetupStrainFSC TD.jl:1935-2075).
+// The loop is the one of fsc_side_w, kept inline here: sharing it changed the register allocation of the fused
+// kernels for the worse (more spill).
 template <class T>
 __device__ __forceinline__ void fsc_strain_side(const T* __restrict__ s, T X, T Y, T Z, T nu, T eps[3][6]) {
   if (s[S_SKIP] != T(0)) return;  // triangle-uniform
@@ -799,6 +868,33 @@ __device__ __forceinline__ void traction3(const T* e, T lam, T G, const T* n, co
   const T pz = -tx * (nz * nx) - ty * (ny * nz) + tz * (T(1) - nz * nz);
   Tss = px * s[0] + py * s[1] + pz * s[2];
   Tds = px * d[0] + py * d[1] + pz * d[2];
+}
+
+// One TD record's full-space strain (real, or image with the image record) at one point turned into the three
+// tractions of each mode on the receiver plane and added to t[m][q] (first: stored instead).  Traction is linear in
+// strain, so the parts of a half-space pair can be converted one by one (not for triangles lying in z = 0, whose
+// image pass needs the accumulated strain; see strain_pair).
+template <class T>
+__device__ __forceinline__ void td_part_traction(const T* __restrict__ r, T X, T Y, T Z, T nu, T lam, T G, const T* n,
+                                                 const T* s, const T* d, T t[3][3], bool first) {
+  T acc[3][6];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[m][c] = T(0);
+  const int mode = td_strain_tdcs(r, X, Y, Z, nu, acc);
+  const T bad = mode == 0 ? qnan<T>() : T(0);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[m][c] += bad;
+    tensor_rotate(acc[m], r + R_VN, r + R_VS, r + R_VD);
+    T a, b, c;
+    traction3(acc[m], lam, G, n, s, d, a, b, c);
+    t[m][0] = first ? a : t[m][0] + a;
+    t[m][1] = first ? b : t[m][1] + b;
+    t[m][2] = first ? c : t[m][2] + c;
+  }
 }
 
 }  // namespace cdj

@@ -424,6 +424,250 @@ __global__ void tdk_near_info(const double* __restrict__ P1, const double* __res
   near[j * 4 + 3] = factor * factor * l;
 }
 
+// ---- half-space assembly with the free-surface correction shared per mesh edge -------------------------
+// A triangle's free-surface strain is the sum over its three sides of AngSetupStrainFSC (TD.jl:1839-1863), and a
+// side traversed the other way round gives the same field with the opposite sign (the pair of angular
+// dislocations swaps its ends and orientation).  Two triangles with a common edge therefore need one evaluation
+// of that edge's pair, not two: the edge is evaluated once per receiver in its owner's direction, for the three
+// unit Burgers vectors of EFCS, turned into tractions on the receiver plane, and each triangle adds +- the
+// responses of its edges weighted by its slip directions.
+//
+// Side id e = k n + j: side k of triangle j with end points (P1, P2), (P2, P3), (P3, P1) for k = 0, 1, 2 (the
+// order of build_side).  Sides with identical end points (exact IEEE equality, either direction) form one edge
+// whose owner is the lowest side id, so the edge set and every value are the same run to run.
+__device__ __forceinline__ void side_ends(const double* P1, const double* P2, const double* P3, int64_t n, int64_t e,
+                                          double a[3], double b[3]) {
+  const int64_t j = e % n;
+  const int k = (int)(e / n);
+  const double* A = k == 0 ? P1 : (k == 1 ? P2 : P3);
+  const double* B = k == 0 ? P2 : (k == 1 ? P3 : P1);
+  for (int c = 0; c < 3; ++c) {
+    a[c] = A[c * n + j];
+    b[c] = B[c * n + j];
+  }
+}
+
+// end points in lexicographic order; returns false when a coordinate is NaN (such a side matches nothing)
+__device__ __forceinline__ bool side_key(const double* P1, const double* P2, const double* P3, int64_t n, int64_t e,
+                                         double k[6]) {
+  double a[3], b[3];
+  side_ends(P1, P2, P3, n, e, a, b);
+  bool swap = false;
+  if (a[0] != b[0])
+    swap = !(a[0] < b[0]);
+  else if (a[1] != b[1])
+    swap = !(a[1] < b[1]);
+  else
+    swap = !(a[2] <= b[2]);
+  for (int c = 0; c < 3; ++c) {
+    k[c] = (swap ? b[c] : a[c]) + 0.0;  // +0.0 folds -0.0 onto 0.0 (they compare equal)
+    k[3 + c] = (swap ? a[c] : b[c]) + 0.0;
+  }
+  bool ok = true;
+  for (int c = 0; c < 6; ++c) ok = ok && k[c] == k[c];
+  return ok;
+}
+
+__device__ __forceinline__ uint64_t key_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256) tdk_fill_i32(int* __restrict__ p, int64_t n, int v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Open-addressing table keyed by the sorted end points: slots[p] holds the first side stored there, own[p] the
+// lowest side id with that key; slot_of[e] = p (-1 for a side with a NaN coordinate).
+__global__ void __launch_bounds__(256) tdk_edge_insert(const double* __restrict__ P1, const double* __restrict__ P2,
+                                                       const double* __restrict__ P3, int64_t n,
+                                                       int* __restrict__ slots, int* __restrict__ own, uint64_t mask,
+                                                       int* __restrict__ slot_of) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= 3 * n) return;
+  double k[6];
+  if (!side_key(P1, P2, P3, n, e, k)) {
+    slot_of[e] = -1;
+    return;
+  }
+  uint64_t h = 0x243f6a8885a308d3ull;
+  for (int c = 0; c < 6; ++c) h = key_mix(h ^ (uint64_t)__double_as_longlong(k[c]));
+  uint64_t p = h & mask;
+  for (uint64_t probe = 0; probe <= mask; ++probe) {
+    const int cur = atomicCAS(&slots[p], -1, (int)e);
+    bool eq = cur == -1;
+    if (!eq) {
+      double q[6];
+      side_key(P1, P2, P3, n, cur, q);
+      eq = true;
+      for (int c = 0; c < 6; ++c) eq = eq && q[c] == k[c];
+    }
+    if (eq) {
+      atomicMin(&own[p], (int)e);
+      slot_of[e] = (int)p;
+      return;
+    }
+    p = (p + 1) & mask;
+  }
+  slot_of[e] = -1;  // table full (cannot happen: it has at least twice as many slots as sides)
+}
+
+// One workgroup of 1024 threads numbers the edges in side-id order: owners get consecutive edge indices
+// (edge_side[t] = owner side of edge t), then every side records its edge and direction:
+// side_edge[e] = 2 t + (1 if side e runs against its owner).  count[0] = number of edges, count[1] = 1 if a
+// triangle lies in z = 0.
+__global__ void __launch_bounds__(1024) tdk_edge_number(const double* __restrict__ P1, const double* __restrict__ P2,
+                                                        const double* __restrict__ P3, int64_t n,
+                                                        const int* __restrict__ own, const int* __restrict__ slot_of,
+                                                        int* __restrict__ owner_idx, int* __restrict__ side_edge,
+                                                        int* __restrict__ edge_side, int* __restrict__ count) {
+  __shared__ int wsum[16];
+  __shared__ int base;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t ns = 3 * n;
+  if (tid == 0) {
+    base = 0;
+    count[1] = 0;
+  }
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < ns; c0 += 1024) {
+    const int64_t e = c0 + tid;
+    int owner = -1;
+    if (e < ns) owner = slot_of[e] < 0 ? (int)e : own[slot_of[e]];
+    const bool is_own = e < ns && owner == (int)e;
+    const unsigned long long m = __ballot(is_own);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wid] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wid; ++w) off += wsum[w];
+    if (is_own) {
+      owner_idx[e] = off + before;
+      edge_side[off + before] = (int)e;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int tot = 0;
+      for (int w = 0; w < 16; ++w) tot += wsum[w];
+      base += tot;
+    }
+    __syncthreads();
+  }
+  // owners' indices are complete (the barrier above also orders the global writes within the workgroup)
+  for (int64_t e = tid; e < ns; e += 1024) {
+    const int o = slot_of[e] < 0 ? (int)e : own[slot_of[e]];
+    double a[3], b[3], oa[3], ob[3];
+    side_ends(P1, P2, P3, n, e, a, b);
+    side_ends(P1, P2, P3, n, o, oa, ob);
+    const bool rev = !(a[0] == oa[0] && a[1] == oa[1] && a[2] == oa[2]);
+    side_edge[e] = 2 * owner_idx[o] + (rev ? 1 : 0);
+  }
+  // triangles lying in z = 0 (their image pass needs the fused kernel)
+  for (int64_t j = tid; j < n; j += 1024)
+    if (P1[2 * n + j] == 0.0 && P2[2 * n + j] == 0.0 && P3[2 * n + j] == 0.0) atomicOr(&count[1], 1);
+  if (tid == 0) count[0] = base;
+}
+
+// Edge records (the side record of the owner side; its Burgers part is not used) for t < count.
+__global__ void __launch_bounds__(128) tdk_edge_setup(const double* __restrict__ P1, const double* __restrict__ P2,
+                                                      const double* __restrict__ P3, int64_t n,
+                                                      const double* __restrict__ rec, const int* __restrict__ edge_side,
+                                                      const int* __restrict__ count, double* __restrict__ erec) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= count[0]) return;
+  const int64_t e = edge_side[t];
+  double a[3], b[3];
+  side_ends(P1, P2, P3, n, e, a, b);
+  build_side(a, b, rec + (e % n) * REC_HS, erec + t * R_SIDE_SIZE);
+}
+
+// Responses of edges [t0, t0 + tpe) at receiver i: Tb[(t * 9 + 3 kE + q) * ldT + i] = traction q (Tn, Tss, Tds) on
+// the receiver plane of the edge's free-surface strain for the unit EFCS Burgers vector e_kE.
+__global__ void __launch_bounds__(256) tdk_edge_response(const double* __restrict__ recv, int64_t nrecv,
+                                                         const double* __restrict__ erec, int64_t nedge, int tpe,
+                                                         double nu, double lam, double G, double* __restrict__ Tb,
+                                                         int64_t ldT) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nrecv) return;
+  const double* rv = recv + i * RECV;
+  const double Xi = rv[0], Yi = rv[1], Zi = rv[2];
+  const int64_t t0 = (int64_t)blockIdx.y * tpe;
+  const int64_t t1 = t0 + tpe < nedge ? t0 + tpe : nedge;
+  const double k4 = (1.0 / (4.0 * kPi)) / (1.0 - nu);
+  for (int64_t t = t0; t < t1; ++t) {
+    const double* s = erec + t * R_SIDE_SIZE;
+    double o[9];
+    if (s[S_SKIP] != 0.0) {  // edge-uniform
+#pragma unroll
+      for (int q = 0; q < 9; ++q) o[q] = 0.0;
+    } else {
+      double w[3][6], sg;
+      fsc_side_w(s, Xi, Yi, Zi, nu, w, sg);
+      const double p = s[S_EY1 + 0], qq = s[S_EY1 + 1];
+      const double c0[3] = {p, qq, 0.0}, c1[3] = {qq, -p, 0.0}, c2[3] = {-0.0, -0.0, -1.0};
+      const double n[3] = {rv[3], rv[4], rv[5]}, sv[3] = {rv[6], rv[7], rv[8]}, d[3] = {rv[9], rv[10], rv[11]};
+#pragma unroll
+      for (int kE = 0; kE < 3; ++kE) {
+        // e_kE in the edge's ADCS is (ey1[kE], ey2[kE], ey3[kE]); configuration I negates the first two
+        const double b1 = sg * c0[kE] * k4, b2 = sg * c1[kE] * k4, b3 = c2[kE] * k4;
+        double e[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) e[c] = (c < 3 ? 1.0 : 0.5) * (b1 * w[0][c] + b2 * w[1][c] + b3 * w[2][c]);
+        tensor_rotate(e, c0, c1, c2);
+        traction3(e, lam, G, n, sv, d, o[3 * kE], o[3 * kE + 1], o[3 * kE + 2]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Tb[(t * 9 + q) * ldT + i] = o[q];
+  }
+}
+
+// Traction entries of the half-space pairs: the real and image TDs evaluated here, the free-surface part as
+// +- sum over the triangle's edges of sum_kE V_m[kE] Tb[edge][kE][q], V_m the unit slip vector of mode m
+// (Vnorm, Vstrike, Vdip).  Not for triangles lying in z = 0.
+__global__ void __launch_bounds__(256, 2) tdk_traction_combine(const double* __restrict__ recv, int64_t nrecv,
+                                                            const double* __restrict__ rec, int64_t ntri, int tpb,
+                                                            double nu, double lam, double G,
+                                                            const int* __restrict__ side_edge,
+                                                            const double* __restrict__ Tb, int64_t ldT,
+                                                            OutPtrs<double> out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nrecv) return;
+  const double* rv = recv + i * RECV;
+  const double Xi = rv[0], Yi = rv[1], Zi = rv[2];
+  const double n[3] = {rv[3], rv[4], rv[5]}, s[3] = {rv[6], rv[7], rv[8]}, d[3] = {rv[9], rv[10], rv[11]};
+  const int64_t j0 = (int64_t)blockIdx.y * tpb;
+  const int64_t j1 = j0 + tpb < ntri ? j0 + tpb : ntri;
+  for (int64_t j = j0; j < j1; ++j) {
+    const double* r = rec + j * REC_HS;
+    double t[3][3];
+    td_part_traction(r, Xi, Yi, Zi, nu, lam, G, n, s, d, t, true);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int se = side_edge[k * ntri + j];
+      const double* T = Tb + (int64_t)(se >> 1) * 9 * ldT + i;
+      const double sgn = (se & 1) ? -1.0 : 1.0;
+      double v[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) v[q] = T[q * ldT];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const double* V = r + (m == 0 ? R_VN : (m == 1 ? R_VS : R_VD));
+#pragma unroll
+        for (int q = 0; q < 3; ++q) t[m][q] += sgn * (V[0] * v[q] + V[1] * v[3 + q] + V[2] * v[6 + q]);
+      }
+    }
+    td_part_traction(r + OFF_IMG, Xi, Yi, Zi, nu, lam, G, n, s, d, t, false);
+    const int64_t o = i + j * out.ld;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) out.p[3 * m + q][o] = out.sign * t[m][q];
+  }
+}
+
 // ---- fixed elements: traction rows at free receivers, displacement rows at fixed ones ------------------
 // (PutDispInfsIntoInfMat / ConcatInfMats, CalculateInfluenceMatrices3D.jl:145-198).  Block k = 3 m + q of
 // the output (mode m, traction q) takes rows of Tf block k at free receivers and rows of Ud block k (the
@@ -447,6 +691,49 @@ __global__ void __launch_bounds__(256) tdk_fixed_scatter(const double* __restric
 }
 
 // ---- launch helpers --------------------------------------------------------------------------------------
+uint64_t edge_table_slots(int64_t ntri) {
+  uint64_t s = 64;
+  while (s < (uint64_t)(6 * ntri)) s <<= 1;
+  return s;
+}
+
+void launch_edge_topology(const double* P1, const double* P2, const double* P3, int64_t ntri, const EdgeTopo& e,
+                          hipStream_t st) {
+  if (ntri <= 0) return;
+  const int64_t ns = 3 * ntri;
+  const uint64_t S = edge_table_slots(ntri);
+  hipLaunchKernelGGL(tdk_fill_i32, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st, e.slots, (int64_t)S, -1);
+  hipLaunchKernelGGL(tdk_fill_i32, dim3((unsigned)((S + 255) / 256)), dim3(256), 0, st, e.own, (int64_t)S,
+                     0x7fffffff);
+  hipLaunchKernelGGL(tdk_edge_insert, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, P1, P2, P3, ntri, e.slots,
+                     e.own, S - 1, e.slot_of);
+  hipLaunchKernelGGL(tdk_edge_number, dim3(1), dim3(1024), 0, st, P1, P2, P3, ntri, e.own, e.slot_of, e.owner_idx,
+                     e.side_edge, e.edge_side, e.count);
+}
+
+void launch_edge_setup(const double* P1, const double* P2, const double* P3, int64_t ntri, const double* rec,
+                       const EdgeTopo& e, double* erec, hipStream_t st) {
+  if (ntri <= 0) return;
+  hipLaunchKernelGGL(tdk_edge_setup, dim3((unsigned)((3 * ntri + 127) / 128)), dim3(128), 0, st, P1, P2, P3, ntri, rec,
+                     e.edge_side, e.count, erec);
+}
+
+void launch_edge_response(const double* recv, int64_t nrecv, const double* erec, int64_t nedge, int tpe, double nu,
+                          double lam, double G, double* Tb, int64_t ldT, hipStream_t st) {
+  if (nrecv <= 0 || nedge <= 0) return;
+  dim3 grid((unsigned)((nrecv + 255) / 256), (unsigned)((nedge + tpe - 1) / tpe));
+  hipLaunchKernelGGL(tdk_edge_response, grid, dim3(256), 0, st, recv, nrecv, erec, nedge, tpe, nu, lam, G, Tb, ldT);
+}
+
+void launch_traction_combine(const double* recv, int64_t nrecv, const double* rec, int64_t ntri, int tpb, double nu,
+                             double lam, double G, const int* side_edge, const double* Tb, int64_t ldT,
+                             const OutPtrs<double>& out, hipStream_t st) {
+  if (nrecv <= 0 || ntri <= 0) return;
+  dim3 grid((unsigned)((nrecv + 255) / 256), (unsigned)((ntri + tpb - 1) / tpb));
+  hipLaunchKernelGGL(tdk_traction_combine, grid, dim3(256), 0, st, recv, nrecv, rec, ntri, tpb, nu, lam, G, side_edge,
+                     Tb, ldT, out);
+}
+
 void launch_fixed_scatter(const double* Tf, const double* Ud, const int32_t* rowmap, int64_t n, int64_t nf,
                           int64_t nfix, int64_t nsrc, const OutPtrs<double>& out, hipStream_t st) {
   if (n <= 0 || nsrc <= 0) return;

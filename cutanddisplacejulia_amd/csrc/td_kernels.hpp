@@ -9,6 +9,7 @@ namespace cdj {
 static constexpr int RECV = 12;  // receiver record: point, normal, strike dir, dip dir
 static constexpr int REC_FS = 32;   // triangle record, full space (elements)
 static constexpr int REC_HS = 160;  // triangle record, half space: real + image TD + 3 free-surface sides
+static constexpr int REC_EDGE = 32;  // mesh-edge record of the half-space assembly (one free-surface side)
 
 // Output descriptor passed by value to the kernels.
 //   influence: p[0..17] strain (StrainInf order), p[18..26] displacement (DispInf order); ld = nobs
@@ -47,6 +48,32 @@ void launch_traction(const double* recv, int64_t nrecv, const T* rec, int64_t nt
 // FP32 arithmetic, FP64 storage (far field of the mixed assembly)
 void launch_traction_f32_to_f64(const double* recv, int64_t nrecv, const float* rec, int64_t ntri, int hs, int tpb,
                                 float nu, float lam, float G, const OutPtrs<double>& out, hipStream_t st);
+// ---- half-space assembly with free-surface responses shared per mesh edge ----
+// Device arrays of the edge topology (sides e = k ntri + j; see tdk_edge_insert / tdk_edge_number).
+struct EdgeTopo {
+  int* slots;      // edge_table_slots(ntri)
+  int* own;        // edge_table_slots(ntri)
+  int* slot_of;    // 3 ntri
+  int* owner_idx;  // 3 ntri
+  int* side_edge;  // 3 ntri: 2 * edge + (1 if the side runs against the edge's owner)
+  int* edge_side;  // 3 ntri: owner side of each edge
+  int* count;      // 2: number of edges, 1 if a triangle lies in z = 0
+};
+uint64_t edge_table_slots(int64_t ntri);
+void launch_edge_topology(const double* P1, const double* P2, const double* P3, int64_t ntri, const EdgeTopo& e,
+                          hipStream_t st);
+// edge records (R_SIDE_SIZE doubles each, at most 3 ntri) from the triangle records `rec` (half space)
+void launch_edge_setup(const double* P1, const double* P2, const double* P3, int64_t ntri, const double* rec,
+                       const EdgeTopo& e, double* erec, hipStream_t st);
+// Tb[(t * 9 + 3 kE + q) * ldT + i]: traction q on receiver i's plane of edge t's free-surface strain for the
+// EFCS unit Burgers vector kE
+void launch_edge_response(const double* recv, int64_t nrecv, const double* erec, int64_t nedge, int tpe, double nu,
+                          double lam, double G, double* Tb, int64_t ldT, hipStream_t st);
+// half-space traction entries from the real and image TDs plus the edges' responses (no triangle in z = 0)
+void launch_traction_combine(const double* recv, int64_t nrecv, const double* rec, int64_t ntri, int tpb, double nu,
+                             double lam, double G, const int* side_edge, const double* Tb, int64_t ldT,
+                             const OutPtrs<double>& out, hipStream_t st);
+
 // Fixed-element rows: out block k (rows 0..n-1, columns 0..nsrc-1, times out.sign) from the compact traction
 // rows Tf (9 blocks nf x nsrc) and displacement rows Ud (9 blocks nfix x nsrc); rowmap[r] >= 0 selects free row
 // rowmap[r], otherwise fixed row -1 - rowmap[r].

@@ -80,13 +80,20 @@ int cdj_create(int device, cdj_ctx** ctx);
 int cdj_destroy(cdj_ctx* ctx);
 const char* cdj_last_error(const cdj_ctx* ctx);
 
-/* Device time (ms, HIP events on the launch stream) of the last assembly kernel run on this context; -1 if
- * none.  Used by benchmarks to measure the kernel alone. */
+/* Device time (ms, HIP events on the launch stream) of the last assembly run on this context (its kernels, from the
+ * first to the last); -1 if none.  Used by benchmarks to measure the kernels alone. */
 double cdj_last_kernel_ms(cdj_ctx* ctx);
 
 /* Entries of the last traction assembly on this context that the mixed mode (CDJ_ASM_MIXED) recomputed in FP64
  * (0 after an FP64 assembly).  No counterpart in the reference (its assembly is FP64 throughout). */
 int64_t cdj_last_near_entries(const cdj_ctx* ctx);
+
+/* Mesh edges of the last FP64 half-space traction assembly on this context, whose free-surface responses were
+ * evaluated once per edge and shared by the edge's triangles (0 when every pair evaluated its own sides: full
+ * space, a triangle lying in z = 0, or cdj_assemble_dev_noalloc), and the device time (ms, HIP events) of that
+ * edge-response launch (-1 when it ran in several receiver chunks or untimed).  For benchmarks; no counterpart in
+ * the reference. */
+int cdj_last_edge_stats(cdj_ctx* ctx, int64_t* edges, double* ms);
 
 /* TD with influence matrices: replaces TD(X,Y,Z,P1,P2,P3,Dss,Dds,Dn,nu,G,DispFlag,StrainFlag,HSflag[,StrainInf,
  * DispInf]) (TD.jl:77-133, 135-216).  strain: 18 nobs x ntri matrices in StrainInf field order (exxDn, eyyDn,
@@ -162,9 +169,10 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
 
 /* cdj_assemble_dev without any device allocation, for callers that cannot allocate (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same arguments, layout,
- * kernels and results as cdj_assemble_dev; its scratch (receiver and triangle records, and for CDJ_ASM_MIXED the
+ * results as cdj_assemble_dev; its scratch (receiver and triangle records, and for CDJ_ASM_MIXED the
  * FP32 records, near-field table and counter) is a fixed array of CDJ_NOALLOC_SCRATCH_BYTES in device memory, one per
- * GPU, held until the process ends.  Needs 256 + 8 (RECV + RS) n bytes, RECV = 12, RS = 32 (full space) or 160
+ * GPU, held until the process ends; in half space it evaluates every pair's three free-surface sides (the allocating
+ * entry point shares them per mesh edge), equal to it within rounding.  Needs 256 + 8 (RECV + RS) n bytes, RECV = 12, RS = 32 (full space) or 160
  * (half space), plus 4 RS n (rounded up to 256) + 32 n for CDJ_ASM_MIXED: at most 95324 triangles in full space and
  * 24385 in half space (FP64 assembly); larger n is rejected with CDJ_ERR_ARG.  Every call on one GPU uses the same
  * array, whatever the context: calls that overlap on one GPU (other threads, other streams) race. */

@@ -212,6 +212,22 @@ def count_traction_flops(normals, mid, P1, P2, P3, nu, G, lam, hs_flag, recv_idx
     return dict(zip(["add", "mul", "div", "sqrt", "log", "atan", "trig", "pairs"], out.tolist()))
 
 
+def count_fsc_side_flops(normals, mid, P1, P2, P3, nu, recv_idx, side_idx):
+    """Operation counts of the free-surface strain of single triangle sides (all three slip modes) at collocation
+    points: side_idx = k n + j is side k (P1P2, P2P3, P3P1) of triangle j.  Returns dict as count_traction_flops."""
+    L = lib()
+    f = L.orc_count_fsc_side_flops
+    f.restype = None
+    f.argtypes = [_p, _p, _p, _p, _p, _i64, _d, _p, _p, _i64, _p]
+    normals, mid = _f64(normals), _f64(mid)
+    P1, P2, P3, n = _tri_arrays(P1, P2, P3)
+    ri = np.ascontiguousarray(recv_idx, dtype=np.int64)
+    si = np.ascontiguousarray(side_idx, dtype=np.int64)
+    out = np.zeros(8)
+    f(_ptr(normals), _ptr(mid), _ptr(P1), _ptr(P2), _ptr(P3), n, float(nu), _ptr(ri), _ptr(si), ri.size, _ptr(out))
+    return dict(zip(["add", "mul", "div", "sqrt", "log", "atan", "trig", "pairs"], out.tolist()))
+
+
 def traction_influence_rows(normals, mid, P1, P2, P3, nu, G, lam, hs_flag, r0, r1, nthreads=0):
     """Traction influence of receiver rows [r0, r1) against all sources (bounded CPU-baseline sample)."""
     L = lib()

@@ -138,6 +138,44 @@ void orc_count_traction_flops(const double* normals, const double* mid, const do
   out[7] = (double)npairs;
 }
 
+// Operation counts of the free-surface strain of single triangle sides (AngSetupStrainFSC TD.jl:1867-2081 for the
+// three slip modes): side k of triangle j (side_idx = k n + j; sides P1P2, P2P3, P3P1) at the collocation point of
+// receiver recv_idx, summed over npairs.  out[8] as orc_count_traction_flops, out[7] = npairs.
+void orc_count_fsc_side_flops(const double* normals, const double* mid, const double* P1, const double* P2,
+                              const double* P3, int64_t n, double nu, const int64_t* recv_idx, const int64_t* side_idx,
+                              int64_t npairs, double* out) {
+  using orc::Cnt;
+  orc::Counts c;
+  for (int64_t p = 0; p < npairs; ++p) {
+    const int64_t i = recv_idx[p], j = side_idx[p] % n, k = side_idx[p] / n;
+    Cnt a[3], b[3], cc[3];
+    for (int q = 0; q < 3; ++q) {
+      a[q] = P1[q * n + j];
+      b[q] = P2[q * n + j];
+      cc[q] = P3[q * n + j];
+    }
+    orc::TriFull<Cnt> f;
+    orc::g_cnt = nullptr;
+    orc::tri_full_setup(a, b, cc, Cnt(1.0), Cnt(1.0), Cnt(1.0), true, f);
+    orc::g_cnt = &c;
+    const Cnt X = mid[i] - normals[i] * 1e-12, Y = mid[n + i] - normals[n + i] * 1e-12,
+              Z = mid[2 * n + i] - normals[2 * n + i] * 1e-12;
+    Cnt e[3][6];
+    for (int m = 0; m < 3; ++m)
+      for (int q = 0; q < 6; ++q) e[m][q] = Cnt(0.0);
+    orc::strain_fsc_side(f.side[k], X, Y, Z, Cnt(1.0), Cnt(1.0), Cnt(1.0), Cnt(nu), e);
+    orc::g_cnt = nullptr;
+  }
+  out[0] = c.add;
+  out[1] = c.mul;
+  out[2] = c.div;
+  out[3] = c.sqrt;
+  out[4] = c.log;
+  out[5] = c.atan;
+  out[6] = c.trig;
+  out[7] = (double)npairs;
+}
+
 // Traction assembly of receiver rows [r0, r1) against all n sources (the reference's CPU path for a block of
 // rows; OpenMP over sources as TD.jl:146).  T9: 9 matrices (r1 - r0) x n, TractionInf order.
 int orc_traction_influence_rows(const double* normals, const double* mid, const double* P1, const double* P2,

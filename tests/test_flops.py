@@ -18,3 +18,19 @@ def test_flops_per_pair_half_space():
     cnt = O.count_traction_flops(nrm, mid, P1, P2, P3, 0.25, 1.0, 0.4, 1, ri, si)
     per_pair = (cnt["add"] + cnt["mul"] + cnt["div"]) / cnt["pairs"]
     assert abs(per_pair - bench.FLOPS_PER_PAIR_HS) / bench.FLOPS_PER_PAIR_HS < 0.01
+
+
+def test_flops_per_edge_unit_half_space():
+    """The dominant kernel's unit: one triangle side's free-surface strain (three slip modes) at one receiver."""
+    c = meshes.config2()
+    P1, P2, P3 = c["P1"], c["P2"], c["P3"]
+    n = P1.shape[0]
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    rng = np.random.default_rng(0)
+    ri = rng.integers(0, n, 6000)
+    si = rng.integers(0, 3 * n, 6000)
+    cnt = O.count_fsc_side_flops(nrm, mid, P1, P2, P3, 0.25, ri, si)
+    per_unit = (cnt["add"] + cnt["mul"] + cnt["div"]) / cnt["pairs"]
+    assert abs(per_unit - bench.FLOPS_PER_EDGE_HS) / bench.FLOPS_PER_EDGE_HS < 0.01
+    # the three sides are most of the pair's work (AngSetupStrainFSC dominates the half-space strain)
+    assert 0.8 < 3 * per_unit / bench.FLOPS_PER_PAIR_HS < 0.9

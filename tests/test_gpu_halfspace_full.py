@@ -1,7 +1,8 @@
 """The benchmark's own matrix at its full size: the half-space collocation matrix of the 45-degree dipping fault
 meshed into 5000 triangles (A 15000 x 15000 FP64), assembled on the device exactly as bench.py times it
-(cdj_assemble_dev, kernel tdk_traction<double, double, true>), against the oracle's rows
-(CalculateInfluenceMatrices3D.jl:2-48 restated) on 256 receivers drawn at random.
+(cdj_assemble_dev: each mesh edge's free-surface part evaluated once and shared by its two triangles), against the
+oracle's rows (CalculateInfluenceMatrices3D.jl:2-48 restated, every triangle's three sides evaluated) on 256
+receivers drawn at random.
 
 Tolerance: max |gpu - oracle| <= 1e-12 x the largest oracle entry of the same slip mode (its three traction
 blocks over the sampled rows).  One scale per mode, not per block: on a planar fault the blocks that couple opening
@@ -18,7 +19,7 @@ from cutanddisplacejulia_amd import api, meshes
 pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
-from cutanddisplacejulia_amd import learned  # noqa: E402
+from cutanddisplacejulia_amd import _lib, learned  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 
@@ -33,7 +34,14 @@ def fault():
     A = torch.empty((3 * n, 3 * n), dtype=torch.float64, device="cuda:0")  # column-major: A[col, row]
     learned.assemble_dev(d["N"], d["M"], d["P1"], d["P2"], d["P3"], n, nu, G, lam, 1, A, 3 * n)
     torch.cuda.synchronize()
-    return dict(P1=P1, P2=P2, P3=P3, n=n, nrm=nrm, mid=mid, nu=nu, G=G, lam=lam, A=A)
+    return dict(P1=P1, P2=P2, P3=P3, n=n, nrm=nrm, mid=mid, nu=nu, G=G, lam=lam, A=A, edges=_edges())
+
+
+def _edges():
+    import ctypes
+    e = ctypes.c_int64()
+    _lib.check(_lib.load().cdj_last_edge_stats(_lib.ctx(), ctypes.byref(e), None))
+    return e.value
 
 
 def _rows(f, rows):
@@ -50,6 +58,8 @@ def _rows(f, rows):
 
 def test_benchmark_matrix_matches_oracle_rows(fault):
     n = fault["n"]
+    # the shared-edge path ran: 50 x 50 quads have 50 * 51 * 2 + 2500 = 7600 edges for 15000 triangle sides
+    assert fault["edges"] == 7600
     rng = np.random.default_rng(20261204)
     starts = np.sort(rng.choice(np.arange(0, n - 32, 32), size=8, replace=False))
     rows = np.concatenate([np.arange(s, s + 32) for s in starts])
@@ -79,7 +89,8 @@ def test_benchmark_matrix_matches_oracle_rows(fault):
 
 def test_flat_free_surface_mesh_traction_matches_oracle():
     """Triangles lying exactly in z = 0 (the reference's flat-triangle image behaviour, TD.jl:1518-1525,
-    1610-1617) and the same mesh moved below the surface: the traction matrices match the oracle."""
+    1610-1617, whose image pass needs the accumulated strain, so every pair evaluates its own sides) and the same mesh
+    moved below the surface (edges shared): the traction matrices match the oracle."""
     P1, P2, P3 = meshes.rect_fault(4, 4, 60.0, 0.0, 2.0, 2.0, 0.0)  # horizontal rectangle at z = 0
     for P in (P1, P2, P3):
         P[:, 2] = 0.0                                                  # exactly (cos 90 leaves ~1e-16)
@@ -90,6 +101,7 @@ def test_flat_free_surface_mesh_traction_matches_oracle():
         assert np.all(Q1[:, 2] == dz)
         nrm, mid = api.CreateFaceNormalAndMidPoint(Q1, Q2, Q3)
         Ti = api.CalculateInfluenceMatrices3D(nrm, mid, Q1, Q2, Q3, nu, G, lam, np.zeros(n), 1, n)
+        assert (_edges() > 0) == (dz != 0.0)
         To = O.traction_influence(nrm, mid, Q1, Q2, Q3, nu, G, lam, 1)
         for m in range(3):
             names = O.TRACTION_FIELDS[3 * m:3 * m + 3]

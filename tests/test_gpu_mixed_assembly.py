@@ -2,8 +2,10 @@
 
 * Classifier: the near-field pass replaces exactly 9 entries per (receiver, source) pair whose collocation point
   lies within near_factor x the source's longest edge of its centroid (counted here on the host).
-* Near entries equal the FP64 assembly's (the same FP64 code runs for them), and the FP64 assembly equals the
-  oracle to 1e-12 (tests/test_gpu_td.py).
+* Near entries equal the FP64 assembly's: bit for bit in full space (the same FP64 code runs for them); in half
+  space the FP64 assembly shares each mesh edge's free-surface part between its two triangles while the near pass
+  evaluates every pair's own sides, so they agree to 1e-12 of the column's largest entry.  The FP64 assembly
+  equals the oracle to 1e-12 (tests/test_gpu_td.py, tests/test_gpu_halfspace_full.py).
 * Far entries: stated bound |A_mixed - A_fp64| <= TOL_FAR x max|column of A_fp64|. Measured worst cases at
   near_factor 2: 1.6e-6 (half space, dipping fault and penny), 1.4e-5 (full space, coplanar square crack, where
   the angular-dislocation terms cancel in the plane).
@@ -62,8 +64,12 @@ def test_mixed_assembly_classifier_and_error_bound(case, f):
     assert ne == 9 * int(near.sum())
     assert np.all(np.diag(near))  # every self entry is near
     N = np.tile(near, (3, 3))
-    assert np.array_equal(Am[N], A64[N])
     colmax = np.max(np.abs(A64), axis=0)
+    if hs:
+        near_err = np.max(np.abs(Am - A64)[N] / np.broadcast_to(colmax, A64.shape)[N])
+        assert near_err <= 1e-12, near_err
+    else:
+        assert np.array_equal(Am[N], A64[N])
     far_err = np.max(np.abs(Am - A64)[~N] / np.broadcast_to(colmax, A64.shape)[~N])
     print("%s f=%g: far max |dA|/colmax %.2e, near entries %d" % (case, f, far_err, ne))
     if f >= 2.0:
