@@ -82,8 +82,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) batch_far_kernel(
   }
 }
 
-// FP64 recompute of the near pairs (collocation point within the source's near radius, near[j] = {centroid,
-// radius^2}) over the FP32 values; a wave with no near lane for a source skips it.
+// FP64 recompute of the near pairs (near_pair over the sources' near records) over the FP32 values; a wave with no
+// near lane for a source skips it.
 template <bool HS>
 __global__ void __launch_bounds__(256) batch_near_kernel(const double* __restrict__ recv,
                                                          const double* __restrict__ rec,
@@ -115,9 +115,7 @@ __global__ void __launch_bounds__(256) batch_near_kernel(const double* __restric
     const int64_t lda = 3 * M.n;
     double* Am = A + M.a_off;
     for (int64_t j = j0; j < j1; ++j) {
-      const double* nj = near + (M.tri0 + j) * 4;
-      const double dx = Xi - nj[0], dy = Yi - nj[1], dz = Zi - nj[2];
-      const bool nr = valid && (dx * dx + dy * dy + dz * dz) < nj[3];
+      const bool nr = valid && near_pair(near + (M.tri0 + j) * NEAR_REC, Xi, Yi, Zi);
       if (!__any(nr)) continue;
       double e[3][6];
       strain_pair<double, HS>(rec + (M.tri0 + j) * RS, Xi, Yi, Zi, nu, e);

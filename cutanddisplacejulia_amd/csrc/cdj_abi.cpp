@@ -223,7 +223,7 @@ int check_hs_dev(cdj_ctx* ctx, const double* Z, int64_t nobs, const double* P1, 
 }
 
 // FP32-far / FP64-near pass pair into `out` (the assembly's mixed mode).  rec: FP64 records (built), rec32
-// (RS * n floats), near (4 n doubles) and cnt (one counter) are device scratch of the caller.
+// (RS * n floats), near (NEAR_REC n doubles) and cnt (one counter) are device scratch of the caller.
 hipError_t traction_mixed(const double* recv, const double* rec, float* rec32, double* near,
                           unsigned long long* cnt, const double* P1, const double* P2, const double* P3, int64_t n,
                           int hs, int tpb, double nu, double lambda, double G, double near_factor,
@@ -597,7 +597,7 @@ namespace {
 size_t assemble_scratch_bytes(int64_t n, int hs, bool mixed) {
   const size_t RS = hs ? REC_HS : REC_FS;
   size_t b = 256 + sizeof(double) * (RECV + RS) * (size_t)n;
-  if (mixed) b += ((sizeof(float) * RS * (size_t)n + 255) / 256) * 256 + sizeof(double) * 4 * (size_t)n;
+  if (mixed) b += ((sizeof(float) * RS * (size_t)n + 255) / 256) * 256 + sizeof(double) * NEAR_REC * (size_t)n;
   return b;
 }
 
@@ -774,7 +774,7 @@ int cdj_traction_influence(cdj_ctx* ctx, const double* normals, const double* mi
   if (mixed) {
     DBuf r32, nr, cnt;
     HIPCHK(r32.alloc(sizeof(float) * RS * n));
-    HIPCHK(nr.alloc(sizeof(double) * 4 * n));
+    HIPCHK(nr.alloc(sizeof(double) * NEAR_REC * n));
     HIPCHK(cnt.alloc(sizeof(unsigned long long)));
     HIPCHK(traction_mixed(recv.as<double>(), rec.as<double>(), r32.as<float>(), nr.as<double>(),
                           cnt.as<unsigned long long>(), d1.as<double>(), d2.as<double>(), d3.as<double>(), n, hs, tpb,
@@ -1651,7 +1651,7 @@ int batch_plan_build(cdj_ctx* ctx, const cdj_mesh* ms, int64_t nmesh, int hs, bo
   HIPCHK(p.rec.alloc(sizeof(double) * RS * p.ntot));
   if (mixed) {
     HIPCHK(p.rec32.alloc(sizeof(float) * RS * p.ntot));
-    HIPCHK(p.near.alloc(sizeof(double) * 4 * p.ntot));
+    HIPCHK(p.near.alloc(sizeof(double) * NEAR_REC * p.ntot));
   }
   HIPCHK(p.meshes.alloc(sizeof(BatchMeshDev) * nmesh));
   HIPCHK(p.items.alloc(sizeof(int64_t) * (nmesh + 1)));

@@ -359,8 +359,8 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) tdk_traction(cons
   }
 }
 
-// Near-field pass of the mixed assembly: pairs whose collocation point lies within the source triangle's
-// near radius (near[j] = {centroid, radius^2}) are recomputed in FP64 and overwrite the FP32 values.  A wave
+// Near-field pass of the mixed assembly: pairs that near_pair selects (near[j * NEAR_REC]: the source's centroid,
+// near radius, plane and size) are recomputed in FP64 and overwrite the FP32 values.  A wave
 // with no near lane for a source skips it, so the FP64 work is proportional to the near pairs' waves.
 template <bool HS>
 __global__ void __launch_bounds__(256) tdk_nearfield(const double* __restrict__ recv, int64_t nrecv,
@@ -379,9 +379,7 @@ __global__ void __launch_bounds__(256) tdk_nearfield(const double* __restrict__ 
   constexpr int RS = HS ? REC_HS : REC_FS;
   unsigned long long cnt = 0;
   for (int64_t j = j0; j < j1; ++j) {
-    const double* nj = near + j * 4;
-    const double dx = Xi - nj[0], dy = Yi - nj[1], dz = Zi - nj[2];
-    const bool nr = valid && (dx * dx + dy * dy + dz * dz) < nj[3];
+    const bool nr = valid && near_pair(near + j * NEAR_REC, Xi, Yi, Zi);
     if (!__any(nr)) continue;
     double e[3][6];
     strain_pair<double, HS>(rec + j * RS, Xi, Yi, Zi, nu, e);
@@ -401,7 +399,7 @@ __global__ void __launch_bounds__(256) tdk_nearfield(const double* __restrict__ 
   if (cnt) atomicAdd(count, cnt * 9ull);
 }
 
-// near[j] = {centroid x, y, z, (near_factor * longest edge)^2}
+// near[j * NEAR_REC] = {centroid x, y, z, (near_factor * L)^2, unit normal x, y, z, L}, L the longest edge
 __global__ void tdk_near_info(const double* __restrict__ P1, const double* __restrict__ P2,
                               const double* __restrict__ P3, int64_t ntri, double factor, double* __restrict__ near) {
   const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -420,8 +418,14 @@ __global__ void tdk_near_info(const double* __restrict__ P1, const double* __res
     for (int k = 0; k < 3; ++k) s2 += (pb[e][k] - pa[e][k]) * (pb[e][k] - pa[e][k]);
     l = s2 > l ? s2 : l;
   }
-  for (int k = 0; k < 3; ++k) near[j * 4 + k] = (a[k] + b[k] + c[k]) / 3.0;
-  near[j * 4 + 3] = factor * factor * l;
+  double* o = near + j * NEAR_REC;
+  for (int k = 0; k < 3; ++k) o[k] = (a[k] + b[k] + c[k]) / 3.0;
+  o[3] = factor * factor * l;
+  const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double nn[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+  const double inv = 1.0 / sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+  for (int k = 0; k < 3; ++k) o[4 + k] = nn[k] * inv;
+  o[7] = sqrt(l);
 }
 
 // ---- half-space assembly with the free-surface correction shared per mesh edge -------------------------

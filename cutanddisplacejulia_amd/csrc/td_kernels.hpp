@@ -10,6 +10,7 @@ static constexpr int RECV = 12;  // receiver record: point, normal, strike dir, 
 static constexpr int REC_FS = 32;   // triangle record, full space (elements)
 static constexpr int REC_HS = 160;  // triangle record, half space: real + image TD + 3 free-surface sides
 static constexpr int REC_EDGE = 32;  // mesh-edge record of the half-space assembly (one free-surface side)
+static constexpr int NEAR_REC = 8;   // near-field record of a source: centroid, (f L)^2, unit normal, L
 
 // Output descriptor passed by value to the kernels.
 //   influence: p[0..17] strain (StrainInf order), p[18..26] displacement (DispInf order); ld = nobs
@@ -79,7 +80,7 @@ void launch_traction_combine(const double* recv, int64_t nrecv, const double* re
 // rowmap[r], otherwise fixed row -1 - rowmap[r].
 void launch_fixed_scatter(const double* Tf, const double* Ud, const int32_t* rowmap, int64_t n, int64_t nf,
                           int64_t nfix, int64_t nsrc, const OutPtrs<double>& out, hipStream_t st);
-// near[j*4] = {centroid, (factor * longest edge)^2}
+// near[j * NEAR_REC] = {centroid, (factor * L)^2, unit normal, L}, L the longest edge (see near_pair)
 void launch_near_info(const double* P1, const double* P2, const double* P3, int64_t ntri, double factor, double* near,
                       hipStream_t st);
 // FP64 recompute of near pairs over the FP32 values; *count accumulates the number of entries replaced

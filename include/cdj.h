@@ -43,8 +43,10 @@ extern "C" {
 
 /* arithmetic of the assembly (cdj_opts.assembly) */
 #define CDJ_ASM_FP64 0        /* every entry in FP64 (the reference's arithmetic) */
-#define CDJ_ASM_MIXED 1       /* FP32 far field, FP64 near field (collocation point within near_factor x the
-                                 source's longest edge of its centroid, which includes every self entry) */
+#define CDJ_ASM_MIXED 1       /* FP32 far field, FP64 near field: a (receiver, source) pair is recomputed in FP64
+                                 when its collocation point lies within near_factor x the source's longest edge L
+                                 of the source's centroid (which includes every self entry), or within 0.1 L of the
+                                 source's plane and 2 near_factor L of its centroid */
 
 /* preconditioners of cdj_precond_solve_dev */
 #define CDJ_PC_NONE 0
@@ -173,7 +175,7 @@ int cdj_assemble_dev(cdj_ctx* ctx, const double* normals, const double* midpoint
  * FP32 records, near-field table and counter) is a fixed array of CDJ_NOALLOC_SCRATCH_BYTES in device memory, one per
  * GPU, held until the process ends; in half space it evaluates every pair's three free-surface sides (the allocating
  * entry point shares them per mesh edge), equal to it within rounding.  Needs 256 + 8 (RECV + RS) n bytes, RECV = 12, RS = 32 (full space) or 160
- * (half space), plus 4 RS n (rounded up to 256) + 32 n for CDJ_ASM_MIXED: at most 95324 triangles in full space and
+ * (half space), plus 4 RS n (rounded up to 256) + 64 n for CDJ_ASM_MIXED: at most 95324 triangles in full space and
  * 24385 in half space (FP64 assembly); larger n is rejected with CDJ_ERR_ARG.  Every call on one GPU uses the same
  * array, whatever the context: calls that overlap on one GPU (other threads, other streams) race. */
 #define CDJ_NOALLOC_SCRATCH_BYTES (32ll << 20)
