@@ -7,10 +7,16 @@
 * AngDisStrainFSC == sym(grad AngDisDispFSC) / (4 pi (1 - nu)) by central differences.
 * Displacement jump across the TD interior equals the Burgers vector (full space).
 * Far field of the TD decays like a point dislocation (strain ~ r^-3).
+* Half space -> full space as depth -> infinity: the free-surface correction plus the image decay like D^-2
+  (displacement) and D^-3 (strain).
+* The half-space solution leaves the free surface traction-free (sigma_xz = sigma_yz = sigma_zz = 0 at z = 0).
+* Face normals and incentre midpoints: the host's CreateFaceNormalAndMidPoint equals the oracle's, and both meet the
+  geometry (unit normals orthogonal to the sides, the midpoint equidistant from the three sides).
 """
 import numpy as np
 import pytest
 
+from cutanddisplacejulia_amd import api
 from oracle import analytic as A
 from oracle import oracle as O
 
@@ -176,6 +182,73 @@ def test_far_field_decay():
         vals.append(np.sqrt(sum(s[k][0, 0] ** 2 for k in s)))
     assert abs(np.log2(vals[0] / vals[1]) - 3) < 0.05
     assert abs(np.log2(vals[1] / vals[2]) - 3) < 0.03
+
+
+def test_halfspace_tends_to_fullspace_with_depth():
+    """A triangle and its observation points moved down by D: |u_HS - u_FS| / |u_FS| ~ D^-2 and the strain's
+    ~ D^-3 (the image dislocation and the free-surface correction are 2D away)."""
+    P = [np.array([[-0.6, -0.4, 0.1]]), np.array([[0.8, -0.2, 0.5]]), np.array([[0.0, 0.7, -0.4]])]
+    pts = np.random.default_rng(7).uniform(-1.5, 1.5, (20, 3))
+    rel = []
+    for D in (40.0, 80.0, 160.0, 320.0):
+        Q = [p + np.array([0.0, 0.0, -D]) for p in P]
+        X, Y, Z = pts[:, 0], pts[:, 1], pts[:, 2] - D
+        sh, dh = O.td(X, Y, Z, *Q, [1.0], [1.0], [1.0], 0.25, 1.0, 1, 1, 1)
+        sf, df = O.td(X, Y, Z, *Q, [1.0], [1.0], [1.0], 0.25, 1.0, 1, 1, 0)
+        du = max(np.max(np.abs(dh[k] - df[k])) for k in dh) / max(np.max(np.abs(df[k])) for k in df)
+        de = max(np.max(np.abs(sh[k] - sf[k])) for k in sh) / max(np.max(np.abs(sf[k])) for k in sf)
+        rel.append((du, de))
+    assert rel[-1][0] < 1e-5 and rel[-1][1] < 1e-7
+    for (u0, e0), (u1, e1) in zip(rel, rel[1:]):
+        assert abs(np.log2(u0 / u1) - 2) < 0.08
+        assert abs(np.log2(e0 / e1) - 3) < 0.12
+
+
+def test_free_surface_is_traction_free():
+    rng = np.random.default_rng(11)
+    nu, G = 0.25, 1.0
+    lam = 2 * G * nu / (1 - 2 * nu)
+    for _ in range(6):
+        c = np.concatenate([rng.uniform(-1, 1, 2), [-rng.uniform(0.1, 2)]])
+        P = [c + rng.uniform(-0.6, 0.6, 3) for _ in range(3)]
+        for p in P:
+            p[2] = min(p[2], -0.02)
+        xy = rng.uniform(-3, 3, (30, 2))
+        s, _ = O.td(xy[:, 0], xy[:, 1], np.zeros(30), *(p[None, :] for p in P), [1.0], [1.0], [1.0], nu, G, 0, 1, 1)
+        for m in ("Dn", "Dss", "Dds"):
+            e = {k: s[k + m][:, 0] for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz")}
+            tr = e["exx"] + e["eyy"] + e["ezz"]
+            inplane = max(np.max(np.abs(v)) for v in (2 * G * e["exx"] + lam * tr, 2 * G * e["eyy"] + lam * tr,
+                                                       2 * G * e["exy"]))
+            surface = max(np.max(np.abs(v)) for v in (2 * G * e["ezz"] + lam * tr, 2 * G * e["exz"],
+                                                       2 * G * e["eyz"]))
+            assert surface <= 1e-12 * inplane
+
+
+def test_face_normals_and_midpoints():
+    rng = np.random.default_rng(13)
+    P1, P2, P3 = (rng.uniform(-5, 5, (200, 3)) for _ in range(3))
+    n_h, m_h = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    n_o, m_o = O.face_normal_midpoint(P1, P2, P3)
+    assert np.max(np.abs(n_h - n_o)) <= 4 * np.finfo(float).eps
+    assert np.max(np.abs(m_h - m_o)) <= 4 * np.finfo(float).eps * np.max(np.abs(m_o))
+    assert np.allclose(np.linalg.norm(n_o, axis=1), 1.0, rtol=0, atol=1e-15)
+    for A_, B_ in ((P1, P2), (P2, P3), (P3, P1)):
+        side = B_ - A_
+        assert np.max(np.abs(np.sum(n_o * side, axis=1)) / np.linalg.norm(side, axis=1)) <= 1e-14
+    # right-handed: n along (P2 - P1) x (P3 - P1)
+    assert np.all(np.sum(n_o * np.cross(P2 - P1, P3 - P1), axis=1) > 0)
+    # incentre: in the triangle's plane, equidistant from the three sides, at the inradius 2 area / perimeter
+
+    def dist(X, A_, B_):
+        t = np.sum((X - A_) * (B_ - A_), axis=1) / np.sum((B_ - A_) ** 2, axis=1)
+        return np.linalg.norm(X - A_ - t[:, None] * (B_ - A_), axis=1)
+    area = 0.5 * np.linalg.norm(np.cross(P2 - P1, P3 - P1), axis=1)
+    per = sum(np.linalg.norm(B_ - A_, axis=1) for A_, B_ in ((P1, P2), (P2, P3), (P3, P1)))
+    r_in = 2 * area / per
+    for A_, B_ in ((P1, P2), (P2, P3), (P3, P1)):
+        assert np.max(np.abs(dist(m_o, A_, B_) - r_in) / r_in) <= 1e-9
+    assert np.max(np.abs(np.sum(n_o * (m_o - P1), axis=1))) <= 1e-12
 
 
 def test_halfspace_rejects_points_above_surface():
