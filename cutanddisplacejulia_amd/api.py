@@ -239,6 +239,31 @@ def TD(X, Y, Z, P1List, P2List, P3List, Dss, Dds, Dn, ν, G, DispFlag, StrainFla
     return S, D
 
 
+def TD_sum(ExxDn, EyyDn, EzzDn, ExyDn, ExzDn, EyzDn,
+           ExxDss, EyyDss, EzzDss, ExyDss, ExzDss, EyzDss,
+           ExxDds, EyyDds, EzzDds, ExyDds, ExzDds, EyzDds,
+           UxDn, UyDn, UzDn, UxDss, UyDss, UzDss, UxDds, UyDds, UzDds):
+    """Sums the three slip modes' influence matrices over the elements (TD_sum.jl:1-41): each output is the
+    row sum of Dn + Dss + Dds, an (m, 1) column; an empty strain (or displacement) set passes through unchanged, as
+    the influence form returns for a flag of 0.  Returns (Exx, Eyy, Ezz, Exy, Exz, Eyz, Ux, Uy, Uz)."""
+    def total(a, b, c):
+        return np.sum(np.asarray(a) + np.asarray(b) + np.asarray(c), axis=1, keepdims=True)
+
+    E = (ExxDn, EyyDn, EzzDn, ExyDn, ExzDn, EyzDn)
+    if np.size(ExxDn) == 0:
+        strains = E
+    else:
+        Es = (ExxDss, EyyDss, EzzDss, ExyDss, ExzDss, EyzDss)
+        Ed = (ExxDds, EyyDds, EzzDds, ExyDds, ExzDds, EyzDds)
+        strains = tuple(total(a, b, c) for a, b, c in zip(E, Es, Ed))
+    if np.size(UxDn) == 0:
+        disps = (UxDn, UyDn, UzDn)
+    else:
+        disps = tuple(total(a, b, c) for a, b, c in ((UxDn, UxDss, UxDds), (UyDn, UyDss, UyDds),
+                                                       (UzDn, UzDss, UzDds)))
+    return strains + disps
+
+
 # ---- influence matrices and slip ------------------------------------------------------------------------
 
 

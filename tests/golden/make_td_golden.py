@@ -1,6 +1,6 @@
 """Writes tests/golden/td_golden.json: displacement and strain influences of single triangular dislocations, to 25
-significant digits (the inputs are doubles, written exactly), at points near edges, vertices and the triangle's plane, and elsewhere, in full space and half
-space.  Run: python tests/golden/make_td_golden.py
+significant digits (the inputs are doubles, written exactly), at points near edges, vertices and the triangle's
+plane, and elsewhere, in full space and half space.  Run: python tests/golden/make_td_golden.py
 
 Displacement: an mpmath restatement at 60 significant digits of the reference's displacement path for one triangle
 and unit slip of each mode (Dn, Dss, Dds):
