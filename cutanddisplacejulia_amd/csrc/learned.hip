@@ -203,10 +203,12 @@ __global__ void __launch_bounds__(256) gemm_f32_kernel(int M, int N, int K, cons
 }
 
 // ---- BF16 inference GEMM on MFMA: C = act(A W + bias), A (M x K, FP32 row-major) rounded to BF16 as it is
-// staged, W (K x N, BF16 row-major), FP32 accumulation on v_mfma_f32_16x16x16bf16.  64 x 64 tile per workgroup,
-// each wave 16 rows x 64 columns (4 MFMA tiles); K and N multiples of 16.
-typedef short s4 __attribute__((ext_vector_type(4)));
+// staged, W (K x N, BF16 row-major), FP32 accumulation on v_mfma_f32_16x16x32_bf16.  64 x 64 tile per workgroup,
+// each wave 16 rows x 64 columns (4 MFMA tiles); K in slices of 32 (a last slice of 16 is zero-padded), K and N
+// multiples of 16.  Operand lanes: lane l holds k 8 (l / 16) .. + 7 of row (column) l % 16.
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 // FP32 -> BF16, round to nearest even (NaN stays a quiet NaN)
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -225,35 +227,42 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
                                                         const unsigned short* __restrict__ W, int ldw,
                                                         float* __restrict__ C, int ldc, const float* __restrict__ bias,
                                                         int act) {
-  __shared__ unsigned short As[64][16 + 4];  // As[r][k]
-  __shared__ unsigned short Ws[64][16 + 4];  // Ws[c][k] (transposed)
+  __shared__ us8 As[64][4 + 1];  // As[r][o] = BF16 bits of A[r][8 o .. 8 o + 7] of the slice
+  __shared__ us8 Ws[64][4 + 1];  // Ws[c][o] = W[8 o .. 8 o + 7][c] of the slice (transposed)
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int li = lane % 16, lk = lane / 16;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   f4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int idx = tid + 256 * e;
-      const int r = idx / 16, kk = idx % 16;  // 16 lanes read one row's 16 consecutive k
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    {
+      // A: thread -> row tid / 4, octet tid % 4 (8 consecutive k); W: thread -> column tid % 64, octet tid / 64
+      const int r = tid / 4, o = tid % 4;
       const int gm = m0 + r;
-      As[r][kk] = f2bf(gm < M ? A[(int64_t)gm * lda + k0 + kk] : 0.f);
-      const int c = idx % 64, kw = idx / 64;  // lanes read consecutive columns of one k row of W
+      us8 va;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int k = k0 + 8 * o + t;
+        va[t] = f2bf(gm < M && k < K ? A[(int64_t)gm * lda + k] : 0.f);
+      }
+      As[r][o] = va;
+      const int c = tid % 64, ow = tid / 64;
       const int gn = n0 + c;
-      Ws[c][kw] = gn < N ? W[(int64_t)(k0 + kw) * ldw + gn] : (unsigned short)0;
+      us8 vw;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int k = k0 + 8 * ow + t;
+        vw[t] = gn < N && k < K ? W[(int64_t)k * ldw + gn] : (unsigned short)0;
+      }
+      Ws[c][ow] = vw;
     }
     __syncthreads();
-    s4 a, bq[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) a[t] = (short)As[16 * wave + li][4 * lk + t];
+    const bf8 a = __builtin_bit_cast(bf8, As[16 * wave + li][lk]);
 #pragma unroll
     for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-      for (int t = 0; t < 4; ++t) bq[tj][t] = (short)Ws[16 * tj + li][4 * lk + t];
-#pragma unroll
-    for (int tj = 0; tj < 4; ++tj) acc[tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[tj], acc[tj], 0, 0, 0);
+      acc[tj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf8, Ws[16 * tj + li][lk]), acc[tj], 0,
+                                                         0, 0);
     __syncthreads();
   }
   // item v of lane l: row 4 (l / 16) + v of the wave's 16 rows, column l % 16 of MFMA tile tj

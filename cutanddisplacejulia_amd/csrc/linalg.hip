@@ -705,7 +705,7 @@ __global__ void __launch_bounds__(256) trsm_batched_kernel(const TrsmBatch<T> jb
 }
 
 // FP16 variant of the trailing update for FP32 factors: C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N] with both
-// operands rounded to FP16 and the products accumulated in FP32 on v_mfma_f32_16x16x16f16.  |L| <= 1 under
+// operands rounded to FP16 and the products accumulated in FP32 on v_mfma_f32_16x16x32_f16.  |L| <= 1 under
 // partial pivoting; each column c of U is scaled by a power of two sigma[c] (max |U[:, c]| * sigma in [0.5, 1))
 // before rounding, and the accumulated product divided by it, so FP16's range never clips the update.
 // The operands are rounded once per update into packed FP16 copies with k contiguous (Lh: row i's K values,
@@ -772,16 +772,18 @@ __global__ void __launch_bounds__(256) pack_u_f16_kernel(const float* __restrict
 
 // 128 x 128 tile per workgroup (4 waves in 2 x 2, 64 x 64 each), K in GH_K slices through two LDS buffers (one
 // barrier per slice), operands from the packed copies: 4 consecutive lanes read one row's (column's) 32 k of a
-// slice as two 16-byte loads each.  Row and column indices are clamped (the extra rows and columns only reach C
-// entries that are not written); the packed copies are zero beyond K.
+// slice as two 16-byte loads each.  One v_mfma_f32_16x16x32_f16 per tile pair and slice: lane l holds k
+// 8 (l / 16) .. + 7 of row (column) l % 16 of both operands.  Row and column indices are clamped (the extra rows
+// and columns only reach C entries that are not written); the packed copies are zero beyond K.
 __global__ void __launch_bounds__(256, 2) gemm_sub_f16_kernel(const _Float16* __restrict__ Lh,
                                                               const _Float16* __restrict__ Uh,
                                                               float* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
                                                               int64_t K, const float* __restrict__ sigma, int gx,
                                                               int gy) {
-  constexpr int QK = GH_K / 4;  // quads of k per row and slice (8)
-  __shared__ h4 Lq[2][GB_M][QK + 1];
-  __shared__ h4 Uq[2][GB_N][QK + 1];
+  static_assert(GH_K == 32, "one 16x16x32 MFMA per slice");
+  constexpr int OK = GH_K / 8;  // octets of k per row and slice (4)
+  __shared__ h8 Lq[2][GB_M][OK + 1];
+  __shared__ h8 Uq[2][GB_N][OK + 1];
   const int tid = threadIdx.x;
   int tx, ty;
   xcd_tile(blockIdx.x, gx, gy, tx, ty);
@@ -789,7 +791,7 @@ __global__ void __launch_bounds__(256, 2) gemm_sub_f16_kernel(const _Float16* __
   const int wave = tid / 64, lane = tid % 64;
   const int wr = (wave % 2) * 64, wc = (wave / 2) * 64;
   const int li = lane % 16, lk = lane / 16;
-  // staging: thread -> row (column) r = tid / 4 + 64 e, quads 2 (tid % 4) .. + 1
+  // staging: thread -> row (column) r = tid / 4 + 64 e, octet tid % 4
   const int sp = tid % 4, sr = tid / 4;
   const _Float16* lsrc[2];
   const _Float16* usrc[2];
@@ -810,10 +812,8 @@ __global__ void __launch_bounds__(256, 2) gemm_sub_f16_kernel(const _Float16* __
   auto stash = [&](int buf) {
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      Lq[buf][sr + 64 * e][2 * sp] = h4{lr[e][0], lr[e][1], lr[e][2], lr[e][3]};
-      Lq[buf][sr + 64 * e][2 * sp + 1] = h4{lr[e][4], lr[e][5], lr[e][6], lr[e][7]};
-      Uq[buf][sr + 64 * e][2 * sp] = h4{ur[e][0], ur[e][1], ur[e][2], ur[e][3]};
-      Uq[buf][sr + 64 * e][2 * sp + 1] = h4{ur[e][4], ur[e][5], ur[e][6], ur[e][7]};
+      Lq[buf][sr + 64 * e][sp] = lr[e];
+      Uq[buf][sr + 64 * e][sp] = ur[e];
     }
   };
   f4 acc[4][4];
@@ -828,20 +828,17 @@ __global__ void __launch_bounds__(256, 2) gemm_sub_f16_kernel(const _Float16* __
   __syncthreads();
   for (int64_t s = 0; s < nk; ++s) {
     const int buf = (int)(s & 1);
+    h8 lv[4], uv[4];
 #pragma unroll
-    for (int kq = 0; kq < QK; kq += 4) {  // 16 k per MFMA: quads kq .. kq + 3, lane group lk takes quad kq + lk
-      h4 lv[4], uv[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        lv[t] = Lq[buf][wr + 16 * t + li][kq + lk];
-        uv[t] = Uq[buf][wc + 16 * t + li][kq + lk];
-      }
-#pragma unroll
-      for (int tj = 0; tj < 4; ++tj)
-#pragma unroll
-        for (int ti = 0; ti < 4; ++ti)
-          acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x16f16(uv[tj], lv[ti], acc[tj][ti], 0, 0, 0);
+    for (int t = 0; t < 4; ++t) {
+      lv[t] = Lq[buf][wr + 16 * t + li][lk];
+      uv[t] = Uq[buf][wc + 16 * t + li][lk];
     }
+#pragma unroll
+    for (int tj = 0; tj < 4; ++tj)
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+        acc[tj][ti] = __builtin_amdgcn_mfma_f32_16x16x32_f16(uv[tj], lv[ti], acc[tj][ti], 0, 0, 0);
     if (s + 1 < nk) {
       stash(buf ^ 1);
       if (s + 2 < nk) fetch((s + 2) * GH_K);
