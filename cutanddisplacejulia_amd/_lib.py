@@ -48,7 +48,7 @@ EXPORTS = ["cdj_create", "cdj_destroy", "cdj_last_error", "cdj_last_kernel_ms", 
            "cdj_precond_solve_dev", "cdj_last_near_entries", "cdj_batched_slip", "cdj_batch_tuning",
            "cdj_traction_rows", "cdj_lstsq", "cdj_crack_tip_elements", "cdj_stress_intensity",
            "cdj_stress_intensity_dev", "cdj_slip_friction", "cdj_slip_fluid_volume", "cdj_mlp_forward",
-           "cdj_assemble_dev_noalloc", "cdj_last_edge_stats"]
+           "cdj_assemble_dev_noalloc", "cdj_last_edge_stats", "cdj_remesh_isotropic", "cdj_remesh_take"]
 
 
 class CdjError(RuntimeError):
@@ -120,6 +120,8 @@ def load(path=LIB_PATH):
     L.cdj_lu_factor_dev.argtypes = [_p, _p, _i64, _i64, _p, _p]
     L.cdj_lu_factor_batched_dev.argtypes = [_p, _p, _p, _p, _i64, _p, _p, _p]
     L.cdj_lu_solve_dev.argtypes = [_p, _p, _i64, _i64, _p, _p, _i64, _i64, _p]
+    L.cdj_remesh_isotropic.argtypes = [_p, _i64, _p, _i64, _d, _i32, _p, _p, _p]
+    L.cdj_remesh_take.argtypes = [_p, _p, _p]
     for name in EXPORTS:
         f = getattr(L, name)
         if name == "cdj_last_kernel_ms":

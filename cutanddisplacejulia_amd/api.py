@@ -9,6 +9,9 @@ Names, argument order and meaning follow the reference so that callers (and test
   ElasticConstantsCheck           src/ElasticConstantsCheck.jl:37-49 (G, nu form)
   GetCrackTipElements3D           src/GetCrackTipElements3D.jl:1
   StressIntensity3D               src/StressIntensity3D.jl:1
+  TD_sum                          src/TD_sum.jl:1-41
+  AreaOfTriangle3D                src/AreaOfTriangle3D.jl:52-131
+  GetDesiredEdgeLength            src/GetDesiredEdgeLength.jl:1-12
 Data types mirror src/CutAndDisplaceJulia.jl:46-91.  Arrays are numpy; N x 3 arrays hold one triangle per row
 (P1[:, 0] are x coordinates), as in the reference.  Every TD / assembly / solve call runs in libcdjhip.so; the
 O(N) host-side helpers here (normals, incentres, remote-stress tractions) prepare inputs as the reference's
@@ -420,6 +423,33 @@ def CreateFaceNormalAndMidPoint(P1, P2, P3):
         raise CdjError(_lib.CDJ_ERR_DEGENERATE, "Slither triangles or triangles with no area exist on your surface")
     M = (P1 * a[:, None] + P2 * bb[:, None] + P3 * c[:, None]) / (a + bb + c)[:, None]
     return f64(N), f64(M)
+
+
+def AreaOfTriangle3D(x1, y1, z1, x2, y2, z2, x3, y3, z3):
+    """Triangle areas by Heron's formula and half perimeters (AreaOfTriangle3D.jl:52-131): side lengths P1-P2,
+    P2-P3, P3-P1; a triangle whose Heron product rounds below zero gets NaN when its sides break the triangle
+    inequality, else 0."""
+    x1, y1, z1, x2, y2, z2, x3, y3, z3 = (np.atleast_1d(np.asarray(v, dtype=np.float64))
+                                          for v in (x1, y1, z1, x2, y2, z2, x3, y3, z3))
+    a = np.sqrt((x1 - x2) ** 2 + (y1 - y2) ** 2 + (z1 - z2) ** 2)
+    b = np.sqrt((x2 - x3) ** 2 + (y2 - y3) ** 2 + (z2 - z3) ** 2)
+    c = np.sqrt((x3 - x1) ** 2 + (y3 - y1) ** 2 + (z3 - z1) ** 2)
+    hp = (a + b + c) / 2.0
+    prod = hp * (hp - a) * (hp - b) * (hp - c)
+    area = np.sqrt(np.maximum(prod, 0.0))
+    bad = (prod < 0) & ((a + b < c) | (b + c < a) | (a + c < b))
+    area[bad] = np.nan
+    return area, hp
+
+
+def GetDesiredEdgeLength(P1, P2, P3, NoTris):
+    """Edge length of the equilateral triangles that tile the surface's area in NoTris triangles, and 2/3 of the
+    largest half perimeter (GetDesiredEdgeLength.jl:1-12): the target passed to the remesher."""
+    P1, P2, P3 = (np.atleast_2d(np.asarray(P, dtype=np.float64)) for P in (P1, P2, P3))
+    area, hp = AreaOfTriangle3D(P1[:, 0], P1[:, 1], P1[:, 2], P2[:, 0], P2[:, 1], P2[:, 2], P3[:, 0], P3[:, 1],
+                                P3[:, 2])
+    desired = np.sum(area) / NoTris
+    return float(np.sqrt(desired / (np.sqrt(3) / 4))), float(np.max(hp) * (2 / 3))
 
 
 def CalculateSSandDSDirs(CosAx, CosAy, CosAz):

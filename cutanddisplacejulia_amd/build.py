@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcdjhip.so")
-SOURCES = ["td_kernels.hip", "linalg.hip", "mixed.hip", "learned.hip", "batch.hip", "crack_tip.hip", "friction.hip", "cdj_abi.cpp"]
+SOURCES = ["td_kernels.hip", "linalg.hip", "mixed.hip", "learned.hip", "batch.hip", "crack_tip.hip", "friction.hip",
+           "remesh.cpp", "cdj_abi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CDJ_OFFLOAD_ARCH", "gfx950")

@@ -318,6 +318,22 @@ int cdj_stress_intensity_dev(cdj_ctx* ctx, const double* midpoints, const double
                              const double* Dds, double G, double nu, uint8_t* free_flags, double* geometry, double* K,
                              void* stream);
 
+/* ---- isotropic remeshing (host) ----
+ * Replaces BuildCGAL.PolygonRemeshingCGAL (CGAL 4.13.1 isotropic_remeshing), which the reference's growth loops
+ * call between OFFExport and OFFReader (test/test_PropagationTest.jl:151-561).  The published algorithm (Botsch &
+ * Kobbelt 2004) with target edge length L: split edges longer than 4/3 L, collapse edges shorter than 4/5 L,
+ * flip for valence 6 (4 on the boundary), tangential relaxation, projection onto the input surface; `iterations`
+ * rounds.  The boundary keeps its shape (its vertices stay on the input's boundary polyline; corners turning by
+ * more than 45 degrees stay put) and triangle orientation is kept.
+ * points: npoints x 3, column-major (x, y, z); tris: ntris x 3, column-major, 0-based.  On success *out holds the
+ * result, of *npoints_out points and *ntris_out triangles, until cdj_remesh_take copies it (same layouts; either
+ * pointer may be NULL) and frees it.  No GPU is used.  CDJ_ERR_DEGENERATE for a triangle with a repeated corner. */
+typedef struct cdj_remesh cdj_remesh;
+int cdj_remesh_isotropic(const double* points, int64_t npoints, const int64_t* tris, int64_t ntris,
+                         double target_edge_length, int32_t iterations, cdj_remesh** out, int64_t* npoints_out,
+                         int64_t* ntris_out);
+int cdj_remesh_take(cdj_remesh* r, double* points, int64_t* tris);
+
 #ifdef __cplusplus
 }
 #endif

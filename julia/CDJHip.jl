@@ -3,13 +3,14 @@
 # Drop-in replacements for TD (src/TD.jl:77, :135, :219), CalculateInfluenceMatrices3D
 # (src/CalculateInfluenceMatrices3D.jl:2), SlipCalculator3D (src/SlipCalculator3D.jl:1),
 # GetCrackTipElements3D (src/GetCrackTipElements3D.jl:1) and StressIntensity3D (src/StressIntensity3D.jl:1) with
-# the same signatures.  Include this file after CutAndDisplaceJulia and call CDJHip.TD(...),
+# the same signatures, and PolygonRemeshingCGAL for BuildCGAL's (the growth loops' remeshing step).  Include this file after CutAndDisplaceJulia and call CDJHip.TD(...),
 # CDJHip.CalculateInfluenceMatrices3D(...) and CDJHip.SlipCalculator3D(...) where the package's functions were
 # called.  The precision mode lives in module-level settings (CDJHip.PRECISION[], CDJHip.ASSEMBLY[]), so the
 # public signatures stay unchanged.
 module CDJHip
 
 using ..CutAndDisplaceJulia: StrainInf, DispInf, Strains, Disps, TractionInf, SetupTractionVector, KnockOutFixedRows,
+    OFFReader, OFFExport,
     TriangleEdges, MixedBoundaryConditionsFriction, MixedBoundaryConditionsFluidVolume
 
 const LIB = get(ENV, "CDJ_LIB", joinpath(@__DIR__, "..", "cutanddisplacejulia_amd", "libcdjhip.so"))
@@ -222,6 +223,25 @@ function StressIntensity3D(Dn, Dss, Dds, G, ν, FaceNormalVector, FeP1P2S, FeP1P
         e.K1 = k[:, 1]; e.K2 = k[:, 2]; e.K3 = k[:, 3]; e.StrainEnergy = k[:, 4]
     end
     return FeP1P2S, FeP1P3S, FeP2P3S
+end
+
+# Isotropic remeshing in place of BuildCGAL.PolygonRemeshingCGAL (file in, file out, as the reference's growth loops
+# call it between OFFExport and OFFReader; test/test_PropagationTest.jl:151-561).  Host code, no GPU.
+function PolygonRemeshingCGAL(OutputDirectory, target_edge_length; iterations=3)
+    (Points, Triangles) = OFFReader(OutputDirectory)
+    pos = Dict(Int(Points[i, 1]) => i for i in 1:size(Points, 1))
+    tris = Int64[pos[Int(t)] - 1 for t in Triangles]          # 0-based rows of Points
+    pts = convert(Matrix{Float64}, Points[:, 2:4])
+    h = Ref{Ptr{Cvoid}}(C_NULL); np = Ref{Int64}(0); nt = Ref{Int64}(0)
+    check(ccall((:cdj_remesh_isotropic, LIB), Cint,
+        (Ptr{Float64}, Int64, Ptr{Int64}, Int64, Float64, Int32, Ref{Ptr{Cvoid}}, Ref{Int64}, Ref{Int64}),
+        pts, size(pts, 1), tris, size(tris, 1), target_edge_length, iterations, h, np, nt))
+    P = zeros(np[], 3); T = zeros(Int64, nt[], 3)
+    check(ccall((:cdj_remesh_take, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Int64}), h[], P, T))
+    Pts = hcat(collect(1.0:np[]), P)
+    base = splitext(OutputDirectory)[1] * "-Remeshed"   # absolute: the package's OFFExport would prefix pwd()
+    OFFExport(Pts, T .+ 1, nt[], np[], base)
+    return base * ".off"
 end
 
 end # module
