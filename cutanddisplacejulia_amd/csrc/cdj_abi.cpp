@@ -2414,7 +2414,19 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
   build_friction_lcp(C, n, dD.as<double>(), dmu.as<double>(), dSf.as<double>(), a.as<double>(), bb.as<double>(), st);
   HIPCHK(hipGetLastError());
   AI.reset();  // the inverse is folded into a: free its 18 n^2 doubles before the Newton scratch
-  HIPCHK(J.alloc(sizeof(double) * N5 * N5));
+  // frictionless elements (mu = Sf = 0) leave degenerate unknowns out of the Newton solve (lcp_keep)
+  const std::vector<int32_t> keep = lcp_keep(mu, Sf, n);
+  const int64_t Nr = (int64_t)keep.size();
+  DBuf dkeep, ar, br, xr;
+  if (Nr < N5) {
+    HIPCHK(dkeep.alloc(sizeof(int32_t) * Nr));
+    HIPCHK(ar.alloc(sizeof(double) * Nr * Nr));
+    HIPCHK(br.alloc(sizeof(double) * Nr));
+    HIPCHK(xr.alloc(sizeof(double) * Nr));
+    HIPCHK(hipMemcpyAsync(dkeep.p, keep.data(), sizeof(int32_t) * Nr, hipMemcpyHostToDevice, st));
+    lcp_gather(a.as<double>(), bb.as<double>(), N5, dkeep.as<int32_t>(), Nr, ar.as<double>(), br.as<double>(), st);
+  }
+  HIPCHK(J.alloc(sizeof(double) * Nr * Nr));
   HIPCHK(x.alloc(sizeof(double) * N5));
   HIPCHK(vec.alloc(sizeof(double) * 8 * N5));
   HIPCHK(red.alloc(sizeof(double) * 8));
@@ -2424,11 +2436,15 @@ int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const do
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
-  HIPCHK(iwork.alloc(sizeof(int) * fn_iwork_ints(N5)));
-  hipError_t fe = fischer_newton(a.as<double>(), bb.as<double>(), N5, x.as<double>(), fp, fr, J.as<double>(),
-                                 ipiv.as<int32_t>(), info.as<int>(), ws.p, vec.as<double>(), red.as<double>(),
-                                 part.as<double>(), iwork.as<int>(), st);
+  HIPCHK(iwork.alloc(sizeof(int) * fn_iwork_ints(Nr)));
+  const bool red_lcp = Nr < N5;
+  const double* la = red_lcp ? ar.as<double>() : a.as<double>();
+  const double* lb = red_lcp ? br.as<double>() : bb.as<double>();
+  double* lx = red_lcp ? xr.as<double>() : x.as<double>();
+  hipError_t fe = fischer_newton(la, lb, Nr, lx, fp, fr, J.as<double>(), ipiv.as<int32_t>(), info.as<int>(), ws.p,
+                                 vec.as<double>(), red.as<double>(), part.as<double>(), iwork.as<int>(), st);
   if (fe != hipSuccess) return hip_fail(ctx, fe, "Fischer-Newton");
+  if (red_lcp) lcp_scatter(xr.as<double>(), dkeep.as<int32_t>(), Nr, x.as<double>(), N5, st);
   HIPCHK(D3.alloc(v3));
   friction_slip(a.as<double>(), bb.as<double>(), x.as<double>(), n, scale, vec.as<double>(), D3.as<double>(),
                 part.as<double>(), st);

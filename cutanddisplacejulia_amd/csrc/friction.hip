@@ -180,6 +180,46 @@ void add_vector(double* y, const double* x, int64_t N, hipStream_t st) {
   hipLaunchKernelGGL(add_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, y, x, N);
 }
 
+std::vector<int32_t> lcp_keep(const double* mu, const double* Sf, int64_t n) {
+  std::vector<int32_t> keep;
+  keep.reserve(5 * n);
+  for (int k = 0; k < 5; ++k)
+    for (int64_t i = 0; i < n; ++i)
+      if (k == 0 || mu[i] != 0.0 || Sf[i] != 0.0) keep.push_back((int32_t)(k * n + i));
+  return keep;
+}
+
+__global__ void __launch_bounds__(256) lcp_gather_kernel(const double* __restrict__ a, const double* __restrict__ bb,
+                                                         int64_t N, const int32_t* __restrict__ keep, int64_t Nr,
+                                                         double* __restrict__ ar, double* __restrict__ br) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= Nr * Nr) return;
+  const int64_t i = t % Nr, j = t / Nr;
+  ar[t] = a[keep[i] + (int64_t)keep[j] * N];
+  if (j == 0) br[i] = bb[keep[i]];
+}
+
+__global__ void lcp_scatter_kernel(const double* __restrict__ xr, const int32_t* __restrict__ keep, int64_t Nr,
+                                   double* __restrict__ x) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < Nr) x[keep[k]] = xr[k];
+}
+
+__global__ void fill_zero_kernel(double* __restrict__ x, int64_t N) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < N) x[i] = 0.0;
+}
+
+void lcp_gather(const double* a, const double* bb, int64_t N, const int32_t* keep, int64_t Nr, double* ar,
+                double* br, hipStream_t st) {
+  hipLaunchKernelGGL(lcp_gather_kernel, dim3(nblk(Nr * Nr, 256)), dim3(256), 0, st, a, bb, N, keep, Nr, ar, br);
+}
+
+void lcp_scatter(const double* xr, const int32_t* keep, int64_t Nr, double* x, int64_t N, hipStream_t st) {
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(nblk(N, 256)), dim3(256), 0, st, x, N);
+  hipLaunchKernelGGL(lcp_scatter_kernel, dim3(nblk(Nr, 256)), dim3(256), 0, st, xr, keep, Nr, x);
+}
+
 void build_friction_lcp_vector(const double* C, int64_t n, const double* D, const double* Sf, double* bb,
                                hipStream_t st) {
   hipLaunchKernelGGL(lcp_vector_kernel, dim3(nblk(5 * n, 256)), dim3(256), 0, st, C, n, D, Sf, bb);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace cdj {
 
@@ -22,6 +23,20 @@ void build_friction_lcp_vector(const double* C, int64_t n, const double* D, cons
 
 // y[0:N] += x[0:N]
 void add_vector(double* y, const double* x, int64_t N, hipStream_t st);
+
+// The LCP unknowns kept (ascending indices into 0 .. 5n - 1): for an element with mu = 0 and Sf = 0 its two traction
+// parts (x2, x3; rows 3n + i and 4n + i read y = -x, so both are 0) and its two negative slip parts (x4, x5; they
+// enter only its own slip rows, so y2, y3 carry the slip's sign) are removed.  The LCP on the rest has the same
+// solution for every kept unknown and no degenerate pair (x = y = 0), on which Fischer-Newton converges only
+// linearly.  All 5n are kept when no element is frictionless.
+std::vector<int32_t> lcp_keep(const double* mu, const double* Sf, int64_t n);
+
+// ar (Nr x Nr, leading dimension Nr) = a[keep, keep] (a: N x N, leading dimension N), br = bb[keep]
+void lcp_gather(const double* a, const double* bb, int64_t N, const int32_t* keep, int64_t Nr, double* ar,
+                double* br, hipStream_t st);
+
+// x[0:N] = 0, then x[keep[k]] = xr[k]
+void lcp_scatter(const double* xr, const int32_t* keep, int64_t Nr, double* x, int64_t N, hipStream_t st);
 
 struct FnParams {
   int max_iter = 100;

@@ -270,7 +270,9 @@ int cdj_mlp_forward(cdj_ctx* ctx, const cdj_mlp* mlp, const double* P1, const do
  * cohesions.  When no element can overcome its frictional strength the slip is zero (SlipCalculator3D.jl:52-59).
  * Otherwise the collocation matrix (scaled by mean half perimeter / G) is inverted on the GPU, the 5n x 5n
  * complementarity problem of SlipCalculator3D.jl:90-167 is built there and solved by a Fischer-Newton method
- * (the Newton systems factored by the GPU LU).  report: iterations = Newton iterations, residual = final
+ * (the Newton systems factored by the GPU LU).  For elements with mu = 0 and Sf = 0 the four unknowns that the
+ * problem fixes at zero or that only split the slip's sign are left out of the Newton solve (same solution; without
+ * this the method stalls on those degenerate pairs).  report: iterations = Newton iterations, residual = final
  * 0.5 ||phi||^2 of the Fischer-Burmeister function, solve_ms = device time of the complementarity solve.
  * 5 n <= 65535. */
 int cdj_slip_friction(cdj_ctx* ctx, const double* P1, const double* P2, const double* P3, int64_t n, double nu,

@@ -323,3 +323,56 @@ def burgmann1994_linear_friction_slip(G, nu, Ts, x, a):
         r = np.sqrt(a ** 2 - x ** 2)
         ach = np.arccosh(a / x)
         return ((1 - nu) / G) * (2 * ((Ts * r) - ((Ts * (1 / np.pi)) * (r + (x ** 2 / a) * ach))))
+
+
+def savage1984_gravity_valley(tectxx, rg, nu, a, b, u, v):
+    """Stresses under a long symmetric valley carved into a half-plane loaded by gravity and a remote horizontal
+    stress (Savage, Powers & Swolfs 1984, USGS OFR 84-827, the complex-potential solution of their RVT program), as
+    Savage1984_GravityValleyStress.jl: the points w = u + i v (v < 0) of the mapping plane go to
+    z = w + a b / (w - i a); returns the total stresses (Sxx, Syy, Sxy) and the mapped points (X, Y), tension
+    positive, Y up.  As the reference, when any point lies within 1e-4 of w = -i a every point takes the Taylor
+    expansion of the second potential about it."""
+    u, v = np.asarray(u, dtype=np.float64), np.asarray(v, dtype=np.float64)
+    I = 1j
+    so = rg * b
+    ai = I * a
+    # potentials at w = -i a: gravity (eq. 4), tectonic (eq. 12) and the tectonic one's second derivative
+    phia = -so * ((4 * a + b) * (1 - nu) + b) / (8 * (1 - nu) * (2 * a + b))
+    phiat = -(tectxx * b) / (4 * (2 * a + b))
+    d2phia = -(tectxx * b * (4 * a + b) * (b - 12 * a)) / (2 * a * (2 * a + b) * (4 * a + b) ** 3)
+    w = u + I * v
+    r = np.sqrt(u ** 2 + (v + a) ** 2)
+    z = w + a * b / (w - ai)                                   # eq. 1
+    wm, wp = w - ai, w + ai
+    X, Y = z.real, z.imag
+    dz = (wm ** 2 - a * b) / wm ** 2
+    aw = -(I * (4 * a + b) / (8 * wm)) - a * b * (w - 3 * ai) / (8 * (1 - nu) * wm ** 3)     # eq. 3
+    phi = -aw * so / dz + a * b * phia / (dz * wm ** 2)        # eq. 6
+    awt = -(a * b * tectxx) / (2 * wm ** 2)                    # eq. 11
+    phit = -awt / dz + a * b * phiat / (dz * wm ** 2)          # eq. 10
+    sumt = 4 * phit.real + tectxx                              # eq. 8
+    sumy = sumt + 4 * phi.real + rg * Y / (1 - nu)
+    dlaw = (I * (4 * a + b) / (8 * wm ** 2) + 2 * a * b / (8 * (1 - nu) * wm ** 3)
+            - 6 * I * a ** 2 * b / (8 * (1 - nu) * wm ** 4))
+    dlphi = -so * dlaw / dz - (2 * a * b * (phi + phia)) / (dz * wm ** 3)
+    d2phi = -(2 * phi / wm ** 2 + 4 * dlphi / wm + so * I * a ** 2 * b / (2 * (1 - nu) * wm ** 5)) / dz
+    bw = -so * (I * (4 * a + b) / (8 * wm) + (1 - 2 * nu) * a * b * (w - 3 * ai) / (8 * (1 - nu) * wm ** 3))  # eq. 7
+    psil = w * dlphi + bw + phi
+    dlphit = -(a * b * tectxx * (4 * a + b) * wm) / (2 * (2 * a + b) * (wm ** 2 - a * b) ** 2)
+    psilt = w * dlphit - awt + phit                            # part of eq. 9
+    if np.any(r < 1.0e-4):
+        psi2 = 0.5 * a * b * d2phi
+        psi2t = 0.5 * a * b * d2phia
+    else:
+        psi2 = a * b * dlphi / wp - a * b * (phi - phia) / wp ** 2
+        psi2t = a * b * dlphit / wp - a * b * (phit - phiat) / wp ** 2
+    psi = -(psil + psi2) / dz
+    psit = -(psilt + psi2t) / dz
+    zbar = np.conj(z)
+    strl = 2 * (zbar * dlphi / dz + psi)                       # eq. 5
+    strlt = 2 * (zbar * dlphit / dz + psit)
+    dif = strlt.real - tectxx + strl.real + rg * Y * (1 - 2 * nu) / (1 - nu)
+    Sxy = (strl.imag + strlt.imag) / 2
+    Sxx = (sumy - dif) / 2
+    Syy = (sumy + dif) / 2
+    return Sxx, Syy, Sxy, X, Y

@@ -93,6 +93,20 @@ def friction_lcp(C, D, mu, Sf, n):
     return a, bb
 
 
+def lcp_keep(mu, Sf, n):
+    """The LCP unknowns left after removing, for every element with mu = 0 and Sf = 0, its two traction parts
+    (x2, x3: rows 3 n + i and 4 n + i read y = -x, so both are 0) and its two negative slip parts (x4, x5: they enter
+    only that element's own slip rows, so the slip's sign is carried by y2, y3).  The LCP on the rest has the same
+    solution for every remaining unknown and no degenerate pair (x = y = 0), on which Fischer-Newton converges only
+    linearly; with no such element nothing is removed."""
+    mu, Sf = np.broadcast_to(mu, (n,)), np.broadcast_to(Sf, (n,))
+    free = (mu == 0) & (Sf == 0)
+    keep = np.ones(5 * n, dtype=bool)
+    for k in range(1, 5):
+        keep[k * n:(k + 1) * n][free] = False
+    return keep
+
+
 def slip_friction(P1, P2, P3, nu, G, lam, mid, nrm, hs_flag, stress6, tractions, mu, Sf):
     """SlipCalculator3D(..., MixedBoundaryConditionsFriction, FixedEls = 0) -> (Dn, Dss, Dds, info)."""
     n = P1.shape[0]
@@ -110,7 +124,9 @@ def slip_friction(P1, P2, P3, nu, G, lam, mid, nrm, hs_flag, stress6, tractions,
     scl = np.mean(hp) / G
     C = np.linalg.inv(A * scl)
     a, bb = friction_lcp(C, C @ b, mu, Sf, n)
-    x, info = fischer_newton(a, bb)
+    keep = lcp_keep(mu, Sf, n)
+    x = np.zeros(5 * n)
+    x[keep], info = fischer_newton(a[np.ix_(keep, keep)], bb[keep])
     y = a @ x + bb
     L = [slice(k * n, (k + 1) * n) for k in range(5)]
     Dn = y[L[0]] * scl

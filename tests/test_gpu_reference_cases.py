@@ -7,6 +7,10 @@ folder), run through this engine with the reference's pass criteria.
 * test_TDvsMogi.jl: a 46-face sphere at 50 m depth in the half space, pressurised, with a 6-face fixed inner
   sphere appended (fixed elements: displacement rows, least-squares solve).  The opening spread must stay below
   0.05 / G, and the surface displacement along x = 0..100 must be within 20 % of Mogi's point source.
+* test_TractionFreeSurface.jl: the SquareGrid.stl plate in six orientations under unit remote stresses; residual
+  tractions below 1e-8 at every collocation point, with the plain and the (frictionless) friction solver.
+* test_TDvsSavageGravityValley.jl: a valley surface under gravity; mean stress residuals against Savage, Powers &
+  Swolfs (1984) below 0.15 on the reference's grid.
 """
 import numpy as np
 import pytest
@@ -93,3 +97,51 @@ def test_mogi_with_fixed_elements():
     print("mogi: surface displacement within %.2f %% of Mogi" % worst)
     for a, d in ((ux, D.Ux), (uy, D.Uy), (uz, D.Uz)):
         assert not _bad(_pct(a, d), 20)
+
+
+def test_traction_free_surface():
+    """test_TractionFreeSurface.jl on the GPU: the 392-triangle plate in six orientations under three unit remote
+    stresses each, solved by SlipCalculator3D and, as the reference also does, by the friction solver with
+    mu = 0, Sf = 0; the total traction at every collocation point (TD summed field there) is below 1e-8."""
+    from tests.test_oracle_reference_cases import hooke, square_plate_cases, unit_stress
+    K, E, lam, nu, G = api.ElasticConstantsCheck(5000.0, 0.25)
+    worst = {"stresses": 0.0, "friction": 0.0}
+    for P, T, keys in square_plate_cases():
+        P1, P2, P3 = (P[T[:, k] - 1, 1:4].copy() for k in range(3))
+        n = P1.shape[0]
+        nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+        z = np.zeros(n)
+        for key in keys:
+            remote = unit_stress(key)
+            st = api.Stresses(*[np.full(n, v) for v in remote])
+            mixed = api.MixedBoundaryConditions(st, api.Tractions(z, z, z))
+            bcs = {"stresses": st,
+                   "friction": api.MixedBoundaryConditionsFriction(mixed, api.FrictionParameters(0.0, 0.0))}
+            for name, bc in bcs.items():
+                out = api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 0, bc, z)
+                Dn, Dss, Dds = out[0], out[1], out[2]
+                S, _ = api.TD(mid[:, 0], mid[:, 1], mid[:, 2], P1, P2, P3, Dss, Dds, Dn, nu, G, 0, 1, 0,
+                              api.Strains(), api.Disps())
+                e = [S.εxx, S.εyy, S.εzz, S.εxy, S.εxz, S.εyz]
+                s = [a + b for a, b in zip(hooke(e, lam, G), remote)]
+                tn, tds, tss = api.CalculateNormalAndShearTractions3D(*s, nrm)
+                worst[name] = max(worst[name], float(np.max(np.abs(np.concatenate([tn, tds, tss])))))
+    print("traction-free plate, 6 orientations x 3 loads: worst residual traction %.2e (stresses), %.2e "
+          "(friction solver)" % (worst["stresses"], worst["friction"]))
+    assert worst["stresses"] < 1e-8 and worst["friction"] < 1e-8
+
+
+def test_savage_gravity_valley():
+    """test_TDvsSavageGravityValley.jl on the GPU: mean stress residuals under the valley below 0.15."""
+    from tests.test_oracle_reference_cases import valley_grid, valley_remote, valley_residuals, valley_setup
+    K, E, lam, nu, G = api.ElasticConstantsCheck(1.0, 0.25)
+    P1, P2, P3 = valley_setup()
+    n = P1.shape[0]
+    nrm, mid = api.CreateFaceNormalAndMidPoint(P1, P2, P3)
+    Dn, Dss, Dds, A, b = api.SlipCalculator3D(P1, P2, P3, nu, G, lam, mid, nrm, 0,
+                                              api.Stresses(*valley_remote(mid, nu)), np.zeros(n))
+    X, Y, Z, Sxx, Syy, Sxy = valley_grid(nu)
+    S, _ = api.TD(X, Y, Z, P1, P2, P3, Dss, Dds, Dn, nu, G, 0, 1, 0, api.Strains(), api.Disps())
+    res = valley_residuals([S.εxx, S.εyy, S.εzz, S.εxy, S.εxz, S.εyz], Z, lam, G, nu, Sxx, Syy, Sxy)
+    print("Savage valley: mean residuals sxx %.4f szz %.4f sxz %.4f" % tuple(res))
+    assert max(abs(r) for r in res) < 0.15

@@ -1,12 +1,18 @@
 """The oracle on the reference's own validation cases and meshes (tests/golden), with the reference's pass
-criteria (CPU; no GPU): test_TDvsEshelbyPennyCrack.jl and test_TDvsMogi.jl.  The oracle's collocation matrix
-is solved with numpy (LAPACK), the fixed-element system by least squares as Julia's `\\` does.
+criteria (CPU; no GPU): test_TDvsEshelbyPennyCrack.jl, test_TDvsMogi.jl, test_TractionFreeSurface.jl and
+test_TDvsSavageGravityValley.jl.  The oracle's collocation matrix is solved with numpy (LAPACK), the fixed-element
+system by least squares as Julia's `\\` does.
 """
 import numpy as np
 
+import os
+
+from cutanddisplacejulia_amd import meshio
 from oracle import analytic as AN
 from oracle import oracle as O
 from tests import gocad
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def _pct(analytic, numeric):
@@ -168,3 +174,154 @@ def test_oracle_fixed_rows_take_the_plane_of_their_free_count():
     # the quirk matters here: with each row's own plane the free rows differ
     own = O.traction_influence(nrm, mid, P1, P2, P3, nu, G, lam, 1)
     assert np.max(np.abs(Tfix["DnTn"][fr] - own["DnTn"][np.ix_(fr, fr)])) > 1e-3
+
+
+# ---- test_TractionFreeSurface.jl ---------------------------------------------------------------------------------
+# The flat 392-triangle plate of SquareGrid.stl in six orientations (normal +-z, +-y, +-x, as the reference builds
+# them by swapping and negating coordinates), each under three unit remote stresses acting on its plane; after the
+# slip solve, the total traction (remote plus the slip's own stress, TD at the collocation points) on every
+# element must be below 1e-8 (TractionFreeSurface.jl:45), by the plain solve and by the frictional solver with
+# mu = 0, Sf = 0, as the reference runs both.  G = 5000, nu = 0.25.
+
+def square_plate_cases():
+    P, T = meshio.STLReader(os.path.join(GOLDEN, "SquareGrid.stl"))
+    P = P.copy()
+    P[:, 3] = 0.0
+    cases = []
+
+    def add(P, stresses):
+        cases.append((P.copy(), T, stresses))
+
+    flat = ("sxz", "syz", "szz")
+    add(P, flat)
+    P[:, [1, 2]] = P[:, [2, 1]]                        # normal flipped by swapping x and y
+    add(P, flat)
+    P[:, [2, 3]] = P[:, [3, 2]]                        # vertical, striking 90 (swap y and z)
+    vert_y = ("sxy", "syz", "syy")
+    add(P, vert_y)
+    P[:, 1] = -P[:, 1]                                 # rotated 180 degrees
+    add(P, vert_y)
+    P[:, [1, 2]] = P[:, [2, 1]]                        # vertical, striking 0 (swap x and y)
+    vert_x = ("sxy", "sxz", "sxx")
+    add(P, vert_x)
+    P[:, 2] = -P[:, 2]                                 # rotated 180 degrees
+    add(P, vert_x)
+    return cases
+
+
+STRESS_KEYS = ("sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def unit_stress(key):
+    return [1.0 if k == key else 0.0 for k in STRESS_KEYS]
+
+
+def hooke(e, lam, G):
+    """HookesLaw3DStrain2Stress.jl: sigma = lam tr(e) I + 2 G e, for (exx, eyy, ezz, exy, exz, eyz)."""
+    tr = e[0] + e[1] + e[2]
+    return [lam * tr + 2 * G * e[0], lam * tr + 2 * G * e[1], lam * tr + 2 * G * e[2], 2 * G * e[3], 2 * G * e[4],
+            2 * G * e[5]]
+
+
+def test_oracle_traction_free_surface():
+    from oracle import friction as F
+    G, nu, lam = _elastic(5000.0, 0.25)
+    worst = 0.0
+    worst_f = 0.0
+    for P, T, keys in square_plate_cases():
+        P1, P2, P3 = (P[T[:, k] - 1, 1:4].copy() for k in range(3))
+        n = P1.shape[0]
+        nrm, mid = O.face_normal_midpoint(P1, P2, P3)
+        A = O.assemble_A(O.traction_influence(nrm, mid, P1, P2, P3, nu, G, lam, 0))
+        for key in keys:
+            remote = unit_stress(key)
+            Tn, Tds, Tss = O.normal_shear_tractions(remote, nrm)
+            x = np.linalg.solve(A, np.concatenate([Tn, Tss, Tds]))
+            Dn, Dss, Dds = x[:n], x[n:2 * n], x[2 * n:]
+            e, _ = O.td_field(mid[:, 0], mid[:, 1], mid[:, 2], P1, P2, P3, Dss, Dds, Dn, nu, G, 0, 1, 0)
+            s = [a + b for a, b in zip(hooke(e, lam, G), remote)]
+            tn, tds, tss = O.normal_shear_tractions(s, nrm)
+            worst = max(worst, float(np.max(np.abs(np.concatenate([tn, tds, tss])))))
+            # the same through the frictional solver with mu = 0, Sf = 0 (TractionFreeSurface.jl:136-140)
+            z = np.zeros(n)
+            Dn, Dss, Dds, _ = F.slip_friction(P1, P2, P3, nu, G, lam, mid, nrm, 0, remote, (z, z, z), 0.0, 0.0)
+            e, _ = O.td_field(mid[:, 0], mid[:, 1], mid[:, 2], P1, P2, P3, Dss, Dds, Dn, nu, G, 0, 1, 0)
+            s = [a + b for a, b in zip(hooke(e, lam, G), remote)]
+            tn, tds, tss = O.normal_shear_tractions(s, nrm)
+            worst_f = max(worst_f, float(np.max(np.abs(np.concatenate([tn, tds, tss])))))
+    print("traction-free plate, 6 orientations x 3 loads: worst residual traction %.2e, %.2e with the frictional "
+          "solver" % (worst, worst_f))
+    assert worst < 1e-8 and worst_f < 1e-8
+
+
+# ---- test_TDvsSavageGravityValley.jl --------------------------------------------------------------------------------
+# A 1019-triangle valley surface (ValleySurface_ProperEq_1000Faces.ts) cut into a body under gravity (rho g = 1,
+# lithostatic sxx = nu / (1 - nu) z, szz = z, syy = nu (sxx + szz) at each element), full space: after the slip
+# solve, the stresses on a 50 x 46 grid under the valley's centre line (y = 0) match Savage, Powers & Swolfs (1984)
+# with a = 2, b = -1: the mean of each residual (sxx, szz, sxz) must be below 0.15 in magnitude
+# (SavageGravityValley.jl:101-105 tests the signed means).
+
+def valley_setup():
+    P, T = gocad.read_ts("ValleySurface_ProperEq_1000Faces.ts")
+    P1, P2, P3 = gocad.p1p2p3(P, T)
+    return P1, P2, P3
+
+
+def valley_remote(mid, nu):
+    z = mid[:, 2]
+    sxx = (nu / (1 - nu)) * z
+    szz = z.copy()
+    syy = nu * (sxx + szz)
+    zero = np.zeros_like(z)
+    return [sxx, syy, szz, zero, zero, zero]
+
+
+def valley_grid(nu):
+    u, v = np.meshgrid(np.linspace(0, 4, 50), np.linspace(-0.3265, -4, 46))
+    Sxx, Syy, Sxy, X, Y = AN.savage1984_gravity_valley(0.0, 1.0, nu, 2, -1, u, v)
+    return X.ravel() + 50, np.zeros(X.size), Y.ravel(), Sxx.ravel(), Syy.ravel(), Sxy.ravel()
+
+
+def valley_residuals(e, Z, lam, G, nu, Sxx, Syy, Sxy):
+    s = hooke(e, lam, G)
+    sxx = (nu / (1 - nu)) * Z + s[0]
+    szz = Z + s[2]
+    sxz = s[4]
+    out = []
+    for num, an in ((sxx, Sxx), (szz, Syy), (sxz, Sxy)):
+        r = num - an
+        out.append(float(np.mean(r[~np.isnan(r)])))
+    return out
+
+
+def test_savage_closed_form_pins():
+    """The restated solution leaves the valley surface (v = 0) traction-free and tends to the lithostatic state
+    (syy = Y, sxx = nu / (1 - nu) Y) far from the valley."""
+    nu = 0.25
+    u = np.linspace(-6, 6, 25)
+    for tect in (0.0, 1.0):
+        Sxx, Syy, Sxy, X, Y = AN.savage1984_gravity_valley(tect, 1.0, nu, 2, -1, u, np.zeros_like(u))
+        w = u - 2j
+        t = (w ** 2 + 2) / w ** 2                        # dz/dw on v = 0: the surface tangent
+        n = np.stack([-t.imag, t.real], 1) / np.abs(t)[:, None]
+        assert np.max(np.abs(Sxx * n[:, 0] + Sxy * n[:, 1])) < 1e-10
+        assert np.max(np.abs(Sxy * n[:, 0] + Syy * n[:, 1])) < 1e-10
+    Sxx, Syy, Sxy, X, Y = AN.savage1984_gravity_valley(0.0, 1.0, nu, 2, -1, np.array([0.0, 40.0]),
+                                                       np.array([-60.0, -40.0]))
+    assert np.allclose(Syy / Y, 1, atol=2e-3) and np.allclose(Sxx / ((nu / (1 - nu)) * Y), 1, atol=2e-3)
+
+
+def test_oracle_savage_gravity_valley():
+    G, nu, lam = _elastic(1.0, 0.25)
+    P1, P2, P3 = valley_setup()
+    n = P1.shape[0]
+    nrm, mid = O.face_normal_midpoint(P1, P2, P3)
+    A = O.assemble_A(O.traction_influence(nrm, mid, P1, P2, P3, nu, G, lam, 0))
+    Tn, Tds, Tss = O.normal_shear_tractions(valley_remote(mid, nu), nrm)
+    x = np.linalg.solve(A, np.concatenate([Tn, Tss, Tds]))
+    Dn, Dss, Dds = x[:n], x[n:2 * n], x[2 * n:]
+    X, Y, Z, Sxx, Syy, Sxy = valley_grid(nu)
+    e, _ = O.td_field(X, Y, Z, P1, P2, P3, Dss, Dds, Dn, nu, G, 0, 1, 0)
+    res = valley_residuals(e, Z, lam, G, nu, Sxx, Syy, Sxy)
+    print("Savage valley (oracle): mean residuals sxx %.4f szz %.4f sxz %.4f" % tuple(res))
+    assert max(abs(r) for r in res) < 0.15  # the reference tests the signed means; |.| is stricter
