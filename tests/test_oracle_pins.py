@@ -12,6 +12,12 @@
 * The half-space solution leaves the free surface traction-free (sigma_xz = sigma_yz = sigma_zz = 0 at z = 0).
 * Face normals and incentre midpoints: the host's CreateFaceNormalAndMidPoint equals the oracle's, and both meet the
   geometry (unit normals orthogonal to the sides, the midpoint equidistant from the three sides).
+* Rigid-motion covariance (full space): rotating and translating triangle and points rotates displacement and
+  strain, for the same global Burgers vector.
+* Far field = Volterra's integral of the Burgers vector against Kelvin's Green's function (Gauss-Legendre on the
+  Duffy map), displacement directly and strain through its differences: an independent pin of the angular-dislocation
+  sums that shares no formula with them.
+* A square triangulated along either diagonal gives the same field under uniform slip (full and half space).
 """
 import numpy as np
 import pytest
@@ -260,3 +266,153 @@ def test_size_mismatch_error():
     with pytest.raises(O.OracleError, match="same dimensions"):
         O.td([0.0], [0.0], [-0.5], [[0, 0, -1.0]], [[1, 0, -1.0]], [[0, 1, -1.0]], [1.0, 2.0], [1.0], [1.0], 0.25, 1.0,
              0, 1, 1)
+
+
+def _tdcs_axes(P1, P2, P3):
+    """Vnorm, Vstrike, Vdip of a triangle (the frame of TD.jl:712-732): the slip modes Dn, Dss, Dds act along them."""
+    vn = np.cross(P2 - P1, P3 - P1)
+    vn /= np.linalg.norm(vn)
+    vs = np.cross([0.0, 0.0, 1.0], vn)
+    if np.linalg.norm(vs) == 0.0:
+        vs = np.array([0.0, vn[2], 0.0])
+    vs /= np.linalg.norm(vs)
+    return vn, vs, np.cross(vn, vs)
+
+
+def _unit_modes(X, P1, P2, P3, hs=0):
+    """Displacement (3 x 3: mode, component) and strain (3 x 6) of unit Dn, Dss, Dds at each point of X (m x 3)."""
+    s, d = O.td(X[:, 0], X[:, 1], X[:, 2], P1[None], P2[None], P3[None], [1.0], [1.0], [1.0], 0.25, 1.0, 1, 1, hs)
+    u = np.stack([np.stack([d[md + "U" + c][:, 0] for c in "xyz"], -1) for md in ("Dn", "Dss", "Dds")], 1)
+    e = np.stack([np.stack([s[k + md][:, 0] for k in ("exx", "eyy", "ezz", "exy", "exz", "eyz")], -1)
+                  for md in ("Dn", "Dss", "Dds")], 1)
+    return u, e
+
+
+def _sym(e6):
+    exx, eyy, ezz, exy, exz, eyz = e6
+    return np.array([[exx, exy, exz], [exy, eyy, eyz], [exz, eyz, ezz]])
+
+
+def test_fullspace_rigid_motion_covariance():
+    """Rotating and translating triangle and points rotates the fields: u'(R x + t) = R u(x) and
+    e' = R e R^T for the same global Burgers vector (each mode's vector R b expressed in the moved triangle's own
+    frame, whose strike and dip directions are not R times the old ones)."""
+    rng = np.random.default_rng(21)
+    P1, P2, P3 = np.array([-0.7, -0.5, 0.2]), np.array([0.9, -0.3, -0.1]), np.array([0.1, 0.8, 0.4])
+    X = rng.uniform(-2, 2, (30, 3))
+    u, e = _unit_modes(X, P1, P2, P3)
+    axes = _tdcs_axes(P1, P2, P3)
+    worst = 0.0
+    for _ in range(4):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        a, b, c, d = q
+        R = np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                      [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                      [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+        t = rng.uniform(-5, 5, 3)
+        Q1, Q2, Q3 = (R @ P + t for P in (P1, P2, P3))
+        u2, e2 = _unit_modes(X @ R.T + t, Q1, Q2, Q3)
+        axes2 = _tdcs_axes(Q1, Q2, Q3)
+        for m in range(3):
+            w = np.array([np.dot(R @ axes[m], ax) for ax in axes2])  # R b_m in the moved frame
+            um = np.einsum("k,pkc->pc", w, u2)
+            em = np.einsum("k,pkc->pc", w, e2)
+            for p in range(X.shape[0]):
+                ru = R @ u[p, m]
+                re = R @ _sym(e[p, m]) @ R.T
+                worst = max(worst, np.max(np.abs(um[p] - ru)) / np.max(np.abs(u[:, m])),
+                            np.max(np.abs(_sym(em[p]) - re)) / np.max(np.abs(e[:, m])))
+    assert worst <= 1e-12, worst
+
+
+def _volterra_disp(X, P1, P2, P3, du, nu, mu=1.0, ngl=16):
+    """Volterra's formula u_k(x) = -int m_pq dG_kp/dx_q dS over the triangle, m = lam (du.n) I + mu (du n + n du),
+    with Kelvin's full-space Green's function G_kp = [(3 - 4 nu) d_kp / r + r_k r_p / r^3] / (16 pi mu (1 - nu));
+    Gauss-Legendre on the Duffy map of the triangle (ngl^2 points)."""
+    lam = 2 * mu * nu / (1 - 2 * nu)
+    n = np.cross(P2 - P1, P3 - P1)
+    area2 = np.linalg.norm(n)
+    n /= area2
+    m = lam * np.dot(du, n) * np.eye(3) + mu * (np.outer(du, n) + np.outer(n, du))
+    g, w = np.polynomial.legendre.leggauss(ngl)
+    g, w = 0.5 * (g + 1), 0.5 * w
+    C = 1.0 / (16 * np.pi * mu * (1 - nu))
+    out = np.zeros((X.shape[0], 3))
+    for s, ws in zip(g, w):
+        for t, wt in zip(g, w):
+            xi = P1 + s * (P2 - P1) + s * t * (P3 - P2)
+            jac = area2 * s * ws * wt
+            r = X - xi
+            rr = np.linalg.norm(r, axis=1)
+            # dG_kp/dx_q contracted with m_pq
+            # d/dx_q (d_kp / r) = -d_kp r_q / r^3 and
+            # d/dx_q (r_k r_p / r^3) = (d_kq r_p + d_pq r_k) / r^3 - 3 r_k r_p r_q / r^5, contracted with m_pq
+            mr = r @ m.T
+            rmr = np.sum(r * mr, axis=1)
+            r3, r5 = rr[:, None] ** 3, rr[:, None] ** 5
+            out -= C * jac * (-(3 - 4 * nu) * mr / r3 + (r @ m + np.trace(m) * r) / r3 - 3 * r * rmr[:, None] / r5)
+    return out
+
+
+def test_far_field_matches_volterra_quadrature():
+    """Full space, points 3 to 12 side lengths away: each mode's displacement equals Volterra's integral of its own
+    displacement jump (the unit axis, signed as measured across the triangle's interior) against Kelvin's Green's
+    function, to 1e-10, and its strain the integral's symmetric gradient (fourth-order central differences)
+    to 1e-8.  An independent pin of the full-space angular-dislocation sums (TD.jl:735-867, 1528-1750)."""
+    nu = 0.25
+    P1, P2, P3 = np.array([-0.7, -0.5, 0.2]), np.array([0.9, -0.3, -0.1]), np.array([0.1, 0.8, 0.4])
+    axes = _tdcs_axes(P1, P2, P3)
+    cen = (P1 + P2 + P3) / 3
+    rng = np.random.default_rng(5)
+    dirs = rng.normal(size=(12, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    X = cen + dirs * rng.uniform(5, 20, (12, 1))
+    u, e = _unit_modes(X, P1, P2, P3)
+    # the displacement jump of each unit mode across the interior
+    h = 1e-9
+    Xp = np.array([cen + h * axes[0], cen - h * axes[0]])
+    uj, _ = _unit_modes(Xp, P1, P2, P3)
+    for m in range(3):
+        jump = uj[0, m] - uj[1, m]
+        du = np.sign(np.dot(jump, axes[m])) * axes[m]
+        assert np.max(np.abs(jump - du)) <= 1e-7
+        uv = _volterra_disp(X, P1, P2, P3, du, nu)
+        assert np.max(np.abs(uv - u[:, m])) <= 1e-10 * np.max(np.abs(u[:, m]))
+        for p in range(X.shape[0]):
+            d = 1e-3 * np.linalg.norm(X[p] - cen)
+            grad = np.zeros((3, 3))
+            for q in range(3):
+                ev = np.zeros(3)
+                ev[q] = d
+                pts = np.array([X[p] + 2 * ev, X[p] + ev, X[p] - ev, X[p] - 2 * ev])
+                v = _volterra_disp(pts, P1, P2, P3, du, nu)
+                grad[:, q] = (-v[0] + 8 * v[1] - 8 * v[2] + v[3]) / (12 * d)
+            es = 0.5 * (grad + grad.T)
+            assert np.max(np.abs(es - _sym(e[p, m]))) <= 1e-8 * np.max(np.abs(e[:, m]))
+
+
+@pytest.mark.parametrize("hs", [0, 1])
+def test_two_triangulations_of_a_square_agree(hs):
+    """A square split along either diagonal, uniform slip: the summed fields agree away from its plane."""
+    c = np.array([0.3, -0.2, -2.0])
+    a, b = np.array([1.0, 0.2, 0.3]), np.array([-0.1, 0.8, 0.4])
+    b = b - np.dot(a, b) / np.dot(a, a) * a
+    V = [c, c + a, c + a + b, c + b]
+    T1 = [(V[0], V[1], V[2]), (V[0], V[2], V[3])]
+    T2 = [(V[0], V[1], V[3]), (V[1], V[2], V[3])]
+    rng = np.random.default_rng(9)
+    X = rng.uniform(-2, 2, (40, 3)) + c
+    X[:, 2] = -np.abs(X[:, 2] - c[2]) - 0.3 if hs else X[:, 2]
+    nrm = np.cross(a, b)
+    nrm /= np.linalg.norm(nrm)
+    X = X[np.abs((X - c) @ nrm) > 0.05]
+    res = []
+    for tris in (T1, T2):
+        P1, P2, P3 = (np.array([t[k] for t in tris]) for k in range(3))
+        s, d = O.td_field(X[:, 0], X[:, 1], X[:, 2], P1, P2, P3, [0.4, 0.4], [-0.9, -0.9], [0.3, 0.3], 0.25, 1.0, 1,
+                          1, hs)
+        res.append((np.array(s), np.array(d)))
+    for k in range(2):
+        ref = res[0][k]
+        assert np.max(np.abs(res[1][k] - ref)) <= 1e-12 * np.max(np.abs(ref))
