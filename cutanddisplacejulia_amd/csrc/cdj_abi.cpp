@@ -178,7 +178,7 @@ int traction_hs_fp64(cdj_ctx* ctx, const double* recv, int64_t nrecv, const doub
   for (int64_t r0 = 0; r0 < nrecv; r0 += chunk) {
     const int64_t m = std::min<int64_t>(chunk, nrecv - r0);
     if (timed && one) HIPCHK(hipEventRecord(ctx->ev2, st));
-    launch_edge_response(recv + r0 * RECV, m, erec, E, 4, nu, lam, G, Tb, m, st);
+    launch_edge_response(recv + r0 * RECV, m, erec, E, nu, lam, G, Tb, m, st);
     if (timed && one) HIPCHK(hipEventRecord(ctx->ev3, st));
     OutPtrs<double> o = out;
     for (int k = 0; k < 9; ++k) o.p[k] = out.p[k] + r0;

@@ -514,13 +514,14 @@ template <class T> __device__ __forceinline__ T qnan() { return __builtin_nan(""
 template <> __device__ __forceinline__ float qnan<float>() { return __builtin_nanf(""); }
 
 // Free-surface correction of one side at one point, strain (AngSetupStrainFSC TD.jl:1935-2075), before the
-// Burgers vector enters: w[k][c] = sum over the side's two angular dislocations (B minus A) of
-// {d1u1, d2u2, d3u3, d2u1+d1u2, d3u1+d1u3, d3u2+d2u3} of the FSC displacement for ADCS Burgers component k (the
+// Burgers vector enters, for ADCS Burgers component K only: w[c] = sum over the side's two angular dislocations
+// (B minus A) of {d1u1, d2u2, d3u3, d2u1+d1u2, d3u1+d1u3, d3u2+d2u3} of the FSC displacement for component K (the
 // strain-FSC closed form without its 1/(4 pi (1-nu))); sg = -1 in configuration I (beta x >= 0), whose v13, v23
 // and first two Burgers components are negated (TD.jl:1951-1962), +1 in configuration II.  The side must not be
-// a skipped one.
-template <class T>
-__device__ __forceinline__ void fsc_side_w(const T* __restrict__ s, T X, T Y, T Z, T nu, T w[3][6], T& sg) {
+// a skipped one.  One component needs three of the nine displacement outputs and their gradients; the quantities
+// they share (FscBase) are formed per call.
+template <int K, class T>
+__device__ __forceinline__ void fsc_side_wk(const T* __restrict__ s, T X, T Y, T Z, T nu, T w[6], T& sg) {
   T x, y, z;
   {
 #pragma clang fp contract(off)
@@ -538,9 +539,7 @@ __device__ __forceinline__ void fsc_side_w(const T* __restrict__ s, T X, T Y, T 
   const T ctB2 = cfg1 ? s[S_COTPS2] : s[S_COTB2];
   const T icB = T(1) / cB;
 #pragma unroll
-  for (int k = 0; k < 3; ++k)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) w[k][c] = T(0);
+  for (int c = 0; c < 6; ++c) w[c] = T(0);
 #pragma unroll 1
   for (int ad = 0; ad < 2; ++ad) {
     const T px = ad == 0 ? x : x - s[S_YB + 0];
@@ -551,37 +550,32 @@ __device__ __forceinline__ void fsc_side_w(const T* __restrict__ s, T X, T Y, T 
     FscBase<T, D3<T>> fb;
     fsc_base<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
                        D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
-    // one output at a time: output q = 3 c + k is component c of the displacement for Burgers component k;
-    // its gradient feeds v_cc (diagonal), and half of the two shear sums it appears in
-#pragma unroll 1
-    for (int q = 0; q < 9; ++q) {
-      const D3<T> u = fsc_out<T, D3<T>>(fb, q);
-      const int c = q / 3, k = q % 3;
-      const T g0 = sgn * u.a, g1 = sgn * u.b, g2 = sgn * u.c;
+    // output 3 c + K is component c of the displacement for Burgers component K; its gradient feeds v_cc
+    // (diagonal) and the two shear sums it appears in
 #pragma unroll
-      for (int kk = 0; kk < 3; ++kk) {
-        if (kk != k) continue;
-        if (c == 0) {
-          w[kk][0] += g0;
-          w[kk][3] += g1;
-          w[kk][4] += sg * g2;
-        } else if (c == 1) {
-          w[kk][3] += g0;
-          w[kk][1] += g1;
-          w[kk][5] += sg * g2;
-        } else {
-          w[kk][4] += sg * g0;
-          w[kk][5] += sg * g1;
-          w[kk][2] += g2;
-        }
+    for (int c = 0; c < 3; ++c) {
+      const D3<T> u = fsc_out<T, D3<T>>(fb, 3 * c + K);
+      const T g0 = sgn * u.a, g1 = sgn * u.b, g2 = sgn * u.c;
+      if (c == 0) {
+        w[0] += g0;
+        w[3] += g1;
+        w[4] += sg * g2;
+      } else if (c == 1) {
+        w[3] += g0;
+        w[1] += g1;
+        w[5] += sg * g2;
+      } else {
+        w[4] += sg * g0;
+        w[5] += sg * g1;
+        w[2] += g2;
       }
     }
   }
 }
 
 // Free-surface correction of one side at one point, strain, added in EFCS (AngSetupStrainFSC TD.jl:1935-2075).
-// The loop is the one of fsc_side_w, kept inline here: sharing it changed the register allocation of the fused
-// kernels for the worse (more spill).
+// The same evaluation as fsc_side_wk for all three components at once, sharing FscBase; kept inline here, since
+// sharing the loop changed the register allocation of the fused kernels for the worse (more spill).
 template <class T>
 __device__ __forceinline__ void fsc_strain_side(const T* __restrict__ s, T X, T Y, T Z, T nu, T eps[3][6]) {
   if (s[S_SKIP] != T(0)) return;  // triangle-uniform

@@ -587,44 +587,81 @@ __global__ void __launch_bounds__(128) tdk_edge_setup(const double* __restrict__
   build_side(a, b, rec + (e % n) * REC_HS, erec + t * R_SIDE_SIZE);
 }
 
-// Responses of edges [t0, t0 + tpe) at receiver i: Tb[(t * 9 + 3 kE + q) * ldT + i] = traction q (Tn, Tss, Tds) on
-// the receiver plane of the edge's free-surface strain for the unit EFCS Burgers vector e_kE.
-__global__ void __launch_bounds__(256) tdk_edge_response(const double* __restrict__ recv, int64_t nrecv,
-                                                         const double* __restrict__ erec, int64_t nedge, int tpe,
-                                                         double nu, double lam, double G, double* __restrict__ Tb,
-                                                         int64_t ldT) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nrecv) return;
-  const double* rv = recv + i * RECV;
+// Edge responses: Tb[(t * 9 + 3 kE + q) * ldT + i] = traction q (Tn, Tss, Tds) on receiver i's plane of edge t's
+// free-surface strain for the unit EFCS Burgers vector e_kE.
+//
+// A workgroup takes 64 receivers (one per lane) and blocks of kEdgeBlock edges.  The work of a block is split into
+// (edge, ADCS Burgers component K) units: a unit evaluates three of the side's nine free-surface displacement
+// outputs and their gradients (fsc_side_wk) and leaves the component's strain w_K in LDS.  After a barrier each
+// (edge, kE) unit combines the three components into e_kE's tractions.  Units are dealt to the four waves in turn,
+// so every wave evaluates each component equally often and the barrier waits on balanced waves.  One component per
+// thread fits two waves per SIMD without accumulator-register copies; all nine outputs at once took ~470 registers
+// per lane (one wave per SIMD, with 40 % of the issued instructions moving values to and from AGPRs).
+constexpr int kEdgeBlock = 8;
+
+__global__ void __launch_bounds__(256, 2) tdk_edge_response(const double* __restrict__ recv, int64_t nrecv,
+                                                            const double* __restrict__ erec, int64_t nedge,
+                                                            int64_t blk0, double nu, double lam, double G,
+                                                            double* __restrict__ Tb, int64_t ldT) {
+  __shared__ double Wl[kEdgeBlock][3][6][64];
+  __shared__ double SGl[kEdgeBlock][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const double* rv = recv + (i < nrecv ? i : nrecv - 1) * RECV;
   const double Xi = rv[0], Yi = rv[1], Zi = rv[2];
-  const int64_t t0 = (int64_t)blockIdx.y * tpe;
-  const int64_t t1 = t0 + tpe < nedge ? t0 + tpe : nedge;
   const double k4 = (1.0 / (4.0 * kPi)) / (1.0 - nu);
-  for (int64_t t = t0; t < t1; ++t) {
-    const double* s = erec + t * R_SIDE_SIZE;
-    double o[9];
-    if (s[S_SKIP] != 0.0) {  // edge-uniform
+  {
+    const int64_t e0 = (blk0 + blockIdx.y) * kEdgeBlock;
+    for (int u = wv; u < 3 * kEdgeBlock; u += 4) {
+      const int eb = u / 3, K = u % 3;
+      const int64_t t = e0 + eb;
+      double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      double sg = 1.0;
+      if (t < nedge) {
+        const double* s = erec + t * R_SIDE_SIZE;
+        if (s[S_SKIP] == 0.0) {  // edge-uniform
+          if (K == 0)
+            fsc_side_wk<0, double>(s, Xi, Yi, Zi, nu, w, sg);
+          else if (K == 1)
+            fsc_side_wk<1, double>(s, Xi, Yi, Zi, nu, w, sg);
+          else
+            fsc_side_wk<2, double>(s, Xi, Yi, Zi, nu, w, sg);
+        }
+      }
 #pragma unroll
-      for (int q = 0; q < 9; ++q) o[q] = 0.0;
-    } else {
-      double w[3][6], sg;
-      fsc_side_w(s, Xi, Yi, Zi, nu, w, sg);
-      const double p = s[S_EY1 + 0], qq = s[S_EY1 + 1];
-      const double c0[3] = {p, qq, 0.0}, c1[3] = {qq, -p, 0.0}, c2[3] = {-0.0, -0.0, -1.0};
-      const double n[3] = {rv[3], rv[4], rv[5]}, sv[3] = {rv[6], rv[7], rv[8]}, d[3] = {rv[9], rv[10], rv[11]};
-#pragma unroll
-      for (int kE = 0; kE < 3; ++kE) {
+      for (int c = 0; c < 6; ++c) Wl[eb][K][c][lane] = w[c];
+      if (K == 0) SGl[eb][lane] = sg;
+    }
+    __syncthreads();
+    // the receiver's frame is read here, not kept live through the evaluation above
+    const double n[3] = {rv[3], rv[4], rv[5]}, sv[3] = {rv[6], rv[7], rv[8]}, d[3] = {rv[9], rv[10], rv[11]};
+    for (int u = wv; u < 3 * kEdgeBlock; u += 4) {
+      const int eb = u / 3, kE = u % 3;
+      const int64_t t = e0 + eb;
+      if (t >= nedge) continue;
+      const double* s = erec + t * R_SIDE_SIZE;
+      double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+      if (s[S_SKIP] == 0.0) {
         // e_kE in the edge's ADCS is (ey1[kE], ey2[kE], ey3[kE]); configuration I negates the first two
-        const double b1 = sg * c0[kE] * k4, b2 = sg * c1[kE] * k4, b3 = c2[kE] * k4;
+        const double p = s[S_EY1 + 0], qq = s[S_EY1 + 1], sg = SGl[eb][lane];
+        const double c0[3] = {p, qq, 0.0}, c1[3] = {qq, -p, 0.0}, c2[3] = {-0.0, -0.0, -1.0};
+        const double b1 = sg * (kE == 0 ? p : (kE == 1 ? qq : 0.0)) * k4;
+        const double b2 = sg * (kE == 0 ? qq : (kE == 1 ? -p : 0.0)) * k4;
+        const double b3 = (kE == 2 ? -1.0 : -0.0) * k4;
         double e[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) e[c] = (c < 3 ? 1.0 : 0.5) * (b1 * w[0][c] + b2 * w[1][c] + b3 * w[2][c]);
+        for (int c = 0; c < 6; ++c)
+          e[c] = (c < 3 ? 1.0 : 0.5) * (b1 * Wl[eb][0][c][lane] + b2 * Wl[eb][1][c][lane] + b3 * Wl[eb][2][c][lane]);
         tensor_rotate(e, c0, c1, c2);
-        traction3(e, lam, G, n, sv, d, o[3 * kE], o[3 * kE + 1], o[3 * kE + 2]);
+        traction3(e, lam, G, n, sv, d, o0, o1, o2);
+      }
+      if (i < nrecv) {
+        double* T = Tb + (t * 9 + 3 * kE) * ldT + i;
+        T[0] = o0;
+        T[ldT] = o1;
+        T[2 * ldT] = o2;
       }
     }
-#pragma unroll
-    for (int q = 0; q < 9; ++q) Tb[(t * 9 + q) * ldT + i] = o[q];
   }
 }
 
@@ -722,11 +759,15 @@ void launch_edge_setup(const double* P1, const double* P2, const double* P3, int
                      e.edge_side, e.count, erec);
 }
 
-void launch_edge_response(const double* recv, int64_t nrecv, const double* erec, int64_t nedge, int tpe, double nu,
+void launch_edge_response(const double* recv, int64_t nrecv, const double* erec, int64_t nedge, double nu,
                           double lam, double G, double* Tb, int64_t ldT, hipStream_t st) {
   if (nrecv <= 0 || nedge <= 0) return;
-  dim3 grid((unsigned)((nrecv + 255) / 256), (unsigned)((nedge + tpe - 1) / tpe));
-  hipLaunchKernelGGL(tdk_edge_response, grid, dim3(256), 0, st, recv, nrecv, erec, nedge, tpe, nu, lam, G, Tb, ldT);
+  const int64_t nblk = (nedge + kEdgeBlock - 1) / kEdgeBlock;
+  for (int64_t b0 = 0; b0 < nblk; b0 += 65535) {
+    const int64_t nb = nblk - b0 < 65535 ? nblk - b0 : 65535;
+    hipLaunchKernelGGL(tdk_edge_response, dim3((unsigned)((nrecv + 63) / 64), (unsigned)nb), dim3(256), 0, st, recv,
+                       nrecv, erec, nedge, b0, nu, lam, G, Tb, ldT);
+  }
 }
 
 void launch_traction_combine(const double* recv, int64_t nrecv, const double* rec, int64_t ntri, int tpb, double nu,

@@ -68,7 +68,7 @@ void launch_edge_setup(const double* P1, const double* P2, const double* P3, int
                        const EdgeTopo& e, double* erec, hipStream_t st);
 // Tb[(t * 9 + 3 kE + q) * ldT + i]: traction q on receiver i's plane of edge t's free-surface strain for the
 // EFCS unit Burgers vector kE
-void launch_edge_response(const double* recv, int64_t nrecv, const double* erec, int64_t nedge, int tpe, double nu,
+void launch_edge_response(const double* recv, int64_t nrecv, const double* erec, int64_t nedge, double nu,
                           double lam, double G, double* Tb, int64_t ldT, hipStream_t st);
 // half-space traction entries from the real and image TDs plus the edges' responses (no triangle in z = 0)
 void launch_traction_combine(const double* recv, int64_t nrecv, const double* rec, int64_t ntri, int tpb, double nu,
