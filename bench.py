@@ -35,22 +35,34 @@ METRIC = "TD influence entries/s; slip-solve time at 1e-10 resid, mixed/learned 
 # runs on the FP64 vector ALUs (the bound is VALU issue, 78.6 TFLOP/s on MI355X); "executed_frac" is the fraction
 # of that peak in the FP64 operations the kernel actually issues (below).
 FLOPS_PER_PAIR_HS = 43940.0
-# Algorithmic work of one (receiver, mesh edge) unit of the dominant kernel, tdk_edge_response: the oracle's
-# operation count of one triangle side's free-surface strain for the three slip modes (AngSetupStrainFSC), averaged
-# over 6000 random (receiver, side) pairs of this workload (tests/test_flops.py recomputes it).  The reference
-# evaluates it once per triangle side and pair; the engine once per mesh edge and receiver (7600 edges for the
-# 15000 sides of this mesh), then each triangle adds +- the responses of its edges (tdk_traction_combine).
-FLOPS_PER_EDGE_HS = 12428.0
+# Work of one (receiver, mesh edge) unit of the dominant kernel, tdk_edge_response: the free-surface strain of the
+# edge's angular-dislocation pair for three unit Burgers vectors, turned into 9 tractions.  The reference evaluates
+# it once per triangle side and pair; the engine once per mesh edge and receiver (7600 edges for the 15000 sides of
+# this mesh), and each triangle then adds +- the responses of its edges (tdk_traction_combine).
+#  * FLOPS_PER_EDGE_HS, the algorithmic work: +,-,*,/ of the engine's formulation (the forward-mode derivative of
+#    the free-surface displacement, every shared quantity formed once), counted by compiling the device header for
+#    the host with a counting scalar that leaves out what the compiler folds (tests/flops/edge_count.cpp;
+#    tests/test_flops.py recomputes it).  2 sqrt, 4 log and 2 atan per unit are not included.
+#  * KERNEL_FLOPS_PER_EDGE_HS: the same count for the kernel as written, which forms the shared quantities once
+#    per Burgers component (three times) so that one component fits two waves per SIMD.
+#  * REF_FLOPS_PER_EDGE_HS: the reference formulation of the same unit (the strain closed form AngDisStrainFSC for
+#    one side and three slip modes), from the op-counting oracle over 6000 random (receiver, side) pairs of this
+#    workload (tests/test_flops.py recomputes it).
+FLOPS_PER_EDGE_HS = 4353.0
+KERNEL_FLOPS_PER_EDGE_HS = 4895.0
+REF_FLOPS_PER_EDGE_HS = 12428.0
 # FP64 flops tdk_edge_response executes per (receiver, edge) on the same workload, from rocprofv3 --pmc
-# SQ_INSTS_VALU_ADD_F64, _MUL_F64, _FMA_F64, _TRANS_F64 (x64 lanes, FMA twice): 1.664e11 per launch over
-# 5000 x 7600 units.
-EXECUTED_FLOPS_PER_EDGE_HS = 4378.5
+# SQ_INSTS_VALU_ADD_F64, _MUL_F64, _FMA_F64, _TRANS_F64 (x64 lanes, FMA twice): 2.232e11 per launch over
+# 5000 x 7600 units.  Above KERNEL_FLOPS_PER_EDGE_HS by the instruction sequences of division, sqrt, log and atan
+# and by the 56 idle lanes of the last receiver block.
+EXECUTED_FLOPS_PER_EDGE_HS = 5874.6
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of tdk_edge_response on the 5000-triangle workload: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE in separate runs, each scaled by what the counter reads for a 1 GiB device copy (fetch x2, write x1):
-# 0.025 GB read + 2.736 GB written (its algorithmic 72 B per unit).
-TRAFFIC_BYTES_EDGE_HS_5000 = 2.761e9
+# 0.064 GB read + 3.205 GB written, against its algorithmic 72 B per unit (2.736 GB written); the excess is the
+# write-back of 36 B per lane of register spill.
+TRAFFIC_BYTES_EDGE_HS_5000 = 3.269e9
 
 
 def _log(msg):
@@ -478,6 +490,7 @@ def main():
     pairs = float(n) * n
     units = float(edges.value) * n   # (receiver, edge) units of one edge-response launch
     achieved_tf = FLOPS_PER_EDGE_HS * units / (eavg * 1e-3) / 1e12
+    reference_tf = REF_FLOPS_PER_EDGE_HS * units / (eavg * 1e-3) / 1e12
     formulation_tf = FLOPS_PER_PAIR_HS * pairs / (kavg * 1e-3) / 1e12
     _log("assembly: %.2f ms/step, kernels %.2f ms (edge responses %.2f ms over %d edges), %.3e entries/s; edge "
          "kernel %.1f TFLOP/s algorithmic" % (ms_per_step, kavg, eavg, edges.value, value, achieved_tf))
@@ -573,14 +586,22 @@ def main():
                      "traffic": TRAFFIC_BYTES_EDGE_HS_5000 if args.n == 50 else None,
                      "kernel": "tdk_edge_response", "kernel_ms": eavg, "flops_per_unit": FLOPS_PER_EDGE_HS,
                      "unit_of_work": "(receiver, mesh edge)", "units_per_launch": units,
+                     "kernel_flops_per_unit": KERNEL_FLOPS_PER_EDGE_HS,
                      "executed_flops_per_unit": EXECUTED_FLOPS_PER_EDGE_HS if args.n == 50 else None,
                      "executed_frac": (EXECUTED_FLOPS_PER_EDGE_HS * units / (eavg * 1e-3) / 1e12 / PEAK_FP64_TFLOPS
                                        if args.n == 50 and EXECUTED_FLOPS_PER_EDGE_HS else None),
+                     "reference_flops_per_unit": REF_FLOPS_PER_EDGE_HS,
+                     "reference_formulation_tflops": reference_tf,
                      "assembly_kernels_ms": kavg,
                      "assembly_formulation_tflops": formulation_tf,
-                     "note": "assembly_formulation_tflops counts the reference formulation's %g flops per pair over "
-                             "the whole assembly; it exceeds the peak because each mesh edge's free-surface part "
-                             "is evaluated once, not once per triangle side" % FLOPS_PER_PAIR_HS},
+                     "note": "flops_per_unit counts the engine's forward-mode formulation with every shared quantity "
+                             "formed once; the kernel forms them once per Burgers component (kernel_flops_per_unit), "
+                             "and executes executed_flops_per_unit (PMC, with the instruction sequences of division "
+                             "and transcendentals).  reference_formulation_tflops and assembly_formulation_tflops "
+                             "count the reference's closed forms (%g flops per edge unit, %g per pair) and exceed "
+                             "the peak: the forward-mode strain is cheaper, and each mesh edge's free-surface part "
+                             "is evaluated once, not once per triangle side" % (REF_FLOPS_PER_EDGE_HS,
+                                                                                FLOPS_PER_PAIR_HS)},
         "cpu_baseline": cpu,
         "parity_vs_oracle_rows": parity,
         "lu_15000": lu15,

@@ -290,25 +290,37 @@ __device__ __forceinline__ void fsc_base(S y1, S y2, S y3, T cosB, T sinB, T cot
   f.Fib = burgers2(y2, y1 - W6 * cotB2);
 }
 
-template <class T, class S> __device__ __forceinline__ S fsc_out(const FscBase<T, S>& f, int o) {
+// Products and sums of FscBase members that several outputs use (no division or transcendental among them).
+template <class T, class S> struct FscDer {
+  S z1b, irb2, irb3, aR, y3ma, rcy, rsy, cpa, y22, y11, q6, q7, p1;
+};
+
+template <class T, class S> __device__ __forceinline__ FscDer<T, S> fsc_der(const FscBase<T, S>& f) {
+  FscDer<T, S> d;
+  d.z1b = f.y1 * f.cosB + f.y3b * f.sinB;
+  d.irb2 = f.irb * f.irb;
+  d.irb3 = d.irb2 * f.irb;
+  d.aR = f.a * f.irb;
+  d.y3ma = f.y3b - f.a;
+  d.rcy = f.rb * f.cosB + f.y3b;
+  d.rsy = f.rb * f.sinB - f.y1;
+  d.cpa = f.cosB + d.aR;
+  d.y22 = f.y2 * f.y2;
+  d.y11 = f.y1 * f.y1;
+  d.q6 = d.y3ma * f.iW6;
+  d.q7 = d.y3ma * f.iW7;
+  d.p1 = T(1) + d.aR * f.icosB;
+  return d;
+}
+
+template <class T, class S>
+__device__ __forceinline__ S fsc_out_d(const FscBase<T, S>& f, const FscDer<T, S>& dq, int o) {
   const T cosB = f.cosB, sinB = f.sinB, cotB = f.cotB, icosB = f.icosB, nu = f.nu, a = f.a, c1 = f.c1, c2 = f.c2,
           nu2 = f.nu2, cot2 = f.cot2;
-  const S &y1 = f.y1, &y2 = f.y2, &y3b = f.y3b, &rb = f.rb, &irb = f.irb, &iW6 = f.iW6, &iW7 = f.iW7,
-          &lW6 = f.lW6, &lW7 = f.lW7, &Fib = f.Fib;
-  // cheap derived quantities are recomputed per output to keep the live set small
-  const S z1b = y1 * cosB + y3b * sinB;
-  const S irb2 = irb * irb;
-  const S irb3 = irb2 * irb;
-  const S aR = a * irb;
-  const S y3ma = y3b - a;
-  const S rcy = rb * cosB + y3b;
-  const S rsy = rb * sinB - y1;
-  const S cpa = cosB + aR;
-  const S y22 = y2 * y2;
-  const S y11 = y1 * y1;
-  const S q6 = y3ma * iW6;
-  const S q7 = y3ma * iW7;
-  const S p1 = T(1) + aR * icosB;
+  const S &y1 = f.y1, &y2 = f.y2, &y3b = f.y3b, &irb = f.irb, &iW6 = f.iW6, &iW7 = f.iW7, &lW6 = f.lW6,
+          &lW7 = f.lW7, &Fib = f.Fib;
+  const S &z1b = dq.z1b, &irb2 = dq.irb2, &irb3 = dq.irb3, &aR = dq.aR, &y3ma = dq.y3ma, &rcy = dq.rcy,
+          &rsy = dq.rsy, &cpa = dq.cpa, &y22 = dq.y22, &y11 = dq.y11, &q6 = dq.q6, &q7 = dq.q7, &p1 = dq.p1;
   switch (o) {
     case 0:  // ux
       return -T(2) * c2 * c1 * cot2 * Fib + c1 * y2 * iW6 * ((c1 - aR) * cotB - y1 * iW6 * (nu + aR)) +
@@ -353,6 +365,12 @@ template <class T, class S> __device__ __forceinline__ S fsc_out(const FscBase<T
       return T(2) * c2 * Fib + T(2) * c2 * sinB * y2 * iW7 * cpa +
              sinB * y2 * y3ma * irb * iW7 * (T(1) + rcy * iW7 * cpa + a * y3b * irb2);
   }
+}
+
+// One output with its derived quantities formed on the spot (callers that evaluate outputs one at a time keep
+// the live set small this way).
+template <class T, class S> __device__ __forceinline__ S fsc_out(const FscBase<T, S>& f, int o) {
+  return fsc_out_d<T, S>(f, fsc_der<T, S>(f), o);
 }
 
 // All nine outputs (order {ux,uy,uz, vx,vy,vz, wx,wy,wz}).
@@ -519,7 +537,7 @@ template <> __device__ __forceinline__ float qnan<float>() { return __builtin_na
 // strain-FSC closed form without its 1/(4 pi (1-nu))); sg = -1 in configuration I (beta x >= 0), whose v13, v23
 // and first two Burgers components are negated (TD.jl:1951-1962), +1 in configuration II.  The side must not be
 // a skipped one.  One component needs three of the nine displacement outputs and their gradients; the quantities
-// they share (FscBase) are formed per call.
+// they share (FscBase, FscDer) are formed per call.
 template <int K, class T>
 __device__ __forceinline__ void fsc_side_wk(const T* __restrict__ s, T X, T Y, T Z, T nu, T w[6], T& sg) {
   T x, y, z;
@@ -550,11 +568,12 @@ __device__ __forceinline__ void fsc_side_wk(const T* __restrict__ s, T X, T Y, T
     FscBase<T, D3<T>> fb;
     fsc_base<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
                        D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
+    const FscDer<T, D3<T>> dq = fsc_der<T, D3<T>>(fb);
     // output 3 c + K is component c of the displacement for Burgers component K; its gradient feeds v_cc
     // (diagonal) and the two shear sums it appears in
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const D3<T> u = fsc_out<T, D3<T>>(fb, 3 * c + K);
+      const D3<T> u = fsc_out_d<T, D3<T>>(fb, dq, 3 * c + K);
       const T g0 = sgn * u.a, g1 = sgn * u.b, g2 = sgn * u.c;
       if (c == 0) {
         w[0] += g0;
@@ -573,7 +592,6 @@ __device__ __forceinline__ void fsc_side_wk(const T* __restrict__ s, T X, T Y, T
   }
 }
 
-// Free-surface correction of one side at one point, strain, added in EFCS (AngSetupStrainFSC TD.jl:1935-2075).
 // The same evaluation as fsc_side_wk for all three components at once, sharing FscBase; kept inline here, since
 // sharing the loop changed the register allocation of the fused kernels for the worse (more spill).
 template <class T>
@@ -874,6 +892,26 @@ __device__ __forceinline__ void traction3(const T* e, T lam, T G, const T* n, co
   const T pz = -tx * (nz * nx) - ty * (ny * nz) + tz * (T(1) - nz * nz);
   Tss = px * s[0] + py * s[1] + pz * s[2];
   Tds = px * d[0] + py * d[1] + pz * d[2];
+}
+
+// Tractions (Tn, Tss, Tds) on the plane with normal n and strike/dip directions sv, d of one side's free-surface
+// strain for the unit EFCS Burgers vector e_kE, from the side's component strains w_K[c * ws] (fsc_side_wk) and its
+// configuration sign sg; k4 = 1/(4 pi (1-nu)).  e_kE in the side's ADCS is (ey1[kE], ey2[kE], ey3[kE]), and
+// configuration I negates its first two components.
+template <class T>
+__device__ __forceinline__ void edge_unit_traction(const T* __restrict__ s, int kE, T sg, const T* w0, const T* w1,
+                                                   const T* w2, int ws, T k4, T lam, T G, const T* n, const T* sv,
+                                                   const T* d, T& o0, T& o1, T& o2) {
+  const T p = s[S_EY1 + 0], qq = s[S_EY1 + 1];
+  const T c0[3] = {p, qq, T(0)}, c1[3] = {qq, -p, T(0)}, c2[3] = {T(-0.0), T(-0.0), T(-1)};
+  const T b1 = sg * (kE == 0 ? p : (kE == 1 ? qq : T(0))) * k4;
+  const T b2 = sg * (kE == 0 ? qq : (kE == 1 ? -p : T(0))) * k4;
+  const T b3 = (kE == 2 ? T(-1) : T(-0.0)) * k4;
+  T e[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) e[c] = (c < 3 ? T(1) : T(0.5)) * (b1 * w0[c * ws] + b2 * w1[c * ws] + b3 * w2[c * ws]);
+  tensor_rotate(e, c0, c1, c2);
+  traction3(e, lam, G, n, sv, d, o0, o1, o2);
 }
 
 // One TD record's full-space strain (real, or image with the image record) at one point turned into the three

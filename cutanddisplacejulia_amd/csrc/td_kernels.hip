@@ -641,20 +641,9 @@ __global__ void __launch_bounds__(256, 2) tdk_edge_response(const double* __rest
       if (t >= nedge) continue;
       const double* s = erec + t * R_SIDE_SIZE;
       double o0 = 0.0, o1 = 0.0, o2 = 0.0;
-      if (s[S_SKIP] == 0.0) {
-        // e_kE in the edge's ADCS is (ey1[kE], ey2[kE], ey3[kE]); configuration I negates the first two
-        const double p = s[S_EY1 + 0], qq = s[S_EY1 + 1], sg = SGl[eb][lane];
-        const double c0[3] = {p, qq, 0.0}, c1[3] = {qq, -p, 0.0}, c2[3] = {-0.0, -0.0, -1.0};
-        const double b1 = sg * (kE == 0 ? p : (kE == 1 ? qq : 0.0)) * k4;
-        const double b2 = sg * (kE == 0 ? qq : (kE == 1 ? -p : 0.0)) * k4;
-        const double b3 = (kE == 2 ? -1.0 : -0.0) * k4;
-        double e[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-          e[c] = (c < 3 ? 1.0 : 0.5) * (b1 * Wl[eb][0][c][lane] + b2 * Wl[eb][1][c][lane] + b3 * Wl[eb][2][c][lane]);
-        tensor_rotate(e, c0, c1, c2);
-        traction3(e, lam, G, n, sv, d, o0, o1, o2);
-      }
+      if (s[S_SKIP] == 0.0)
+        edge_unit_traction(s, kE, SGl[eb][lane], &Wl[eb][0][0][lane], &Wl[eb][1][0][lane], &Wl[eb][2][0][lane], 64,
+                           k4, lam, G, n, sv, d, o0, o1, o2);
       if (i < nrecv) {
         double* T = Tb + (t * 9 + 3 * kE) * ldT + i;
         T[0] = o0;
