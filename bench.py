@@ -48,8 +48,8 @@ FLOPS_PER_PAIR_HS = 43940.0
 #  * REF_FLOPS_PER_EDGE_HS: the reference formulation of the same unit (the strain closed form AngDisStrainFSC for
 #    one side and three slip modes), from the op-counting oracle over 6000 random (receiver, side) pairs of this
 #    workload (tests/test_flops.py recomputes it).
-FLOPS_PER_EDGE_HS = 4353.0
-KERNEL_FLOPS_PER_EDGE_HS = 4895.0
+FLOPS_PER_EDGE_HS = 4279.0
+KERNEL_FLOPS_PER_EDGE_HS = 4825.0
 REF_FLOPS_PER_EDGE_HS = 12428.0
 # FP64 flops tdk_edge_response executes per (receiver, edge) on the same workload, from rocprofv3 --pmc
 # SQ_INSTS_VALU_ADD_F64, _MUL_F64, _FMA_F64, _TRANS_F64 (x64 lanes, FMA twice): 2.232e11 per launch over

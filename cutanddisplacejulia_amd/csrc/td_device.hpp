@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "td_kernels.hpp"
 
@@ -87,52 +88,170 @@ static constexpr int OFF_IMG = R_TD_SIZE;
 static constexpr int OFF_SIDE = 2 * R_TD_SIZE;
 
 // ---- forward-mode dual number: value + d/dy1, d/dy2, d/dy3 ---------------------------------------------
-template <class T> struct D3 {
+// DS<T, M> carries the derivative directions in the bit set M (bit k: d/dy(k+1)); the others are zero by
+// construction and are neither read nor computed.  The free-surface code starts from the coordinates y1, y2, y3
+// (one direction each) and its first products keep a few directions only: with full duals their zero parts cost
+// real instructions, because IEEE arithmetic does not let the compiler fold x * 0 (x may be infinite or NaN) or
+// x + 0 (x may be -0).  D3<T> is the dense case.
+template <class T, int M> struct DS {
   T v, a, b, c;
 };
-template <class T> __device__ __forceinline__ D3<T> mk(T v) { return {v, T(0), T(0), T(0)}; }
-template <class T> __device__ __forceinline__ D3<T> operator+(D3<T> x, D3<T> y) { return {x.v + y.v, x.a + y.a, x.b + y.b, x.c + y.c}; }
-template <class T> __device__ __forceinline__ D3<T> operator-(D3<T> x, D3<T> y) { return {x.v - y.v, x.a - y.a, x.b - y.b, x.c - y.c}; }
-template <class T> __device__ __forceinline__ D3<T> operator-(D3<T> x) { return {-x.v, -x.a, -x.b, -x.c}; }
-template <class T> __device__ __forceinline__ D3<T> operator*(D3<T> x, D3<T> y) {
-  return {x.v * y.v, x.a * y.v + x.v * y.a, x.b * y.v + x.v * y.b, x.c * y.v + x.v * y.c};
+template <class T> using D3 = DS<T, 7>;
+
+template <int k, class T, int M> __device__ __forceinline__ T dcomp(const DS<T, M>& x) {
+  return k == 0 ? x.a : (k == 1 ? x.b : x.c);
 }
-template <class T> __device__ __forceinline__ D3<T> operator+(D3<T> x, T s) { return {x.v + s, x.a, x.b, x.c}; }
-template <class T> __device__ __forceinline__ D3<T> operator+(T s, D3<T> x) { return {x.v + s, x.a, x.b, x.c}; }
-template <class T> __device__ __forceinline__ D3<T> operator-(D3<T> x, T s) { return {x.v - s, x.a, x.b, x.c}; }
-template <class T> __device__ __forceinline__ D3<T> operator-(T s, D3<T> x) { return {s - x.v, -x.a, -x.b, -x.c}; }
-template <class T> __device__ __forceinline__ D3<T> operator*(D3<T> x, T s) { return {x.v * s, x.a * s, x.b * s, x.c * s}; }
-template <class T> __device__ __forceinline__ D3<T> operator*(T s, D3<T> x) { return {x.v * s, x.a * s, x.b * s, x.c * s}; }
-template <class T> __device__ __forceinline__ D3<T> rcp(D3<T> x) {
-  const T r = T(1) / x.v;
+template <int k, int M1, int M2, class T> __device__ __forceinline__ T d_add(T x, T y) {
+  constexpr bool X = (M1 >> k) & 1, Y = (M2 >> k) & 1;
+  if constexpr (X && Y) return x + y;
+  else if constexpr (X) return x;
+  else if constexpr (Y) return y;
+  else return T(0);  // never read
+}
+template <int k, int M1, int M2, class T> __device__ __forceinline__ T d_sub(T x, T y) {
+  constexpr bool X = (M1 >> k) & 1, Y = (M2 >> k) & 1;
+  if constexpr (X && Y) return x - y;
+  else if constexpr (X) return x;
+  else if constexpr (Y) return -y;
+  else return T(0);
+}
+template <int k, int M1, int M2, class T> __device__ __forceinline__ T d_mul(T xv, T xd, T yv, T yd) {
+  constexpr bool X = (M1 >> k) & 1, Y = (M2 >> k) & 1;
+  if constexpr (X && Y) return xd * yv + xv * yd;
+  else if constexpr (X) return xd * yv;
+  else if constexpr (Y) return xv * yd;
+  else return T(0);
+}
+template <class T, int M1, int M2> __device__ __forceinline__ DS<T, M1 | M2> operator+(DS<T, M1> x, DS<T, M2> y) {
+  return {x.v + y.v, d_add<0, M1, M2>(x.a, y.a), d_add<1, M1, M2>(x.b, y.b), d_add<2, M1, M2>(x.c, y.c)};
+}
+template <class T, int M1, int M2> __device__ __forceinline__ DS<T, M1 | M2> operator-(DS<T, M1> x, DS<T, M2> y) {
+  return {x.v - y.v, d_sub<0, M1, M2>(x.a, y.a), d_sub<1, M1, M2>(x.b, y.b), d_sub<2, M1, M2>(x.c, y.c)};
+}
+template <class T, int M1, int M2> __device__ __forceinline__ DS<T, M1 | M2> operator*(DS<T, M1> x, DS<T, M2> y) {
+  return {x.v * y.v, d_mul<0, M1, M2>(x.v, x.a, y.v, y.a), d_mul<1, M1, M2>(x.v, x.b, y.v, y.b),
+          d_mul<2, M1, M2>(x.v, x.c, y.v, y.c)};
+}
+// scaling a present direction; absent ones stay unread
+template <int k, int M, class T> __device__ __forceinline__ T d_scale(T x, T s) {
+  if constexpr ((M >> k) & 1) return x * s;
+  else return T(0);
+}
+template <int k, int M, class T> __device__ __forceinline__ T d_neg(T x) {
+  if constexpr ((M >> k) & 1) return -x;
+  else return T(0);
+}
+template <class T, int M> __device__ __forceinline__ DS<T, M> operator-(DS<T, M> x) {
+  return {-x.v, d_neg<0, M>(x.a), d_neg<1, M>(x.b), d_neg<2, M>(x.c)};
+}
+template <class T, int M> __device__ __forceinline__ DS<T, M> operator+(DS<T, M> x, T s) { return {x.v + s, x.a, x.b, x.c}; }
+template <class T, int M> __device__ __forceinline__ DS<T, M> operator+(T s, DS<T, M> x) { return {x.v + s, x.a, x.b, x.c}; }
+template <class T, int M> __device__ __forceinline__ DS<T, M> operator-(DS<T, M> x, T s) { return {x.v - s, x.a, x.b, x.c}; }
+template <class T, int M> __device__ __forceinline__ DS<T, M> operator-(T s, DS<T, M> x) {
+  return {s - x.v, d_neg<0, M>(x.a), d_neg<1, M>(x.b), d_neg<2, M>(x.c)};
+}
+template <class T, int M> __device__ __forceinline__ DS<T, M> operator*(DS<T, M> x, T s) {
+  return {x.v * s, d_scale<0, M>(x.a, s), d_scale<1, M>(x.b, s), d_scale<2, M>(x.c, s)};
+}
+template <class T, int M> __device__ __forceinline__ DS<T, M> operator*(T s, DS<T, M> x) {
+  return {x.v * s, d_scale<0, M>(x.a, s), d_scale<1, M>(x.b, s), d_scale<2, M>(x.c, s)};
+}
+// the dense form of a sparse dual (absent directions become explicit zeros)
+template <class T, int M> __device__ __forceinline__ D3<T> dense(DS<T, M> x) {
+  return {x.v, ((M >> 0) & 1) ? x.a : T(0), ((M >> 1) & 1) ? x.b : T(0), ((M >> 2) & 1) ? x.c : T(0)};
+}
+template <class T> __device__ __forceinline__ T dense(T x) { return x; }
+
+// Reciprocals and square roots on the derivative path.  In FP64 an IEEE quotient is a sequence of about ten
+// instructions (operand scaling, refinement, fix-up) and a square root about seventeen.  The forward-mode
+// free-surface code takes them of finite, nonzero values in the normal range only (a zero or infinite operand is a
+// point on a singular line of the closed form), so in FP64 it starts from the hardware's approximations and refines
+// them: two Newton steps r <- r + r (1 - x r) for 1/x, and for sqrt(q) two steps of Goldschmidt's coupled iteration
+// (g -> sqrt(q), h -> 1/(2 sqrt(q))) with a final Newton correction of g.  Both land within an ulp or two of the
+// correctly rounded result.  Other scalar types divide.
+template <class T> __device__ __forceinline__ T rcp_nr(T x) {
+  if constexpr (std::is_same<T, double>::value) {
+    T r = __builtin_amdgcn_rcp(x);
+    T e = fma(-x, r, T(1));
+    r = fma(r, e, r);
+    e = fma(-x, r, T(1));
+    return fma(r, e, r);
+  } else {
+    return T(1) / x;
+  }
+}
+template <class T> __device__ __forceinline__ void sqrt_hrsqrt(T q, T& s, T& h) {
+  if constexpr (std::is_same<T, double>::value) {
+    const T y = __builtin_amdgcn_rsq(q);
+    T g = q * y, hh = T(0.5) * y;
+    T r = fma(-hh, g, T(0.5));
+    g = fma(g, r, g);
+    hh = fma(hh, r, hh);
+    r = fma(-hh, g, T(0.5));
+    g = fma(g, r, g);
+    hh = fma(hh, r, hh);
+    const T d = fma(-g, g, q);
+    s = fma(d, hh, g);
+    h = hh;
+  } else {
+    s = sqrt(q);
+    h = T(0.5) / s;
+  }
+}
+template <class T, int M> __device__ __forceinline__ DS<T, M> rcp(DS<T, M> x) {
+  const T r = rcp_nr(x.v);
   const T m = -r * r;
-  return {r, x.a * m, x.b * m, x.c * m};
+  return {r, d_scale<0, M>(x.a, m), d_scale<1, M>(x.b, m), d_scale<2, M>(x.c, m)};
 }
 template <class T> __device__ __forceinline__ T rcp(T x) { return T(1) / x; }
-template <class T> __device__ __forceinline__ D3<T> dsqrt(D3<T> x) {
-  const T s = sqrt(x.v);
-  const T h = T(0.5) / s;
-  return {s, x.a * h, x.b * h, x.c * h};
+template <class T, int M> __device__ __forceinline__ DS<T, M> dsqrt(DS<T, M> x) {
+  T s, h;
+  sqrt_hrsqrt(x.v, s, h);
+  return {s, d_scale<0, M>(x.a, h), d_scale<1, M>(x.b, h), d_scale<2, M>(x.c, h)};
 }
 template <class T> __device__ __forceinline__ T dsqrt(T x) { return sqrt(x); }
-template <class T> __device__ __forceinline__ D3<T> dlog(D3<T> x) {
-  const T r = T(1) / x.v;
-  return {log(x.v), x.a * r, x.b * r, x.c * r};
+// sqrt(q) and its reciprocal together (the reciprocal is 2 h, h from the same iteration)
+template <class T, int M> __device__ __forceinline__ void dsqrt_rcp(DS<T, M> q, DS<T, M>& s, DS<T, M>& is) {
+  T v, h;
+  sqrt_hrsqrt(q.v, v, h);
+  s = {v, d_scale<0, M>(q.a, h), d_scale<1, M>(q.b, h), d_scale<2, M>(q.c, h)};
+  const T r = T(2) * h;
+  const T m = -r * r;
+  is = {r, d_scale<0, M>(s.a, m), d_scale<1, M>(s.b, m), d_scale<2, M>(s.c, m)};
+}
+template <class T> __device__ __forceinline__ void dsqrt_rcp(T q, T& s, T& is) {
+  s = sqrt(q);
+  is = T(1) / s;
+}
+template <class T, int M> __device__ __forceinline__ DS<T, M> dlog(DS<T, M> x) {
+  const T r = rcp_nr(x.v);
+  return {log(x.v), d_scale<0, M>(x.a, r), d_scale<1, M>(x.b, r), d_scale<2, M>(x.c, r)};
 }
 template <class T> __device__ __forceinline__ T dlog(T x) { return log(x); }
-template <class T> __device__ __forceinline__ D3<T> datan(D3<T> x) {
-  const T r = T(1) / (T(1) + x.v * x.v);
-  return {atan(x.v), x.a * r, x.b * r, x.c * r};
+template <class T, int M> __device__ __forceinline__ DS<T, M> datan(DS<T, M> x) {
+  const T r = rcp_nr(T(1) + x.v * x.v);
+  return {atan(x.v), d_scale<0, M>(x.a, r), d_scale<1, M>(x.b, r), d_scale<2, M>(x.c, r)};
 }
 template <class T> __device__ __forceinline__ T datan(T x) { return atan(x); }
 // 2*atan(-y2/D), the FSC Burgers' function (TD.jl:1430), with the derivative written as
 // 2*(y2 dD - D dy2)/(D^2 + y2^2) so it stays finite where D crosses zero.
 template <class T> __device__ __forceinline__ T burgers2(T y2, T D) { return T(2) * atan(-y2 / D); }
-template <class T> __device__ __forceinline__ D3<T> burgers2(D3<T> y2, D3<T> D) {
-  const T f = T(2) / (D.v * D.v + y2.v * y2.v);
-  return {T(2) * atan(-y2.v / D.v), (y2.v * D.a - D.v * y2.a) * f, (y2.v * D.b - D.v * y2.b) * f,
-          (y2.v * D.c - D.v * y2.c) * f};
+template <class T, int M1, int M2> __device__ __forceinline__ DS<T, M1 | M2> burgers2(DS<T, M1> y2, DS<T, M2> D) {
+  const T f = T(2) * rcp_nr(D.v * D.v + y2.v * y2.v);
+  // y2 dD - D dy2, per direction
+  const DS<T, M1 | M2> g = DS<T, M2>{T(0), D.a, D.b, D.c} * y2.v - DS<T, M1>{T(0), y2.a, y2.b, y2.c} * D.v;
+  return {T(2) * atan(-y2.v / D.v), d_scale<0, M1 | M2>(g.a, f), d_scale<1, M1 | M2>(g.b, f),
+          d_scale<2, M1 | M2>(g.c, f)};
 }
+
+// The type of a value with the derivative directions M: itself for a plain scalar, DS<T, M> for a dual.
+template <class S, int M> struct SparseOf {
+  using type = S;
+};
+template <class T, int M0, int M> struct SparseOf<DS<T, M0>, M> {
+  using type = DS<T, M>;
+};
+template <class S, int M> using Sp = typename SparseOf<S, M>::type;
 
 // ---- small linear algebra -------------------------------------------------------------------------------
 // T' = A T A^T with A given by its three columns c0, c1, c2 (TensorTransformation3D!.jl:35-53 with A in
@@ -177,14 +296,17 @@ template <class T> __device__ __forceinline__ void ad_strain(T x, T y, T z, T co
   const T zeta = y * sinA + z * cosA;
   const T x2 = x * x, y2 = y * y, z2 = z * z;
   const T r2 = x2 + y2 + z2;
-  const T r = sqrt(r2);
-  const T ir = T(1) / r;
+  // r, 1/r, 1/(r-z), 1/W through rcp_nr / sqrt_hrsqrt: where one of them is infinite the point lies on the
+  // triangle's edge, whose pairs are NaN by the configuration rule (mode 0) whatever these return
+  T r, hr;
+  sqrt_hrsqrt(r2, r, hr);
+  const T ir = T(2) * hr;
   const T ir2 = ir * ir;
   const T ir3 = ir2 * ir;
   const T rmz = r - z;
-  const T irz = T(1) / rmz;
+  const T irz = rcp_nr(rmz);
   const T W = zeta - r;
-  const T iW = T(1) / W;
+  const T iW = rcp_nr(W);
   const T iWr = iW * ir;         // 1/(W r)
   const T iW2 = iW * iW;         // 1/W^2
   const T iW2r = iW2 * ir;       // 1/(W^2 r)
@@ -259,12 +381,16 @@ template <class T> __device__ __forceinline__ void ad_disp(T x, T y, T z, T cosA
 // output separately (fsc_out), so a caller can evaluate the outputs one at a time and keep register use low.
 template <class T, class S> struct FscBase {
   T cosB, sinB, cotB, icosB, nu, a, c1, c2, nu2, cot2;
-  S y1, y2, y3b, rb, irb, iW6, iW7, lW6, lW7, Fib;
+  Sp<S, 1> y1;
+  Sp<S, 2> y2;
+  Sp<S, 4> y3b;
+  S rb, irb, iW6, iW7, lW6, lW7, Fib;
 };
 
+// y1, y2, y3: the point in the angular dislocation's coordinates; as duals each carries its own direction only.
 template <class T, class S>
-__device__ __forceinline__ void fsc_base(S y1, S y2, S y3, T cosB, T sinB, T cotB, T cotB2, T icosB, T nu, T a,
-                                         FscBase<T, S>& f) {
+__device__ __forceinline__ void fsc_base(Sp<S, 1> y1, Sp<S, 2> y2, Sp<S, 4> y3, T cosB, T sinB, T cotB, T cotB2,
+                                         T icosB, T nu, T a, FscBase<T, S>& f) {
   f.cosB = cosB;
   f.sinB = sinB;
   f.cotB = cotB;
@@ -278,9 +404,8 @@ __device__ __forceinline__ void fsc_base(S y1, S y2, S y3, T cosB, T sinB, T cot
   f.y1 = y1;
   f.y2 = y2;
   f.y3b = y3 + T(2) * a;
-  const S z3b = f.y3b * cosB - y1 * sinB;
-  f.rb = dsqrt(y1 * y1 + y2 * y2 + f.y3b * f.y3b);
-  f.irb = rcp(f.rb);
+  const Sp<S, 5> z3b = f.y3b * cosB - y1 * sinB;
+  dsqrt_rcp(y1 * y1 + y2 * y2 + f.y3b * f.y3b, f.rb, f.irb);
   const S W6 = f.rb + f.y3b;
   const S W7 = f.rb + z3b;
   f.iW6 = rcp(W6);
@@ -292,7 +417,11 @@ __device__ __forceinline__ void fsc_base(S y1, S y2, S y3, T cosB, T sinB, T cot
 
 // Products and sums of FscBase members that several outputs use (no division or transcendental among them).
 template <class T, class S> struct FscDer {
-  S z1b, irb2, irb3, aR, y3ma, rcy, rsy, cpa, y22, y11, q6, q7, p1;
+  Sp<S, 5> z1b;
+  Sp<S, 4> y3ma;
+  Sp<S, 2> y22;
+  Sp<S, 1> y11;
+  S irb2, irb3, aR, rcy, rsy, cpa, q6, q7, p1;
 };
 
 template <class T, class S> __device__ __forceinline__ FscDer<T, S> fsc_der(const FscBase<T, S>& f) {
@@ -317,53 +446,59 @@ template <class T, class S>
 __device__ __forceinline__ S fsc_out_d(const FscBase<T, S>& f, const FscDer<T, S>& dq, int o) {
   const T cosB = f.cosB, sinB = f.sinB, cotB = f.cotB, icosB = f.icosB, nu = f.nu, a = f.a, c1 = f.c1, c2 = f.c2,
           nu2 = f.nu2, cot2 = f.cot2;
-  const S &y1 = f.y1, &y2 = f.y2, &y3b = f.y3b, &irb = f.irb, &iW6 = f.iW6, &iW7 = f.iW7, &lW6 = f.lW6,
-          &lW7 = f.lW7, &Fib = f.Fib;
-  const S &z1b = dq.z1b, &irb2 = dq.irb2, &irb3 = dq.irb3, &aR = dq.aR, &y3ma = dq.y3ma, &rcy = dq.rcy,
-          &rsy = dq.rsy, &cpa = dq.cpa, &y22 = dq.y22, &y11 = dq.y11, &q6 = dq.q6, &q7 = dq.q7, &p1 = dq.p1;
+  const auto &y1 = f.y1;
+  const auto &y2 = f.y2;
+  const auto &y3b = f.y3b;
+  const S &irb = f.irb, &iW6 = f.iW6, &iW7 = f.iW7, &lW6 = f.lW6, &lW7 = f.lW7, &Fib = f.Fib;
+  const auto &z1b = dq.z1b;
+  const auto &y3ma = dq.y3ma;
+  const auto &y22 = dq.y22;
+  const auto &y11 = dq.y11;
+  const S &irb2 = dq.irb2, &irb3 = dq.irb3, &aR = dq.aR, &rcy = dq.rcy, &rsy = dq.rsy, &cpa = dq.cpa, &q6 = dq.q6,
+          &q7 = dq.q7, &p1 = dq.p1;
   switch (o) {
     case 0:  // ux
-      return -T(2) * c2 * c1 * cot2 * Fib + c1 * y2 * iW6 * ((c1 - aR) * cotB - y1 * iW6 * (nu + aR)) +
+      return dense(-T(2) * c2 * c1 * cot2 * Fib + c1 * y2 * iW6 * ((c1 - aR) * cotB - y1 * iW6 * (nu + aR)) +
              c1 * cosB * cotB * y2 * iW7 * cpa + a * cotB * y2 * y3ma * irb3 +
              y2 * y3ma * irb * iW6 * (-c1 * cotB + y1 * iW6 * (nu2 + aR) + a * y1 * irb2) +
              y2 * y3ma * irb * iW7 *
-                 (cosB * iW7 * (rcy * (c1 * cosB - aR) * cotB + T(2) * c2 * cosB * rsy) - a * cosB * cotB * y3b * irb2);
+                 (cosB * iW7 * (rcy * (c1 * cosB - aR) * cotB + T(2) * c2 * cosB * rsy) - a * cosB * cotB * y3b * irb2));
     case 1:  // uy
-      return c1 * ((T(2) * c2 * cot2 + nu) * lW6 - (T(2) * c2 * cot2 + T(1)) * cosB * lW7) +
+      return dense(c1 * ((T(2) * c2 * cot2 + nu) * lW6 - (T(2) * c2 * cot2 + T(1)) * cosB * lW7) +
              c1 * iW6 * (-c1 * cotB * y1 + nu * y3b - a + a * cotB * y1 * irb + y11 * iW6 * (nu + aR)) -
              c1 * cotB * iW7 * (cosB * z1b - a * icosB * rsy * irb) - a * cotB * y1 * y3ma * irb3 +
              q6 * (nu2 + irb * (c1 * cotB * y1 + a) - y11 * irb * iW6 * (nu2 + aR) - a * y11 * irb3) +
-             cotB * q7 * (-cosB * sinB + a * icosB * y1 * y3b * irb3 + rsy * irb * (T(2) * c2 * cosB - rcy * iW7 * p1));
+             cotB * q7 * (-cosB * sinB + a * icosB * y1 * y3b * irb3 + rsy * irb * (T(2) * c2 * cosB - rcy * iW7 * p1)));
     case 2:  // uz
-      return c1 * (y2 * iW6 * (T(1) + aR) - cosB * y2 * iW7 * cpa) - y2 * y3ma * irb * (a * irb2 + iW6) +
-             cosB * y2 * y3ma * irb * iW7 * (rcy * iW7 * cpa + a * y3b * irb2);
+      return dense(c1 * (y2 * iW6 * (T(1) + aR) - cosB * y2 * iW7 * cpa) - y2 * y3ma * irb * (a * irb2 + iW6) +
+             cosB * y2 * y3ma * irb * iW7 * (rcy * iW7 * cpa + a * y3b * irb2));
     case 3:  // vx
-      return c1 * ((T(2) * c2 * cot2 - nu) * lW6 - (T(2) * c2 * cot2 + c1) * cosB * lW7) -
+      return dense(c1 * ((T(2) * c2 * cot2 - nu) * lW6 - (T(2) * c2 * cot2 + c1) * cosB * lW7) -
              c1 * iW6 * (y1 * cotB * (c1 - aR) + nu * y3b - a + y22 * iW6 * (nu + aR)) -
              c1 * cotB * z1b * iW7 * cpa - a * cotB * y1 * y3ma * irb3 +
              q6 * (-nu2 + irb * (c1 * cotB * y1 - a) + y22 * irb * iW6 * (nu2 + aR) + a * y22 * irb3) +
              q7 * (cosB * cosB - irb * (c1 * cotB * z1b + a * cosB) + a * cotB * y3b * z1b * irb3 -
-                   irb * iW7 * (cosB * cosB * y22 - a * cotB * z1b * irb * rcy));
+                   irb * iW7 * (cosB * cosB * y22 - a * cotB * z1b * irb * rcy)));
     case 4:  // vy
-      return T(2) * c2 * c1 * cot2 * Fib + c1 * y2 * iW6 * (-(c1 - aR) * cotB + y1 * iW6 * (nu + aR)) -
+      return dense(T(2) * c2 * c1 * cot2 * Fib + c1 * y2 * iW6 * (-(c1 - aR) * cotB + y1 * iW6 * (nu + aR)) -
              c1 * cotB * y2 * iW7 * p1 - a * cotB * y2 * y3ma * irb3 +
              y2 * y3ma * irb * iW6 * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb * (irb + iW6)) +
-             cotB * y2 * y3ma * irb * iW7 * (-T(2) * c2 * cosB + rcy * iW7 * p1 + a * icosB * y3b * irb2);
+             cotB * y2 * y3ma * irb * iW7 * (-T(2) * c2 * cosB + rcy * iW7 * p1 + a * icosB * y3b * irb2));
     case 5:  // vz
-      return c1 * (-sinB * lW7 - y1 * iW6 * (T(1) + aR) + z1b * iW7 * cpa) + y1 * y3ma * irb * (a * irb2 + iW6) -
+      return dense(c1 * (-sinB * lW7 - y1 * iW6 * (T(1) + aR) + z1b * iW7 * cpa) + y1 * y3ma * irb * (a * irb2 + iW6) -
              q7 * (sinB * (cosB - aR) + z1b * irb * (T(1) + a * y3b * irb2) -
-                   irb * iW7 * (cosB * sinB * y22 - a * z1b * irb * rcy));
+                   irb * iW7 * (cosB * sinB * y22 - a * z1b * irb * rcy)));
     case 6:  // wx
-      return T(2) * c2 * (c1 * cotB * Fib + y2 * iW6 * (nu2 + aR) - cosB * y2 * iW7 * cpa) +
-             y2 * y3ma * irb * (nu2 * iW6 + a * irb2) + cosB * y2 * y3ma * irb * iW7 * (c1 - rcy * iW7 * cpa - a * y3b * irb2);
+      return dense(T(2) * c2 * (c1 * cotB * Fib + y2 * iW6 * (nu2 + aR) - cosB * y2 * iW7 * cpa) +
+             y2 * y3ma * irb * (nu2 * iW6 + a * irb2) + cosB * y2 * y3ma * irb * iW7 * (c1 - rcy * iW7 * cpa - a * y3b * irb2));
     case 7:  // wy
-      return -T(2) * c2 * c1 * cotB * (lW6 - cosB * lW7) - T(2) * c2 * y1 * iW6 * (nu2 + aR) +
+      return dense(-T(2) * c2 * c1 * cotB * (lW6 - cosB * lW7) - T(2) * c2 * y1 * iW6 * (nu2 + aR) +
              T(2) * c2 * z1b * iW7 * cpa + y3ma * irb * (c1 * cotB - nu2 * y1 * iW6 - a * y1 * irb2) -
              q7 * (cosB * sinB + cotB * rcy * irb * (T(2) * c2 * cosB - rcy * iW7) +
-                   aR * (sinB - y3b * z1b * irb2 - z1b * rcy * irb * iW7));
+                   aR * (sinB - y3b * z1b * irb2 - z1b * rcy * irb * iW7)));
     default:  // wz
-      return T(2) * c2 * Fib + T(2) * c2 * sinB * y2 * iW7 * cpa +
-             sinB * y2 * y3ma * irb * iW7 * (T(1) + rcy * iW7 * cpa + a * y3b * irb2);
+      return dense(T(2) * c2 * Fib + T(2) * c2 * sinB * y2 * iW7 * cpa +
+             sinB * y2 * y3ma * irb * iW7 * (T(1) + rcy * iW7 * cpa + a * y3b * irb2));
   }
 }
 
@@ -566,8 +701,8 @@ __device__ __forceinline__ void fsc_side_wk(const T* __restrict__ s, T X, T Y, T
     const T aa = ad == 0 ? s[S_AA] : s[S_AB];
     const T sgn = ad == 0 ? T(-1) : T(1);
     FscBase<T, D3<T>> fb;
-    fsc_base<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
-                       D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
+    fsc_base<T, D3<T>>(DS<T, 1>{sg * px, T(1), T(0), T(0)}, DS<T, 2>{sg * py, T(0), T(1), T(0)},
+                       DS<T, 4>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
     const FscDer<T, D3<T>> dq = fsc_der<T, D3<T>>(fb);
     // output 3 c + K is component c of the displacement for Burgers component K; its gradient feeds v_cc
     // (diagonal) and the two shear sums it appears in
@@ -630,8 +765,8 @@ __device__ __forceinline__ void fsc_strain_side(const T* __restrict__ s, T X, T 
     const T aa = ad == 0 ? s[S_AA] : s[S_AB];
     const T sgn = ad == 0 ? T(-1) : T(1);
     FscBase<T, D3<T>> fb;
-    fsc_base<T, D3<T>>(D3<T>{sg * px, T(1), T(0), T(0)}, D3<T>{sg * py, T(0), T(1), T(0)},
-                       D3<T>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
+    fsc_base<T, D3<T>>(DS<T, 1>{sg * px, T(1), T(0), T(0)}, DS<T, 2>{sg * py, T(0), T(1), T(0)},
+                       DS<T, 4>{pz, T(0), T(0), T(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
     // one output at a time: output q = 3 c + k is component c of the displacement for Burgers component k;
     // its gradient feeds v_cc (diagonal), and half of the two shear sums it appears in
 #pragma unroll 1
@@ -894,24 +1029,42 @@ __device__ __forceinline__ void traction3(const T* e, T lam, T G, const T* n, co
   Tds = px * d[0] + py * d[1] + pz * d[2];
 }
 
-// Tractions (Tn, Tss, Tds) on the plane with normal n and strike/dip directions sv, d of one side's free-surface
-// strain for the unit EFCS Burgers vector e_kE, from the side's component strains w_K[c * ws] (fsc_side_wk) and its
-// configuration sign sg; k4 = 1/(4 pi (1-nu)).  e_kE in the side's ADCS is (ey1[kE], ey2[kE], ey3[kE]), and
-// configuration I negates its first two components.
+// Tractions (t0, t1, t2) = (normal, strike, dip) on the receiver plane of one side's free-surface strain for unit
+// ADCS Burgers component K, before the factor 1/(4 pi (1-nu)) and the configuration's sign: w (fsc_side_wk) holds
+// {d1u1, d2u2, d3u3, d2u1+d1u2, d3u1+d1u3, d3u2+d2u3} in the side's ADCS.  Traction is invariant under a rotation
+// of both the strain and the plane, so the receiver's normal n and strike/dip directions sv, d are turned into the
+// ADCS (columns ey1 = (p, q, 0), ey2 = (q, -p, 0), ey3 = (0, 0, -1) in EFCS) instead of the strain into EFCS
+// (TensorTransformation3D!.jl:35-53 followed by the traction of CalculateInfluenceMatrices3D.jl:92-110).
 template <class T>
-__device__ __forceinline__ void edge_unit_traction(const T* __restrict__ s, int kE, T sg, const T* w0, const T* w1,
-                                                   const T* w2, int ws, T k4, T lam, T G, const T* n, const T* sv,
-                                                   const T* d, T& o0, T& o1, T& o2) {
-  const T p = s[S_EY1 + 0], qq = s[S_EY1 + 1];
-  const T c0[3] = {p, qq, T(0)}, c1[3] = {qq, -p, T(0)}, c2[3] = {T(-0.0), T(-0.0), T(-1)};
-  const T b1 = sg * (kE == 0 ? p : (kE == 1 ? qq : T(0))) * k4;
-  const T b2 = sg * (kE == 0 ? qq : (kE == 1 ? -p : T(0))) * k4;
-  const T b3 = (kE == 2 ? T(-1) : T(-0.0)) * k4;
-  T e[6];
-#pragma unroll
-  for (int c = 0; c < 6; ++c) e[c] = (c < 3 ? T(1) : T(0.5)) * (b1 * w0[c * ws] + b2 * w1[c * ws] + b3 * w2[c * ws]);
-  tensor_rotate(e, c0, c1, c2);
-  traction3(e, lam, G, n, sv, d, o0, o1, o2);
+__device__ __forceinline__ void edge_comp_traction(const T* __restrict__ s, const T w[6], T lam, T G, const T* n,
+                                                   const T* sv, const T* d, T& t0, T& t1, T& t2) {
+  const T p = s[S_EY1 + 0], q = s[S_EY1 + 1];
+  const T na[3] = {p * n[0] + q * n[1], q * n[0] - p * n[1], -n[2]};
+  const T sa[3] = {p * sv[0] + q * sv[1], q * sv[0] - p * sv[1], -sv[2]};
+  const T da[3] = {p * d[0] + q * d[1], q * d[0] - p * d[1], -d[2]};
+  const T e[6] = {w[0], w[1], w[2], T(0.5) * w[3], T(0.5) * w[4], T(0.5) * w[5]};
+  traction3(e, lam, G, na, sa, da, t0, t1, t2);
+}
+
+// Tractions (o0, o1, o2) of one side's free-surface strain for the unit EFCS Burgers vector e_kE, from the component
+// tractions tK[q * ts] (edge_comp_traction, K = 0, 1, 2) and the configuration sign sg; k4 = 1/(4 pi (1-nu)).
+// e_kE in the side's ADCS is (ey1[kE], ey2[kE], ey3[kE]) = (p, q, 0), (q, -p, 0), (0, 0, -1) for kE = 0, 1, 2, and
+// configuration I negates its first two components (TD.jl:1951-1962).
+template <class T>
+__device__ __forceinline__ void edge_unit_combine(const T* __restrict__ s, int kE, T sg, const T* t0, const T* t1,
+                                                  const T* t2, int ts, T k4, T& o0, T& o1, T& o2) {
+  const T p = s[S_EY1 + 0], q = s[S_EY1 + 1];
+  const T g = sg * k4;
+  if (kE == 2) {
+    o0 = -k4 * t2[0];
+    o1 = -k4 * t2[ts];
+    o2 = -k4 * t2[2 * ts];
+  } else {
+    const T b1 = g * (kE == 0 ? p : q), b2 = g * (kE == 0 ? q : -p);
+    o0 = b1 * t0[0] + b2 * t1[0];
+    o1 = b1 * t0[ts] + b2 * t1[ts];
+    o2 = b1 * t0[2 * ts] + b2 * t1[2 * ts];
+  }
 }
 
 // One TD record's full-space strain (real, or image with the image record) at one point turned into the three
@@ -928,13 +1081,23 @@ __device__ __forceinline__ void td_part_traction(const T* __restrict__ r, T X, T
     for (int c = 0; c < 6; ++c) acc[m][c] = T(0);
   const int mode = td_strain_tdcs(r, X, Y, Z, nu, acc);
   const T bad = mode == 0 ? qnan<T>() : T(0);
+  // traction is invariant under a rotation of both the strain and the plane: the receiver's directions are turned
+  // into the TDCS (components along Vnorm, Vstrike, Vdip) once, instead of each mode's strain into EFCS
+  const T* Vn = r + R_VN;
+  const T* Vs = r + R_VS;
+  const T* Vd = r + R_VD;
+  const T nt[3] = {Vn[0] * n[0] + Vn[1] * n[1] + Vn[2] * n[2], Vs[0] * n[0] + Vs[1] * n[1] + Vs[2] * n[2],
+                   Vd[0] * n[0] + Vd[1] * n[1] + Vd[2] * n[2]};
+  const T st[3] = {Vn[0] * s[0] + Vn[1] * s[1] + Vn[2] * s[2], Vs[0] * s[0] + Vs[1] * s[1] + Vs[2] * s[2],
+                   Vd[0] * s[0] + Vd[1] * s[1] + Vd[2] * s[2]};
+  const T dt[3] = {Vn[0] * d[0] + Vn[1] * d[1] + Vn[2] * d[2], Vs[0] * d[0] + Vs[1] * d[1] + Vs[2] * d[2],
+                   Vd[0] * d[0] + Vd[1] * d[1] + Vd[2] * d[2]};
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[m][c] += bad;
-    tensor_rotate(acc[m], r + R_VN, r + R_VS, r + R_VD);
     T a, b, c;
-    traction3(acc[m], lam, G, n, s, d, a, b, c);
+    traction3(acc[m], lam, G, nt, st, dt, a, b, c);
     t[m][0] = first ? a : t[m][0] + a;
     t[m][1] = first ? b : t[m][1] + b;
     t[m][2] = first ? c : t[m][2] + c;

@@ -592,8 +592,9 @@ __global__ void __launch_bounds__(128) tdk_edge_setup(const double* __restrict__
 //
 // A workgroup takes 64 receivers (one per lane) and blocks of kEdgeBlock edges.  The work of a block is split into
 // (edge, ADCS Burgers component K) units: a unit evaluates three of the side's nine free-surface displacement
-// outputs and their gradients (fsc_side_wk) and leaves the component's strain w_K in LDS.  After a barrier each
-// (edge, kE) unit combines the three components into e_kE's tractions.  Units are dealt to the four waves in turn,
+// outputs and their gradients (fsc_side_wk), turns the component's strain into its three tractions on the
+// receiver's plane (edge_comp_traction) and leaves them in LDS.  After a barrier each (edge, kE) unit combines the
+// three components' tractions into e_kE's (edge_unit_combine).  Units are dealt to the four waves in turn,
 // so every wave evaluates each component equally often and the barrier waits on balanced waves.  One component per
 // thread fits two waves per SIMD without accumulator-register copies; all nine outputs at once took ~470 registers
 // per lane (one wave per SIMD, with 40 % of the issued instructions moving values to and from AGPRs).
@@ -603,7 +604,7 @@ __global__ void __launch_bounds__(256, 2) tdk_edge_response(const double* __rest
                                                             const double* __restrict__ erec, int64_t nedge,
                                                             int64_t blk0, double nu, double lam, double G,
                                                             double* __restrict__ Tb, int64_t ldT) {
-  __shared__ double Wl[kEdgeBlock][3][6][64];
+  __shared__ double Tl[kEdgeBlock][3][3][64];
   __shared__ double SGl[kEdgeBlock][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int64_t i = (int64_t)blockIdx.x * 64 + lane;
@@ -615,26 +616,29 @@ __global__ void __launch_bounds__(256, 2) tdk_edge_response(const double* __rest
     for (int u = wv; u < 3 * kEdgeBlock; u += 4) {
       const int eb = u / 3, K = u % 3;
       const int64_t t = e0 + eb;
-      double w[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      double t0 = 0.0, t1 = 0.0, t2 = 0.0;
       double sg = 1.0;
       if (t < nedge) {
         const double* s = erec + t * R_SIDE_SIZE;
         if (s[S_SKIP] == 0.0) {  // edge-uniform
+          double w[6];
           if (K == 0)
             fsc_side_wk<0, double>(s, Xi, Yi, Zi, nu, w, sg);
           else if (K == 1)
             fsc_side_wk<1, double>(s, Xi, Yi, Zi, nu, w, sg);
           else
             fsc_side_wk<2, double>(s, Xi, Yi, Zi, nu, w, sg);
+          // the receiver's frame is read here, not kept live through the evaluation above
+          const double n[3] = {rv[3], rv[4], rv[5]}, sv[3] = {rv[6], rv[7], rv[8]}, d[3] = {rv[9], rv[10], rv[11]};
+          edge_comp_traction(s, w, lam, G, n, sv, d, t0, t1, t2);
         }
       }
-#pragma unroll
-      for (int c = 0; c < 6; ++c) Wl[eb][K][c][lane] = w[c];
+      Tl[eb][K][0][lane] = t0;
+      Tl[eb][K][1][lane] = t1;
+      Tl[eb][K][2][lane] = t2;
       if (K == 0) SGl[eb][lane] = sg;
     }
     __syncthreads();
-    // the receiver's frame is read here, not kept live through the evaluation above
-    const double n[3] = {rv[3], rv[4], rv[5]}, sv[3] = {rv[6], rv[7], rv[8]}, d[3] = {rv[9], rv[10], rv[11]};
     for (int u = wv; u < 3 * kEdgeBlock; u += 4) {
       const int eb = u / 3, kE = u % 3;
       const int64_t t = e0 + eb;
@@ -642,8 +646,8 @@ __global__ void __launch_bounds__(256, 2) tdk_edge_response(const double* __rest
       const double* s = erec + t * R_SIDE_SIZE;
       double o0 = 0.0, o1 = 0.0, o2 = 0.0;
       if (s[S_SKIP] == 0.0)
-        edge_unit_traction(s, kE, SGl[eb][lane], &Wl[eb][0][0][lane], &Wl[eb][1][0][lane], &Wl[eb][2][0][lane], 64,
-                           k4, lam, G, n, sv, d, o0, o1, o2);
+        edge_unit_combine(s, kE, SGl[eb][lane], &Tl[eb][0][0][lane], &Tl[eb][1][0][lane], &Tl[eb][2][0][lane], 64, k4,
+                          o0, o1, o2);
       if (i < nrecv) {
         double* T = Tb + (t * 9 + 3 * kE) * ldT + i;
         T[0] = o0;

@@ -7,7 +7,8 @@
 //
 // Printed (JSON):
 //   wk[K]    fsc_side_wk<K>: one ADCS Burgers component, the side's shared quantities formed in the call
-//   traction edge_unit_traction for the three EFCS unit Burgers vectors: the unit's tractions from the components
+//   traction edge_comp_traction for the three ADCS components and edge_unit_combine for the three EFCS unit
+//            Burgers vectors: the unit's tractions from the components
 //   shared   the part of fsc_side_wk that does not depend on K: the side frame and, per angular dislocation, the
 //            point's ADCS offsets, FscBase and FscDer
 //   single   all three components with the shared part formed once (the formulation's work for the unit, each
@@ -101,8 +102,8 @@ static void side_pass(const C* s, C X, C Y, C Z, C nu, bool outputs, C w[3][6]) 
     const C aa = ad == 0 ? s[S_AA] : s[S_AB];
     const C sgn = ad == 0 ? C(-1) : C(1);
     FscBase<C, D3<C>> fb;
-    fsc_base<C, D3<C>>(D3<C>{sg * px, C(1), C(0), C(0)}, D3<C>{sg * py, C(0), C(1), C(0)},
-                       D3<C>{pz, C(0), C(0), C(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
+    fsc_base<C, D3<C>>(DS<C, 1>{sg * px, C(1), C(0), C(0)}, DS<C, 2>{sg * py, C(0), C(1), C(0)},
+                       DS<C, 4>{pz, C(0), C(0), C(1)}, cB, sB, ctB, ctB2, icB, nu, aa, fb);
     const FscDer<C, D3<C>> dq = fsc_der<C, D3<C>>(fb);
     if (!outputs) continue;
     for (int k = 0; k < 3; ++k)
@@ -161,10 +162,12 @@ int main() {
     }
     opc::Counts c;
     opc::g = &c;
-    C o0, o1, o2;
+    C t[3][3], o0, o1, o2;
+    const C* w[3] = {w0, w1, w2};
+    for (int K = 0; K < 3; ++K)
+      edge_comp_traction<C>(s, w[K], opc::dat(0.4), opc::dat(1.0), n, sv, d, t[K][0], t[K][1], t[K][2]);
     for (int kE = 0; kE < 3; ++kE)
-      edge_unit_traction<C>(s, kE, opc::dat(-1.0), w0, w1, w2, 1, opc::dat(0.1), opc::dat(0.4), opc::dat(1.0), n, sv,
-                            d, o0, o1, o2);
+      edge_unit_combine<C>(s, kE, opc::dat(-1.0), t[0], t[1], t[2], 1, opc::dat(0.1), o0, o1, o2);
     opc::g = nullptr;
     print("traction", c);
   }
