@@ -41,28 +41,30 @@ FLOPS_PER_PAIR_HS = 43940.0
 # this mesh), and each triangle then adds +- the responses of its edges (tdk_traction_combine).
 #  * FLOPS_PER_EDGE_HS, the algorithmic work: +,-,*,/ of the engine's formulation (the forward-mode derivative of
 #    the free-surface displacement, every shared quantity formed once), counted by compiling the device header for
-#    the host with a counting scalar that leaves out what the compiler folds (tests/flops/edge_count.cpp;
-#    tests/test_flops.py recomputes it).  2 sqrt, 4 log and 2 atan per unit are not included.
+#    the host with a counting scalar that leaves out what the compiler folds and what never reaches the unit's
+#    outputs: only the gradients of the displacement are needed, so its values, logarithms and arctangents are
+#    dead (tests/flops/edge_count.cpp; tests/test_flops.py recomputes it).  2 sqrt per unit are not included.
 #  * KERNEL_FLOPS_PER_EDGE_HS: the same count for the kernel as written, which forms the shared quantities once
 #    per Burgers component (three times) so that one component fits two waves per SIMD.
 #  * REF_FLOPS_PER_EDGE_HS: the reference formulation of the same unit (the strain closed form AngDisStrainFSC for
 #    one side and three slip modes), from the op-counting oracle over 6000 random (receiver, side) pairs of this
 #    workload (tests/test_flops.py recomputes it).
-FLOPS_PER_EDGE_HS = 4279.0
-KERNEL_FLOPS_PER_EDGE_HS = 4825.0
+FLOPS_PER_EDGE_HS = 4086.0
+KERNEL_FLOPS_PER_EDGE_HS = 4536.0
 REF_FLOPS_PER_EDGE_HS = 12428.0
 # FP64 flops tdk_edge_response executes per (receiver, edge) on the same workload, from rocprofv3 --pmc
-# SQ_INSTS_VALU_ADD_F64, _MUL_F64, _FMA_F64, _TRANS_F64 (x64 lanes, FMA twice): 2.232e11 per launch over
-# 5000 x 7600 units.  Above KERNEL_FLOPS_PER_EDGE_HS by the instruction sequences of division, sqrt, log and atan
-# and by the 56 idle lanes of the last receiver block.
-EXECUTED_FLOPS_PER_EDGE_HS = 5874.6
+# SQ_INSTS_VALU_ADD_F64, _MUL_F64, _FMA_F64, _TRANS_F64 (x64 lanes, FMA twice): 1.740e11 per launch over
+# 5000 x 7600 units.  It differs from KERNEL_FLOPS_PER_EDGE_HS by the Newton and Goldschmidt steps of the
+# reciprocals and square roots (+), the compiler's common subexpressions within a unit (-) and the 56 idle
+# lanes of the last receiver block (+).
+EXECUTED_FLOPS_PER_EDGE_HS = 4579.7
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of tdk_edge_response on the 5000-triangle workload: rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE in separate runs, each scaled by what the counter reads for a 1 GiB device copy (fetch x2, write x1):
-# 0.064 GB read + 3.205 GB written, against its algorithmic 72 B per unit (2.736 GB written); the excess is the
-# write-back of 36 B per lane of register spill.
-TRAFFIC_BYTES_EDGE_HS_5000 = 3.269e9
+# 0.074 GB read + 2.975 GB written, against its algorithmic 72 B per unit (2.736 GB written); the excess is the
+# write-back of 20 B per lane of register spill.
+TRAFFIC_BYTES_EDGE_HS_5000 = 3.049e9
 
 
 def _log(msg):
@@ -595,9 +597,12 @@ def main():
                      "assembly_kernels_ms": kavg,
                      "assembly_formulation_tflops": formulation_tf,
                      "note": "flops_per_unit counts the engine's forward-mode formulation with every shared quantity "
-                             "formed once; the kernel forms them once per Burgers component (kernel_flops_per_unit), "
-                             "and executes executed_flops_per_unit (PMC, with the instruction sequences of division "
-                             "and transcendentals).  reference_formulation_tflops and assembly_formulation_tflops "
+                             "formed once and only the operations that reach the unit's tractions; the kernel forms "
+                             "the shared quantities once per Burgers component (kernel_flops_per_unit), and executes "
+                             "executed_flops_per_unit (PMC, with the Newton steps of reciprocals and square roots).  "
+                             "Under this FP64 load the chip runs at about 2.0 GHz (GRBM_GUI_ACTIVE per dispatch), "
+                             "against the 2.4 GHz of the peak figure.  "
+                             "reference_formulation_tflops and assembly_formulation_tflops "
                              "count the reference's closed forms (%g flops per edge unit, %g per pair) and exceed "
                              "the peak: the forward-mode strain is cheaper, and each mesh edge's free-surface part "
                              "is evaluated once, not once per triangle side" % (REF_FLOPS_PER_EDGE_HS,

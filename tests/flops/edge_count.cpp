@@ -13,15 +13,28 @@
 //            point's ADCS offsets, FscBase and FscDer
 //   single   all three components with the shared part formed once (the formulation's work for the unit, each
 //            quantity evaluated once); wk[0] + wk[1] + wk[2] = single + 2 shared is checked by the caller
+//   live_wk[K], live_single
+//            the same passes counting only the operations whose results reach the pass's outputs (the strain
+//            components w): the kernel needs the gradients of the free-surface displacement, not its values, so
+//            the logarithms, arctangents and value chains that feed only displacement values are dead, and the
+//            compiler removes them
+//   live_traction  the traction part counted the same way (all of it is live)
 // Test infrastructure (tests/test_flops.py); not part of the library.
 #include <cmath>
 #include <cstdio>
+#include <vector>
 
 namespace opc {
 struct Counts {
   double add = 0, mul = 0, div = 0, sqrt = 0, log = 0, atan = 0;
 };
 static Counts* g = nullptr;
+// Optional tape of the counted operations (kind, operand nodes; -1 for data and literals), for liveness.
+struct Node {
+  char kind;  // '+', '*', '/', 's' sqrt, 'l' log, 't' atan
+  int a, b;
+};
+static std::vector<Node>* tape = nullptr;
 // k marks a value known when the code is compiled (a literal, or arithmetic on literals only).  Operations whose
 // result the compiler folds are not counted: arithmetic on two known values, a product with a known 0, a sum with
 // a known 0, a product or quotient with a known +-1 (a sign change is a free source modifier).  The derivative
@@ -29,20 +42,32 @@ static Counts* g = nullptr;
 struct C {
   double v;
   bool k;
+  int n = -1;  // tape node that produced the value, -1 for data and literals
   C() : v(0), k(true) {}
   C(double x) : v(x), k(true) {}
 };
 inline C dat(double x) { C c(x); c.k = false; return c; }
 inline C res(double x, bool known) { C c(x); c.k = known; return c; }
+// an unknown result of a counted operation, recorded on the tape when one is active
+inline C op(double x, char kind, const C& a, const C& b) {
+  C c(x);
+  c.k = false;
+  if (tape) {
+    tape->push_back({kind, a.n, b.n});
+    c.n = (int)tape->size() - 1;
+  }
+  return c;
+}
+inline C neg(const C& a) { C c = res(-a.v, a.k); c.n = a.n; return c; }
 inline bool zero(C a) { return a.k && a.v == 0.0; }
 inline bool unit(C a) { return a.k && (a.v == 1.0 || a.v == -1.0); }
-inline C operator-(C a) { return res(-a.v, a.k); }
+inline C operator-(C a) { return neg(a); }
 inline C operator+(C a, C b) {
   if (a.k && b.k) return C(a.v + b.v);
   if (zero(a)) return b;
   if (zero(b)) return a;
   if (g) g->add++;
-  return res(a.v + b.v, false);
+  return op(a.v + b.v, '+', a, b);
 }
 inline C operator-(C a, C b) { return a + (-b); }
 inline C operator*(C a, C b) {
@@ -51,14 +76,14 @@ inline C operator*(C a, C b) {
   if (unit(a)) return a.v > 0 ? b : -b;
   if (unit(b)) return b.v > 0 ? a : -a;
   if (g) g->mul++;
-  return res(a.v * b.v, false);
+  return op(a.v * b.v, '*', a, b);
 }
 inline C operator/(C a, C b) {
   if (a.k && b.k) return C(a.v / b.v);
   if (zero(a)) return C(0.0);
   if (unit(b)) return b.v > 0 ? a : -a;
   if (g) g->div++;
-  return res(a.v / b.v, false);
+  return op(a.v / b.v, '/', a, b);
 }
 inline C& operator+=(C& a, C b) { a = a + b; return a; }
 inline bool operator>=(C a, C b) { return a.v >= b.v; }
@@ -67,9 +92,37 @@ inline bool operator<(C a, C b) { return a.v < b.v; }
 inline bool operator>(C a, C b) { return a.v > b.v; }
 inline bool operator==(C a, C b) { return a.v == b.v; }
 inline bool operator!=(C a, C b) { return a.v != b.v; }
-inline C sqrt(C a) { if (a.k) return C(std::sqrt(a.v)); if (g) g->sqrt++; return res(std::sqrt(a.v), false); }
-inline C log(C a) { if (a.k) return C(std::log(a.v)); if (g) g->log++; return res(std::log(a.v), false); }
-inline C atan(C a) { if (a.k) return C(std::atan(a.v)); if (g) g->atan++; return res(std::atan(a.v), false); }
+inline C sqrt(C a) { if (a.k) return C(std::sqrt(a.v)); if (g) g->sqrt++; return op(std::sqrt(a.v), 's', a, C()); }
+inline C log(C a) { if (a.k) return C(std::log(a.v)); if (g) g->log++; return op(std::log(a.v), 'l', a, C()); }
+inline C atan(C a) { if (a.k) return C(std::atan(a.v)); if (g) g->atan++; return op(std::atan(a.v), 't', a, C()); }
+// operation counts of the tape nodes that the roots depend on
+inline Counts live(const std::vector<Node>& t, const std::vector<int>& roots) {
+  std::vector<char> mark(t.size(), 0);
+  std::vector<int> stack;
+  for (int r : roots)
+    if (r >= 0) stack.push_back(r);
+  while (!stack.empty()) {
+    const int k = stack.back();
+    stack.pop_back();
+    if (mark[k]) continue;
+    mark[k] = 1;
+    if (t[k].a >= 0) stack.push_back(t[k].a);
+    if (t[k].b >= 0) stack.push_back(t[k].b);
+  }
+  Counts c;
+  for (size_t k = 0; k < t.size(); ++k) {
+    if (!mark[k]) continue;
+    switch (t[k].kind) {
+      case '+': c.add++; break;
+      case '*': c.mul++; break;
+      case '/': c.div++; break;
+      case 's': c.sqrt++; break;
+      case 'l': c.log++; break;
+      default: c.atan++; break;
+    }
+  }
+  return c;
+}
 }  // namespace opc
 
 #include "../../cutanddisplacejulia_amd/csrc/td_device.hpp"
@@ -187,7 +240,58 @@ int main() {
     opc::g = &c;
     side_pass(s, X, Y, Z, nu, true, w);
     opc::g = nullptr;
-    print("single", c, true);
+    print("single", c);
+  }
+  // live counts
+  for (int K = 0; K < 3; ++K) {
+    std::vector<opc::Node> t;
+    opc::tape = &t;
+    C w[6], sg;
+    if (K == 0) fsc_side_wk<0, C>(s, X, Y, Z, nu, w, sg);
+    if (K == 1) fsc_side_wk<1, C>(s, X, Y, Z, nu, w, sg);
+    if (K == 2) fsc_side_wk<2, C>(s, X, Y, Z, nu, w, sg);
+    opc::tape = nullptr;
+    std::vector<int> roots;
+    for (int c = 0; c < 6; ++c) roots.push_back(w[c].n);
+    roots.push_back(sg.n);
+    char name[16];
+    std::snprintf(name, sizeof(name), "live_wk%d", K);
+    print(name, opc::live(t, roots));
+  }
+  {
+    std::vector<opc::Node> t;
+    C w[3][6];
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < 6; ++q) w[k][q] = C(0);
+    opc::tape = &t;
+    side_pass(s, X, Y, Z, nu, true, w);
+    opc::tape = nullptr;
+    std::vector<int> roots;
+    for (int k = 0; k < 3; ++k)
+      for (int q = 0; q < 6; ++q) roots.push_back(w[k][q].n);
+    print("live_single", opc::live(t, roots));
+  }
+  {
+    std::vector<opc::Node> t;
+    C w0[6], w1[6], w2[6], n[3], sv[3], d[3];
+    for (int c = 0; c < 6; ++c) w0[c] = w1[c] = w2[c] = opc::dat(0.1 * c + 0.05);
+    for (int c = 0; c < 3; ++c) {
+      n[c] = opc::dat(0.3 + 0.1 * c);
+      sv[c] = opc::dat(0.5 - 0.1 * c);
+      d[c] = opc::dat(0.2 + 0.2 * c);
+    }
+    opc::tape = &t;
+    C tt[3][3], o[3][3];
+    const C* w[3] = {w0, w1, w2};
+    for (int K = 0; K < 3; ++K)
+      edge_comp_traction<C>(s, w[K], opc::dat(0.4), opc::dat(1.0), n, sv, d, tt[K][0], tt[K][1], tt[K][2]);
+    for (int kE = 0; kE < 3; ++kE)
+      edge_unit_combine<C>(s, kE, opc::dat(-1.0), tt[0], tt[1], tt[2], 1, opc::dat(0.1), o[kE][0], o[kE][1], o[kE][2]);
+    opc::tape = nullptr;
+    std::vector<int> roots;
+    for (int kE = 0; kE < 3; ++kE)
+      for (int q = 0; q < 3; ++q) roots.push_back(o[kE][q].n);
+    print("live_traction", opc::live(t, roots), true);
   }
   std::printf("}\n");
   return 0;

@@ -78,9 +78,23 @@ def test_edge_unit_count_matches_the_kernels_functions(edge_counts):
     assert min(_flops(c["wk%d" % k]) - _flops(c["shared"]) for k in range(3)) > 500
 
 
+def test_edge_unit_live_counts(edge_counts):
+    """Only operations that reach the outputs count: the displacement values' logarithms and arctangents are dead
+    (the unit needs gradients), the square roots and the traction part are all live, and liveness never adds."""
+    c = edge_counts
+    for name in ("wk0", "wk1", "wk2", "single", "traction"):
+        live = c["live_" + name]
+        assert live["log"] == 0 and live["atan"] == 0, name
+        assert live["sqrt"] == c[name]["sqrt"], name
+        for op in ("add", "mul", "div"):
+            assert live[op] <= c[name][op], (name, op)
+    assert _flops(c["live_traction"]) >= _flops(c["traction"]) - 2
+
+
 def test_edge_unit_flops_frozen_in_bench(edge_counts):
     c = edge_counts
-    assert _flops(c["single"]) + _flops(c["traction"]) == bench.FLOPS_PER_EDGE_HS
-    assert sum(_flops(c["wk%d" % k]) for k in range(3)) + _flops(c["traction"]) == bench.KERNEL_FLOPS_PER_EDGE_HS
+    assert _flops(c["live_single"]) + _flops(c["live_traction"]) == bench.FLOPS_PER_EDGE_HS
+    assert (sum(_flops(c["live_wk%d" % k]) for k in range(3)) + _flops(c["live_traction"])
+            == bench.KERNEL_FLOPS_PER_EDGE_HS)
     # the forward-mode formulation is far cheaper than the reference's strain closed form for the same unit
     assert bench.FLOPS_PER_EDGE_HS < 0.4 * bench.REF_FLOPS_PER_EDGE_HS
