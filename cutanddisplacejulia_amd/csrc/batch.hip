@@ -67,16 +67,14 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) batch_far_kernel(
     const int64_t lda = 3 * M.n;
     TO* Am = A + M.a_off;
     for (int64_t j = j0; j < j1; ++j) {
-      T e[3][6];
-      strain_pair<T, HS>(rec + (M.tri0 + j) * RS, Xi, Yi, Zi, nu, e);
+      T t[3][3];
+      pair_traction<T, HS>(rec + (M.tri0 + j) * RS, Xi, Yi, Zi, nu, lam, G, n, s, d, t);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        T tn, tss, tds;
-        traction3(e[m], lam, G, n, s, d, tn, tss, tds);
         TO* col = Am + (m * M.n + j) * lda + il;
-        col[0] = (TO)(-tn);
-        col[M.n] = (TO)(-tss);
-        col[2 * M.n] = (TO)(-tds);
+        col[0] = (TO)(-t[m][0]);
+        col[M.n] = (TO)(-t[m][1]);
+        col[2 * M.n] = (TO)(-t[m][2]);
       }
     }
   }
@@ -117,17 +115,15 @@ __global__ void __launch_bounds__(256) batch_near_kernel(const double* __restric
     for (int64_t j = j0; j < j1; ++j) {
       const bool nr = valid && near_pair(near + (M.tri0 + j) * NEAR_REC, Xi, Yi, Zi);
       if (!__any(nr)) continue;
-      double e[3][6];
-      strain_pair<double, HS>(rec + (M.tri0 + j) * RS, Xi, Yi, Zi, nu, e);
+      double t[3][3];
+      pair_traction<double, HS>(rec + (M.tri0 + j) * RS, Xi, Yi, Zi, nu, lam, G, n, s, d, t);
       if (nr) {
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
-          double tn, tss, tds;
-          traction3(e[m], lam, G, n, s, d, tn, tss, tds);
           double* col = Am + (m * M.n + j) * lda + il;
-          col[0] = -tn;
-          col[M.n] = -tss;
-          col[2 * M.n] = -tds;
+          col[0] = -t[m][0];
+          col[M.n] = -t[m][1];
+          col[2 * M.n] = -t[m][2];
         }
         ++cnt;
       }

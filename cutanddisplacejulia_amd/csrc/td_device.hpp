@@ -1104,4 +1104,21 @@ __device__ __forceinline__ void td_part_traction(const T* __restrict__ r, T X, T
   }
 }
 
+// The nine tractions t[m][q] (mode m, traction q) of one (receiver, source) pair on the receiver plane (n, s, d).
+// Full space: from the TDCS strain, with the receiver's directions turned into the TDCS (td_part_traction).  Half
+// space: from the EFCS strain of strain_pair, which adds the free-surface sides and the image (including the
+// reference's flat-triangle branch) in EFCS.
+template <class T, bool HS>
+__device__ __forceinline__ void pair_traction(const T* __restrict__ rec, T X, T Y, T Z, T nu, T lam, T G, const T* n,
+                                              const T* s, const T* d, T t[3][3]) {
+  if constexpr (!HS) {
+    td_part_traction(rec, X, Y, Z, nu, lam, G, n, s, d, t, true);
+  } else {
+    T e[3][6];
+    strain_pair<T, HS>(rec, X, Y, Z, nu, e);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) traction3(e[m], lam, G, n, s, d, t[m][0], t[m][1], t[m][2]);
+  }
+}
+
 }  // namespace cdj

@@ -344,18 +344,13 @@ __global__ void __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) tdk_traction(cons
   constexpr int RS = HS ? REC_HS : REC_FS;
   const T sg = (T)out.sign;
   for (int64_t j = j0; j < j1; ++j) {
-    const T* r = rec + j * RS;
-    T e[3][6];
-    strain_pair<T, HS>(r, Xi, Yi, Zi, nu, e);
     const int64_t o = i + j * out.ld;
+    T t[3][3];
+    pair_traction<T, HS>(rec + j * RS, Xi, Yi, Zi, nu, lam, G, n, s, d, t);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      T tn, tss, tds;
-      traction3(e[m], lam, G, n, s, d, tn, tss, tds);
-      out.p[3 * m + 0][o] = (TO)(sg * tn);
-      out.p[3 * m + 1][o] = (TO)(sg * tss);
-      out.p[3 * m + 2][o] = (TO)(sg * tds);
-    }
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) out.p[3 * m + q][o] = (TO)(sg * t[m][q]);
   }
 }
 
@@ -381,18 +376,15 @@ __global__ void __launch_bounds__(256) tdk_nearfield(const double* __restrict__ 
   for (int64_t j = j0; j < j1; ++j) {
     const bool nr = valid && near_pair(near + j * NEAR_REC, Xi, Yi, Zi);
     if (!__any(nr)) continue;
-    double e[3][6];
-    strain_pair<double, HS>(rec + j * RS, Xi, Yi, Zi, nu, e);
+    // the same evaluation as the FP64 assembly (tdk_traction), so near entries equal its entries
+    double t[3][3];
+    pair_traction<double, HS>(rec + j * RS, Xi, Yi, Zi, nu, lam, G, n, s, d, t);
     if (nr) {
       const int64_t o = i + j * out.ld;
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
-        double tn, tss, tds;
-        traction3(e[m], lam, G, n, s, d, tn, tss, tds);
-        out.p[3 * m + 0][o] = out.sign * tn;
-        out.p[3 * m + 1][o] = out.sign * tss;
-        out.p[3 * m + 2][o] = out.sign * tds;
-      }
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) out.p[3 * m + q][o] = out.sign * t[m][q];
       ++cnt;
     }
   }
