@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "linalg.hpp"
+#include "mixed.hpp"
 
 namespace cg = cooperative_groups;
 
@@ -634,17 +635,41 @@ __device__ __forceinline__ void gemm_sub_tile(const T* __restrict__ L, int64_t l
     __syncthreads();
   }
   // acc[tj][ti][v] = (L U)[row = wr + 16 ti + li, col = wc + 16 tj + Mfma::row(lk, v)]
+  // C's read-modify-write, one column tile tj (16 entries per lane) at a time: its loads are issued together, from
+  // clamped (always valid) addresses, and the next tile's loads are in flight while this one is stored.  Predicated
+  // element by element, each load waited for its own round trip before the store that followed it.
+  int64_t rows[4], cols[4][4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti) rows[ti] = i0 + wr + 16 * ti + li;
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
-    for (int ti = 0; ti < 4; ++ti) {
-      const int64_t row = i0 + wr + 16 * ti + li;
+    for (int v = 0; v < 4; ++v) cols[tj][v] = j0 + wc + 16 * tj + Mfma<T>::row(lk, v);
+  auto cload = [&](int tj, T (&cv)[4][4]) {
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int64_t col = j0 + wc + 16 * tj + Mfma<T>::row(lk, v);
-        if (row < M && col < N) C[row + col * ldc] -= acc[tj][ti][v];
+        const int64_t r = rows[ti] < M ? rows[ti] : M - 1, c = cols[tj][v] < N ? cols[tj][v] : N - 1;
+        cv[ti][v] = C[r + c * ldc];
       }
-    }
+  };
+  auto cstore = [&](int tj, const T (&cv)[4][4]) {
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (rows[ti] < M && cols[tj][v] < N) C[rows[ti] + cols[tj][v] * ldc] = cv[ti][v] - acc[tj][ti][v];
+  };
+  T ca[4][4], cb[4][4];
+  cload(0, ca);
+  cload(1, cb);
+  cstore(0, ca);
+  cload(2, ca);
+  cstore(1, cb);
+  cload(3, cb);
+  cstore(2, ca);
+  cstore(3, cb);
 }
 
 template <class T>
@@ -846,19 +871,46 @@ __global__ void __launch_bounds__(256, 2) gemm_sub_f16_kernel(const _Float16* __
     __syncthreads();
   }
   // acc[tj][ti][v] = (L sigma U)[row = wr + 16 ti + li, col = wc + 16 tj + 4 lk + v]
+  // C's read-modify-write as in gemm_sub_tile: all loads of a column tile issued together from clamped addresses,
+  // the next tile's loads in flight while this one is stored.
+  int64_t rows[4], cols[4][4];
+#pragma unroll
+  for (int ti = 0; ti < 4; ++ti) rows[ti] = i0 + wr + 16 * ti + li;
+  float sg[4][4];
 #pragma unroll
   for (int tj = 0; tj < 4; ++tj)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t col = j0 + wc + 16 * tj + 4 * lk + v;
-      if (col >= N) continue;
-      const float inv = 1.0f / sigma[col];
-#pragma unroll
-      for (int ti = 0; ti < 4; ++ti) {
-        const int64_t row = i0 + wr + 16 * ti + li;
-        if (row < M) C[row + col * ldc] -= acc[tj][ti][v] * inv;
-      }
+      cols[tj][v] = j0 + wc + 16 * tj + 4 * lk + v;
+      sg[tj][v] = sigma[cols[tj][v] < N ? cols[tj][v] : N - 1];
     }
+  auto cload = [&](int tj, float (&cv)[4][4]) {
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = rows[ti] < M ? rows[ti] : M - 1, c = cols[tj][v] < N ? cols[tj][v] : N - 1;
+        cv[ti][v] = C[r + c * ldc];
+      }
+  };
+  auto cstore = [&](int tj, const float (&cv)[4][4]) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const float s = 1.0f / sg[tj][v];
+#pragma unroll
+      for (int ti = 0; ti < 4; ++ti)
+        if (rows[ti] < M && cols[tj][v] < N) C[rows[ti] + cols[tj][v] * ldc] = cv[ti][v] - acc[tj][ti][v] * s;
+    }
+  };
+  float ca[4][4], cb[4][4];
+  cload(0, ca);
+  cload(1, cb);
+  cstore(0, ca);
+  cload(2, ca);
+  cstore(1, cb);
+  cload(3, cb);
+  cstore(2, ca);
+  cstore(3, cb);
 }
 
 // ---- triangular solves with nrhs <= 64 right-hand sides -------------------------------------------------
@@ -1444,9 +1496,26 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int32_t* ipiv, T*
                        ldb);
     e = hipMemcpyAsync(B, tmp, sizeof(T) * ldb * nrhs, hipMemcpyDeviceToDevice, st);
   }
-  if (e == hipSuccess) e = tri_solve_lower_unit<T>(LU, n, lda, B, nrhs, ldb, st);
-  if (e == hipSuccess) e = tri_solve_upper<T>(LU, n, lda, B, nrhs, ldb, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (nrhs == 1 && n <= trsv_persistent_max_n<T>()) {
+    // one right-hand side: persistent solves (one launch each, the 64-row blocks chained through flags) in place
+    // of two launches per block
+    int* flags = nullptr;
+    const int64_t nf = (n + LU_NB - 1) / LU_NB + 1;  // block flags, then the error word
+    if (e == hipSuccess) e = hipMalloc(&flags, sizeof(int) * nf);
+    if (e == hipSuccess) e = hipMemsetAsync(flags, 0, sizeof(int) * nf, st);
+    TrsvState s{flags, flags + nf - 1, 0, 1ll << 20};
+    if (e == hipSuccess) e = trsv_persistent<T>(LU, n, lda, B, true, s, st);
+    if (e == hipSuccess) e = trsv_persistent<T>(LU, n, lda, B, false, s, st);
+    int herr = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, flags + nf - 1, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && herr != 0) e = hipErrorLaunchTimeOut;  // a block's wait overran
+    if (flags) (void)hipFree(flags);
+  } else {
+    if (e == hipSuccess) e = tri_solve_lower_unit<T>(LU, n, lda, B, nrhs, ldb, st);
+    if (e == hipSuccess) e = tri_solve_upper<T>(LU, n, lda, B, nrhs, ldb, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+  }
   (void)hipFree(perm);
   if (tmp) (void)hipFree(tmp);
   return e;

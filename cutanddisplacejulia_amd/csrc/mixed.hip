@@ -70,6 +70,17 @@ __device__ __forceinline__ bool wait_flag(const int* flag, int epoch, long long 
   return true;
 }
 
+// value of v in lane l (l wave-uniform), through scalar registers
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned long long)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)((unsigned long long)b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // LOWER: unit lower (forward) solve; otherwise upper (backward) solve with the diagonal.
 // x (length n, precision T) holds the right-hand side on entry and the solution on exit.
 template <class T, bool LOWER>
@@ -125,7 +136,34 @@ __global__ void __launch_bounds__(256) trsv_persistent_kernel(const T* __restric
   if (tid < nr) rhs[tid] = (T)((double)rhs[tid] - (accd[0][tid] + accd[1][tid] + accd[2][tid] + accd[3][tid]));
   __syncthreads();
   // diagonal block: wave 0, lane i holds x[r0 + i]; step r broadcasts x[r0 + r] (the block is staged in LDS)
-  if (tid < 64) {
+  if (tid < 64 && nr == TB) {
+    // full block, steps unrolled: the broadcast is a register read of a fixed lane and the LDS reads of the
+    // block do not wait for the chain; the division by the diagonal is a multiplication by its reciprocal
+    // (formed before the chain) with one correction step
+    T xi = rhs[tid];
+    if (LOWER) {
+#pragma unroll
+      for (int r = 0; r < TB - 1; ++r) {
+        const T xr = lane_value(xi, r);
+        const T dr = D[tid][r];
+        if (tid > r) xi -= dr * xr;
+      }
+    } else {
+      const T d = D[tid][tid];
+      const T rd = T(1) / d;
+#pragma unroll
+      for (int r = TB - 1; r >= 0; --r) {
+        if (tid == r) {
+          const T q = xi * rd;
+          xi = fma(fma(-q, d, xi), rd, q);
+        }
+        const T xr = lane_value(xi, r);
+        const T dr = D[tid][r];
+        if (tid < r) xi -= dr * xr;
+      }
+    }
+    __hip_atomic_store(&x[r0 + tid], xi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else if (tid < 64) {
     T xi = tid < nr ? rhs[tid] : T(0);
     if (LOWER) {
       for (int r = 0; r < nr - 1; ++r) {
@@ -487,6 +525,22 @@ hipError_t trsv_persistent(const T* LU, int64_t n, int64_t lda, T* x, bool lower
 }
 
 template <class T>
+int64_t trsv_persistent_max_n() {
+  // one 64-row block per workgroup, all co-resident; thread-safe one-time query
+  static const int64_t cap = [] {
+    int dev = 0, cus = 0, lo = 0, up = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return (int64_t)0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return (int64_t)0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&lo, trsv_persistent_kernel<T, true>, 256, 0) != hipSuccess)
+      return (int64_t)0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&up, trsv_persistent_kernel<T, false>, 256, 0) != hipSuccess)
+      return (int64_t)0;
+    return (int64_t)std::min(lo, up) * cus * TB;
+  }();
+  return cap;
+}
+
+template <class T>
 hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* r, double* z,
                             T* work, TrsvState& s, hipStream_t st) {
   hipLaunchKernelGGL((permute_cast_kernel<double, T>), dim3(nblk(n, 256)), dim3(256), 0, st, perm, r, work, n);
@@ -752,6 +806,8 @@ template void gemv_splitk<float>(const float*, int64_t, int64_t, int64_t, const 
                                  const double*, double, double*, hipStream_t);
 template hipError_t trsv_persistent<float>(const float*, int64_t, int64_t, float*, bool, TrsvState&, hipStream_t);
 template hipError_t trsv_persistent<double>(const double*, int64_t, int64_t, double*, bool, TrsvState&, hipStream_t);
+template int64_t trsv_persistent_max_n<float>();
+template int64_t trsv_persistent_max_n<double>();
 template hipError_t apply_lu_precond<float>(const float*, int64_t, int64_t, const int32_t*, const double*, double*,
                                             float*, TrsvState&, hipStream_t);
 template hipError_t apply_lu_precond<double>(const double*, int64_t, int64_t, const int32_t*, const double*, double*,

@@ -51,6 +51,10 @@ void gemv_splitk(const TA* A, int64_t m, int64_t n, int64_t lda, const double* x
 template <class T>
 hipError_t trsv_persistent(const T* LU, int64_t n, int64_t lda, T* x, bool lower, TrsvState& s, hipStream_t st);
 
+// Largest n the persistent solve takes (all its 64-row blocks co-resident); 0 if the query failed.
+template <class T>
+int64_t trsv_persistent_max_n();
+
 // z = U^-1 L^-1 P r with factors of precision T (work: n elements of T).
 template <class T>
 hipError_t apply_lu_precond(const T* LU, int64_t n, int64_t lda, const int32_t* perm, const double* r, double* z,
