@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "linalg.hpp"
@@ -28,11 +30,13 @@ namespace {
 
 constexpr int PT = 256;        // panel threads
 constexpr int MAX_P = 1024;    // max panel workgroups
-constexpr int MAX_RPB = PT;    // panel rows per workgroup, one per thread (LDS multipliers: 256 x 64 x 8 = 128 KiB)
+constexpr int MAX_RPB = PT;    // panel rows per workgroup, one per thread
 constexpr int LU_OUTER = 256;      // columns per outer block (rank of the trailing update)
 constexpr int LU_OUTER_BIG = 512;  // the same for n >= LU_BIG: the trailing matrix is read and written half as often
 constexpr int64_t LU_BIG = 32768;  // (measured: 512 is faster at n = 48672, slower at 2485, 15000 and 19602)
 inline int64_t lu_outer(int64_t n) { return n >= LU_BIG ? LU_OUTER_BIG : LU_OUTER; }
+constexpr int PERM_STRIDE = 1 + 4 * LU_NB;                        // ints of one panel's row permutation
+constexpr int PERM_INTS = (LU_OUTER_BIG / LU_NB) * PERM_STRIDE;   // the permutations of one outer block's panels
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -55,13 +59,16 @@ template <> struct Mfma<float> {
 // Panel factorization: one cooperative launch per panel of nbw <= 64 columns; the panel's m rows are split over P
 // co-resident workgroups, one row per thread.
 //
-// Rows never move.  Each thread keeps the not-yet-eliminated part of its row in registers, shifted so that x[0] is
-// always the current column j (x[i] is column j + i), and tracks the row's current position `pos`.  The row
-// interchange of step j (position j <-> pivot position) is a relabelling: the pivot row takes position j, the row
-// that held position j takes the pivot's old position.  Multipliers go to LDS (Lsh, by thread), pivot rows to a
-// packed LDS copy of U (Ush), and every thread writes its finished row to its final position at the end.  The
-// arithmetic, the pivot rule (largest |a|, ties to the lowest position, as idamax) and hence the result are those
-// of the row-moving algorithm, bit for bit.
+// Rows never move.  Each thread keeps its row in registers, shifted so that x[0] is always the current column j
+// (x[i] is column j + i while i < nbw - j), and tracks the row's current position `pos`.  The row interchange of
+// step j (position j <-> pivot position) is a relabelling: the pivot row takes position j, the row that held
+// position j takes the pivot's old position.  Each step's shift frees x[63], which receives the step's multiplier:
+// after t steps multiplier c sits in x[64 - t + c].  The pivot row used by the shift is zero beyond its active
+// part, so the shift carries the multipliers along unchanged.  A pivot row stops shifting and keeps its U row in
+// x[0..) with its multipliers behind it; every thread writes its finished row to its final position at the end.
+// No LDS holds rows or multipliers (about 5 KiB per workgroup in all), so a panel workgroup fits on a CU beside
+// other kernels' workgroups.  The arithmetic, the pivot rule (largest |a|, ties to the lowest position, as idamax)
+// and hence the result are those of the row-moving algorithm, bit for bit (a zero may change sign).
 //
 // Per column: each wave finds its candidate with DPP maxima of the 64-bit key bits(|a|) (a non-negative double's
 // bit pattern orders like its value) and stages the candidate row in LDS; after one workgroup barrier every thread
@@ -81,8 +88,6 @@ template <> struct Mfma<float> {
 constexpr uint64_t SLOT_TAG = 0xFFFull;
 constexpr uint64_t SLOT_NONE = 1ull << 63;
 constexpr int XREC = 3 + 2 * LU_NB;              // tagged words per record: key (2), position (1), row (2 per entry)
-constexpr int USH = LU_NB * (LU_NB + 1) / 2;    // packed U rows of one panel
-__device__ __forceinline__ int urow(int j) { return j * LU_NB - (j * (j - 1)) / 2; }  // offset of U row j
 
 template <int CTRL>
 __device__ __forceinline__ long long dpp64(long long v) {
@@ -122,10 +127,8 @@ template <class T, int QPL>
 __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb, int32_t* ipiv,
                                            int* info, unsigned long long* xw, unsigned long long* slots, int* bar_err,
                                            int* perm, const int P, const int b) {
-  extern __shared__ unsigned char smem_raw[];
-  T* Lsh = reinterpret_cast<T*>(smem_raw);  // multipliers of this workgroup's rows: Lsh[c * rpb + tid]
-  T* Ush = Lsh + (size_t)rpb * LU_NB;        // U rows of the panel, packed (row j at urow(j), columns j..)
-  __shared__ T stage[2][PT / 64][LU_NB];     // each wave's candidate row (active part), by column parity
+  __shared__ T stage[2][PT / 64][LU_NB];     // each wave's candidate row, zero beyond the active part, by parity
+  __shared__ T upad[2][LU_NB];               // the pivot row from the exchange (P > 1), zero beyond the active part
   __shared__ long long wk[2][PT / 64];       // each wave's candidate key (-1: none) and position
   __shared__ int wp[2][PT / 64];
   __shared__ int s_prow[2];
@@ -138,7 +141,7 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
   const bool mine = tid < nr;
   int pos = (int)(r0 + tid);  // current position of this thread's row
   bool done = !mine;          // row already taken as a pivot (or no row)
-  bool pivot = false;         // this row is U row `pos`
+  bool pivot = false;         // this row is U row `pos` (chosen at step pos)
   if (tid == 0) s_info = 0;
   T x[LU_NB];
   {
@@ -200,14 +203,15 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
     const T* uu;  // pivot row, active part
     int prow;
     if (P == 1) {
-      uu = stage[par][bw];
       prow = bk >= 0 ? bpos : j;
-      if (wave == 0 && lane < na) Ush[urow(j) + lane] = uu[lane];
+      if (wave == 0) upad[par][lane] = lane < na ? stage[par][bw][lane] : T(0);
       if (tid == 0) {
         s_piv[j] = prow;
         const double gv = bk >= 0 ? __longlong_as_double(bk) : -1.0;
         if (!(gv > 0.0) && s_info == 0) s_info = (int)(k + j + 1);
       }
+      __syncthreads();
+      uu = upad[par];
     } else {
       if (wave == 0) {
         unsigned long long* rec = xw + (size_t)(par * MAX_P + b) * XREC;
@@ -322,7 +326,7 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
               ok = ok && (uint32_t)(w0 >> 32) == tg && (uint32_t)(w1 >> 32) == tg && (uint32_t)(w2 >> 32) == tg;
             }
             if (__all(ok)) {
-              if (need) Ush[urow(j) + lane] = (T)join_words(a0, a1);
+              upad[par][lane] = need ? (T)join_words(a0, a1) : T(0);
               if (lane == 63) {
                 gv = join_words(w0, w1);
                 gr = (int)(uint32_t)w2;
@@ -345,7 +349,7 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
         }
       }
       __syncthreads();
-      uu = Ush + urow(j);
+      uu = upad[par];
       prow = s_prow[par];
     }
     // the pivot row becomes U row j; every other active row takes its multiplier and the rank-1 update
@@ -356,24 +360,35 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
         pivot = true;
       } else {
         if (pos == j) pos = prow;
-        const T piv = uu[0];
+        const T piv = uu[0];  // the same for the whole workgroup
         T l = x[0];
         if (piv != T(0)) {
           l = x[0] / piv;
+          // uu is zero beyond the active part, so x - l * 0 moves the multipliers (and the unused tail) along
+          // unchanged; only a row that already holds an infinite or NaN value (then l * 0 is NaN) differs from
+          // the row-moving algorithm, in that row's earlier multipliers
 #pragma unroll
           for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1] - l * uu[i + 1];
         } else {
 #pragma unroll
           for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1];
         }
-        Lsh[j * rpb + tid] = l;
+        x[LU_NB - 1] = l;
       }
     }
   }
   __syncthreads();
-  if (mine)
-    for (int c = 0; c < nbw; ++c)
-      A[(k + pos) + (k + c) * lda] = (pivot && c >= pos) ? Ush[urow(pos) + (c - pos)] : Lsh[c * rpb + tid];
+  if (mine) {
+    // a pivot row (chosen at step pos): U columns pos.. in x[0..), multiplier c < pos in x[64 - pos + c]; any
+    // other row: multiplier c in x[64 - nbw + c]
+    const int lead = pivot ? LU_NB - pos : LU_NB - nbw;  // register holding multiplier 0
+    T* const out = A + (k + pos) + k * lda;
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) {
+      const int c = i >= lead ? i - lead : (pivot && i < nbw - pos ? pos + i : -1);
+      if (c >= 0) out[c * lda] = x[i];
+    }
+  }
   if (b == 0 && tid < nbw) ipiv[k + tid] = (int32_t)(k + s_piv[tid]);
   if (b == 0 && tid == 0 && s_info != 0 && *info == 0) *info = s_info;
   // block 0, wave 0: the panel's swaps as one row permutation of the touched rows (rows j < nbw and the pivot
@@ -482,6 +497,49 @@ template <class T>
 __global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw,
                                                     const int* __restrict__ perm) {
   laswp_body<T>(A, lda, ncols, k, nbw, perm, (int64_t)blockIdx.x);
+}
+
+// The interchanges of one panel in the columns [c0, c1) only (the panel's own columns excepted).
+template <class T>
+__global__ void __launch_bounds__(256) laswp_range_kernel(T* A, int64_t lda, int64_t c0, int64_t c1, int64_t k,
+                                                          int nbw, const int* __restrict__ perm) {
+  const int64_t c = c0 + (int64_t)blockIdx.x * 2 + threadIdx.x / 128;
+  const int t = threadIdx.x % 128;
+  const int cnt = perm[0];
+  const bool act = c < c1 && !(c >= k && c < k + nbw) && t < cnt;
+  T v = T(0);
+  int dst = 0;
+  if (act) {
+    dst = perm[1 + 2 * t];
+    v = A[(k + perm[2 + 2 * t]) + c * lda];
+  }
+  __syncthreads();
+  if (act) A[(k + dst) + c * lda] = v;
+}
+
+// The interchanges of the np panels of the block [K0, K1) (panel p at column K0 + LU_NB p, its permutation at
+// perms + p * PERM_STRIDE), in panel order, in the columns outside the block: [0, K0) and [K1, ncols).
+template <class T>
+__global__ void __launch_bounds__(256) laswp_deferred_kernel(T* A, int64_t lda, int64_t K0, int64_t K1, int64_t ncols,
+                                                             int np, const int* __restrict__ perms) {
+  const int64_t idx = (int64_t)blockIdx.x * 2 + threadIdx.x / 128;
+  const int64_t c = idx < K0 ? idx : idx + (K1 - K0);
+  const int t = threadIdx.x % 128;
+  for (int p = 0; p < np; ++p) {
+    const int* perm = perms + p * PERM_STRIDE;
+    const int64_t k = K0 + (int64_t)p * LU_NB;
+    const int cnt = perm[0];
+    const bool act = c < ncols && t < cnt;
+    T v = T(0);
+    int dst = 0;
+    if (act) {
+      dst = perm[1 + 2 * t];
+      v = A[(k + perm[2 + 2 * t]) + c * lda];
+    }
+    __syncthreads();
+    if (act) A[(k + dst) + c * lda] = v;
+    __syncthreads();
+  }
 }
 
 // The row interchanges of the panels at column k of several matrices (job j owns workgroups [b0, ...)).
@@ -1097,13 +1155,84 @@ struct TrsmList {
   }
 };
 
+// The trailing update C -= L U of an outer block, in two steps so that disjoint column ranges of C can be updated
+// by different streams from one preparation: trailing_prepare forms what every range reads (for the FP16 update,
+// the column scales and the packed FP16 copies of L and of U's N columns, in the workspace after sigma),
+// trailing_cols updates columns [j0, j1) of C.
+template <class T>
+struct Trailing {
+  T* L;
+  T* U;
+  T* C;
+  int64_t lda, M, N, K;
+  float* sigma;
+  bool fp16;
+  _Float16* Lh = nullptr;
+  _Float16* Uh = nullptr;
+};
+
+template <class T>
+Trailing<T> trailing_prepare(T* L, T* U, T* C, int64_t lda, int64_t M, int64_t N, int64_t K, float* sigma, bool fp16,
+                             hipStream_t st) {
+  Trailing<T> t{L, U, C, lda, M, N, K, sigma, fp16 && std::is_same<T, float>::value && M > 0 && N > 0 && K > 0};
+  if (t.fp16) {
+    // packed FP16 copies after the column scales in the same workspace region, 256-byte aligned
+    const uintptr_t pk = (reinterpret_cast<uintptr_t>(sigma) + sizeof(float) * (size_t)(M > N ? M : N) + 255) &
+                         ~(uintptr_t)255;
+    t.Lh = reinterpret_cast<_Float16*>(pk);
+    t.Uh = t.Lh + M * GH_LD;
+    hipLaunchKernelGGL(pack_l_f16_kernel, dim3(nblk(M, 32)), dim3(256), 0, st, (const float*)L, lda, M, K, t.Lh);
+    hipLaunchKernelGGL(pack_u_f16_kernel, dim3(nblk(N, 4)), dim3(256), 0, st, (const float*)U, lda, N, K, sigma,
+                       t.Uh);
+  }
+  return t;
+}
+
+template <class T>
+void trailing_cols(const Trailing<T>& t, int64_t j0, int64_t j1, hipStream_t st) {
+  const int64_t N = j1 - j0;
+  if (t.M <= 0 || N <= 0 || t.K <= 0) return;
+  if (!t.fp16) {
+    gemm_sub<T>(t.L, t.lda, t.U + j0 * t.lda, t.lda, t.C + j0 * t.lda, t.lda, t.M, N, t.K, st);
+    return;
+  }
+  const int gx = (int)nblk(t.M, GB_M), gy = (int)nblk(N, GB_N);
+  hipLaunchKernelGGL(gemm_sub_f16_kernel, dim3(gx * gy), dim3(256), 0, st, (const _Float16*)t.Lh,
+                     (const _Float16*)(t.Uh + j0 * GH_LD), reinterpret_cast<float*>(t.C + j0 * t.lda), t.lda, t.M, N,
+                     t.K, (const float*)(t.sigma + j0), gx, gy);
+}
+
+// Second stream and events of one factorization's look-ahead (created per call, released when the call returns;
+// the runtime frees them once their work has completed).
+struct LookAhead {
+  hipStream_t s2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  hipError_t init() {
+    hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+    return e;
+  }
+  ~LookAhead() {
+    if (fork) (void)hipEventDestroy(fork);
+    if (join) (void)hipEventDestroy(join);
+    if (s2) (void)hipStreamDestroy(s2);
+  }
+};
+
+// Look-ahead is on unless CDJ_LU_LOOKAHEAD=0 (read per call; the factors are the same either way).
+inline bool lookahead_enabled() {
+  const char* v = std::getenv("CDJ_LU_LOOKAHEAD");
+  return !(v && v[0] == '0');
+}
+
 }  // namespace
 
 size_t lu_workspace_bytes(int64_t n) {
   const int64_t nn = n > 0 ? n : 1;
-  // the panels' slots and records, flags and the panel permutation; then n column scales and the two packed FP16
+  // the panels' slots and records, flags and the panels' permutations; then n column scales and the two packed FP16
   // operand copies of the FP16 trailing update (n x GH_LD halves each)
-  return sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) + sizeof(int) * (2 + 1 + 4 * LU_NB) +
+  return sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) + sizeof(int) * (2 + PERM_INTS) +
          f16_pack_offset(nn) + 2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
 }
 
@@ -1112,35 +1241,15 @@ size_t lu_workspace_bytes(int64_t n) {
 // date with the earlier panels of its block (left-looking inside the block); the rest of the matrix then gets a
 // single rank-lu_outer(n) update (right-looking across blocks), so the trailing matrix is read and written once
 // per outer block.
-template <class T>
-void trailing_update(T* L, T* U, T* C, int64_t lda, int64_t M, int64_t N, int64_t K, float* sigma, bool fp16,
-                     hipStream_t st);
-
-template <>
-void trailing_update<double>(double* L, double* U, double* C, int64_t lda, int64_t M, int64_t N, int64_t K, float*,
-                             bool, hipStream_t st) {
-  gemm_sub<double>(L, lda, U, lda, C, lda, M, N, K, st);
-}
-
-template <>
-void trailing_update<float>(float* L, float* U, float* C, int64_t lda, int64_t M, int64_t N, int64_t K,
-                            float* sigma, bool fp16, hipStream_t st) {
-  if (!fp16 || M <= 0 || N <= 0 || K <= 0) {
-    gemm_sub<float>(L, lda, U, lda, C, lda, M, N, K, st);
-    return;
-  }
-  // packed FP16 copies after the column scales in the same workspace region, 256-byte aligned
-  const uintptr_t pk = (reinterpret_cast<uintptr_t>(sigma) + sizeof(float) * (size_t)(M > N ? M : N) + 255) &
-                       ~(uintptr_t)255;
-  _Float16* Lh = reinterpret_cast<_Float16*>(pk);
-  _Float16* Uh = Lh + M * GH_LD;
-  hipLaunchKernelGGL(pack_l_f16_kernel, dim3(nblk(M, 32)), dim3(256), 0, st, (const float*)L, lda, M, K, Lh);
-  hipLaunchKernelGGL(pack_u_f16_kernel, dim3(nblk(N, 4)), dim3(256), 0, st, (const float*)U, lda, N, K, sigma, Uh);
-  const int gx = (int)nblk(M, GB_M), gy = (int)nblk(N, GB_N);
-  hipLaunchKernelGGL(gemm_sub_f16_kernel, dim3(gx * gy), dim3(256), 0, st, (const _Float16*)Lh, (const _Float16*)Uh, C,
-                     lda, M, N, K, (const float*)sigma, gx, gy);
-}
-
+//
+// Look-ahead: the trailing update of block b is split.  The next block's columns are updated first on the
+// caller's stream, whose next work is that block's panels; the rest runs at the same time on a second stream.
+// The panels' row interchanges reach only their own block's columns then; the interchanges in every other column
+// (among them block b's multipliers, which the second stream's product reads) wait for that product and are
+// applied panel by panel afterwards.  Each entry goes through the same operations as without look-ahead, so the
+// factors are the same bit for bit.  A panel launched beside the running product is an ordinary launch: its
+// workgroups take CUs as the product's workgroups finish (a panel workgroup needs 6.5 KiB of LDS and fits beside
+// one product workgroup), and its waits are bounded as always.
 template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
                           hipStream_t st, bool fp16_update) {
@@ -1151,8 +1260,8 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   w += xbytes;
   unsigned* bar = reinterpret_cast<unsigned*>(w);  // unused word, then the panels' wait-overrun flag
   int* bar_err = reinterpret_cast<int*>(bar + 1);
-  int* perm = bar_err + 1;                          // 1 + 4 LU_NB ints: the last panel's row permutation
-  float* sigma = reinterpret_cast<float*>(perm + 1 + 4 * LU_NB);  // n column scales (FP16 trailing update)
+  int* perms = bar_err + 1;                         // the row permutations of one outer block's panels
+  float* sigma = reinterpret_cast<float*>(perms + PERM_INTS);  // n column scales (FP16 trailing update)
   if (ncols > n) fp16_update = false;
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
   if (e != hipSuccess) return e;
@@ -1161,22 +1270,27 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   // all ones: slot tag 4095 and record tag 2^32 - 1 match no column of this factorization
   e = hipMemsetAsync(slots, 0xFF, xbytes, st);
   if (e != hipSuccess) return e;
-  // raise the dynamic LDS limit once per process (up to 144 KiB per panel block); thread-safe initialisation
-  static const hipError_t lds_set = [] {
-    const int lds = (int)(sizeof(T) * ((size_t)MAX_RPB * LU_NB + USH));
-    hipError_t r = hipFuncSetAttribute((const void*)lu_panel_kernel<T, 2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       lds);
-    if (r == hipSuccess)
-      r = hipFuncSetAttribute((const void*)lu_panel_kernel<T, MAX_P / 64>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    return r;
-  }();
-  if (lds_set != hipSuccess) return lds_set;
   const int64_t ob = lu_outer(n);
+  // panel workgroups that fit on the device at once (ordinary launches beside a running product rely on it)
+  static const int panel_capacity = [] {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_kernel<T, MAX_P / 64>, PT, 0) !=
+        hipSuccess)
+      return 0;
+    return per_cu * cus;
+  }();
+  LookAhead la;
+  const bool ahead = lookahead_enabled() && n > ob && panel_capacity > 0;
+  if (ahead && (e = la.init()) != hipSuccess) return e;
+  bool pending = false;  // the previous block's update of the columns right of this block runs on la.s2
   for (int64_t K0 = 0; K0 < n; K0 += ob) {
     const int64_t K1 = std::min<int64_t>(n, K0 + ob);
-    for (int64_t k = K0; k < K1; k += LU_NB) {
+    int np = 0;
+    for (int64_t k = K0; k < K1; k += LU_NB, ++np) {
       const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
+      int* perm = perms + np * PERM_STRIDE;
       if (k > K0) {
         // this panel's columns: U rows from the block's earlier panels, then their update of rows [k, n)
         block_trsm<T>(A, lda, K0, k, k, nbw, st);
@@ -1186,43 +1300,65 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       const int P = (int)((m + MAX_RPB - 1) / MAX_RPB);
       int rpb = (int)((m + P - 1) / P);
       if (P > MAX_P) return hipErrorInvalidValue;
-      size_t shm = sizeof(T) * ((size_t)rpb * LU_NB + USH);
       void* args[] = {&A, (void*)&lda, &k, (void*)&m, (void*)&nbw, &rpb, &ipiv, &info, &xw, &slots, &bar_err, &perm};
       const void* kern = P <= 128 ? (const void*)lu_panel_kernel<T, 2> : (const void*)lu_panel_kernel<T, MAX_P / 64>;
-      e = hipLaunchCooperativeKernel(kern, dim3(P), dim3(PT), args, shm, st);
+      if (pending && P <= panel_capacity)
+        e = hipLaunchKernel(kern, dim3(P), dim3(PT), args, 0, st);
+      else
+        e = hipLaunchCooperativeKernel(kern, dim3(P), dim3(PT), args, 0, st);
       if (e != hipSuccess) return e;
-      if (ncols > nbw)
+      if (pending) {
+        if (K1 - K0 > nbw)
+          hipLaunchKernelGGL(laswp_range_kernel<T>, dim3(nblk(K1 - K0, 2)), dim3(256), 0, st, A, lda, K0, K1, k, nbw,
+                             (const int*)perm);
+      } else if (ncols > nbw) {
         hipLaunchKernelGGL(laswp_kernel<T>, dim3(nblk(ncols, 2)), dim3(256), 0, st, A, lda, ncols, k, nbw,
                            (const int*)perm);
+      }
     }
+    if (pending) {
+      // the previous block's product is done: this block's interchanges in all other columns
+      if ((e = hipStreamWaitEvent(st, la.join, 0)) != hipSuccess) return e;
+      if (ncols - (K1 - K0) > 0)
+        hipLaunchKernelGGL(laswp_deferred_kernel<T>, dim3(nblk(ncols - (K1 - K0), 2)), dim3(256), 0, st, A, lda, K0,
+                           K1, ncols, np, (const int*)perms);
+      pending = false;
+    }
+    if (K1 >= ncols) break;
     // U12 of the block, then the rank-(K1 - K0) update of everything right of and below it
     block_trsm<T>(A, lda, K0, K1, K1, ncols - K1, st);
-    trailing_update<T>(A + K1 + K0 * lda, A + K0 + K1 * lda, A + K1 + K1 * lda, lda, n - K1, ncols - K1, K1 - K0,
-                       sigma, fp16_update, st);
+    const Trailing<T> t = trailing_prepare<T>(A + K1 + K0 * lda, A + K0 + K1 * lda, A + K1 + K1 * lda, lda, n - K1,
+                                              ncols - K1, K1 - K0, sigma, fp16_update, st);
+    const int64_t K2 = std::min<int64_t>(n, K1 + ob);  // the next block's columns are [K1, K2)
+    if (ahead && K1 < n && K2 < ncols) {
+      if ((e = hipEventRecord(la.fork, st)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(la.s2, la.fork, 0)) != hipSuccess) return e;
+      trailing_cols<T>(t, K2 - K1, ncols - K1, la.s2);
+      if ((e = hipEventRecord(la.join, la.s2)) != hipSuccess) return e;
+      trailing_cols<T>(t, 0, K2 - K1, st);
+      pending = true;
+    } else {
+      trailing_cols<T>(t, 0, ncols - K1, st);
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  if (pending && (e = hipStreamWaitEvent(st, la.join, 0)) != hipSuccess) return e;
   hipLaunchKernelGGL(barrier_status_kernel, dim3(1), dim3(1), 0, st, (const int*)bar_err, info);
   return hipGetLastError();
 }
 
-// Co-resident workgroups of the batched panel kernel (what one cooperative launch may hold); the first call also
-// raises the kernel's dynamic LDS limit.  Thread-safe one-time set-up.
+// Co-resident workgroups of the batched panel kernel (what one cooperative launch may hold).  Thread-safe one-time
+// set-up.
 template <class T>
 int lu_batched_capacity() {
   static const int capacity = [] {
-    const int lds_max = (int)(sizeof(T) * ((size_t)MAX_RPB * LU_NB + USH));
-    hipError_t r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, 2>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
-    if (r == hipSuccess)
-      r = hipFuncSetAttribute((const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lds_max);
     int dev = 0, cus = 0, per_cu = 0;
-    if (r == hipSuccess) r = hipGetDevice(&dev);
+    hipError_t r = hipGetDevice(&dev);
     if (r == hipSuccess) r = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (r == hipSuccess)
       r = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)lu_panel_batched_kernel<T, MAX_P / 64>,
-                                                       PT, lds_max);
+                                                       PT, 0);
     return r == hipSuccess ? per_cu * cus : 0;
   }();
   return capacity;
@@ -1259,7 +1395,7 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
     v.bar = reinterpret_cast<unsigned*>(w);
     v.bar_err = reinterpret_cast<int*>(v.bar + 1);
     v.perm = v.bar_err + 1;
-    v.sigma = reinterpret_cast<float*>(v.perm + 1 + 4 * LU_NB);
+    v.sigma = reinterpret_cast<float*>(v.perm + PERM_INTS);
     hipLaunchKernelGGL(lu_init_kernel, dim3(64), dim3(256), 0, st, J.info, v.bar, v.slots,
                        (int64_t)(v.xbytes / sizeof(unsigned long long)));
     nmax = std::max(nmax, J.n);
@@ -1303,7 +1439,7 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
       while (a < act.size()) {
         PanelBatch<T> pb;
         pb.njobs = 0;
-        int blocks = 0, rpb_max = 1, pmax = 1;
+        int blocks = 0, pmax = 1;
         for (; a < act.size() && pb.njobs < PANEL_JOBS; ++a) {
           const LuJob<T>& J = jobs[act[a]];
           const int64_t m = J.n - k;
@@ -1325,16 +1461,14 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
           q.bar_err = wv[act[a]].bar_err;
           q.perm = wv[act[a]].perm;
           blocks += P;
-          rpb_max = std::max(rpb_max, q.rpb);
           pmax = std::max(pmax, P);
         }
         if (blocks > capacity) return hipErrorCooperativeLaunchTooLarge;
-        const size_t shm = sizeof(T) * ((size_t)rpb_max * LU_NB + USH);
         int64_t kk = k;
         void* args[] = {&pb, &kk};
         const void* kern = pmax <= 128 ? (const void*)lu_panel_batched_kernel<T, 2>
                                        : (const void*)lu_panel_batched_kernel<T, MAX_P / 64>;
-        hipError_t e = hipLaunchCooperativeKernel(kern, dim3(blocks), dim3(PT), args, shm, st);
+        hipError_t e = hipLaunchCooperativeKernel(kern, dim3(blocks), dim3(PT), args, 0, st);
         if (e != hipSuccess) return e;
       }
       // the panels' row interchanges in the other columns, SWAP_JOBS matrices per launch
@@ -1380,10 +1514,11 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
       const LuJob<T>& J = jobs[j];
       if (J.n <= K0) continue;
       const int64_t K1 = std::min<int64_t>(J.n, K0 + LU_OUTER), ld = J.lda;
-      if (fp16_update && J.ncols == J.n)
-        trailing_update<T>(J.A + K1 + K0 * ld, J.A + K0 + K1 * ld, J.A + K1 + K1 * ld, ld, J.n - K1, J.ncols - K1,
-                           K1 - K0, wv[j].sigma, true, st);
-      else
+      if (fp16_update && J.ncols == J.n) {
+        const Trailing<T> t = trailing_prepare<T>(J.A + K1 + K0 * ld, J.A + K0 + K1 * ld, J.A + K1 + K1 * ld, ld,
+                                                  J.n - K1, J.ncols - K1, K1 - K0, wv[j].sigma, true, st);
+        trailing_cols<T>(t, 0, J.ncols - K1, st);
+      } else
         gl.add(J.A + K1 + K0 * ld, ld, J.A + K0 + K1 * ld, ld, J.A + K1 + K1 * ld, ld, J.n - K1, J.ncols - K1, K1 - K0);
     }
     gl.launch(st);
