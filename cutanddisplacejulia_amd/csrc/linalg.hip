@@ -37,6 +37,7 @@ constexpr int64_t LU_BIG = 32768;  // (measured: 512 is faster at n = 48672, slo
 inline int64_t lu_outer(int64_t n) { return n >= LU_BIG ? LU_OUTER_BIG : LU_OUTER; }
 constexpr int PERM_STRIDE = 1 + 4 * LU_NB;                        // ints of one panel's row permutation
 constexpr int PERM_INTS = (LU_OUTER_BIG / LU_NB) * PERM_STRIDE;   // the permutations of one outer block's panels
+constexpr int PERM_SETS = 2;  // two blocks' permutations (one block's are still read while the next is factored)
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -101,6 +102,13 @@ __device__ __forceinline__ long long readlane64(long long v, int l) {
   return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 __device__ __forceinline__ long long kmax64(long long a, long long b) { return a > b ? a : b; }
+// value of v in lane l (l a compile-time constant), through scalar registers
+__device__ __forceinline__ float bcast_lane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast_lane(double v, int l) {
+  return __longlong_as_double(readlane64(__double_as_longlong(v), l));
+}
 
 // wave-wide maximum, in every lane: DPP within rows of 16 lanes, then the four row maxima
 __device__ __forceinline__ long long wave_max(long long k) {
@@ -127,8 +135,7 @@ template <class T, int QPL>
 __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb, int32_t* ipiv,
                                            int* info, unsigned long long* xw, unsigned long long* slots, int* bar_err,
                                            int* perm, const int P, const int b) {
-  __shared__ T stage[2][PT / 64][LU_NB];     // each wave's candidate row, zero beyond the active part, by parity
-  __shared__ T upad[2][LU_NB];               // the pivot row from the exchange (P > 1), zero beyond the active part
+  __shared__ T upad[2][LU_NB];               // the pivot row, zero beyond the active part, by column parity
   __shared__ long long wk[2][PT / 64];       // each wave's candidate key (-1: none) and position
   __shared__ int wp[2][PT / 64];
   __shared__ int s_prow[2];
@@ -158,9 +165,10 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
     // wave candidate: largest key, ties to the lowest position
     const long long key = done ? -1ll : (long long)__double_as_longlong(fabs((double)x[0]));
     const long long kw = wave_max(key);
+    int wl = 0;  // this wave's candidate lane
     if (kw >= 0) {
       unsigned long long msk = __ballot(key == kw);
-      int wl = __ffsll((long long)msk) - 1;
+      wl = __ffsll((long long)msk) - 1;
       if (__popcll(msk) > 1) {
         int bp = __builtin_amdgcn_readlane(pos, wl);
         for (msk &= msk - 1; msk; msk &= msk - 1) {
@@ -173,12 +181,6 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
         }
       }
       if (lane == wl) {
-#pragma unroll
-        for (int c8 = 0; c8 < LU_NB; c8 += 8)
-          if (c8 < na) {
-#pragma unroll
-            for (int i = c8; i < c8 + 8; ++i) stage[par][wave][i] = x[i];
-          }
         wk[par][wave] = kw;
         wp[par][wave] = pos;
       }
@@ -200,23 +202,29 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
         bw = w;
       }
     }
-    const T* uu;  // pivot row, active part
-    int prow;
-    if (P == 1) {
-      prow = bk >= 0 ? bpos : j;
-      if (wave == 0) upad[par][lane] = lane < na ? stage[par][bw][lane] : T(0);
-      if (tid == 0) {
-        s_piv[j] = prow;
-        const double gv = bk >= 0 ? __longlong_as_double(bk) : -1.0;
-        if (!(gv > 0.0) && s_info == 0) s_info = (int)(k + j + 1);
+    // the candidate's wave stages its row in upad (zero beyond the active part: the entries the candidate lane
+    // writes past it are overwritten by the lanes >= na after it, in order, as one wave's LDS accesses are) and,
+    // with several workgroups, runs the exchange; only the winning wave's lane writes a row to LDS
+    if (wave == bw) {
+      if (bk >= 0 && lane == wl) {
+#pragma unroll
+        for (int c8 = 0; c8 < LU_NB; c8 += 8)
+          if (c8 < na) {
+#pragma unroll
+            for (int i = c8; i < c8 + 8; ++i) upad[par][i] = x[i];
+          }
       }
-      __syncthreads();
-      uu = upad[par];
-    } else {
-      if (wave == 0) {
+      if (lane >= na) upad[par][lane] = T(0);
+      if (P == 1) {
+        if (lane == 0) {
+          s_piv[j] = bk >= 0 ? bpos : j;
+          const double gv = bk >= 0 ? __longlong_as_double(bk) : -1.0;
+          if (!(gv > 0.0) && s_info == 0) s_info = (int)(k + j + 1);
+        }
+      } else {
         unsigned long long* rec = xw + (size_t)(par * MAX_P + b) * XREC;
         if (bk >= 0 && lane < na) {
-          const unsigned long long xv = (unsigned long long)__double_as_longlong((double)stage[par][bw][lane]);
+          const unsigned long long xv = (unsigned long long)__double_as_longlong((double)upad[par][lane]);
           put_tagged(rec + 3 + 2 * lane, tg, (uint32_t)xv);
           put_tagged(rec + 4 + 2 * lane, tg, (uint32_t)(xv >> 32));
         }
@@ -245,28 +253,16 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
           }
           __builtin_amdgcn_s_sleep(1);
         }
-        uint64_t kmax = 0;
-        int qmax = -1;
+        // the largest truncated key, ties to the lowest workgroup: one DPP maximum of (key | 1023 - q)
+        long long vq = -1;
 #pragma unroll
         for (int i = 0; i < QPL; ++i) {
           const int q = lane + 64 * i;
-          if (q < P && !(wq[i] & SLOT_NONE)) {
-            const uint64_t kk = wq[i] & ~SLOT_TAG;
-            if (qmax < 0 || kk > kmax) {
-              kmax = kk;
-              qmax = q;
-            }
-          }
+          if (q < P && !(wq[i] & SLOT_NONE)) vq = kmax64(vq, (long long)((wq[i] & ~SLOT_TAG) | (uint64_t)(1023 - q)));
         }
-        for (int off = 32; off > 0; off >>= 1) {
-          const uint64_t k2 = __shfl_xor(kmax, off);
-          const int q2 = __shfl_xor(qmax, off);
-          if (q2 >= 0 && (qmax < 0 || k2 > kmax || (k2 == kmax && q2 < qmax))) {
-            kmax = k2;
-            qmax = q2;
-          }
-        }
-        int gw = qmax;
+        const long long vmax = wave_max(vq);
+        int gw = vmax >= 0 ? 1023 - (int)(vmax & 1023) : -1;
+        const uint64_t kmax = vmax >= 0 ? ((uint64_t)vmax & ~SLOT_TAG) : 0;
         double gv = -1.0;
         int gr = -1;
         int ntie = 0;
@@ -296,18 +292,23 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
                 }
                 __builtin_amdgcn_s_sleep(1);
               }
-              const double vq = join_words(w0, w1);
+              const double vv = join_words(w0, w1);
               const int rq = (int)(uint32_t)w2;
-              if (gw < 0 || vq > gv || (vq == gv && rq < gr)) {
-                gv = vq;
+              if (gw < 0 || vv > gv || (vv == gv && rq < gr)) {
+                gv = vv;
                 gr = rq;
                 gw = q;
               }
             }
           }
         }
-        // the winner's record in one batch: lane i its row entry i, lane 63 also key and position
-        if (gw >= 0) {
+        if (gw == b) {  // this workgroup's candidate: its row is in upad already
+          if (ntie <= 1) {
+            gv = __longlong_as_double(bk);
+            gr = bpos;
+          }
+        } else if (gw >= 0) {
+          // the winner's record in one batch: lane i its row entry i, lane 63 also key and position
           const unsigned long long* rw = xw + (size_t)(par * MAX_P + gw) * XREC;
           const bool need = lane < na;
           for (long long t = 0;; ++t) {
@@ -348,10 +349,14 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
           if (!(gv > 0.0) && s_info == 0) s_info = (int)(k + j + 1);
         }
       }
-      __syncthreads();
-      uu = upad[par];
-      prow = s_prow[par];
     }
+    __syncthreads();
+    const int prow = P == 1 ? (bk >= 0 ? bpos : j) : s_prow[par];
+    // the pivot row, lane i holding its column j + i (zero beyond the active part); its entries reach the update
+    // as scalar operands (lane reads), not as 64-lane LDS broadcasts.  u is loaded and its lane 0 read by every
+    // lane before the divergent update below, which reads the other lanes of the same register.
+    const T u = upad[par][lane];
+    const T piv = bcast_lane(u, 0);
     // the pivot row becomes U row j; every other active row takes its multiplier and the rank-1 update
     if (!done) {
       if (pos == prow) {
@@ -360,15 +365,25 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
         pivot = true;
       } else {
         if (pos == j) pos = prow;
-        const T piv = uu[0];  // the same for the whole workgroup
         T l = x[0];
         if (piv != T(0)) {
           l = x[0] / piv;
-          // uu is zero beyond the active part, so x - l * 0 moves the multipliers (and the unused tail) along
+          // u is zero beyond the active part, so x - l * 0 moves the multipliers (and the unused tail) along
           // unchanged; only a row that already holds an infinite or NaN value (then l * 0 is NaN) differs from
           // the row-moving algorithm, in that row's earlier multipliers
+          // the first half of the row's entries reach every lane as lane reads into scalar registers (VALU),
+          // the second half as LDS broadcasts: the two paths run side by side, each carrying half the row
 #pragma unroll
-          for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1] - l * uu[i + 1];
+          for (int i0 = 0; i0 < LU_NB / 2; i0 += 8) {
+            T ub[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) ub[e] = bcast_lane(u, i0 + e + 1);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[i0 + e] = x[i0 + e + 1] - l * ub[e];
+          }
+          const T* uh = upad[par];
+#pragma unroll
+          for (int i = LU_NB / 2; i + 1 < LU_NB; ++i) x[i] = x[i + 1] - l * uh[i + 1];
         } else {
 #pragma unroll
           for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1];
@@ -517,19 +532,18 @@ __global__ void __launch_bounds__(256) laswp_range_kernel(T* A, int64_t lda, int
   if (act) A[(k + dst) + c * lda] = v;
 }
 
-// The interchanges of the np panels of the block [K0, K1) (panel p at column K0 + LU_NB p, its permutation at
-// perms + p * PERM_STRIDE), in panel order, in the columns outside the block: [0, K0) and [K1, ncols).
+// The interchanges of the np panels of the block starting at column K0 (panel p at column K0 + LU_NB p, its
+// permutation at perms + p * PERM_STRIDE), in panel order, in the columns [c0, c1) (outside the block).
 template <class T>
-__global__ void __launch_bounds__(256) laswp_deferred_kernel(T* A, int64_t lda, int64_t K0, int64_t K1, int64_t ncols,
+__global__ void __launch_bounds__(256) laswp_deferred_kernel(T* A, int64_t lda, int64_t K0, int64_t c0, int64_t c1,
                                                              int np, const int* __restrict__ perms) {
-  const int64_t idx = (int64_t)blockIdx.x * 2 + threadIdx.x / 128;
-  const int64_t c = idx < K0 ? idx : idx + (K1 - K0);
+  const int64_t c = c0 + (int64_t)blockIdx.x * 2 + threadIdx.x / 128;
   const int t = threadIdx.x % 128;
   for (int p = 0; p < np; ++p) {
     const int* perm = perms + p * PERM_STRIDE;
     const int64_t k = K0 + (int64_t)p * LU_NB;
     const int cnt = perm[0];
-    const bool act = c < ncols && t < cnt;
+    const bool act = c < c1 && t < cnt;
     T v = T(0);
     int dst = 0;
     if (act) {
@@ -580,9 +594,32 @@ __device__ __forceinline__ void trsm_body(T* A, int64_t lda, int64_t k, int nbw,
   }
   __syncthreads();
   const int64_t cb = c0 + (blk * 4 + wave) * cpw;
-  for (int i = 0; i < cpw; ++i) {
-    const int64_t c = cb + i;
-    if (c >= c0 + ncols) break;
+  const int64_t ce = std::min<int64_t>(cb + cpw, c0 + ncols);  // this wave's columns [cb, ce)
+  if (nbw == LU_NB) {
+    // full block: two columns at a time, their chains interleaved, the next pair's loads in flight during the
+    // chains; steps unrolled, x[q] reaching the lanes as a lane read into scalar registers, the lanes' L11 entries
+    // from LDS independently of the chain (the same FMAs in the same order as the generic loop below)
+    auto ld = [&](int64_t c) { return c < ce ? A[k + lane + c * lda] : T(0); };
+    T na0 = ld(cb), na1 = ld(cb + 1);
+    for (int64_t c = cb; c < ce; c += 2) {
+      T x0 = na0, x1 = na1;
+      na0 = ld(c + 2);
+      na1 = ld(c + 3);
+#pragma unroll
+      for (int q = 0; q < LU_NB - 1; ++q) {
+        const T lq = LT[q][lane];
+        const T x0q = bcast_lane(x0, q), x1q = bcast_lane(x1, q);
+        if (lane > q) {
+          x0 -= lq * x0q;
+          x1 -= lq * x1q;
+        }
+      }
+      A[k + lane + c * lda] = x0;
+      if (c + 1 < ce) A[k + lane + (c + 1) * lda] = x1;
+    }
+    return;
+  }
+  for (int64_t c = cb; c < ce; ++c) {
     T* col = A + k + c * lda;
     T x = lane < nbw ? col[lane] : T(0);
     for (int q = 0; q < nbw - 1; ++q) {
@@ -598,6 +635,7 @@ __global__ void __launch_bounds__(256) trsm_kernel(T* A, int64_t lda, int64_t k,
                                                    int cpw) {
   trsm_body<T>(A, lda, k, nbw, c0, ncols, cpw, (int64_t)blockIdx.x);
 }
+
 
 // C[0:M, 0:N] -= L[0:M, 0:K] * U[0:K, 0:N], column-major with leading dimensions ldl, ldu, ldc, any K.
 // 128 x 128 tile per workgroup (4 waves in 2 x 2, 64 x 64 each = 4 x 4 MFMA tiles).  K is streamed in slices of
@@ -1096,7 +1134,7 @@ void gemm_sub(const T* L, int64_t ldl, const T* U, int64_t ldu, T* C, int64_t ld
 template <class T>
 void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t nc, hipStream_t st) {
   if (nc <= 0) return;
-  const int cpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, nc / 8192));
+  const int cpw = (int)std::min<int64_t>(16, std::max<int64_t>(1, nc / 1024));  // about 1024 waves for wide X
   for (int64_t s = r0; s < r1; s += LU_NB) {
     const int nb = (int)std::min<int64_t>(LU_NB, r1 - s);
     hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nc, 4 * cpw)), dim3(256), 0, st, A, lda, s, nb, c0, nc, cpw);
@@ -1206,16 +1244,18 @@ void trailing_cols(const Trailing<T>& t, int64_t j0, int64_t j1, hipStream_t st)
 // the runtime frees them once their work has completed).
 struct LookAhead {
   hipStream_t s2 = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, panels = nullptr;
   hipError_t init() {
     hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&panels, hipEventDisableTiming);
     return e;
   }
   ~LookAhead() {
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
+    if (panels) (void)hipEventDestroy(panels);
     if (s2) (void)hipStreamDestroy(s2);
   }
 };
@@ -1232,7 +1272,7 @@ size_t lu_workspace_bytes(int64_t n) {
   const int64_t nn = n > 0 ? n : 1;
   // the panels' slots and records, flags and the panels' permutations; then n column scales and the two packed FP16
   // operand copies of the FP16 trailing update (n x GH_LD halves each)
-  return sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) + sizeof(int) * (2 + PERM_INTS) +
+  return sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) + sizeof(int) * (2 + PERM_SETS * PERM_INTS) +
          f16_pack_offset(nn) + 2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
 }
 
@@ -1260,8 +1300,8 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   w += xbytes;
   unsigned* bar = reinterpret_cast<unsigned*>(w);  // unused word, then the panels' wait-overrun flag
   int* bar_err = reinterpret_cast<int*>(bar + 1);
-  int* perms = bar_err + 1;                         // the row permutations of one outer block's panels
-  float* sigma = reinterpret_cast<float*>(perms + PERM_INTS);  // n column scales (FP16 trailing update)
+  int* perm_sets = bar_err + 1;                     // the row permutations of an outer block's panels, two sets
+  float* sigma = reinterpret_cast<float*>(perm_sets + PERM_SETS * PERM_INTS);  // n column scales (FP16 update)
   if (ncols > n) fp16_update = false;
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
   if (e != hipSuccess) return e;
@@ -1285,8 +1325,10 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   const bool ahead = lookahead_enabled() && n > ob && panel_capacity > 0;
   if (ahead && (e = la.init()) != hipSuccess) return e;
   bool pending = false;  // the previous block's update of the columns right of this block runs on la.s2
-  for (int64_t K0 = 0; K0 < n; K0 += ob) {
+  bool side = false;     // work was queued on la.s2
+  for (int64_t K0 = 0, blk = 0; K0 < n; K0 += ob, ++blk) {
     const int64_t K1 = std::min<int64_t>(n, K0 + ob);
+    int* perms = perm_sets + (blk % PERM_SETS) * PERM_INTS;
     int np = 0;
     for (int64_t k = K0; k < K1; k += LU_NB, ++np) {
       const int nbw = (int)std::min<int64_t>(LU_NB, K1 - k);
@@ -1317,11 +1359,19 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       }
     }
     if (pending) {
-      // the previous block's product is done: this block's interchanges in all other columns
+      // this block's interchanges in the other columns: those left of the block on the second stream, behind its
+      // product (nothing on this stream reads them again; this permutation set is rewritten only two blocks on,
+      // after a join that follows them), those right of it here once the product is done
+      if (K0 > 0) {
+        if ((e = hipEventRecord(la.panels, st)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(la.s2, la.panels, 0)) != hipSuccess) return e;
+        hipLaunchKernelGGL(laswp_deferred_kernel<T>, dim3(nblk(K0, 2)), dim3(256), 0, la.s2, A, lda, K0, (int64_t)0,
+                           K0, np, (const int*)perms);
+      }
       if ((e = hipStreamWaitEvent(st, la.join, 0)) != hipSuccess) return e;
-      if (ncols - (K1 - K0) > 0)
-        hipLaunchKernelGGL(laswp_deferred_kernel<T>, dim3(nblk(ncols - (K1 - K0), 2)), dim3(256), 0, st, A, lda, K0,
-                           K1, ncols, np, (const int*)perms);
+      if (ncols > K1)
+        hipLaunchKernelGGL(laswp_deferred_kernel<T>, dim3(nblk(ncols - K1, 2)), dim3(256), 0, st, A, lda, K0, K1,
+                           ncols, np, (const int*)perms);
       pending = false;
     }
     if (K1 >= ncols) break;
@@ -1337,13 +1387,17 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       if ((e = hipEventRecord(la.join, la.s2)) != hipSuccess) return e;
       trailing_cols<T>(t, 0, K2 - K1, st);
       pending = true;
+      side = true;
     } else {
       trailing_cols<T>(t, 0, ncols - K1, st);
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (pending && (e = hipStreamWaitEvent(st, la.join, 0)) != hipSuccess) return e;
+  if (side) {  // everything queued on the second stream
+    if ((e = hipEventRecord(la.join, la.s2)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(st, la.join, 0)) != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(barrier_status_kernel, dim3(1), dim3(1), 0, st, (const int*)bar_err, info);
   return hipGetLastError();
 }
@@ -1395,7 +1449,7 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
     v.bar = reinterpret_cast<unsigned*>(w);
     v.bar_err = reinterpret_cast<int*>(v.bar + 1);
     v.perm = v.bar_err + 1;
-    v.sigma = reinterpret_cast<float*>(v.perm + PERM_INTS);
+    v.sigma = reinterpret_cast<float*>(v.perm + PERM_SETS * PERM_INTS);
     hipLaunchKernelGGL(lu_init_kernel, dim3(64), dim3(256), 0, st, J.info, v.bar, v.slots,
                        (int64_t)(v.xbytes / sizeof(unsigned long long)));
     nmax = std::max(nmax, J.n);
