@@ -414,6 +414,7 @@ def main():
                     help="receiver rows per timed CPU-baseline run (5 runs, median); 1000 rows take ~1.1 s on 16 "
                          "cores")
     ap.add_argument("--no-cpu-lu", action="store_true", help="skip the host LU baseline")
+    ap.add_argument("--no-rocsolver", action="store_true", help="skip the rocSOLVER getrf comparison")
     ap.add_argument("--solve-step", type=int, default=40,
                     help="penny-crack growth step of the slip-solve comparison (40: 6534 triangles)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -529,13 +530,16 @@ def main():
         lu_tf = (2.0 / 3.0) * N3 ** 3 / t_lu / 1e12
         _log("FP64 LU+solve: %.1f ms (%.1f TFLOP/s), residual %.2e" % (1e3 * t_lu, lu_tf, res))
         lu15 = {"fp64_lu_ms": 1e3 * t_lu, "fp64_lu_tflops": lu_tf, "fp64_residual": res, "n": N3}
-        try:
-            rs = rocsolver_lu_ms(A0)
-            lu15["rocsolver_getrf_ms"] = rs
-            _log("rocSOLVER getrf (torch.linalg.lu_factor) of the same matrix: %.1f ms" % rs)
-        except Exception as e:  # noqa: BLE001  (a library comparison only)
+        if args.no_rocsolver:
             lu15["rocsolver_getrf_ms"] = None
-            _log("rocSOLVER comparison unavailable: %s" % e)
+        else:
+            try:
+                rs = rocsolver_lu_ms(A0)
+                lu15["rocsolver_getrf_ms"] = rs
+                _log("rocSOLVER getrf (torch.linalg.lu_factor) of the same matrix: %.1f ms" % rs)
+            except Exception as e:  # noqa: BLE001  (a library comparison only)
+                lu15["rocsolver_getrf_ms"] = None
+                _log("rocSOLVER comparison unavailable: %s" % e)
         if not args.no_cpu and not args.no_cpu_lu:
             cpu_lu = cpu_lu_baseline(A0, b)
             _log("CPU LU+solve (scipy/OpenBLAS, %s threads): %.0f ms" % (os.environ.get("OMP_NUM_THREADS", "all"),

@@ -38,6 +38,9 @@ static __device__ __align__(256) unsigned char scratch[CDJ_NOALLOC_SCRATCH_BYTES
 struct cdj_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream of the blocked LU's look-ahead (non-blocking, default priority; used only with `stream`-ordered
+  // work through events)
+  hipStream_t side = nullptr;
   std::string err;
   // grow-only device scratch for per-call set-up records (triangle and receiver records)
   void* scratch = nullptr;
@@ -383,8 +386,11 @@ int cdj_create(int device, cdj_ctx** out) {
   // A blocking stream: it is ordered with the legacy null stream, which is where a caller that passes stream 0
   // (a framework's default stream) has queued the work producing our inputs and consuming our outputs.
   e = hipStreamCreateWithFlags(&c->stream, hipStreamDefault);
+  // the LU's look-ahead stream: non-blocking (ordered with the caller's stream only through events)
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_create_err = hipGetErrorString(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CDJ_ERR_DEVICE;
   }
@@ -397,6 +403,7 @@ int cdj_destroy(cdj_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   // drain the stream, then release what was made on it (graphs and their buffers) before the stream itself
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   batch_plan_free(ctx->batch);
   ctx->batch = nullptr;
   if (ctx->scratch) (void)hipFree(ctx->scratch);
@@ -406,6 +413,7 @@ int cdj_destroy(cdj_ctx* ctx) {
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->ev2) (void)hipEventDestroy(ctx->ev2);
   if (ctx->ev3) (void)hipEventDestroy(ctx->ev3);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CDJ_OK;
@@ -810,7 +818,7 @@ int cdj_lu_factor_dev(cdj_ctx* ctx, double* A, int64_t n, int64_t lda, int32_t* 
   DBuf ws, info;
   HIPCHK(ws.alloc(lu_workspace_bytes(n)));
   HIPCHK(info.alloc(sizeof(int)));
-  HIPCHK(lu_factor<double>(A, n, lda, ipiv, info.as<int>(), ws.p, st));
+  HIPCHK(lu_factor<double>(A, n, lda, ipiv, info.as<int>(), ws.p, st, false, ctx->side));
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
@@ -883,7 +891,8 @@ int solve_augmented_fp64(cdj_ctx* ctx, double* AB, int64_t n, int64_t lda, int64
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, st));
-  HIPCHK(lu_factor_cols<double>(AB, n, n + nrhs, lda, ipiv.as<int32_t>(), info.as<int>(), ws.p, st));
+  HIPCHK(lu_factor_cols<double>(AB, n, n + nrhs, lda, ipiv.as<int32_t>(), info.as<int>(), ws.p, st, false,
+                                ctx->side));
   HIPCHK(tri_solve_upper<double>(AB, n, lda, AB + n * lda, nrhs, lda, st));
   HIPCHK(hipEventRecord(e1, st));
   int h = 0;
@@ -945,7 +954,7 @@ int solve_mixed32(cdj_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
   HIPCHK(info.alloc(sizeof(int)));
   convert_matrix_d2f(A, lda, A32.as<float>(), n, n, n, st);
   const bool fp16 = opts && opts->precision == CDJ_PREC_MIXED16;
-  HIPCHK(lu_factor<float>(A32.as<float>(), n, n, ipiv.as<int32_t>(), info.as<int>(), ws.p, st, fp16));
+  HIPCHK(lu_factor<float>(A32.as<float>(), n, n, ipiv.as<int32_t>(), info.as<int>(), ws.p, st, fp16, ctx->side));
   int h = 0;
   HIPCHK(hipMemcpyAsync(&h, info.p, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(lu_permutation(ipiv.as<int32_t>(), n, perm.as<int32_t>(), st));

@@ -1240,14 +1240,14 @@ void trailing_cols(const Trailing<T>& t, int64_t j0, int64_t j1, hipStream_t st)
                      t.K, (const float*)(t.sigma + j0), gx, gy);
 }
 
-// Second stream and events of one factorization's look-ahead (created per call, released when the call returns;
-// the runtime frees them once their work has completed).
+// The caller's second stream and one factorization's events for the look-ahead (the events are created per call
+// and released when it returns; the runtime frees them once the work that records them has completed).
 struct LookAhead {
   hipStream_t s2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr, panels = nullptr;
-  hipError_t init() {
-    hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+  hipError_t init(hipStream_t side) {
+    s2 = side;
+    hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&panels, hipEventDisableTiming);
     return e;
@@ -1256,7 +1256,6 @@ struct LookAhead {
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
     if (panels) (void)hipEventDestroy(panels);
-    if (s2) (void)hipStreamDestroy(s2);
   }
 };
 
@@ -1292,7 +1291,7 @@ size_t lu_workspace_bytes(int64_t n) {
 // one product workgroup), and its waits are bounded as always.
 template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
-                          hipStream_t st, bool fp16_update) {
+                          hipStream_t st, bool fp16_update, hipStream_t side) {
   char* w = static_cast<char*>(ws);
   unsigned long long* slots = reinterpret_cast<unsigned long long*>(w);  // the panels' slot words, then records
   unsigned long long* xw = slots + 2 * MAX_P;
@@ -1322,10 +1321,10 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
     return per_cu * cus;
   }();
   LookAhead la;
-  const bool ahead = lookahead_enabled() && n > ob && panel_capacity > 0;
-  if (ahead && (e = la.init()) != hipSuccess) return e;
+  const bool ahead = side != nullptr && side != st && lookahead_enabled() && n > ob && panel_capacity > 0;
+  if (ahead && (e = la.init(side)) != hipSuccess) return e;
   bool pending = false;  // the previous block's update of the columns right of this block runs on la.s2
-  bool side = false;     // work was queued on la.s2
+  bool queued = false;   // work was queued on la.s2
   for (int64_t K0 = 0, blk = 0; K0 < n; K0 += ob, ++blk) {
     const int64_t K1 = std::min<int64_t>(n, K0 + ob);
     int* perms = perm_sets + (blk % PERM_SETS) * PERM_INTS;
@@ -1387,14 +1386,14 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       if ((e = hipEventRecord(la.join, la.s2)) != hipSuccess) return e;
       trailing_cols<T>(t, 0, K2 - K1, st);
       pending = true;
-      side = true;
+      queued = true;
     } else {
       trailing_cols<T>(t, 0, ncols - K1, st);
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (side) {  // everything queued on the second stream
+  if (queued) {  // everything queued on the second stream
     if ((e = hipEventRecord(la.join, la.s2)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(st, la.join, 0)) != hipSuccess) return e;
   }
@@ -1586,8 +1585,8 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
 
 template <class T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st,
-                     bool fp16_update) {
-  return lu_factor_cols<T>(A, n, n, lda, ipiv, info, ws, st, fp16_update);
+                     bool fp16_update, hipStream_t side) {
+  return lu_factor_cols<T>(A, n, n, lda, ipiv, info, ws, st, fp16_update, side);
 }
 
 // Permutation of the rows produced by the pivots: row i of P A is row perm[i] of A.
@@ -1666,7 +1665,7 @@ __global__ void set_identity_kernel(T* X, int64_t n, int64_t ldx) {
 
 hipError_t invert(double* AI, int64_t n, int32_t* ipiv, int* info, void* ws, hipStream_t st) {
   hipLaunchKernelGGL(set_identity_kernel<double>, dim3(nblk(n * n, 256)), dim3(256), 0, st, AI + n * n, n, n);
-  hipError_t e = lu_factor_cols<double>(AI, n, 2 * n, n, ipiv, info, ws, st, false);
+  hipError_t e = lu_factor_cols<double>(AI, n, 2 * n, n, ipiv, info, ws, st, false, nullptr);
   if (e != hipSuccess) return e;
   return tri_solve_upper_many<double>(AI, n, n, AI + n * n, n, n, st);
 }
@@ -1736,11 +1735,13 @@ template int lu_batched_capacity<double>();
 template int lu_batched_capacity<float>();
 template hipError_t lu_factor_batched<double>(const LuJob<double>*, int, hipStream_t, bool);
 template hipError_t lu_factor_batched<float>(const LuJob<float>*, int, hipStream_t, bool);
-template hipError_t lu_factor<double>(double*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
-template hipError_t lu_factor<float>(float*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
+template hipError_t lu_factor<double>(double*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool,
+                                      hipStream_t);
+template hipError_t lu_factor<float>(float*, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool, hipStream_t);
 template hipError_t lu_factor_cols<double>(double*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t,
-                                           bool);
-template hipError_t lu_factor_cols<float>(float*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool);
+                                           bool, hipStream_t);
+template hipError_t lu_factor_cols<float>(float*, int64_t, int64_t, int64_t, int32_t*, int*, void*, hipStream_t, bool,
+                                          hipStream_t);
 template hipError_t lu_solve<double>(const double*, int64_t, int64_t, const int32_t*, double*, int64_t, int64_t, hipStream_t);
 template hipError_t lu_solve<float>(const float*, int64_t, int64_t, const int32_t*, float*, int64_t, int64_t, hipStream_t);
 template hipError_t tri_solve_upper<double>(const double*, int64_t, int64_t, double*, int64_t, int64_t, hipStream_t);

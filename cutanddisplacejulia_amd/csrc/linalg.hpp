@@ -17,15 +17,18 @@ size_t lu_workspace_bytes(int64_t n);
 // lu_workspace_bytes(n) bytes.  T = double or float.
 // fp16_update (T = float only): the rank-256 trailing updates take FP16 operands (U scaled per column by a power
 // of two) accumulated in FP32 on MFMA; panels, catch-up updates and triangular solves stay FP32.
+// side: a second stream (not st) for the look-ahead, which runs most of each trailing update there beside the next
+// block's panels on st; st waits for all of it before the call's last work.  NULL: no look-ahead.  Either way the
+// factors are the same bit for bit.
 template <class T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, int32_t* ipiv, int* info, void* ws, hipStream_t st,
-                     bool fp16_update = false);
+                     bool fp16_update = false, hipStream_t side = nullptr);
 
 // LU of an n x ncols matrix (ncols >= n): the extra columns are treated as right-hand sides, so on return they
 // hold L^-1 P B.
 template <class T>
 hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* ipiv, int* info, void* ws,
-                          hipStream_t st, bool fp16_update = false);
+                          hipStream_t st, bool fp16_update = false, hipStream_t side = nullptr);
 
 // One matrix of a batched factorization: as lu_factor_cols(A, n, ncols, lda, ipiv, info, ws, ...), with ws of
 // lu_workspace_bytes(n) bytes (one per matrix).
