@@ -451,11 +451,180 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
   }
 }
 
+// skip (may be NULL): the diagonal-pivot fast path's words {violation, zero-pivot column}; with no violation the
+// fast path has factored the panel already, and the kernel only passes its zero-pivot column on to info.
 template <class T, int QPL>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
                                                       int32_t* ipiv, int* info, unsigned long long* xw,
-                                                      unsigned long long* slots, int* bar_err, int* perm) {
+                                                      unsigned long long* slots, int* bar_err, int* perm,
+                                                      const int* skip) {
+  if (skip && skip[0] == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && skip[1] != 0 && *info == 0) *info = skip[1];
+    return;
+  }
   panel_body<T, QPL>(A, lda, k, m, nbw, rpb, ipiv, info, xw, slots, bar_err, perm, (int)gridDim.x, (int)blockIdx.x);
+}
+
+// ---- diagonal-pivot fast path of a panel ----------------------------------------------------------------
+// Partial pivoting takes, at column j, the row of largest |a| (ties to the lowest position).  When that is the
+// diagonal row at every column of the panel (the collocation matrices of this engine never pivot), the panel's
+// factors are those of elimination without interchanges, which needs no grid-wide exchange: one wave factors the
+// nbw x nbw diagonal block (fp_diag_kernel), then every row below solves against its U on its own
+// (fp_rows_kernel).  Both compare each candidate with the diagonal as the exchange does (64-bit keys bits(|a|),
+// ties to the diagonal) and raise fpw[0] when another row would have been taken; both first copy the rows they
+// change to `save`.  On a violation fp_restore_kernel puts the rows back and lu_panel_kernel factors the panel with
+// the exchange; otherwise both return at once.  Each entry receives the exchange panel's operations (the same
+// divisions and FMAs in the same order), so the factors are bit-identical either way.
+template <class T>
+__device__ __forceinline__ long long abs_key(T v) {
+  return (long long)__double_as_longlong(fabs((double)v));
+}
+
+// fpw: [0] 0 = no violation, 1 = violation (rows saved), 2 = fast path not tried; [1] the fast path's first zero
+// pivot (1-based global column, or 0); [2] panels of this factorization that fell back.  After FP_GIVE_UP
+// fall-backs a factorization stops trying the fast path (a matrix that pivots would pay for every attempt).
+constexpr int FP_GIVE_UP = 2;
+
+// One wave; lane r holds row k + r of the block, shifted as in panel_body.  Writes ipiv (identity), an empty
+// permutation, fpw[0] (violation inside the block) and fpw[1].
+template <class T>
+__global__ void __launch_bounds__(64) fp_diag_kernel(T* A, int64_t lda, int64_t k, int nbw, int32_t* ipiv, int* perm,
+                                                     int* fpw, T* save, int64_t lds) {
+  __shared__ T urow[LU_NB];  // the pivot row of the current column, zero beyond its active part
+  const int lane = threadIdx.x;
+  if (fpw[2] >= FP_GIVE_UP) {
+    if (lane == 0) fpw[0] = 2;
+    return;
+  }
+  const bool mine = lane < nbw;
+  T x[LU_NB];
+  {
+    const int64_t row = k + (mine ? lane : 0);
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c) x[c] = A[row + (k + (c < nbw ? c : nbw - 1)) * lda];
+  }
+  if (mine) {
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c < nbw) save[lane + c * lds] = x[c];
+  }
+  bool done = !mine, viol = false;
+  int zinfo = 0;
+  for (int j = 0; j < nbw; ++j) {
+    const int na = nbw - j;
+    const long long key = done ? -1ll : abs_key(x[0]);
+    const long long kmax = wave_max(key);
+    const long long kj = readlane64(key, j);
+    viol = viol || kmax > kj;
+    if (!(__longlong_as_double(kj) > 0.0) && zinfo == 0) zinfo = (int)(k + j + 1);
+    if (lane == j) {
+#pragma unroll
+      for (int c8 = 0; c8 < LU_NB; c8 += 8)
+        if (c8 < na) {
+#pragma unroll
+          for (int i = c8; i < c8 + 8; ++i) urow[i] = x[i];
+        }
+    }
+    if (lane >= na) urow[lane] = T(0);
+    __builtin_amdgcn_wave_barrier();
+    const T u = urow[lane];
+    const T piv = bcast_lane(u, 0);
+    __builtin_amdgcn_wave_barrier();
+    if (!done) {
+      if (lane == j) {
+        done = true;
+      } else {
+        T l = x[0];
+        if (piv != T(0)) {
+          l = x[0] / piv;
+#pragma unroll
+          for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1] - l * urow[i + 1];
+        } else {
+#pragma unroll
+          for (int i = 0; i + 1 < LU_NB; ++i) x[i] = x[i + 1];
+        }
+        x[LU_NB - 1] = l;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (mine) {
+    // row `lane` was taken at step lane: U columns lane.. in x[0..), multiplier c < lane in x[64 - lane + c]
+    const int lead = LU_NB - lane;
+    T* const out = A + (k + lane) + k * lda;
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) {
+      const int c = i >= lead ? i - lead : (i < nbw - lane ? lane + i : -1);
+      if (c >= 0) out[c * lda] = x[i];
+    }
+    ipiv[k + lane] = (int32_t)(k + lane);
+  }
+  if (lane == 0) {
+    perm[0] = 0;
+    fpw[0] = viol ? 1 : 0;
+    fpw[1] = zinfo;
+  }
+}
+
+// Rows k + nbw + r (r < nrows): one thread per row, the row in registers, U of the block from LDS.  Column j
+// divides the row's entry by u_jj (none when u_jj is 0) and subtracts the multiple of U's row j from the entries
+// right of it: the operations the exchange panel applies to a row that is not taken.
+template <class T>
+__global__ void __launch_bounds__(256) fp_rows_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t nrows, int* fpw,
+                                                      T* save, int64_t lds) {
+  __shared__ T U[LU_NB][LU_NB + 1];  // U[j][c]: row j of the block's U, zero left of the diagonal and from nbw on
+  __shared__ long long ukey[LU_NB];
+  const int tid = threadIdx.x;
+  if (fpw[0] == 2) return;
+  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+    const int j = idx % LU_NB, c = idx / LU_NB;
+    U[j][c] = (c >= j && c < nbw) ? A[(k + j) + (k + c) * lda] : T(0);
+  }
+  if (tid < LU_NB) ukey[tid] = tid < nbw ? abs_key(A[(k + tid) + (k + tid) * lda]) : 0;
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * 256 + tid;
+  const bool mine = r < nrows;
+  const int64_t row = k + nbw + (mine ? r : nrows - 1);
+  T x[LU_NB];
+#pragma unroll
+  for (int c = 0; c < LU_NB; ++c) x[c] = A[row + (k + (c < nbw ? c : nbw - 1)) * lda];
+  if (mine) {
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c < nbw) save[(nbw + r) + c * lds] = x[c];
+  }
+  bool viol = false;
+#pragma unroll
+  for (int j = 0; j < LU_NB; ++j) {
+    if (j < nbw) {
+      viol = viol || abs_key(x[j]) > ukey[j];
+      const T piv = U[j][j];
+      T l = x[j];
+      if (piv != T(0)) {
+        l = x[j] / piv;
+#pragma unroll
+        for (int c = j + 1; c < LU_NB; ++c) x[c] -= l * U[j][c];
+      }
+      x[j] = l;
+    }
+  }
+  if (__any(mine && viol) && (tid & 63) == 0) atomicOr(&fpw[0], 1);
+  if (mine) {
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c < nbw) A[row + (k + c) * lda] = x[c];
+  }
+}
+
+// After a violation: the panel's rows [k, k + m) back from `save`.
+template <class T>
+__global__ void __launch_bounds__(256) fp_restore_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t m, int* fpw,
+                                                         const T* __restrict__ save, int64_t lds) {
+  if (fpw[0] != 1) return;
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r == 0) fpw[2] += 1;  // one thread of the grid counts the fall-back
+  if (r >= m) return;
+  for (int c = 0; c < nbw; ++c) A[(k + r) + (k + c) * lda] = save[r + c * lds];
 }
 
 // Panels of several matrices at the same column k in one cooperative launch: matrix j owns workgroups
@@ -838,6 +1007,16 @@ constexpr int GH_LD = LU_OUTER_BIG; // k stride of the packed copies (the larges
 
 // Bytes from the column scales to the packed FP16 copies: n floats, rounded up to 256 bytes
 inline size_t f16_pack_offset(int64_t n) { return ((sizeof(float) * (size_t)n + 255) / 256) * 256; }
+// Layout of lu_factor's workspace: the panels' slots and records, flags, the panels' permutations and the fast
+// path's two words; n column scales and the two packed FP16 operand copies of the FP16 trailing update (n x GH_LD
+// halves each); then, 256-byte aligned, the fast path's copy of a panel's rows.
+inline size_t fast_save_offset(int64_t n) {
+  const int64_t nn = n > 0 ? n : 1;
+  const size_t head = sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) +
+                      sizeof(int) * (2 + PERM_SETS * PERM_INTS + 4) + f16_pack_offset(nn) +
+                      2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
+  return (head + 255) / 256 * 256;
+}
 
 // Lh[i * GH_LD + k] = fp16(L[i, k]) (zero for k >= K): 32 rows per workgroup, read down the columns (lanes over
 // rows) and written along the rows (lanes over 8-value groups of k) through an LDS transpose
@@ -1259,20 +1438,22 @@ struct LookAhead {
   }
 };
 
-// Look-ahead is on unless CDJ_LU_LOOKAHEAD=0 (read per call; the factors are the same either way).
+// Look-ahead is on unless CDJ_LU_LOOKAHEAD=0, the panels' diagonal-pivot fast path unless CDJ_LU_FASTPANEL=0 (read
+// per call; the factors are the same either way).
 inline bool lookahead_enabled() {
   const char* v = std::getenv("CDJ_LU_LOOKAHEAD");
+  return !(v && v[0] == '0');
+}
+inline bool fastpanel_enabled() {
+  const char* v = std::getenv("CDJ_LU_FASTPANEL");
   return !(v && v[0] == '0');
 }
 
 }  // namespace
 
 size_t lu_workspace_bytes(int64_t n) {
-  const int64_t nn = n > 0 ? n : 1;
-  // the panels' slots and records, flags and the panels' permutations; then n column scales and the two packed FP16
-  // operand copies of the FP16 trailing update (n x GH_LD halves each)
-  return sizeof(unsigned long long) * 2 * MAX_P * (1 + (size_t)XREC) + sizeof(int) * (2 + PERM_SETS * PERM_INTS) +
-         f16_pack_offset(nn) + 2 * sizeof(_Float16) * (size_t)nn * GH_LD + 1024;
+  // then the fast path's copy of a panel's rows (n x LU_NB entries, FP64 at most)
+  return fast_save_offset(n) + sizeof(double) * (size_t)(n > 0 ? n : 1) * LU_NB;
 }
 
 // Blocked LU with partial pivoting of the n x n leading block of the n x ncols matrix A (extra columns are
@@ -1300,7 +1481,9 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   unsigned* bar = reinterpret_cast<unsigned*>(w);  // unused word, then the panels' wait-overrun flag
   int* bar_err = reinterpret_cast<int*>(bar + 1);
   int* perm_sets = bar_err + 1;                     // the row permutations of an outer block's panels, two sets
-  float* sigma = reinterpret_cast<float*>(perm_sets + PERM_SETS * PERM_INTS);  // n column scales (FP16 update)
+  int* fpw = perm_sets + PERM_SETS * PERM_INTS;       // the fast path's {violation, zero-pivot column}
+  float* sigma = reinterpret_cast<float*>(fpw + 4);  // n column scales (FP16 update)
+  T* save = reinterpret_cast<T*>(static_cast<char*>(ws) + fast_save_offset(n));  // a panel's rows (n x LU_NB)
   if (ncols > n) fp16_update = false;
   hipError_t e = hipMemsetAsync(info, 0, sizeof(int), st);
   if (e != hipSuccess) return e;
@@ -1308,6 +1491,8 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   if (e != hipSuccess) return e;
   // all ones: slot tag 4095 and record tag 2^32 - 1 match no column of this factorization
   e = hipMemsetAsync(slots, 0xFF, xbytes, st);
+  if (e != hipSuccess) return e;
+  e = hipMemsetAsync(fpw, 0, 4 * sizeof(int), st);
   if (e != hipSuccess) return e;
   const int64_t ob = lu_outer(n);
   // panel workgroups that fit on the device at once (ordinary launches beside a running product rely on it)
@@ -1322,6 +1507,7 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
   }();
   LookAhead la;
   const bool ahead = side != nullptr && side != st && lookahead_enabled() && n > ob && panel_capacity > 0;
+  const bool fast = fastpanel_enabled();
   if (ahead && (e = la.init(side)) != hipSuccess) return e;
   bool pending = false;  // the previous block's update of the columns right of this block runs on la.s2
   bool queued = false;   // work was queued on la.s2
@@ -1341,7 +1527,19 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       const int P = (int)((m + MAX_RPB - 1) / MAX_RPB);
       int rpb = (int)((m + P - 1) / P);
       if (P > MAX_P) return hipErrorInvalidValue;
-      void* args[] = {&A, (void*)&lda, &k, (void*)&m, (void*)&nbw, &rpb, &ipiv, &info, &xw, &slots, &bar_err, &perm};
+      const int* skip = nullptr;
+      if (fast) {
+        // the diagonal-pivot fast path; the exchange panel below runs only if it found another pivot row
+        hipLaunchKernelGGL(fp_diag_kernel<T>, dim3(1), dim3(LU_NB), 0, st, A, lda, k, nbw, ipiv, perm, fpw, save, m);
+        if (m > nbw)
+          hipLaunchKernelGGL(fp_rows_kernel<T>, dim3(nblk(m - nbw, 256)), dim3(256), 0, st, A, lda, k, nbw, m - nbw,
+                             fpw, save, m);
+        hipLaunchKernelGGL(fp_restore_kernel<T>, dim3(nblk(m, 256)), dim3(256), 0, st, A, lda, k, nbw, m, fpw,
+                           (const T*)save, m);
+        skip = fpw;
+      }
+      void* args[] = {&A,    (void*)&lda, &k,     (void*)&m, (void*)&nbw, &rpb, &ipiv, &info,
+                      &xw,   &slots,      &bar_err, &perm,   (void*)&skip};
       const void* kern = P <= 128 ? (const void*)lu_panel_kernel<T, 2> : (const void*)lu_panel_kernel<T, MAX_P / 64>;
       if (pending && P <= panel_capacity)
         e = hipLaunchKernel(kern, dim3(P), dim3(PT), args, 0, st);
