@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "linalg.hpp"
@@ -485,17 +486,56 @@ __device__ __forceinline__ long long abs_key(T v) {
 // fall-backs a factorization stops trying the fast path (a matrix that pivots would pay for every attempt).
 constexpr int FP_GIVE_UP = 2;
 
-// One wave; lane r holds row k + r of the block, shifted as in panel_body.  Writes ipiv (identity), an empty
-// permutation, fpw[0] (violation inside the block) and fpw[1].
+// Step J of the diagonal block (J a constant, so that every register index is one): the pivot row's entries reach
+// the lanes as lane reads into scalar registers, a group at a time and before the products that use them (the reads
+// then issue back to back); the rows below the diagonal divide by the pivot and subtract its multiple.  Each lane
+// keeps a running violation flag (a row below holding a larger |a| than the pivot).
+template <class T, bool FULL, int J>
+__device__ __forceinline__ void fp_step(T (&x)[LU_NB], int lane, int nbw, int64_t k, bool mine, bool& viol,
+                                        int& zinfo) {
+  if (FULL) nbw = LU_NB;  // a full block: every column test below is a constant
+  if (J >= nbw) return;  // uniform
+  const T piv = bcast_lane(x[J], J);
+  const long long pk = abs_key(piv);
+  if (!(__longlong_as_double(pk) > 0.0) && zinfo == 0) zinfo = (int)(k + J + 1);
+  const bool below = lane > J && mine;
+  T l = x[J];
+  if (below) {
+    viol = viol || abs_key(x[J]) > pk;
+    if (piv != T(0)) l = x[J] / piv;
+  }
+  constexpr int B = sizeof(T) == 8 ? 4 : 8;  // entries per group (FP64 entries take two scalar registers each)
+  if (piv != T(0)) {
+#pragma unroll
+    for (int c0 = J + 1; c0 < LU_NB; c0 += B) {
+      T u[B];
+#pragma unroll
+      for (int e = 0; e < B; ++e)
+        if (c0 + e < LU_NB) u[e] = c0 + e < nbw ? bcast_lane(x[(c0 + e) & (LU_NB - 1)], J) : T(0);
+      if (below) {
+#pragma unroll
+        for (int e = 0; e < B; ++e)
+          if (c0 + e < LU_NB && c0 + e < nbw) x[(c0 + e) & (LU_NB - 1)] -= l * u[e];
+      }
+    }
+  }
+  if (below) x[J] = l;
+}
+
+template <class T, bool FULL, int... J>
+__device__ __forceinline__ void fp_steps(T (&x)[LU_NB], int lane, int nbw, int64_t k, bool mine, bool& viol,
+                                         int& zinfo, std::integer_sequence<int, J...>) {
+  (fp_step<T, FULL, J>(x, lane, nbw, k, mine, viol, zinfo), ...);
+}
+
+// One wave factors the diagonal block: lane r holds row k + r, shifted as in panel_body; each step's pivot row is
+// staged in LDS by its lane (zero beyond its active part) and read back as broadcasts.  Both forms write ipiv
+// (identity), an empty permutation, fpw[0] (violation inside the block) and fpw[1].
 template <class T>
-__global__ void __launch_bounds__(64) fp_diag_kernel(T* A, int64_t lda, int64_t k, int nbw, int32_t* ipiv, int* perm,
-                                                     int* fpw, T* save, int64_t lds) {
+__device__ __forceinline__ void fp_diag_lds(T* A, int64_t lda, int64_t k, int nbw, int32_t* ipiv, int* perm,
+                                            int* fpw, T* save, int64_t lds) {
   __shared__ T urow[LU_NB];  // the pivot row of the current column, zero beyond its active part
   const int lane = threadIdx.x;
-  if (fpw[2] >= FP_GIVE_UP) {
-    if (lane == 0) fpw[0] = 2;
-    return;
-  }
   const bool mine = lane < nbw;
   T x[LU_NB];
   {
@@ -564,6 +604,59 @@ __global__ void __launch_bounds__(64) fp_diag_kernel(T* A, int64_t lda, int64_t 
     fpw[0] = viol ? 1 : 0;
     fpw[1] = zinfo;
   }
+}
+
+// The same with lane r holding row k + r in registers unshifted, x[c] = column k + c, the steps unrolled (fp_step)
+// and the pivot row's entries as lane reads (faster for FP32 rows; an FP64 row's lane reads need too many scalar
+// registers).
+template <class T>
+__device__ __forceinline__ void fp_diag_lanes(T* A, int64_t lda, int64_t k, int nbw, int32_t* ipiv, int* perm,
+                                              int* fpw, T* save, int64_t lds) {
+  const int lane = threadIdx.x;
+  const bool mine = lane < nbw;
+  T x[LU_NB];
+  {
+    const int64_t row = k + (mine ? lane : 0);
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c) x[c] = A[row + (k + (c < nbw ? c : nbw - 1)) * lda];
+  }
+  if (mine) {
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c < nbw) save[lane + c * lds] = x[c];
+  }
+  bool viol = false;
+  int zinfo = 0;
+  if (nbw == LU_NB)
+    fp_steps<T, true>(x, lane, nbw, k, mine, viol, zinfo, std::make_integer_sequence<int, LU_NB>{});
+  else
+    fp_steps<T, false>(x, lane, nbw, k, mine, viol, zinfo, std::make_integer_sequence<int, LU_NB>{});
+  if (mine) {
+    T* const out = A + (k + lane) + k * lda;
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c < nbw) out[c * lda] = x[c];
+    ipiv[k + lane] = (int32_t)(k + lane);
+  }
+  const bool any = __any(viol);
+  if (lane == 0) {
+    perm[0] = 0;
+    fpw[0] = any ? 1 : 0;
+    fpw[1] = zinfo;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(64) fp_diag_kernel(T* A, int64_t lda, int64_t k, int nbw, int32_t* ipiv, int* perm,
+                                                     int* fpw, T* save, int64_t lds) {
+  if (fpw[2] >= FP_GIVE_UP) {
+    if (threadIdx.x == 0) fpw[0] = 2;
+    return;
+  }
+  if constexpr (sizeof(T) == 8)
+    fp_diag_lds<T>(A, lda, k, nbw, ipiv, perm, fpw, save, lds);
+  else
+    fp_diag_lanes<T>(A, lda, k, nbw, ipiv, perm, fpw, save, lds);
 }
 
 // Rows k + nbw + r (r < nrows): one thread per row, the row in registers, U of the block from LDS.  Column j
