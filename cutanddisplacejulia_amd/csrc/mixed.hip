@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "linalg.hpp"
@@ -40,12 +41,55 @@ __global__ void __launch_bounds__(256) gemv_part_kernel(const TA* __restrict__ A
   if (i >= m) return;
   double s0 = 0.0, s1 = 0.0;
   int64_t c = c0;
+  // unrolled so that many column loads are in flight per wave (the sums keep their order)
+#pragma unroll 8
   for (; c + 1 < c1; c += 2) {
     s0 += (double)A[i + c * lda] * xs[c - c0];
     s1 += (double)A[i + (c + 1) * lda] * xs[c + 1 - c0];
   }
   if (c < c1) s0 += (double)A[i + c * lda] * xs[c - c0];
   part[(int64_t)blockIdx.y * m + i] = s0 + s1;
+}
+
+// The FP32 form for an even leading dimension: two rows per thread, read as one 8-byte pair per column (a wave's
+// load then covers 512 contiguous bytes, as the FP64 form's does); each row's sum is formed exactly as above.
+__global__ void __launch_bounds__(256) gemv_part2_f32_kernel(const float* __restrict__ A, int64_t m, int64_t n,
+                                                             int64_t lda, const double* __restrict__ x, int chunk,
+                                                             double* __restrict__ part) {
+  __shared__ double xs[1024];
+  const int64_t c0 = (int64_t)blockIdx.y * chunk;
+  const int64_t c1 = std::min<int64_t>(n, c0 + chunk);
+  for (int64_t c = c0 + threadIdx.x; c < c1; c += 256) xs[c - c0] = x[c];
+  __syncthreads();
+  const int64_t i = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 2;
+  if (i >= m) return;
+  double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;  // row i: a0 + a1, row i + 1: b0 + b1
+  int64_t c = c0;
+  if (i + 1 < m) {
+#pragma unroll 8
+    for (; c + 1 < c1; c += 2) {
+      const float2 p = *reinterpret_cast<const float2*>(A + i + c * lda);
+      const float2 q = *reinterpret_cast<const float2*>(A + i + (c + 1) * lda);
+      a0 += (double)p.x * xs[c - c0];
+      b0 += (double)p.y * xs[c - c0];
+      a1 += (double)q.x * xs[c + 1 - c0];
+      b1 += (double)q.y * xs[c + 1 - c0];
+    }
+    if (c < c1) {
+      const float2 p = *reinterpret_cast<const float2*>(A + i + c * lda);
+      a0 += (double)p.x * xs[c - c0];
+      b0 += (double)p.y * xs[c - c0];
+    }
+    part[(int64_t)blockIdx.y * m + i] = a0 + a1;
+    part[(int64_t)blockIdx.y * m + i + 1] = b0 + b1;
+  } else {
+    for (; c + 1 < c1; c += 2) {
+      a0 += (double)A[i + c * lda] * xs[c - c0];
+      a1 += (double)A[i + (c + 1) * lda] * xs[c + 1 - c0];
+    }
+    if (c < c1) a0 += (double)A[i + c * lda] * xs[c - c0];
+    part[(int64_t)blockIdx.y * m + i] = a0 + a1;
+  }
 }
 
 __global__ void gemv_reduce_kernel(const double* __restrict__ part, int64_t m, int nchunk, double alpha,
@@ -266,6 +310,8 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int nvec, i
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nvec) return;
   double s = 0.0;
+  // unrolled so that the loads are in flight together (the additions keep their order)
+#pragma unroll 16
   for (int q = 0; q < nparts; ++q) s += part[(int64_t)j * nparts + q];
   out[j] = s;
 }
@@ -505,9 +551,14 @@ void gemv_splitk(const TA* A, int64_t m, int64_t n, int64_t lda, const double* x
   if (m <= 0) return;
   const int chunk = gemv_chunk(m, n);
   const int nchunk = (int)((n + chunk - 1) / chunk);
-  if (nchunk > 0)
-    hipLaunchKernelGGL(gemv_part_kernel<TA>, dim3(nblk(m, 256), (unsigned)nchunk), dim3(256), 0, st, A, m, n, lda, x,
-                       chunk, part);
+  if (nchunk > 0) {
+    if (std::is_same<TA, float>::value && lda % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 8 == 0)
+      hipLaunchKernelGGL(gemv_part2_f32_kernel, dim3(nblk(m, 512), (unsigned)nchunk), dim3(256), 0, st,
+                         reinterpret_cast<const float*>(A), m, n, lda, x, chunk, part);
+    else
+      hipLaunchKernelGGL(gemv_part_kernel<TA>, dim3(nblk(m, 256), (unsigned)nchunk), dim3(256), 0, st, A, m, n, lda,
+                         x, chunk, part);
+  }
   hipLaunchKernelGGL(gemv_reduce_kernel, dim3(nblk(m, 256)), dim3(256), 0, st, (const double*)part, m, nchunk, alpha,
                      yin, beta, y);
 }
