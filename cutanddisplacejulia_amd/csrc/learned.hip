@@ -74,10 +74,13 @@ __global__ void geom_kernel(const double* __restrict__ P1, const double* __restr
 // k nearest centroids (itself first), ties broken by index; brute force, one thread per triangle.
 // k nearest centroids of each element (itself first, distance -1), K a compile-time bound so the sorted lists
 // stay in registers: candidates are staged through LDS 256 at a time and inserted by an unrolled compare-shift.
+// Launched with KNN_THREADS threads per workgroup (one wave), so that the few thousand elements of a mesh still
+// spread over most CUs (256-thread workgroups left 26 of them busy for 6534 triangles).
+constexpr int KNN_THREADS = 64;
 template <int K>
-__global__ void __launch_bounds__(256) knn_kernel(const float* __restrict__ geo, int64_t n, int k,
-                                                  int32_t* __restrict__ nbr) {
-  __shared__ float cx[256], cy[256], cz[256];
+__global__ void __launch_bounds__(KNN_THREADS) knn_kernel(const float* __restrict__ geo, int64_t n, int k,
+                                                          int32_t* __restrict__ nbr) {
+  __shared__ float cx[KNN_THREADS], cy[KNN_THREADS], cz[KNN_THREADS];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const bool valid = i < n;
   float bd[K];
@@ -88,7 +91,7 @@ __global__ void __launch_bounds__(256) knn_kernel(const float* __restrict__ geo,
     bi[q] = -1;
   }
   const float px = valid ? geo[i * GEO] : 0.f, py = valid ? geo[i * GEO + 1] : 0.f, pz = valid ? geo[i * GEO + 2] : 0.f;
-  for (int64_t t0 = 0; t0 < n; t0 += 256) {
+  for (int64_t t0 = 0; t0 < n; t0 += KNN_THREADS) {
     const int64_t jt = t0 + threadIdx.x;
     if (jt < n) {
       cx[threadIdx.x] = geo[jt * GEO];
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(256) knn_kernel(const float* __restrict__ geo,
       cz[threadIdx.x] = geo[jt * GEO + 2];
     }
     __syncthreads();
-    const int cnt = (int)(n - t0 < 256 ? n - t0 : 256);
+    const int cnt = (int)(n - t0 < KNN_THREADS ? n - t0 : KNN_THREADS);
     if (valid)
       for (int c = 0; c < cnt; ++c) {
         const int64_t j = t0 + c;
@@ -525,9 +528,11 @@ hipError_t mesh_prepare(const double* P1, const double* P2, const double* P3, in
   if ((e = hipMalloc(&f.feat, sizeof(float) * (int64_t)k * NF * n))) return e;
   hipLaunchKernelGGL(geom_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, P1, P2, P3, n, f.geo);
   if (k <= 16)
-    hipLaunchKernelGGL(knn_kernel<16>, dim3(nblk(n, 256)), dim3(256), 0, st, (const float*)f.geo, n, k, f.nbr);
+    hipLaunchKernelGGL(knn_kernel<16>, dim3(nblk(n, KNN_THREADS)), dim3(KNN_THREADS), 0, st, (const float*)f.geo, n, k,
+                       f.nbr);
   else
-    hipLaunchKernelGGL(knn_kernel<KNN_MAX>, dim3(nblk(n, 256)), dim3(256), 0, st, (const float*)f.geo, n, k, f.nbr);
+    hipLaunchKernelGGL(knn_kernel<KNN_MAX>, dim3(nblk(n, KNN_THREADS)), dim3(KNN_THREADS), 0, st, (const float*)f.geo,
+                       n, k, f.nbr);
   hipLaunchKernelGGL(features_kernel, dim3(nblk(n, 128)), dim3(128), 0, st, (const float*)f.geo,
                      (const int32_t*)f.nbr, n, k, f.feat);
   return hipGetLastError();
