@@ -606,9 +606,8 @@ __device__ __forceinline__ void fp_diag_lds(T* A, int64_t lda, int64_t k, int nb
   }
 }
 
-// The same with lane r holding row k + r in registers unshifted, x[c] = column k + c, the steps unrolled (fp_step)
-// and the pivot row's entries as lane reads (faster for FP32 rows; an FP64 row's lane reads need too many scalar
-// registers).
+// The same for a full block (nbw = 64) with lane r holding row k + r in registers unshifted, x[c] = column k + c,
+// the steps unrolled (fp_step) and the pivot row's entries as grouped lane reads.
 template <class T>
 __device__ __forceinline__ void fp_diag_lanes(T* A, int64_t lda, int64_t k, int nbw, int32_t* ipiv, int* perm,
                                               int* fpw, T* save, int64_t lds) {
@@ -627,10 +626,7 @@ __device__ __forceinline__ void fp_diag_lanes(T* A, int64_t lda, int64_t k, int 
   }
   bool viol = false;
   int zinfo = 0;
-  if (nbw == LU_NB)
-    fp_steps<T, true>(x, lane, nbw, k, mine, viol, zinfo, std::make_integer_sequence<int, LU_NB>{});
-  else
-    fp_steps<T, false>(x, lane, nbw, k, mine, viol, zinfo, std::make_integer_sequence<int, LU_NB>{});
+  fp_steps<T, true>(x, lane, LU_NB, k, mine, viol, zinfo, std::make_integer_sequence<int, LU_NB>{});
   if (mine) {
     T* const out = A + (k + lane) + k * lda;
 #pragma unroll
@@ -646,17 +642,19 @@ __device__ __forceinline__ void fp_diag_lanes(T* A, int64_t lda, int64_t k, int 
   }
 }
 
-template <class T>
+// FULL (nbw == 64, every panel but possibly the last): the lane-read form, each kernel its own register
+// allocation; a narrower last panel: the LDS-staged form.
+template <class T, bool FULL>
 __global__ void __launch_bounds__(64) fp_diag_kernel(T* A, int64_t lda, int64_t k, int nbw, int32_t* ipiv, int* perm,
                                                      int* fpw, T* save, int64_t lds) {
   if (fpw[2] >= FP_GIVE_UP) {
     if (threadIdx.x == 0) fpw[0] = 2;
     return;
   }
-  if constexpr (sizeof(T) == 8)
-    fp_diag_lds<T>(A, lda, k, nbw, ipiv, perm, fpw, save, lds);
+  if constexpr (FULL)
+    fp_diag_lanes<T>(A, lda, k, LU_NB, ipiv, perm, fpw, save, lds);
   else
-    fp_diag_lanes<T>(A, lda, k, nbw, ipiv, perm, fpw, save, lds);
+    fp_diag_lds<T>(A, lda, k, nbw, ipiv, perm, fpw, save, lds);
 }
 
 // Rows k + nbw + r (r < nrows): one thread per row, the row in registers, U of the block from LDS.  Column j
@@ -1623,7 +1621,12 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       const int* skip = nullptr;
       if (fast) {
         // the diagonal-pivot fast path; the exchange panel below runs only if it found another pivot row
-        hipLaunchKernelGGL(fp_diag_kernel<T>, dim3(1), dim3(LU_NB), 0, st, A, lda, k, nbw, ipiv, perm, fpw, save, m);
+        if (nbw == LU_NB)
+          hipLaunchKernelGGL((fp_diag_kernel<T, true>), dim3(1), dim3(LU_NB), 0, st, A, lda, k, nbw, ipiv, perm, fpw,
+                             save, m);
+        else
+          hipLaunchKernelGGL((fp_diag_kernel<T, false>), dim3(1), dim3(LU_NB), 0, st, A, lda, k, nbw, ipiv, perm, fpw,
+                             save, m);
         if (m > nbw)
           hipLaunchKernelGGL(fp_rows_kernel<T>, dim3(nblk(m - nbw, 256)), dim3(256), 0, st, A, lda, k, nbw, m - nbw,
                              fpw, save, m);
