@@ -137,6 +137,34 @@ def test_fp64_factors_bitwise_equal_with_and_without_lookahead(n):
     assert np.max(np.abs(Lm @ Um - PA)) <= 1e-12 * np.max(np.abs(A))
 
 
+def test_fast_panel_zero_diagonal_falls_back():
+    """A zero on the diagonal with nonzero entries below: partial pivoting swaps there, so the fast path must fall
+    back in that panel; the factors and pivots are the exchange panel's."""
+    n = 900
+    A = _dominant(n, 21)
+    A[100, 100] = 0.0
+    print("[gpu-test] fast panel zero diagonal n=%d" % n, flush=True)
+    f0, p0 = _factor_dev(A, CDJ_LU_FASTPANEL=False)
+    f1, p1 = _factor_dev(A, CDJ_LU_FASTPANEL=True)
+    assert p1[100] != 100
+    assert np.array_equal(p0, p1) and np.array_equal(f0, f1)
+
+
+def test_fast_panel_zero_column_reports_the_same_pivot():
+    """An exactly zero column: no row can serve as pivot, partial pivoting keeps the diagonal and reports the first
+    zero pivot; the fast path reports the same column."""
+    n = 700
+    A = _dominant(n, 22)
+    A[:, 70] = 0.0
+    print("[gpu-test] fast panel zero column n=%d" % n, flush=True)
+    msgs = []
+    for flag in (False, True):
+        with pytest.raises(api.CdjError) as ei:
+            _factor_dev(A, CDJ_LU_FASTPANEL=flag)
+        msgs.append(str(ei.value))
+    assert msgs[0] == msgs[1] and "zero pivot 71" in msgs[1], msgs
+
+
 @pytest.mark.parametrize("mode", ["fp64", "mixed32", "mixed16"])
 def test_fast_panel_solves_bitwise_equal_to_exchange(mode):
     n, nrhs = 1500, 2
