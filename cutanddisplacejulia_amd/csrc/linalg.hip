@@ -661,8 +661,8 @@ __global__ void __launch_bounds__(64) fp_diag_kernel(T* A, int64_t lda, int64_t 
 // divides the row's entry by u_jj (none when u_jj is 0) and subtracts the multiple of U's row j from the entries
 // right of it: the operations the exchange panel applies to a row that is not taken.
 template <class T>
-__global__ void __launch_bounds__(256) fp_rows_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t nrows, int* fpw,
-                                                      T* save, int64_t lds) {
+__device__ __forceinline__ void fp_rows_body(T* A, int64_t lda, int64_t k, int nbw, int64_t nrows, int* fpw, T* save,
+                                             int64_t lds, int64_t blk) {
   __shared__ T U[LU_NB][LU_NB + 1];  // U[j][c]: row j of the block's U, zero left of the diagonal and from nbw on
   __shared__ long long ukey[LU_NB];
   const int tid = threadIdx.x;
@@ -673,7 +673,7 @@ __global__ void __launch_bounds__(256) fp_rows_kernel(T* A, int64_t lda, int64_t
   }
   if (tid < LU_NB) ukey[tid] = tid < nbw ? abs_key(A[(k + tid) + (k + tid) * lda]) : 0;
   __syncthreads();
-  const int64_t r = (int64_t)blockIdx.x * 256 + tid;
+  const int64_t r = blk * 256 + tid;
   const bool mine = r < nrows;
   const int64_t row = k + nbw + (mine ? r : nrows - 1);
   T x[LU_NB];
@@ -707,15 +707,77 @@ __global__ void __launch_bounds__(256) fp_rows_kernel(T* A, int64_t lda, int64_t
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) fp_rows_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t nrows, int* fpw,
+                                                      T* save, int64_t lds) {
+  fp_rows_body<T>(A, lda, k, nbw, nrows, fpw, save, lds, (int64_t)blockIdx.x);
+}
+
 // After a violation: the panel's rows [k, k + m) back from `save`.
 template <class T>
-__global__ void __launch_bounds__(256) fp_restore_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t m, int* fpw,
-                                                         const T* __restrict__ save, int64_t lds) {
+__device__ __forceinline__ void fp_restore_body(T* A, int64_t lda, int64_t k, int nbw, int64_t m, int* fpw,
+                                                const T* __restrict__ save, int64_t lds, int64_t blk) {
   if (fpw[0] != 1) return;
-  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t r = blk * 256 + threadIdx.x;
   if (r == 0) fpw[2] += 1;  // one thread of the grid counts the fall-back
   if (r >= m) return;
   for (int c = 0; c < nbw; ++c) A[(k + r) + (k + c) * lda] = save[r + c * lds];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) fp_restore_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t m, int* fpw,
+                                                         const T* __restrict__ save, int64_t lds) {
+  fp_restore_body<T>(A, lda, k, nbw, m, fpw, save, lds, (int64_t)blockIdx.x);
+}
+
+// The fast path for the panels of several matrices at the same column k (the batched factorization): job j owns
+// workgroups [b0, b0 + nb) of a launch (nb = 1 for the diagonal blocks).
+template <class T>
+struct FpJob {
+  T* A;
+  int64_t lda, m;  // m: the panel's rows, [k, k + m)
+  int32_t nbw, b0;
+  int32_t* ipiv;
+  int* perm;
+  int* fpw;
+  T* save;
+};
+constexpr int FP_JOBS = 24;
+template <class T>
+struct FpBatch {
+  int njobs;
+  FpJob<T> job[FP_JOBS];
+};
+template <class T>
+__device__ __forceinline__ int fp_job_of(const FpBatch<T>& jb) {
+  int j = 0;
+  while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].b0) ++j;
+  return j;
+}
+
+template <class T, bool FULL>
+__global__ void __launch_bounds__(64) fp_diag_batched_kernel(const FpBatch<T> jb, int64_t k) {
+  const FpJob<T>& J = jb.job[blockIdx.x];
+  if (J.fpw[2] >= FP_GIVE_UP) {
+    if (threadIdx.x == 0) J.fpw[0] = 2;
+    return;
+  }
+  if constexpr (FULL)
+    fp_diag_lanes<T>(J.A, J.lda, k, LU_NB, J.ipiv, J.perm, J.fpw, J.save, J.m);
+  else
+    fp_diag_lds<T>(J.A, J.lda, k, J.nbw, J.ipiv, J.perm, J.fpw, J.save, J.m);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) fp_rows_batched_kernel(const FpBatch<T> jb, int64_t k) {
+  const FpJob<T>& J = jb.job[fp_job_of(jb)];
+  fp_rows_body<T>(J.A, J.lda, k, J.nbw, J.m - J.nbw, J.fpw, J.save, J.m, (int64_t)blockIdx.x - J.b0);
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) fp_restore_batched_kernel(const FpBatch<T> jb, int64_t k) {
+  const FpJob<T>& J = jb.job[fp_job_of(jb)];
+  fp_restore_body<T>(J.A, J.lda, k, J.nbw, J.m, J.fpw, J.save, J.m, (int64_t)blockIdx.x - J.b0);
 }
 
 // Panels of several matrices at the same column k in one cooperative launch: matrix j owns workgroups
@@ -731,6 +793,7 @@ struct PanelJob {
   unsigned long long* slots;
   int* bar_err;
   int* perm;
+  const int* skip;  // the fast path's words (NULL: none), as in lu_panel_kernel
 };
 constexpr int PANEL_JOBS = 24;  // matrices per launch (the job table travels in the kernel arguments)
 template <class T>
@@ -744,6 +807,10 @@ __global__ void __launch_bounds__(PT) lu_panel_batched_kernel(const PanelBatch<T
   int j = 0;
   while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].b0) ++j;
   const PanelJob<T>& J = jb.job[j];
+  if (J.skip && J.skip[0] == 0) {  // every workgroup of the job leaves, so none waits for another
+    if ((int)blockIdx.x == J.b0 && threadIdx.x == 0 && J.skip[1] != 0 && *J.info == 0) *J.info = J.skip[1];
+    return;
+  }
   panel_body<T, QPL>(J.A, J.lda, k, J.m, J.nbw, J.rpb, J.ipiv, J.info, J.xw, J.slots, J.bar_err, J.perm, J.P,
                      (int)blockIdx.x - J.b0);
 }
@@ -1377,13 +1444,14 @@ __global__ void kkt_kernel(const double* __restrict__ A, int64_t lda, int64_t m,
 // A matrix's factorization state before its first panel: info and the wait flags cleared, slots and records all
 // ones (tag 4095 / 2^32 - 1 match no column).  A kernel rather than memsets, so the factorization can be captured
 // in a graph.
-__global__ void lu_init_kernel(int* info, unsigned* bar, unsigned long long* slots, int64_t nslots) {
+__global__ void lu_init_kernel(int* info, unsigned* bar, unsigned long long* slots, int64_t nslots, int* fpw) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t == 0) {
     *info = 0;
     bar[0] = 0;
     bar[1] = 0;
   }
+  if (fpw && t < 4) fpw[t] = 0;
   for (int64_t i = t; i < nslots; i += (int64_t)gridDim.x * blockDim.x) slots[i] = ~0ull;
 }
 
@@ -1723,8 +1791,9 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
   struct WsView {
     unsigned long long *slots, *xw;
     unsigned* bar;
-    int *bar_err, *perm;
+    int *bar_err, *perm, *fpw;
     float* sigma;
+    T* save;
     size_t xbytes;
   };
   std::vector<WsView> wv(njobs);
@@ -1742,13 +1811,16 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
     v.bar = reinterpret_cast<unsigned*>(w);
     v.bar_err = reinterpret_cast<int*>(v.bar + 1);
     v.perm = v.bar_err + 1;
-    v.sigma = reinterpret_cast<float*>(v.perm + PERM_SETS * PERM_INTS);
+    v.fpw = v.perm + PERM_SETS * PERM_INTS;  // the same layout as lu_factor_cols'
+    v.sigma = reinterpret_cast<float*>(v.fpw + 4);
+    v.save = reinterpret_cast<T*>(static_cast<char*>(J.ws) + fast_save_offset(J.n));
     hipLaunchKernelGGL(lu_init_kernel, dim3(64), dim3(256), 0, st, J.info, v.bar, v.slots,
-                       (int64_t)(v.xbytes / sizeof(unsigned long long)));
+                       (int64_t)(v.xbytes / sizeof(unsigned long long)), v.fpw);
     nmax = std::max(nmax, J.n);
   }
   const int capacity = lu_batched_capacity<T>();
   if (capacity <= 0) return hipErrorInvalidConfiguration;
+  const bool fast = fastpanel_enabled();
   std::vector<int> act;
   GemmList<T> gl;
   TrsmList<T> tl;
@@ -1781,6 +1853,40 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
         }
         gl.launch(st);
       }
+      // the diagonal-pivot fast path of every matrix's panel (as in lu_factor_cols), FP_JOBS matrices per launch;
+      // the exchange panels below run only for the matrices where it found another pivot row
+      if (fast) {
+        for (size_t a0 = 0; a0 < act.size(); a0 += FP_JOBS) {
+          const size_t a1 = std::min(act.size(), a0 + FP_JOBS);
+          FpBatch<T> full, part, all;
+          full.njobs = part.njobs = all.njobs = 0;
+          for (size_t a = a0; a < a1; ++a) {
+            const LuJob<T>& J = jobs[act[a]];
+            const WsView& v = wv[act[a]];
+            const int nbw = nbw_of(J);
+            const FpJob<T> q{J.A, J.lda, J.n - k, nbw, 0, J.ipiv, v.perm, v.fpw, v.save};
+            (nbw == LU_NB ? full : part).job[(nbw == LU_NB ? full : part).njobs++] = q;
+            all.job[all.njobs++] = q;
+          }
+          int64_t kk = k;
+          if (full.njobs)
+            hipLaunchKernelGGL((fp_diag_batched_kernel<T, true>), dim3(full.njobs), dim3(LU_NB), 0, st, full, kk);
+          if (part.njobs)
+            hipLaunchKernelGGL((fp_diag_batched_kernel<T, false>), dim3(part.njobs), dim3(LU_NB), 0, st, part, kk);
+          int blocks = 0;
+          for (int q = 0; q < all.njobs; ++q) {
+            all.job[q].b0 = blocks;
+            blocks += (int)nblk(all.job[q].m - all.job[q].nbw, 256);
+          }
+          if (blocks > 0) hipLaunchKernelGGL(fp_rows_batched_kernel<T>, dim3(blocks), dim3(256), 0, st, all, kk);
+          blocks = 0;
+          for (int q = 0; q < all.njobs; ++q) {
+            all.job[q].b0 = blocks;
+            blocks += (int)nblk(all.job[q].m, 256);
+          }
+          hipLaunchKernelGGL(fp_restore_batched_kernel<T>, dim3(blocks), dim3(256), 0, st, all, kk);
+        }
+      }
       // the panels, as few launches as co-residency and the job table allow
       size_t a = 0;
       while (a < act.size()) {
@@ -1807,6 +1913,7 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
           q.slots = wv[act[a]].slots;
           q.bar_err = wv[act[a]].bar_err;
           q.perm = wv[act[a]].perm;
+          q.skip = fast ? wv[act[a]].fpw : nullptr;
           blocks += P;
           pmax = std::max(pmax, P);
         }

@@ -194,7 +194,9 @@ def _lu_single(L, ctx, sp, A, n):
 def test_batched_lu_bitwise_equals_single():
     """cdj_lu_factor_batched_dev factors each matrix exactly as cdj_lu_factor_dev does alone (same operations in
     the same order), over sizes that need one and several panel workgroups, more matrices than one panel launch
-    holds, integer-valued matrices with exact pivot ties across workgroups, and a singular matrix."""
+    holds, integer-valued matrices with exact pivot ties across workgroups, diagonally dominant matrices (every panel
+    taken by the diagonal-pivot fast path) and dominant ones with a forced row interchange (that panel falls back to
+    the exchange), and a singular matrix."""
     import ctypes
     import torch
     from cutanddisplacejulia_amd import _lib
@@ -207,6 +209,12 @@ def test_batched_lu_bitwise_equals_single():
     for t, n in enumerate(sizes):
         if t % 3 == 1:  # small integers: many exactly equal candidates, including across workgroups
             a = rng.integers(-3, 4, size=(n, n)).astype(np.float64) + 7.0 * np.eye(n)
+        elif t % 3 == 2:  # diagonally dominant: partial pivoting keeps the diagonal
+            a = rng.uniform(-1, 1, size=(n, n)) / max(n, 1)
+            a[np.arange(n), np.arange(n)] = rng.uniform(1, 2, n) * rng.choice([-1, 1], n)
+            if t % 6 == 5 and n > 2:  # one column where a lower row wins
+                j = int(rng.integers(0, n - 1))
+                a[int(rng.integers(j + 1, n)), j] = 10.0
         else:
             a = rng.standard_normal((n, n))
         mats.append(torch.from_numpy(np.asfortranarray(a).ravel(order="F").copy()).to("cuda:0"))
