@@ -659,7 +659,10 @@ __global__ void __launch_bounds__(64) fp_diag_kernel(T* A, int64_t lda, int64_t 
 
 // Rows k + nbw + r (r < nrows): one thread per row, the row in registers, U of the block from LDS.  Column j
 // divides the row's entry by u_jj (none when u_jj is 0) and subtracts the multiple of U's row j from the entries
-// right of it: the operations the exchange panel applies to a row that is not taken.
+// right of it: the operations the exchange panel applies to a row that is not taken.  The loads of U and of the
+// row are issued before the first wait (FP32; in FP64 the row's after U is staged); an FP32 block without a zero
+// pivot (fpw[1] == 0, from the diagonal block) takes a loop with no branch on the pivot, which the compiler schedules
+// across columns.
 template <class T>
 __device__ __forceinline__ void fp_rows_body(T* A, int64_t lda, int64_t k, int nbw, int64_t nrows, int* fpw, T* save,
                                              int64_t lds, int64_t blk) {
@@ -667,36 +670,62 @@ __device__ __forceinline__ void fp_rows_body(T* A, int64_t lda, int64_t k, int n
   __shared__ long long ukey[LU_NB];
   const int tid = threadIdx.x;
   if (fpw[0] == 2) return;
-  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
-    const int j = idx % LU_NB, c = idx / LU_NB;
-    U[j][c] = (c >= j && c < nbw) ? A[(k + j) + (k + c) * lda] : T(0);
+  constexpr int UPT = LU_NB * LU_NB / 256;  // U entries staged per thread
+  T ureg[UPT];
+#pragma unroll
+  for (int i = 0; i < UPT; ++i) {
+    const int idx = tid + i * 256, j = idx % LU_NB, c = idx / LU_NB;
+    ureg[i] = A[(k + (j < nbw ? j : nbw - 1)) + (k + (c < nbw ? c : nbw - 1)) * lda];  // clamped: always valid
   }
-  if (tid < LU_NB) ukey[tid] = tid < nbw ? abs_key(A[(k + tid) + (k + tid) * lda]) : 0;
-  __syncthreads();
+  auto stage = [&] {
+#pragma unroll
+    for (int i = 0; i < UPT; ++i) {
+      const int idx = tid + i * 256, j = idx % LU_NB, c = idx / LU_NB;
+      U[j][c] = (c >= j && c < nbw) ? ureg[i] : T(0);
+      if (j == c) ukey[j] = c < nbw ? abs_key(ureg[i]) : 0;
+    }
+  };
+  if (sizeof(T) == 8) stage();  // FP64: U leaves registers before the row arrives (168 registers, no spill)
   const int64_t r = blk * 256 + tid;
   const bool mine = r < nrows;
   const int64_t row = k + nbw + (mine ? r : nrows - 1);
   T x[LU_NB];
 #pragma unroll
   for (int c = 0; c < LU_NB; ++c) x[c] = A[row + (k + (c < nbw ? c : nbw - 1)) * lda];
+  if (sizeof(T) == 4) stage();
+  __syncthreads();
   if (mine) {
 #pragma unroll
     for (int c = 0; c < LU_NB; ++c)
       if (c < nbw) save[(nbw + r) + c * lds] = x[c];
   }
   bool viol = false;
+  // (FP32 only: in FP64 the compiler's schedule of this loop needs more than 168 registers and spills)
+  if (sizeof(T) == 4 && fpw[1] == 0) {
 #pragma unroll
-  for (int j = 0; j < LU_NB; ++j) {
-    if (j < nbw) {
-      viol = viol || abs_key(x[j]) > ukey[j];
-      const T piv = U[j][j];
-      T l = x[j];
-      if (piv != T(0)) {
-        l = x[j] / piv;
+    for (int j = 0; j < LU_NB; ++j) {
+      if (j < nbw) {
+        viol = viol || abs_key(x[j]) > ukey[j];
+        const T l = x[j] / U[j][j];
 #pragma unroll
         for (int c = j + 1; c < LU_NB; ++c) x[c] -= l * U[j][c];
+        x[j] = l;
       }
-      x[j] = l;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < LU_NB; ++j) {
+      if (j < nbw) {
+        viol = viol || abs_key(x[j]) > ukey[j];
+        const T piv = U[j][j];
+        T l = x[j];
+        if (piv != T(0)) {
+          l = x[j] / piv;
+#pragma unroll
+          for (int c = j + 1; c < LU_NB; ++c) x[c] -= l * U[j][c];
+        }
+        x[j] = l;
+      }
     }
   }
   if (__any(mine && viol) && (tid & 63) == 0) atomicOr(&fpw[0], 1);
@@ -707,9 +736,11 @@ __device__ __forceinline__ void fp_rows_body(T* A, int64_t lda, int64_t k, int n
   }
 }
 
+// At most 168 registers per lane (three workgroups per CU): a workgroup then fits where one workgroup of a running
+// trailing-update product retires, instead of waiting for a whole CU.
 template <class T>
-__global__ void __launch_bounds__(256) fp_rows_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t nrows, int* fpw,
-                                                      T* save, int64_t lds) {
+__global__ void __launch_bounds__(256, 3) fp_rows_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t nrows,
+                                                         int* fpw, T* save, int64_t lds) {
   fp_rows_body<T>(A, lda, k, nbw, nrows, fpw, save, lds, (int64_t)blockIdx.x);
 }
 
@@ -769,7 +800,7 @@ __global__ void __launch_bounds__(64) fp_diag_batched_kernel(const FpBatch<T> jb
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) fp_rows_batched_kernel(const FpBatch<T> jb, int64_t k) {
+__global__ void __launch_bounds__(256, 3) fp_rows_batched_kernel(const FpBatch<T> jb, int64_t k) {
   const FpJob<T>& J = jb.job[fp_job_of(jb)];
   fp_rows_body<T>(J.A, J.lda, k, J.nbw, J.m - J.nbw, J.fpw, J.save, J.m, (int64_t)blockIdx.x - J.b0);
 }
