@@ -1503,7 +1503,9 @@ void gemm_sub(const T* L, int64_t ldl, const T* U, int64_t ldu, T* C, int64_t ld
 template <class T>
 void block_trsm(T* A, int64_t lda, int64_t r0, int64_t r1, int64_t c0, int64_t nc, hipStream_t st) {
   if (nc <= 0) return;
-  const int cpw = (int)std::min<int64_t>(16, std::max<int64_t>(1, nc / 1024));  // about 1024 waves for wide X
+  // columns per wave: about nc / 1024, at most 4 (more columns per wave lengthen each wave's chain of solves more
+  // than they save in staging L11: 64 x 18000 FP64 in 24 us at 4, 29 us at 16)
+  const int cpw = (int)std::min<int64_t>(4, std::max<int64_t>(1, nc / 1024));
   for (int64_t s = r0; s < r1; s += LU_NB) {
     const int nb = (int)std::min<int64_t>(LU_NB, r1 - s);
     hipLaunchKernelGGL(trsm_kernel<T>, dim3(nblk(nc, 4 * cpw)), dim3(256), 0, st, A, lda, s, nb, c0, nc, cpw);
