@@ -852,25 +852,29 @@ __global__ void __launch_bounds__(PT) lu_panel_batched_kernel(const PanelBatch<T
 // 128 threads each: all reads of a column complete before any write.
 template <class T>
 __device__ __forceinline__ void laswp_body(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw,
-                                           const int* __restrict__ perm, int64_t blk) {
-  const int64_t c = blk * 2 + threadIdx.x / 128;
-  const int t = threadIdx.x % 128;
+                                           const int* __restrict__ perm, int64_t blk, int64_t nblocks) {
   const int cnt = perm[0];
-  const bool act = c < ncols && !(c >= k && c < k + nbw) && t < cnt;
-  T v = T(0);
-  int dst = 0;
-  if (act) {
-    dst = perm[1 + 2 * t];
-    v = A[(k + perm[2 + 2 * t]) + c * lda];
+  if (cnt == 0) return;  // no interchange (the diagonal-pivot fast path held): the workgroup leaves at once
+  const int t = threadIdx.x % 128;
+  // column pairs blk, blk + nblocks, ... (the same count for every thread of the workgroup)
+  for (int64_t c2 = blk; 2 * c2 < ncols; c2 += nblocks) {
+    const int64_t c = 2 * c2 + threadIdx.x / 128;
+    const bool act = c < ncols && !(c >= k && c < k + nbw) && t < cnt;
+    T v = T(0);
+    int dst = 0;
+    if (act) {
+      dst = perm[1 + 2 * t];
+      v = A[(k + perm[2 + 2 * t]) + c * lda];
+    }
+    __syncthreads();
+    if (act) A[(k + dst) + c * lda] = v;
   }
-  __syncthreads();
-  if (act) A[(k + dst) + c * lda] = v;
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) laswp_kernel(T* A, int64_t lda, int64_t ncols, int64_t k, int nbw,
                                                     const int* __restrict__ perm) {
-  laswp_body<T>(A, lda, ncols, k, nbw, perm, (int64_t)blockIdx.x);
+  laswp_body<T>(A, lda, ncols, k, nbw, perm, (int64_t)blockIdx.x, (int64_t)gridDim.x);
 }
 
 // The interchanges of one panel in the columns [c0, c1) only (the panel's own columns excepted).
@@ -921,8 +925,9 @@ struct SwapJob {
   T* A;
   int64_t lda, ncols;
   const int* perm;
-  int32_t nbw, b0;
+  int32_t nbw, b0, nb;  // workgroups [b0, b0 + nb) of the launch
 };
+constexpr int SWAP_BLOCKS = 256;  // workgroups per matrix at most; each walks its share of the column pairs
 constexpr int SWAP_JOBS = 48;
 template <class T>
 struct SwapBatch {
@@ -935,7 +940,7 @@ __global__ void __launch_bounds__(256) laswp_batched_kernel(const SwapBatch<T> j
   int j = 0;
   while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].b0) ++j;
   const SwapJob<T>& J = jb.job[j];
-  laswp_body<T>(J.A, J.lda, J.ncols, k, J.nbw, J.perm, (int64_t)blockIdx.x - J.b0);
+  laswp_body<T>(J.A, J.lda, J.ncols, k, J.nbw, J.perm, (int64_t)blockIdx.x - J.b0, (int64_t)J.nb);
 }
 
 // A[k:k+nbw, c0:c0+ncols] = L11^-1 A[k:k+nbw, c0:...] with L11 unit lower (the panel's diagonal block).  One
@@ -2015,8 +2020,9 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
           const LuJob<T>& J = jobs[act[a2]];
           const int nbw = nbw_of(J);
           if (J.ncols <= nbw) continue;
-          sb.job[sb.njobs++] = SwapJob<T>{J.A, J.lda, J.ncols, (const int*)wv[act[a2]].perm, nbw, blocks};
-          blocks += (int)nblk(J.ncols, 2);
+          const int nb = (int)std::min<int64_t>(SWAP_BLOCKS, nblk(J.ncols, 2));
+          sb.job[sb.njobs++] = SwapJob<T>{J.A, J.lda, J.ncols, (const int*)wv[act[a2]].perm, nbw, blocks, nb};
+          blocks += nb;
         }
       }
     }
