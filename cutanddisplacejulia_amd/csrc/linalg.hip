@@ -1590,7 +1590,9 @@ struct TrsmList {
   std::vector<TrsmJob<T>> v;
   void add(T* A, int64_t lda, int64_t k, int nbw, int64_t c0, int64_t nc) {
     if (nc <= 0) return;
-    const int cpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, nc / 8192));
+    // as block_trsm: about nc / 1024 columns per wave, at most 4 (each workgroup stages L11, so a launch over 24
+    // meshes of thousands of columns at one column per wave staged it ~48k times)
+    const int cpw = (int)std::min<int64_t>(4, std::max<int64_t>(1, nc / 1024));
     v.push_back(TrsmJob<T>{A, lda, k, c0, nc, nbw, cpw, 0, 0});
   }
   void launch(hipStream_t st) {
