@@ -1026,10 +1026,10 @@ __device__ __forceinline__ void xcd_tile(int b, int gx, int gy, int& tx, int& ty
   ty = w / gs;
 }
 
-// vec (FP64 only, decided by the host): L and U are 16-byte aligned with even leading dimensions and K is a multiple
-// of GB_K, so an interior tile fetches its operands as pairs of doubles, half the load instructions of the
-// element-wise fetch (the FP64 product is bound by operand load issue: 47 -> 54 TFLOP/s at 16128^2 x 256).  The
-// staged LDS contents, and so the results, are the same either way.
+// vec (decided by the host): L and U are aligned to two elements with even leading dimensions and K is a multiple
+// of GB_K, so an interior tile fetches its operands as pairs of elements, half the load instructions of the
+// element-wise fetch (the FP64 product is bound by operand load issue: 43.9 -> 52.4 TFLOP/s at 16128^2 x 256).
+// The staged LDS contents, and so the results, are the same either way.
 template <class T>
 __device__ __forceinline__ void gemm_sub_tile(const T* __restrict__ L, int64_t ldl, const T* __restrict__ U,
                                               int64_t ldu, T* __restrict__ C, int64_t ldc, int64_t M, int64_t N,
@@ -1049,19 +1049,19 @@ __device__ __forceinline__ void gemm_sub_tile(const T* __restrict__ L, int64_t l
   const T* Lrow = L + (i0 + il < M ? i0 + il : M - 1);
   T lr[PER], ur[PER];
   // paired fetch: L rows 2 lp, 2 lp + 1 at k = lkp + 4 e; U k = 2 up, 2 up + 1 of column ujp + 32 e
-  vec = vec && sizeof(T) == 8 && i0 + GB_M <= M && j0 + GB_N <= N;
+  vec = vec && i0 + GB_M <= M && j0 + GB_N <= N;
   const int lp = tid % 64, lkp = tid / 64, up = tid % 8, ujp = tid / 8;
-  typedef double d2 __attribute__((ext_vector_type(2)));
+  typedef T t2 __attribute__((ext_vector_type(2)));
   auto fetch = [&](int64_t k0) {
-    if (sizeof(T) == 8 && vec) {
+    if (vec) {
 #pragma unroll
       for (int e = 0; e < PER / 2; ++e) {
-        const d2 a = *reinterpret_cast<const d2*>(L + (i0 + 2 * lp) + (k0 + lkp + 4 * e) * ldl);
-        const d2 b = *reinterpret_cast<const d2*>(U + (k0 + 2 * up) + (j0 + ujp + 32 * e) * ldu);
-        lr[2 * e] = (T)a.x;
-        lr[2 * e + 1] = (T)a.y;
-        ur[2 * e] = (T)b.x;
-        ur[2 * e + 1] = (T)b.y;
+        const t2 a = *reinterpret_cast<const t2*>(L + (i0 + 2 * lp) + (k0 + lkp + 4 * e) * ldl);
+        const t2 b = *reinterpret_cast<const t2*>(U + (k0 + 2 * up) + (j0 + ujp + 32 * e) * ldu);
+        lr[2 * e] = a.x;
+        lr[2 * e + 1] = a.y;
+        ur[2 * e] = b.x;
+        ur[2 * e + 1] = b.y;
       }
       return;
     }
@@ -1077,7 +1077,7 @@ __device__ __forceinline__ void gemm_sub_tile(const T* __restrict__ L, int64_t l
     }
   };
   auto stash = [&](int buf) {
-    if (sizeof(T) == 8 && vec) {
+    if (vec) {
 #pragma unroll
       for (int e = 0; e < PER / 2; ++e) {
         Ls[buf][lkp + 4 * e][2 * lp] = lr[2 * e];
@@ -1526,11 +1526,11 @@ __global__ void barrier_status_kernel(const int* bar_err, int* info) {
   if (*bar_err) *info = -1;
 }
 
-// The paired operand fetch of gemm_sub_tile applies (FP64, both operands 16-byte aligned with even leading
+// The paired operand fetch of gemm_sub_tile applies (both operands aligned to two elements with even leading
 // dimensions, whole k slices).
 template <class T>
 int gemm_vec_ok(const T* L, int64_t ldl, const T* U, int64_t ldu, int64_t K) {
-  return sizeof(T) == 8 && reinterpret_cast<uintptr_t>(L) % 16 == 0 && reinterpret_cast<uintptr_t>(U) % 16 == 0 &&
+  return reinterpret_cast<uintptr_t>(L) % (2 * sizeof(T)) == 0 && reinterpret_cast<uintptr_t>(U) % (2 * sizeof(T)) == 0 &&
          ldl % 2 == 0 && ldu % 2 == 0 && K % GB_K == 0;
 }
 
