@@ -136,7 +136,7 @@ __device__ __forceinline__ double join_words(unsigned long long lo, unsigned lon
 template <class T, int QPL>
 __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb, int32_t* ipiv,
                                            int* info, unsigned long long* xw, unsigned long long* slots, int* bar_err,
-                                           int* perm, const int P, const int b) {
+                                           int* perm, const int P, const int b, const T* __restrict__ save) {
   __shared__ T upad[2][LU_NB];               // the pivot row, zero beyond the active part, by column parity
   __shared__ long long wk[2][PT / 64];       // each wave's candidate key (-1: none) and position
   __shared__ int wp[2][PT / 64];
@@ -154,10 +154,14 @@ __device__ __forceinline__ void panel_body(T* A, int64_t lda, int64_t k, int64_t
   if (tid == 0) s_info = 0;
   T x[LU_NB];
   {
-    // unconditional loads from clamped (in-range) addresses; the clamped values are never used
-    const int64_t row = k + (mine ? pos : (m > 0 ? m - 1 : 0));
+    // unconditional loads from clamped (in-range) addresses; the clamped values are never used.  save (after the
+    // diagonal-pivot fast path found another pivot row): the panel's original rows, m x nbw with leading dimension
+    // m, which the fast path copied before changing them in A
+    const int64_t rel = mine ? pos : (m > 0 ? m - 1 : 0);
+    const T* const src = save ? save : A + k + k * lda;
+    const int64_t lds = save ? m : lda;
 #pragma unroll
-    for (int c = 0; c < LU_NB; ++c) x[c] = A[row + (k + (c < nbw ? c : nbw - 1)) * lda];
+    for (int c = 0; c < LU_NB; ++c) x[c] = src[rel + (c < nbw ? c : nbw - 1) * lds];
   }
   for (int j = 0; j < nbw; ++j) {
     const int par = j & 1;
@@ -459,12 +463,20 @@ template <class T, int QPL>
 __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t k, int64_t m, int nbw, int rpb,
                                                       int32_t* ipiv, int* info, unsigned long long* xw,
                                                       unsigned long long* slots, int* bar_err, int* perm,
-                                                      const int* skip) {
-  if (skip && skip[0] == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0 && skip[1] != 0 && *info == 0) *info = skip[1];
-    return;
+                                                      int* skip, const T* save) {
+  const T* src = nullptr;
+  if (skip) {
+    if (skip[0] == 0) {
+      if (blockIdx.x == 0 && threadIdx.x == 0 && skip[1] != 0 && *info == 0) *info = skip[1];
+      return;
+    }
+    if (skip[0] == 1) {  // the fast path changed the rows in A: factor the saved originals, count the fall-back
+      src = save;
+      if (blockIdx.x == 0 && threadIdx.x == 0) skip[2] += 1;
+    }
   }
-  panel_body<T, QPL>(A, lda, k, m, nbw, rpb, ipiv, info, xw, slots, bar_err, perm, (int)gridDim.x, (int)blockIdx.x);
+  panel_body<T, QPL>(A, lda, k, m, nbw, rpb, ipiv, info, xw, slots, bar_err, perm, (int)gridDim.x, (int)blockIdx.x,
+                     src);
 }
 
 // ---- diagonal-pivot fast path of a panel ----------------------------------------------------------------
@@ -474,8 +486,8 @@ __global__ void __launch_bounds__(PT) lu_panel_kernel(T* A, int64_t lda, int64_t
 // nbw x nbw diagonal block (fp_diag_kernel), then every row below solves against its U on its own
 // (fp_rows_kernel).  Both compare each candidate with the diagonal as the exchange does (64-bit keys bits(|a|),
 // ties to the diagonal) and raise fpw[0] when another row would have been taken; both first copy the rows they
-// change to `save`.  On a violation fp_restore_kernel puts the rows back and lu_panel_kernel factors the panel with
-// the exchange; otherwise both return at once.  Each entry receives the exchange panel's operations (the same
+// change to `save`.  On a violation lu_panel_kernel factors the panel with the exchange from the rows in `save` (its
+// final write covers every entry the fast path changed); otherwise it returns at once.  Each entry receives the exchange panel's operations (the same
 // divisions and FMAs in the same order), so the factors are bit-identical either way.
 template <class T>
 __device__ __forceinline__ long long abs_key(T v) {
@@ -745,23 +757,6 @@ __global__ void __launch_bounds__(256, 3) fp_rows_kernel(T* A, int64_t lda, int6
   fp_rows_body<T>(A, lda, k, nbw, nrows, fpw, save, lds, (int64_t)blockIdx.x);
 }
 
-// After a violation: the panel's rows [k, k + m) back from `save`.
-template <class T>
-__device__ __forceinline__ void fp_restore_body(T* A, int64_t lda, int64_t k, int nbw, int64_t m, int* fpw,
-                                                const T* __restrict__ save, int64_t lds, int64_t blk) {
-  if (fpw[0] != 1) return;
-  const int64_t r = blk * 256 + threadIdx.x;
-  if (r == 0) fpw[2] += 1;  // one thread of the grid counts the fall-back
-  if (r >= m) return;
-  for (int c = 0; c < nbw; ++c) A[(k + r) + (k + c) * lda] = save[r + c * lds];
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) fp_restore_kernel(T* A, int64_t lda, int64_t k, int nbw, int64_t m, int* fpw,
-                                                         const T* __restrict__ save, int64_t lds) {
-  fp_restore_body<T>(A, lda, k, nbw, m, fpw, save, lds, (int64_t)blockIdx.x);
-}
-
 // The fast path for the panels of several matrices at the same column k (the batched factorization): job j owns
 // workgroups [b0, b0 + nb) of a launch (nb = 1 for the diagonal blocks).
 template <class T>
@@ -806,12 +801,6 @@ __global__ void __launch_bounds__(256, 3) fp_rows_batched_kernel(const FpBatch<T
   fp_rows_body<T>(J.A, J.lda, k, J.nbw, J.m - J.nbw, J.fpw, J.save, J.m, (int64_t)blockIdx.x - J.b0);
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) fp_restore_batched_kernel(const FpBatch<T> jb, int64_t k) {
-  const FpJob<T>& J = jb.job[fp_job_of(jb)];
-  fp_restore_body<T>(J.A, J.lda, k, J.nbw, J.m, J.fpw, J.save, J.m, (int64_t)blockIdx.x - J.b0);
-}
-
 // Panels of several matrices at the same column k in one cooperative launch: matrix j owns workgroups
 // [b0, b0 + P) and exchanges its pivot candidates only among them, through its own workspace.
 template <class T>
@@ -825,7 +814,8 @@ struct PanelJob {
   unsigned long long* slots;
   int* bar_err;
   int* perm;
-  const int* skip;  // the fast path's words (NULL: none), as in lu_panel_kernel
+  int* skip;      // the fast path's words (NULL: none), as in lu_panel_kernel
+  const T* save;  // the fast path's copy of the panel's rows, as in lu_panel_kernel
 };
 constexpr int PANEL_JOBS = 24;  // matrices per launch (the job table travels in the kernel arguments)
 template <class T>
@@ -839,12 +829,19 @@ __global__ void __launch_bounds__(PT) lu_panel_batched_kernel(const PanelBatch<T
   int j = 0;
   while (j + 1 < jb.njobs && (int)blockIdx.x >= jb.job[j + 1].b0) ++j;
   const PanelJob<T>& J = jb.job[j];
-  if (J.skip && J.skip[0] == 0) {  // every workgroup of the job leaves, so none waits for another
-    if ((int)blockIdx.x == J.b0 && threadIdx.x == 0 && J.skip[1] != 0 && *J.info == 0) *J.info = J.skip[1];
-    return;
+  const T* src = nullptr;
+  if (J.skip) {
+    if (J.skip[0] == 0) {  // every workgroup of the job leaves, so none waits for another
+      if ((int)blockIdx.x == J.b0 && threadIdx.x == 0 && J.skip[1] != 0 && *J.info == 0) *J.info = J.skip[1];
+      return;
+    }
+    if (J.skip[0] == 1) {
+      src = J.save;
+      if ((int)blockIdx.x == J.b0 && threadIdx.x == 0) J.skip[2] += 1;
+    }
   }
   panel_body<T, QPL>(J.A, J.lda, k, J.m, J.nbw, J.rpb, J.ipiv, J.info, J.xw, J.slots, J.bar_err, J.perm, J.P,
-                     (int)blockIdx.x - J.b0);
+                     (int)blockIdx.x - J.b0, src);
 }
 
 // Row interchanges of one panel applied to columns [0, k) and [k + nbw, ncols), as the permutation of the
@@ -1767,7 +1764,8 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
       const int P = (int)((m + MAX_RPB - 1) / MAX_RPB);
       int rpb = (int)((m + P - 1) / P);
       if (P > MAX_P) return hipErrorInvalidValue;
-      const int* skip = nullptr;
+      int* skip = nullptr;
+      const T* csave = save;
       if (fast) {
         // the diagonal-pivot fast path; the exchange panel below runs only if it found another pivot row
         if (nbw == LU_NB)
@@ -1779,12 +1777,10 @@ hipError_t lu_factor_cols(T* A, int64_t n, int64_t ncols, int64_t lda, int32_t* 
         if (m > nbw)
           hipLaunchKernelGGL(fp_rows_kernel<T>, dim3(nblk(m - nbw, 256)), dim3(256), 0, st, A, lda, k, nbw, m - nbw,
                              fpw, save, m);
-        hipLaunchKernelGGL(fp_restore_kernel<T>, dim3(nblk(m, 256)), dim3(256), 0, st, A, lda, k, nbw, m, fpw,
-                           (const T*)save, m);
         skip = fpw;
       }
-      void* args[] = {&A,    (void*)&lda, &k,     (void*)&m, (void*)&nbw, &rpb, &ipiv, &info,
-                      &xw,   &slots,      &bar_err, &perm,   (void*)&skip};
+      void* args[] = {&A,  (void*)&lda, &k,       (void*)&m, (void*)&nbw,  &rpb, &ipiv, &info,
+                      &xw, &slots,      &bar_err, &perm,     (void*)&skip, (void*)&csave};
       const void* kern = P <= 128 ? (const void*)lu_panel_kernel<T, 2> : (const void*)lu_panel_kernel<T, MAX_P / 64>;
       if (pending && P <= panel_capacity)
         e = hipLaunchKernel(kern, dim3(P), dim3(PT), args, 0, st);
@@ -1960,12 +1956,6 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
             blocks += (int)nblk(all.job[q].m - all.job[q].nbw, 256);
           }
           if (blocks > 0) hipLaunchKernelGGL(fp_rows_batched_kernel<T>, dim3(blocks), dim3(256), 0, st, all, kk);
-          blocks = 0;
-          for (int q = 0; q < all.njobs; ++q) {
-            all.job[q].b0 = blocks;
-            blocks += (int)nblk(all.job[q].m, 256);
-          }
-          hipLaunchKernelGGL(fp_restore_batched_kernel<T>, dim3(blocks), dim3(256), 0, st, all, kk);
         }
       }
       // the panels, as few launches as co-residency and the job table allow
@@ -1995,6 +1985,7 @@ hipError_t lu_factor_batched(const LuJob<T>* jobs, int njobs, hipStream_t st, bo
           q.bar_err = wv[act[a]].bar_err;
           q.perm = wv[act[a]].perm;
           q.skip = fast ? wv[act[a]].fpw : nullptr;
+          q.save = wv[act[a]].save;
           blocks += P;
           pmax = std::max(pmax, P);
         }
